@@ -1,0 +1,1039 @@
+/*
+ * ORACLE — test infrastructure only (see fdm_oracle.h).
+ *
+ * Plain-C restatement of the FDMNES finite-difference path, 1-based like the
+ * reference so each loop reads against its Fortran source:
+ *   reseau    general.f90:9439-9561   grid points, radial exchange sort
+ *   laplac    general.f90:9732-9871   stencil coefficients, neighbour table
+ *   bordure   general.f90:9895-10076  outer-sphere point lists and weights
+ *   Ylmpt     general.f90:11505-11570, cylm :11606-11688, ylmcr :11692-11720
+ *   phiso     mat.f90:739-1028 (non-Rydberg branch :975-994)
+ *   cbessneu  sphere.f90:7226-7248, cbessneur sphere.f90:7356-7378
+ *   newind    mat.f90:1761-1921
+ *   calcMatRow mat.f90:1034-1511 (grid rows :1142-1398, outer rows :1401-1511)
+ *   mat_solve mat_solve_gaussian.f90:67-358
+ * Constants follow the reference's module declarations (main.f90:36-83).
+ */
+#define _POSIX_C_SOURCE 199309L
+#include "fdm_oracle.h"
+
+#include <complex.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+/* main.f90:55-83 */
+static const double PI = 3.1415926535897932384626433832;
+static const double SQRT_2 = 1.41421356237309504880168872420970;
+static const double EPS6 = 1.e-6;
+static const double EPS10 = 1.e-10;
+static const double EPS15 = 1.e-15;
+static const double EPSPOS = 1.e-4;
+
+struct orc_cluster {
+  double rsort, adimp;
+  int iord, lmaxso, nvois, nx;
+  int npso, npoint, nligne, nligneso, nlmso, nsortf, nsort;
+  double *xyz;     /* xyz(4,npso): index 4*(i-1)+k-1 */
+  int *indice;     /* indice(npso,3) */
+  int *numia;      /* numia(npso) */
+  double *rvol;    /* rvol(npoint) */
+  int *ivois;      /* ivois(npso,nvois) */
+  double clapl[13];
+  int *isrt;       /* isrt(nsort) */
+  double *poidso;  /* poidso(nsortf) */
+  double *ylmso;   /* Ylmso(nsort, nlmso) */
+  int *lso, *mso, *iso; /* (nlmso) */
+  int *newp;       /* new(0:npoint) */
+  int *lb1, *lb2, *newinv; /* (nligne) */
+};
+
+#define XYZ(c, k, i) ((c)->xyz[4 * ((i)-1) + (k)-1])
+#define IND(c, i, k) ((c)->indice[((k)-1) * (c)->npso + (i)-1])
+#define IVOIS(c, i, iv) ((c)->ivois[((iv)-1) * (c)->npso + (i)-1])
+#define YLMSO(c, ib, lm) ((c)->ylmso[((lm)-1) * (c)->nsort + (ib)-1])
+
+static double cpu_seconds(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_THREAD_CPUTIME_ID, &ts);
+  return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+/* ---------------------------------------------------------------- reseau */
+/* general.f90:9439-9561 for a cubic grid, point group P1, Moy_loc = Green = F. */
+static int orc_reseau(orc_cluster *c) {
+  const double rmax = c->rsort / c->adimp + SQRT_2 * c->iord / 2 + EPSPOS; /* :9476 */
+  const int nx = (int)ceil(rmax) + 1;
+  c->nx = nx;
+  int cap = 0;
+  for (int ix = -nx; ix <= nx; ix++)
+    for (int iy = -nx; iy <= nx; iy++)
+      for (int iz = -nx; iz <= nx; iz++) {
+        double v1 = ix, v2 = iy, v3 = iz;
+        if (sqrt(v1 * v1 + v2 * v2 + v3 * v3) <= rmax) cap++;
+      }
+  c->npso = cap;
+  c->xyz = calloc((size_t)4 * cap, sizeof(double));
+  c->indice = calloc((size_t)3 * cap, sizeof(int));
+  c->numia = calloc((size_t)cap + 1, sizeof(int));
+  if (!c->xyz || !c->indice || !c->numia) return -1;
+  int i = 0;
+  for (int ix = -nx; ix <= nx; ix++) /* :9480-9514 loop order ix, iy, iz */
+    for (int iy = -nx; iy <= nx; iy++)
+      for (int iz = -nx; iz <= nx; iz++) {
+        double v1 = ix, v2 = iy, v3 = iz;
+        double dist = sqrt(v1 * v1 + v2 * v2 + v3 * v3);
+        if (dist > rmax) continue;
+        i++;
+        IND(c, i, 1) = ix;
+        IND(c, i, 2) = iy;
+        IND(c, i, 3) = iz;
+        XYZ(c, 4, i) = c->adimp * dist;
+      }
+  const int npso = i;
+  /* Exchange sort by distance with tolerance epspos, :9528-9543 */
+  for (int a = 1; a <= npso; a++) {
+    for (int b = a + 1; b <= npso; b++) {
+      if (XYZ(c, 4, a) < XYZ(c, 4, b) + EPSPOS) continue;
+      double t = XYZ(c, 4, a);
+      XYZ(c, 4, a) = XYZ(c, 4, b);
+      XYZ(c, 4, b) = t;
+      for (int k = 1; k <= 3; k++) {
+        int ti = IND(c, a, k);
+        IND(c, a, k) = IND(c, b, k);
+        IND(c, b, k) = ti;
+      }
+    }
+  }
+  for (int a = 1; a <= npso; a++) /* :9545-9553 */
+    for (int k = 1; k <= 3; k++) XYZ(c, k, a) = IND(c, a, k) * c->adimp;
+  int a;
+  for (a = 1; a <= npso; a++) /* :9556-9559 */
+    if (XYZ(c, 4, a) > c->rsort + EPSPOS) break;
+  c->npoint = a - 1;
+  for (int p = 1; p <= npso; p++) c->numia[p - 1] = (p <= c->npoint) ? 0 : -2;
+  /* rvol: with P1 every point is counted once and rvmax = 1, :9582-9651 */
+  c->rvol = malloc(sizeof(double) * (c->npoint + 1));
+  for (int p = 1; p <= c->npoint; p++) c->rvol[p - 1] = 1.0;
+  return 0;
+}
+
+/* ---------------------------------------------------------------- laplac */
+/* general.f90:9732-9871, cubic grid. */
+static int orc_laplac(orc_cluster *c) {
+  const int nx = c->nx, w = 2 * nx + 1;
+  int indpro[13][3];
+  memset(indpro, 0, sizeof(indpro));
+  double ad2 = c->adimp * c->adimp;
+  double coeford[2] = {1.0, 0.0};
+  double cl;
+  if (c->iord == 2) {
+    cl = 1.0 / ad2;
+  } else {
+    cl = (4.0 / 3.0) / ad2;
+    coeford[1] = -1.0 / 16;
+  }
+  int iv = 0;
+  for (int io = 1; io <= c->iord / 2; io++)
+    for (int i = 1; i <= 3; i++)
+      for (int is = -1; is <= 1; is += 2) {
+        iv++;
+        c->clapl[iv] = coeford[io - 1] * cl;
+        indpro[iv][i - 1] = is * io;
+      }
+  c->nvois = iv;
+  double s = 0;
+  for (int k = 1; k <= c->nvois; k++) s += c->clapl[k];
+  c->clapl[0] = -s;
+  /* mpres(-nx:nx,...) */
+  int *mpres = calloc((size_t)w * w * w, sizeof(int));
+  if (!mpres) return -1;
+  for (int p = 1; p <= c->npso; p++)
+    mpres[((IND(c, p, 1) + nx) * w + (IND(c, p, 2) + nx)) * w + IND(c, p, 3) + nx] = p;
+  c->ivois = calloc((size_t)c->npso * c->nvois, sizeof(int));
+  for (int p = 1; p <= c->npso; p++)
+    for (int k = 1; k <= c->nvois; k++) {
+      int la[3];
+      for (int d = 0; d < 3; d++) la[d] = IND(c, p, d + 1) + indpro[k][d];
+      if (abs(la[0]) <= nx && abs(la[1]) <= nx && abs(la[2]) <= nx)
+        IVOIS(c, p, k) = mpres[((la[0] + nx) * w + (la[1] + nx)) * w + la[2] + nx];
+      else
+        IVOIS(c, p, k) = 0;
+    }
+  free(mpres);
+  return 0;
+}
+
+/* ---------------------------------------------------------------- bordure */
+/* general.f90:9895-10076 with natome = 0, Green = F. */
+static int orc_bordure(orc_cluster *c) {
+  const int nv = 2 * c->nvois / c->iord;
+  int nsortf = 0, iss = 0;
+  for (int i = 1; i <= c->npoint; i++) { /* :9924-9938 */
+    if (c->numia[i - 1] != 0) continue;
+    for (int iv = 1; iv <= nv; iv++) {
+      int j = IVOIS(c, i, iv);
+      if (j == 0) continue;
+      if (c->numia[j - 1] == -2) {
+        if (iss != i) nsortf++;
+        iss = i;
+      }
+    }
+  }
+  int nsort = nsortf;
+  for (int i = c->npoint + 1; i <= c->npso; i++) /* :9961-9969 */
+    for (int iv = 1; iv <= c->nvois; iv++) {
+      int j = IVOIS(c, i, iv);
+      if (j == 0) continue;
+      if (c->numia[j - 1] != 0) continue;
+      nsort++;
+      break;
+    }
+  c->isrt = calloc((size_t)nsort + 1, sizeof(int));
+  c->poidso = calloc((size_t)nsortf + 1, sizeof(double));
+  /* second pass, :9987-10013 (iss is not reset, as in the reference) */
+  nsortf = 0;
+  double pdsort = 0;
+  for (int i = 1; i <= c->npoint; i++) {
+    if (c->numia[i - 1] != 0) continue;
+    for (int iv = 1; iv <= nv; iv++) {
+      int j = IVOIS(c, i, iv);
+      if (j == 0) continue;
+      if (c->numia[j - 1] == -2) {
+        if (iss != i) {
+          pdsort += c->rvol[i - 1];
+          nsortf++;
+          c->isrt[nsortf - 1] = i;
+        }
+        iss = i;
+      }
+    }
+  }
+  nsort = nsortf;
+  for (int i = c->npoint + 1; i <= c->npso; i++) /* :10041-10050 */
+    for (int iv = 1; iv <= c->nvois; iv++) {
+      int j = IVOIS(c, i, iv);
+      if (j == 0) continue;
+      if (c->numia[j - 1] != 0) continue;
+      nsort++;
+      c->isrt[nsort - 1] = i;
+      break;
+    }
+  if (nsortf > 0) { /* :10070-10076 */
+    double fac = 4 * PI / pdsort;
+    for (int ib = 1; ib <= nsortf; ib++) c->poidso[ib - 1] = fac * c->rvol[c->isrt[ib - 1] - 1];
+  }
+  c->nsortf = nsortf;
+  c->nsort = nsort;
+  return 0;
+}
+
+/* ---------------------------------------------------------------- Ylm */
+/* cylm, general.f90:11606-11688: complex Y(L,m), m >= 0, index L(L+1)/2+1+m. */
+static void orc_cylm(int lmax, const double *v, double r, double complex *ylmc) {
+  double cos_t, sin_t = 0, cot_t = 0;
+  double complex exphi, sin_t_exp_phi;
+  cos_t = (r > EPS6) ? v[2] / r : 1.0;
+  double den = sqrt(v[0] * v[0] + v[1] * v[1]);
+  if (den > EPS6) {
+    exphi = CMPLX(v[0], v[1]) / den;
+    sin_t = sqrt(1 - cos_t * cos_t);
+    sin_t_exp_phi = CMPLX(v[0], v[1]) / r;
+    cot_t = cos_t / sin_t;
+  } else {
+    exphi = 1.0;
+    sin_t = 0;
+    sin_t_exp_phi = 0;
+  }
+  ylmc[0] = 1.0 / sqrt(4 * PI);
+  for (int L = 1; L <= lmax; L++) { /* Y(L,L) */
+    int lm = ((L + 1) * (L + 2)) / 2, lm1 = (L * (L + 1)) / 2;
+    double f = -sqrt(1 + 0.5 / L);
+    ylmc[lm - 1] = f * sin_t_exp_phi * ylmc[lm1 - 1];
+  }
+  if (lmax > 0) ylmc[1] = sqrt(3.0 / (4 * PI)) * cos_t; /* Y(1,0) */
+  for (int L = 2; L <= lmax; L++) { /* Y(L,L-1) */
+    int lm = ((L + 1) * (L + 2)) / 2 - 1, lm1 = (L * L + L) / 2 - 1;
+    double f = -sqrt((2 * L + 1.0) / (2 * L - 2));
+    ylmc[lm - 1] = f * sin_t_exp_phi * ylmc[lm1 - 1];
+  }
+  exphi = conj(exphi);
+  for (int L = 2; L <= lmax; L++) { /* Y(L,m), L-2 >= m >= 1 */
+    int lm0 = (L * L + L) / 2;
+    for (int m = L - 2; m >= 1; m--) {
+      int lm = lm0 + m + 1;
+      double f = -2 * (m + 1.0) / sqrt(L * (L + 1.0) - m * (m + 1.0));
+      double g = -sqrt((L * (L + 1.0) - (m + 1.0) * (m + 2.0)) / (L * (L + 1.0) - m * (m + 1.0)));
+      if (fabs(sin_t) > EPS10)
+        ylmc[lm - 1] = exphi * (cot_t * f * ylmc[lm] + exphi * g * ylmc[lm + 1]);
+      else
+        ylmc[lm - 1] = 0;
+    }
+  }
+  for (int L = 2; L <= lmax; L++) { /* Y(L,0) */
+    int lm = (L * (L + 1)) / 2 + 1, lm1 = ((L - 1) * L) / 2 + 1, lm2 = ((L - 2) * (L - 1)) / 2 + 1;
+    double f = sqrt((2 * L + 1.0) / (2 * L - 1.0)) * (2 * L - 1.0) / L;
+    double g = -sqrt((2 * L + 1.0) / (2 * L - 3.0)) * (L - 1.0) / L;
+    ylmc[lm - 1] = f * cos_t * ylmc[lm1 - 1] + g * ylmc[lm2 - 1];
+  }
+}
+
+/* ylmcr, general.f90:11692-11720 */
+void orc_ylm_real(int lmax, const double *v, double r, double *ylmr) {
+  int nlmc = ((lmax + 1) * (lmax + 2)) / 2;
+  double complex *ylmc = malloc(sizeof(double complex) * nlmc);
+  orc_cylm(lmax, v, r, ylmc);
+  int lm = 0;
+  for (int L = 0; L <= lmax; L++) {
+    int l2 = (L * L + L) / 2 + 1;
+    for (int m = -L; m <= L; m++) {
+      lm++;
+      int lm0 = l2 + abs(m);
+      double sgn = (abs(m) % 2) ? -1.0 : 1.0;
+      if (m < 0)
+        ylmr[lm - 1] = sgn * SQRT_2 * cimag(ylmc[lm0 - 1]);
+      else if (m == 0)
+        ylmr[lm - 1] = creal(ylmc[lm0 - 1]);
+      else
+        ylmr[lm - 1] = sgn * SQRT_2 * creal(ylmc[lm0 - 1]);
+    }
+  }
+  free(ylmc);
+}
+
+/* Ylmpt for the outer sphere (ia = 0), general.f90:11545-11568 */
+static void orc_ylmpt(orc_cluster *c) {
+  const int nlmr = (c->lmaxso + 1) * (c->lmaxso + 1);
+  c->ylmso = calloc((size_t)c->nsort * nlmr, sizeof(double));
+  double *y = malloc(sizeof(double) * nlmr);
+  for (int ib = 1; ib <= c->nsort; ib++) {
+    int i = c->isrt[ib - 1];
+    double w[3] = {XYZ(c, 1, i), XYZ(c, 2, i), XYZ(c, 3, i)};
+    double r = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+    orc_ylm_real(c->lmaxso, w, r, y);
+    for (int lm = 1; lm <= nlmr; lm++) YLMSO(c, ib, lm) = y[lm - 1];
+  }
+  free(y);
+}
+
+/* ---------------------------------------------------------------- newind */
+/* mat.f90:1761-1921, natome = 0, nspino = 1, regular solution. */
+static void orc_newind(orc_cluster *c) {
+  const int nspino = 1;
+  c->newp = calloc((size_t)c->npoint + 1, sizeof(int));
+  int ii = 0;
+  for (int i = 1; i <= c->npoint; i++) {
+    if (c->numia[i - 1] == 0) {
+      ii += nspino;
+      c->newp[i] = ii;
+    } else {
+      c->newp[i] = 0;
+    }
+  }
+  const int nligne = c->nligne;
+  c->lb1 = malloc(sizeof(int) * (nligne + 1));
+  c->lb2 = malloc(sizeof(int) * (nligne + 1));
+  c->newinv = calloc((size_t)nligne + 1, sizeof(int));
+  for (ii = 1; ii <= nligne; ii++) {
+    c->lb1[ii] = ii;
+    c->lb2[ii] = ii;
+  }
+  const int n = nspino - 1;
+  for (int i = 1; i <= c->npoint; i++) {
+    if (c->numia[i - 1] != 0) continue;
+    ii = c->newp[i];
+    for (int iv = 1; iv <= c->nvois; iv++) {
+      int j = IVOIS(c, i, iv);
+      int ia = c->numia[j - 1];
+      if (ia == 0) {
+        int nj = c->newp[j];
+        if (nj - n < c->lb1[ii]) c->lb1[ii] = nj - n;
+        if (nj > c->lb2[ii]) c->lb2[ii] = nj;
+      } else if (ia == -2) {
+        c->lb2[ii] = nligne;
+      }
+    }
+  }
+  int lb11 = nligne - c->nlmso + 1; /* :1871-1880 */
+  for (int ib = 1; ib <= c->nsortf; ib++) {
+    int r = c->newp[c->isrt[ib - 1]] - n;
+    if (r < lb11) lb11 = r;
+  }
+  for (ii = nligne - c->nlmso + 1; ii <= nligne; ii++) {
+    c->lb1[ii] = lb11;
+    c->lb2[ii] = nligne;
+  }
+  for (ii = nligne - 1; ii >= 1; ii--) /* :1882-1884 */
+    if (c->lb1[ii + 1] < c->lb1[ii]) c->lb1[ii] = c->lb1[ii + 1];
+  for (int i = 1; i <= c->npoint; i++) {
+    ii = c->newp[i];
+    if (ii != 0) c->newinv[ii] = i;
+  }
+}
+
+orc_cluster *orc_cluster_new(double rsort, double adimp, int iord, int lmaxso) {
+  orc_cluster *c = calloc(1, sizeof(orc_cluster));
+  if (!c) return NULL;
+  c->rsort = rsort;
+  c->adimp = adimp;
+  c->iord = iord;
+  c->lmaxso = lmaxso;
+  if (orc_reseau(c) || orc_laplac(c) || orc_bordure(c)) {
+    orc_cluster_free(c);
+    return NULL;
+  }
+  orc_ylmpt(c);
+  /* outer harmonics with P1: Lm = 1..(lmaxso+1)^2 in (L, m) order, iso = 1 */
+  c->nlmso = (lmaxso + 1) * (lmaxso + 1);
+  c->lso = malloc(sizeof(int) * (c->nlmso + 1));
+  c->mso = malloc(sizeof(int) * (c->nlmso + 1));
+  c->iso = malloc(sizeof(int) * (c->nlmso + 1));
+  int lm = 0;
+  for (int L = 0; L <= lmaxso; L++)
+    for (int m = -L; m <= L; m++) {
+      lm++;
+      c->lso[lm] = L;
+      c->mso[lm] = m;
+      c->iso[lm] = 1;
+    }
+  c->nligne = c->npoint + c->nlmso; /* mat.f90:150, npr = npoint */
+  c->nligneso = c->nligne - c->nlmso;
+  orc_newind(c);
+  return c;
+}
+
+void orc_cluster_free(orc_cluster *c) {
+  if (!c) return;
+  free(c->xyz); free(c->indice); free(c->numia); free(c->rvol); free(c->ivois);
+  free(c->isrt); free(c->poidso); free(c->ylmso); free(c->lso); free(c->mso); free(c->iso);
+  free(c->newp); free(c->lb1); free(c->lb2); free(c->newinv);
+  free(c);
+}
+
+void orc_cluster_sizes(const orc_cluster *c, int *s) {
+  s[0] = c->npoint; s[1] = c->npso; s[2] = c->nligne; s[3] = c->nlmso;
+  s[4] = c->nsortf; s[5] = c->nsort; s[6] = c->nvois;
+}
+void orc_cluster_xyz(const orc_cluster *c, double *xyz) { memcpy(xyz, c->xyz, sizeof(double) * 4 * c->npso); }
+void orc_cluster_indice(const orc_cluster *c, int *ind) { memcpy(ind, c->indice, sizeof(int) * 3 * c->npso); }
+void orc_cluster_ivois(const orc_cluster *c, int *iv) { memcpy(iv, c->ivois, sizeof(int) * c->npso * c->nvois); }
+void orc_cluster_boundary(const orc_cluster *c, int *isrt, double *poidso) {
+  memcpy(isrt, c->isrt, sizeof(int) * c->nsort);
+  memcpy(poidso, c->poidso, sizeof(double) * c->nsortf);
+}
+void orc_cluster_ylmso(const orc_cluster *c, double *y) { memcpy(y, c->ylmso, sizeof(double) * c->nsort * c->nlmso); }
+void orc_cluster_clapl(const orc_cluster *c, double *cl) { memcpy(cl, c->clapl, sizeof(double) * (c->nvois + 1)); }
+void orc_cluster_profile(const orc_cluster *c, int *lb1, int *lb2, int *newinv, int *newp) {
+  memcpy(lb1, c->lb1 + 1, sizeof(int) * c->nligne);
+  memcpy(lb2, c->lb2 + 1, sizeof(int) * c->nligne);
+  if (newinv) memcpy(newinv, c->newinv + 1, sizeof(int) * c->nligne);
+  if (newp) memcpy(newp, c->newp, sizeof(int) * (c->npoint + 1));
+}
+
+/* ---------------------------------------------------------------- Bessel */
+void orc_cbessneu(double fn_re, double fn_im, double z_re, double z_im, int lmax, double *bes, double *neu) {
+  /* sphere.f90:7226-7248; bes/neu hold (re,im) pairs for L = 0..lmax */
+  double complex fnorm = CMPLX(fn_re, fn_im), z = CMPLX(z_re, z_im);
+  double complex *b = malloc(sizeof(double complex) * (lmax + 1));
+  double complex *n = malloc(sizeof(double complex) * (lmax + 1));
+  n[0] = -fnorm * ccos(z) / z;
+  b[0] = fnorm * csin(z) / z;
+  if (lmax >= 1) {
+    n[1] = n[0] / z - b[0];
+    b[1] = b[0] / z + n[0];
+    for (int L = 2; L <= lmax; L++) {
+      int L1 = 2 * L - 1;
+      n[L] = L1 * n[L - 1] / z - n[L - 2];
+      b[L] = L1 * b[L - 1] / z - b[L - 2];
+    }
+  }
+  for (int L = 0; L <= lmax; L++) {
+    bes[2 * L] = creal(b[L]); bes[2 * L + 1] = cimag(b[L]);
+    neu[2 * L] = creal(n[L]); neu[2 * L + 1] = cimag(n[L]);
+  }
+  free(b); free(n);
+}
+
+void orc_cbessneur(double fnorm, double z, int lmax, double *bes, double *neu) {
+  /* sphere.f90:7356-7378 */
+  neu[0] = -fnorm * cos(z) / z;
+  bes[0] = fnorm * sin(z) / z;
+  if (lmax == 0) return;
+  neu[1] = neu[0] / z - bes[0];
+  bes[1] = bes[0] / z + neu[0];
+  for (int L = 2; L <= lmax; L++) {
+    int L1 = 2 * L - 1;
+    neu[L] = L1 * neu[L - 1] / z - neu[L - 2];
+    bes[L] = L1 * bes[L - 1] / z - bes[L - 2];
+  }
+}
+
+/* phiso non-Rydberg branch, mat.f90:771-792 and :975-994.
+ * Complex: bes/neu are (nsort, 0:lmaxso) of (re,im) pairs, ib fastest. */
+void orc_phiso(const orc_cluster *c, int E_comp, double Ecinetic_out, double Eimag, double *bes, double *neu) {
+  const int lmax = c->lmaxso, ns = c->nsort;
+  double *tb = malloc(sizeof(double) * 2 * (lmax + 1)), *tn = malloc(sizeof(double) * 2 * (lmax + 1));
+  if (E_comp) {
+    double complex konde = csqrt(CMPLX(Ecinetic_out, Eimag));
+    if (cabs(konde) < EPS10) konde = CMPLX(EPS10, 0.0);
+    double complex fnormc = csqrt(konde / PI);
+    for (int ib = 1; ib <= ns; ib++) {
+      int i = c->isrt[ib - 1];
+      double p1 = XYZ(c, 1, i), p2 = XYZ(c, 2, i), p3 = XYZ(c, 3, i);
+      double complex z = konde * sqrt(p1 * p1 + p2 * p2 + p3 * p3);
+      orc_cbessneu(creal(fnormc), cimag(fnormc), creal(z), cimag(z), lmax, tb, tn);
+      for (int L = 0; L <= lmax; L++) {
+        size_t o = 2 * ((size_t)L * ns + ib - 1);
+        bes[o] = tb[2 * L]; bes[o + 1] = tb[2 * L + 1];
+        neu[o] = tn[2 * L]; neu[o + 1] = tn[2 * L + 1];
+      }
+    }
+  } else {
+    double konder = sqrt(Ecinetic_out);
+    double fnorm = sqrt(konder / PI);
+    for (int ib = 1; ib <= ns; ib++) {
+      int i = c->isrt[ib - 1];
+      double p1 = XYZ(c, 1, i), p2 = XYZ(c, 2, i), p3 = XYZ(c, 3, i);
+      double zr = konder * sqrt(p1 * p1 + p2 * p2 + p3 * p3);
+      orc_cbessneur(fnorm, zr, lmax, tb, tn);
+      for (int L = 0; L <= lmax; L++) {
+        size_t o = (size_t)L * ns + ib - 1;
+        bes[o] = tb[L];
+        neu[o] = tn[L];
+      }
+    }
+  }
+  free(tb); free(tn);
+}
+
+/* ---------------------------------------------------------------- calcMatRow */
+typedef struct {
+  const orc_cluster *c;
+  const double *V;
+  double Enervide, Eimag;
+  int E_comp, cal_comp, basereel;
+  const double *bes, *neu; /* complex (E_comp) or real, (nsort, 0:lmaxso) */
+} row_ctx;
+
+static inline double complex ctx_bes(const row_ctx *x, int ib, int L) {
+  size_t o = (size_t)L * x->c->nsort + ib - 1;
+  return x->E_comp ? CMPLX(x->bes[2 * o], x->bes[2 * o + 1]) : CMPLX(x->bes[o], 0.0);
+}
+static inline double complex ctx_neu(const row_ctx *x, int ib, int L) {
+  size_t o = (size_t)L * x->c->nsort + ib - 1;
+  return x->E_comp ? CMPLX(x->neu[2 * o], x->neu[2 * o + 1]) : CMPLX(x->neu[o], 0.0);
+}
+
+/* mat.f90:1034-1511 restricted to the oracle's branch set.  abvr/abvi are
+ * indexed by absolute column (1-based); smr/smi rows are (nlmso) at row ii.
+ * abvi may be NULL when cal_comp == 0. */
+static void orc_calcmatrow(const row_ctx *x, int ii, double *abvr, double *abvi, double *smr_ii, double *smi_ii) {
+  const orc_cluster *c = x->c;
+  const int nlmso = c->nlmso, nligneso = c->nligneso;
+  const int i = c->newinv[ii];
+  const int cal = x->cal_comp;
+  if (i > 0) {
+    /* Diagonal term, :1147-1150 */
+    double vme = x->V[i - 1] - x->Enervide;
+    abvr[ii] = vme - c->clapl[0];
+    if (x->E_comp) abvi[ii] = -x->Eimag;
+    for (int iv = 1; iv <= c->nvois; iv++) {
+      int j = IVOIS(c, i, iv);
+      int ia = c->numia[j - 1];
+      /* :1164-1232 with Kar = 1, Kari = 0, Repres_comp = F */
+      double cfvs_r = -c->clapl[iv];
+      double cfvs_i = 0.0;
+      cfvs_r = 1.0 * cfvs_r;
+      cfvs_r = cfvs_r * c->rvol[i - 1];
+      if (cal) cfvs_i = cfvs_i * c->rvol[i - 1];
+      if (ia == 0) { /* :1227-1234 */
+        int jj = c->newp[j];
+        abvr[jj] = abvr[jj] + cfvs_r / c->rvol[j - 1];
+        if (cal) abvi[jj] = abvi[jj] + cfvs_i / c->rvol[j - 1];
+      } else if (ia == -2) { /* :1312-1398 */
+        int ib;
+        for (ib = c->nsortf + 1; ib <= c->nsort; ib++)
+          if (j == c->isrt[ib - 1]) break;
+        double complex cfvs = CMPLX(cfvs_r, cfvs_i);
+        for (int lm = 1; lm <= nlmso; lm++) { /* regular solution: not irregular */
+          int L = c->lso[lm], m = c->mso[lm];
+          int lm0 = L * L + L + 1 + m;
+          int jjj = nligneso + lm;
+          if (cal) {
+            double complex ycomp = CMPLX(YLMSO(c, ib, lm0), 0.0);
+            double complex cfac;
+            if (x->basereel) {
+              cfac = ctx_neu(x, ib, L);
+            } else {
+              cfac = ctx_bes(x, ib, L) + I * ctx_neu(x, ib, L);
+              if (!x->E_comp) cfac = CMPLX(creal(ctx_bes(x, ib, L)), creal(ctx_neu(x, ib, L)));
+            }
+            double complex cfv = cfac * ycomp * cfvs;
+            abvr[jjj] = abvr[jjj] + creal(cfv);
+            abvi[jjj] = abvi[jjj] + cimag(cfv);
+          } else {
+            double fr = YLMSO(c, ib, lm0) * creal(ctx_neu(x, ib, L));
+            abvr[jjj] = abvr[jjj] + cfvs_r * fr;
+          }
+        }
+        for (int lm = 1; lm <= nlmso; lm++) {
+          int L = c->lso[lm], m = c->mso[lm];
+          int lm0 = L * L + L + 1 + m;
+          if (cal) {
+            double complex ycomp = CMPLX(YLMSO(c, ib, lm0), 0.0);
+            double complex cfv = ycomp * ctx_bes(x, ib, L) * cfvs;
+            smr_ii[lm - 1] = smr_ii[lm - 1] - creal(cfv);
+            smi_ii[lm - 1] = smi_ii[lm - 1] - cimag(cfv);
+          } else {
+            double fr = YLMSO(c, ib, lm0) * creal(ctx_bes(x, ib, L));
+            smr_ii[lm - 1] = smr_ii[lm - 1] - cfvs_r * fr;
+          }
+        }
+      }
+    }
+  } else if (i == 0) {
+    /* Outer-sphere rows, :1401-1511 */
+    const int lm = ii - nligneso;
+    const int m = c->mso[lm];
+    const int lm0 = c->lso[lm] * c->lso[lm] + c->lso[lm] + 1 + m;
+    const int isp = c->iso[lm];
+    for (int lmp = 1; lmp <= nlmso; lmp++) {
+      if (c->iso[lmp] != isp) continue;
+      int L = c->lso[lmp], mp = c->mso[lmp];
+      int lmp0 = L * L + L + 1 + mp;
+      int ljj = nligneso + lmp;
+      double complex charmc = 0;
+      double charmr = 0;
+      for (int ib = 1; ib <= c->nsortf; ib++) {
+        if (cal) {
+          double complex cfac = CMPLX(YLMSO(c, ib, lm0) * YLMSO(c, ib, lmp0), 0.0);
+          cfac = c->poidso[ib - 1] * cfac;
+          if (x->basereel) {
+            charmc = charmc + cfac * ctx_neu(x, ib, L);
+          } else if (x->E_comp) {
+            charmc = charmc + cfac * (ctx_bes(x, ib, L) + I * ctx_neu(x, ib, L));
+          } else {
+            charmc = charmc + cfac * CMPLX(creal(ctx_bes(x, ib, L)), creal(ctx_neu(x, ib, L)));
+          }
+        } else {
+          double fac = c->poidso[ib - 1] * YLMSO(c, ib, lm0) * YLMSO(c, ib, lmp0);
+          charmr = charmr + fac * creal(ctx_neu(x, ib, L));
+        }
+      }
+      if (cal) {
+        abvr[ljj] = abvr[ljj] + creal(charmc);
+        abvi[ljj] = abvi[ljj] + cimag(charmc);
+      } else {
+        abvr[ljj] = abvr[ljj] + charmr;
+      }
+    }
+    for (int ib = 1; ib <= c->nsortf; ib++) {
+      int j = c->isrt[ib - 1];
+      int jj = c->newp[j] - 1 + isp;
+      double fac = c->poidso[ib - 1] * YLMSO(c, ib, lm0);
+      double complex cfac = CMPLX(fac, 0.0);
+      abvr[jj] = abvr[jj] - fac / c->rvol[j - 1];
+      for (int lmf = 1; lmf <= nlmso; lmf++) {
+        if (isp != c->iso[lmf]) continue;
+        int L = c->lso[lmf], mf = c->mso[lmf];
+        int lmf0 = L * L + L + 1 + mf;
+        if (cal) {
+          double complex cfad = cfac * CMPLX(YLMSO(c, ib, lmf0), 0.0);
+          cfad = cfad * (x->E_comp ? ctx_bes(x, ib, L) : CMPLX(creal(ctx_bes(x, ib, L)), 0.0));
+          smr_ii[lmf - 1] = smr_ii[lmf - 1] - creal(cfad);
+          smi_ii[lmf - 1] = smi_ii[lmf - 1] - cimag(cfad);
+        } else {
+          double fad = fac * YLMSO(c, ib, lmf0);
+          smr_ii[lmf - 1] = smr_ii[lmf - 1] - fad * creal(ctx_bes(x, ib, L));
+        }
+      }
+    }
+  }
+}
+
+/* ---------------------------------------------------------------- mat_solve */
+/* Structural pattern of row ii as the reference's symbolic pass marks it,
+ * mat_solve_gaussian.f90:85-139 (natome = 0, nspino = 1, regular). */
+static void orc_mark_row(const orc_cluster *c, int ii, unsigned char *colnz) {
+  const int i = c->newinv[ii];
+  if (i > 0) {
+    for (int iv = 1; iv <= c->nvois; iv++) {
+      int j = IVOIS(c, i, iv);
+      int ia = c->numia[j - 1];
+      if (ia == 0) {
+        colnz[c->newp[j]] = 1;
+      } else if (ia == -2) {
+        for (int k = c->nligneso + 1; k <= c->nligneso + c->nlmso; k++) colnz[k] = 1;
+      }
+    }
+  } else if (i == 0) {
+    for (int k = c->nligneso + 1; k <= c->nligneso + c->nlmso; k++) colnz[k] = 1;
+    int lm = ii - c->nligneso;
+    int isp = c->iso[lm];
+    for (int ib = 1; ib <= c->nsortf; ib++) {
+      int j = c->isrt[ib - 1];
+      colnz[c->newp[j] - 1 + isp] = 1;
+    }
+  }
+}
+
+typedef void (*mark_fn)(const void *, int, unsigned char *);
+typedef void (*fill_fn)(const void *, int, double *, double *, double *, double *);
+
+/* The generic body of mat_solve_gaussian.f90:67-358.  mark() reproduces the
+ * symbolic pass per row, fill() stands for the calcMatRow call. */
+static int orc_mat_solve_core(int nligne, int nlmso, int cal_comp, const int *lb1, const int *lb2,
+                              const void *ud, mark_fn mark, fill_fn fill, double *smr, double *smi,
+                              int row_stop, int *fail_row, double *t_fill, double *t_tria, int64_t *nnz_out) {
+  unsigned char *colnz = calloc((size_t)nligne + 1, 1);
+  int *nb_not_zero = calloc((size_t)nligne + 1, sizeof(int));
+  int64_t *lbz = calloc((size_t)nligne + 1, sizeof(int64_t));
+  if (!colnz || !nb_not_zero || !lbz) return -1;
+  int64_t nb_not_zero_tot = 0;
+  int *ind_inv = NULL;
+  /* Symbolic passes, :68-152 */
+  for (int pass = 1; pass <= 2; pass++) {
+    int second = (pass == 2);
+    if (second) {
+      ind_inv = malloc(sizeof(int) * (size_t)(nb_not_zero_tot > 0 ? nb_not_zero_tot : 1));
+      if (!ind_inv) return -1;
+    }
+    nb_not_zero_tot = 0;
+    memset(colnz, 0, (size_t)nligne + 1);
+    memset(nb_not_zero, 0, sizeof(int) * ((size_t)nligne + 1));
+    for (int ii = nligne; ii >= 1; ii--) {
+      mark(ud, ii, colnz);
+      for (int j = ii - 1; j >= 1; j--) {
+        if (!colnz[j]) continue;
+        nb_not_zero_tot++;
+        if (second) {
+          nb_not_zero[ii]++;
+          ind_inv[nb_not_zero_tot - 1] = j;
+        }
+      }
+    }
+  }
+  int *ind = malloc(sizeof(int) * (size_t)(nb_not_zero_tot > 0 ? nb_not_zero_tot : 1)); /* :154-158 */
+  for (int64_t lk = 1; lk <= nb_not_zero_tot; lk++) ind[lk - 1] = ind_inv[nb_not_zero_tot - lk];
+  free(ind_inv);
+  lbz[1] = 0; /* :162-165 */
+  for (int ii = 2; ii <= nligne; ii++) lbz[ii] = lbz[ii - 1] + nb_not_zero[ii - 1];
+  if (nnz_out) *nnz_out = nb_not_zero_tot;
+
+  double *abr = calloc((size_t)(nb_not_zero_tot > 0 ? nb_not_zero_tot : 1), sizeof(double));
+  double *abi = cal_comp ? calloc((size_t)(nb_not_zero_tot > 0 ? nb_not_zero_tot : 1), sizeof(double)) : NULL;
+  double *abvr = calloc((size_t)nligne + 1, sizeof(double));
+  double *abvi = cal_comp ? calloc((size_t)nligne + 1, sizeof(double)) : NULL;
+  if (!abr || (cal_comp && !abi) || !abvr || (cal_comp && !abvi)) return -1;
+  int status = 0;
+  double tf = 0, tt = 0;
+  if (row_stop < 1) row_stop = 1;
+  for (int ii = nligne; ii >= row_stop; ii--) { /* boucle_ii, :196-317 */
+    double tp1 = cpu_seconds();
+    const int l1 = lb1[ii], l2 = lb2[ii];
+    memset(abvr + l1, 0, sizeof(double) * (size_t)(l2 - l1 + 1));
+    if (cal_comp) memset(abvi + l1, 0, sizeof(double) * (size_t)(l2 - l1 + 1));
+    double *smr_ii = smr + (size_t)(ii - 1) * nlmso;
+    double *smi_ii = cal_comp ? smi + (size_t)(ii - 1) * nlmso : NULL;
+    fill(ud, ii, abvr, abvi, smr_ii, smi_ii);
+    double tp2 = cpu_seconds();
+    tf += tp2 - tp1;
+    for (int j = l2; j >= ii + 1; j--) { /* :231-262 */
+      const double *smr_j = smr + (size_t)(j - 1) * nlmso;
+      if (cal_comp) {
+        const double facr = abvr[j], faci = abvi[j];
+        if (fabs(facr) > EPS15 || fabs(faci) > EPS15) {
+          const int64_t b = lbz[j], e = lbz[j] + nb_not_zero[j];
+          for (int64_t lk = b; lk < e; lk++) {
+            const int k = ind[lk];
+            abvr[k] = abvr[k] - facr * abr[lk] + faci * abi[lk];
+            abvi[k] = abvi[k] - facr * abi[lk] - faci * abr[lk];
+          }
+          const double *smi_j = smi + (size_t)(j - 1) * nlmso;
+          for (int l = 0; l < nlmso; l++) {
+            double sr = smr_ii[l] - facr * smr_j[l] + faci * smi_j[l];
+            double si = smi_ii[l] - facr * smi_j[l] - faci * smr_j[l];
+            smr_ii[l] = sr;
+            smi_ii[l] = si;
+          }
+        }
+      } else {
+        const double fac = abvr[j];
+        if (fabs(fac) > EPS15) {
+          const int64_t b = lbz[j], e = lbz[j] + nb_not_zero[j];
+          for (int64_t lk = b; lk < e; lk++) {
+            const int k = ind[lk];
+            abvr[k] = abvr[k] - fac * abr[lk];
+          }
+          for (int l = 0; l < nlmso; l++) smr_ii[l] = smr_ii[l] - fac * smr_j[l];
+        }
+      }
+    }
+    /* Normalisation, :264-302 */
+    if (cal_comp) {
+      double den = abvr[ii] * abvr[ii] + abvi[ii] * abvi[ii];
+      if (den < 1.e-20) {
+        if (fail_row) *fail_row = ii;
+        status = 1;
+        break;
+      }
+      double vnr = abvr[ii] / den, vni = -abvi[ii] / den;
+      const int64_t b = lbz[ii], e = lbz[ii] + nb_not_zero[ii];
+      for (int64_t lk = b; lk < e; lk++) {
+        const int k = ind[lk];
+        abr[lk] = abvr[k] * vnr - abvi[k] * vni;
+        abi[lk] = abvi[k] * vnr + abvr[k] * vni;
+      }
+      for (int l = 0; l < nlmso; l++) {
+        double fr = smr_ii[l] * vnr - smi_ii[l] * vni;
+        double fi = smr_ii[l] * vni + smi_ii[l] * vnr;
+        smr_ii[l] = fr;
+        smi_ii[l] = fi;
+      }
+    } else {
+      if (fabs(abvr[ii]) < 1.e-10) {
+        if (fail_row) *fail_row = ii;
+        status = 1;
+        break;
+      }
+      double vnr = 1 / abvr[ii];
+      const int64_t b = lbz[ii], e = lbz[ii] + nb_not_zero[ii];
+      for (int64_t lk = b; lk < e; lk++) abr[lk] = abvr[ind[lk]] * vnr;
+      for (int l = 0; l < nlmso; l++) smr_ii[l] = smr_ii[l] * vnr;
+    }
+    tt += cpu_seconds() - tp2;
+  }
+  if (status == 0 && row_stop == 1) {
+    for (int ii = 2; ii <= nligne; ii++) { /* Resolution, :348-358 */
+      double *sr = smr + (size_t)(ii - 1) * nlmso;
+      double *si = cal_comp ? smi + (size_t)(ii - 1) * nlmso : NULL;
+      for (int64_t lk = lbz[ii]; lk < lbz[ii] + nb_not_zero[ii]; lk++) {
+        const int j = ind[lk];
+        const double *srj = smr + (size_t)(j - 1) * nlmso;
+        if (cal_comp) {
+          const double *sij = smi + (size_t)(j - 1) * nlmso;
+          for (int l = 0; l < nlmso; l++) {
+            double a = sr[l] - abr[lk] * srj[l] + abi[lk] * sij[l];
+            double b = si[l] - abr[lk] * sij[l] - abi[lk] * srj[l];
+            sr[l] = a;
+            si[l] = b;
+          }
+        } else {
+          for (int l = 0; l < nlmso; l++) sr[l] = sr[l] - abr[lk] * srj[l];
+        }
+      }
+    }
+  }
+  if (t_fill) *t_fill = tf;
+  if (t_tria) *t_tria = tt;
+  free(colnz); free(nb_not_zero); free(lbz); free(ind); free(abr); free(abi); free(abvr); free(abvi);
+  return status;
+}
+
+typedef struct {
+  row_ctx rc;
+  int inc_wave;
+  double *scratch_r, *scratch_i;
+} grid_ud;
+
+static void grid_mark(const void *ud, int ii, unsigned char *colnz) {
+  orc_mark_row(((const grid_ud *)ud)->rc.c, ii, colnz);
+}
+static void grid_fill(const void *ud, int ii, double *abvr, double *abvi, double *smr_ii, double *smi_ii) {
+  const grid_ud *g = (const grid_ud *)ud;
+  if (g->inc_wave) {
+    orc_calcmatrow(&g->rc, ii, abvr, abvi, smr_ii, smi_ii);
+  } else {
+    int n = g->rc.c->nlmso;
+    memset(g->scratch_r, 0, sizeof(double) * n);
+    memset(g->scratch_i, 0, sizeof(double) * n);
+    orc_calcmatrow(&g->rc, ii, abvr, abvi, g->scratch_r, g->scratch_i);
+  }
+}
+
+static int make_ctx(const orc_cluster *c, const double *V, double Enervide, double Eimag, double Ecinetic_out,
+                    int E_comp, int cal_comp, int basereel, row_ctx *rc, double **bes, double **neu) {
+  /* Cal_comp / Basereel rules of mat.f90:96-104 are the caller's choice;
+   * the oracle only requires consistency: E_comp implies cal_comp. */
+  if (E_comp && !cal_comp) return -2;
+  if (!basereel && !cal_comp) return -2;
+  size_t nb = (size_t)c->nsort * (c->lmaxso + 1) * (E_comp ? 2 : 1);
+  *bes = malloc(sizeof(double) * (nb ? nb : 1));
+  *neu = malloc(sizeof(double) * (nb ? nb : 1));
+  orc_phiso(c, E_comp, Ecinetic_out, Eimag, *bes, *neu);
+  rc->c = c; rc->V = V; rc->Enervide = Enervide; rc->Eimag = Eimag;
+  rc->E_comp = E_comp; rc->cal_comp = cal_comp; rc->basereel = basereel;
+  rc->bes = *bes; rc->neu = *neu;
+  return 0;
+}
+
+int orc_solve(const orc_cluster *c, const double *V, double Enervide, double Eimag, double Ecinetic_out,
+              int E_comp, int cal_comp, int basereel, int inc_wave, double *smr, double *smi, int row_stop,
+              int *fail_row, double *t_fill, double *t_tria, int64_t *nnz_out) {
+  grid_ud g;
+  double *bes, *neu;
+  int rc = make_ctx(c, V, Enervide, Eimag, Ecinetic_out, E_comp, cal_comp, basereel, &g.rc, &bes, &neu);
+  if (rc) return rc;
+  g.inc_wave = inc_wave;
+  g.scratch_r = calloc(c->nlmso, sizeof(double));
+  g.scratch_i = calloc(c->nlmso, sizeof(double));
+  rc = orc_mat_solve_core(c->nligne, c->nlmso, cal_comp, c->lb1, c->lb2, &g, grid_mark, grid_fill, smr, smi,
+                          row_stop, fail_row, t_fill, t_tria, nnz_out);
+  free(g.scratch_r); free(g.scratch_i); free(bes); free(neu);
+  return rc;
+}
+
+int orc_assemble_dense(const orc_cluster *c, const double *V, double Enervide, double Eimag, double Ecinetic_out,
+                       int E_comp, int cal_comp, int basereel, double *a_re, double *a_im, double *rhs_re,
+                       double *rhs_im) {
+  row_ctx rc;
+  double *bes, *neu;
+  int st = make_ctx(c, V, Enervide, Eimag, Ecinetic_out, E_comp, cal_comp, basereel, &rc, &bes, &neu);
+  if (st) return st;
+  const int n = c->nligne, nl = c->nlmso;
+  double *br = calloc((size_t)n + 1, sizeof(double)), *bi = calloc((size_t)n + 1, sizeof(double));
+  memset(rhs_re, 0, sizeof(double) * (size_t)n * nl);
+  if (rhs_im) memset(rhs_im, 0, sizeof(double) * (size_t)n * nl);
+  double *dummy = calloc(nl, sizeof(double));
+  for (int ii = 1; ii <= n; ii++) {
+    memset(br, 0, sizeof(double) * (n + 1));
+    memset(bi, 0, sizeof(double) * (n + 1));
+    orc_calcmatrow(&rc, ii, br, cal_comp ? bi : NULL, rhs_re + (size_t)(ii - 1) * nl,
+                   cal_comp ? rhs_im + (size_t)(ii - 1) * nl : dummy);
+    for (int k = 1; k <= n; k++) {
+      a_re[(size_t)(k - 1) * n + ii - 1] = br[k];
+      if (a_im) a_im[(size_t)(k - 1) * n + ii - 1] = cal_comp ? bi[k] : 0.0;
+    }
+  }
+  free(br); free(bi); free(dummy); free(bes); free(neu);
+  return 0;
+}
+
+int orc_assemble_csr(const orc_cluster *c, const double *V, double Enervide, double Eimag, double Ecinetic_out,
+                     int E_comp, int cal_comp, int basereel, int64_t *row_ptr, int *col, double *val_re,
+                     double *val_im, double *rhs_re, double *rhs_im, int64_t *nnz) {
+  const int n = c->nligne, nl = c->nlmso;
+  unsigned char *mk = calloc((size_t)n + 1, 1);
+  /* pattern of row ii: its own diagonal plus the columns the symbolic pass marks */
+  int64_t cnt = 0;
+  for (int ii = 1; ii <= n; ii++) {
+    memset(mk, 0, (size_t)n + 1);
+    orc_mark_row(c, ii, mk);
+    mk[ii] = 1;
+    for (int k = 1; k <= n; k++) cnt += mk[k];
+  }
+  if (!col) {
+    *nnz = cnt;
+    free(mk);
+    return 0;
+  }
+  row_ctx rc;
+  double *bes, *neu;
+  int st = make_ctx(c, V, Enervide, Eimag, Ecinetic_out, E_comp, cal_comp, basereel, &rc, &bes, &neu);
+  if (st) { free(mk); return st; }
+  double *br = calloc((size_t)n + 1, sizeof(double)), *bi = calloc((size_t)n + 1, sizeof(double));
+  double *dummy = calloc(nl, sizeof(double));
+  memset(rhs_re, 0, sizeof(double) * (size_t)n * nl);
+  if (rhs_im) memset(rhs_im, 0, sizeof(double) * (size_t)n * nl);
+  int64_t p = 0;
+  row_ptr[0] = 0;
+  for (int ii = 1; ii <= n; ii++) {
+    memset(mk, 0, (size_t)n + 1);
+    orc_mark_row(c, ii, mk);
+    mk[ii] = 1;
+    const int l1 = c->lb1[ii], l2 = c->lb2[ii];
+    for (int k = l1; k <= l2; k++) { br[k] = 0; bi[k] = 0; }
+    orc_calcmatrow(&rc, ii, br, cal_comp ? bi : NULL, rhs_re + (size_t)(ii - 1) * nl,
+                   cal_comp ? rhs_im + (size_t)(ii - 1) * nl : dummy);
+    for (int k = 1; k <= n; k++) {
+      if (!mk[k]) continue;
+      col[p] = k;
+      val_re[p] = br[k];
+      if (val_im) val_im[p] = cal_comp ? bi[k] : 0.0;
+      p++;
+    }
+    row_ptr[ii] = p;
+  }
+  *nnz = p;
+  free(mk); free(br); free(bi); free(dummy); free(bes); free(neu);
+  return 0;
+}
+
+/* ---------------------------------------------------------------- generic */
+typedef struct {
+  const int64_t *row_ptr;
+  const int *col;
+  const double *vr, *vi;
+  int cal;
+} csr_ud;
+
+static void csr_mark(const void *ud, int ii, unsigned char *colnz) {
+  const csr_ud *u = (const csr_ud *)ud;
+  for (int64_t p = u->row_ptr[ii - 1]; p < u->row_ptr[ii]; p++) colnz[u->col[p]] = 1;
+}
+static void csr_fill(const void *ud, int ii, double *abvr, double *abvi, double *smr_ii, double *smi_ii) {
+  const csr_ud *u = (const csr_ud *)ud;
+  (void)smr_ii; (void)smi_ii;
+  for (int64_t p = u->row_ptr[ii - 1]; p < u->row_ptr[ii]; p++) {
+    abvr[u->col[p]] += u->vr[p];
+    if (u->cal) abvi[u->col[p]] += u->vi[p];
+  }
+}
+
+int orc_mat_solve_csr(int nligne, int nlmso, int cal_comp, const int *lb1, const int *lb2, const int64_t *row_ptr,
+                      const int *col, const double *val_re, const double *val_im, double *smr, double *smi,
+                      int *fail_row) {
+  int *l1 = malloc(sizeof(int) * (nligne + 1)), *l2 = malloc(sizeof(int) * (nligne + 1));
+  for (int i = 1; i <= nligne; i++) { l1[i] = lb1[i - 1]; l2[i] = lb2[i - 1]; }
+  csr_ud u = {row_ptr, col, val_re, val_im, cal_comp};
+  int st = orc_mat_solve_core(nligne, nlmso, cal_comp, l1, l2, &u, csr_mark, csr_fill, smr, smi, 1, fail_row,
+                              NULL, NULL, NULL);
+  free(l1); free(l2);
+  return st;
+}
+
+/* ---------------------------------------------------------------- dense */
+int orc_dense_solve(int n, int nrhs, double *a_re, double *a_im, double *b_re, double *b_im) {
+  double complex *a = malloc(sizeof(double complex) * (size_t)n * n);
+  double complex *b = malloc(sizeof(double complex) * (size_t)n * nrhs);
+  for (size_t k = 0; k < (size_t)n * n; k++) a[k] = CMPLX(a_re[k], a_im ? a_im[k] : 0.0);
+  for (size_t k = 0; k < (size_t)n * nrhs; k++) b[k] = CMPLX(b_re[k], b_im ? b_im[k] : 0.0);
+#define A(i, j) a[(size_t)(j) * n + (i)]
+#define B(i, j) b[(size_t)(j) * n + (i)]
+  int st = 0;
+  for (int k = 0; k < n; k++) {
+    int p = k;
+    double best = cabs(A(k, k));
+    for (int i = k + 1; i < n; i++)
+      if (cabs(A(i, k)) > best) { best = cabs(A(i, k)); p = i; }
+    if (best == 0.0) { st = 1; break; }
+    if (p != k) {
+      for (int j = 0; j < n; j++) { double complex t = A(k, j); A(k, j) = A(p, j); A(p, j) = t; }
+      for (int j = 0; j < nrhs; j++) { double complex t = B(k, j); B(k, j) = B(p, j); B(p, j) = t; }
+    }
+    double complex piv = A(k, k);
+    for (int i = k + 1; i < n; i++) {
+      double complex f = A(i, k) / piv;
+      if (f == 0) continue;
+      for (int j = k + 1; j < n; j++) A(i, j) -= f * A(k, j);
+      for (int j = 0; j < nrhs; j++) B(i, j) -= f * B(k, j);
+    }
+  }
+  if (!st) {
+    for (int j = 0; j < nrhs; j++)
+      for (int i = n - 1; i >= 0; i--) {
+        double complex s = B(i, j);
+        for (int k = i + 1; k < n; k++) s -= A(i, k) * B(k, j);
+        B(i, j) = s / A(i, i);
+      }
+    for (size_t k = 0; k < (size_t)n * nrhs; k++) {
+      b_re[k] = creal(b[k]);
+      if (b_im) b_im[k] = cimag(b[k]);
+    }
+  }
+#undef A
+#undef B
+  free(a); free(b);
+  return st;
+}

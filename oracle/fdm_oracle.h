@@ -1,0 +1,111 @@
+/*
+ * ORACLE — test infrastructure only.
+ *
+ * A plain-C, single-threaded restatement of the FDMNES finite-difference path
+ * (grid -> stencil -> outer-sphere boundary -> row assembly -> Gaussian band
+ * solver).  It is the checker for the HIP library in fdmnes_amd/: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and
+ * never as the thing measured or shipped.
+ *
+ * Every function cites the reference file:line it follows (paths relative to
+ * the reference's source/ directory).  Indices are 1-based wherever the
+ * reference's are, so each loop can be read against the Fortran line by line.
+ *
+ * Scope: cubic grid, point group P1 (rvol = 1, characters = 1), no atoms
+ * (natome = 0), nspino = 1, no spin-orbit, non-relativistic, real spherical
+ * harmonics (Ylm_comp = F), regular solution only (n_RI = 1), non-Rydberg
+ * outer-sphere functions.  This is the branch set the synthetic clusters use.
+ */
+#ifndef FDM_ORACLE_H
+#define FDM_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_cluster orc_cluster;
+
+/* Build grid + stencil + boundary + Ylm for a sphere of radius rsort (bohr),
+ * grid step adimp (bohr), finite-difference order iord (2 or 4) and outer
+ * expansion lmaxso.  Returns NULL on failure. */
+orc_cluster *orc_cluster_new(double rsort, double adimp, int iord, int lmaxso);
+void orc_cluster_free(orc_cluster *c);
+
+/* sizes: [0]=npoint [1]=npso [2]=nligne [3]=nlmso [4]=nsortf [5]=nsort [6]=nvois */
+void orc_cluster_sizes(const orc_cluster *c, int *sizes);
+/* xyz(4, npso) column-major (x,y,z,r) in bohr, reference order */
+void orc_cluster_xyz(const orc_cluster *c, double *xyz);
+/* indice(npso,3) as int32 column-major (ix block, then iy, then iz) */
+void orc_cluster_indice(const orc_cluster *c, int *ind);
+/* ivois(npso, nvois) column-major, 1-based point indices, 0 = none */
+void orc_cluster_ivois(const orc_cluster *c, int *ivois);
+/* isrt(nsort), poidso(nsortf) */
+void orc_cluster_boundary(const orc_cluster *c, int *isrt, double *poidso);
+/* Ylmso(nsort, nlmso) column-major */
+void orc_cluster_ylmso(const orc_cluster *c, double *ylmso);
+/* clapl(0:nvois) */
+void orc_cluster_clapl(const orc_cluster *c, double *clapl);
+/* lb1, lb2, newinv (nligne each), new(0:npoint) — newind restatement */
+void orc_cluster_profile(const orc_cluster *c, int *lb1, int *lb2, int *newinv, int *newp);
+
+/* Outer-sphere Bessel/Neumann (phiso, non-Rydberg branch).  E_comp selects
+ * complex (bes/neu interleaved re,im; shape (nsort, 0:lmaxso)) or real. */
+void orc_phiso(const orc_cluster *c, int E_comp, double Ecinetic_out, double Eimag,
+               double *bes, double *neu);
+
+/* One energy point: assemble every row with the calcMatRow restatement and
+ * run the mat_solve restatement on it.
+ *   V[npoint]        potential (Ry) on the interior points
+ *   smr[nlmso*nligne], smi[...]  in: initial RHS (column-major, RHS fastest)
+ *                    out: solution.  smi may be NULL when cal_comp == 0.
+ *   inc_wave         1 = calcMatRow adds its incoming-wave RHS (reference
+ *                    behaviour), 0 = the RHS rows it produces are discarded.
+ *   row_stop         elimination stops after row row_stop (1 = full solve;
+ *                    >1 = partial timing sample, no substitution).
+ * Returns 0, or 1 on a vanishing pivot (*fail_row = ii). */
+int orc_solve(const orc_cluster *c, const double *V, double Enervide, double Eimag,
+              double Ecinetic_out, int E_comp, int cal_comp, int basereel, int inc_wave,
+              double *smr, double *smi, int row_stop, int *fail_row,
+              double *t_fill, double *t_tria, int64_t *nnz_out);
+
+/* Assemble the full matrix densely (n <= ~4000) with calcMatRow, plus its RHS.
+ * a_re/a_im: n*n column-major; rhs_re/rhs_im: nlmso*n. */
+int orc_assemble_dense(const orc_cluster *c, const double *V, double Enervide, double Eimag,
+                       double Ecinetic_out, int E_comp, int cal_comp, int basereel,
+                       double *a_re, double *a_im, double *rhs_re, double *rhs_im);
+
+/* Assemble into compressed rows (structural pattern, as the reference's
+ * symbolic pass sees it; exact zeros kept).  Call with col == NULL to get
+ * the count in *nnz first.  rows are 1-based, cols 1-based. */
+int orc_assemble_csr(const orc_cluster *c, const double *V, double Enervide, double Eimag,
+                     double Ecinetic_out, int E_comp, int cal_comp, int basereel,
+                     int64_t *row_ptr, int *col, double *val_re, double *val_im,
+                     double *rhs_re, double *rhs_im, int64_t *nnz);
+
+/* Generic restatement of mat_solve_gaussian.f90:67-358 on a matrix given in
+ * compressed rows (1-based columns, structural pattern), with profile lb1/lb2.
+ * The symbolic pass marks, cumulatively from the last row upward, the columns
+ * of each row's pattern — the same rule as the reference's ColNotZero sweep.
+ * smr/smi as in orc_solve (the RHS is taken as given). */
+int orc_mat_solve_csr(int nligne, int nlmso, int cal_comp, const int *lb1, const int *lb2,
+                      const int64_t *row_ptr, const int *col, const double *val_re,
+                      const double *val_im, double *smr, double *smi, int *fail_row);
+
+/* Dense complex LU with partial pivoting (brute-force check), column-major
+ * n*n, nrhs right-hand sides b (n*nrhs column-major) overwritten by x.
+ * Returns 0 or 1 if singular. */
+int orc_dense_solve(int n, int nrhs, double *a_re, double *a_im, double *b_re, double *b_im);
+
+/* Spherical Bessel/Neumann recurrences (sphere.f90:7226-7248, 7356-7378). */
+void orc_cbessneu(double fn_re, double fn_im, double z_re, double z_im, int lmax,
+                  double *bes, double *neu);
+void orc_cbessneur(double fnorm, double z, int lmax, double *bes, double *neu);
+/* Real spherical harmonics of a direction (general.f90:11606-11720). */
+void orc_ylm_real(int lmax, const double *v, double r, double *ylmr);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

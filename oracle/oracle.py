@@ -1,0 +1,250 @@
+"""ORACLE — test infrastructure only.
+
+ctypes wrapper over ``oracle/liboracle.so``, the plain-C restatement of the
+FDMNES finite-difference path (see ``fdm_oracle.c`` for the file:line map).
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s cpu_baseline
+leg may import this module, and only as the checker.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_lp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.orc_cluster_new.restype = C.c_void_p
+        L.orc_cluster_new.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int]
+        L.orc_cluster_free.argtypes = [C.c_void_p]
+        L.orc_cluster_sizes.argtypes = [C.c_void_p, _ip]
+        L.orc_cluster_xyz.argtypes = [C.c_void_p, _dp]
+        L.orc_cluster_indice.argtypes = [C.c_void_p, _ip]
+        L.orc_cluster_ivois.argtypes = [C.c_void_p, _ip]
+        L.orc_cluster_boundary.argtypes = [C.c_void_p, _ip, _dp]
+        L.orc_cluster_ylmso.argtypes = [C.c_void_p, _dp]
+        L.orc_cluster_clapl.argtypes = [C.c_void_p, _dp]
+        L.orc_cluster_profile.argtypes = [C.c_void_p, _ip, _ip, _ip, _ip]
+        L.orc_phiso.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, _dp, _dp]
+        L.orc_solve.restype = C.c_int
+        L.orc_solve.argtypes = [C.c_void_p, _dp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                C.c_int, C.c_int, _dp, C.c_void_p, C.c_int, C.POINTER(C.c_int),
+                                C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+        L.orc_assemble_dense.restype = C.c_int
+        L.orc_assemble_dense.argtypes = [C.c_void_p, _dp, C.c_double, C.c_double, C.c_double, C.c_int,
+                                         C.c_int, C.c_int, _dp, _dp, _dp, _dp]
+        L.orc_assemble_csr.restype = C.c_int
+        L.orc_assemble_csr.argtypes = [C.c_void_p, _dp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                       C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                       C.c_void_p, C.POINTER(C.c_int64)]
+        L.orc_mat_solve_csr.restype = C.c_int
+        L.orc_mat_solve_csr.argtypes = [C.c_int, C.c_int, C.c_int, _ip, _ip, _lp, _ip, _dp, C.c_void_p, _dp,
+                                        C.c_void_p, C.POINTER(C.c_int)]
+        L.orc_dense_solve.restype = C.c_int
+        L.orc_dense_solve.argtypes = [C.c_int, C.c_int, _dp, C.c_void_p, _dp, C.c_void_p]
+        L.orc_cbessneu.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, _dp, _dp]
+        L.orc_cbessneur.argtypes = [C.c_double, C.c_double, C.c_int, _dp, _dp]
+        L.orc_ylm_real.argtypes = [C.c_int, _dp, C.c_double, _dp]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class Cluster:
+    """Grid, stencil, boundary and profile of one spherical FDM cluster.
+
+    ``rsort`` and ``adimp`` in bohr (reseau, general.f90:9439)."""
+
+    def __init__(self, rsort: float, adimp: float, lmaxso: int, iord: int = 4):
+        L = lib()
+        self._h = L.orc_cluster_new(rsort, adimp, iord, lmaxso)
+        if not self._h:
+            raise MemoryError("orc_cluster_new failed")
+        s = np.zeros(7, np.int32)
+        L.orc_cluster_sizes(self._h, s)
+        (self.npoint, self.npso, self.nligne, self.nlmso, self.nsortf, self.nsort, self.nvois) = map(int, s)
+        self.lmaxso = lmaxso
+        self.nligneso = self.nligne - self.nlmso
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc_cluster_free(self._h)
+            self._h = None
+
+    @property
+    def xyz(self):
+        a = np.zeros(4 * self.npso)
+        lib().orc_cluster_xyz(self._h, a)
+        return a.reshape(self.npso, 4)  # row p = (x, y, z, r) of point p+1
+
+    @property
+    def indice(self):
+        a = np.zeros(3 * self.npso, np.int32)
+        lib().orc_cluster_indice(self._h, a)
+        return a.reshape(3, self.npso).T.copy()
+
+    @property
+    def ivois(self):
+        a = np.zeros(self.npso * self.nvois, np.int32)
+        lib().orc_cluster_ivois(self._h, a)
+        return a.reshape(self.nvois, self.npso).T.copy()
+
+    @property
+    def boundary(self):
+        isrt = np.zeros(self.nsort, np.int32)
+        poidso = np.zeros(max(self.nsortf, 1))
+        lib().orc_cluster_boundary(self._h, isrt, poidso)
+        return isrt, poidso[: self.nsortf]
+
+    @property
+    def ylmso(self):
+        a = np.zeros(self.nsort * self.nlmso)
+        lib().orc_cluster_ylmso(self._h, a)
+        return a.reshape(self.nlmso, self.nsort).T.copy()
+
+    @property
+    def clapl(self):
+        a = np.zeros(self.nvois + 1)
+        lib().orc_cluster_clapl(self._h, a)
+        return a
+
+    def profile(self):
+        lb1 = np.zeros(self.nligne, np.int32)
+        lb2 = np.zeros(self.nligne, np.int32)
+        newinv = np.zeros(self.nligne, np.int32)
+        newp = np.zeros(self.npoint + 1, np.int32)
+        lib().orc_cluster_profile(self._h, lb1, lb2, newinv, newp)
+        return lb1, lb2, newinv, newp
+
+    def phiso(self, E_comp: bool, Ecinetic_out: float, Eimag: float):
+        n = self.nsort * (self.lmaxso + 1) * (2 if E_comp else 1)
+        b = np.zeros(max(n, 1))
+        nn = np.zeros(max(n, 1))
+        lib().orc_phiso(self._h, int(E_comp), Ecinetic_out, Eimag, b, nn)
+        if E_comp:
+            b = b[: 2 * self.nsort * (self.lmaxso + 1)].view(np.complex128)
+            nn = nn[: 2 * self.nsort * (self.lmaxso + 1)].view(np.complex128)
+        return b.reshape(self.lmaxso + 1, self.nsort).T, nn.reshape(self.lmaxso + 1, self.nsort).T
+
+    def solve(self, V, Enervide, Eimag, Ecinetic_out, E_comp, cal_comp, basereel, rhs=None,
+              inc_wave=True, row_stop=1):
+        """mat_solve restatement on calcMatRow rows.  rhs: (nligne, nlmso)
+        complex or real initial RHS (zero if None).  Returns (status, x,
+        fail_row, t_fill, t_tria, nnz)."""
+        V = np.ascontiguousarray(V, np.float64)
+        n, nl = self.nligne, self.nlmso
+        smr = np.zeros(n * nl)
+        smi = np.zeros(n * nl) if cal_comp else None
+        if rhs is not None:
+            smr[:] = np.real(rhs).reshape(-1)
+            if cal_comp:
+                smi[:] = np.imag(rhs).reshape(-1)
+        fr, tf, tt, nnz = C.c_int(0), C.c_double(0), C.c_double(0), C.c_int64(0)
+        st = lib().orc_solve(self._h, V, Enervide, Eimag, Ecinetic_out, int(E_comp), int(cal_comp),
+                             int(basereel), int(inc_wave), smr, _ptr(smi), int(row_stop), C.byref(fr),
+                             C.byref(tf), C.byref(tt), C.byref(nnz))
+        x = smr.reshape(n, nl) if not cal_comp else (smr + 1j * smi).reshape(n, nl)
+        return st, x, fr.value, tf.value, tt.value, nnz.value
+
+    def assemble_dense(self, V, Enervide, Eimag, Ecinetic_out, E_comp, cal_comp, basereel):
+        n, nl = self.nligne, self.nlmso
+        ar, ai = np.zeros(n * n), np.zeros(n * n)
+        rr, ri = np.zeros(n * nl), np.zeros(n * nl)
+        st = lib().orc_assemble_dense(self._h, np.ascontiguousarray(V, np.float64), Enervide, Eimag,
+                                      Ecinetic_out, int(E_comp), int(cal_comp), int(basereel), ar, ai, rr, ri)
+        if st:
+            raise ValueError(f"orc_assemble_dense: {st}")
+        A = (ar + 1j * ai).reshape(n, n).T  # column-major -> A[row, col]
+        b = (rr + 1j * ri).reshape(n, nl)
+        return (A, b) if cal_comp else (A.real.copy(), b.real.copy())
+
+    def assemble_csr(self, V, Enervide, Eimag, Ecinetic_out, E_comp, cal_comp, basereel):
+        L = lib()
+        V = np.ascontiguousarray(V, np.float64)
+        nnz = C.c_int64(0)
+        L.orc_assemble_csr(self._h, V, Enervide, Eimag, Ecinetic_out, int(E_comp), int(cal_comp),
+                           int(basereel), None, None, None, None, None, None, C.byref(nnz))
+        n, nl = self.nligne, self.nlmso
+        rp = np.zeros(n + 1, np.int64)
+        col = np.zeros(nnz.value, np.int32)
+        vr, vi = np.zeros(nnz.value), np.zeros(nnz.value)
+        rr, ri = np.zeros(n * nl), np.zeros(n * nl)
+        st = L.orc_assemble_csr(self._h, V, Enervide, Eimag, Ecinetic_out, int(E_comp), int(cal_comp),
+                                int(basereel), _ptr(rp), _ptr(col), _ptr(vr), _ptr(vi), _ptr(rr), _ptr(ri),
+                                C.byref(nnz))
+        if st:
+            raise ValueError(f"orc_assemble_csr: {st}")
+        vals = vr + 1j * vi if cal_comp else vr
+        b = (rr + 1j * ri).reshape(n, nl) if cal_comp else rr.reshape(n, nl)
+        return rp, col, vals, b
+
+
+def mat_solve_csr(lb1, lb2, row_ptr, col, vals, rhs, cal_comp):
+    """Generic mat_solve restatement (1-based lb1/lb2/col).  rhs (n, nrhs)."""
+    n, nl = rhs.shape
+    smr = np.ascontiguousarray(np.real(rhs), np.float64).reshape(-1).copy()
+    smi = np.ascontiguousarray(np.imag(rhs), np.float64).reshape(-1).copy() if cal_comp else None
+    vr = np.ascontiguousarray(np.real(vals), np.float64)
+    vi = np.ascontiguousarray(np.imag(vals), np.float64) if cal_comp else None
+    fr = C.c_int(0)
+    st = lib().orc_mat_solve_csr(n, nl, int(cal_comp), np.ascontiguousarray(lb1, np.int32),
+                                 np.ascontiguousarray(lb2, np.int32), np.ascontiguousarray(row_ptr, np.int64),
+                                 np.ascontiguousarray(col, np.int32), vr, _ptr(vi), smr, _ptr(smi), C.byref(fr))
+    x = (smr + 1j * smi).reshape(n, nl) if cal_comp else smr.reshape(n, nl)
+    return st, x, fr.value
+
+
+def dense_solve(A, B):
+    """Partial-pivoting dense LU (brute force).  A (n,n), B (n,nrhs)."""
+    n = A.shape[0]
+    nrhs = B.shape[1]
+    ar = np.ascontiguousarray(np.real(A).T).reshape(-1).copy()
+    ai = np.ascontiguousarray(np.imag(A).T).reshape(-1).copy()
+    br = np.ascontiguousarray(np.real(B).T).reshape(-1).copy()
+    bi = np.ascontiguousarray(np.imag(B).T).reshape(-1).copy()
+    st = lib().orc_dense_solve(n, nrhs, ar, _ptr(ai), br, _ptr(bi))
+    if st:
+        raise np.linalg.LinAlgError("singular")
+    return (br + 1j * bi).reshape(nrhs, n).T
+
+
+def cbessneu(fnorm: complex, z: complex, lmax: int):
+    b = np.zeros(2 * (lmax + 1))
+    n = np.zeros(2 * (lmax + 1))
+    lib().orc_cbessneu(fnorm.real, fnorm.imag, z.real, z.imag, lmax, b, n)
+    return b.view(np.complex128), n.view(np.complex128)
+
+
+def cbessneur(fnorm: float, z: float, lmax: int):
+    b = np.zeros(lmax + 1)
+    n = np.zeros(lmax + 1)
+    lib().orc_cbessneur(fnorm, z, lmax, b, n)
+    return b, n
+
+
+def ylm_real(lmax: int, v, r: float):
+    y = np.zeros((lmax + 1) ** 2)
+    lib().orc_ylm_real(lmax, np.ascontiguousarray(v, np.float64), r, y)
+    return y

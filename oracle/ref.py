@@ -1,0 +1,203 @@
+"""ORACLE — test infrastructure only.
+
+ctypes harness over ``oracle/_ref/libfdmnes_ref.so``: the reference FDMNES
+Fortran compiled from its own sources by ``oracle/Makefile.ref`` (flang).  It
+calls the reference's own ``newind`` (mat.f90:1761), ``phiso`` (mat.f90:739),
+``cbessneu``/``cbessneur`` (sphere.f90:7226/7356), ``cylm``/``ylmcr``
+(general.f90:11606/11692) and ``mat_solve`` (mat_solve_gaussian.f90:5, which
+calls the reference ``calcMatRow``, mat.f90:1034) on arrays of a cluster built
+by the oracle, so the restatement in ``fdm_oracle.c`` is pinned against the
+reference itself.  All Fortran arguments are passed by reference; logicals are
+4-byte integers (1 = .true.).
+
+A vanishing pivot makes the reference ``STOP``; callers that expect one run
+the call in a child process.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF_LIB = os.path.join(HERE, "_ref", "libfdmnes_ref.so")
+NOPSM = 64  # main.f90:46
+
+_lib = None
+
+
+def available() -> bool:
+    return os.path.exists(REF_LIB)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        # fdmx.f90 calls ZLOG, a non-standard intrinsic flang leaves external;
+        # lazy binding leaves that out-of-path symbol unresolved and unused.
+        _lib = C.CDLL(REF_LIB, mode=os.RTLD_LAZY | os.RTLD_LOCAL)
+    return _lib
+
+
+class _Args:
+    """Keeps every by-reference argument alive for the duration of a call."""
+
+    def __init__(self):
+        self.keep = []
+
+    def i(self, v):
+        a = np.array([v], np.int32)
+        self.keep.append(a)
+        return a.ctypes.data_as(C.c_void_p)
+
+    def d(self, v):
+        a = np.array([v], np.float64)
+        self.keep.append(a)
+        return a.ctypes.data_as(C.c_void_p)
+
+    def b(self, v):
+        return self.i(1 if v else 0)
+
+    def arr(self, a, dtype):
+        a = np.asfortranarray(np.asarray(a, dtype))
+        if a.size == 0:
+            a = np.zeros(1, dtype)
+        self.keep.append(a)
+        return a.ctypes.data_as(C.c_void_p)
+
+
+def cbessneu(fnorm: complex, z: complex, lmax: int):
+    A = _Args()
+    b = np.zeros(lmax + 1, np.complex128)
+    n = np.zeros(lmax + 1, np.complex128)
+    lib().cbessneu_(A.arr([fnorm], np.complex128), A.arr([z], np.complex128), A.i(lmax), A.i(lmax),
+                    b.ctypes.data_as(C.c_void_p), n.ctypes.data_as(C.c_void_p))
+    return b, n
+
+
+def cbessneur(fnorm: float, z: float, lmax: int):
+    A = _Args()
+    b = np.zeros(lmax + 1)
+    n = np.zeros(lmax + 1)
+    lib().cbessneur_(A.d(fnorm), A.d(z), A.i(lmax), A.i(lmax), b.ctypes.data_as(C.c_void_p),
+                     n.ctypes.data_as(C.c_void_p))
+    return b, n
+
+
+def ylm_real(lmax: int, v, r: float):
+    A = _Args()
+    nlmc = ((lmax + 1) * (lmax + 2)) // 2
+    nlmr = (lmax + 1) ** 2
+    yc = np.zeros(nlmc, np.complex128)
+    yr = np.zeros(nlmr)
+    lib().cylm_(A.i(lmax), A.arr(v, np.float64), A.d(r), yc.ctypes.data_as(C.c_void_p), A.i(nlmc))
+    lib().ylmcr_(A.i(lmax), A.i(nlmc), A.i(nlmr), yc.ctypes.data_as(C.c_void_p), yr.ctypes.data_as(C.c_void_p))
+    return yr
+
+
+def newind(cl):
+    """Reference newind on an oracle cluster (natome = 0, nspino = 1)."""
+    A = _Args()
+    n = cl.nligne
+    lb1 = np.zeros(n, np.int32)
+    lb2 = np.zeros(n, np.int32)
+    newinv = np.zeros(n, np.int32)
+    newp = np.zeros(cl.npoint + 1, np.int32)
+    isrt, _ = cl.boundary
+    numia = np.zeros(cl.npso, np.int32)
+    numia[cl.npoint:] = -2
+    xyz = cl.xyz.T  # (4, npso) Fortran order
+    lib().newind_(A.arr([0.0], np.float64), A.arr([0], np.int32), A.arr([0], np.int32), A.i(0), A.b(False),
+                  A.arr(isrt, np.int32), A.arr(cl.ivois, np.int32), lb1.ctypes.data_as(C.c_void_p),
+                  lb2.ctypes.data_as(C.c_void_p), A.i(0), A.i(1), A.i(0), A.arr([0], np.int32), A.i(1),
+                  newp.ctypes.data_as(C.c_void_p), newinv.ctypes.data_as(C.c_void_p), A.i(n),
+                  A.arr([0], np.int32), A.i(cl.nlmso), A.i(cl.npoint), A.i(cl.nsortf), A.i(1), A.i(cl.npso),
+                  A.i(cl.nsort), A.arr(numia, np.int32), A.i(cl.nvois), A.arr(xyz, np.float64))
+    return lb1, lb2, newinv, newp
+
+
+def phiso(cl, E_comp: bool, Ecinetic_out: float, Eimag: float, Enervide: float = 0.0):
+    """Reference phiso, non-Rydberg branch.  Returns (Bessel, Neuman) as
+    (nsort, lmaxso+1) arrays (complex when E_comp)."""
+    A = _Args()
+    isrt, _ = cl.boundary
+    ns, lm = cl.nsort, cl.lmaxso
+    if E_comp:
+        bes = np.zeros((ns, lm + 1), np.complex128, order="F")
+        neu = np.zeros((ns, lm + 1), np.complex128, order="F")
+        besr = np.zeros(1)
+        neur = np.zeros(1)
+        nsc, nsr = ns, 0
+    else:
+        bes = np.zeros(1, np.complex128)
+        neu = np.zeros(1, np.complex128)
+        besr = np.zeros((ns, lm + 1), order="F")
+        neur = np.zeros((ns, lm + 1), order="F")
+        nsc, nsr = 0, ns
+    p = lambda a: a.ctypes.data_as(C.c_void_p)
+    lib().phiso_(A.d(0.0), p(bes), p(besr), A.b(E_comp), A.d(Ecinetic_out), A.d(0.0), A.d(Eimag), A.b(False),
+                 A.d(Enervide), A.i(0), A.i(1), A.arr(isrt, np.int32), A.i(lm), A.i(0), p(neu), p(neur),
+                 A.i(cl.npso), A.i(ns), A.i(nsc), A.i(nsr), A.i(1), A.i(ns), A.d(0.0), A.d(0.0), A.b(False),
+                 A.d(0.0), A.arr(cl.xyz.T, np.float64))
+    return (bes, neu) if E_comp else (besr, neur)
+
+
+def mat_solve(cl, V, Enervide, Eimag, bes, neu, E_comp, cal_comp, basereel, rhs=None):
+    """Reference mat_solve (93 arguments, mat_solve_gaussian.f90:5-12) with
+    the reference calcMatRow assembling each row.  Returns the solution
+    (nligne, nlmso), real or complex."""
+    A = _Args()
+    n, nl, ns, lm = cl.nligne, cl.nlmso, cl.nsort, cl.lmaxso
+    lb1, lb2, newinv, newp = cl.profile()
+    isrt, poidso = cl.boundary
+    numia = np.zeros(cl.npso, np.int32)
+    numia[cl.npoint:] = -2
+    nvois = cl.nvois
+    smr = np.zeros((nl, n), order="F")
+    smi = np.zeros((nl, n), order="F") if cal_comp else np.zeros((1, 1), order="F")
+    if rhs is not None:
+        smr[:, :] = np.real(rhs).T
+        if cal_comp:
+            smi[:, :] = np.imag(rhs).T
+    if E_comp:
+        Bc, Nc = np.asfortranarray(bes, np.complex128), np.asfortranarray(neu, np.complex128)
+        Br, Nr = np.zeros(1), np.zeros(1)
+        nsc, nsr = ns, 0
+    else:
+        Bc, Nc = np.zeros(1, np.complex128), np.zeros(1, np.complex128)
+        Br, Nr = np.asfortranarray(bes, np.float64), np.asfortranarray(neu, np.float64)
+        nsc, nsr = 0, ns
+    kar = np.ones((NOPSM, 1), order="F")
+    kari = np.zeros((NOPSM, 1), order="F")
+    lso = np.array([L for L in range(lm + 1) for m in range(-L, L + 1)], np.int32)
+    mso = np.array([m for L in range(lm + 1) for m in range(-L, L + 1)], np.int32)
+    iso = np.ones(nl, np.int32)
+    isvois = np.ones((cl.npso, nvois), np.int32, order="F")
+    tf = np.zeros(1)
+    tt = np.zeros(1)
+    p = lambda a: a.ctypes.data_as(C.c_void_p)
+    args = [
+        A.b(False), A.b(basereel), p(Bc), p(Br), A.b(cal_comp), A.arr(np.zeros(nvois), np.float64),
+        A.arr(cl.clapl, np.float64), A.b(E_comp), A.d(Eimag), A.d(Enervide), A.arr(np.zeros(3), np.float64),
+        A.arr([0], np.int32), A.arr([0], np.int32), A.arr([0], np.int32), A.i(0), A.i(1), A.i(1), A.i(0),
+        A.b(False), A.arr([0], np.int32), A.arr(iso, np.int32), A.i(1), A.arr(isrt, np.int32),
+        A.arr(isvois, np.int32), A.arr(cl.ivois, np.int32), A.arr(kar, np.float64), A.arr(kari, np.float64),
+        A.arr([0], np.int32), A.arr(lb1, np.int32), A.arr(lb2, np.int32), A.i(lm), A.arr(lso, np.int32),
+        A.arr([0], np.int32), A.i(1), A.i(0), A.arr(mso, np.int32), A.i(1), A.i(1), A.i(0), A.i(1),
+        A.arr([0], np.int32), A.arr([0], np.int32), A.i(1), p(Nc), p(Nr), A.arr(newp, np.int32),
+        A.arr(newinv, np.int32), A.i(1), A.i(1), A.i(cl.npoint), A.i(n), A.i(n if cal_comp else 0),
+        A.i(cl.nligneso), A.i(1), A.i(1), A.i(1), A.i(nl), A.arr([0], np.int32), A.i(nl),
+        A.i(nl if cal_comp else 0), A.i(1), A.i(1), A.i(cl.npoint), A.i(cl.npso), A.i(max(cl.nsortf, 1)),
+        A.i(nl), A.i(ns), A.i(nsc), A.i(nsr), A.i(cl.nsortf), A.i(1), A.i(1), A.i(1), A.i(1), A.i(ns),
+        A.arr(numia, np.int32), A.i(nvois), A.arr(np.zeros(1), np.float64), A.arr(np.zeros(1), np.float64),
+        A.arr(poidso, np.float64), A.b(False), A.b(False), A.arr(np.ones(cl.npoint), np.float64), p(smi), p(smr),
+        A.b(False), A.i(0), p(tf), p(tt), A.arr(np.asarray(V, np.float64), np.float64), A.b(False),
+        A.arr(np.zeros(1), np.float64), A.arr(cl.ylmso, np.float64),
+    ]
+    assert len(args) == 93
+    lib().mat_solve_(*args)
+    x = smr.T.copy()
+    if cal_comp:
+        x = x + 1j * smi.T
+    return x
