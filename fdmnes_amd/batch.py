@@ -1,0 +1,113 @@
+"""Device-resident batched solver (``fdmgpu_solver_*`` in include/fdmgpu.h).
+
+Device memory comes from torch tensors; the library launches on the torch
+stream current at creation, so torch events bracket its work."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+
+FP64, FP32 = 0, 1
+_V = C.c_void_p
+
+
+def _sigs(L):
+    if getattr(L, "_solver_sigs", False):
+        return
+    P = C.POINTER
+    L.fdmgpu_solver_create.argtypes = [C.c_int, C.c_int, C.c_int, _V, P(_V)]
+    L.fdmgpu_solver_destroy.argtypes = [_V]
+    L.fdmgpu_solver_destroy.restype = None
+    L.fdmgpu_solver_add_system.argtypes = [_V, C.c_int, _lib.ip, _lib.ip, _lib.lp, _lib.ip, P(C.c_int)]
+    L.fdmgpu_solver_solve.argtypes = [_V, C.c_int, P(C.c_int), P(_V), P(_V), P(_V), C.c_int, C.c_double,
+                                      P(C.c_int), P(C.c_int), P(C.c_double)]
+    L.fdmgpu_solver_adjoint.argtypes = [_V, C.c_int, P(C.c_int), P(_V), P(_V), P(_V)]
+    L.fdmgpu_solver_sigma.argtypes = [_V, C.c_int, P(C.c_int), P(_V), P(_V), P(C.c_double)]
+    L.fdmgpu_solver_options.argtypes = [_V, C.c_int, C.c_int, C.c_int]
+    L.fdmgpu_solver_stats.argtypes = [_V, P(C.c_double)]
+    L._solver_sigs = True
+
+
+def _ptrs(ts):
+    arr = (_V * len(ts))()
+    for k, t in enumerate(ts):
+        if not t.is_contiguous():
+            raise ValueError("device buffers must be contiguous")
+        arr[k] = t.data_ptr()
+    return arr
+
+
+def as_real(t: torch.Tensor) -> torch.Tensor:
+    """Interleaved float64 view of a complex128 tensor (float64 passes through)."""
+    return torch.view_as_real(t) if t.is_complex() else t
+
+
+class DeviceSolver:
+    def __init__(self, precision: int, cal_comp: bool, nrhs: int, stream: torch.cuda.Stream | None = None):
+        L = _lib.lib()
+        _sigs(L)
+        self.L = L
+        self.cal = bool(cal_comp)
+        self.nrhs = nrhs
+        self.precision = precision
+        st = (stream or torch.cuda.current_stream()).cuda_stream
+        h = _V()
+        _lib.check(L.fdmgpu_solver_create(precision, int(self.cal), nrhs, _V(st), C.byref(h)))
+        self._h = h
+        self.sizes = []
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self.L.fdmgpu_solver_destroy(self._h)
+            self._h = None
+
+    def add_system(self, lb1, lb2, row_ptr, col) -> int:
+        s = C.c_int(-1)
+        _lib.check(self.L.fdmgpu_solver_add_system(self._h, len(lb1), np.ascontiguousarray(lb1, np.int32),
+                                                   np.ascontiguousarray(lb2, np.int32),
+                                                   np.ascontiguousarray(row_ptr, np.int64),
+                                                   np.ascontiguousarray(col, np.int32), C.byref(s)))
+        self.sizes.append(len(lb1))
+        return s.value
+
+    def options(self, use_graphs=True, time_updates=False, persist_wg=0):
+        _lib.check(self.L.fdmgpu_solver_options(self._h, int(use_graphs), int(time_updates), int(persist_wg)))
+
+    def solve(self, slots, vals, b, x, max_sweeps=5, tol=1e-12, raise_on_pivot=True):
+        """vals/b/x: lists of device tensors (complex128 or float64)."""
+        n = len(slots)
+        ids = (C.c_int * n)(*slots)
+        fr = (C.c_int * n)()
+        sw = (C.c_int * n)()
+        rel = (C.c_double * n)()
+        rc = self.L.fdmgpu_solver_solve(self._h, n, ids, _ptrs([as_real(v) for v in vals]),
+                                        _ptrs([as_real(t) for t in b]), _ptrs([as_real(t) for t in x]),
+                                        int(max_sweeps), float(tol), fr, sw, rel)
+        if rc == 1 and not raise_on_pivot:
+            return list(fr), list(sw), list(rel)
+        _lib.check(rc, max(fr))
+        return list(fr), list(sw), list(rel)
+
+    def adjoint(self, slots, src, x, grad):
+        n = len(slots)
+        ids = (C.c_int * n)(*slots)
+        _lib.check(self.L.fdmgpu_solver_adjoint(self._h, n, ids, _ptrs([as_real(t) for t in src]),
+                                                _ptrs([as_real(t) for t in x]), _ptrs(grad)))
+
+    def sigma(self, slots, src, x) -> np.ndarray:
+        n = len(slots)
+        ids = (C.c_int * n)(*slots)
+        out = (C.c_double * n)()
+        _lib.check(self.L.fdmgpu_solver_sigma(self._h, n, ids, _ptrs([as_real(t) for t in src]),
+                                              _ptrs([as_real(t) for t in x]), out))
+        return np.array(out[:])
+
+    def stats(self) -> dict:
+        out = (C.c_double * 6)()
+        _lib.check(self.L.fdmgpu_solver_stats(self._h, out))
+        return {"update_ms": out[0], "update_launches": int(out[1]), "tile_macs": out[2],
+                "profile_macs": out[3], "factor_bytes": out[4], "profile_entries": out[5]}

@@ -1,0 +1,324 @@
+// C ABI of libfdmgpu.so (declared in include/fdmgpu.h).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "../../include/fdmgpu.h"
+#include "batch.h"
+
+using namespace fdmgpu;
+
+namespace {
+
+struct Lib {
+  bool ready = false;
+  int device = -1;
+  hipStream_t stream = nullptr;
+};
+Lib g_lib;
+
+int ensure_init() {
+  if (g_lib.ready) {
+    FDM_HIP(hipSetDevice(g_lib.device));
+    return 0;
+  }
+  return fdmgpu_init(0);
+}
+
+double now() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// Device buffer that frees itself.
+template <class T>
+struct DBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  int alloc(size_t count) {
+    n = count;
+    FDM_HIP(hipMalloc(&p, sizeof(T) * (count ? count : 1)));
+    return 0;
+  }
+  ~DBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+int check_csr(int n, const int *lb1, const int *lb2, const int64_t *row_ptr, const int *col) {
+  if (row_ptr[0] != 0) {
+    set_error("row_ptr[0] must be 0");
+    return FDMGPU_EINVAL;
+  }
+  for (int i = 0; i < n; i++) {
+    if (row_ptr[i + 1] < row_ptr[i]) {
+      set_error("row_ptr must be non-decreasing");
+      return FDMGPU_EINVAL;
+    }
+    for (int64_t q = row_ptr[i]; q < row_ptr[i + 1]; q++)
+      if (col[q] < lb1[i] || col[q] > lb2[i]) {
+        set_error("column " + std::to_string(col[q]) + " of row " + std::to_string(i + 1) +
+                  " lies outside its profile [lb1, lb2]");
+        return FDMGPU_EINVAL;
+      }
+  }
+  return 0;
+}
+
+// Host-pointer drop-in on top of a one-slot device solver.
+template <class R, bool C>
+int host_solve(int n, int nrhs, const int *lb1, const int *lb2, const int64_t *row_ptr, const int *col,
+               const double *a_re, const double *a_im, double *smr, double *smi, int max_sweeps, double tol,
+               int *sweeps, double *final_rel, int *fail_row, double *t_fill, double *t_tria) {
+  constexpr int NC = C ? 2 : 1;
+  const double t0 = now();
+  Solver<R, C> sv;
+  sv.use_graphs = false;
+  int slot = -1;
+  if (sv.init(nrhs, g_lib.stream)) return FDMGPU_EHIP;
+  int rc = sv.add_system(n, lb1, lb2, row_ptr, col, &slot);
+  if (rc) return rc == -1 ? FDMGPU_EINVAL : FDMGPU_EHIP;
+  const int64_t nnz = row_ptr[n];
+  std::vector<double> hv((size_t)std::max<int64_t>(nnz, 1) * NC), hb((size_t)n * nrhs * NC);
+  for (int64_t q = 0; q < nnz; q++) {
+    hv[q * NC] = a_re[q];
+    if (C) hv[q * NC + 1] = a_im[q];
+  }
+  for (int64_t i = 0; i < (int64_t)n * nrhs; i++) {
+    hb[i * NC] = smr[i];
+    if (C) hb[i * NC + 1] = smi[i];
+  }
+  DBuf<double> dv, db, dx;
+  if (dv.alloc(hv.size()) || db.alloc(hb.size()) || dx.alloc(hb.size())) return FDMGPU_ENOMEM;
+  FDM_HIP(hipMemcpy(dv.p, hv.data(), sizeof(double) * hv.size(), hipMemcpyHostToDevice));
+  FDM_HIP(hipMemcpy(db.p, hb.data(), sizeof(double) * hb.size(), hipMemcpyHostToDevice));
+  const double t1 = now();
+  const double *vp = dv.p, *bp = db.p;
+  double *xp = dx.p;
+  int fr = 0, sw = 0;
+  double rel = 0;
+  if (sv.solve(1, &slot, &vp, &bp, &xp, max_sweeps, tol, &fr, &sw, &rel)) return FDMGPU_EHIP;
+  const double t2 = now();
+  if (t_fill) *t_fill += t1 - t0;
+  if (t_tria) *t_tria += t2 - t1;
+  if (fail_row) *fail_row = fr;
+  if (fr) return FDMGPU_PIVOT;
+  FDM_HIP(hipMemcpy(hb.data(), dx.p, sizeof(double) * hb.size(), hipMemcpyDeviceToHost));
+  for (int64_t i = 0; i < (int64_t)n * nrhs; i++) {
+    smr[i] = hb[i * NC];
+    if (C) smi[i] = hb[i * NC + 1];
+  }
+  if (sweeps) *sweeps = sw;
+  if (final_rel) *final_rel = rel;
+  return FDMGPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fdmgpu_init(int device) {
+  if (g_lib.ready && g_lib.device == device) {
+    FDM_HIP(hipSetDevice(device));
+    return 0;
+  }
+  int count = 0;
+  FDM_HIP(hipGetDeviceCount(&count));
+  if (device < 0 || device >= count) {
+    set_error("no HIP device " + std::to_string(device));
+    return FDMGPU_EINVAL;
+  }
+  FDM_HIP(hipSetDevice(device));
+  if (g_lib.stream) (void)hipStreamDestroy(g_lib.stream);
+  FDM_HIP(hipStreamCreateWithFlags(&g_lib.stream, hipStreamNonBlocking));
+  g_lib.device = device;
+  g_lib.ready = true;
+  return 0;
+}
+
+int fdmgpu_finalize(void) {
+  if (g_lib.stream) (void)hipStreamDestroy(g_lib.stream);
+  g_lib = Lib();
+  return 0;
+}
+
+const char *fdmgpu_last_error(void) { return get_error(); }
+
+void fdmgpu_solver_name(char out[5]) { memcpy(out, "HIP  ", 5); }
+
+int fdmgpu_mat_solve(int nligne, int nlmso, int cal_comp, const int *lb1, const int *lb2, const int64_t *row_ptr,
+                     const int *col, const double *a_re, const double *a_im, double *smr, double *smi,
+                     int *fail_row, double *t_fill, double *t_tria) {
+  if (nligne <= 0 || nlmso <= 0 || !lb1 || !lb2 || !row_ptr || !col || !a_re || !smr ||
+      (cal_comp && (!a_im || !smi))) {
+    set_error("fdmgpu_mat_solve: invalid argument");
+    return FDMGPU_EINVAL;
+  }
+  int rc = ensure_init();
+  if (rc) return rc;
+  return cal_comp ? host_solve<double, true>(nligne, nlmso, lb1, lb2, row_ptr, col, a_re, a_im, smr, smi, 0, 0.0,
+                                             nullptr, nullptr, fail_row, t_fill, t_tria)
+                  : host_solve<double, false>(nligne, nlmso, lb1, lb2, row_ptr, col, a_re, a_im, smr, smi, 0, 0.0,
+                                              nullptr, nullptr, fail_row, t_fill, t_tria);
+}
+
+int fdmgpu_mat_solve_mixed(int nligne, int nlmso, int cal_comp, const int *lb1, const int *lb2,
+                           const int64_t *row_ptr, const int *col, const double *a_re, const double *a_im,
+                           double *smr, double *smi, int max_sweeps, double tol, int *sweeps, double *final_rel,
+                           int *fail_row) {
+  if (nligne <= 0 || nlmso <= 0 || nlmso > 256 || !lb1 || !lb2 || !row_ptr || !col || !a_re || !smr ||
+      (cal_comp && (!a_im || !smi)) || max_sweeps < 0) {
+    set_error("fdmgpu_mat_solve_mixed: invalid argument");
+    return FDMGPU_EINVAL;
+  }
+  int rc = ensure_init();
+  if (rc) return rc;
+  return cal_comp ? host_solve<float, true>(nligne, nlmso, lb1, lb2, row_ptr, col, a_re, a_im, smr, smi, max_sweeps,
+                                            tol, sweeps, final_rel, fail_row, nullptr, nullptr)
+                  : host_solve<float, false>(nligne, nlmso, lb1, lb2, row_ptr, col, a_re, a_im, smr, smi,
+                                             max_sweeps, tol, sweeps, final_rel, fail_row, nullptr, nullptr);
+}
+
+// ---------------------------------------------------------------- device solver
+struct fdmgpu_solver {
+  int prec = 0, cal = 0;
+  void *impl = nullptr;
+};
+
+#define FDM_DISPATCH(S, CALL)                                                          \
+  ((S)->prec == FDMGPU_FP64                                                            \
+       ? ((S)->cal ? ((Solver<double, true> *)(S)->impl)->CALL                         \
+                   : ((Solver<double, false> *)(S)->impl)->CALL)                       \
+       : ((S)->cal ? ((Solver<float, true> *)(S)->impl)->CALL                          \
+                   : ((Solver<float, false> *)(S)->impl)->CALL))
+
+static int map_rc(int rc) { return rc == 0 ? 0 : (rc == -1 ? FDMGPU_EINVAL : FDMGPU_EHIP); }
+
+int fdmgpu_solver_create(int precision, int cal_comp, int nrhs, void *hip_stream, fdmgpu_solver **out) {
+  if (!out || nrhs <= 0 || nrhs > 256 || (precision != FDMGPU_FP64 && precision != FDMGPU_FP32)) {
+    set_error("fdmgpu_solver_create: invalid argument");
+    return FDMGPU_EINVAL;
+  }
+  int rc = ensure_init();
+  if (rc) return rc;
+  fdmgpu_solver *s = new fdmgpu_solver();
+  s->prec = precision;
+  s->cal = cal_comp ? 1 : 0;
+  hipStream_t st = hip_stream ? (hipStream_t)hip_stream : g_lib.stream;
+  if (precision == FDMGPU_FP64) {
+    if (s->cal) s->impl = new Solver<double, true>();
+    else s->impl = new Solver<double, false>();
+  } else {
+    if (s->cal) s->impl = new Solver<float, true>();
+    else s->impl = new Solver<float, false>();
+  }
+  rc = FDM_DISPATCH(s, init(nrhs, st));
+  if (rc) {
+    fdmgpu_solver_destroy(s);
+    return map_rc(rc);
+  }
+  *out = s;
+  return 0;
+}
+
+void fdmgpu_solver_destroy(fdmgpu_solver *s) {
+  if (!s) return;
+  if (s->impl) {
+    if (s->prec == FDMGPU_FP64) {
+      if (s->cal) delete (Solver<double, true> *)s->impl;
+      else delete (Solver<double, false> *)s->impl;
+    } else {
+      if (s->cal) delete (Solver<float, true> *)s->impl;
+      else delete (Solver<float, false> *)s->impl;
+    }
+  }
+  delete s;
+}
+
+int fdmgpu_solver_add_system(fdmgpu_solver *s, int nligne, const int *lb1, const int *lb2, const int64_t *row_ptr,
+                             const int *col, int *slot) {
+  if (!s || nligne <= 0 || !lb1 || !lb2 || !row_ptr || !col || !slot) {
+    set_error("fdmgpu_solver_add_system: invalid argument");
+    return FDMGPU_EINVAL;
+  }
+  return map_rc(FDM_DISPATCH(s, add_system(nligne, lb1, lb2, row_ptr, col, slot)));
+}
+
+int fdmgpu_solver_solve(fdmgpu_solver *s, int count, const int *slots, const double *const *d_vals,
+                        const double *const *d_b, double *const *d_x, int max_sweeps, double tol, int *fail_rows,
+                        int *sweeps, double *rel) {
+  if (!s || count <= 0 || count > MAXG || !slots || !d_vals || !d_b || !d_x || !fail_rows) {
+    set_error("fdmgpu_solver_solve: invalid argument");
+    return FDMGPU_EINVAL;
+  }
+  int rc = map_rc(FDM_DISPATCH(s, solve(count, slots, d_vals, d_b, d_x, max_sweeps, tol, fail_rows, sweeps, rel)));
+  if (rc) return rc;
+  for (int k = 0; k < count; k++)
+    if (fail_rows[k]) return FDMGPU_PIVOT;
+  return 0;
+}
+
+int fdmgpu_solver_adjoint(fdmgpu_solver *s, int count, const int *slots, const double *const *d_src,
+                          const double *const *d_x, double *const *d_grad) {
+  if (!s || count <= 0 || count > MAXG || !slots || !d_src || !d_x || !d_grad) {
+    set_error("fdmgpu_solver_adjoint: invalid argument");
+    return FDMGPU_EINVAL;
+  }
+  return map_rc(FDM_DISPATCH(s, adjoint(count, slots, d_src, d_x, d_grad)));
+}
+
+int fdmgpu_solver_sigma(fdmgpu_solver *s, int count, const int *slots, const double *const *d_src,
+                        const double *const *d_x, double *sigma) {
+  if (!s || count <= 0 || !slots || !d_src || !d_x || !sigma) {
+    set_error("fdmgpu_solver_sigma: invalid argument");
+    return FDMGPU_EINVAL;
+  }
+  return map_rc(FDM_DISPATCH(s, sigma(count, slots, d_src, d_x, sigma)));
+}
+
+int fdmgpu_solver_options(fdmgpu_solver *s, int use_graphs, int time_updates, int persist_wg) {
+  if (!s) return FDMGPU_EINVAL;
+  auto apply = [&](auto *p) {
+    p->use_graphs = use_graphs != 0;
+    p->grp.time_updates = time_updates != 0;
+    if (persist_wg > 0) p->grp.persist_wg = persist_wg;
+  };
+  if (s->prec == FDMGPU_FP64) {
+    if (s->cal) apply((Solver<double, true> *)s->impl);
+    else apply((Solver<double, false> *)s->impl);
+  } else {
+    if (s->cal) apply((Solver<float, true> *)s->impl);
+    else apply((Solver<float, false> *)s->impl);
+  }
+  return 0;
+}
+
+int fdmgpu_solver_stats(const fdmgpu_solver *s, double *out) {
+  if (!s || !out) return FDMGPU_EINVAL;
+  auto fill = [&](auto *p) {
+    out[0] = p->grp.update_ms;
+    out[1] = (double)p->grp.update_launches;
+    double tm = 0, fm = 0, sm = 0;
+    for (auto &sl : p->slots) {
+      tm += sl.plan->tile_update_macs();
+      fm += sl.factor_macs;
+      sm += sl.solve_macs;
+    }
+    out[2] = tm;
+    out[3] = fm;
+    out[4] = (double)p->grp.factor_bytes();
+    out[5] = sm;
+  };
+  if (s->prec == FDMGPU_FP64) {
+    if (s->cal) fill((Solver<double, true> *)s->impl);
+    else fill((Solver<double, false> *)s->impl);
+  } else {
+    if (s->cal) fill((Solver<float, true> *)s->impl);
+    else fill((Solver<float, false> *)s->impl);
+  }
+  return 0;
+}
+
+}  // extern "C"

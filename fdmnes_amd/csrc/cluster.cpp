@@ -1,0 +1,404 @@
+// Synthetic FDM cluster builder (host).  See include/fdmgpu_cluster.h for the
+// reference routines each step stands in for.
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/fdmgpu_cluster.h"
+
+namespace {
+
+typedef std::complex<double> cd;
+constexpr double kPi = 3.1415926535897932384626433832;
+constexpr double kEpsPos = 1e-4;  // tolerance on point positions (main.f90:83)
+
+struct GridPoint {
+  int ix, iy, iz;
+};
+
+// Stencil directions in the reference's neighbour order (laplac,
+// general.f90:9760-9772): distance io = 1, 2; axis x, y, z; side -, +.
+const int kDir[12][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1},
+                         {-2, 0, 0}, {2, 0, 0}, {0, -2, 0}, {0, 2, 0}, {0, 0, -2}, {0, 0, 2}};
+
+// Real spherical harmonics Y_lm(r^) for l <= lmax, index l*l + l + m,
+// tesseral convention without the Condon-Shortley phase (the reference's
+// cylm + ylmcr, general.f90:11606-11720).  Normalised associated Legendre
+// functions by the standard three-term recurrence in l.
+void real_ylm(int lmax, double x, double y, double z, double *out) {
+  const double r = std::sqrt(x * x + y * y + z * z);
+  const double rho = std::sqrt(x * x + y * y);
+  const double ct = (r > 1e-6) ? z / r : 1.0;
+  double st = 0;
+  cd eph(1.0, 0.0);
+  if (rho > 1e-6) {
+    st = std::sqrt(1 - ct * ct);
+    eph = cd(x, y) / rho;
+  }
+  const int nl = lmax + 1;
+  std::vector<double> P((size_t)nl * nl, 0.0);  // P[l*nl + m]
+  P[0] = 1.0 / std::sqrt(4 * kPi);
+  for (int m = 1; m <= lmax; m++) P[m * nl + m] = std::sqrt((2 * m + 1.0) / (2.0 * m)) * st * P[(m - 1) * nl + m - 1];
+  for (int m = 0; m < lmax; m++) P[(m + 1) * nl + m] = std::sqrt(2 * m + 3.0) * ct * P[m * nl + m];
+  for (int m = 0; m <= lmax; m++)
+    for (int l = m + 2; l <= lmax; l++) {
+      const double a = std::sqrt((4.0 * l * l - 1) / ((double)l * l - (double)m * m));
+      const double b = std::sqrt(((l - 1.0) * (l - 1.0) - (double)m * m) / (4.0 * (l - 1.0) * (l - 1.0) - 1));
+      P[l * nl + m] = a * (ct * P[(l - 1) * nl + m] - b * P[(l - 2) * nl + m]);
+    }
+  cd em(1.0, 0.0);
+  std::vector<cd> pw(nl);
+  for (int m = 0; m <= lmax; m++) {
+    pw[m] = em;
+    em *= eph;
+  }
+  for (int l = 0; l <= lmax; l++) {
+    out[l * l + l] = P[l * nl];
+    for (int m = 1; m <= l; m++) {
+      const double v = std::sqrt(2.0) * P[l * nl + m];
+      out[l * l + l + m] = (st > 0) ? v * pw[m].real() : 0.0;
+      out[l * l + l - m] = (st > 0) ? v * pw[m].imag() : 0.0;
+    }
+  }
+}
+
+// Spherical Bessel and Neumann functions times fnorm, by upward recurrence
+// (the published algorithm of sphere.f90:7226-7248): bes_l = fnorm j_l(z),
+// neu_l = fnorm y_l(z).
+template <class T>
+void bessel_neumann(T fnorm, T z, int lmax, T *bes, T *neu) {
+  neu[0] = -fnorm * std::cos(z) / z;
+  bes[0] = fnorm * std::sin(z) / z;
+  if (lmax < 1) return;
+  neu[1] = neu[0] / z - bes[0];
+  bes[1] = bes[0] / z + neu[0];
+  for (int l = 2; l <= lmax; l++) {
+    neu[l] = T(2 * l - 1) * neu[l - 1] / z - neu[l - 2];
+    bes[l] = T(2 * l - 1) * bes[l - 1] / z - bes[l - 2];
+  }
+}
+
+}  // namespace
+
+struct fdmgpu_cluster {
+  double rsort = 0, adimp = 0;
+  int lmaxso = 0, nlmso = 0;
+  int npoint = 0, npso = 0, nligne = 0;
+  std::vector<GridPoint> pt;   // npso, radial order
+  std::vector<double> rad;     // npso, |index| * adimp
+  std::vector<int> nbr;        // npso * 12, 0-based point or -1
+  std::vector<int> bnd;        // boundary points: inner (nsortf) then outer
+  int nsortf = 0;
+  std::vector<int> slot;       // per point: index in bnd of an outer point, -1 otherwise
+  std::vector<double> ylm;     // bnd.size() * nlmso
+  std::vector<int> lb1, lb2;   // nligne, 1-based
+  std::vector<int64_t> rp;     // compressed-row structure, nligne + 1
+  std::vector<int> cols;       // 1-based
+  double clapl[13] = {0};
+};
+
+namespace {
+
+int build(fdmgpu_cluster *c) {
+  // Grid points inside rsort/adimp + sqrt(2)*iord/2 (reseau, general.f90:9476)
+  // visited in (ix, iy, iz) lexicographic order, then ordered by radius with
+  // the reference's exchange sort (:9528-9543): a later point is swapped in
+  // only if it is closer by more than epspos, so equal radii keep the
+  // arrival order this exchange pattern produces.
+  const double rmax = c->rsort / c->adimp + 1.41421356237309504880168872420970 * 4 / 2 + kEpsPos;
+  const int nx = (int)std::ceil(rmax) + 1;
+  for (int ix = -nx; ix <= nx; ix++)
+    for (int iy = -nx; iy <= nx; iy++)
+      for (int iz = -nx; iz <= nx; iz++) {
+        const double d = std::sqrt((double)ix * ix + (double)iy * iy + (double)iz * iz);
+        if (d > rmax) continue;
+        c->pt.push_back({ix, iy, iz});
+        c->rad.push_back(c->adimp * d);
+      }
+  const int n = (int)c->pt.size();
+  double *r = c->rad.data();
+  GridPoint *p = c->pt.data();
+  for (int a = 0; a < n; a++) {
+    double ra = r[a];
+    for (int b = a + 1; b < n; b++) {
+      if (ra < r[b] + kEpsPos) continue;
+      std::swap(r[a], r[b]);
+      std::swap(p[a], p[b]);
+      ra = r[a];
+    }
+  }
+  c->npso = n;
+  int np = 0;
+  while (np < n && !(r[np] > c->rsort + kEpsPos)) np++;
+  c->npoint = np;
+  // Neighbour table (laplac, general.f90:9839-9871)
+  const int w = 2 * nx + 1;
+  std::vector<int> where((size_t)w * w * w, -1);
+  auto key = [&](int x, int y, int z) { return ((size_t)(x + nx) * w + (y + nx)) * w + (z + nx); };
+  for (int i = 0; i < n; i++) where[key(p[i].ix, p[i].iy, p[i].iz)] = i;
+  c->nbr.assign((size_t)n * 12, -1);
+  for (int i = 0; i < n; i++)
+    for (int v = 0; v < 12; v++) {
+      const int x = p[i].ix + kDir[v][0], y = p[i].iy + kDir[v][1], z = p[i].iz + kDir[v][2];
+      if (std::abs(x) <= nx && std::abs(y) <= nx && std::abs(z) <= nx) c->nbr[(size_t)i * 12 + v] = where[key(x, y, z)];
+    }
+  const double c1 = (4.0 / 3.0) / (c->adimp * c->adimp), c2 = (-1.0 / 16) * c1;  // general.f90:9754-9776
+  double s = 0;
+  for (int v = 1; v <= 12; v++) {
+    c->clapl[v] = (v <= 6) ? c1 : c2;
+    s += c->clapl[v];
+  }
+  c->clapl[0] = -s;
+  // Outer-sphere point lists (bordure, general.f90:9918-10076): interior points
+  // with a first-shell neighbour outside, then outside points touching the
+  // interior; quadrature weight 4 pi / nsortf each (unit point volumes).
+  for (int i = 0; i < np; i++)
+    for (int v = 0; v < 6; v++) {
+      const int j = c->nbr[(size_t)i * 12 + v];
+      if (j >= np) {
+        c->bnd.push_back(i);
+        break;
+      }
+    }
+  c->nsortf = (int)c->bnd.size();
+  c->slot.assign(n, -1);
+  for (int i = np; i < n; i++)
+    for (int v = 0; v < 12; v++) {
+      const int j = c->nbr[(size_t)i * 12 + v];
+      if (j >= 0 && j < np) {
+        c->slot[i] = (int)c->bnd.size();
+        c->bnd.push_back(i);
+        break;
+      }
+    }
+  // Real harmonics at the outer-sphere points (Ylmpt, general.f90:11545-11568)
+  c->nlmso = (c->lmaxso + 1) * (c->lmaxso + 1);
+  c->ylm.assign(c->bnd.size() * c->nlmso, 0.0);
+  for (size_t b = 0; b < c->bnd.size(); b++) {
+    const GridPoint &q = p[c->bnd[b]];
+    real_ylm(c->lmaxso, q.ix * c->adimp, q.iy * c->adimp, q.iz * c->adimp, &c->ylm[b * c->nlmso]);
+  }
+  // Rows and profile (newind, mat.f90:1812-1884)
+  const int nl = np + c->nlmso;
+  c->nligne = nl;
+  c->lb1.resize(nl);
+  c->lb2.resize(nl);
+  for (int ii = 1; ii <= nl; ii++) c->lb1[ii - 1] = c->lb2[ii - 1] = ii;
+  for (int i = 0; i < np; i++) {
+    int &l1 = c->lb1[i], &l2 = c->lb2[i];
+    for (int v = 0; v < 12; v++) {
+      const int j = c->nbr[(size_t)i * 12 + v];
+      if (j < 0) return -1;  // an interior point always has its 12 neighbours on the grid
+      if (j < np) {
+        l1 = std::min(l1, j + 1);
+        l2 = std::max(l2, j + 1);
+      } else {
+        l2 = nl;
+      }
+    }
+  }
+  int lo = np + 1;
+  for (int b = 0; b < c->nsortf; b++) lo = std::min(lo, c->bnd[b] + 1);
+  for (int ii = np + 1; ii <= nl; ii++) {
+    c->lb1[ii - 1] = lo;
+    c->lb2[ii - 1] = nl;
+  }
+  for (int ii = nl - 1; ii >= 1; ii--) c->lb1[ii - 1] = std::min(c->lb1[ii - 1], c->lb1[ii]);
+  // Compressed-row structure: diagonal, interior neighbours, outer columns
+  // (grid rows touching the outside); outer rows: boundary points then the
+  // outer columns.  Columns ascending.
+  c->rp.assign(nl + 1, 0);
+  c->cols.clear();
+  std::vector<int> tmp;
+  for (int i = 0; i < np; i++) {
+    tmp.clear();
+    tmp.push_back(i + 1);
+    bool outside = false;
+    for (int v = 0; v < 12; v++) {
+      const int j = c->nbr[(size_t)i * 12 + v];
+      if (j < np)
+        tmp.push_back(j + 1);
+      else
+        outside = true;
+    }
+    std::sort(tmp.begin(), tmp.end());
+    if (outside)
+      for (int l = 0; l < c->nlmso; l++) tmp.push_back(np + l + 1);
+    c->cols.insert(c->cols.end(), tmp.begin(), tmp.end());
+    c->rp[i + 1] = (int64_t)c->cols.size();
+  }
+  for (int l = 0; l < c->nlmso; l++) {
+    for (int b = 0; b < c->nsortf; b++) c->cols.push_back(c->bnd[b] + 1);
+    for (int k = 0; k < c->nlmso; k++) c->cols.push_back(np + k + 1);
+    c->rp[np + l + 1] = (int64_t)c->cols.size();
+  }
+  return 0;
+}
+
+inline int lof(int lm) {  // l of the harmonic index l*l + l + m
+  int l = 0;
+  while ((l + 1) * (l + 1) <= lm) l++;
+  return l;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fdmgpu_cluster_create(double rsort, double adimp, int lmaxso, fdmgpu_cluster **out) {
+  if (!out || !(rsort > 0) || !(adimp > 0) || lmaxso < 0 || lmaxso > 40) return -1;
+  fdmgpu_cluster *c = new (std::nothrow) fdmgpu_cluster();
+  if (!c) return -3;
+  c->rsort = rsort;
+  c->adimp = adimp;
+  c->lmaxso = lmaxso;
+  if (build(c)) {
+    delete c;
+    return -1;
+  }
+  *out = c;
+  return 0;
+}
+
+void fdmgpu_cluster_destroy(fdmgpu_cluster *c) { delete c; }
+
+int fdmgpu_cluster_sizes(const fdmgpu_cluster *c, int *s) {
+  s[0] = c->npoint;
+  s[1] = c->npso;
+  s[2] = c->nligne;
+  s[3] = c->nlmso;
+  s[4] = c->nsortf;
+  s[5] = (int)c->bnd.size();
+  return 0;
+}
+
+int fdmgpu_cluster_points(const fdmgpu_cluster *c, double *xyz) {
+  for (int i = 0; i < c->npoint; i++) {
+    xyz[3 * i] = c->pt[i].ix * c->adimp;
+    xyz[3 * i + 1] = c->pt[i].iy * c->adimp;
+    xyz[3 * i + 2] = c->pt[i].iz * c->adimp;
+  }
+  return 0;
+}
+
+int fdmgpu_cluster_profile(const fdmgpu_cluster *c, int *lb1, int *lb2) {
+  std::memcpy(lb1, c->lb1.data(), sizeof(int) * c->nligne);
+  std::memcpy(lb2, c->lb2.data(), sizeof(int) * c->nligne);
+  return 0;
+}
+
+int64_t fdmgpu_cluster_nnz(const fdmgpu_cluster *c) { return (int64_t)c->cols.size(); }
+
+int fdmgpu_cluster_assemble(const fdmgpu_cluster *c, const double *V, double Enervide, double Eimag,
+                            double Ecinetic_out, int E_comp, int basereel, int64_t *row_ptr, int *col, double *re,
+                            double *im, double *rhs_re, double *rhs_im) {
+  const bool cal = E_comp || !basereel;
+  const int np = c->npoint, nl = c->nlmso, nb = (int)c->bnd.size(), L = c->lmaxso;
+  // Outer-sphere functions at every boundary point (phiso, non-Rydberg
+  // branch, mat.f90:771-792, :975-994): k = sqrt(Ec + i Eimag), fnorm = sqrt(k/pi).
+  std::vector<cd> B((size_t)nb * (L + 1)), N((size_t)nb * (L + 1));
+  if (E_comp) {
+    cd k = std::sqrt(cd(Ecinetic_out, Eimag));
+    if (std::abs(k) < 1e-10) k = cd(1e-10, 0);
+    const cd fn = std::sqrt(k / kPi);
+    for (int b = 0; b < nb; b++)
+      bessel_neumann<cd>(fn, k * c->rad[c->bnd[b]], L, &B[(size_t)b * (L + 1)], &N[(size_t)b * (L + 1)]);
+  } else {
+    const double k = std::sqrt(Ecinetic_out);
+    const double fn = std::sqrt(k / kPi);
+    std::vector<double> br(L + 1), nr(L + 1);
+    for (int b = 0; b < nb; b++) {
+      bessel_neumann<double>(fn, k * c->rad[c->bnd[b]], L, br.data(), nr.data());
+      for (int l = 0; l <= L; l++) {
+        B[(size_t)b * (L + 1) + l] = br[l];
+        N[(size_t)b * (L + 1) + l] = nr[l];
+      }
+    }
+  }
+  // Outer-column factor of an outside point (calcMatRow, mat.f90:1330-1346):
+  // the Neumann function in the real basis, j + i n in the complex one.
+  auto cfac = [&](int b, int l) -> cd {
+    const cd bb = B[(size_t)b * (L + 1) + l], nn = N[(size_t)b * (L + 1) + l];
+    if (basereel) return nn;
+    return bb + cd(0, 1) * nn;
+  };
+  std::memcpy(row_ptr, c->rp.data(), sizeof(int64_t) * (c->nligne + 1));
+  std::memcpy(col, c->cols.data(), sizeof(int) * c->cols.size());
+  if (rhs_re) std::memset(rhs_re, 0, sizeof(double) * (size_t)c->nligne * nl);
+  if (rhs_im && cal) std::memset(rhs_im, 0, sizeof(double) * (size_t)c->nligne * nl);
+  std::vector<cd> row(nl), rrow(nl);
+  for (int i = 0; i < np; i++) {
+    const int64_t q0 = c->rp[i], q1 = c->rp[i + 1];
+    std::fill(row.begin(), row.end(), cd(0));
+    std::fill(rrow.begin(), rrow.end(), cd(0));
+    const cd diag(V[i] - Enervide - c->clapl[0], E_comp ? -Eimag : 0.0);  // mat.f90:1147-1150
+    // interior entries in column order; the stencil weight of a neighbour is -clapl(v)
+    int64_t q = q0;
+    for (; q < q1 && col[q] <= np; q++) {
+      const int j = col[q] - 1;
+      cd v = 0;
+      if (j == i) {
+        v = diag;
+      } else {
+        for (int d = 0; d < 12; d++)
+          if (c->nbr[(size_t)i * 12 + d] == j) v = -c->clapl[d + 1];
+      }
+      re[q] = v.real();
+      if (cal) im[q] = v.imag();
+    }
+    for (int d = 0; d < 12; d++) {  // outside neighbours couple to the outer unknowns, mat.f90:1312-1398
+      const int j = c->nbr[(size_t)i * 12 + d];
+      if (j < np) continue;
+      const int b = c->slot[j];
+      const double w = -c->clapl[d + 1];
+      for (int lm = 0; lm < nl; lm++) {
+        const int l = lof(lm);
+        const double y = c->ylm[(size_t)b * nl + lm];
+        row[lm] += cfac(b, l) * y * w;
+        rrow[lm] -= y * B[(size_t)b * (L + 1) + l] * w;
+      }
+    }
+    for (int lm = 0; q < q1; q++, lm++) {
+      re[q] = row[lm].real();
+      if (cal) im[q] = row[lm].imag();
+    }
+    if (rhs_re)
+      for (int lm = 0; lm < nl; lm++) {
+        rhs_re[(size_t)i * nl + lm] = rrow[lm].real();
+        if (cal && rhs_im) rhs_im[(size_t)i * nl + lm] = rrow[lm].imag();
+      }
+  }
+  const double pw = 4 * kPi / c->nsortf;
+  for (int lm = 0; lm < nl; lm++) {  // outer rows, mat.f90:1401-1511
+    const int ii = np + lm;
+    int64_t q = c->rp[ii];
+    for (int b = 0; b < c->nsortf; b++, q++) {
+      re[q] = -pw * c->ylm[(size_t)b * nl + lm];
+      if (cal) im[q] = 0;
+    }
+    for (int lp = 0; lp < nl; lp++, q++) {
+      const int l = lof(lp);
+      cd s = 0;
+      for (int b = 0; b < c->nsortf; b++)
+        s += (pw * c->ylm[(size_t)b * nl + lm] * c->ylm[(size_t)b * nl + lp]) * cfac(b, l);
+      re[q] = s.real();
+      if (cal) im[q] = s.imag();
+    }
+    if (rhs_re)
+      for (int lf = 0; lf < nl; lf++) {
+        const int l = lof(lf);
+        cd s = 0;
+        for (int b = 0; b < c->nsortf; b++)
+          s += (pw * c->ylm[(size_t)b * nl + lm] * c->ylm[(size_t)b * nl + lf]) * B[(size_t)b * (L + 1) + l];
+        rhs_re[(size_t)ii * nl + lf] = -s.real();
+        if (cal && rhs_im) rhs_im[(size_t)ii * nl + lf] = -s.imag();
+      }
+  }
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,32 @@
+// FP64 iterative refinement around FP32 factors, and the sigma reduction.
+#pragma once
+#include "solver.h"
+
+namespace fdmgpu {
+
+// Compressed-row matrix on the device (FP64, 1-based columns).
+struct DevCsr {
+  int n = 0;
+  int64_t nnz = 0;
+  const int64_t *row_ptr = nullptr;
+  const int *col = nullptr;
+  const double *vals = nullptr;  // interleaved (re, im) when complex
+};
+
+// R = B - A X over nrhs right-hand sides, all rows*nrhs interleaved when C.
+template <bool C>
+int residual(const DevCsr &a, const double *B, const double *X, double *R, int nrhs, hipStream_t s);
+
+// Per right-hand side squared 2-norms of V (n rows), summed on the host.
+template <bool C>
+int col_norms2(const double *V, int n, int nrhs, double *out, double *dscratch, int scratch_elems,
+               hipStream_t s);
+
+// dst (R) <- src (double), elementwise, count elements.
+template <class R>
+int convert(const double *src, R *dst, size_t count, hipStream_t s);
+// X += D (D in precision R), count elements.
+template <class R>
+int accumulate(double *X, const R *D, size_t count, hipStream_t s);
+
+}  // namespace fdmgpu

@@ -1,0 +1,101 @@
+// Host-side orchestration of the block-skyline UL solver.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+
+namespace fdmgpu {
+
+void set_error(const std::string &msg);
+const char *get_error();
+
+#define FDM_HIP(call)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess) {                                                        \
+      set_error(std::string(#call) + ": " + hipGetErrorString(e_));                \
+      return -2;                                                                   \
+    }                                                                              \
+  } while (0)
+
+// Block layout of one profile matrix (host copy plus device mirror).
+struct Plan {
+  int n = 0, NB = 0;
+  std::vector<int> J0, J1, I0, Jx;
+  std::vector<int64_t> tbase;
+  int64_t ntiles = 0;
+  int maxlen = 0;  // longest tile range of any solve kind
+  // device mirrors
+  int *dJ0 = nullptr, *dJ1 = nullptr, *dI0 = nullptr, *dJx = nullptr;
+  int64_t *dtbase = nullptr;
+
+  Plan() = default;
+  Plan(const Plan &) = delete;
+  Plan &operator=(const Plan &) = delete;
+  ~Plan() { release(); }
+
+  // lb1/lb2 1-based inclusive; returns 0 or -1 with an error message.
+  int build(int n, const int *lb1, const int *lb2);
+  int upload();
+  void release();
+  DevPlan dev() const { return DevPlan{n, NB, dJ0, dJ1, dI0, dJx, dtbase}; }
+  // Algorithmic work on the element profile: complex (or real) multiply-adds
+  // of the elimination, and stored entries of the factors (per RHS, the
+  // multiply-adds of one forward + one backward substitution).
+  static double factor_macs(int n, const int *lb1, const int *lb2);
+  static double solve_macs(int n, const int *lb1, const int *lb2);
+  // multiply-adds the blocked kernels actually perform (tile granularity)
+  double tile_update_macs() const;
+};
+
+// A group of independent systems factorised and solved by grouped launches
+// (one launch per kernel and block step serves every member).  Members are
+// aligned so their last block steps coincide.
+template <class R, bool C>
+struct Group {
+  static constexpr int NC = Elem<R, C>::NC;
+  struct Member {
+    Plan *plan = nullptr;
+    R *tiles = nullptr, *dinv = nullptr, *X = nullptr, *part = nullptr;
+    int *fail = nullptr;
+    int nrhs = 0;
+  };
+  std::vector<Member> m;
+  std::vector<char> active;  // members taking part in factor()/solve()
+  SysDev<R> *d_sys = nullptr;
+  int persist_wg = 1024;     // workgroups of the persistent tile GEMM
+  hipStream_t stream = nullptr;
+  int maxparts = 1;
+  // per-kernel timing (optional): cumulative ms of the G_UPDATE launches
+  bool time_updates = false;
+  double update_ms = 0;
+  int64_t update_launches = 0;
+  std::vector<hipEvent_t> ev;
+
+  Group() = default;
+  Group(const Group &) = delete;
+  Group &operator=(const Group &) = delete;
+  ~Group() { release(); }
+
+  // Allocates a member (tiles, inverses, RHS, partial sums).  X may be given
+  // (device, (NB*TB) x nrhs, interleaved) or allocated when nullptr.
+  int add(Plan *plan, int nrhs, R *X = nullptr);
+  int finalize(hipStream_t s);  // uploads descriptors; call after the last add()
+  void release();
+  size_t factor_bytes() const;
+  int zero(int g);  // zero member g's tiles and pivot flag
+  // vals: interleaved (re, im) when complex
+  int scatter_csr(int g, const int64_t *d_row_ptr, const int *d_col, const double *d_vals);
+  int factor();
+  int solve(int kind);  // in place on every member's X
+  int read_fail(std::vector<int> &rows);
+
+ private:
+  std::vector<char> owns_x;
+};
+
+}  // namespace fdmgpu

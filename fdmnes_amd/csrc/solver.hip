@@ -1,0 +1,319 @@
+// Block-skyline UL factorisation and block triangular solves (host side).
+#include "solver.h"
+
+#include <algorithm>
+#include <cstring>
+
+namespace fdmgpu {
+
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+const char *get_error() { return g_err.c_str(); }
+
+int Plan::build(int n_, const int *lb1, const int *lb2) {
+  if (n_ <= 0) {
+    set_error("nligne must be positive");
+    return -1;
+  }
+  for (int i = 0; i < n_; i++) {
+    const int ii = i + 1;
+    if (lb1[i] < 1 || lb1[i] > ii || lb2[i] < ii || lb2[i] > n_) {
+      set_error("profile bounds out of range at row " + std::to_string(ii));
+      return -1;
+    }
+    if (i > 0 && lb1[i] < lb1[i - 1]) {
+      set_error("lb1 must be non-decreasing (newind's sweep, mat.f90:1882); row " + std::to_string(ii));
+      return -1;
+    }
+  }
+  release();
+  n = n_;
+  NB = (n + TB - 1) / TB;
+  J0.assign(NB, 0);
+  J1.assign(NB, 0);
+  I0.assign(NB, 0);
+  Jx.assign(NB, 0);
+  tbase.assign(NB, 0);
+  for (int I = 0; I < NB; I++) {
+    const int r0 = I * TB, r1 = std::min(n, (I + 1) * TB);
+    int j0 = I, j1 = I;
+    if (r0 < n) j0 = std::min(j0, (lb1[r0] - 1) / TB);
+    for (int r = r0; r < r1; r++) j1 = std::max(j1, (lb2[r] - 1) / TB);
+    J0[I] = j0;
+    J1[I] = j1;
+  }
+  for (int I = 1; I < NB; I++) J1[I] = std::max(J1[I], J1[I - 1]);
+  int64_t acc = 0;
+  for (int I = 0; I < NB; I++) {
+    tbase[I] = acc;
+    acc += J1[I] - J0[I] + 1;
+  }
+  ntiles = acc;
+  int I = 0;
+  for (int K = 0; K < NB; K++) {  // first block row reaching column K (J1 non-decreasing)
+    while (J1[I] < K) I++;
+    I0[K] = I;
+  }
+  int J = NB - 1;
+  for (int K = NB - 1; K >= 0; K--) {  // last block row whose envelope starts at or before K
+    while (J0[J] > K) J--;
+    Jx[K] = J;
+  }
+  maxlen = 0;
+  for (int kind = 0; kind < 4; kind++)
+    for (int K = 0; K < NB; K++) {
+      int jb, je;
+      solve_range(kind, K, J0[K], J1[K], I0[K], Jx[K], &jb, &je);
+      maxlen = std::max(maxlen, je - jb);
+    }
+  return 0;
+}
+
+int Plan::upload() {
+  FDM_HIP(hipMalloc(&dJ0, sizeof(int) * NB));
+  FDM_HIP(hipMalloc(&dJ1, sizeof(int) * NB));
+  FDM_HIP(hipMalloc(&dI0, sizeof(int) * NB));
+  FDM_HIP(hipMalloc(&dJx, sizeof(int) * NB));
+  FDM_HIP(hipMalloc(&dtbase, sizeof(int64_t) * NB));
+  FDM_HIP(hipMemcpy(dJ0, J0.data(), sizeof(int) * NB, hipMemcpyHostToDevice));
+  FDM_HIP(hipMemcpy(dJ1, J1.data(), sizeof(int) * NB, hipMemcpyHostToDevice));
+  FDM_HIP(hipMemcpy(dI0, I0.data(), sizeof(int) * NB, hipMemcpyHostToDevice));
+  FDM_HIP(hipMemcpy(dJx, Jx.data(), sizeof(int) * NB, hipMemcpyHostToDevice));
+  FDM_HIP(hipMemcpy(dtbase, tbase.data(), sizeof(int64_t) * NB, hipMemcpyHostToDevice));
+  return 0;
+}
+
+void Plan::release() {
+  if (dJ0) (void)hipFree(dJ0);
+  if (dJ1) (void)hipFree(dJ1);
+  if (dI0) (void)hipFree(dI0);
+  if (dJx) (void)hipFree(dJx);
+  if (dtbase) (void)hipFree(dtbase);
+  dJ0 = dJ1 = dI0 = dJx = nullptr;
+  dtbase = nullptr;
+}
+
+// Reference elimination work on the element profile: row ii is updated by
+// every row j in (ii, lb2(ii)] over that row's stored part [lb1(j), j).
+double Plan::factor_macs(int n, const int *lb1, const int *lb2) {
+  double s = 0;
+  for (int i = 0; i < n; i++) {
+    const int ii = i + 1;
+    for (int j = ii + 1; j <= lb2[i]; j++) s += (double)(j - lb1[j - 1]);
+  }
+  return s;
+}
+
+double Plan::solve_macs(int n, const int *lb1, const int *lb2) {
+  double s = 0;
+  for (int i = 0; i < n; i++) s += (double)(lb2[i] - (i + 1)) + (double)((i + 1) - lb1[i]);
+  return s;
+}
+
+double Plan::tile_update_macs() const {
+  double s = 0;
+  for (int K = 0; K < NB; K++) s += (double)(K - I0[K]) * (double)(K - J0[K]);
+  return s * (double)TB * TB * TB;
+}
+
+template <class R, bool C>
+int Group<R, C>::add(Plan *plan, int nrhs, R *X) {
+  if ((int)m.size() >= MAXG) {
+    set_error("a group holds at most 16 systems");
+    return -1;
+  }
+  Member mb;
+  mb.plan = plan;
+  mb.nrhs = nrhs;
+  const size_t te = (size_t)TB * TB * NC;
+  FDM_HIP(hipMalloc(&mb.tiles, sizeof(R) * te * (size_t)plan->ntiles));
+  FDM_HIP(hipMalloc(&mb.dinv, sizeof(R) * te * 2 * (size_t)plan->NB));
+  FDM_HIP(hipMalloc(&mb.fail, sizeof(int)));
+  if (X) {
+    mb.X = X;
+    owns_x.push_back(0);
+  } else {
+    FDM_HIP(hipMalloc(&mb.X, sizeof(R) * (size_t)plan->NB * TB * nrhs * NC));
+    owns_x.push_back(1);
+  }
+  m.push_back(mb);
+  active.push_back(1);
+  return 0;
+}
+
+template <class R, bool C>
+int Group<R, C>::finalize(hipStream_t s) {
+  stream = s;
+  maxparts = 1;
+  for (auto &mb : m) maxparts = std::max(maxparts, (mb.plan->maxlen + TPW - 1) / TPW);
+  for (auto &mb : m) {
+    const int nch = (mb.nrhs + RC - 1) / RC;
+    if (mb.part) (void)hipFree(mb.part);
+    FDM_HIP(hipMalloc(&mb.part, sizeof(R) * (size_t)maxparts * nch * TB * RC * NC));
+  }
+  std::vector<SysDev<R>> h(m.size());
+  for (size_t g = 0; g < m.size(); g++)
+    h[g] = SysDev<R>{m[g].plan->dev(), m[g].tiles, m[g].dinv, m[g].fail, m[g].X, m[g].part, m[g].nrhs};
+  if (d_sys) (void)hipFree(d_sys);
+  FDM_HIP(hipMalloc(&d_sys, sizeof(SysDev<R>) * h.size()));
+  FDM_HIP(hipMemcpy(d_sys, h.data(), sizeof(SysDev<R>) * h.size(), hipMemcpyHostToDevice));
+  return 0;
+}
+
+template <class R, bool C>
+void Group<R, C>::release() {
+  for (size_t g = 0; g < m.size(); g++) {
+    auto &mb = m[g];
+    if (mb.tiles) (void)hipFree(mb.tiles);
+    if (mb.dinv) (void)hipFree(mb.dinv);
+    if (mb.part) (void)hipFree(mb.part);
+    if (mb.fail) (void)hipFree(mb.fail);
+    if (mb.X && owns_x[g]) (void)hipFree(mb.X);
+  }
+  m.clear();
+  active.clear();
+  owns_x.clear();
+  if (d_sys) (void)hipFree(d_sys);
+  d_sys = nullptr;
+  for (auto e : ev) (void)hipEventDestroy(e);
+  ev.clear();
+}
+
+template <class R, bool C>
+size_t Group<R, C>::factor_bytes() const {
+  size_t s = 0;
+  for (auto &mb : m) s += sizeof(R) * (size_t)TB * TB * NC * ((size_t)mb.plan->ntiles + 2 * (size_t)mb.plan->NB);
+  return s;
+}
+
+template <class R, bool C>
+int Group<R, C>::zero(int g) {
+  auto &mb = m[g];
+  const size_t cnt = (size_t)TB * TB * NC * (size_t)mb.plan->ntiles;
+  hipLaunchKernelGGL((k_fill<R>), dim3(2048), dim3(256), 0, stream, mb.tiles, cnt, R(0));
+  hipLaunchKernelGGL((k_fill<int>), dim3(1), dim3(64), 0, stream, mb.fail, (size_t)1, 0);
+  FDM_HIP(hipGetLastError());
+  return 0;
+}
+
+template <class R, bool C>
+int Group<R, C>::scatter_csr(int g, const int64_t *d_row_ptr, const int *d_col, const double *d_vals) {
+  auto &mb = m[g];
+  const int rows = mb.plan->NB * TB;
+  hipLaunchKernelGGL((k_scatter_csr<R, C>), dim3((rows + 255) / 256), dim3(256), 0, stream, mb.plan->dev(), mb.tiles,
+                     d_row_ptr, d_col, d_vals);
+  FDM_HIP(hipGetLastError());
+  return 0;
+}
+
+namespace {
+inline void push(Items &it, int g, int K, int nwg) {
+  if (it.count == 0) it.off[0] = 0;
+  it.sys[it.count] = g;
+  it.K[it.count] = K;
+  it.off[it.count + 1] = it.off[it.count] + nwg;
+  it.count++;
+}
+}  // namespace
+
+template <class R, bool C>
+int Group<R, C>::factor() {
+  int maxNB = 0;
+  for (size_t g = 0; g < m.size(); g++)
+    if (active[g]) maxNB = std::max(maxNB, m[g].plan->NB);
+  if (time_updates) {
+    for (auto e : ev) (void)hipEventDestroy(e);
+    ev.assign(2 * (size_t)maxNB, nullptr);
+    for (auto &e : ev) FDM_HIP(hipEventCreate(&e));
+  }
+  int nev = 0;
+  for (int s = 0; s < maxNB; s++) {
+    Items ip{}, itr{}, itc{}, itu{};
+    for (int g = 0; g < (int)m.size(); g++) {
+      if (!active[g]) continue;
+      const Plan &pl = *m[g].plan;
+      const int t = s - (maxNB - pl.NB);
+      if (t < 0) continue;
+      const int K = pl.NB - 1 - t;
+      const int nJ = K - pl.J0[K], nI = K - pl.I0[K];
+      push(ip, g, K, 1);
+      if (nJ > 0) push(itr, g, K, nJ);
+      if (nI > 0) push(itc, g, K, nI);
+      if (nI > 0 && nJ > 0) push(itu, g, K, nI * nJ);
+    }
+    hipLaunchKernelGGL((k_panel<R, C>), dim3(ip.count), dim3(NT), 0, stream, (const SysDev<R> *)d_sys, ip);
+    if (itr.count)
+      hipLaunchKernelGGL((k_tile_gemm<R, C>), dim3(std::min(itr.off[itr.count], persist_wg)), dim3(NT), 0, stream,
+                         (const SysDev<R> *)d_sys, itr, (int)G_ROWTRSM);
+    if (itc.count)
+      hipLaunchKernelGGL((k_tile_gemm<R, C>), dim3(std::min(itc.off[itc.count], persist_wg)), dim3(NT), 0, stream,
+                         (const SysDev<R> *)d_sys, itc, (int)G_COLTRSM);
+    if (itu.count) {
+      if (time_updates) FDM_HIP(hipEventRecord(ev[nev], stream));
+      hipLaunchKernelGGL((k_tile_gemm<R, C>), dim3(std::min(itu.off[itu.count], persist_wg)), dim3(NT), 0, stream,
+                         (const SysDev<R> *)d_sys, itu, (int)G_UPDATE);
+      if (time_updates) FDM_HIP(hipEventRecord(ev[nev + 1], stream));
+      nev += 2;
+    }
+  }
+  FDM_HIP(hipGetLastError());
+  if (time_updates) {
+    FDM_HIP(hipStreamSynchronize(stream));
+    for (int e = 0; e < nev; e += 2) {
+      float ms = 0;
+      FDM_HIP(hipEventElapsedTime(&ms, ev[e], ev[e + 1]));
+      update_ms += ms;
+      update_launches++;
+    }
+  }
+  return 0;
+}
+
+template <class R, bool C>
+int Group<R, C>::solve(int kind) {
+  int maxNB = 0, nch = 1;
+  for (size_t g = 0; g < m.size(); g++) {
+    if (!active[g]) continue;
+    maxNB = std::max(maxNB, m[g].plan->NB);
+    nch = std::max(nch, (m[g].nrhs + RC - 1) / RC);
+  }
+  const bool asc = (kind == S_LOWER || kind == S_UPPER_T);
+  for (int s = 0; s < maxNB; s++) {
+    Items ipart{}, ifin{};
+    for (int g = 0; g < (int)m.size(); g++) {
+      if (!active[g]) continue;
+      const Plan &pl = *m[g].plan;
+      const int t = s - (maxNB - pl.NB);
+      if (t < 0) continue;
+      const int K = asc ? t : pl.NB - 1 - t;
+      int jb, je;
+      solve_range(kind, K, pl.J0[K], pl.J1[K], pl.I0[K], pl.Jx[K], &jb, &je);
+      const int np = (je - jb + TPW - 1) / TPW;
+      if (np > 0) push(ipart, g, K, np);
+      push(ifin, g, K, 1);
+    }
+    if (ipart.count)
+      hipLaunchKernelGGL((k_solve_partial<R, C>), dim3(ipart.off[ipart.count], nch), dim3(NT), 0, stream,
+                         (const SysDev<R> *)d_sys, ipart, kind, maxparts);
+    hipLaunchKernelGGL((k_solve_finish<R, C>), dim3(ifin.count, nch), dim3(NT), 0, stream, (const SysDev<R> *)d_sys,
+                       ifin, kind, maxparts);
+  }
+  FDM_HIP(hipGetLastError());
+  return 0;
+}
+
+template <class R, bool C>
+int Group<R, C>::read_fail(std::vector<int> &rows) {
+  rows.assign(m.size(), 0);
+  for (size_t g = 0; g < m.size(); g++)
+    FDM_HIP(hipMemcpyAsync(&rows[g], m[g].fail, sizeof(int), hipMemcpyDeviceToHost, stream));
+  FDM_HIP(hipStreamSynchronize(stream));
+  return 0;
+}
+
+template struct Group<double, true>;
+template struct Group<double, false>;
+template struct Group<float, true>;
+template struct Group<float, false>;
+
+}  // namespace fdmgpu

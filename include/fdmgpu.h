@@ -1,0 +1,143 @@
+/*
+ * fdmgpu — MI355X (gfx950) solver for the FDMNES finite-difference system.
+ *
+ * C ABI of libfdmgpu.so.  Plain pointers and sizes only.  Every function
+ * returns 0 on success, FDMGPU_PIVOT (1) when a vanishing pivot stops the
+ * factorisation (the reference's STOP, see below), or a negative error code;
+ * fdmgpu_last_error() describes the last failure of the calling thread.
+ *
+ * Conventions shared with the reference (mat_solve_gaussian.f90:5-65):
+ *   - nligne unknowns (rows), nlmso right-hand sides;
+ *   - lb1/lb2: 1-based inclusive column bounds of each row's profile
+ *     (newind, mat.f90:1812-1884); lb1 must be non-decreasing;
+ *   - smr/smi: RHS in, solution out, column-major smr(nlmso, nligne) —
+ *     the RHS index is fastest — split into real and imaginary arrays;
+ *     smi is ignored (may be NULL) when cal_comp == 0 (real arithmetic,
+ *     mat.f90:96-104);
+ *   - the matrix is passed as compressed rows gathered from calcMatRow's
+ *     dense row buffers (mat.f90:1034): row_ptr[nligne+1] 0-based offsets,
+ *     col[] 1-based column numbers, a_re[]/a_im[] values; every column
+ *     must lie inside [lb1(ii), lb2(ii)].
+ * Pivoting policy is the reference's: none.  Rows are eliminated from the
+ * last to the first (a UL factorisation, A = U L with L unit lower); the
+ * solve stops with FDMGPU_PIVOT and *fail_row = ii (1-based) at the first
+ * row, in that order, whose pivot has |a_ii|^2 < 1e-20 (complex,
+ * mat_solve_gaussian.f90:266-273) or |a_ii| < 1e-10 (real, :289-295).
+ * On FDMGPU_PIVOT the RHS arrays are left unchanged.
+ */
+#ifndef FDMGPU_H
+#define FDMGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FDMGPU_OK 0
+#define FDMGPU_PIVOT 1
+#define FDMGPU_EINVAL (-1)
+#define FDMGPU_EHIP (-2)
+#define FDMGPU_ENOMEM (-3)
+#define FDMGPU_ESTATE (-4)
+
+/* Precision of the factorisation. */
+#define FDMGPU_FP64 0  /* complex128 / float64 factors */
+#define FDMGPU_FP32 1  /* complex64 / float32 factors, FP64 refinement */
+
+/* Library set-up on HIP device `device` (no reference counterpart: the
+ * Fortran program has no device).  Idempotent. */
+int fdmgpu_init(int device);
+/* Releases cached device buffers. */
+int fdmgpu_finalize(void);
+const char *fdmgpu_last_error(void);
+/* Name of the solver, 5 characters, as getSolverParams returns it
+ * (mat_solve_gaussian.f90:376-389 returns 'Gauss'); writes "HIP  ". */
+void fdmgpu_solver_name(char out[5]);
+
+/*
+ * Drop-in for the body of mat_solve (mat_solve_gaussian.f90:67-358): the
+ * symbolic pass, the bottom-up elimination with the pivot test and the
+ * forward substitution, in FP64, on the GPU.  Solves A x = b for all nlmso
+ * right-hand sides; b is read from smr/smi and x written back in place.
+ * t_fill / t_tria (may be NULL) are incremented by the seconds spent in
+ * device assembly and in factorisation + substitution, like the reference's
+ * Time_fill / Time_tria accumulators (mat_solve_gaussian.f90:198-315).
+ */
+int fdmgpu_mat_solve(int nligne, int nlmso, int cal_comp, const int *lb1, const int *lb2,
+                     const int64_t *row_ptr, const int *col, const double *a_re, const double *a_im,
+                     double *smr, double *smi, int *fail_row, double *t_fill, double *t_tria);
+
+/*
+ * Same contract as fdmgpu_mat_solve, factorising in FP32 (complex64 when
+ * cal_comp) and recovering FP64 accuracy by iterative refinement: residual
+ * r = b - A x and the update x += d in FP64 with the FP64 matrix, d from the
+ * FP32 factors.  At most max_sweeps refinement sweeps; stops when
+ * max over right-hand sides of ||r||_2/||b||_2 <= tol.  If refinement does
+ * not reach tol (growth without pivoting can ruin FP32 factors), the system
+ * is refactorised in FP64 and *sweeps is returned negative (-(sweeps done)).
+ * No counterpart in the reference (it is FP64 only); same callers as
+ * mat_solve.  *final_rel (may be NULL) receives the last residual ratio.
+ */
+int fdmgpu_mat_solve_mixed(int nligne, int nlmso, int cal_comp, const int *lb1, const int *lb2,
+                           const int64_t *row_ptr, const int *col, const double *a_re, const double *a_im,
+                           double *smr, double *smi, int max_sweeps, double tol, int *sweeps,
+                           double *final_rel, int *fail_row);
+
+/*
+ * Device-resident solver for the energy loop (fdm.f90:3493-4192 calls mat
+ * once per energy point; here many points are solved together).  A solver
+ * holds up to 16 system slots of one precision and arithmetic; each slot is
+ * a structure (profile lb1/lb2 + compressed-row pattern) with its own device
+ * factor storage.  A solve call factorises and solves the listed slots
+ * together — every kernel launch is grouped over them, so points of clusters
+ * of uneven size share launches — and the launch sequence of each slot set
+ * is captured once in a HIP graph and replayed.  Device pointers:
+ *   d_vals[k]: matrix values of point k in the slot's pattern order,
+ *              interleaved (re, im) when complex;
+ *   d_b[k], d_x[k]: (nligne, nrhs) row-major, interleaved when complex
+ *              (the smr/smi layout of mat_solve with re/im interleaved).
+ * No counterpart in the reference (it solves one point at a time on the
+ * CPU); callers: the energy-loop driver and the batched benchmark.
+ */
+typedef struct fdmgpu_solver fdmgpu_solver;
+
+/* precision FDMGPU_FP64 or FDMGPU_FP32; hip_stream may be NULL (library
+ * stream) or a hipStream_t the solver launches on. */
+int fdmgpu_solver_create(int precision, int cal_comp, int nrhs, void *hip_stream, fdmgpu_solver **out);
+void fdmgpu_solver_destroy(fdmgpu_solver *s);
+/* Adds a slot; allocates its factor storage. */
+int fdmgpu_solver_add_system(fdmgpu_solver *s, int nligne, const int *lb1, const int *lb2, const int64_t *row_ptr,
+                             const int *col, int *slot);
+/* Solves count points on distinct slots.  FP32 solvers refine in FP64 (at
+ * most max_sweeps sweeps to max_l ||r_l||/||b_l|| <= tol) and fall back to
+ * FP64 factors per point when refinement does not converge (sweeps[k] < 0).
+ * fail_rows[k] gets the reference's failing row (0 = none); returns
+ * FDMGPU_PIVOT if any point failed.  sweeps/rel may be NULL. */
+int fdmgpu_solver_solve(fdmgpu_solver *s, int count, const int *slots, const double *const *d_vals,
+                        const double *const *d_b, double *const *d_x, int max_sweeps, double tol, int *fail_rows,
+                        int *sweeps, double *rel);
+/* Adjoint of sigma(E) = -Im sum_L s_L^T x_L over the diagonal potential:
+ * with the factors left by the last solve of each slot (FP64 complex
+ * solvers), solves A^T lam = s by the transposed substitutions and writes
+ * d_grad[k][i] = Im sum_L lam_{i,L} x_{i,L} = d sigma / d V_i (nligne
+ * entries).  No counterpart in the reference (it has no derivatives);
+ * callers: potential fitting. */
+int fdmgpu_solver_adjoint(fdmgpu_solver *s, int count, const int *slots, const double *const *d_src,
+                          const double *const *d_x, double *const *d_grad);
+/* sigma[k] = -Im sum_{i,L} src_{i,L} x_{i,L} (host output). */
+int fdmgpu_solver_sigma(fdmgpu_solver *s, int count, const int *slots, const double *const *d_src,
+                        const double *const *d_x, double *sigma);
+/* Tuning: HIP-graph capture on/off, event timing of the trailing-update
+ * launches on/off (disables graphs), workgroups of the persistent GEMM. */
+int fdmgpu_solver_options(fdmgpu_solver *s, int use_graphs, int time_updates, int persist_wg);
+/* out[0] ms spent in trailing-update launches (when timed), out[1] their
+ * count, out[2] multiply-adds of one factorisation of every slot at tile
+ * granularity, out[3] the same on the element profile, out[4] factor bytes
+ * of all slots, out[5] profile entries (solve multiply-adds per RHS). */
+int fdmgpu_solver_stats(const fdmgpu_solver *s, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,53 @@
+/*
+ * fdmgpu_cluster — builder of the synthetic FDM systems the solver is
+ * measured on: a cubic grid inside a sphere, ordered by radius, with the
+ * 4th-order Laplacian stencil and the outer-sphere expansion rows, i.e. the
+ * inputs the reference's mat_solve receives for a cluster without atoms in
+ * point group P1.  It replaces the reference's set-up chain
+ *   reseau   general.f90:9439   (grid and radial order)
+ *   laplac   general.f90:9732   (stencil, neighbour table)
+ *   bordure  general.f90:9895   (outer-sphere point lists and weights)
+ *   Ylmpt    general.f90:11505  (real harmonics at the outer points)
+ *   phiso    mat.f90:739        (Bessel/Neumann functions, non-Rydberg)
+ *   newind   mat.f90:1761       (row numbering, profile lb1/lb2)
+ *   calcMatRow mat.f90:1034     (rows of the matrix and incoming-wave RHS)
+ * All lengths in bohr, energies in Rydberg.  Rows are 1..nligne: grid point
+ * i (1-based, radial order) is row i, the nlmso outer-sphere unknowns are the
+ * last rows.  Arithmetic is complex (cal_comp) when E_comp or !basereel
+ * (mat.f90:96-104).
+ */
+#ifndef FDMGPU_CLUSTER_H
+#define FDMGPU_CLUSTER_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fdmgpu_cluster fdmgpu_cluster;
+
+/* Sphere radius rsort and grid step adimp (bohr), outer expansion lmaxso
+ * (nlmso = (lmaxso+1)^2 outer rows). */
+int fdmgpu_cluster_create(double rsort, double adimp, int lmaxso, fdmgpu_cluster **out);
+void fdmgpu_cluster_destroy(fdmgpu_cluster *c);
+/* sizes[0..5] = npoint, npso, nligne, nlmso, nsortf, nsort */
+int fdmgpu_cluster_sizes(const fdmgpu_cluster *c, int *sizes);
+/* (x, y, z) of the npoint interior points in row order, bohr; xyz[3*npoint] */
+int fdmgpu_cluster_points(const fdmgpu_cluster *c, double *xyz);
+/* lb1/lb2[nligne], 1-based inclusive */
+int fdmgpu_cluster_profile(const fdmgpu_cluster *c, int *lb1, int *lb2);
+/* number of stored entries of the compressed rows */
+int64_t fdmgpu_cluster_nnz(const fdmgpu_cluster *c);
+/* One energy point: compressed rows (row_ptr[nligne+1] 0-based offsets,
+ * col 1-based, values re/im) and the incoming-wave RHS calcMatRow adds
+ * (rhs_re/rhs_im (nligne, nlmso) row-major; may be NULL).  V[npoint] in Ry.
+ * im/rhs_im are only written when the arithmetic is complex. */
+int fdmgpu_cluster_assemble(const fdmgpu_cluster *c, const double *V, double Enervide, double Eimag,
+                            double Ecinetic_out, int E_comp, int basereel, int64_t *row_ptr, int *col,
+                            double *re, double *im, double *rhs_re, double *rhs_im);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

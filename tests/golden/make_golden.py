@@ -1,0 +1,73 @@
+"""Writes the golden vectors of tests/golden/ from the reference itself.
+
+Runs the reference FDMNES routines compiled from their own Fortran sources
+(oracle/_ref, built by ``make -f oracle/Makefile.ref``) on small synthetic
+clusters and stores inputs and outputs as .npz fixtures:
+  newind (mat.f90:1761)  -> lb1, lb2, newinv, new
+  phiso (mat.f90:739)    -> Bessel / Neuman at the outer-sphere points
+  mat_solve (mat_solve_gaussian.f90:5, with the reference calcMatRow)
+                          -> solution smr/smi, for the three branch sets
+                             (complex energy; real; complex arithmetic at real E)
+  cbessneu / cbessneur (sphere.f90:7226/7356), cylm + ylmcr (general.f90:11606/11692)
+The cluster arrays fed to the reference (grid, neighbours, boundary lists,
+Ylmso) come from the oracle's restatement; the fixtures store the profile
+and matrix-relevant inputs so the tests can rebuild each case.
+
+    python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+from oracle import oracle as O  # noqa: E402
+from oracle import ref as R  # noqa: E402
+
+BOHR = 0.52917721067
+
+CASES = [
+    # name, R (A), h (A), lmaxso, E (Ry), Eimag (Ry), E_comp, cal_comp, basereel, seed
+    ("c128_R1.2_h0.25", 1.2, 0.25, 2, 0.7, 0.05, True, True, False, 0),
+    ("real_R1.2_h0.25", 1.2, 0.25, 2, 0.7, 0.0, False, False, True, 1),
+    ("cal_R1.2_h0.25", 1.2, 0.25, 2, 0.7, 0.0, False, True, False, 2),
+    ("c128_R1.6_h0.3_l3", 1.6, 0.3, 3, 1.3, 0.0367, True, True, False, 3),
+]
+
+
+def main():
+    if not R.available():
+        raise SystemExit("oracle/_ref/libfdmnes_ref.so is missing: make -f oracle/Makefile.ref")
+    for name, Ra, h, lm, E, Ei, E_comp, cal, basereel, seed in CASES:
+        cl = O.Cluster(Ra / BOHR, h / BOHR, lm)
+        rng = np.random.default_rng(seed)
+        V = rng.uniform(-0.5, 0.5, cl.npoint) - 1.0
+        lb1, lb2, newinv, newp = R.newind(cl)
+        bes, neu = R.phiso(cl, E_comp, E, Ei, E)
+        x = R.mat_solve(cl, V, E, Ei, bes, neu, E_comp, cal, basereel)
+        np.savez_compressed(
+            os.path.join(HERE, f"{name}.npz"), R=Ra, h=h, lmaxso=lm, E=E, Eimag=Ei, E_comp=E_comp, cal_comp=cal,
+            basereel=basereel, V=V, lb1=lb1, lb2=lb2, newinv=newinv, new=newp, bessel=bes, neuman=neu, x=x)
+        print(name, cl.nligne, np.abs(x).max())
+    # special functions
+    zs = [0.3 + 0.05j, 1.7 + 0.2j, 4.2 + 0.01j, 9.5 + 1.0j]
+    fn = [np.sqrt((z / 2.0) / np.pi) for z in zs]
+    bc = [R.cbessneu(f, z, 6) for f, z in zip(fn, zs)]
+    zr = [0.4, 1.9, 5.5, 12.0]
+    br = [R.cbessneur(np.sqrt(z / 3.0 / np.pi), z, 6) for z in zr]
+    dirs = np.array([[1.0, 2.0, 3.0], [0.0, 0.0, 2.0], [0.0, 0.0, -1.5], [-1.0, 0.5, 0.0], [0.3, -0.7, -0.2]])
+    ylm = [R.ylm_real(4, v, float(np.linalg.norm(v))) for v in dirs]
+    np.savez_compressed(os.path.join(HERE, "special_functions.npz"), zs=np.array(zs), fn=np.array(fn),
+                        bes_c=np.array([b for b, _ in bc]), neu_c=np.array([n for _, n in bc]), zr=np.array(zr),
+                        fnr=np.array([np.sqrt(z / 3.0 / np.pi) for z in zr]),
+                        bes_r=np.array([b for b, _ in br]), neu_r=np.array([n for _, n in br]), dirs=dirs,
+                        ylm=np.array(ylm))
+    print("special functions written")
+
+
+if __name__ == "__main__":
+    main()
