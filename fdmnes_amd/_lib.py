@@ -11,6 +11,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfdmgpu.so")
+HOST_LIB_PATH = os.path.join(HERE, "libfdmgpu_host.so")
 
 _lib = None
 
@@ -48,9 +49,17 @@ SIGNATURES = {
 
 
 def load(path: str = LIB_PATH):
-    """Loads the library without touching the GPU (symbol check only)."""
+    """Loads the library without touching the GPU (symbol check only).
+
+    torch ships its own HIP runtime with the same soname as the system one;
+    it is imported first so that one runtime serves both (two runtimes in a
+    process cannot share the device)."""
     global _lib
     if _lib is None:
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(path):
             raise FdmGpuError(f"{path} is missing: build it with `python -m fdmnes_amd.build`")
         L = C.CDLL(path)
@@ -73,6 +82,19 @@ def lib():
         check(L.fdmgpu_init(int(os.environ.get("FDMGPU_DEVICE", "0"))))
         _inited = True
     return L
+
+
+_host = None
+
+
+def load_host():
+    """The host-only library (synthetic cluster builder); no HIP inside."""
+    global _host
+    if _host is None:
+        if not os.path.exists(HOST_LIB_PATH):
+            raise FdmGpuError(f"{HOST_LIB_PATH} is missing: build it with `python -m fdmnes_amd.build`")
+        _host = C.CDLL(HOST_LIB_PATH)
+    return _host
 
 
 def check(rc: int, fail_row: int = 0):

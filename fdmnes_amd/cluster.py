@@ -14,7 +14,7 @@ RYDB = 13.605693012905  # eV per Rydberg from main.f90:64's CODATA-2014 constant
 
 
 def _L():
-    L = _lib.load()
+    L = _lib.load_host()
     if not hasattr(L, "_cluster_sigs"):
         L.fdmgpu_cluster_create.restype = C.c_int
         L.fdmgpu_cluster_create.argtypes = [C.c_double, C.c_double, C.c_int, C.POINTER(C.c_void_p)]

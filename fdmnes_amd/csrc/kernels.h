@@ -70,17 +70,22 @@ __device__ __forceinline__ R *tile_ptr(const DevPlan &p, R *tiles, int I, int J)
 
 // ---------------------------------------------------------------- MFMA
 template <class R> struct Mfma;
+// Operands: lane l holds A[l%16][l/16] and B[l/16][l%16].  Output v of lane
+// l is D[row(l/16, v)][l%16]; the row map differs between the FP64 and the
+// FP32 instruction (checked on the device by tests/test_gpu_mfma_layout.py).
 template <> struct Mfma<double> {
   typedef double v4 __attribute__((ext_vector_type(4)));
   static __device__ __forceinline__ v4 run(double a, double b, v4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int lk, int v) { return lk + 4 * v; }
 };
 template <> struct Mfma<float> {
   typedef float v4 __attribute__((ext_vector_type(4)));
   static __device__ __forceinline__ v4 run(float a, float b, v4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int lk, int v) { return 4 * lk + v; }
 };
 
 enum GemmMode { G_UPDATE = 0, G_ROWTRSM = 1, G_COLTRSM = 2 };
@@ -189,14 +194,13 @@ __global__ __launch_bounds__(NT) void k_tile_gemm(const SysDev<R> *__restrict__ 
         }
       }
   }
-  // Output layout of a 16x16 MFMA block: lane holds rows 4*(lane/16)+v, column lane%16.
 #pragma unroll
   for (int a = 0; a < 2; a++)
 #pragma unroll
     for (int bb = 0; bb < 2; bb++)
 #pragma unroll
       for (int v = 0; v < 4; v++) {
-        const int r = r0 + a * 16 + lk * 4 + v, c = c0 + bb * 16 + li;
+        const int r = r0 + a * 16 + Mfma<R>::row(lk, v), c = c0 + bb * 16 + li;
         const int64_t e = (int64_t)(r * TB + c) * NC;
         if (mode == G_UPDATE) {
           Cg[e] -= accr[a][bb][v];
@@ -206,6 +210,154 @@ __global__ __launch_bounds__(NT) void k_tile_gemm(const SysDev<R> *__restrict__ 
           if constexpr (C) Cg[e + 1] = acci[a][bb][v];
         }
       }
+  }
+}
+
+// ---------------------------------------------------------------- trailing update
+// Work items of the trailing-update kernel: per item an output rectangle of
+// tiles [i0, i0 + n/nj) x [j0, j0 + nj) and a range of block steps
+// [Kb, Kb + nk).  Tile (I,J) accumulates U(I,K) L(K,J) over the steps K of the
+// range whose envelope reaches it (I0(K) <= I and J0(K) <= J; I0, J0 are
+// non-decreasing in K so these steps are a prefix of the range).
+constexpr int MAXT = 2 * MAXG;
+struct TrailItems {
+  int count;
+  int sys[MAXT];
+  int Kb[MAXT];
+  int nk[MAXT];
+  int i0[MAXT];
+  int j0[MAXT];
+  int nj[MAXT];
+  int off[MAXT + 1];
+};
+
+__device__ __forceinline__ int find_trail(const TrailItems &it, int b) {
+  int g = 0;
+  while (g + 1 < it.count && b >= it.off[g + 1]) g++;
+  return g;
+}
+
+constexpr int KC = 16;  // k-chunk of the LDS pipeline
+
+// tile(I,J) -= sum_K tile(I,K) tile(K,J).  Persistent grid over the work
+// items of all systems; per tile the K dimension (nk*TB) streams through a
+// two-buffer LDS pipeline in chunks of KC, the next chunk's global loads in
+// flight while the current chunk feeds the MFMAs.  Four waves, each a 32x32
+// quadrant in 2x2 blocks of 16x16x4 MFMAs; a complex product is four real
+// MFMAs.  LDS holds A chunks k-major (As[k][row]) so the MFMA A operand reads
+// 16 consecutive rows per k.
+template <class R, bool C>
+__global__ __launch_bounds__(NT, 2) void k_trail(const SysDev<R> *__restrict__ sys, TrailItems it) {
+  constexpr int NC = Elem<R, C>::NC;
+  constexpr int LD = TB + 1;
+  constexpr int PER = TB * KC / NT;  // elements per thread per chunk and operand (4)
+  __shared__ R As[2][NC][KC][LD];
+  __shared__ R Bs[2][NC][KC][LD];
+  typedef typename Mfma<R>::v4 v4;
+  typedef R r2 __attribute__((ext_vector_type(2)));
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int r0 = (wave >> 1) * 32, c0 = (wave & 1) * 32;
+  const int li = lane & 15, lk = lane >> 4;
+  const int total = it.off[it.count];
+  for (int wb = blockIdx.x; wb < total; wb += gridDim.x) {
+    const int g = find_trail(it, wb);
+    const int b = wb - it.off[g];
+    const SysDev<R> &S = sys[it.sys[g]];
+    const DevPlan &p = S.p;
+    const int I = it.i0[g] + b / it.nj[g], J = it.j0[g] + b % it.nj[g];
+    int m = 0;
+    while (m < it.nk[g] && p.I0[it.Kb[g] + m] <= I && p.J0[it.Kb[g] + m] <= J) m++;
+    const int nch = m * (TB / KC);
+    v4 accr[2][2], acci[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int bb = 0; bb < 2; bb++) {
+        accr[a][bb] = v4{0, 0, 0, 0};
+        acci[a][bb] = v4{0, 0, 0, 0};
+      }
+    // chunk c: step K = Kb + c/4, columns kc..kc+15 of A = tile(I,K), rows of B = tile(K,J)
+    r2 pa[PER], pb[PER];
+    R qa[PER], qb[PER];
+    auto fetch = [&](int c) {
+      const int K = it.Kb[g] + c / (TB / KC), kc = (c % (TB / KC)) * KC;
+      const R *A = tile_ptr<R, C>(p, S.tiles, I, K);
+      const R *B = tile_ptr<R, C>(p, S.tiles, K, J);
+#pragma unroll
+      for (int q = 0; q < PER; q++) {
+        const int e = tid + q * NT;
+        const int ar = e / KC, ak = e % KC;  // A: 64 rows x 16 k
+        const int bk = e / TB, bc = e % TB;  // B: 16 k x 64 cols
+        if constexpr (C) {
+          pa[q] = reinterpret_cast<const r2 *>(A)[ar * TB + kc + ak];
+          pb[q] = reinterpret_cast<const r2 *>(B)[(kc + bk) * TB + bc];
+        } else {
+          qa[q] = A[ar * TB + kc + ak];
+          qb[q] = B[(kc + bk) * TB + bc];
+        }
+      }
+    };
+    if (wb != (int)blockIdx.x) __syncthreads();  // previous item done with LDS
+    if (nch > 0) fetch(0);
+    for (int c = 0; c < nch; c++) {
+      const int buf = c & 1;
+#pragma unroll
+      for (int q = 0; q < PER; q++) {
+        const int e = tid + q * NT;
+        const int ar = e / KC, ak = e % KC, bk = e / TB, bc = e % TB;
+        if constexpr (C) {
+          As[buf][0][ak][ar] = pa[q].x;
+          As[buf][NC - 1][ak][ar] = pa[q].y;
+          Bs[buf][0][bk][bc] = pb[q].x;
+          Bs[buf][NC - 1][bk][bc] = pb[q].y;
+        } else {
+          As[buf][0][ak][ar] = qa[q];
+          Bs[buf][0][bk][bc] = qb[q];
+        }
+      }
+      __syncthreads();
+      if (c + 1 < nch) fetch(c + 1);
+#pragma unroll
+      for (int k4 = 0; k4 < KC; k4 += 4) {
+        R ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+        for (int a = 0; a < 2; a++) {
+          ar[a] = As[buf][0][k4 + lk][r0 + a * 16 + li];
+          if constexpr (C) ai[a] = As[buf][NC - 1][k4 + lk][r0 + a * 16 + li];
+        }
+#pragma unroll
+        for (int bb = 0; bb < 2; bb++) {
+          br[bb] = Bs[buf][0][k4 + lk][c0 + bb * 16 + li];
+          if constexpr (C) bi[bb] = Bs[buf][NC - 1][k4 + lk][c0 + bb * 16 + li];
+        }
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+          for (int bb = 0; bb < 2; bb++) {
+            accr[a][bb] = Mfma<R>::run(ar[a], br[bb], accr[a][bb]);
+            if constexpr (C) {
+              accr[a][bb] = Mfma<R>::run(-ai[a], bi[bb], accr[a][bb]);
+              acci[a][bb] = Mfma<R>::run(ar[a], bi[bb], acci[a][bb]);
+              acci[a][bb] = Mfma<R>::run(ai[a], br[bb], acci[a][bb]);
+            }
+          }
+      }
+    }
+    if (nch > 0) {
+      R *Cg = tile_ptr<R, C>(p, S.tiles, I, J);
+#pragma unroll
+      for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int bb = 0; bb < 2; bb++)
+#pragma unroll
+          for (int v = 0; v < 4; v++) {
+            const int r = r0 + a * 16 + Mfma<R>::row(lk, v), cc = c0 + bb * 16 + li;
+            const int64_t e = (int64_t)(r * TB + cc) * NC;
+            Cg[e] -= accr[a][bb][v];
+            if constexpr (C) Cg[e + 1] -= acci[a][bb][v];
+          }
+    }
   }
 }
 

@@ -67,7 +67,8 @@ struct Group {
   std::vector<Member> m;
   std::vector<char> active;  // members taking part in factor()/solve()
   SysDev<R> *d_sys = nullptr;
-  int persist_wg = 1024;     // workgroups of the persistent tile GEMM
+  int persist_wg = 1024;     // workgroups of the persistent tile GEMMs
+  int gk = 4;                // block steps per trailing update (0: one-step tile GEMM)
   hipStream_t stream = nullptr;
   int maxparts = 1;
   // per-kernel timing (optional): cumulative ms of the G_UPDATE launches

@@ -216,6 +216,28 @@ inline void push(Items &it, int g, int K, int nwg) {
 }
 }  // namespace
 
+namespace {
+inline void push_trail(TrailItems &it, int g, int Kb, int nk, int i0, int ni, int j0, int nj) {
+  if (ni <= 0 || nj <= 0 || nk <= 0) return;
+  if (it.count == 0) it.off[0] = 0;
+  const int c = it.count;
+  it.sys[c] = g;
+  it.Kb[c] = Kb;
+  it.nk[c] = nk;
+  it.i0[c] = i0;
+  it.j0[c] = j0;
+  it.nj[c] = nj;
+  it.off[c + 1] = it.off[c] + ni * nj;
+  it.count++;
+}
+}  // namespace
+
+// Right-looking blocked UL from the last block row up.  With gk > 0 the
+// block steps are taken in outer groups of gk: each inner step K factorises
+// the diagonal tile, forms its L row and U column, and updates only the
+// tiles of the group's own block rows and columns ("cross"); the rest of the
+// trailing matrix receives the whole group's rank-(gk*TB) update in one pass
+// at the group's last step.
 template <class R, bool C>
 int Group<R, C>::factor() {
   int maxNB = 0;
@@ -223,12 +245,22 @@ int Group<R, C>::factor() {
     if (active[g]) maxNB = std::max(maxNB, m[g].plan->NB);
   if (time_updates) {
     for (auto e : ev) (void)hipEventDestroy(e);
-    ev.assign(2 * (size_t)maxNB, nullptr);
+    ev.assign(4 * (size_t)maxNB, nullptr);
     for (auto &e : ev) FDM_HIP(hipEventCreate(&e));
   }
   int nev = 0;
+  auto timed = [&](auto launch) -> int {
+    if (time_updates) FDM_HIP(hipEventRecord(ev[nev], stream));
+    launch();
+    if (time_updates) {
+      FDM_HIP(hipEventRecord(ev[nev + 1], stream));
+      nev += 2;
+    }
+    return 0;
+  };
   for (int s = 0; s < maxNB; s++) {
     Items ip{}, itr{}, itc{}, itu{};
+    TrailItems cross{}, trail{};
     for (int g = 0; g < (int)m.size(); g++) {
       if (!active[g]) continue;
       const Plan &pl = *m[g].plan;
@@ -239,7 +271,19 @@ int Group<R, C>::factor() {
       push(ip, g, K, 1);
       if (nJ > 0) push(itr, g, K, nJ);
       if (nI > 0) push(itc, g, K, nI);
-      if (nI > 0 && nJ > 0) push(itu, g, K, nI * nJ);
+      if (gk <= 0) {
+        if (nI > 0 && nJ > 0) push(itu, g, K, nI * nJ);
+        continue;
+      }
+      const int Khi = pl.NB - 1 - gk * (t / gk), Klo = std::max(Khi - gk + 1, 0);
+      // rows of the group below K, every column of K's reach
+      const int ia = std::max(pl.I0[K], Klo);
+      push_trail(cross, g, K, 1, ia, K - ia, pl.J0[K], nJ);
+      // rows above the group, the group's columns below K
+      const int ja = std::max(pl.J0[K], Klo);
+      push_trail(cross, g, K, 1, pl.I0[K], std::min(Klo, K) - pl.I0[K], ja, K - ja);
+      if (K == Klo)
+        push_trail(trail, g, Klo, Khi - Klo + 1, pl.I0[Klo], Klo - pl.I0[Klo], pl.J0[Klo], Klo - pl.J0[Klo]);
     }
     hipLaunchKernelGGL((k_panel<R, C>), dim3(ip.count), dim3(NT), 0, stream, (const SysDev<R> *)d_sys, ip);
     if (itr.count)
@@ -248,13 +292,21 @@ int Group<R, C>::factor() {
     if (itc.count)
       hipLaunchKernelGGL((k_tile_gemm<R, C>), dim3(std::min(itc.off[itc.count], persist_wg)), dim3(NT), 0, stream,
                          (const SysDev<R> *)d_sys, itc, (int)G_COLTRSM);
-    if (itu.count) {
-      if (time_updates) FDM_HIP(hipEventRecord(ev[nev], stream));
-      hipLaunchKernelGGL((k_tile_gemm<R, C>), dim3(std::min(itu.off[itu.count], persist_wg)), dim3(NT), 0, stream,
-                         (const SysDev<R> *)d_sys, itu, (int)G_UPDATE);
-      if (time_updates) FDM_HIP(hipEventRecord(ev[nev + 1], stream));
-      nev += 2;
-    }
+    if (itu.count)
+      timed([&]() {
+        hipLaunchKernelGGL((k_tile_gemm<R, C>), dim3(std::min(itu.off[itu.count], persist_wg)), dim3(NT), 0, stream,
+                           (const SysDev<R> *)d_sys, itu, (int)G_UPDATE);
+      });
+    if (cross.count)
+      timed([&]() {
+        hipLaunchKernelGGL((k_trail<R, C>), dim3(std::min(cross.off[cross.count], persist_wg)), dim3(NT), 0, stream,
+                           (const SysDev<R> *)d_sys, cross);
+      });
+    if (trail.count)
+      timed([&]() {
+        hipLaunchKernelGGL((k_trail<R, C>), dim3(std::min(trail.off[trail.count], persist_wg)), dim3(NT), 0, stream,
+                           (const SysDev<R> *)d_sys, trail);
+      });
   }
   FDM_HIP(hipGetLastError());
   if (time_updates) {

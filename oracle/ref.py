@@ -36,7 +36,9 @@ def lib():
     if _lib is None:
         # fdmx.f90 calls ZLOG, a non-standard intrinsic flang leaves external;
         # lazy binding leaves that out-of-path symbol unresolved and unused.
-        _lib = C.CDLL(REF_LIB, mode=os.RTLD_LAZY | os.RTLD_LOCAL)
+        # Global scope: a drop-in mat_solve loaded later resolves the
+        # reference's calcMatRow and write_error from here.
+        _lib = C.CDLL(REF_LIB, mode=os.RTLD_LAZY | os.RTLD_GLOBAL)
     return _lib
 
 
@@ -143,10 +145,11 @@ def phiso(cl, E_comp: bool, Ecinetic_out: float, Eimag: float, Enervide: float =
     return (bes, neu) if E_comp else (besr, neur)
 
 
-def mat_solve(cl, V, Enervide, Eimag, bes, neu, E_comp, cal_comp, basereel, rhs=None):
+def mat_solve(cl, V, Enervide, Eimag, bes, neu, E_comp, cal_comp, basereel, rhs=None, fn=None):
     """Reference mat_solve (93 arguments, mat_solve_gaussian.f90:5-12) with
     the reference calcMatRow assembling each row.  Returns the solution
-    (nligne, nlmso), real or complex."""
+    (nligne, nlmso), real or complex.  ``fn`` calls another procedure with
+    the same argument surface instead (a drop-in replacement)."""
     A = _Args()
     n, nl, ns, lm = cl.nligne, cl.nlmso, cl.nsort, cl.lmaxso
     lb1, lb2, newinv, newp = cl.profile()
@@ -196,7 +199,7 @@ def mat_solve(cl, V, Enervide, Eimag, bes, neu, E_comp, cal_comp, basereel, rhs=
         A.arr(np.zeros(1), np.float64), A.arr(cl.ylmso, np.float64),
     ]
     assert len(args) == 93
-    lib().mat_solve_(*args)
+    (fn or lib().mat_solve_)(*args)
     x = smr.T.copy()
     if cal_comp:
         x = x + 1j * smi.T
