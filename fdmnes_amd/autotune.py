@@ -1,0 +1,71 @@
+"""Autotuner of the tile schedule: picks the trailing-update depth gk
+(effective panel width 64*gk) and the persistent grid sizes for a system
+structure by timing candidate schedules on one group of energy points, and
+caches the choice in a JSON file keyed by structure, precision and group
+size."""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import time
+
+import numpy as np
+
+CACHE = os.environ.get("FDMGPU_TUNE_CACHE", os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                                                         "tune_cache.json"))
+CANDIDATES = [
+    {"gk": 2, "trail_wg": 512, "solve_wg": 512},
+    {"gk": 4, "trail_wg": 512, "solve_wg": 512},
+    {"gk": 4, "trail_wg": 1024, "solve_wg": 512},
+    {"gk": 8, "trail_wg": 1024, "solve_wg": 512},
+]
+
+
+def key(lb1, lb2, precision: int, group: int) -> str:
+    h = hashlib.sha1(np.ascontiguousarray(lb1, np.int32).tobytes() + np.ascontiguousarray(lb2, np.int32).tobytes())
+    return f"{h.hexdigest()[:16]}:p{precision}:g{group}"
+
+
+def load() -> dict:
+    try:
+        with open(CACHE) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def tune(solver, slots, vals, b, x, lb1, lb2, precision: int, candidates=CANDIDATES, repeats: int = 1,
+         log=None) -> dict:
+    """Times every candidate schedule on this group (after one warm-up solve
+    each, which builds its graphs) and applies and caches the fastest."""
+    import torch
+    k = key(lb1, lb2, precision, len(slots))
+    cache = load()
+    if k in cache:
+        best = cache[k]["best"]
+        solver.tune(best["gk"], best["trail_wg"], 1, best["solve_wg"])
+        return cache[k]
+    results = []
+    for c in candidates:
+        solver.tune(c["gk"], c["trail_wg"], 1, c["solve_wg"])
+        solver.solve(slots, vals, b, x)  # warm-up: builds this schedule's graphs
+        torch.cuda.synchronize()
+        t0 = time.time()
+        for _ in range(repeats):
+            solver.solve(slots, vals, b, x)
+        torch.cuda.synchronize()
+        dt = (time.time() - t0) / repeats
+        results.append(dict(c, seconds=dt))
+        if log:
+            log(f"autotune {c}: {dt:.3f}s")
+    best = min(results, key=lambda r: r["seconds"])
+    solver.tune(best["gk"], best["trail_wg"], 1, best["solve_wg"])
+    entry = {"best": {k2: best[k2] for k2 in ("gk", "trail_wg", "solve_wg")}, "trials": results}
+    cache[k] = entry
+    try:
+        with open(CACHE, "w") as f:
+            json.dump(cache, f, indent=1)
+    except OSError:
+        pass
+    return entry

@@ -295,6 +295,25 @@ int fdmgpu_solver_options(fdmgpu_solver *s, int use_graphs, int time_updates, in
   return 0;
 }
 
+int fdmgpu_solver_tune(fdmgpu_solver *s, int gk, int trail_wg, int persist_solve, int solve_wg) {
+  if (!s || gk > 16) return FDMGPU_EINVAL;
+  auto apply = [&](auto *p) {
+    if (gk >= 0) p->grp.gk = gk;
+    if (trail_wg > 0) p->grp.persist_wg = trail_wg;
+    if (persist_solve >= 0) p->grp.persist_solve = persist_solve != 0;
+    if (solve_wg > 0) p->grp.solve_wg = solve_wg;
+    p->drop_graphs();
+  };
+  if (s->prec == FDMGPU_FP64) {
+    if (s->cal) apply((Solver<double, true> *)s->impl);
+    else apply((Solver<double, false> *)s->impl);
+  } else {
+    if (s->cal) apply((Solver<float, true> *)s->impl);
+    else apply((Solver<float, false> *)s->impl);
+  }
+  return 0;
+}
+
 int fdmgpu_solver_stats(const fdmgpu_solver *s, double *out) {
   if (!s || !out) return FDMGPU_EINVAL;
   auto fill = [&](auto *p) {

@@ -37,6 +37,8 @@ struct Solver {
   Group<R, C> grp;
   bool finalized = false;
   std::map<uint32_t, hipGraphExec_t> graphs;  // factorisation graphs by active-slot mask
+  std::map<uint32_t, hipGraphExec_t> sweeps_g;  // refinement-sweep graphs by active-slot mask
+  double *hnorm = nullptr;                       // pinned host copy of the norm partials
   bool use_graphs = true;
   // refinement / fallback statistics of the last solve
   std::vector<int> last_sweeps;
@@ -52,9 +54,11 @@ struct Solver {
             int max_sweeps, double tol, int *fail_rows, int *sweeps, double *rel);
   int adjoint(int count, const int *ids, const double *const *src, const double *const *x, double *const *grad);
   int sigma(int count, const int *ids, const double *const *src, const double *const *x, double *out);
+  void drop_graphs();  // after a schedule change
 
  private:
   int factor_phase(uint32_t mask);
+  int sweep_phase(uint32_t mask);
   int fallback_fp64(int slot, int *fail_row);
 };
 

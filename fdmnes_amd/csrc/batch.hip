@@ -48,6 +48,8 @@ __global__ void k_adjoint_grad(const double *__restrict__ lam, const double *__r
 template <class R, bool C>
 Solver<R, C>::~Solver() {
   for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
+  for (auto &kv : sweeps_g) (void)hipGraphExecDestroy(kv.second);
+  if (hnorm) (void)hipHostFree(hnorm);
   grp.release();
   for (auto &s : slots) {
     if (s.row_ptr) (void)hipFree(s.row_ptr);
@@ -115,7 +117,7 @@ int Solver<R, C>::add_system(int n, const int *lb1, const int *lb2, const int64_
     FDM_HIP(hipMalloc(&sl.Xd, sizeof(double) * xe));
     FDM_HIP(hipMalloc(&sl.Rr, sizeof(double) * xe));
     FDM_HIP(hipMemset(sl.Rr, 0, sizeof(double) * xe));
-    FDM_HIP(hipMalloc(&sl.scr, sizeof(double) * 256 * std::max(nrhs, 1)));
+    FDM_HIP(hipMalloc(&sl.scr, sizeof(double) * NORM_GRID * std::max(nrhs, 1)));
   }
   if (grp.add(sl.plan.get(), nrhs)) return -2;
   *slot = (int)slots.size();
@@ -188,6 +190,50 @@ int Solver<R, C>::factor_phase(uint32_t mask) {
   return 0;
 }
 
+// One refinement sweep over the active slots: d = A32^-1 r (FP32 factors),
+// x += d, r = b - A x (FP64), squared residual norms per right-hand side.
+// Captured once per active mask.
+template <class R, bool C>
+int Solver<R, C>::sweep_phase(uint32_t mask) {
+  auto record = [&]() -> int {
+    for (size_t g = 0; g < slots.size(); g++) {
+      if (!grp.active[g]) continue;
+      const size_t tot = (size_t)slots[g].plan->NB * TB * nrhs * NC;
+      if (convert<R>(slots[g].Rr, grp.m[g].X, tot, stream)) return -2;
+    }
+    if (grp.solve(S_UPPER) || grp.solve(S_LOWER)) return -2;
+    for (size_t g = 0; g < slots.size(); g++) {
+      if (!grp.active[g]) continue;
+      Slot &sl = slots[g];
+      const size_t tot = (size_t)sl.plan->NB * TB * nrhs * NC;
+      DevCsr a{sl.n, sl.nnz, sl.row_ptr, sl.col, sl.vals};
+      if (accumulate<R>(sl.Xd, grp.m[g].X, tot, stream) || residual<C>(a, sl.B, sl.Xd, sl.Rr, nrhs, stream) ||
+          launch_norms2<C>(sl.Rr, sl.n, nrhs, sl.scr, stream))
+        return -2;
+    }
+    return 0;
+  };
+  if (!use_graphs || grp.time_updates) return record();
+  auto it = sweeps_g.find(mask);
+  if (it == sweeps_g.end()) {
+    hipGraph_t graph;
+    FDM_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    int rc = record();
+    hipError_t e = hipStreamEndCapture(stream, &graph);
+    if (rc) return rc;
+    if (e != hipSuccess) {
+      set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+      return -2;
+    }
+    hipGraphExec_t exec;
+    FDM_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(graph);
+    it = sweeps_g.emplace(mask, exec).first;
+  }
+  FDM_HIP(hipGraphLaunch(it->second, stream));
+  return 0;
+}
+
 template <class R, bool C>
 int Solver<R, C>::fallback_fp64(int g, int *fail_row) {
   Slot &sl = slots[g];
@@ -240,44 +286,45 @@ int Solver<R, C>::solve(int count, const int *ids, const double *const *vals, co
   std::vector<int> act;
   for (int k = 0; k < count; k++)
     if (fr[ids[k]] == 0) act.push_back(k);
+  if (!hnorm) FDM_HIP(hipHostMalloc(&hnorm, sizeof(double) * MAXG * NORM_GRID * std::max(nrhs, 1)));
+  const size_t np = (size_t)NORM_GRID * nrhs;
   std::vector<std::vector<double>> bn(count, std::vector<double>(nrhs)), rn(count, std::vector<double>(nrhs));
   std::vector<double> r(count, INFINITY);
   std::vector<int> sw(count, 0);
   for (int k : act) {
     Slot &sl = slots[ids[k]];
     const size_t tot = (size_t)sl.plan->NB * TB * nrhs * NC;
-    if (col_norms2<C>(sl.B, sl.n, nrhs, bn[k].data(), sl.scr, 256 * nrhs, stream)) return -2;
+    if (launch_norms2<C>(sl.B, sl.n, nrhs, sl.scr, stream)) return -2;
+    FDM_HIP(hipMemcpyAsync(hnorm + k * np, sl.scr, sizeof(double) * np, hipMemcpyDeviceToHost, stream));
     FDM_HIP(hipMemsetAsync(sl.Xd, 0, sizeof(double) * tot, stream));
     FDM_HIP(hipMemcpyAsync(sl.Rr, sl.B, sizeof(double) * tot, hipMemcpyDeviceToDevice, stream));
   }
-  std::vector<int> ids_act;
-  for (int k : act) ids_act.push_back(ids[k]);
+  FDM_HIP(hipStreamSynchronize(stream));
+  for (int k : act) sum_norm_partials(hnorm + k * np, nrhs, bn[k].data());
   for (int it = 0; !act.empty(); it++) {
-    if (set_active((int)ids_act.size(), ids_act.data())) return -1;
+    std::vector<int> ids_act;
+    uint32_t m = 0;
     for (int k : act) {
-      Slot &sl = slots[ids[k]];
-      const size_t tot = (size_t)sl.plan->NB * TB * nrhs * NC;
-      if (convert<R>(sl.Rr, grp.m[ids[k]].X, tot, stream)) return -2;
+      ids_act.push_back(ids[k]);
+      m |= 1u << ids[k];
     }
-    if (grp.solve(S_UPPER) || grp.solve(S_LOWER)) return -2;
+    if (set_active((int)ids_act.size(), ids_act.data())) return -1;
+    if (sweep_phase(m)) return -2;
+    for (int k : act)
+      FDM_HIP(hipMemcpyAsync(hnorm + k * np, slots[ids[k]].scr, sizeof(double) * np, hipMemcpyDeviceToHost, stream));
+    FDM_HIP(hipStreamSynchronize(stream));
     for (int k : act) {
-      Slot &sl = slots[ids[k]];
-      const size_t tot = (size_t)sl.plan->NB * TB * nrhs * NC;
-      DevCsr a{sl.n, sl.nnz, sl.row_ptr, sl.col, sl.vals};
-      if (accumulate<R>(sl.Xd, grp.m[ids[k]].X, tot, stream) || residual<C>(a, sl.B, sl.Xd, sl.Rr, nrhs, stream) ||
-          col_norms2<C>(sl.Rr, sl.n, nrhs, rn[k].data(), sl.scr, 256 * nrhs, stream))
-        return -2;
-      double m = 0;
-      for (int l = 0; l < nrhs; l++) m = std::max(m, bn[k][l] > 0 ? std::sqrt(rn[k][l] / bn[k][l]) : std::sqrt(rn[k][l]));
-      r[k] = m;
+      sum_norm_partials(hnorm + k * np, nrhs, rn[k].data());
+      double mx = 0;
+      for (int l = 0; l < nrhs; l++)
+        mx = std::max(mx, bn[k][l] > 0 ? std::sqrt(rn[k][l] / bn[k][l]) : std::sqrt(rn[k][l]));
+      r[k] = mx;
       sw[k] = it;
     }
     std::vector<int> keep;
     for (int k : act)
       if (!(r[k] <= tol) && r[k] == r[k] && it < max_sweeps) keep.push_back(k);
     act.swap(keep);
-    ids_act.clear();
-    for (int k : act) ids_act.push_back(ids[k]);
   }
   // members that did not converge (or whose FP32 pivots failed) fall back to FP64
   for (int k = 0; k < count; k++) {
@@ -296,6 +343,7 @@ int Solver<R, C>::solve(int count, const int *ids, const double *const *vals, co
     if (rel) rel[k] = r[k];
   }
   FDM_HIP(hipStreamSynchronize(stream));
+  if (grp.check_solve()) return -2;
   last_sweeps = sw;
   last_rel = r;
   return 0;
@@ -323,6 +371,7 @@ int Solver<R, C>::adjoint(int count, const int *ids, const double *const *src, c
   }
   FDM_HIP(hipGetLastError());
   FDM_HIP(hipStreamSynchronize(stream));
+  if (grp.check_solve()) return -2;
   return 0;
 }
 
@@ -342,6 +391,14 @@ int Solver<R, C>::sigma(int count, const int *ids, const double *const *src, con
     return -2;
   }
   return 0;
+}
+
+template <class R, bool C>
+void Solver<R, C>::drop_graphs() {
+  for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
+  for (auto &kv : sweeps_g) (void)hipGraphExecDestroy(kv.second);
+  graphs.clear();
+  sweeps_g.clear();
 }
 
 template struct Solver<double, true>;

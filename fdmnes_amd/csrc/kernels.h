@@ -44,6 +44,7 @@ struct SysDev {
   R *X;     // right-hand sides / solution, (NB*TB) x nrhs, interleaved when complex
   R *part;  // solve partial sums
   int nrhs;
+  int *done;  // per block: solved flag of the persistent substitution
 };
 
 struct Items {
@@ -243,8 +244,7 @@ constexpr int KC = 16;  // k-chunk of the LDS pipeline
 // items of all systems; per tile the K dimension (nk*TB) streams through a
 // two-buffer LDS pipeline in chunks of KC, the next chunk's global loads in
 // flight while the current chunk feeds the MFMAs.  Four waves, each a 32x32
-// quadrant in 2x2 blocks of 16x16x4 MFMAs; a complex product is four real
-// MFMAs.  LDS holds A chunks k-major (As[k][row]) so the MFMA A operand reads
+// quadrant in 2x2 blocks of 16x16x4 MFMAs.  LDS holds A chunks k-major (As[k][row]) so the MFMA A operand reads
 // 16 consecutive rows per k.
 template <class R, bool C>
 __global__ __launch_bounds__(NT, 2) void k_trail(const SysDev<R> *__restrict__ sys, TrailItems it) {
@@ -259,6 +259,10 @@ __global__ __launch_bounds__(NT, 2) void k_trail(const SysDev<R> *__restrict__ s
   const int lane = tid & 63, wave = tid >> 6;
   const int r0 = (wave >> 1) * 32, c0 = (wave & 1) * 32;
   const int li = lane & 15, lk = lane >> 4;
+  // FP32 complex products use three real MFMAs (Gauss: P1 = ar br,
+  // P2 = ai bi, P3 = (ar + ai)(br + bi); re = P1 - P2, im = P3 - P1 - P2);
+  // the FP64 path keeps four so its rounding matches an ordinary product.
+  constexpr bool M3 = C && sizeof(R) == 4;
   const int total = it.off[it.count];
   for (int wb = blockIdx.x; wb < total; wb += gridDim.x) {
     const int g = find_trail(it, wb);
@@ -269,7 +273,8 @@ __global__ __launch_bounds__(NT, 2) void k_trail(const SysDev<R> *__restrict__ s
     int m = 0;
     while (m < it.nk[g] && p.I0[it.Kb[g] + m] <= I && p.J0[it.Kb[g] + m] <= J) m++;
     const int nch = m * (TB / KC);
-    v4 accr[2][2], acci[2][2];
+    constexpr int NX = M3 ? 2 : 1;  // P2 accumulators only for the 3M form
+    v4 accr[2][2], acci[2][2], accx[NX][NX];
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
@@ -277,6 +282,10 @@ __global__ __launch_bounds__(NT, 2) void k_trail(const SysDev<R> *__restrict__ s
         accr[a][bb] = v4{0, 0, 0, 0};
         acci[a][bb] = v4{0, 0, 0, 0};
       }
+#pragma unroll
+    for (int a = 0; a < NX; a++)
+#pragma unroll
+      for (int bb = 0; bb < NX; bb++) accx[a][bb] = v4{0, 0, 0, 0};
     // chunk c: step K = Kb + c/4, columns kc..kc+15 of A = tile(I,K), rows of B = tile(K,J)
     r2 pa[PER], pb[PER];
     R qa[PER], qb[PER];
@@ -331,18 +340,43 @@ __global__ __launch_bounds__(NT, 2) void k_trail(const SysDev<R> *__restrict__ s
           br[bb] = Bs[buf][0][k4 + lk][c0 + bb * 16 + li];
           if constexpr (C) bi[bb] = Bs[buf][NC - 1][k4 + lk][c0 + bb * 16 + li];
         }
+        if constexpr (M3) {
+          R as[2], bs[2];
 #pragma unroll
-        for (int a = 0; a < 2; a++)
+          for (int a = 0; a < 2; a++) as[a] = ar[a] + ai[a];
 #pragma unroll
-          for (int bb = 0; bb < 2; bb++) {
-            accr[a][bb] = Mfma<R>::run(ar[a], br[bb], accr[a][bb]);
-            if constexpr (C) {
-              accr[a][bb] = Mfma<R>::run(-ai[a], bi[bb], accr[a][bb]);
-              acci[a][bb] = Mfma<R>::run(ar[a], bi[bb], acci[a][bb]);
-              acci[a][bb] = Mfma<R>::run(ai[a], br[bb], acci[a][bb]);
+          for (int bb = 0; bb < 2; bb++) bs[bb] = br[bb] + bi[bb];
+#pragma unroll
+          for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int bb = 0; bb < 2; bb++) {
+              accr[a][bb] = Mfma<R>::run(ar[a], br[bb], accr[a][bb]);   // P1
+              accx[a % NX][bb % NX] = Mfma<R>::run(ai[a], bi[bb], accx[a % NX][bb % NX]);  // P2
+              acci[a][bb] = Mfma<R>::run(as[a], bs[bb], acci[a][bb]);   // P3
             }
-          }
+        } else {
+#pragma unroll
+          for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int bb = 0; bb < 2; bb++) {
+              accr[a][bb] = Mfma<R>::run(ar[a], br[bb], accr[a][bb]);
+              if constexpr (C) {
+                accr[a][bb] = Mfma<R>::run(-ai[a], bi[bb], accr[a][bb]);
+                acci[a][bb] = Mfma<R>::run(ar[a], bi[bb], acci[a][bb]);
+                acci[a][bb] = Mfma<R>::run(ai[a], br[bb], acci[a][bb]);
+              }
+            }
+        }
       }
+    }
+    if constexpr (M3) {
+#pragma unroll
+      for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int bb = 0; bb < 2; bb++) {
+          acci[a][bb] = acci[a][bb] - accr[a][bb] - accx[a % NX][bb % NX];
+          accr[a][bb] = accr[a][bb] - accx[a % NX][bb % NX];
+        }
     }
     if (nch > 0) {
       R *Cg = tile_ptr<R, C>(p, S.tiles, I, J);
@@ -368,6 +402,20 @@ __global__ __launch_bounds__(NT, 2) void k_trail(const SysDev<R> *__restrict__ s
 // |a|^2 < 1e-20 (complex) or |a| < 1e-10 (real); the largest failing row
 // (the first in elimination order) is recorded 1-based in *fail.  One
 // workgroup per item.
+//
+// Elimination step q (one barrier): every row r < q subtracts
+// (a_rq / a_qq) * row q from its columns k < q; thread (r = tid/4) owns row r
+// and the columns k = tid%4 (mod 4).  The normalised row q (the stored L row,
+// a_qk / a_qq) is kept in a register and written back during step q-1, when
+// row q is no longer read.  The inverses are column-parallel substitutions,
+// four lanes per column combining partial sums by lane shuffles.
+template <class R>
+__device__ __forceinline__ R lane_sum4(R v) {
+  v += __shfl_xor(v, 1);
+  v += __shfl_xor(v, 2);
+  return v;
+}
+
 template <class R, bool C>
 __global__ __launch_bounds__(NT) void k_panel(const SysDev<R> *__restrict__ sys, Items it) {
   constexpr int NC = Elem<R, C>::NC;
@@ -380,8 +428,11 @@ __global__ __launch_bounds__(NT) void k_panel(const SysDev<R> *__restrict__ sys,
   const int K = it.K[blockIdx.x];
   R *T = tile_ptr<R, C>(p, S.tiles, K, K);
   stage_tile<R, C, LD>(T, D);
+  __syncthreads();
+  const int myr = tid >> 2, part = tid & 3;
+  int pend = -1;
+  R pr = 0, pi = 0;
   for (int q = TB - 1; q >= 0; --q) {
-    __syncthreads();
     const R dr = D[0][q][q];
     R di = 0;
     if constexpr (C) di = D[NC - 1][q][q];
@@ -395,108 +446,113 @@ __global__ __launch_bounds__(NT) void k_panel(const SysDev<R> *__restrict__ sys,
       if (tid == 0 && fabs(dr) < R(1e-10) && K * TB + q < p.n) atomicMax(S.fail, K * TB + q + 1);
       vr = R(1) / dr;
     }
-    for (int k = tid; k < q; k += NT) {  // normalise the stored row, :274-280 / :296-300
+    if (pend >= 0) {  // normalised row q+1, element k = tid
+      D[0][pend][tid] = pr;
+      if constexpr (C) D[NC - 1][pend][tid] = pi;
+    }
+    pend = -1;
+    if (tid < q) {  // normalise the stored row, :274-280 / :296-300
+      const R xr = D[0][q][tid];
       if constexpr (C) {
-        const R xr = D[0][q][k], xi = D[NC - 1][q][k];
-        D[0][q][k] = xr * vr - xi * vi;
-        D[NC - 1][q][k] = xi * vr + xr * vi;
+        const R xi = D[NC - 1][q][tid];
+        pr = xr * vr - xi * vi;
+        pi = xi * vr + xr * vi;
       } else {
-        D[0][q][k] = D[0][q][k] * vr;
+        pr = xr * vr;
+      }
+      pend = q;
+    }
+    if (myr < q) {  // eliminate column q from row myr
+      R fr = D[0][myr][q], fi = 0;
+      if constexpr (C) {
+        fi = D[NC - 1][myr][q];
+        const R t = fr * vr - fi * vi;
+        fi = fr * vi + fi * vr;
+        fr = t;
+      } else {
+        fr = fr * vr;
+      }
+      for (int k = part; k < q; k += 4) {
+        const R lr = D[0][q][k];
+        if constexpr (C) {
+          const R li = D[NC - 1][q][k];
+          D[0][myr][k] = D[0][myr][k] - fr * lr + fi * li;
+          D[NC - 1][myr][k] = D[NC - 1][myr][k] - fr * li - fi * lr;
+        } else {
+          D[0][myr][k] = D[0][myr][k] - fr * lr;
+        }
       }
     }
     __syncthreads();
-    for (int idx = tid; idx < q * q; idx += NT) {  // eliminate column q from the rows above
-      const int r = idx / q, k = idx % q;
-      if constexpr (C) {
-        const R fr = D[0][r][q], fi = D[NC - 1][r][q];
-        const R lr = D[0][q][k], li = D[NC - 1][q][k];
-        D[0][r][k] = D[0][r][k] - fr * lr + fi * li;
-        D[NC - 1][r][k] = D[NC - 1][r][k] - fr * li - fi * lr;
-      } else {
-        D[0][r][k] = D[0][r][k] - D[0][r][q] * D[0][q][k];
-      }
-    }
   }
-  __syncthreads();
   for (int e = tid; e < TB * TB; e += NT) {
     T[(int64_t)e * NC] = D[0][e / TB][e % TB];
     if constexpr (C) T[(int64_t)e * NC + 1] = D[NC - 1][e / TB][e % TB];
   }
-  // X = U_D^-1 (upper, non-unit): X U = I, columns j ascending.
-  for (int e = tid; e < TB * TB; e += NT) {
-    X[0][e / TB][e % TB] = 0;
-    if constexpr (C) X[NC - 1][e / TB][e % TB] = 0;
-  }
-  __syncthreads();
-  for (int j = 0; j < TB; j++) {
-    const R ujr = D[0][j][j];
-    R uji = 0;
-    if constexpr (C) uji = D[NC - 1][j][j];
-    R ivr, ivi = 0;
-    if constexpr (C) {
-      const R den = ujr * ujr + uji * uji;
-      ivr = ujr / den;
-      ivi = -uji / den;
-    } else {
-      ivr = R(1) / ujr;
-    }
-    if (tid < j) {
-      const int i = tid;
+  const int j = tid >> 2;
+  // X = U_D^-1: U X = I, column j from its last row up.
+  for (int i = TB - 1; i >= 0; --i) {
+    if (i <= j) {
       R sr = 0, si = 0;
-      for (int k = i; k < j; k++) {
-        const R xr = X[0][i][k], ur = D[0][k][j];
+      for (int k = i + 1 + part; k <= j; k += 4) {
+        const R ur = D[0][i][k], xr = X[0][k][j];
         if constexpr (C) {
-          const R xi = X[NC - 1][i][k], ui = D[NC - 1][k][j];
-          sr += xr * ur - xi * ui;
-          si += xr * ui + xi * ur;
+          const R ui = D[NC - 1][i][k], xi = X[NC - 1][k][j];
+          sr += ur * xr - ui * xi;
+          si += ur * xi + ui * xr;
         } else {
-          sr += xr * ur;
+          sr += ur * xr;
         }
       }
+      sr = lane_sum4(sr);
+      if constexpr (C) si = lane_sum4(si);
+      sr = (i == j ? R(1) : R(0)) - sr;
+      si = -si;
+      const R ur = D[0][i][i];
       if constexpr (C) {
-        X[0][i][j] = -(sr * ivr - si * ivi);
-        X[NC - 1][i][j] = -(sr * ivi + si * ivr);
+        const R ui = D[NC - 1][i][i], den = ur * ur + ui * ui;
+        const R ivr = ur / den, ivi = -ui / den;
+        X[0][i][j] = sr * ivr - si * ivi;
+        X[NC - 1][i][j] = sr * ivi + si * ivr;
       } else {
-        X[0][i][j] = -sr * ivr;
+        X[0][i][j] = sr / ur;
       }
-    } else if (tid == j) {
-      X[0][j][j] = ivr;
-      if constexpr (C) X[NC - 1][j][j] = ivi;
+    } else if (part == 0) {
+      X[0][i][j] = 0;
+      if constexpr (C) X[NC - 1][i][j] = 0;
     }
-    __syncthreads();
   }
+  __syncthreads();
   R *Ui = S.dinv + (int64_t)K * 2 * TB * TB * NC;
   for (int e = tid; e < TB * TB; e += NT) {
     Ui[(int64_t)e * NC] = X[0][e / TB][e % TB];
     if constexpr (C) Ui[(int64_t)e * NC + 1] = X[NC - 1][e / TB][e % TB];
   }
   __syncthreads();
-  // X = L_D^-1 (unit lower): X L = I, columns j descending.
-  for (int e = tid; e < TB * TB; e += NT) {
-    const int r = e / TB, c = e % TB;
-    X[0][r][c] = (r == c) ? R(1) : R(0);
-    if constexpr (C) X[NC - 1][r][c] = 0;
-  }
-  __syncthreads();
-  for (int j = TB - 2; j >= 0; j--) {
-    const int i = j + 1 + tid;
-    if (i < TB) {
+  // X = L_D^-1 (unit lower): L X = I, column j from its diagonal down.
+  for (int i = 0; i < TB; ++i) {
+    if (i >= j) {
       R sr = 0, si = 0;
-      for (int k = j + 1; k <= i; k++) {
-        const R xr = X[0][i][k], lr = D[0][k][j];
+      for (int k = j + part; k < i; k += 4) {
+        const R lr = D[0][i][k], xr = X[0][k][j];
         if constexpr (C) {
-          const R xi = X[NC - 1][i][k], li = D[NC - 1][k][j];
-          sr += xr * lr - xi * li;
-          si += xr * li + xi * lr;
+          const R li = D[NC - 1][i][k], xi = X[NC - 1][k][j];
+          sr += lr * xr - li * xi;
+          si += lr * xi + li * xr;
         } else {
-          sr += xr * lr;
+          sr += lr * xr;
         }
       }
-      X[0][i][j] = -sr;
+      sr = lane_sum4(sr);
+      if constexpr (C) si = lane_sum4(si);
+      X[0][i][j] = (i == j ? R(1) : R(0)) - sr;
       if constexpr (C) X[NC - 1][i][j] = -si;
+    } else if (part == 0) {
+      X[0][i][j] = 0;
+      if constexpr (C) X[NC - 1][i][j] = 0;
     }
-    __syncthreads();
   }
+  __syncthreads();
   R *Li = S.dinv + ((int64_t)K * 2 + 1) * TB * TB * NC;
   for (int e = tid; e < TB * TB; e += NT) {
     Li[(int64_t)e * NC] = X[0][e / TB][e % TB];
@@ -648,6 +704,174 @@ __global__ __launch_bounds__(NT) void k_solve_finish(const SysDev<R> *__restrict
       S.X[gi] = accr[u];
       if constexpr (C) S.X[gi + 1] = acci[u];
     }
+  }
+}
+
+// Persistent block triangular solve: one launch runs every block step of
+// every listed system.  Task t (in dependency order: block steps aligned at
+// their end across systems, as the factorisation's launches are) is taken
+// from a ticket counter, so a task only ever waits on tasks with smaller
+// tickets, which running workgroups hold: progress needs no scheduling
+// assumption.  A task accumulates T(K,J) x_J over J in S(K), waiting on each
+// J's done flag (agent-scope acquire), then writes x_K and publishes its own
+// flag (agent-scope release, so other XCDs' caches see x_K).  Waits are
+// bounded: past SPIN_LIMIT polls the task records an error and goes on, so
+// the kernel always terminates.
+constexpr int SPIN_LIMIT = 1 << 22;
+
+// Systems of one persistent solve: system sys[i] runs its NB[i] block steps
+// at global steps off[i] .. off[i] + NB[i] - 1 (aligned at the end).  Tasks
+// are numbered step by step, systems in list order within a step.
+struct SolveSet {
+  int count;
+  int maxNB;
+  int sys[MAXG];
+  int off[MAXG];
+  int nb[MAXG];
+};
+
+__device__ __forceinline__ int tasks_before(const SolveSet &ss, int step) {
+  int t = 0;
+  for (int i = 0; i < ss.count; i++) t += max(0, min(step, ss.off[i] + ss.nb[i]) - ss.off[i]);
+  return t;
+}
+
+__device__ __forceinline__ void wait_flag(const int *flag, int *err) {
+  int n = 0;
+  while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+    if (++n > SPIN_LIMIT) {
+      atomicExch(err, 1);
+      break;
+    }
+    __builtin_amdgcn_s_sleep(2);
+  }
+}
+
+template <class R, bool C>
+__global__ __launch_bounds__(NT) void k_solve_persist(const SysDev<R> *__restrict__ sys, SolveSet ss, int kind,
+                                                      int *counter, int *err) {
+  constexpr int NC = Elem<R, C>::NC;
+  constexpr int LD = TB + 1;
+  __shared__ R Ts[NC][TB][LD];
+  __shared__ R Xs[NC][TB][RC + 1];
+  __shared__ int tk;
+  const int tid = threadIdx.x, r = tid >> 2, gq = tid & 3;
+  const bool trans = (kind == S_LOWER_T || kind == S_UPPER_T);
+  const bool upper = (kind == S_UPPER || kind == S_UPPER_T);
+  const bool asc = (kind == S_LOWER || kind == S_UPPER_T);
+  const int ntasks = tasks_before(ss, ss.maxNB);
+  for (;;) {
+    __syncthreads();
+    if (tid == 0) tk = atomicAdd(counter, 1);
+    __syncthreads();
+    const int t = tk;
+    if (t >= ntasks) break;
+    // task -> (step, system): last step whose first task is <= t
+    int lo = 0, hi = ss.maxNB - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (tasks_before(ss, mid) <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    int k = t - tasks_before(ss, lo), i = 0;
+    for (; i < ss.count; i++) {
+      if (lo >= ss.off[i] && lo < ss.off[i] + ss.nb[i]) {
+        if (k == 0) break;
+        k--;
+      }
+    }
+    const int g = ss.sys[i];
+    const int tg = lo - ss.off[i];
+    const int K = asc ? tg : ss.nb[i] - 1 - tg;
+    const SysDev<R> &S = sys[g];
+    const DevPlan &p = S.p;
+    const int nrhs = S.nrhs;
+    int jb, je;
+    solve_range(kind, K, p.J0[K], p.J1[K], p.I0[K], p.Jx[K], &jb, &je);
+    for (int rc0 = 0; rc0 < nrhs; rc0 += RC) {
+      R accr[4] = {0, 0, 0, 0}, acci[4] = {0, 0, 0, 0};
+      // oldest dependencies first: the last one is the neighbour block
+      for (int q = 0; q < je - jb; q++) {
+        const int J = asc ? jb + q : je - 1 - q;
+        if (tid == 0) wait_flag(&S.done[J], err);
+        __syncthreads();
+        const R *T = trans ? tile_ptr<R, C>(p, S.tiles, J, K) : tile_ptr<R, C>(p, S.tiles, K, J);
+        stage_tile<R, C, LD>(T, Ts);
+        for (int e = tid; e < TB * RC; e += NT) {
+          const int rr = e / RC, cc = e % RC;
+          const int64_t gi = ((int64_t)(J * TB + rr) * nrhs + rc0 + cc) * NC;
+          const bool ok = rc0 + cc < nrhs;
+          Xs[0][rr][cc] = ok ? S.X[gi] : R(0);
+          if constexpr (C) Xs[NC - 1][rr][cc] = ok ? S.X[gi + 1] : R(0);
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int k = 0; k < TB; k++) {
+          const R tr = trans ? Ts[0][k][r] : Ts[0][r][k];
+          R ti = 0;
+          if constexpr (C) ti = trans ? Ts[NC - 1][k][r] : Ts[NC - 1][r][k];
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            const R xr = Xs[0][k][gq * 4 + u];
+            if constexpr (C) {
+              const R xi = Xs[NC - 1][k][gq * 4 + u];
+              accr[u] += tr * xr - ti * xi;
+              acci[u] += tr * xi + ti * xr;
+            } else {
+              accr[u] += tr * xr;
+            }
+          }
+        }
+        __syncthreads();
+      }
+      // x_K = Dinv'(K) (r_K - acc)
+      const R *Dg = S.dinv + ((int64_t)K * 2 + (upper ? 0 : 1)) * TB * TB * NC;
+      stage_tile<R, C, LD>(Dg, Ts);
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int cc = gq * 4 + u;
+        const int64_t gi = ((int64_t)(K * TB + r) * nrhs + rc0 + cc) * NC;
+        const bool ok = rc0 + cc < nrhs;
+        R sr = ok ? S.X[gi] : R(0), si = 0;
+        if constexpr (C) si = ok ? S.X[gi + 1] : R(0);
+        Xs[0][r][cc] = sr - accr[u];
+        if constexpr (C) Xs[NC - 1][r][cc] = si - acci[u];
+      }
+      __syncthreads();
+      R yr[4] = {0, 0, 0, 0}, yi[4] = {0, 0, 0, 0};
+#pragma unroll 4
+      for (int k = 0; k < TB; k++) {
+        const R dr = trans ? Ts[0][k][r] : Ts[0][r][k];
+        R di = 0;
+        if constexpr (C) di = trans ? Ts[NC - 1][k][r] : Ts[NC - 1][r][k];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const R tr = Xs[0][k][gq * 4 + u];
+          if constexpr (C) {
+            const R ti = Xs[NC - 1][k][gq * 4 + u];
+            yr[u] += dr * tr - di * ti;
+            yi[u] += dr * ti + di * tr;
+          } else {
+            yr[u] += dr * tr;
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int cc = gq * 4 + u;
+        if (rc0 + cc < nrhs) {
+          const int64_t gi = ((int64_t)(K * TB + r) * nrhs + rc0 + cc) * NC;
+          S.X[gi] = yr[u];
+          if constexpr (C) S.X[gi + 1] = yi[u];
+        }
+      }
+      __syncthreads();
+    }
+    // publish x_K to every XCD, then the flag
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) __hip_atomic_store(&S.done[K], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 

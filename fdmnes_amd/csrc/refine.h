@@ -22,6 +22,14 @@ template <bool C>
 int col_norms2(const double *V, int n, int nrhs, double *out, double *dscratch, int scratch_elems,
                hipStream_t s);
 
+// Launch-only variant (graph-capturable): partial sums of NORM_GRID blocks
+// per right-hand side into dscratch[NORM_GRID * nrhs]; finish with
+// sum_norm_partials on the host copy.
+constexpr int NORM_GRID = 256;
+template <bool C>
+int launch_norms2(const double *V, int n, int nrhs, double *dscratch, hipStream_t s);
+void sum_norm_partials(const double *h, int nrhs, double *out);
+
 // dst (R) <- src (double), elementwise, count elements.
 template <class R>
 int convert(const double *src, R *dst, size_t count, hipStream_t s);

@@ -99,6 +99,25 @@ int col_norms2(const double *V, int n, int nrhs, double *out, double *dscratch, 
   return 0;
 }
 
+template <bool C>
+int launch_norms2(const double *V, int n, int nrhs, double *dscratch, hipStream_t s) {
+  if (nrhs > 256) {
+    set_error("launch_norms2: at most 256 right-hand sides");
+    return -1;
+  }
+  hipLaunchKernelGGL((k_norms2<C>), dim3(NORM_GRID), dim3(256), 256 * sizeof(double), s, V, n, nrhs, dscratch);
+  FDM_HIP(hipGetLastError());
+  return 0;
+}
+
+void sum_norm_partials(const double *h, int nrhs, double *out) {
+  for (int l = 0; l < nrhs; l++) {
+    double acc = 0;
+    for (int g = 0; g < NORM_GRID; g++) acc += h[(size_t)g * nrhs + l];
+    out[l] = acc;
+  }
+}
+
 template <class R>
 int convert(const double *src, R *dst, size_t count, hipStream_t s) {
   hipLaunchKernelGGL((k_convert<R>), dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, src, dst, count);
@@ -117,7 +136,11 @@ template int residual<true>(const DevCsr &, const double *, const double *, doub
 template int residual<false>(const DevCsr &, const double *, const double *, double *, int, hipStream_t);
 template int col_norms2<true>(const double *, int, int, double *, double *, int, hipStream_t);
 template int col_norms2<false>(const double *, int, int, double *, double *, int, hipStream_t);
+template int launch_norms2<true>(const double *, int, int, double *, hipStream_t);
+template int launch_norms2<false>(const double *, int, int, double *, hipStream_t);
 template int convert<float>(const double *, float *, size_t, hipStream_t);
 template int accumulate<float>(double *, const float *, size_t, hipStream_t);
+template int convert<double>(const double *, double *, size_t, hipStream_t);
+template int accumulate<double>(double *, const double *, size_t, hipStream_t);
 
 }  // namespace fdmgpu

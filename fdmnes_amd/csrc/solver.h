@@ -62,6 +62,7 @@ struct Group {
     Plan *plan = nullptr;
     R *tiles = nullptr, *dinv = nullptr, *X = nullptr, *part = nullptr;
     int *fail = nullptr;
+    int *done = nullptr;  // NB block flags of the persistent substitution
     int nrhs = 0;
   };
   std::vector<Member> m;
@@ -69,6 +70,9 @@ struct Group {
   SysDev<R> *d_sys = nullptr;
   int persist_wg = 1024;     // workgroups of the persistent tile GEMMs
   int gk = 4;                // block steps per trailing update (0: one-step tile GEMM)
+  bool persist_solve = true; // one dependency-driven launch per substitution
+  int solve_wg = 512;        // workgroups of the persistent substitution
+  int *ctl = nullptr;        // device: [0] task counter, [1] wait-timeout flag
   hipStream_t stream = nullptr;
   int maxparts = 1;
   // per-kernel timing (optional): cumulative ms of the G_UPDATE launches
@@ -93,7 +97,8 @@ struct Group {
   int scatter_csr(int g, const int64_t *d_row_ptr, const int *d_col, const double *d_vals);
   int factor();
   int solve(int kind);  // in place on every member's X
-  int read_fail(std::vector<int> &rows);
+  int read_fail(std::vector<int> &rows);  // also reports a substitution wait timeout
+  int check_solve();                      // substitution wait timeout only
 
  private:
   std::vector<char> owns_x;

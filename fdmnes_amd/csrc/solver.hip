@@ -129,6 +129,7 @@ int Group<R, C>::add(Plan *plan, int nrhs, R *X) {
   FDM_HIP(hipMalloc(&mb.tiles, sizeof(R) * te * (size_t)plan->ntiles));
   FDM_HIP(hipMalloc(&mb.dinv, sizeof(R) * te * 2 * (size_t)plan->NB));
   FDM_HIP(hipMalloc(&mb.fail, sizeof(int)));
+  FDM_HIP(hipMalloc(&mb.done, sizeof(int) * plan->NB));
   if (X) {
     mb.X = X;
     owns_x.push_back(0);
@@ -153,8 +154,12 @@ int Group<R, C>::finalize(hipStream_t s) {
   }
   std::vector<SysDev<R>> h(m.size());
   for (size_t g = 0; g < m.size(); g++)
-    h[g] = SysDev<R>{m[g].plan->dev(), m[g].tiles, m[g].dinv, m[g].fail, m[g].X, m[g].part, m[g].nrhs};
+    h[g] = SysDev<R>{m[g].plan->dev(), m[g].tiles, m[g].dinv, m[g].fail, m[g].X, m[g].part, m[g].nrhs, m[g].done};
   if (d_sys) (void)hipFree(d_sys);
+  if (!ctl) {
+    FDM_HIP(hipMalloc(&ctl, sizeof(int) * 2));
+    FDM_HIP(hipMemset(ctl, 0, sizeof(int) * 2));
+  }
   FDM_HIP(hipMalloc(&d_sys, sizeof(SysDev<R>) * h.size()));
   FDM_HIP(hipMemcpy(d_sys, h.data(), sizeof(SysDev<R>) * h.size(), hipMemcpyHostToDevice));
   return 0;
@@ -168,6 +173,7 @@ void Group<R, C>::release() {
     if (mb.dinv) (void)hipFree(mb.dinv);
     if (mb.part) (void)hipFree(mb.part);
     if (mb.fail) (void)hipFree(mb.fail);
+    if (mb.done) (void)hipFree(mb.done);
     if (mb.X && owns_x[g]) (void)hipFree(mb.X);
   }
   m.clear();
@@ -175,6 +181,8 @@ void Group<R, C>::release() {
   owns_x.clear();
   if (d_sys) (void)hipFree(d_sys);
   d_sys = nullptr;
+  if (ctl) (void)hipFree(ctl);
+  ctl = nullptr;
   for (auto e : ev) (void)hipEventDestroy(e);
   ev.clear();
 }
@@ -249,8 +257,13 @@ int Group<R, C>::factor() {
     for (auto &e : ev) FDM_HIP(hipEventCreate(&e));
   }
   int nev = 0;
+  // A start event queued behind unfinished kernels would be stamped before
+  // they complete; drain the stream first so the pair brackets one launch.
   auto timed = [&](auto launch) -> int {
-    if (time_updates) FDM_HIP(hipEventRecord(ev[nev], stream));
+    if (time_updates) {
+      FDM_HIP(hipStreamSynchronize(stream));
+      FDM_HIP(hipEventRecord(ev[nev], stream));
+    }
     launch();
     if (time_updates) {
       FDM_HIP(hipEventRecord(ev[nev + 1], stream));
@@ -330,6 +343,26 @@ int Group<R, C>::solve(int kind) {
     nch = std::max(nch, (m[g].nrhs + RC - 1) / RC);
   }
   const bool asc = (kind == S_LOWER || kind == S_UPPER_T);
+  if (persist_solve) {
+    SolveSet ss{};
+    ss.maxNB = maxNB;
+    int ntask = 0;
+    for (int g = 0; g < (int)m.size(); g++) {
+      if (!active[g]) continue;
+      ss.sys[ss.count] = g;
+      ss.nb[ss.count] = m[g].plan->NB;
+      ss.off[ss.count] = maxNB - m[g].plan->NB;
+      ss.count++;
+      ntask += m[g].plan->NB;
+      hipLaunchKernelGGL((k_fill<int>), dim3((m[g].plan->NB + 255) / 256), dim3(256), 0, stream, m[g].done,
+                         (size_t)m[g].plan->NB, 0);
+    }
+    hipLaunchKernelGGL((k_fill<int>), dim3(1), dim3(64), 0, stream, ctl, (size_t)1, 0);
+    hipLaunchKernelGGL((k_solve_persist<R, C>), dim3(std::min(ntask, solve_wg)), dim3(NT), 0, stream,
+                       (const SysDev<R> *)d_sys, ss, kind, ctl, ctl + 1);
+    FDM_HIP(hipGetLastError());
+    return 0;
+  }
   for (int s = 0; s < maxNB; s++) {
     Items ipart{}, ifin{};
     for (int g = 0; g < (int)m.size(); g++) {
@@ -357,9 +390,30 @@ int Group<R, C>::solve(int kind) {
 template <class R, bool C>
 int Group<R, C>::read_fail(std::vector<int> &rows) {
   rows.assign(m.size(), 0);
+  int h[2] = {0, 0};
   for (size_t g = 0; g < m.size(); g++)
     FDM_HIP(hipMemcpyAsync(&rows[g], m[g].fail, sizeof(int), hipMemcpyDeviceToHost, stream));
+  if (ctl) FDM_HIP(hipMemcpyAsync(h, ctl, sizeof(int) * 2, hipMemcpyDeviceToHost, stream));
   FDM_HIP(hipStreamSynchronize(stream));
+  if (h[1]) {
+    set_error("persistent substitution: a dependency wait timed out");
+    (void)hipMemsetAsync(ctl + 1, 0, sizeof(int), stream);
+    return -2;
+  }
+  return 0;
+}
+
+template <class R, bool C>
+int Group<R, C>::check_solve() {
+  int h = 0;
+  if (!ctl) return 0;
+  FDM_HIP(hipMemcpyAsync(&h, ctl + 1, sizeof(int), hipMemcpyDeviceToHost, stream));
+  FDM_HIP(hipStreamSynchronize(stream));
+  if (h) {
+    set_error("persistent substitution: a dependency wait timed out");
+    (void)hipMemsetAsync(ctl + 1, 0, sizeof(int), stream);
+    return -2;
+  }
   return 0;
 }
 
