@@ -80,6 +80,36 @@ def _cpu_worker(wname, k, target_s):
                       f"rate scaled by that fraction")
 
 
+def traffic_from_profile():
+    """HBM bytes per k_trail launch measured by rocprofv3 PMC runs (FETCH_SIZE,
+    WRITE_SIZE) of this benchmark, scaled by the same counters on a plain
+    copy of known size; committed under profiles/.  None if absent."""
+    p = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic_k_trail_fe59_R5.0_mixed.json")
+    try:
+        with open(p) as f:
+            return json.load(f)["bytes_per_launch"]
+    except (OSError, ValueError, KeyError):
+        return None
+
+
+def roofline(sv32, slots32, vals, src, G):
+    import torch
+    sv32.options(use_graphs=False, time_updates=True)
+    grp = list(range(G))
+    xs = [torch.zeros_like(src) for _ in range(G)]
+    sv32.solve(slots32, [vals[i] for i in grp], [src] * G, xs)
+    st = sv32.stats()
+    upd_s = st["update_ms"] / 1e3
+    alg_flops = 8.0 * st["profile_macs"]  # profile multiply-adds of the G slots' factorisations
+    ach = alg_flops / upd_s / 1e12
+    roof = {"bound": "mfma", "achieved": ach, "peak": PEAK["fp32"], "unit": "TFLOP/s", "frac": ach / PEAK["fp32"],
+            "traffic": traffic_from_profile(), "kernel": "k_trail<float,complex> (trailing update: cross + group launches)",
+            "launches": st["update_launches"], "avg_launch_us": 1e6 * upd_s / max(st["update_launches"], 1),
+            "algorithmic_flops_per_launch": alg_flops / max(st["update_launches"], 1),
+            "executed_tile_flops": 8.0 * st["tile_macs"]}
+    return roof
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,6 +121,8 @@ def main():
     ap.add_argument("--cpu-procs", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--no-roofline", action="store_true", help="skip the per-launch timed pass (profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs shard energy points "
@@ -128,6 +160,11 @@ def main():
         sv = B.DeviceSolver(precision, True, s.shape[1])
         slots = [sv.add_system(lb1, lb2, rp, col) for _ in range(G)]
         xs = [torch.zeros_like(src) for _ in range(G)]
+        if not args.no_tune:  # schedule from the autotuner's cache, tuned on the first group if absent
+            from fdmnes_amd import autotune
+            ent = autotune.tune(sv, slots, [vals[i] for i in range(G)], [src] * G, xs, lb1, lb2, precision,
+                                log=log)
+            tuned[tag] = ent["best"]
         sig = np.zeros(n_pts)
         sweeps = []
         stream = torch.cuda.current_stream()
@@ -149,6 +186,7 @@ def main():
         log(f"{tag}: {K} steps x {G} points in {wall:.2f}s (events {gpu_ms / 1e3:.2f}s)")
         return sv, slots, xs, sig, wall, sweeps
 
+    tuned = {}
     # mixed precision (headline)
     sv32, slots32, _, sig32, wall32, sweeps32 = run(B.FP32, "mixed")
     st32 = sv32.stats()
@@ -172,19 +210,11 @@ def main():
         del sv64, xs64, grads
     # roofline of the dominant kernel: the FP32 trailing update, timed per
     # launch with HIP events on the solver's stream (one instrumented group)
-    sv32.options(use_graphs=False, time_updates=True)
-    grp = list(range(G))
-    xs = [torch.zeros_like(src) for _ in range(G)]
-    sv32.solve(slots32, [vals[i] for i in grp], [src] * G, xs)
-    st = sv32.stats()
-    upd_s = st["update_ms"] / 1e3
-    alg_flops = 8.0 * st["profile_macs"]  # profile multiply-adds of the G slots' factorisations
-    ach = alg_flops / upd_s / 1e12
-    roof = {"bound": "mfma", "achieved": ach, "peak": PEAK["fp32"], "unit": "TFLOP/s", "frac": ach / PEAK["fp32"],
-            "traffic": None, "kernel": "k_tile_gemm<float,complex> (trailing update)",
-            "launches": st["update_launches"], "avg_launch_us": 1e6 * upd_s / max(st["update_launches"], 1),
-            "algorithmic_flops_per_launch": alg_flops / max(st["update_launches"], 1),
-            "executed_tile_flops": 8.0 * st["tile_macs"]}
+    roof = None
+    if args.no_roofline:
+        pass
+    else:
+        roof = roofline(sv32, slots32, vals, src, G)
     del sv32
     cpu = None
     if not args.no_cpu:
@@ -198,7 +228,7 @@ def main():
         "config": {"workload": w.name, "nligne": cl.nligne, "nrhs": int(s.shape[1]), "points_per_step": G,
                    "energies_eV": [float(e * cfg.RYDB) for e in energies[idx[W * G:]]][:8],
                    "factor_bytes_per_point": st32["factor_bytes"] / G, "profile_cmacs_per_point": st32["profile_macs"] / G},
-        "refine_sweeps": sweeps32[W * G:],
+        "refine_sweeps": sweeps32[W * G:], "schedule": tuned,
         "roofline": roof, "cpu_baseline": cpu,
     }
     res.update(out)

@@ -15,10 +15,12 @@ import numpy as np
 CACHE = os.environ.get("FDMGPU_TUNE_CACHE", os.path.join(os.path.dirname(os.path.abspath(__file__)),
                                                          "tune_cache.json"))
 CANDIDATES = [
-    {"gk": 2, "trail_wg": 512, "solve_wg": 512},
-    {"gk": 4, "trail_wg": 512, "solve_wg": 512},
-    {"gk": 4, "trail_wg": 1024, "solve_wg": 512},
+    {"gk": 4, "trail_wg": 768, "solve_wg": 512},
+    {"gk": 8, "trail_wg": 512, "solve_wg": 512},
+    {"gk": 8, "trail_wg": 768, "solve_wg": 512},
     {"gk": 8, "trail_wg": 1024, "solve_wg": 512},
+    {"gk": 12, "trail_wg": 768, "solve_wg": 512},
+    {"gk": 16, "trail_wg": 768, "solve_wg": 512},
 ]
 
 
