@@ -15,13 +15,20 @@ import numpy as np
 CACHE = os.environ.get("FDMGPU_TUNE_CACHE", os.path.join(os.path.dirname(os.path.abspath(__file__)),
                                                          "tune_cache.json"))
 CANDIDATES = [
-    {"gk": 4, "trail_wg": 768, "solve_wg": 512},
-    {"gk": 8, "trail_wg": 512, "solve_wg": 512},
-    {"gk": 8, "trail_wg": 768, "solve_wg": 512},
-    {"gk": 8, "trail_wg": 1024, "solve_wg": 512},
-    {"gk": 12, "trail_wg": 768, "solve_wg": 512},
-    {"gk": 16, "trail_wg": 768, "solve_wg": 512},
+    {"gk": 8, "trail_wg": 768, "solve_wg": 512, "trail_tile": 64},
+    {"gk": 8, "trail_wg": 1024, "solve_wg": 512, "trail_tile": 64},
+    {"gk": 12, "trail_wg": 768, "solve_wg": 512, "trail_tile": 64},
+    {"gk": 8, "trail_wg": 256, "solve_wg": 512, "trail_tile": 128},
+    {"gk": 8, "trail_wg": 512, "solve_wg": 512, "trail_tile": 128},
+    {"gk": 12, "trail_wg": 256, "solve_wg": 512, "trail_tile": 128},
+    {"gk": 12, "trail_wg": 512, "solve_wg": 512, "trail_tile": 128},
+    {"gk": 16, "trail_wg": 512, "solve_wg": 512, "trail_tile": 128},
 ]
+KNOBS = ("gk", "trail_wg", "solve_wg", "trail_tile")
+
+
+def apply(solver, c: dict) -> None:
+    solver.tune(c["gk"], c["trail_wg"], 1, c["solve_wg"], c.get("trail_tile", 64))
 
 
 def key(lb1, lb2, precision: int, group: int) -> str:
@@ -45,12 +52,13 @@ def tune(solver, slots, vals, b, x, lb1, lb2, precision: int, candidates=CANDIDA
     k = key(lb1, lb2, precision, len(slots))
     cache = load()
     if k in cache:
-        best = cache[k]["best"]
-        solver.tune(best["gk"], best["trail_wg"], 1, best["solve_wg"])
+        apply(solver, cache[k]["best"])
         return cache[k]
     results = []
+    # the FP64 macro-tile kernel spills (its accumulators do not fit the register file)
+    candidates = [c for c in candidates if precision != 0 or c.get("trail_tile", 64) == 64]
     for c in candidates:
-        solver.tune(c["gk"], c["trail_wg"], 1, c["solve_wg"])
+        apply(solver, c)
         solver.solve(slots, vals, b, x)  # warm-up: builds this schedule's graphs
         torch.cuda.synchronize()
         t0 = time.time()
@@ -62,8 +70,8 @@ def tune(solver, slots, vals, b, x, lb1, lb2, precision: int, candidates=CANDIDA
         if log:
             log(f"autotune {c}: {dt:.3f}s")
     best = min(results, key=lambda r: r["seconds"])
-    solver.tune(best["gk"], best["trail_wg"], 1, best["solve_wg"])
-    entry = {"best": {k2: best[k2] for k2 in ("gk", "trail_wg", "solve_wg")}, "trials": results}
+    apply(solver, best)
+    entry = {"best": {k2: best[k2] for k2 in KNOBS}, "trials": results}
     cache[k] = entry
     try:
         with open(CACHE, "w") as f:

@@ -295,10 +295,14 @@ int fdmgpu_solver_options(fdmgpu_solver *s, int use_graphs, int time_updates, in
   return 0;
 }
 
-int fdmgpu_solver_tune(fdmgpu_solver *s, int gk, int trail_wg, int persist_solve, int solve_wg) {
-  if (!s || gk > 16) return FDMGPU_EINVAL;
+int fdmgpu_solver_tune(fdmgpu_solver *s, int gk, int trail_wg, int persist_solve, int solve_wg, int trail_tile) {
+  if (!s || gk > 16 || (trail_tile > 0 && trail_tile != 64 && trail_tile != 128)) {
+    set_error("fdmgpu_solver_tune: gk must be <= 16 and trail_tile 64 or 128");
+    return FDMGPU_EINVAL;
+  }
   auto apply = [&](auto *p) {
     if (gk >= 0) p->grp.gk = gk;
+    if (trail_tile > 0) p->grp.trail_tile = trail_tile;
     if (trail_wg > 0) p->grp.persist_wg = trail_wg;
     if (persist_solve >= 0) p->grp.persist_solve = persist_solve != 0;
     if (solve_wg > 0) p->grp.solve_wg = solve_wg;

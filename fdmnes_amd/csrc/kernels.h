@@ -17,6 +17,8 @@
 // serves the item g with off[g] <= b < off[g+1].
 #pragma once
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 #include <stdint.h>
 
 namespace fdmgpu {
@@ -25,6 +27,17 @@ constexpr int TB = 64;    // tile edge = panel width
 constexpr int NT = 256;   // threads per workgroup
 constexpr int RC = 16;    // right-hand sides per solve chunk
 constexpr int MAXG = 16;  // systems per grouped launch
+
+// Device pointers held in structs are generic (flat) to the compiler, and
+// flat loads count in lgkmcnt as well as vmcnt, so a wait for LDS results
+// would also wait for every global load in flight.  Kernels access them
+// through global-address-space views (global_* instructions, vmcnt only).
+template <class T>
+using gptr = __attribute__((address_space(1))) T *;
+template <class T>
+__device__ __forceinline__ gptr<T> G(T *p) {
+  return (gptr<T>)p;
+}
 
 struct DevPlan {
   int n;   // real rows
@@ -65,8 +78,8 @@ template <class R> struct Elem<R, true> { static constexpr int NC = 2; };
 template <class R> struct Elem<R, false> { static constexpr int NC = 1; };
 
 template <class R, bool C>
-__device__ __forceinline__ R *tile_ptr(const DevPlan &p, R *tiles, int I, int J) {
-  return tiles + (p.tbase[I] + (J - p.J0[I])) * (int64_t)(TB * TB * Elem<R, C>::NC);
+__device__ __forceinline__ gptr<R> tile_ptr(const DevPlan &p, R *tiles, int I, int J) {
+  return G(tiles) + (G(p.tbase)[I] + (J - G(p.J0)[I])) * (int64_t)(TB * TB * Elem<R, C>::NC);
 }
 
 // ---------------------------------------------------------------- MFMA
@@ -92,12 +105,12 @@ template <> struct Mfma<float> {
 enum GemmMode { G_UPDATE = 0, G_ROWTRSM = 1, G_COLTRSM = 2 };
 
 // Stage one TB x TB tile (global, interleaved) into split LDS planes.
-template <class R, bool C, int LD>
-__device__ __forceinline__ void stage_tile(const R *__restrict__ g, R (*s)[TB][LD]) {
+template <class R, bool C, int LD, class P>
+__device__ __forceinline__ void stage_tile(P g, R (*s)[TB][LD]) {
   const int tid = threadIdx.x;
   if constexpr (C) {
     typedef R r2 __attribute__((ext_vector_type(2)));
-    const r2 *g2 = reinterpret_cast<const r2 *>(g);
+    const gptr<const r2> g2 = (gptr<const r2>)g;
 #pragma unroll
     for (int q = 0; q < TB * TB / NT; q++) {
       int e = tid + q * NT;
@@ -137,22 +150,22 @@ __global__ __launch_bounds__(NT) void k_tile_gemm(const SysDev<R> *__restrict__ 
   const int K = it.K[g];
   const int I0 = p.I0[K], J0 = p.J0[K];
   const int nJ = K - J0;
-  const R *A;
-  const R *B;
-  R *Cg;
+  gptr<const R> A;
+  gptr<const R> B;
+  gptr<R> Cg;
   if (mode == G_UPDATE) {
     const int I = I0 + b / nJ, J = J0 + b % nJ;
     A = tile_ptr<R, C>(p, S.tiles, I, K);
     B = tile_ptr<R, C>(p, S.tiles, K, J);
     Cg = tile_ptr<R, C>(p, S.tiles, I, J);
   } else if (mode == G_ROWTRSM) {
-    A = S.dinv + (int64_t)K * 2 * TB * TB * NC;  // Uinv
+    A = G(S.dinv) + (int64_t)K * 2 * TB * TB * NC;  // Uinv
     Cg = tile_ptr<R, C>(p, S.tiles, K, J0 + b);
     B = Cg;
   } else {
     Cg = tile_ptr<R, C>(p, S.tiles, I0 + b, K);
     A = Cg;
-    B = S.dinv + ((int64_t)K * 2 + 1) * TB * TB * NC;  // Linv
+    B = G(S.dinv) + ((int64_t)K * 2 + 1) * TB * TB * NC;  // Linv
   }
   stage_tile<R, C, LD>(A, As);
   stage_tile<R, C, LD>(B, Bs);
@@ -229,7 +242,9 @@ struct TrailItems {
   int i0[MAXT];
   int j0[MAXT];
   int nj[MAXT];
-  int off[MAXT + 1];
+  int ni[MAXT];
+  int off[MAXT + 1];   // tile offsets (k_trail)
+  int moff[MAXT + 1];  // 2 x 2 macro-tile offsets (k_trail2)
 };
 
 __device__ __forceinline__ int find_trail(const TrailItems &it, int b) {
@@ -246,13 +261,24 @@ constexpr int KC = 16;  // k-chunk of the LDS pipeline
 // flight while the current chunk feeds the MFMAs.  Four waves, each a 32x32
 // quadrant in 2x2 blocks of 16x16x4 MFMAs.  LDS holds A chunks k-major (As[k][row]) so the MFMA A operand reads
 // 16 consecutive rows per k.
+//
+// Work order: dispatch places workgroup w on XCD w % NXCD, and each XCD has
+// its own L2.  The work list is cut into NXCD contiguous ranges, one per XCD
+// (the grid is a multiple of NXCD), and inside an item the tiles run in bands
+// of GM block rows, column by column, so the workgroups resident on one XCD
+// at a time cover a compact GM x (resident/GM) patch and share its U row
+// tiles and L column tiles in their L2 instead of each fetching them again.
+constexpr int NXCD = 8;
+constexpr int GM = 8;
 template <class R, bool C>
 __global__ __launch_bounds__(NT, 2) void k_trail(const SysDev<R> *__restrict__ sys, TrailItems it) {
   constexpr int NC = Elem<R, C>::NC;
   constexpr int LD = TB + 1;
   constexpr int PER = TB * KC / NT;  // elements per thread per chunk and operand (4)
+  constexpr int64_t TE = (int64_t)TB * TB * NC;  // elements per tile
   __shared__ R As[2][NC][KC][LD];
   __shared__ R Bs[2][NC][KC][LD];
+  __shared__ int64_t boff[16];  // per step of the item: offset of tile (K, J)
   typedef typename Mfma<R>::v4 v4;
   typedef R r2 __attribute__((ext_vector_type(2)));
   const int tid = threadIdx.x;
@@ -264,15 +290,30 @@ __global__ __launch_bounds__(NT, 2) void k_trail(const SysDev<R> *__restrict__ s
   // the FP64 path keeps four so its rounding matches an ordinary product.
   constexpr bool M3 = C && sizeof(R) == 4;
   const int total = it.off[it.count];
-  for (int wb = blockIdx.x; wb < total; wb += gridDim.x) {
+  const int per = (total + NXCD - 1) / NXCD, xcd = blockIdx.x % NXCD, lanes = gridDim.x / NXCD;
+  for (int jw = blockIdx.x / NXCD; jw < per; jw += lanes) {
+    const int wb = xcd * per + jw;
+    if (wb >= total) break;
     const int g = find_trail(it, wb);
     const int b = wb - it.off[g];
     const SysDev<R> &S = sys[it.sys[g]];
     const DevPlan &p = S.p;
-    const int I = it.i0[g] + b / it.nj[g], J = it.j0[g] + b % it.nj[g];
+    const int nj = it.nj[g], ni = (it.off[g + 1] - it.off[g]) / nj;
+    const int band = b / (GM * nj), hr = min(GM, ni - band * GM), q = b - band * GM * nj;
+    const int I = it.i0[g] + band * GM + q % hr, J = it.j0[g] + q / hr;
+    const int Kb = it.Kb[g];
+    const gptr<const int> I0g = G(p.I0), J0g = G(p.J0);
+    const gptr<const int64_t> tbg = G(p.tbase);
     int m = 0;
-    while (m < it.nk[g] && p.I0[it.Kb[g] + m] <= I && p.J0[it.Kb[g] + m] <= J) m++;
+    while (m < it.nk[g] && I0g[Kb + m] <= I && J0g[Kb + m] <= J) m++;
     const int nch = m * (TB / KC);
+    // Tile addresses of the item, outside the chunk loop: A = tile(I, K) are
+    // consecutive tiles of block row I; the offsets of B = tile(K, J) go to LDS.
+    if (jw != (int)blockIdx.x / NXCD) __syncthreads();  // previous item done with LDS
+    if (tid < m) boff[tid] = (tbg[Kb + tid] + (J - J0g[Kb + tid])) * TE;
+    const gptr<const R> Ab = G(S.tiles) + (tbg[I] + (Kb - J0g[I])) * TE;
+    const gptr<const R> tiles = G(S.tiles);
+    __syncthreads();
     constexpr int NX = M3 ? 2 : 1;  // P2 accumulators only for the 3M form
     v4 accr[2][2], acci[2][2], accx[NX][NX];
 #pragma unroll
@@ -286,35 +327,243 @@ __global__ __launch_bounds__(NT, 2) void k_trail(const SysDev<R> *__restrict__ s
     for (int a = 0; a < NX; a++)
 #pragma unroll
       for (int bb = 0; bb < NX; bb++) accx[a][bb] = v4{0, 0, 0, 0};
-    // chunk c: step K = Kb + c/4, columns kc..kc+15 of A = tile(I,K), rows of B = tile(K,J)
-    r2 pa[PER], pb[PER];
-    R qa[PER], qb[PER];
-    auto fetch = [&](int c) {
-      const int K = it.Kb[g] + c / (TB / KC), kc = (c % (TB / KC)) * KC;
-      const R *A = tile_ptr<R, C>(p, S.tiles, I, K);
-      const R *B = tile_ptr<R, C>(p, S.tiles, K, J);
+    // chunk c: step K = Kb + c/4, columns kc..kc+15 of A = tile(I,K), rows of B = tile(K,J).
+    // DEPTH register sets keep that many chunks' loads in flight while the
+    // MFMAs consume the LDS copy of the current one (two for FP32, whose
+    // register budget allows it; one for FP64).
+    constexpr int DEPTH = sizeof(R) == 4 ? 2 : 1;
+    r2 pa[DEPTH][PER], pb[DEPTH][PER];
+    R qa[DEPTH][PER], qb[DEPTH][PER];
+    auto fetch = [&](int c, auto set) {
+      constexpr int s = decltype(set)::value;
+      const int st = c / (TB / KC), kc = (c % (TB / KC)) * KC;
+      const gptr<const R> A = Ab + st * TE;
+      const gptr<const R> B = tiles + boff[st];
 #pragma unroll
       for (int q = 0; q < PER; q++) {
         const int e = tid + q * NT;
         const int ar = e / KC, ak = e % KC;  // A: 64 rows x 16 k
         const int bk = e / TB, bc = e % TB;  // B: 16 k x 64 cols
         if constexpr (C) {
-          pa[q] = reinterpret_cast<const r2 *>(A)[ar * TB + kc + ak];
-          pb[q] = reinterpret_cast<const r2 *>(B)[(kc + bk) * TB + bc];
+          pa[s][q] = ((gptr<const r2>)A)[ar * TB + kc + ak];
+          pb[s][q] = ((gptr<const r2>)B)[(kc + bk) * TB + bc];
         } else {
-          qa[q] = A[ar * TB + kc + ak];
-          qb[q] = B[(kc + bk) * TB + bc];
+          qa[s][q] = A[ar * TB + kc + ak];
+          qb[s][q] = B[(kc + bk) * TB + bc];
         }
       }
     };
-    if (wb != (int)blockIdx.x) __syncthreads();  // previous item done with LDS
-    if (nch > 0) fetch(0);
-    for (int c = 0; c < nch; c++) {
+    auto step = [&](int c, auto set) {
+      constexpr int s = decltype(set)::value;
       const int buf = c & 1;
 #pragma unroll
       for (int q = 0; q < PER; q++) {
         const int e = tid + q * NT;
         const int ar = e / KC, ak = e % KC, bk = e / TB, bc = e % TB;
+        if constexpr (C) {
+          As[buf][0][ak][ar] = pa[s][q].x;
+          As[buf][NC - 1][ak][ar] = pa[s][q].y;
+          Bs[buf][0][bk][bc] = pb[s][q].x;
+          Bs[buf][NC - 1][bk][bc] = pb[s][q].y;
+        } else {
+          As[buf][0][ak][ar] = qa[s][q];
+          Bs[buf][0][bk][bc] = qb[s][q];
+        }
+      }
+      __syncthreads();
+      if (c + DEPTH < nch) fetch(c + DEPTH, set);
+#pragma unroll
+        for (int k4 = 0; k4 < KC; k4 += 4) {
+          R ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+          for (int a = 0; a < 2; a++) {
+            ar[a] = As[buf][0][k4 + lk][r0 + a * 16 + li];
+            if constexpr (C) ai[a] = As[buf][NC - 1][k4 + lk][r0 + a * 16 + li];
+          }
+#pragma unroll
+          for (int bb = 0; bb < 2; bb++) {
+            br[bb] = Bs[buf][0][k4 + lk][c0 + bb * 16 + li];
+            if constexpr (C) bi[bb] = Bs[buf][NC - 1][k4 + lk][c0 + bb * 16 + li];
+          }
+          if constexpr (M3) {
+            R as[2], bs[2];
+#pragma unroll
+            for (int a = 0; a < 2; a++) as[a] = ar[a] + ai[a];
+#pragma unroll
+            for (int bb = 0; bb < 2; bb++) bs[bb] = br[bb] + bi[bb];
+#pragma unroll
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+              for (int bb = 0; bb < 2; bb++) {
+                accr[a][bb] = Mfma<R>::run(ar[a], br[bb], accr[a][bb]);   // P1
+                accx[a % NX][bb % NX] = Mfma<R>::run(ai[a], bi[bb], accx[a % NX][bb % NX]);  // P2
+                acci[a][bb] = Mfma<R>::run(as[a], bs[bb], acci[a][bb]);   // P3
+              }
+          } else {
+#pragma unroll
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+              for (int bb = 0; bb < 2; bb++) {
+                accr[a][bb] = Mfma<R>::run(ar[a], br[bb], accr[a][bb]);
+                if constexpr (C) {
+                  accr[a][bb] = Mfma<R>::run(-ai[a], bi[bb], accr[a][bb]);
+                  acci[a][bb] = Mfma<R>::run(ar[a], bi[bb], acci[a][bb]);
+                  acci[a][bb] = Mfma<R>::run(ai[a], br[bb], acci[a][bb]);
+                }
+              }
+          }
+        }
+    };
+    using S0 = std::integral_constant<int, 0>;
+    using S1 = std::integral_constant<int, DEPTH - 1>;
+    if (nch > 0) fetch(0, S0{});
+    if constexpr (DEPTH == 2)
+      if (nch > 1) fetch(1, S1{});
+    for (int c = 0; c < nch; c += DEPTH) {
+      step(c, S0{});
+      if constexpr (DEPTH == 2)
+        if (c + 1 < nch) step(c + 1, S1{});
+    }
+    if constexpr (M3) {
+#pragma unroll
+      for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int bb = 0; bb < 2; bb++) {
+          acci[a][bb] = acci[a][bb] - accr[a][bb] - accx[a % NX][bb % NX];
+          accr[a][bb] = accr[a][bb] - accx[a % NX][bb % NX];
+        }
+    }
+    if (nch > 0) {
+      // all 16 loads of the read-modify-write first, then the stores
+      const gptr<R> Cg = tile_ptr<R, C>(p, S.tiles, I, J);
+      R cr[2][2][4], ci[2][2][4];
+#pragma unroll
+      for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int bb = 0; bb < 2; bb++)
+#pragma unroll
+          for (int v = 0; v < 4; v++) {
+            const int r = r0 + a * 16 + Mfma<R>::row(lk, v), cc = c0 + bb * 16 + li;
+            if constexpr (C) {
+              const r2 t = ((gptr<r2>)Cg)[r * TB + cc];
+              cr[a][bb][v] = t.x;
+              ci[a][bb][v] = t.y;
+            } else {
+              cr[a][bb][v] = Cg[r * TB + cc];
+            }
+          }
+#pragma unroll
+      for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int bb = 0; bb < 2; bb++)
+#pragma unroll
+          for (int v = 0; v < 4; v++) {
+            const int r = r0 + a * 16 + Mfma<R>::row(lk, v), cc = c0 + bb * 16 + li;
+            if constexpr (C)
+              ((gptr<r2>)Cg)[r * TB + cc] = r2{cr[a][bb][v] - accr[a][bb][v], ci[a][bb][v] - acci[a][bb][v]};
+            else
+              Cg[r * TB + cc] = cr[a][bb][v] - accr[a][bb][v];
+          }
+    }
+  }
+}
+
+// Macro-tiled trailing update: a workgroup of 8 waves updates a 2 x 2 block
+// of storage tiles (128 x 128 elements), so every U row chunk and L column
+// chunk staged in LDS feeds twice the MFMA work of the one-tile kernel and
+// the L2 traffic per flop halves.  Wave w owns the 32 x 64 patch at macro
+// rows 32*(w/2), columns 64*(w%2): 2 x 4 blocks of 16x16x4 MFMAs.  Items
+// count macro tiles (moff); at an item's last odd row or column the missing
+// storage tiles are treated as zero.  Within the macro tile the steps K of
+// the range that reach a storage tile (I0(K) <= I, J0(K) <= J) differ by
+// tile, so the chunk loads zero the rows of tiles (I+a, K) with I0(K) > I+a
+// and the columns of tiles (K, J+b) with J0(K) > J+b: the workgroup runs the
+// longest prefix and every tile receives exactly its own steps.
+constexpr int NT2 = 2 * NT;
+constexpr int GM2 = GM / 2;  // macro rows per band of the in-XCD order
+
+template <class R, bool C>
+__global__ __launch_bounds__(NT2, 1) void k_trail2(const SysDev<R> *__restrict__ sys, TrailItems it) {
+  constexpr int NC = Elem<R, C>::NC;
+  constexpr int MT = 2 * TB;
+  constexpr int LD = MT + 1;
+  constexpr int PER = MT * KC / NT2;  // elements per thread per chunk and operand (4)
+  __shared__ R As[2][NC][KC][LD];
+  __shared__ R Bs[2][NC][KC][LD];
+  typedef typename Mfma<R>::v4 v4;
+  typedef R r2 __attribute__((ext_vector_type(2)));
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int r0 = (wave >> 1) * 32, c0 = (wave & 1) * 64;
+  const int li = lane & 15, lk = lane >> 4;
+  constexpr bool M3 = C && sizeof(R) == 4;
+  constexpr int NX = M3 ? 4 : 1;
+  const int total = it.moff[it.count];
+  const int per = (total + NXCD - 1) / NXCD, xcd = blockIdx.x % NXCD, lanes = gridDim.x / NXCD;
+  for (int jw = blockIdx.x / NXCD; jw < per; jw += lanes) {
+    const int wb = xcd * per + jw;
+    if (wb >= total) break;
+    int g = 0;
+    while (g + 1 < it.count && wb >= it.moff[g + 1]) g++;
+    const int b = wb - it.moff[g];
+    const SysDev<R> &S = sys[it.sys[g]];
+    const DevPlan &p = S.p;
+    const int ni = it.ni[g], nj = it.nj[g];
+    const int mi = (ni + 1) >> 1, mj = (nj + 1) >> 1;
+    const int band = b / (GM2 * mj), hr = min(GM2, mi - band * GM2), q = b - band * GM2 * mj;
+    const int ti = band * GM2 + q % hr, tj = q / hr;  // macro tile within the item
+    const int I = it.i0[g] + 2 * ti, J = it.j0[g] + 2 * tj;
+    const int rows_in = min(2, ni - 2 * ti), cols_in = min(2, nj - 2 * tj);
+    const int Il = I + rows_in - 1, Jl = J + cols_in - 1;  // last storage row / column present
+    int m = 0;
+    while (m < it.nk[g] && p.I0[it.Kb[g] + m] <= Il && p.J0[it.Kb[g] + m] <= Jl) m++;
+    const int nch = m * (TB / KC);
+    v4 accr[2][4], acci[2][4], accx[M3 ? 2 : 1][NX];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int bb = 0; bb < 4; bb++) {
+        accr[a][bb] = v4{0, 0, 0, 0};
+        acci[a][bb] = v4{0, 0, 0, 0};
+      }
+#pragma unroll
+    for (int a = 0; a < (M3 ? 2 : 1); a++)
+#pragma unroll
+      for (int bb = 0; bb < NX; bb++) accx[a][bb] = v4{0, 0, 0, 0};
+    // chunk c: step K = Kb + c/4, k columns kc..kc+15.  Thread element e =
+    // tid + q*NT2: A row e/KC (storage tile I + row/64), B column e%MT
+    // (storage tile J + col/64).
+    r2 pa[PER], pb[PER];
+    R qa[PER], qb[PER];
+    auto fetch = [&](int c) {
+      const int K = it.Kb[g] + c / (TB / KC), kc = (c % (TB / KC)) * KC;
+#pragma unroll
+      for (int q = 0; q < PER; q++) {
+        const int e = tid + q * NT2;
+        const int ar = e / KC, ak = e % KC;
+        const int bk = e / MT, bc = e % MT;
+        const int ta = ar / TB, tb = bc / TB;
+        const bool oka = ta < rows_in && p.I0[K] <= I + ta;
+        const bool okb = tb < cols_in && p.J0[K] <= J + tb;
+        if constexpr (C) {
+          pa[q] = oka ? ((gptr<const r2>)tile_ptr<R, C>(p, S.tiles, I + ta, K))[(ar % TB) * TB + kc + ak]
+                      : r2{0, 0};
+          pb[q] = okb ? ((gptr<const r2>)tile_ptr<R, C>(p, S.tiles, K, J + tb))[(kc + bk) * TB + bc % TB]
+                      : r2{0, 0};
+        } else {
+          qa[q] = oka ? tile_ptr<R, C>(p, S.tiles, I + ta, K)[(ar % TB) * TB + kc + ak] : R(0);
+          qb[q] = okb ? tile_ptr<R, C>(p, S.tiles, K, J + tb)[(kc + bk) * TB + bc % TB] : R(0);
+        }
+      }
+    };
+    if (jw != (int)blockIdx.x / NXCD) __syncthreads();  // previous item done with LDS
+    if (nch > 0) fetch(0);
+    for (int c = 0; c < nch; c++) {
+      const int buf = c & 1;
+#pragma unroll
+      for (int q = 0; q < PER; q++) {
+        const int e = tid + q * NT2;
+        const int ar = e / KC, ak = e % KC, bk = e / MT, bc = e % MT;
         if constexpr (C) {
           As[buf][0][ak][ar] = pa[q].x;
           As[buf][NC - 1][ak][ar] = pa[q].y;
@@ -329,36 +578,36 @@ __global__ __launch_bounds__(NT, 2) void k_trail(const SysDev<R> *__restrict__ s
       if (c + 1 < nch) fetch(c + 1);
 #pragma unroll
       for (int k4 = 0; k4 < KC; k4 += 4) {
-        R ar[2], ai[2], br[2], bi[2];
+        R ar[2], ai[2], br[4], bi[4];
 #pragma unroll
         for (int a = 0; a < 2; a++) {
           ar[a] = As[buf][0][k4 + lk][r0 + a * 16 + li];
           if constexpr (C) ai[a] = As[buf][NC - 1][k4 + lk][r0 + a * 16 + li];
         }
 #pragma unroll
-        for (int bb = 0; bb < 2; bb++) {
+        for (int bb = 0; bb < 4; bb++) {
           br[bb] = Bs[buf][0][k4 + lk][c0 + bb * 16 + li];
           if constexpr (C) bi[bb] = Bs[buf][NC - 1][k4 + lk][c0 + bb * 16 + li];
         }
         if constexpr (M3) {
-          R as[2], bs[2];
+          R as[2], bs[4];
 #pragma unroll
           for (int a = 0; a < 2; a++) as[a] = ar[a] + ai[a];
 #pragma unroll
-          for (int bb = 0; bb < 2; bb++) bs[bb] = br[bb] + bi[bb];
+          for (int bb = 0; bb < 4; bb++) bs[bb] = br[bb] + bi[bb];
 #pragma unroll
           for (int a = 0; a < 2; a++)
 #pragma unroll
-            for (int bb = 0; bb < 2; bb++) {
-              accr[a][bb] = Mfma<R>::run(ar[a], br[bb], accr[a][bb]);   // P1
-              accx[a % NX][bb % NX] = Mfma<R>::run(ai[a], bi[bb], accx[a % NX][bb % NX]);  // P2
-              acci[a][bb] = Mfma<R>::run(as[a], bs[bb], acci[a][bb]);   // P3
+            for (int bb = 0; bb < 4; bb++) {
+              accr[a][bb] = Mfma<R>::run(ar[a], br[bb], accr[a][bb]);                           // P1
+              accx[a % 2][bb % NX] = Mfma<R>::run(ai[a], bi[bb], accx[a % 2][bb % NX]);  // P2
+              acci[a][bb] = Mfma<R>::run(as[a], bs[bb], acci[a][bb]);                           // P3
             }
         } else {
 #pragma unroll
           for (int a = 0; a < 2; a++)
 #pragma unroll
-            for (int bb = 0; bb < 2; bb++) {
+            for (int bb = 0; bb < 4; bb++) {
               accr[a][bb] = Mfma<R>::run(ar[a], br[bb], accr[a][bb]);
               if constexpr (C) {
                 accr[a][bb] = Mfma<R>::run(-ai[a], bi[bb], accr[a][bb]);
@@ -373,23 +622,42 @@ __global__ __launch_bounds__(NT, 2) void k_trail(const SysDev<R> *__restrict__ s
 #pragma unroll
       for (int a = 0; a < 2; a++)
 #pragma unroll
-        for (int bb = 0; bb < 2; bb++) {
-          acci[a][bb] = acci[a][bb] - accr[a][bb] - accx[a % NX][bb % NX];
-          accr[a][bb] = accr[a][bb] - accx[a % NX][bb % NX];
+        for (int bb = 0; bb < 4; bb++) {
+          acci[a][bb] = acci[a][bb] - accr[a][bb] - accx[a % 2][bb % NX];
+          accr[a][bb] = accr[a][bb] - accx[a % 2][bb % NX];
         }
     }
-    if (nch > 0) {
-      R *Cg = tile_ptr<R, C>(p, S.tiles, I, J);
+    // this wave's patch lies in storage tile (I + r0/64, J + c0/64)
+    const int ta = r0 / TB, tb = c0 / TB;
+    if (nch > 0 && ta < rows_in && tb < cols_in) {
+      const gptr<R> Cg = tile_ptr<R, C>(p, S.tiles, I + ta, J + tb);
+      R cr[2][4][4], ci[2][4][4];
 #pragma unroll
       for (int a = 0; a < 2; a++)
 #pragma unroll
-        for (int bb = 0; bb < 2; bb++)
+        for (int bb = 0; bb < 4; bb++)
 #pragma unroll
           for (int v = 0; v < 4; v++) {
-            const int r = r0 + a * 16 + Mfma<R>::row(lk, v), cc = c0 + bb * 16 + li;
-            const int64_t e = (int64_t)(r * TB + cc) * NC;
-            Cg[e] -= accr[a][bb][v];
-            if constexpr (C) Cg[e + 1] -= acci[a][bb][v];
+            const int r = (r0 % TB) + a * 16 + Mfma<R>::row(lk, v), cc = bb * 16 + li;
+            if constexpr (C) {
+              const r2 t = ((gptr<r2>)Cg)[r * TB + cc];
+              cr[a][bb][v] = t.x;
+              ci[a][bb][v] = t.y;
+            } else {
+              cr[a][bb][v] = Cg[r * TB + cc];
+            }
+          }
+#pragma unroll
+      for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int bb = 0; bb < 4; bb++)
+#pragma unroll
+          for (int v = 0; v < 4; v++) {
+            const int r = (r0 % TB) + a * 16 + Mfma<R>::row(lk, v), cc = bb * 16 + li;
+            if constexpr (C)
+              ((gptr<r2>)Cg)[r * TB + cc] = r2{cr[a][bb][v] - accr[a][bb][v], ci[a][bb][v] - acci[a][bb][v]};
+            else
+              Cg[r * TB + cc] = cr[a][bb][v] - accr[a][bb][v];
           }
     }
   }
@@ -426,7 +694,7 @@ __global__ __launch_bounds__(NT) void k_panel(const SysDev<R> *__restrict__ sys,
   const SysDev<R> &S = sys[it.sys[blockIdx.x]];
   const DevPlan &p = S.p;
   const int K = it.K[blockIdx.x];
-  R *T = tile_ptr<R, C>(p, S.tiles, K, K);
+  const gptr<R> T = tile_ptr<R, C>(p, S.tiles, K, K);
   stage_tile<R, C, LD>(T, D);
   __syncthreads();
   const int myr = tid >> 2, part = tid & 3;
@@ -523,7 +791,7 @@ __global__ __launch_bounds__(NT) void k_panel(const SysDev<R> *__restrict__ sys,
     }
   }
   __syncthreads();
-  R *Ui = S.dinv + (int64_t)K * 2 * TB * TB * NC;
+  const gptr<R> Ui = G(S.dinv) + (int64_t)K * 2 * TB * TB * NC;
   for (int e = tid; e < TB * TB; e += NT) {
     Ui[(int64_t)e * NC] = X[0][e / TB][e % TB];
     if constexpr (C) Ui[(int64_t)e * NC + 1] = X[NC - 1][e / TB][e % TB];
@@ -553,7 +821,7 @@ __global__ __launch_bounds__(NT) void k_panel(const SysDev<R> *__restrict__ sys,
     }
   }
   __syncthreads();
-  R *Li = S.dinv + ((int64_t)K * 2 + 1) * TB * TB * NC;
+  const gptr<R> Li = G(S.dinv) + ((int64_t)K * 2 + 1) * TB * TB * NC;
   for (int e = tid; e < TB * TB; e += NT) {
     Li[(int64_t)e * NC] = X[0][e / TB][e % TB];
     if constexpr (C) Li[(int64_t)e * NC + 1] = X[NC - 1][e / TB][e % TB];
@@ -603,15 +871,15 @@ __global__ __launch_bounds__(NT) void k_solve_partial(const SysDev<R> *__restric
   const bool trans = (kind == S_LOWER_T || kind == S_UPPER_T);
   R accr[4] = {0, 0, 0, 0}, acci[4] = {0, 0, 0, 0};
   for (int J = t0; J < t1; J++) {
-    const R *T = trans ? tile_ptr<R, C>(p, S.tiles, J, K) : tile_ptr<R, C>(p, S.tiles, K, J);
+    const gptr<const R> T = trans ? tile_ptr<R, C>(p, S.tiles, J, K) : tile_ptr<R, C>(p, S.tiles, K, J);
     __syncthreads();
     stage_tile<R, C, LD>(T, Ts);
     for (int e = tid; e < TB * RC; e += NT) {
       const int rr = e / RC, cc = e % RC;
       const int64_t gi = ((int64_t)(J * TB + rr) * nrhs + rc0 + cc) * NC;
       const bool ok = rc0 + cc < nrhs;
-      Xs[0][rr][cc] = ok ? S.X[gi] : R(0);
-      if constexpr (C) Xs[NC - 1][rr][cc] = ok ? S.X[gi + 1] : R(0);
+      Xs[0][rr][cc] = ok ? G(S.X)[gi] : R(0);
+      if constexpr (C) Xs[NC - 1][rr][cc] = ok ? G(S.X)[gi + 1] : R(0);
     }
     __syncthreads();
 #pragma unroll 4
@@ -632,7 +900,7 @@ __global__ __launch_bounds__(NT) void k_solve_partial(const SysDev<R> *__restric
       }
     }
   }
-  R *P = S.part + ((int64_t)(blockIdx.y * maxparts + q) * TB * RC) * NC;
+  const gptr<R> P = G(S.part) + ((int64_t)(blockIdx.y * maxparts + q) * TB * RC) * NC;
 #pragma unroll
   for (int u = 0; u < 4; u++) {
     P[(int64_t)(r * RC + gq * 4 + u) * NC] = accr[u];
@@ -660,17 +928,17 @@ __global__ __launch_bounds__(NT) void k_solve_finish(const SysDev<R> *__restrict
   const int tid = threadIdx.x, r = tid >> 2, gq = tid & 3;
   const bool upper = (kind == S_UPPER || kind == S_UPPER_T);
   const bool trans = (kind == S_LOWER_T || kind == S_UPPER_T);
-  const R *Dg = S.dinv + ((int64_t)K * 2 + (upper ? 0 : 1)) * TB * TB * NC;
+  const gptr<const R> Dg = G(S.dinv) + ((int64_t)K * 2 + (upper ? 0 : 1)) * TB * TB * NC;
   stage_tile<R, C, LD>(Dg, Ds);
 #pragma unroll
   for (int u = 0; u < 4; u++) {
     const int cc = gq * 4 + u;
     const int64_t gi = ((int64_t)(K * TB + r) * nrhs + rc0 + cc) * NC;
     const bool ok = rc0 + cc < nrhs;
-    R sr = ok ? S.X[gi] : R(0), si = 0;
-    if constexpr (C) si = ok ? S.X[gi + 1] : R(0);
+    R sr = ok ? G(S.X)[gi] : R(0), si = 0;
+    if constexpr (C) si = ok ? G(S.X)[gi + 1] : R(0);
     for (int pp = 0; pp < nparts; pp++) {
-      const R *P = S.part + ((int64_t)(blockIdx.y * maxparts + pp) * TB * RC) * NC;
+      const gptr<const R> P = G(S.part) + ((int64_t)(blockIdx.y * maxparts + pp) * TB * RC) * NC;
       sr -= P[(int64_t)(r * RC + cc) * NC];
       if constexpr (C) si -= P[(int64_t)(r * RC + cc) * NC + 1];
     }
@@ -701,8 +969,8 @@ __global__ __launch_bounds__(NT) void k_solve_finish(const SysDev<R> *__restrict
     const int cc = gq * 4 + u;
     if (rc0 + cc < nrhs) {
       const int64_t gi = ((int64_t)(K * TB + r) * nrhs + rc0 + cc) * NC;
-      S.X[gi] = accr[u];
-      if constexpr (C) S.X[gi + 1] = acci[u];
+      G(S.X)[gi] = accr[u];
+      if constexpr (C) G(S.X)[gi + 1] = acci[u];
     }
   }
 }
@@ -736,26 +1004,86 @@ __device__ __forceinline__ int tasks_before(const SolveSet &ss, int step) {
   return t;
 }
 
-__device__ __forceinline__ void wait_flag(const int *flag, int *err) {
-  int n = 0;
-  while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
-    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
-    if (++n > SPIN_LIMIT) {
-      atomicExch(err, 1);
-      break;
+// Dependency q of a task is block base + dir * q.  Called by the 64 lanes of
+// wave 0: polls the flags of dependencies q .. q+63 (those below n) with
+// relaxed loads until dependency q is complete, then takes one agent-scope
+// acquire fence, so the x blocks those flags publish are visible to the
+// workgroup after its next barrier, and returns how many consecutive
+// dependencies from q on are complete.  Polls are bounded: past SPIN_LIMIT
+// the error flag is set, and once it is set the wait returns 0.
+__device__ __forceinline__ int ready_run(const int *done, int base, int dir, int q, int n, int *err) {
+  const int lane = threadIdx.x & 63;
+  for (int spins = 0;; spins++) {
+    const bool ok = q + lane < n &&
+                    __hip_atomic_load(&done[base + dir * (q + lane)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+    const uint64_t ready = __ballot(ok);
+    if (ready & 1) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      return ~ready == 0 ? 64 : __builtin_ctzll(~ready);
+    }
+    if (__builtin_amdgcn_readfirstlane(__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) != 0)
+      return 0;
+    if (spins > SPIN_LIMIT) {
+      if (lane == 0) atomicExch(err, 1);
+      return 0;
     }
     __builtin_amdgcn_s_sleep(2);
   }
 }
 
+// acc += T x_J for thread (r, gq): row r of the staged tile (transposed when
+// trans) times columns gq*4 .. gq*4+3 of the staged x block.
+template <class R, bool C, int LD>
+__device__ __forceinline__ void tile_mac(const R (*Ts)[TB][LD], const R (*Xs)[TB][RC + 1], bool trans, int r,
+                                         int gq, R *accr, R *acci) {
+#pragma unroll 4
+  for (int k = 0; k < TB; k++) {
+    const R tr = trans ? Ts[0][k][r] : Ts[0][r][k];
+    R ti = 0;
+    if constexpr (C) ti = trans ? Ts[1][k][r] : Ts[1][r][k];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const R xr = Xs[0][k][gq * 4 + u];
+      if constexpr (C) {
+        const R xi = Xs[1][k][gq * 4 + u];
+        accr[u] += tr * xr - ti * xi;
+        acci[u] += tr * xi + ti * xr;
+      } else {
+        accr[u] += tr * xr;
+      }
+    }
+  }
+}
+
+// Stages rows J*TB .. J*TB+63, columns rc0 .. rc0+RC-1 of X (zero past nrhs).
+template <class R, bool C>
+__device__ __forceinline__ void stage_x(gptr<const R> X, int J, int rc0, int nrhs, R (*Xs)[TB][RC + 1]) {
+  constexpr int NC = Elem<R, C>::NC;
+  for (int e = threadIdx.x; e < TB * RC; e += NT) {
+    const int rr = e / RC, cc = e % RC;
+    const int64_t gi = ((int64_t)(J * TB + rr) * nrhs + rc0 + cc) * NC;
+    const bool ok = rc0 + cc < nrhs;
+    Xs[0][rr][cc] = ok ? X[gi] : R(0);
+    if constexpr (C) Xs[NC - 1][rr][cc] = ok ? X[gi + 1] : R(0);
+  }
+}
+
+// Task K of a persistent solve accumulates its dependencies oldest first,
+// each run of complete ones after a single acquire, so it follows the wave
+// of completed block steps.  Before waiting on the last one (the neighbour
+// K -/+ 1) it stages everything that does not depend on it (the neighbour's
+// tile, the diagonal inverse, r_K): the chain from one block step to the
+// next carries only the neighbour's x block, one 64x64 tile product, the
+// diagonal product and the publication.
 template <class R, bool C>
 __global__ __launch_bounds__(NT) void k_solve_persist(const SysDev<R> *__restrict__ sys, SolveSet ss, int kind,
                                                       int *counter, int *err) {
   constexpr int NC = Elem<R, C>::NC;
   constexpr int LD = TB + 1;
   __shared__ R Ts[NC][TB][LD];
+  __shared__ R Ds[NC][TB][LD];
   __shared__ R Xs[NC][TB][RC + 1];
-  __shared__ int tk;
+  __shared__ int tk, nrun;
   const int tid = threadIdx.x, r = tid >> 2, gq = tid & 3;
   const bool trans = (kind == S_LOWER_T || kind == S_UPPER_T);
   const bool upper = (kind == S_UPPER || kind == S_UPPER_T);
@@ -789,81 +1117,70 @@ __global__ __launch_bounds__(NT) void k_solve_persist(const SysDev<R> *__restric
     const int nrhs = S.nrhs;
     int jb, je;
     solve_range(kind, K, p.J0[K], p.J1[K], p.I0[K], p.Jx[K], &jb, &je);
+    // neighbour nb (the last dependency to complete) and the older range [ob, oe)
+    const bool has_nb = je > jb;
+    const int nb = asc ? je - 1 : jb;
+    const int ob = has_nb ? (asc ? jb : jb + 1) : 0, oe = has_nb ? (asc ? je - 1 : je) : 0;
+    auto tile = [&](int J) { return trans ? tile_ptr<R, C>(p, S.tiles, J, K) : tile_ptr<R, C>(p, S.tiles, K, J); };
+    const gptr<const R> Dg = G(S.dinv) + ((int64_t)K * 2 + (upper ? 0 : 1)) * TB * TB * NC;
     for (int rc0 = 0; rc0 < nrhs; rc0 += RC) {
       R accr[4] = {0, 0, 0, 0}, acci[4] = {0, 0, 0, 0};
-      // oldest dependencies first: the last one is the neighbour block
-      for (int q = 0; q < je - jb; q++) {
-        const int J = asc ? jb + q : je - 1 - q;
-        if (tid == 0) wait_flag(&S.done[J], err);
-        __syncthreads();
-        const R *T = trans ? tile_ptr<R, C>(p, S.tiles, J, K) : tile_ptr<R, C>(p, S.tiles, K, J);
-        stage_tile<R, C, LD>(T, Ts);
-        for (int e = tid; e < TB * RC; e += NT) {
-          const int rr = e / RC, cc = e % RC;
-          const int64_t gi = ((int64_t)(J * TB + rr) * nrhs + rc0 + cc) * NC;
-          const bool ok = rc0 + cc < nrhs;
-          Xs[0][rr][cc] = ok ? S.X[gi] : R(0);
-          if constexpr (C) Xs[NC - 1][rr][cc] = ok ? S.X[gi + 1] : R(0);
+      const int nold = oe - ob, base = asc ? ob : oe - 1, dir = asc ? 1 : -1;
+      for (int q = 0; q < nold;) {
+        if (tid < 64) {
+          int run = ready_run(S.done, base, dir, q, nold, err);
+          if (run == 0) run = nold - q;  // error: finish without waiting
+          if (tid == 0) nrun = run;
         }
         __syncthreads();
-#pragma unroll 4
-        for (int k = 0; k < TB; k++) {
-          const R tr = trans ? Ts[0][k][r] : Ts[0][r][k];
-          R ti = 0;
-          if constexpr (C) ti = trans ? Ts[NC - 1][k][r] : Ts[NC - 1][r][k];
-#pragma unroll
-          for (int u = 0; u < 4; u++) {
-            const R xr = Xs[0][k][gq * 4 + u];
-            if constexpr (C) {
-              const R xi = Xs[NC - 1][k][gq * 4 + u];
-              accr[u] += tr * xr - ti * xi;
-              acci[u] += tr * xi + ti * xr;
-            } else {
-              accr[u] += tr * xr;
-            }
-          }
+        const int run = nrun;
+        for (int u = 0; u < run; u++) {
+          const int J = base + dir * (q + u);
+          stage_tile<R, C, LD>(tile(J), Ts);
+          stage_x<R, C>(G(S.X), J, rc0, nrhs, Xs);
+          __syncthreads();
+          tile_mac<R, C, LD>(Ts, Xs, trans, r, gq, accr, acci);
+          __syncthreads();
         }
-        __syncthreads();
+        q += run;
       }
-      // x_K = Dinv'(K) (r_K - acc)
-      const R *Dg = S.dinv + ((int64_t)K * 2 + (upper ? 0 : 1)) * TB * TB * NC;
-      stage_tile<R, C, LD>(Dg, Ts);
+      // everything independent of the neighbour
+      if (has_nb) stage_tile<R, C, LD>(tile(nb), Ts);
+      stage_tile<R, C, LD>(Dg, Ds);
+      R sr[4] = {0, 0, 0, 0}, si[4] = {0, 0, 0, 0};
 #pragma unroll
       for (int u = 0; u < 4; u++) {
         const int cc = gq * 4 + u;
         const int64_t gi = ((int64_t)(K * TB + r) * nrhs + rc0 + cc) * NC;
-        const bool ok = rc0 + cc < nrhs;
-        R sr = ok ? S.X[gi] : R(0), si = 0;
-        if constexpr (C) si = ok ? S.X[gi + 1] : R(0);
-        Xs[0][r][cc] = sr - accr[u];
-        if constexpr (C) Xs[NC - 1][r][cc] = si - acci[u];
+        if (rc0 + cc < nrhs) {
+          sr[u] = G(S.X)[gi];
+          if constexpr (C) si[u] = G(S.X)[gi + 1];
+        }
+      }
+      if (has_nb) {
+        if (tid < 64) (void)ready_run(S.done, nb, 1, 0, 1, err);
+        __syncthreads();
+        stage_x<R, C>(G(S.X), nb, rc0, nrhs, Xs);
+        __syncthreads();
+        tile_mac<R, C, LD>(Ts, Xs, trans, r, gq, accr, acci);
+      }
+      __syncthreads();
+      // x_K = Dinv'(K) (r_K - acc)
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        Xs[0][r][gq * 4 + u] = sr[u] - accr[u];
+        if constexpr (C) Xs[NC - 1][r][gq * 4 + u] = si[u] - acci[u];
       }
       __syncthreads();
       R yr[4] = {0, 0, 0, 0}, yi[4] = {0, 0, 0, 0};
-#pragma unroll 4
-      for (int k = 0; k < TB; k++) {
-        const R dr = trans ? Ts[0][k][r] : Ts[0][r][k];
-        R di = 0;
-        if constexpr (C) di = trans ? Ts[NC - 1][k][r] : Ts[NC - 1][r][k];
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-          const R tr = Xs[0][k][gq * 4 + u];
-          if constexpr (C) {
-            const R ti = Xs[NC - 1][k][gq * 4 + u];
-            yr[u] += dr * tr - di * ti;
-            yi[u] += dr * ti + di * tr;
-          } else {
-            yr[u] += dr * tr;
-          }
-        }
-      }
+      tile_mac<R, C, LD>(Ds, Xs, trans, r, gq, yr, yi);
 #pragma unroll
       for (int u = 0; u < 4; u++) {
         const int cc = gq * 4 + u;
         if (rc0 + cc < nrhs) {
           const int64_t gi = ((int64_t)(K * TB + r) * nrhs + rc0 + cc) * NC;
-          S.X[gi] = yr[u];
-          if constexpr (C) S.X[gi + 1] = yi[u];
+          G(S.X)[gi] = yr[u];
+          if constexpr (C) G(S.X)[gi + 1] = yi[u];
         }
       }
       __syncthreads();
@@ -894,13 +1211,13 @@ __global__ void k_scatter_csr(DevPlan p, R *__restrict__ tiles, const int64_t *_
   if (row >= p.NB * TB) return;
   const int I = row / TB, rr = row % TB;
   if (row >= p.n) {
-    R *T = tile_ptr<R, C>(p, tiles, I, I);
+    const gptr<R> T = tile_ptr<R, C>(p, tiles, I, I);
     T[(int64_t)(rr * TB + rr) * NC] = R(1);
     return;
   }
   for (int64_t q = row_ptr[row]; q < row_ptr[row + 1]; q++) {
     const int c = col[q] - 1;
-    R *T = tile_ptr<R, C>(p, tiles, I, c / TB);
+    const gptr<R> T = tile_ptr<R, C>(p, tiles, I, c / TB);
     const int64_t e = (int64_t)(rr * TB + (c % TB)) * NC;
     T[e] = R(vals[q * NC]);
     if constexpr (C) T[e + 1] = R(vals[q * NC + 1]);

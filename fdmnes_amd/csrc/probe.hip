@@ -36,3 +36,20 @@ extern "C" int fdmgpu_probe_mfma(int prec, double *host_out) {
   (void)hipFree(d);
   return e == hipSuccess ? 0 : -2;
 }
+
+// XCD placement probe: workgroup w of a grid of nwg workgroups (256 threads)
+// writes the id of the XCD (accelerator complex die) it runs on, read from
+// the XCC_ID hardware register (hwreg 20, bits 3:0), to out[w].
+__global__ void k_probe_xcc(int *out) {
+  if (threadIdx.x == 0) out[blockIdx.x] = __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 15;
+}
+
+extern "C" int fdmgpu_probe_xcc(int nwg, int *host_out) {
+  if (nwg <= 0) return -1;
+  int *d = nullptr;
+  if (hipMalloc(&d, sizeof(int) * (size_t)nwg) != hipSuccess) return -2;
+  hipLaunchKernelGGL(k_probe_xcc, dim3(nwg), dim3(256), 0, 0, d);
+  hipError_t e = hipMemcpy(host_out, d, sizeof(int) * (size_t)nwg, hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  return e == hipSuccess ? 0 : -2;
+}

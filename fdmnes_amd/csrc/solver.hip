@@ -227,7 +227,7 @@ inline void push(Items &it, int g, int K, int nwg) {
 namespace {
 inline void push_trail(TrailItems &it, int g, int Kb, int nk, int i0, int ni, int j0, int nj) {
   if (ni <= 0 || nj <= 0 || nk <= 0) return;
-  if (it.count == 0) it.off[0] = 0;
+  if (it.count == 0) it.off[0] = it.moff[0] = 0;
   const int c = it.count;
   it.sys[c] = g;
   it.Kb[c] = Kb;
@@ -235,10 +235,18 @@ inline void push_trail(TrailItems &it, int g, int Kb, int nk, int i0, int ni, in
   it.i0[c] = i0;
   it.j0[c] = j0;
   it.nj[c] = nj;
+  it.ni[c] = ni;
   it.off[c + 1] = it.off[c] + ni * nj;
+  it.moff[c + 1] = it.moff[c] + ((ni + 1) / 2) * ((nj + 1) / 2);
   it.count++;
 }
 }  // namespace
+
+// Grid of k_trail: at most persist_wg workgroups, a multiple of NXCD.
+static int xcd_grid(int items, int persist_wg) {
+  const int g = std::max(std::min(items, persist_wg), 1);
+  return (g + NXCD - 1) / NXCD * NXCD;
+}
 
 // Right-looking blocked UL from the last block row up.  With gk > 0 the
 // block steps are taken in outer groups of gk: each inner step K factorises
@@ -312,13 +320,17 @@ int Group<R, C>::factor() {
       });
     if (cross.count)
       timed([&]() {
-        hipLaunchKernelGGL((k_trail<R, C>), dim3(std::min(cross.off[cross.count], persist_wg)), dim3(NT), 0, stream,
+        hipLaunchKernelGGL((k_trail<R, C>), dim3(xcd_grid(cross.off[cross.count], persist_wg)), dim3(NT), 0, stream,
                            (const SysDev<R> *)d_sys, cross);
       });
     if (trail.count)
       timed([&]() {
-        hipLaunchKernelGGL((k_trail<R, C>), dim3(std::min(trail.off[trail.count], persist_wg)), dim3(NT), 0, stream,
-                           (const SysDev<R> *)d_sys, trail);
+        if (trail_tile == 128)
+          hipLaunchKernelGGL((k_trail2<R, C>), dim3(xcd_grid(trail.moff[trail.count], persist_wg)), dim3(NT2), 0,
+                             stream, (const SysDev<R> *)d_sys, trail);
+        else
+          hipLaunchKernelGGL((k_trail<R, C>), dim3(xcd_grid(trail.off[trail.count], persist_wg)), dim3(NT), 0,
+                             stream, (const SysDev<R> *)d_sys, trail);
       });
   }
   FDM_HIP(hipGetLastError());

@@ -13,14 +13,18 @@ def _dev(a):
     return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
 
 
-def _run(precision, systems, repeats=1, max_sweeps=5, tol=1e-12):
+def _run(precision, systems, repeats=1, max_sweeps=5, tol=1e-12, tune=None):
     import torch
     from fdmnes_amd.batch import DeviceSolver
     nrhs = systems[0]["rhs"].shape[1]
-    sv = DeviceSolver(precision, True, nrhs)
+    cal = systems[0]["cal"]
+    dt = np.complex128 if cal else np.float64
+    sv = DeviceSolver(precision, cal, nrhs)
+    if tune:
+        sv.tune(**tune)
     slots = [sv.add_system(s["lb1"], s["lb2"], s["rp"], s["col"]) for s in systems]
-    vals = [_dev(s["vals"].astype(np.complex128)) for s in systems]
-    b = [_dev(s["rhs"].astype(np.complex128)) for s in systems]
+    vals = [_dev(s["vals"].astype(dt)) for s in systems]
+    b = [_dev(s["rhs"].astype(dt)) for s in systems]
     x = [torch.zeros_like(t) for t in b]
     out = None
     for _ in range(repeats):  # graph replays must reproduce the first run
@@ -41,6 +45,35 @@ def test_grouped_uneven_clusters_fp64():
     for s, x in zip(systems, xs):
         xo = oracle_solution(s)
         assert np.abs(x - xo).max() / np.abs(xo).max() <= 1e-10
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("trail_tile,gk", [(64, 0), (64, 3), (128, 3), (128, 8), (128, 16)])
+def test_schedules_match_oracle(precision, trail_tile, gk):
+    """Every tile schedule (one-step updates, two-level blocking with 64 x 64
+    or 2 x 2 macro output tiles, odd item edges) gives the oracle's solution."""
+    systems = [small_system(R=1.2, seed=11, E=0.4), small_system(R=1.9, seed=12, E=1.1),
+               small_system(R=2.2, seed=13, E=2.3)]
+    print("precision", precision, "tile", trail_tile, "gk", gk, "sizes", [len(s["lb1"]) for s in systems], flush=True)
+    _, _, xs, sw, _, _, _ = _run(precision, systems, repeats=2,
+                                  tune=dict(gk=gk, trail_wg=64, trail_tile=trail_tile, persist_solve=1))
+    for s, x in zip(systems, xs):
+        xo = oracle_solution(s)
+        assert np.abs(x - xo).max() / np.abs(xo).max() <= 1e-10
+
+
+@pytest.mark.parametrize("trail_tile", [64, 128])
+def test_real_arithmetic_schedules(trail_tile):
+    """Real arithmetic (Basereel without complex energy) through both update kernels."""
+    systems = [small_system(R=1.6, seed=14, E=0.9, E_comp=False, basereel=True),
+               small_system(R=2.0, seed=15, E=1.7, E_comp=False, basereel=True)]
+    assert not systems[0]["cal"]
+    for precision in (0, 1):
+        print("real precision", precision, "tile", trail_tile, flush=True)
+        _, _, xs, _, _, _, _ = _run(precision, systems, repeats=2, tune=dict(gk=4, trail_wg=64, trail_tile=trail_tile))
+        for s, x in zip(systems, xs):
+            xo = oracle_solution(s)
+            assert np.abs(x - xo).max() / np.abs(xo).max() <= 1e-10
 
 
 def test_grouped_mixed_precision():
