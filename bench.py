@@ -100,12 +100,18 @@ def roofline(sv32, slots32, vals, src, G):
     sv32.solve(slots32, [vals[i] for i in grp], [src] * G, xs)
     st = sv32.stats()
     upd_s = st["update_ms"] / 1e3
-    alg_flops = 8.0 * st["profile_macs"]  # profile multiply-adds of the G slots' factorisations
+    # algorithmic flops of the update launches: the element-profile multiply-adds
+    # of the G factorisations times the trailing updates' share of the tile-level
+    # work (the rest is panels and triangular block products)
+    share = st["tile_macs"] / (st["tile_macs"] + st["tile_other_macs"])
+    alg_flops = 8.0 * st["profile_macs"] * share
     ach = alg_flops / upd_s / 1e12
     roof = {"bound": "mfma", "achieved": ach, "peak": PEAK["fp32"], "unit": "TFLOP/s", "frac": ach / PEAK["fp32"],
             "traffic": traffic_from_profile(), "kernel": "k_trail<float,complex> (trailing update: cross + group launches)",
             "launches": st["update_launches"], "avg_launch_us": 1e6 * upd_s / max(st["update_launches"], 1),
             "algorithmic_flops_per_launch": alg_flops / max(st["update_launches"], 1),
+            "update_share_of_factorisation": share,
+            "algorithmic_bytes_per_launch": st["update_bytes"] / max(st["update_launches"], 1),
             "executed_tile_flops": 8.0 * st["tile_macs"]}
     return roof
 

@@ -112,7 +112,8 @@ class DeviceSolver:
         return np.array(out[:])
 
     def stats(self) -> dict:
-        out = (C.c_double * 6)()
+        out = (C.c_double * 8)()
         _lib.check(self.L.fdmgpu_solver_stats(self._h, out))
         return {"update_ms": out[0], "update_launches": int(out[1]), "tile_macs": out[2],
-                "profile_macs": out[3], "factor_bytes": out[4], "profile_entries": out[5]}
+                "profile_macs": out[3], "factor_bytes": out[4], "profile_entries": out[5],
+                "tile_other_macs": out[6], "update_bytes": out[7]}

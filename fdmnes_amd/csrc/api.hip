@@ -323,9 +323,10 @@ int fdmgpu_solver_stats(const fdmgpu_solver *s, double *out) {
   auto fill = [&](auto *p) {
     out[0] = p->grp.update_ms;
     out[1] = (double)p->grp.update_launches;
-    double tm = 0, fm = 0, sm = 0;
+    double tm = 0, fm = 0, sm = 0, om = 0;
     for (auto &sl : p->slots) {
       tm += sl.plan->tile_update_macs();
+      om += sl.plan->tile_other_macs();
       fm += sl.factor_macs;
       sm += sl.solve_macs;
     }
@@ -333,6 +334,8 @@ int fdmgpu_solver_stats(const fdmgpu_solver *s, double *out) {
     out[3] = fm;
     out[4] = (double)p->grp.factor_bytes();
     out[5] = sm;
+    out[6] = om;
+    out[7] = p->grp.update_bytes;
   };
   if (s->prec == FDMGPU_FP64) {
     if (s->cal) fill((Solver<double, true> *)s->impl);

@@ -50,6 +50,7 @@ struct Plan {
   static double solve_macs(int n, const int *lb1, const int *lb2);
   // multiply-adds the blocked kernels actually perform (tile granularity)
   double tile_update_macs() const;
+  double tile_other_macs() const;  // panels (2/3 TB^3) and triangular block products (TB^3/2 per tile)
 };
 
 // A group of independent systems factorised and solved by grouped launches
@@ -80,6 +81,7 @@ struct Group {
   bool time_updates = false;
   double update_ms = 0;
   int64_t update_launches = 0;
+  double update_bytes = 0;  // algorithmic bytes of the timed launches (C read + write, A and B read once)
   std::vector<hipEvent_t> ev;
 
   Group() = default;

@@ -110,6 +110,12 @@ double Plan::solve_macs(int n, const int *lb1, const int *lb2) {
   return s;
 }
 
+double Plan::tile_other_macs() const {
+  double s = 0;
+  for (int K = 0; K < NB; K++) s += 0.5 * (double)((K - I0[K]) + (K - J0[K])) + 2.0 / 3.0;
+  return s * (double)TB * TB * TB;
+}
+
 double Plan::tile_update_macs() const {
   double s = 0;
   for (int K = 0; K < NB; K++) s += (double)(K - I0[K]) * (double)(K - J0[K]);
@@ -313,6 +319,17 @@ int Group<R, C>::factor() {
     if (itc.count)
       hipLaunchKernelGGL((k_tile_gemm<R, C>), dim3(std::min(itc.off[itc.count], persist_wg)), dim3(NT), 0, stream,
                          (const SysDev<R> *)d_sys, itc, (int)G_COLTRSM);
+    if (time_updates) {
+      const double tb = (double)TB * TB * NC * sizeof(R);
+      for (int c = 0; c < itu.count; c++) {
+        const Plan &pl = *m[itu.sys[c]].plan;
+        const int K = itu.K[c], nI = K - pl.I0[K], nJ = K - pl.J0[K];
+        update_bytes += tb * (2.0 * nI * nJ + nI + nJ);
+      }
+      for (const TrailItems *ti : {&cross, &trail})
+        for (int c = 0; c < ti->count; c++)
+          update_bytes += tb * (2.0 * ti->ni[c] * ti->nj[c] + (double)(ti->ni[c] + ti->nj[c]) * ti->nk[c]);
+    }
     if (itu.count)
       timed([&]() {
         hipLaunchKernelGGL((k_tile_gemm<R, C>), dim3(std::min(itu.off[itu.count], persist_wg)), dim3(NT), 0, stream,

@@ -140,9 +140,14 @@ int fdmgpu_solver_options(fdmgpu_solver *s, int use_graphs, int time_updates, in
  * rebuilt). */
 int fdmgpu_solver_tune(fdmgpu_solver *s, int gk, int trail_wg, int persist_solve, int solve_wg, int trail_tile);
 /* out[0] ms spent in trailing-update launches (when timed), out[1] their
- * count, out[2] multiply-adds of one factorisation of every slot at tile
- * granularity, out[3] the same on the element profile, out[4] factor bytes
- * of all slots, out[5] profile entries (solve multiply-adds per RHS). */
+ * count, out[2] multiply-adds of the trailing updates of one factorisation
+ * of every slot at tile granularity, out[3] multiply-adds of the whole
+ * factorisation on the element profile, out[4] factor bytes of all slots,
+ * out[5] profile entries (solve multiply-adds per RHS), out[6] tile-level
+ * multiply-adds of the panels and triangular block products (the part of
+ * the factorisation outside the trailing updates), out[7] algorithmic bytes
+ * of the timed trailing-update launches (every output tile read and
+ * written once, every U and L tile of the step range read once). */
 int fdmgpu_solver_stats(const fdmgpu_solver *s, double *out);
 
 #ifdef __cplusplus
