@@ -9,7 +9,9 @@
 // the tiles in place: tile (K,K) holds U_D (upper incl. diagonal) and L_D
 // (strict lower, unit diagonal implied) of the diagonal block; tiles (K,J<K)
 // hold the unit-lower factor L; tiles (I<K,K) hold U.
-// dinv(K) holds U_D^-1 and L_D^-1 of each diagonal block.
+// dinv(K) holds DSLOTS tiles per diagonal block: U_D^-1, L_D^-1, and the
+// neighbour products MU(K) = U_D^-1 U(K,K+1) and ML(K) = L_D^-1 L(K,K-1) the
+// substitutions use (x_K = y_K - M x_neighbour).
 //
 // Every kernel is grouped: one launch serves the same step of up to MAXG
 // independent systems (energy points, clusters of different size).  A launch
@@ -73,6 +75,9 @@ __device__ __forceinline__ int find_item(const Items &it, int b) {
   return g;
 }
 
+constexpr int DSLOTS = 4;
+enum DinvSlot { D_UINV = 0, D_LINV = 1, D_MU = 2, D_ML = 3 };
+
 template <class R, bool C> struct Elem;
 template <class R> struct Elem<R, true> { static constexpr int NC = 2; };
 template <class R> struct Elem<R, false> { static constexpr int NC = 1; };
@@ -80,6 +85,11 @@ template <class R> struct Elem<R, false> { static constexpr int NC = 1; };
 template <class R, bool C>
 __device__ __forceinline__ gptr<R> tile_ptr(const DevPlan &p, R *tiles, int I, int J) {
   return G(tiles) + (G(p.tbase)[I] + (J - G(p.J0)[I])) * (int64_t)(TB * TB * Elem<R, C>::NC);
+}
+
+template <class R, bool C>
+__device__ __forceinline__ gptr<R> dinv_tile(const SysDev<R> &S, int K, int slot) {
+  return G(S.dinv) + ((int64_t)K * DSLOTS + slot) * (int64_t)(TB * TB * Elem<R, C>::NC);
 }
 
 // ---------------------------------------------------------------- MFMA
@@ -159,13 +169,24 @@ __global__ __launch_bounds__(NT) void k_tile_gemm(const SysDev<R> *__restrict__ 
     B = tile_ptr<R, C>(p, S.tiles, K, J);
     Cg = tile_ptr<R, C>(p, S.tiles, I, J);
   } else if (mode == G_ROWTRSM) {
-    A = G(S.dinv) + (int64_t)K * 2 * TB * TB * NC;  // Uinv
-    Cg = tile_ptr<R, C>(p, S.tiles, K, J0 + b);
-    B = Cg;
+    A = dinv_tile<R, C>(S, K, D_UINV);
+    if (b < nJ) {
+      Cg = tile_ptr<R, C>(p, S.tiles, K, J0 + b);
+      B = Cg;
+    } else {  // MU(K) = Uinv(K) U(K,K+1); U(K,K+1) was finished at step K+1
+      B = tile_ptr<R, C>(p, S.tiles, K, K + 1);
+      Cg = dinv_tile<R, C>(S, K, D_MU);
+    }
   } else {
-    Cg = tile_ptr<R, C>(p, S.tiles, I0 + b, K);
-    A = Cg;
-    B = G(S.dinv) + ((int64_t)K * 2 + 1) * TB * TB * NC;  // Linv
+    if (b < K - I0) {
+      Cg = tile_ptr<R, C>(p, S.tiles, I0 + b, K);
+      A = Cg;
+      B = dinv_tile<R, C>(S, K, D_LINV);
+    } else {  // ML(K) = Linv(K) L(K,K-1); L(K,K-1) was finished by this step's ROWTRSM launch
+      A = dinv_tile<R, C>(S, K, D_LINV);
+      B = tile_ptr<R, C>(p, S.tiles, K, K - 1);
+      Cg = dinv_tile<R, C>(S, K, D_ML);
+    }
   }
   stage_tile<R, C, LD>(A, As);
   stage_tile<R, C, LD>(B, Bs);
@@ -271,14 +292,26 @@ constexpr int KC = 16;  // k-chunk of the LDS pipeline
 constexpr int NXCD = 8;
 constexpr int GM = 8;
 template <class R, bool C>
-__global__ __launch_bounds__(NT, 2) void k_trail(const SysDev<R> *__restrict__ sys, TrailItems it) {
+__global__ __launch_bounds__(NT, sizeof(R) == 4 ? 3 : 2) void k_trail(const SysDev<R> *__restrict__ sys, TrailItems it) {
   constexpr int NC = Elem<R, C>::NC;
-  constexpr int LD = TB + 1;
+  // FP32: LDS rows of 80 words, so the four k rows an MFMA operand read
+  // touches (lanes 16*lk .. 16*lk+15 read row k4+lk) fall in four disjoint
+  // groups of 16 banks; A is stored with its row index XOR-ed by sw(k) (a
+  // multiple of 4 below 16), so the staging writes, 16 k values x 4 rows per
+  // wave, also hit 64 distinct banks.  FP64 keeps rows of 65 elements, which
+  // lets two workgroups share a CU's LDS.
+  constexpr bool F32 = sizeof(R) == 4;
+  constexpr int LD = F32 ? TB + 16 : TB + 1;
   constexpr int PER = TB * KC / NT;  // elements per thread per chunk and operand (4)
+  auto sw = [](int k) { return F32 ? ((k & 15) >> 2) << 2 : 0; };
   constexpr int64_t TE = (int64_t)TB * TB * NC;  // elements per tile
   __shared__ R As[2][NC][KC][LD];
   __shared__ R Bs[2][NC][KC][LD];
-  __shared__ int64_t boff[16];  // per step of the item: offset of tile (K, J)
+  // per step t of the current item (K = Kb + t): I0(K), J0(K), tbase(K),
+  // loaded once per item; the tiles of the item then need no global metadata
+  // beyond tbase(I), J0(I) of their row
+  __shared__ int kI0[16], kJ0[16];
+  __shared__ int64_t ktb[16];
   typedef typename Mfma<R>::v4 v4;
   typedef R r2 __attribute__((ext_vector_type(2)));
   const int tid = threadIdx.x;
@@ -291,6 +324,7 @@ __global__ __launch_bounds__(NT, 2) void k_trail(const SysDev<R> *__restrict__ s
   constexpr bool M3 = C && sizeof(R) == 4;
   const int total = it.off[it.count];
   const int per = (total + NXCD - 1) / NXCD, xcd = blockIdx.x % NXCD, lanes = gridDim.x / NXCD;
+  int gcur = -1;
   for (int jw = blockIdx.x / NXCD; jw < per; jw += lanes) {
     const int wb = xcd * per + jw;
     if (wb >= total) break;
@@ -301,19 +335,26 @@ __global__ __launch_bounds__(NT, 2) void k_trail(const SysDev<R> *__restrict__ s
     const int nj = it.nj[g], ni = (it.off[g + 1] - it.off[g]) / nj;
     const int band = b / (GM * nj), hr = min(GM, ni - band * GM), q = b - band * GM * nj;
     const int I = it.i0[g] + band * GM + q % hr, J = it.j0[g] + q / hr;
-    const int Kb = it.Kb[g];
-    const gptr<const int> I0g = G(p.I0), J0g = G(p.J0);
-    const gptr<const int64_t> tbg = G(p.tbase);
-    int m = 0;
-    while (m < it.nk[g] && I0g[Kb + m] <= I && J0g[Kb + m] <= J) m++;
-    const int nch = m * (TB / KC);
-    // Tile addresses of the item, outside the chunk loop: A = tile(I, K) are
-    // consecutive tiles of block row I; the offsets of B = tile(K, J) go to LDS.
+    const int Kb = it.Kb[g], nk = it.nk[g];
+    // row metadata of A = tile(I, K) (consecutive tiles of block row I), in flight during the set-up
+    const int64_t rowA = G(p.tbase)[I] + (Kb - G(p.J0)[I]);
     if (jw != (int)blockIdx.x / NXCD) __syncthreads();  // previous item done with LDS
-    if (tid < m) boff[tid] = (tbg[Kb + tid] + (J - J0g[Kb + tid])) * TE;
-    const gptr<const R> Ab = G(S.tiles) + (tbg[I] + (Kb - J0g[I])) * TE;
+    if (g != gcur) {
+      if (tid < nk) {
+        kI0[tid] = G(p.I0)[Kb + tid];
+        kJ0[tid] = G(p.J0)[Kb + tid];
+        ktb[tid] = G(p.tbase)[Kb + tid];
+      }
+      __syncthreads();
+      gcur = g;
+    }
+    // steps of the range reaching tile (I, J): a prefix, counted by a ballot
+    const bool reach = lane < nk && kI0[lane] <= I && kJ0[lane] <= J;
+    const uint64_t rb = __ballot(reach);
+    const int m = ~rb == 0 ? 64 : __builtin_ctzll(~rb);
+    const int nch = m * (TB / KC);
     const gptr<const R> tiles = G(S.tiles);
-    __syncthreads();
+    const gptr<const R> Ab = tiles + rowA * TE;
     constexpr int NX = M3 ? 2 : 1;  // P2 accumulators only for the 3M form
     v4 accr[2][2], acci[2][2], accx[NX][NX];
 #pragma unroll
@@ -338,7 +379,7 @@ __global__ __launch_bounds__(NT, 2) void k_trail(const SysDev<R> *__restrict__ s
       constexpr int s = decltype(set)::value;
       const int st = c / (TB / KC), kc = (c % (TB / KC)) * KC;
       const gptr<const R> A = Ab + st * TE;
-      const gptr<const R> B = tiles + boff[st];
+      const gptr<const R> B = tiles + (ktb[st] + (J - kJ0[st])) * TE;
 #pragma unroll
       for (int q = 0; q < PER; q++) {
         const int e = tid + q * NT;
@@ -361,12 +402,12 @@ __global__ __launch_bounds__(NT, 2) void k_trail(const SysDev<R> *__restrict__ s
         const int e = tid + q * NT;
         const int ar = e / KC, ak = e % KC, bk = e / TB, bc = e % TB;
         if constexpr (C) {
-          As[buf][0][ak][ar] = pa[s][q].x;
-          As[buf][NC - 1][ak][ar] = pa[s][q].y;
+          As[buf][0][ak][ar ^ sw(ak)] = pa[s][q].x;
+          As[buf][NC - 1][ak][ar ^ sw(ak)] = pa[s][q].y;
           Bs[buf][0][bk][bc] = pb[s][q].x;
           Bs[buf][NC - 1][bk][bc] = pb[s][q].y;
         } else {
-          As[buf][0][ak][ar] = qa[s][q];
+          As[buf][0][ak][ar ^ sw(ak)] = qa[s][q];
           Bs[buf][0][bk][bc] = qb[s][q];
         }
       }
@@ -377,8 +418,8 @@ __global__ __launch_bounds__(NT, 2) void k_trail(const SysDev<R> *__restrict__ s
           R ar[2], ai[2], br[2], bi[2];
 #pragma unroll
           for (int a = 0; a < 2; a++) {
-            ar[a] = As[buf][0][k4 + lk][r0 + a * 16 + li];
-            if constexpr (C) ai[a] = As[buf][NC - 1][k4 + lk][r0 + a * 16 + li];
+            ar[a] = As[buf][0][k4 + lk][r0 + a * 16 + (li ^ sw(k4 + lk))];
+            if constexpr (C) ai[a] = As[buf][NC - 1][k4 + lk][r0 + a * 16 + (li ^ sw(k4 + lk))];
           }
 #pragma unroll
           for (int bb = 0; bb < 2; bb++) {
@@ -671,12 +712,17 @@ __global__ __launch_bounds__(NT2, 1) void k_trail2(const SysDev<R> *__restrict__
 // (the first in elimination order) is recorded 1-based in *fail.  One
 // workgroup per item.
 //
-// Elimination step q (one barrier): every row r < q subtracts
-// (a_rq / a_qq) * row q from its columns k < q; thread (r = tid/4) owns row r
-// and the columns k = tid%4 (mod 4).  The normalised row q (the stored L row,
-// a_qk / a_qq) is kept in a register and written back during step q-1, when
-// row q is no longer read.  The inverses are column-parallel substitutions,
-// four lanes per column combining partial sums by lane shuffles.
+// Thread (r = tid/4, p = tid%4) keeps the 16 elements (r, p + 4t) of the tile
+// in registers; the step loops stay rolled (a short loop body keeps the
+// instruction cache warm), so the element of the current step is picked from
+// the register array by selects over the constant index t.  Elimination step q: the four threads of row q take the
+// pivot from their group (lane shuffle), normalise the row's lower part (the
+// stored L row, a_qk / a_qq) and publish it to LDS with zeros from column q
+// on; after one barrier every row r < q subtracts a_rq * row q, a_rq coming
+// from its own group by shuffle.  The inverses then run without barriers:
+// four lanes per column j hold x_k (k = p + 4t) in registers; entries not yet
+// computed are zero, so each substitution step is a full 64-term dot product
+// with the published factor row, reduced across the four lanes by shuffles.
 template <class R>
 __device__ __forceinline__ R lane_sum4(R v) {
   v += __shfl_xor(v, 1);
@@ -688,143 +734,167 @@ template <class R, bool C>
 __global__ __launch_bounds__(NT) void k_panel(const SysDev<R> *__restrict__ sys, Items it) {
   constexpr int NC = Elem<R, C>::NC;
   constexpr int LD = TB + 1;
-  __shared__ R D[NC][TB][LD];
-  __shared__ R X[NC][TB][LD];
-  const int tid = threadIdx.x;
+  constexpr int NQ = TB / 4;
+  __shared__ R P[NC][TB][LD];  // published rows, then the factored tile
+  __shared__ R Vd[NC][TB];     // 1 / U_D(i,i)
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int myr = tid >> 2, part = tid & 3;
   const SysDev<R> &S = sys[it.sys[blockIdx.x]];
   const DevPlan &p = S.p;
   const int K = it.K[blockIdx.x];
   const gptr<R> T = tile_ptr<R, C>(p, S.tiles, K, K);
-  stage_tile<R, C, LD>(T, D);
-  __syncthreads();
-  const int myr = tid >> 2, part = tid & 3;
-  int pend = -1;
-  R pr = 0, pi = 0;
+  R ar[NQ], ai[NQ];
+#pragma unroll
+  for (int t = 0; t < NQ; t++) {
+    const int64_t e = (int64_t)(myr * TB + part + 4 * t) * NC;
+    ar[t] = T[e];
+    ai[t] = C ? T[e + 1] : R(0);
+  }
+#pragma unroll 1
   for (int q = TB - 1; q >= 0; --q) {
-    const R dr = D[0][q][q];
-    R di = 0;
-    if constexpr (C) di = D[NC - 1][q][q];
-    R vr, vi = 0;
-    if constexpr (C) {
-      const R den = dr * dr + di * di;
-      if (tid == 0 && den < R(1e-20) && K * TB + q < p.n) atomicMax(S.fail, K * TB + q + 1);
-      vr = dr / den;
-      vi = -di / den;
-    } else {
-      if (tid == 0 && fabs(dr) < R(1e-10) && K * TB + q < p.n) atomicMax(S.fail, K * TB + q + 1);
-      vr = R(1) / dr;
+    const int src = (lane & ~3) | (q & 3), tq = q >> 2;
+    R mr = 0, mi = 0;
+#pragma unroll
+    for (int t = 0; t < NQ; t++) {
+      mr = t == tq ? ar[t] : mr;
+      mi = t == tq ? ai[t] : mi;
     }
-    if (pend >= 0) {  // normalised row q+1, element k = tid
-      D[0][pend][tid] = pr;
-      if constexpr (C) D[NC - 1][pend][tid] = pi;
-    }
-    pend = -1;
-    if (tid < q) {  // normalise the stored row, :274-280 / :296-300
-      const R xr = D[0][q][tid];
+    const R fr = __shfl(mr, src);  // a_{myr,q}; the pivot in row q's group
+    const R fi = C ? __shfl(mi, src) : R(0);
+    if (myr == q) {
+      R vr, vi = 0;
       if constexpr (C) {
-        const R xi = D[NC - 1][q][tid];
-        pr = xr * vr - xi * vi;
-        pi = xi * vr + xr * vi;
+        const R den = fr * fr + fi * fi;
+        if (part == 0 && den < R(1e-20) && K * TB + q < p.n) atomicMax(S.fail, K * TB + q + 1);
+        vr = fr / den;
+        vi = -fi / den;
       } else {
-        pr = xr * vr;
+        if (part == 0 && fabs(fr) < R(1e-10) && K * TB + q < p.n) atomicMax(S.fail, K * TB + q + 1);
+        vr = R(1) / fr;
       }
-      pend = q;
-    }
-    if (myr < q) {  // eliminate column q from row myr
-      R fr = D[0][myr][q], fi = 0;
-      if constexpr (C) {
-        fi = D[NC - 1][myr][q];
-        const R t = fr * vr - fi * vi;
-        fi = fr * vi + fi * vr;
-        fr = t;
-      } else {
-        fr = fr * vr;
-      }
-      for (int k = part; k < q; k += 4) {
-        const R lr = D[0][q][k];
-        if constexpr (C) {
-          const R li = D[NC - 1][q][k];
-          D[0][myr][k] = D[0][myr][k] - fr * lr + fi * li;
-          D[NC - 1][myr][k] = D[NC - 1][myr][k] - fr * li - fi * lr;
+#pragma unroll
+      for (int t = 0; t < NQ; t++) {
+        const int k = part + 4 * t;
+        if (k < q) {  // normalise the stored row, :274-280 / :296-300
+          const R lr = ar[t] * vr - ai[t] * vi, li = ai[t] * vr + ar[t] * vi;
+          ar[t] = lr;
+          ai[t] = li;
+          P[0][q][k] = lr;
+          if constexpr (C) P[NC - 1][q][k] = li;
         } else {
-          D[0][myr][k] = D[0][myr][k] - fr * lr;
+          P[0][q][k] = 0;
+          if constexpr (C) P[NC - 1][q][k] = 0;
         }
+      }
+      if (part == 0) {
+        Vd[0][q] = vr;
+        if constexpr (C) Vd[NC - 1][q] = vi;
       }
     }
     __syncthreads();
+    if (myr < q) {  // eliminate column q from row myr
+#pragma unroll
+      for (int t = 0; t < NQ; t++) {
+        const int k = part + 4 * t;
+        const R lr = P[0][q][k];
+        if constexpr (C) {
+          const R li = P[NC - 1][q][k];
+          ar[t] = ar[t] - fr * lr + fi * li;
+          ai[t] = ai[t] - fr * li - fi * lr;
+        } else {
+          ar[t] = ar[t] - fr * lr;
+        }
+      }
+    }
   }
-  for (int e = tid; e < TB * TB; e += NT) {
-    T[(int64_t)e * NC] = D[0][e / TB][e % TB];
-    if constexpr (C) T[(int64_t)e * NC + 1] = D[NC - 1][e / TB][e % TB];
+  __syncthreads();  // every read of the published rows done
+#pragma unroll
+  for (int t = 0; t < NQ; t++) {
+    const int k = part + 4 * t;
+    const int64_t e = (int64_t)(myr * TB + k) * NC;
+    T[e] = ar[t];
+    if constexpr (C) T[e + 1] = ai[t];
+    P[0][myr][k] = ar[t];
+    if constexpr (C) P[NC - 1][myr][k] = ai[t];
   }
-  const int j = tid >> 2;
+  __syncthreads();
+  const int j = tid >> 2;  // column of the inverses
+  R xr[NQ], xi[NQ];
   // X = U_D^-1: U X = I, column j from its last row up.
+#pragma unroll
+  for (int t = 0; t < NQ; t++) xr[t] = xi[t] = 0;
+#pragma unroll 1
   for (int i = TB - 1; i >= 0; --i) {
-    if (i <= j) {
-      R sr = 0, si = 0;
-      for (int k = i + 1 + part; k <= j; k += 4) {
-        const R ur = D[0][i][k], xr = X[0][k][j];
-        if constexpr (C) {
-          const R ui = D[NC - 1][i][k], xi = X[NC - 1][k][j];
-          sr += ur * xr - ui * xi;
-          si += ur * xi + ui * xr;
-        } else {
-          sr += ur * xr;
-        }
-      }
-      sr = lane_sum4(sr);
-      if constexpr (C) si = lane_sum4(si);
-      sr = (i == j ? R(1) : R(0)) - sr;
-      si = -si;
-      const R ur = D[0][i][i];
+    R sr = 0, si = 0;
+#pragma unroll
+    for (int t = 0; t < NQ; t++) {
+      const R ur = P[0][i][part + 4 * t];
       if constexpr (C) {
-        const R ui = D[NC - 1][i][i], den = ur * ur + ui * ui;
-        const R ivr = ur / den, ivi = -ui / den;
-        X[0][i][j] = sr * ivr - si * ivi;
-        X[NC - 1][i][j] = sr * ivi + si * ivr;
+        const R ui = P[NC - 1][i][part + 4 * t];
+        sr += ur * xr[t] - ui * xi[t];
+        si += ur * xi[t] + ui * xr[t];
       } else {
-        X[0][i][j] = sr / ur;
+        sr += ur * xr[t];
       }
-    } else if (part == 0) {
-      X[0][i][j] = 0;
-      if constexpr (C) X[NC - 1][i][j] = 0;
+    }
+    sr = lane_sum4(sr);
+    if constexpr (C) si = lane_sum4(si);
+    const R br = (i == j ? R(1) : R(0)) - sr, bi = -si;
+    R nr, ni = 0;
+    if constexpr (C) {
+      const R vr = Vd[0][i], vi = Vd[NC - 1][i];
+      nr = br * vr - bi * vi;
+      ni = br * vi + bi * vr;
+    } else {
+      nr = br * Vd[0][i];
+    }
+    const int ti = part == (i & 3) ? i >> 2 : -1;
+#pragma unroll
+    for (int t = 0; t < NQ; t++) {
+      xr[t] = t == ti ? nr : xr[t];
+      xi[t] = t == ti ? ni : xi[t];
     }
   }
-  __syncthreads();
-  const gptr<R> Ui = G(S.dinv) + (int64_t)K * 2 * TB * TB * NC;
-  for (int e = tid; e < TB * TB; e += NT) {
-    Ui[(int64_t)e * NC] = X[0][e / TB][e % TB];
-    if constexpr (C) Ui[(int64_t)e * NC + 1] = X[NC - 1][e / TB][e % TB];
+  const gptr<R> Ui = dinv_tile<R, C>(S, K, D_UINV);
+#pragma unroll
+  for (int t = 0; t < NQ; t++) {
+    const int64_t e = (int64_t)((part + 4 * t) * TB + j) * NC;
+    Ui[e] = xr[t];
+    if constexpr (C) Ui[e + 1] = xi[t];
   }
-  __syncthreads();
   // X = L_D^-1 (unit lower): L X = I, column j from its diagonal down.
+#pragma unroll
+  for (int t = 0; t < NQ; t++) xr[t] = xi[t] = 0;
+#pragma unroll 1
   for (int i = 0; i < TB; ++i) {
-    if (i >= j) {
-      R sr = 0, si = 0;
-      for (int k = j + part; k < i; k += 4) {
-        const R lr = D[0][i][k], xr = X[0][k][j];
-        if constexpr (C) {
-          const R li = D[NC - 1][i][k], xi = X[NC - 1][k][j];
-          sr += lr * xr - li * xi;
-          si += lr * xi + li * xr;
-        } else {
-          sr += lr * xr;
-        }
+    R sr = 0, si = 0;
+#pragma unroll
+    for (int t = 0; t < NQ; t++) {
+      const R lr = P[0][i][part + 4 * t];
+      if constexpr (C) {
+        const R li = P[NC - 1][i][part + 4 * t];
+        sr += lr * xr[t] - li * xi[t];
+        si += lr * xi[t] + li * xr[t];
+      } else {
+        sr += lr * xr[t];
       }
-      sr = lane_sum4(sr);
-      if constexpr (C) si = lane_sum4(si);
-      X[0][i][j] = (i == j ? R(1) : R(0)) - sr;
-      if constexpr (C) X[NC - 1][i][j] = -si;
-    } else if (part == 0) {
-      X[0][i][j] = 0;
-      if constexpr (C) X[NC - 1][i][j] = 0;
+    }
+    sr = lane_sum4(sr);
+    if constexpr (C) si = lane_sum4(si);
+    const R nr = (i == j ? R(1) : R(0)) - sr, ni = -si;
+    const int ti = part == (i & 3) ? i >> 2 : -1;
+#pragma unroll
+    for (int t = 0; t < NQ; t++) {
+      xr[t] = t == ti ? nr : xr[t];
+      xi[t] = t == ti ? ni : xi[t];
     }
   }
-  __syncthreads();
-  const gptr<R> Li = G(S.dinv) + ((int64_t)K * 2 + 1) * TB * TB * NC;
-  for (int e = tid; e < TB * TB; e += NT) {
-    Li[(int64_t)e * NC] = X[0][e / TB][e % TB];
-    if constexpr (C) Li[(int64_t)e * NC + 1] = X[NC - 1][e / TB][e % TB];
+  const gptr<R> Li = dinv_tile<R, C>(S, K, D_LINV);
+#pragma unroll
+  for (int t = 0; t < NQ; t++) {
+    const int64_t e = (int64_t)((part + 4 * t) * TB + j) * NC;
+    Li[e] = xr[t];
+    if constexpr (C) Li[e + 1] = xi[t];
   }
 }
 
@@ -928,7 +998,7 @@ __global__ __launch_bounds__(NT) void k_solve_finish(const SysDev<R> *__restrict
   const int tid = threadIdx.x, r = tid >> 2, gq = tid & 3;
   const bool upper = (kind == S_UPPER || kind == S_UPPER_T);
   const bool trans = (kind == S_LOWER_T || kind == S_UPPER_T);
-  const gptr<const R> Dg = G(S.dinv) + ((int64_t)K * 2 + (upper ? 0 : 1)) * TB * TB * NC;
+  const gptr<const R> Dg = dinv_tile<R, C>(S, K, upper ? D_UINV : D_LINV);
   stage_tile<R, C, LD>(Dg, Ds);
 #pragma unroll
   for (int u = 0; u < 4; u++) {
@@ -1070,11 +1140,13 @@ __device__ __forceinline__ void stage_x(gptr<const R> X, int J, int rc0, int nrh
 
 // Task K of a persistent solve accumulates its dependencies oldest first,
 // each run of complete ones after a single acquire, so it follows the wave
-// of completed block steps.  Before waiting on the last one (the neighbour
-// K -/+ 1) it stages everything that does not depend on it (the neighbour's
-// tile, the diagonal inverse, r_K): the chain from one block step to the
-// next carries only the neighbour's x block, one 64x64 tile product, the
-// diagonal product and the publication.
+// of completed block steps.  The last dependency is the neighbour nb = K -/+ 1.
+// For the forward substitutions everything else is done before waiting on it:
+// y_K = D^-1 (r_K - sum_old T(K,J) x_J) and the staging of the neighbour
+// product M(K) = D^-1 T(K,nb) (MU / ML, formed during the factorisation), so
+// the chain from one block step to the next carries only x_nb, one 64x64
+// tile product x_K = y_K - M(K) x_nb and the publication.  The transposed
+// substitutions stage T(nb,K)^T and D^-1 before the wait and apply both after.
 template <class R, bool C>
 __global__ __launch_bounds__(NT) void k_solve_persist(const SysDev<R> *__restrict__ sys, SolveSet ss, int kind,
                                                       int *counter, int *err) {
@@ -1122,7 +1194,7 @@ __global__ __launch_bounds__(NT) void k_solve_persist(const SysDev<R> *__restric
     const int nb = asc ? je - 1 : jb;
     const int ob = has_nb ? (asc ? jb : jb + 1) : 0, oe = has_nb ? (asc ? je - 1 : je) : 0;
     auto tile = [&](int J) { return trans ? tile_ptr<R, C>(p, S.tiles, J, K) : tile_ptr<R, C>(p, S.tiles, K, J); };
-    const gptr<const R> Dg = G(S.dinv) + ((int64_t)K * 2 + (upper ? 0 : 1)) * TB * TB * NC;
+    const gptr<const R> Dg = dinv_tile<R, C>(S, K, upper ? D_UINV : D_LINV);
     for (int rc0 = 0; rc0 < nrhs; rc0 += RC) {
       R accr[4] = {0, 0, 0, 0}, acci[4] = {0, 0, 0, 0};
       const int nold = oe - ob, base = asc ? ob : oe - 1, dir = asc ? 1 : -1;
@@ -1145,7 +1217,7 @@ __global__ __launch_bounds__(NT) void k_solve_persist(const SysDev<R> *__restric
         q += run;
       }
       // everything independent of the neighbour
-      if (has_nb) stage_tile<R, C, LD>(tile(nb), Ts);
+      if (has_nb) stage_tile<R, C, LD>(trans ? tile(nb) : dinv_tile<R, C>(S, K, upper ? D_MU : D_ML), Ts);
       stage_tile<R, C, LD>(Dg, Ds);
       R sr[4] = {0, 0, 0, 0}, si[4] = {0, 0, 0, 0};
 #pragma unroll
@@ -1156,6 +1228,41 @@ __global__ __launch_bounds__(NT) void k_solve_persist(const SysDev<R> *__restric
           sr[u] = G(S.X)[gi];
           if constexpr (C) si[u] = G(S.X)[gi + 1];
         }
+      }
+      if (!trans) {
+        // y_K = D^-1 (r_K - acc), then x_K = y_K - M(K) x_nb
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          Xs[0][r][gq * 4 + u] = sr[u] - accr[u];
+          if constexpr (C) Xs[NC - 1][r][gq * 4 + u] = si[u] - acci[u];
+        }
+        __syncthreads();
+        R yr[4] = {0, 0, 0, 0}, yi[4] = {0, 0, 0, 0};
+        tile_mac<R, C, LD>(Ds, Xs, false, r, gq, yr, yi);
+        if (has_nb) {
+          if (tid < 64) (void)ready_run(S.done, nb, 1, 0, 1, err);
+          __syncthreads();
+          stage_x<R, C>(G(S.X), nb, rc0, nrhs, Xs);
+          __syncthreads();
+          R zr[4] = {0, 0, 0, 0}, zi[4] = {0, 0, 0, 0};
+          tile_mac<R, C, LD>(Ts, Xs, false, r, gq, zr, zi);
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            yr[u] -= zr[u];
+            yi[u] -= zi[u];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const int cc = gq * 4 + u;
+          if (rc0 + cc < nrhs) {
+            const int64_t gi = ((int64_t)(K * TB + r) * nrhs + rc0 + cc) * NC;
+            G(S.X)[gi] = yr[u];
+            if constexpr (C) G(S.X)[gi + 1] = yi[u];
+          }
+        }
+        __syncthreads();
+        continue;
       }
       if (has_nb) {
         if (tid < 64) (void)ready_run(S.done, nb, 1, 0, 1, err);

@@ -133,7 +133,7 @@ int Group<R, C>::add(Plan *plan, int nrhs, R *X) {
   mb.nrhs = nrhs;
   const size_t te = (size_t)TB * TB * NC;
   FDM_HIP(hipMalloc(&mb.tiles, sizeof(R) * te * (size_t)plan->ntiles));
-  FDM_HIP(hipMalloc(&mb.dinv, sizeof(R) * te * 2 * (size_t)plan->NB));
+  FDM_HIP(hipMalloc(&mb.dinv, sizeof(R) * te * DSLOTS * (size_t)plan->NB));
   FDM_HIP(hipMalloc(&mb.fail, sizeof(int)));
   FDM_HIP(hipMalloc(&mb.done, sizeof(int) * plan->NB));
   if (X) {
@@ -196,7 +196,7 @@ void Group<R, C>::release() {
 template <class R, bool C>
 size_t Group<R, C>::factor_bytes() const {
   size_t s = 0;
-  for (auto &mb : m) s += sizeof(R) * (size_t)TB * TB * NC * ((size_t)mb.plan->ntiles + 2 * (size_t)mb.plan->NB);
+  for (auto &mb : m) s += sizeof(R) * (size_t)TB * TB * NC * ((size_t)mb.plan->ntiles + DSLOTS * (size_t)mb.plan->NB);
   return s;
 }
 
@@ -296,8 +296,11 @@ int Group<R, C>::factor() {
       const int K = pl.NB - 1 - t;
       const int nJ = K - pl.J0[K], nI = K - pl.I0[K];
       push(ip, g, K, 1);
-      if (nJ > 0) push(itr, g, K, nJ);
-      if (nI > 0) push(itc, g, K, nI);
+      // one more workgroup each for the substitutions' neighbour products
+      // MU(K) (tile (K,K+1) exists) and ML(K) (tile (K,K-1) exists)
+      const int mu = K + 1 <= pl.J1[K] ? 1 : 0, ml = K - 1 >= pl.J0[K] ? 1 : 0;
+      if (nJ + mu > 0) push(itr, g, K, nJ + mu);
+      if (nI + ml > 0) push(itc, g, K, nI + ml);
       if (gk <= 0) {
         if (nI > 0 && nJ > 0) push(itu, g, K, nI * nJ);
         continue;
