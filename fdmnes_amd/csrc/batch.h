@@ -22,6 +22,8 @@ struct Slot {
   double *Xd = nullptr;        // device, FP64 solution (refinement)
   double *Rr = nullptr;        // device, FP64 residual (refinement)
   double *scr = nullptr;       // device, norm partials
+  int *long_rows = nullptr;    // device, rows with more than LONG_ROW entries
+  int nlong = 0;
   double factor_macs = 0, solve_macs = 0;
 };
 

@@ -53,6 +53,7 @@ Solver<R, C>::~Solver() {
   grp.release();
   for (auto &s : slots) {
     if (s.row_ptr) (void)hipFree(s.row_ptr);
+    if (s.long_rows) (void)hipFree(s.long_rows);
     if (s.col) (void)hipFree(s.col);
     if (s.vals) (void)hipFree(s.vals);
     if (s.B) (void)hipFree(s.B);
@@ -113,6 +114,14 @@ int Solver<R, C>::add_system(int n, const int *lb1, const int *lb2, const int64_
   FDM_HIP(hipMemcpy(sl.row_ptr, row_ptr, sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice));
   FDM_HIP(hipMemcpy(sl.col, col, sizeof(int) * sl.nnz, hipMemcpyHostToDevice));
   FDM_HIP(hipMemset(sl.B, 0, sizeof(double) * xe));
+  std::vector<int> lr;
+  for (int i = 0; i < n; i++)
+    if (row_ptr[i + 1] - row_ptr[i] > LONG_ROW) lr.push_back(i);
+  sl.nlong = (int)lr.size();
+  if (sl.nlong) {
+    FDM_HIP(hipMalloc(&sl.long_rows, sizeof(int) * lr.size()));
+    FDM_HIP(hipMemcpy(sl.long_rows, lr.data(), sizeof(int) * lr.size(), hipMemcpyHostToDevice));
+  }
   if (std::is_same<R, float>::value) {
     FDM_HIP(hipMalloc(&sl.Xd, sizeof(double) * xe));
     FDM_HIP(hipMalloc(&sl.Rr, sizeof(double) * xe));
@@ -206,7 +215,7 @@ int Solver<R, C>::sweep_phase(uint32_t mask) {
       if (!grp.active[g]) continue;
       Slot &sl = slots[g];
       const size_t tot = (size_t)sl.plan->NB * TB * nrhs * NC;
-      DevCsr a{sl.n, sl.nnz, sl.row_ptr, sl.col, sl.vals};
+      DevCsr a{sl.n, sl.nnz, sl.row_ptr, sl.col, sl.vals, sl.long_rows, sl.nlong};
       if (accumulate<R>(sl.Xd, grp.m[g].X, tot, stream) || residual<C>(a, sl.B, sl.Xd, sl.Rr, nrhs, stream) ||
           launch_norms2<C>(sl.Rr, sl.n, nrhs, sl.scr, stream))
         return -2;

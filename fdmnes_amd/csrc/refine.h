@@ -11,7 +11,12 @@ struct DevCsr {
   const int64_t *row_ptr = nullptr;
   const int *col = nullptr;
   const double *vals = nullptr;  // interleaved (re, im) when complex
+  const int *long_rows = nullptr;  // rows longer than LONG_ROW entries (the outer-sphere rows)
+  int nlong = 0;
 };
+
+// Rows with more entries than this are summed by a whole workgroup.
+constexpr int LONG_ROW = 256;
 
 // R = B - A X over nrhs right-hand sides, all rows*nrhs interleaved when C.
 template <bool C>

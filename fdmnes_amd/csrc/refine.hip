@@ -7,7 +7,8 @@
 
 namespace fdmgpu {
 
-// One thread per (row, right-hand side): r = b - sum_q a_q x_{col_q}.
+// One thread per (row, right-hand side): r = b - sum_q a_q x_{col_q}.  Rows
+// longer than LONG_ROW are left to k_residual_long.
 template <bool C>
 __global__ void k_residual(int n, const int64_t *__restrict__ rp, const int *__restrict__ col,
                            const double *__restrict__ av, const double *__restrict__ B,
@@ -16,6 +17,7 @@ __global__ void k_residual(int n, const int64_t *__restrict__ rp, const int *__r
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (int64_t)n * nrhs) return;
   const int row = (int)(t / nrhs), l = (int)(t % nrhs);
+  if (rp[row + 1] - rp[row] > LONG_ROW) return;
   double sr = B[t * NC], si = C ? B[t * NC + 1] : 0.0;
   for (int64_t q = rp[row]; q < rp[row + 1]; q++) {
     const int64_t xi = ((int64_t)(col[q] - 1) * nrhs + l) * NC;
@@ -30,6 +32,50 @@ __global__ void k_residual(int n, const int64_t *__restrict__ rp, const int *__r
   }
   R[t * NC] = sr;
   if constexpr (C) R[t * NC + 1] = si;
+}
+
+// Long rows (the dense outer-sphere rows): workgroup (row, chunk of 16
+// right-hand sides); thread (e, l) sums entries rp[row] + e, e + 64, ... for
+// right-hand side l, then the 64 partial sums of each l are added in LDS.
+constexpr int LONG_LANES = 64;
+template <bool C>
+__global__ __launch_bounds__(LONG_LANES * 16) void k_residual_long(const int *__restrict__ rows,
+                                                                 const int64_t *__restrict__ rp,
+                                                                 const int *__restrict__ col,
+                                                                 const double *__restrict__ av,
+                                                                 const double *__restrict__ B,
+                                                                 const double *__restrict__ X,
+                                                                 double *__restrict__ R, int nrhs) {
+  constexpr int NC = C ? 2 : 1;
+  __shared__ double sh[NC][LONG_LANES][17];
+  const int row = rows[blockIdx.x];
+  const int l = threadIdx.x & 15, e = threadIdx.x >> 4, lg = blockIdx.y * 16 + l;
+  double sr = 0, si = 0;
+  if (lg < nrhs) {
+    for (int64_t q = rp[row] + e; q < rp[row + 1]; q += LONG_LANES) {
+      const int64_t xi = ((int64_t)(col[q] - 1) * nrhs + lg) * NC;
+      if constexpr (C) {
+        const double xr = X[xi], xim = X[xi + 1], ar = av[2 * q], ai = av[2 * q + 1];
+        sr += ar * xr - ai * xim;
+        si += ar * xim + ai * xr;
+      } else {
+        sr += av[q] * X[xi];
+      }
+    }
+  }
+  sh[0][e][l] = sr;
+  if constexpr (C) sh[NC - 1][e][l] = si;
+  __syncthreads();
+  if (e == 0 && lg < nrhs) {
+    double tr = 0, ti = 0;
+    for (int k = 0; k < LONG_LANES; k++) {
+      tr += sh[0][k][l];
+      if constexpr (C) ti += sh[NC - 1][k][l];
+    }
+    const int64_t t = ((int64_t)row * nrhs + lg) * NC;
+    R[t] = B[t] - tr;
+    if constexpr (C) R[t + 1] = B[t + 1] - ti;
+  }
 }
 
 // Partial squared norms per right-hand side: block b sums rows b, b+grid, ...
@@ -74,6 +120,9 @@ int residual(const DevCsr &a, const double *B, const double *X, double *R, int n
   const int64_t tot = (int64_t)a.n * nrhs;
   hipLaunchKernelGGL((k_residual<C>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, a.n, a.row_ptr, a.col,
                      a.vals, B, X, R, nrhs);
+  if (a.nlong > 0)
+    hipLaunchKernelGGL((k_residual_long<C>), dim3(a.nlong, (nrhs + 15) / 16), dim3(LONG_LANES * 16), 0, s,
+                       a.long_rows, a.row_ptr, a.col, a.vals, B, X, R, nrhs);
   FDM_HIP(hipGetLastError());
   return 0;
 }
