@@ -80,6 +80,25 @@ def _cpu_worker(wname, k, target_s):
                       f"rate scaled by that fraction")
 
 
+def dropin_rates(cl, lb1, lb2, V, s, w):
+    """Seconds per energy point through fdmgpu_mat_solve / fdmgpu_mat_solve_mixed
+    (host buffers in and out, device set-up per call), the Fortran drop-in's
+    path; the second of two calls is reported."""
+    from fdmnes_amd import configs as cfg
+    from fdmnes_amd import solver as SV
+    E = float(w.energies_Ry[len(w.energies_Ry) // 2])
+    rp, col, vals, _ = cl.assemble(V, E, w.eimag_Ry, cfg.ecinetic_out(E), True, False)
+    res = {}
+    for tag, fn in (("fp64", SV.mat_solve), ("mixed", SV.mat_solve_mixed)):
+        for _ in range(2):
+            t0 = time.time()
+            fn(lb1, lb2, rp, col, vals, s, True)
+            dt = time.time() - t0
+        res[f"dropin_{tag}_s_per_point"] = dt
+        log(f"drop-in {tag}: {dt:.2f} s per point (host buffers)")
+    return res
+
+
 def traffic_from_profile():
     """HBM bytes per k_trail launch measured by rocprofv3 PMC runs (FETCH_SIZE,
     WRITE_SIZE) of this benchmark, scaled by the same counters on a plain
@@ -129,6 +148,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-roofline", action="store_true", help="skip the per-launch timed pass (profiler runs)")
+    ap.add_argument("--dropin", action="store_true",
+                    help="also time the host-buffer drop-in entry points (PCIe and set-up included), one point each")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs shard energy points "
@@ -219,6 +240,8 @@ def main():
                     "max_rel_sigma_err": float(err.max()), "adjoint_over_forward": t_adj / t_fwd,
                     "adjoint_s_per_group": t_adj, "fp64_ms_per_step": 1e3 * t_fwd})
         del sv64, xs64, grads
+    if args.dropin:
+        out.update(dropin_rates(cl, lb1, lb2, V, s, w))
     cpu = None
     if not args.no_cpu:
         procs = args.cpu_procs or min(16, os.cpu_count() or 1)

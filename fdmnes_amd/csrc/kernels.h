@@ -1138,6 +1138,63 @@ __device__ __forceinline__ void stage_x(gptr<const R> X, int J, int rc0, int nrh
   }
 }
 
+// A tile and an x block (rows J*TB.., columns rc0..rc0+RC-1) held in registers
+// between their global load and their LDS store, so the next pair's loads are
+// in flight while the current pair is used.
+template <class R, bool C>
+struct TileRegs {
+  typedef R r2 __attribute__((ext_vector_type(2)));
+  static constexpr int PER = TB * TB / NT;
+  r2 v2[C ? PER : 1];
+  R v1[C ? 1 : PER];
+  __device__ __forceinline__ void load(gptr<const R> g) {
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      const int e = threadIdx.x + q * NT;
+      if constexpr (C) v2[q] = ((gptr<const r2>)g)[e];
+      else v1[q] = g[e];
+    }
+  }
+  template <int LD>
+  __device__ __forceinline__ void store(R (*s)[TB][LD]) const {
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      const int e = threadIdx.x + q * NT;
+      if constexpr (C) {
+        s[0][e / TB][e % TB] = v2[q].x;
+        s[1][e / TB][e % TB] = v2[q].y;
+      } else {
+        s[0][e / TB][e % TB] = v1[q];
+      }
+    }
+  }
+};
+
+template <class R, bool C>
+struct XRegs {
+  static constexpr int NC = Elem<R, C>::NC;
+  static constexpr int PER = TB * RC / NT;
+  R re[PER], im[PER];
+  __device__ __forceinline__ void load(gptr<const R> X, int J, int rc0, int nrhs) {
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      const int e = threadIdx.x + q * NT, rr = e / RC, cc = e % RC;
+      const int64_t gi = ((int64_t)(J * TB + rr) * nrhs + rc0 + cc) * NC;
+      const bool ok = rc0 + cc < nrhs;
+      re[q] = ok ? X[gi] : R(0);
+      im[q] = (C && ok) ? X[gi + NC - 1] : R(0);
+    }
+  }
+  __device__ __forceinline__ void store(R (*Xs)[TB][RC + 1]) const {
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      const int e = threadIdx.x + q * NT, rr = e / RC, cc = e % RC;
+      Xs[0][rr][cc] = re[q];
+      if constexpr (C) Xs[NC - 1][rr][cc] = im[q];
+    }
+  }
+};
+
 // Task K of a persistent solve accumulates its dependencies oldest first,
 // each run of complete ones after a single acquire, so it follows the wave
 // of completed block steps.  The last dependency is the neighbour nb = K -/+ 1.
@@ -1206,13 +1263,34 @@ __global__ __launch_bounds__(NT) void k_solve_persist(const SysDev<R> *__restric
         }
         __syncthreads();
         const int run = nrun;
-        for (int u = 0; u < run; u++) {
-          const int J = base + dir * (q + u);
-          stage_tile<R, C, LD>(tile(J), Ts);
-          stage_x<R, C>(G(S.X), J, rc0, nrhs, Xs);
-          __syncthreads();
-          tile_mac<R, C, LD>(Ts, Xs, trans, r, gq, accr, acci);
-          __syncthreads();
+        if constexpr (sizeof(R) == 4) {
+          // FP32: the next dependency's tile and x block load into registers
+          // while the current pair is multiplied
+          TileRegs<R, C> tr;
+          XRegs<R, C> xr;
+          tr.load(tile(base + dir * q));
+          xr.load(G(S.X), base + dir * q, rc0, nrhs);
+          for (int u = 0; u < run; u++) {
+            tr.store(Ts);
+            xr.store(Xs);
+            __syncthreads();
+            if (u + 1 < run) {
+              const int Jn = base + dir * (q + u + 1);
+              tr.load(tile(Jn));
+              xr.load(G(S.X), Jn, rc0, nrhs);
+            }
+            tile_mac<R, C, LD>(Ts, Xs, trans, r, gq, accr, acci);
+            __syncthreads();
+          }
+        } else {
+          for (int u = 0; u < run; u++) {
+            const int J = base + dir * (q + u);
+            stage_tile<R, C, LD>(tile(J), Ts);
+            stage_x<R, C>(G(S.X), J, rc0, nrhs, Xs);
+            __syncthreads();
+            tile_mac<R, C, LD>(Ts, Xs, trans, r, gq, accr, acci);
+            __syncthreads();
+          }
         }
         q += run;
       }

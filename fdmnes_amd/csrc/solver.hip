@@ -254,12 +254,15 @@ static int xcd_grid(int items, int persist_wg) {
   return (g + NXCD - 1) / NXCD * NXCD;
 }
 
-// Right-looking blocked UL from the last block row up.  With gk > 0 the
-// block steps are taken in outer groups of gk: each inner step K factorises
-// the diagonal tile, forms its L row and U column, and updates only the
-// tiles of the group's own block rows and columns ("cross"); the rest of the
-// trailing matrix receives the whole group's rank-(gk*TB) update in one pass
-// at the group's last step.
+// Blocked UL from the last block row up.  With gk > 0 the block steps are
+// taken in outer groups of gk.  Inside a group the update is left-looking:
+// just before step K factorises its diagonal tile, block row K (columns
+// J0(K)..K) and block column K (rows I0(K)..K-1) receive the contributions
+// of the group's earlier steps K+1..Khi in one launch ("cross"); step K then
+// forms its L row and U column.  The rest of the trailing matrix (rows and
+// columns below the group) receives the whole group's rank-(gk*TB) update in
+// one pass at the group's last step.  Without groups (gk = 0) every step
+// updates its whole trailing block right-looking.
 template <class R, bool C>
 int Group<R, C>::factor() {
   int maxNB = 0;
@@ -306,15 +309,20 @@ int Group<R, C>::factor() {
         continue;
       }
       const int Khi = pl.NB - 1 - gk * (t / gk), Klo = std::max(Khi - gk + 1, 0);
-      // rows of the group below K, every column of K's reach
-      const int ia = std::max(pl.I0[K], Klo);
-      push_trail(cross, g, K, 1, ia, K - ia, pl.J0[K], nJ);
-      // rows above the group, the group's columns below K
-      const int ja = std::max(pl.J0[K], Klo);
-      push_trail(cross, g, K, 1, pl.I0[K], std::min(Klo, K) - pl.I0[K], ja, K - ja);
+      if (K < Khi) {
+        // block row K, columns J0(K)..K (diagonal tile included), steps K+1..Khi
+        push_trail(cross, g, K + 1, Khi - K, K, 1, pl.J0[K], nJ + 1);
+        // block column K, rows I0(K)..K-1
+        push_trail(cross, g, K + 1, Khi - K, pl.I0[K], nI, K, 1);
+      }
       if (K == Klo)
         push_trail(trail, g, Klo, Khi - Klo + 1, pl.I0[Klo], Klo - pl.I0[Klo], pl.J0[Klo], Klo - pl.J0[Klo]);
     }
+    if (cross.count)
+      timed([&]() {
+        hipLaunchKernelGGL((k_trail<R, C>), dim3(xcd_grid(cross.off[cross.count], persist_wg)), dim3(NT), 0, stream,
+                           (const SysDev<R> *)d_sys, cross);
+      });
     hipLaunchKernelGGL((k_panel<R, C>), dim3(ip.count), dim3(NT), 0, stream, (const SysDev<R> *)d_sys, ip);
     if (itr.count)
       hipLaunchKernelGGL((k_tile_gemm<R, C>), dim3(std::min(itr.off[itr.count], persist_wg)), dim3(NT), 0, stream,
@@ -337,11 +345,6 @@ int Group<R, C>::factor() {
       timed([&]() {
         hipLaunchKernelGGL((k_tile_gemm<R, C>), dim3(std::min(itu.off[itu.count], persist_wg)), dim3(NT), 0, stream,
                            (const SysDev<R> *)d_sys, itu, (int)G_UPDATE);
-      });
-    if (cross.count)
-      timed([&]() {
-        hipLaunchKernelGGL((k_trail<R, C>), dim3(xcd_grid(cross.off[cross.count], persist_wg)), dim3(NT), 0, stream,
-                           (const SysDev<R> *)d_sys, cross);
       });
     if (trail.count)
       timed([&]() {
