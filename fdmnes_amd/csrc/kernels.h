@@ -1125,6 +1125,31 @@ __device__ __forceinline__ void tile_mac(const R (*Ts)[TB][LD], const R (*Xs)[TB
   }
 }
 
+// acc(64 x RC) += op(T)(64 x 64) X(64 x RC) on MFMA 16x16x4 (RC = 16): wave w
+// computes rows 16w .. 16w+15; lane l accumulates rows 16w + Mfma<R>::row(l/16, v),
+// column l%16.  op(T) = T, or T^T when trans.  Complex: four real MFMAs.
+template <class R, bool C, int LD>
+__device__ __forceinline__ void tile_mfma(const R (*Ts)[TB][LD], const R (*Xs)[TB][RC + 1], bool trans,
+                                          typename Mfma<R>::v4 &accr, typename Mfma<R>::v4 &acci) {
+  static_assert(RC == 16, "one 16-column MFMA block per wave");
+  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+  const int row = (threadIdx.x >> 6) * 16 + li;
+#pragma unroll 4
+  for (int k4 = 0; k4 < TB; k4 += 4) {
+    const int k = k4 + lk;
+    const R ar = trans ? Ts[0][k][row] : Ts[0][row][k];
+    const R br = Xs[0][k][li];
+    accr = Mfma<R>::run(ar, br, accr);
+    if constexpr (C) {
+      const R ai = trans ? Ts[1][k][row] : Ts[1][row][k];
+      const R bi = Xs[1][k][li];
+      accr = Mfma<R>::run(-ai, bi, accr);
+      acci = Mfma<R>::run(ar, bi, acci);
+      acci = Mfma<R>::run(ai, br, acci);
+    }
+  }
+}
+
 // Stages rows J*TB .. J*TB+63, columns rc0 .. rc0+RC-1 of X (zero past nrhs).
 template <class R, bool C>
 __device__ __forceinline__ void stage_x(gptr<const R> X, int J, int rc0, int nrhs, R (*Xs)[TB][RC + 1]) {
@@ -1213,7 +1238,7 @@ __global__ __launch_bounds__(NT) void k_solve_persist(const SysDev<R> *__restric
   __shared__ R Ds[NC][TB][LD];
   __shared__ R Xs[NC][TB][RC + 1];
   __shared__ int tk, nrun;
-  const int tid = threadIdx.x, r = tid >> 2, gq = tid & 3;
+  const int tid = threadIdx.x;
   const bool trans = (kind == S_LOWER_T || kind == S_UPPER_T);
   const bool upper = (kind == S_UPPER || kind == S_UPPER_T);
   const bool asc = (kind == S_LOWER || kind == S_UPPER_T);
@@ -1252,8 +1277,16 @@ __global__ __launch_bounds__(NT) void k_solve_persist(const SysDev<R> *__restric
     const int ob = has_nb ? (asc ? jb : jb + 1) : 0, oe = has_nb ? (asc ? je - 1 : je) : 0;
     auto tile = [&](int J) { return trans ? tile_ptr<R, C>(p, S.tiles, J, K) : tile_ptr<R, C>(p, S.tiles, K, J); };
     const gptr<const R> Dg = dinv_tile<R, C>(S, K, upper ? D_UINV : D_LINV);
+    // thread (wave w, lane l) holds rows rows(v) = 16w + Mfma<R>::row(l/16, v),
+    // v = 0..3, of right-hand side column rc0 + l%16 (the MFMA output layout)
+    typedef typename Mfma<R>::v4 v4;
+    const int lane = tid & 63, li = lane & 15, lk = lane >> 4, w16 = (tid >> 6) * 16;
+    auto xidx = [&](int v, int rc0) {
+      return ((int64_t)(K * TB + w16 + Mfma<R>::row(lk, v)) * nrhs + rc0 + li) * NC;
+    };
     for (int rc0 = 0; rc0 < nrhs; rc0 += RC) {
-      R accr[4] = {0, 0, 0, 0}, acci[4] = {0, 0, 0, 0};
+      v4 accr = {0, 0, 0, 0}, acci = {0, 0, 0, 0};
+      const bool colok = rc0 + li < nrhs;
       const int nold = oe - ob, base = asc ? ob : oe - 1, dir = asc ? 1 : -1;
       for (int q = 0; q < nold;) {
         if (tid < 64) {
@@ -1279,7 +1312,7 @@ __global__ __launch_bounds__(NT) void k_solve_persist(const SysDev<R> *__restric
               tr.load(tile(Jn));
               xr.load(G(S.X), Jn, rc0, nrhs);
             }
-            tile_mac<R, C, LD>(Ts, Xs, trans, r, gq, accr, acci);
+            tile_mfma<R, C, LD>(Ts, Xs, trans, accr, acci);
             __syncthreads();
           }
         } else {
@@ -1288,7 +1321,7 @@ __global__ __launch_bounds__(NT) void k_solve_persist(const SysDev<R> *__restric
             stage_tile<R, C, LD>(tile(J), Ts);
             stage_x<R, C>(G(S.X), J, rc0, nrhs, Xs);
             __syncthreads();
-            tile_mac<R, C, LD>(Ts, Xs, trans, r, gq, accr, acci);
+            tile_mfma<R, C, LD>(Ts, Xs, trans, accr, acci);
             __syncthreads();
           }
         }
@@ -1299,73 +1332,55 @@ __global__ __launch_bounds__(NT) void k_solve_persist(const SysDev<R> *__restric
       stage_tile<R, C, LD>(Dg, Ds);
       R sr[4] = {0, 0, 0, 0}, si[4] = {0, 0, 0, 0};
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
-        const int cc = gq * 4 + u;
-        const int64_t gi = ((int64_t)(K * TB + r) * nrhs + rc0 + cc) * NC;
-        if (rc0 + cc < nrhs) {
-          sr[u] = G(S.X)[gi];
-          if constexpr (C) si[u] = G(S.X)[gi + 1];
+      for (int v = 0; v < 4; v++) {
+        if (colok) {
+          sr[v] = G(S.X)[xidx(v, rc0)];
+          if constexpr (C) si[v] = G(S.X)[xidx(v, rc0) + 1];
         }
       }
+      v4 yr = {0, 0, 0, 0}, yi = {0, 0, 0, 0};
       if (!trans) {
         // y_K = D^-1 (r_K - acc), then x_K = y_K - M(K) x_nb
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
-          Xs[0][r][gq * 4 + u] = sr[u] - accr[u];
-          if constexpr (C) Xs[NC - 1][r][gq * 4 + u] = si[u] - acci[u];
+        for (int v = 0; v < 4; v++) {
+          Xs[0][w16 + Mfma<R>::row(lk, v)][li] = sr[v] - accr[v];
+          if constexpr (C) Xs[NC - 1][w16 + Mfma<R>::row(lk, v)][li] = si[v] - acci[v];
         }
         __syncthreads();
-        R yr[4] = {0, 0, 0, 0}, yi[4] = {0, 0, 0, 0};
-        tile_mac<R, C, LD>(Ds, Xs, false, r, gq, yr, yi);
+        tile_mfma<R, C, LD>(Ds, Xs, false, yr, yi);
         if (has_nb) {
           if (tid < 64) (void)ready_run(S.done, nb, 1, 0, 1, err);
           __syncthreads();
           stage_x<R, C>(G(S.X), nb, rc0, nrhs, Xs);
           __syncthreads();
-          R zr[4] = {0, 0, 0, 0}, zi[4] = {0, 0, 0, 0};
-          tile_mac<R, C, LD>(Ts, Xs, false, r, gq, zr, zi);
-#pragma unroll
-          for (int u = 0; u < 4; u++) {
-            yr[u] -= zr[u];
-            yi[u] -= zi[u];
-          }
+          v4 zr = {0, 0, 0, 0}, zi = {0, 0, 0, 0};
+          tile_mfma<R, C, LD>(Ts, Xs, false, zr, zi);
+          yr -= zr;
+          yi -= zi;
         }
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-          const int cc = gq * 4 + u;
-          if (rc0 + cc < nrhs) {
-            const int64_t gi = ((int64_t)(K * TB + r) * nrhs + rc0 + cc) * NC;
-            G(S.X)[gi] = yr[u];
-            if constexpr (C) G(S.X)[gi + 1] = yi[u];
-          }
+      } else {
+        if (has_nb) {
+          if (tid < 64) (void)ready_run(S.done, nb, 1, 0, 1, err);
+          __syncthreads();
+          stage_x<R, C>(G(S.X), nb, rc0, nrhs, Xs);
+          __syncthreads();
+          tile_mfma<R, C, LD>(Ts, Xs, trans, accr, acci);
         }
         __syncthreads();
-        continue;
-      }
-      if (has_nb) {
-        if (tid < 64) (void)ready_run(S.done, nb, 1, 0, 1, err);
-        __syncthreads();
-        stage_x<R, C>(G(S.X), nb, rc0, nrhs, Xs);
-        __syncthreads();
-        tile_mac<R, C, LD>(Ts, Xs, trans, r, gq, accr, acci);
-      }
-      __syncthreads();
-      // x_K = Dinv'(K) (r_K - acc)
+        // x_K = Dinv'(K) (r_K - acc)
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
-        Xs[0][r][gq * 4 + u] = sr[u] - accr[u];
-        if constexpr (C) Xs[NC - 1][r][gq * 4 + u] = si[u] - acci[u];
+        for (int v = 0; v < 4; v++) {
+          Xs[0][w16 + Mfma<R>::row(lk, v)][li] = sr[v] - accr[v];
+          if constexpr (C) Xs[NC - 1][w16 + Mfma<R>::row(lk, v)][li] = si[v] - acci[v];
+        }
+        __syncthreads();
+        tile_mfma<R, C, LD>(Ds, Xs, trans, yr, yi);
       }
-      __syncthreads();
-      R yr[4] = {0, 0, 0, 0}, yi[4] = {0, 0, 0, 0};
-      tile_mac<R, C, LD>(Ds, Xs, trans, r, gq, yr, yi);
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
-        const int cc = gq * 4 + u;
-        if (rc0 + cc < nrhs) {
-          const int64_t gi = ((int64_t)(K * TB + r) * nrhs + rc0 + cc) * NC;
-          G(S.X)[gi] = yr[u];
-          if constexpr (C) G(S.X)[gi + 1] = yi[u];
+      for (int v = 0; v < 4; v++) {
+        if (colok) {
+          G(S.X)[xidx(v, rc0)] = yr[v];
+          if constexpr (C) G(S.X)[xidx(v, rc0) + 1] = yi[v];
         }
       }
       __syncthreads();
