@@ -18,11 +18,8 @@ CANDIDATES = [
     {"gk": 8, "trail_wg": 768, "solve_wg": 512, "trail_tile": 64},
     {"gk": 8, "trail_wg": 1024, "solve_wg": 512, "trail_tile": 64},
     {"gk": 12, "trail_wg": 768, "solve_wg": 512, "trail_tile": 64},
-    {"gk": 8, "trail_wg": 256, "solve_wg": 512, "trail_tile": 128},
-    {"gk": 8, "trail_wg": 512, "solve_wg": 512, "trail_tile": 128},
-    {"gk": 12, "trail_wg": 256, "solve_wg": 512, "trail_tile": 128},
+    {"gk": 16, "trail_wg": 768, "solve_wg": 512, "trail_tile": 64},
     {"gk": 12, "trail_wg": 512, "solve_wg": 512, "trail_tile": 128},
-    {"gk": 16, "trail_wg": 512, "solve_wg": 512, "trail_tile": 128},
 ]
 KNOBS = ("gk", "trail_wg", "solve_wg", "trail_tile")
 
@@ -55,7 +52,7 @@ def tune(solver, slots, vals, b, x, lb1, lb2, precision: int, candidates=CANDIDA
         apply(solver, cache[k]["best"])
         return cache[k]
     results = []
-    # the FP64 macro-tile kernel spills (its accumulators do not fit the register file)
+    # the FP64 two-tile update spills (its accumulators do not fit the register file)
     candidates = [c for c in candidates if precision != 0 or c.get("trail_tile", 64) == 64]
     for c in candidates:
         apply(solver, c)

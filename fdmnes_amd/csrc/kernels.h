@@ -264,8 +264,8 @@ struct TrailItems {
   int j0[MAXT];
   int nj[MAXT];
   int ni[MAXT];
-  int off[MAXT + 1];   // tile offsets (k_trail)
-  int moff[MAXT + 1];  // 2 x 2 macro-tile offsets (k_trail2)
+  int off[MAXT + 1];   // tile offsets (one tile per workgroup)
+  int moff[MAXT + 1];  // offsets of 1 x 2 tile blocks (two tiles per workgroup)
 };
 
 __device__ __forceinline__ int find_trail(const TrailItems &it, int b) {
@@ -277,36 +277,51 @@ __device__ __forceinline__ int find_trail(const TrailItems &it, int b) {
 constexpr int KC = 16;  // k-chunk of the LDS pipeline
 
 // tile(I,J) -= sum_K tile(I,K) tile(K,J).  Persistent grid over the work
-// items of all systems; per tile the K dimension (nk*TB) streams through a
-// two-buffer LDS pipeline in chunks of KC, the next chunk's global loads in
-// flight while the current chunk feeds the MFMAs.  Four waves, each a 32x32
-// quadrant in 2x2 blocks of 16x16x4 MFMAs.  LDS holds A chunks k-major (As[k][row]) so the MFMA A operand reads
-// 16 consecutive rows per k.
+// items of all systems.  A workgroup updates TJ consecutive tiles of a block
+// row (a 64 x 64*TJ output: TJ = 1 or 2); the K dimension (m steps of 64)
+// streams through a two-buffer LDS pipeline in chunks of KC, the next chunks'
+// global loads in flight while the current chunk feeds the MFMAs.  Four waves,
+// each 32 rows x 32*TJ columns in 2 x 2*TJ blocks of 16x16x4 MFMAs.  LDS holds
+// A chunks k-major (As[k][row]) so the MFMA A operand reads 16 consecutive
+// rows per k.  With TJ = 2 every A chunk feeds twice the MFMA work, the
+// barriers per multiply-add halve, and the bytes loaded per multiply-add drop
+// by a quarter; at an item's last odd column the second tile is absent and its
+// B columns load as zero.  The steps of the range reaching a tile are a prefix
+// (I0, J0 non-decreasing in K); the workgroup runs the longest prefix of its
+// tiles, and B columns of a tile (K, J) that does not exist (J0(K) > J) load
+// as zero, so every tile receives exactly its own steps.
 //
 // Work order: dispatch places workgroup w on XCD w % NXCD, and each XCD has
 // its own L2.  The work list is cut into NXCD contiguous ranges, one per XCD
-// (the grid is a multiple of NXCD), and inside an item the tiles run in bands
-// of GM block rows, column by column, so the workgroups resident on one XCD
-// at a time cover a compact GM x (resident/GM) patch and share its U row
-// tiles and L column tiles in their L2 instead of each fetching them again.
+// (the grid is a multiple of NXCD), and inside an item the output blocks run
+// in bands of GM block rows, column by column, so the workgroups resident on
+// one XCD at a time cover a compact patch and share its U row tiles and L
+// column tiles in their L2 instead of each fetching them again.
 constexpr int NXCD = 8;
 constexpr int GM = 8;
-template <class R, bool C>
-__global__ __launch_bounds__(NT, sizeof(R) == 4 ? 3 : 2) void k_trail(const SysDev<R> *__restrict__ sys, TrailItems it) {
+template <class R, bool C, int TJ>
+__global__ __launch_bounds__(NT, (sizeof(R) == 4 && TJ == 1) ? 3 : 2) void k_trail(const SysDev<R> *__restrict__ sys, TrailItems it) {
+  static_assert(TJ == 1 || TJ == 2, "one or two tiles per workgroup");
   constexpr int NC = Elem<R, C>::NC;
-  // FP32: LDS rows of 80 words, so the four k rows an MFMA operand read
-  // touches (lanes 16*lk .. 16*lk+15 read row k4+lk) fall in four disjoint
-  // groups of 16 banks; A is stored with its row index XOR-ed by sw(k) (a
-  // multiple of 4 below 16), so the staging writes, 16 k values x 4 rows per
-  // wave, also hit 64 distinct banks.  FP64 keeps rows of 65 elements, which
-  // lets two workgroups share a CU's LDS.
+  constexpr int W = TB * TJ;  // output columns per workgroup
+  constexpr int NBJ = 2 * TJ;  // 16-column MFMA blocks per wave
+  // FP32: A rows of 80 words, so the four k rows an MFMA operand read touches
+  // (lanes 16*lk .. 16*lk+15 read row k4+lk) fall in four disjoint groups of
+  // 16 banks; A is stored with its row index XOR-ed by sw(k) (a multiple of 4
+  // below 16), so the staging writes, 16 k values x 4 rows per wave, also hit
+  // 64 distinct banks.  B: rows of 80 words (TJ = 1), or of 128 words with
+  // the column XOR-ed by swb(k) = 16*(k%4) (TJ = 2, which keeps three
+  // workgroups' LDS within a CU).  FP64 keeps rows of 65 elements.
   constexpr bool F32 = sizeof(R) == 4;
-  constexpr int LD = F32 ? TB + 16 : TB + 1;
-  constexpr int PER = TB * KC / NT;  // elements per thread per chunk and operand (4)
+  constexpr int LDA = F32 ? TB + 16 : TB + 1;
+  constexpr int LDB = F32 ? (TJ == 1 ? TB + 16 : W) : W + 1;
+  constexpr int PERA = TB * KC / NT;  // A elements per thread per chunk (4)
+  constexpr int PERB = W * KC / NT;   // B elements per thread per chunk (4 * TJ)
   auto sw = [](int k) { return F32 ? ((k & 15) >> 2) << 2 : 0; };
+  auto swb = [](int k) { return (F32 && TJ == 2) ? (k & 3) << 4 : 0; };
   constexpr int64_t TE = (int64_t)TB * TB * NC;  // elements per tile
-  __shared__ R As[2][NC][KC][LD];
-  __shared__ R Bs[2][NC][KC][LD];
+  __shared__ R As[2][NC][KC][LDA];
+  __shared__ R Bs[2][NC][KC][LDB];
   // per step t of the current item (K = Kb + t): I0(K), J0(K), tbase(K),
   // loaded once per item; the tiles of the item then need no global metadata
   // beyond tbase(I), J0(I) of their row
@@ -316,25 +331,29 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 3 : 2) void k_trail(const SysD
   typedef R r2 __attribute__((ext_vector_type(2)));
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
-  const int r0 = (wave >> 1) * 32, c0 = (wave & 1) * 32;
+  const int r0 = (wave >> 1) * 32, c0 = (wave & 1) * 32 * TJ;
   const int li = lane & 15, lk = lane >> 4;
   // FP32 complex products use three real MFMAs (Gauss: P1 = ar br,
   // P2 = ai bi, P3 = (ar + ai)(br + bi); re = P1 - P2, im = P3 - P1 - P2);
   // the FP64 path keeps four so its rounding matches an ordinary product.
   constexpr bool M3 = C && sizeof(R) == 4;
-  const int total = it.off[it.count];
+  const int total = TJ == 1 ? it.off[it.count] : it.moff[it.count];
   const int per = (total + NXCD - 1) / NXCD, xcd = blockIdx.x % NXCD, lanes = gridDim.x / NXCD;
   int gcur = -1;
   for (int jw = blockIdx.x / NXCD; jw < per; jw += lanes) {
     const int wb = xcd * per + jw;
     if (wb >= total) break;
-    const int g = find_trail(it, wb);
-    const int b = wb - it.off[g];
+    int g = 0;
+    if constexpr (TJ == 1) g = find_trail(it, wb);
+    else
+      while (g + 1 < it.count && wb >= it.moff[g + 1]) g++;
+    const int b = wb - (TJ == 1 ? it.off[g] : it.moff[g]);
     const SysDev<R> &S = sys[it.sys[g]];
     const DevPlan &p = S.p;
-    const int nj = it.nj[g], ni = (it.off[g + 1] - it.off[g]) / nj;
-    const int band = b / (GM * nj), hr = min(GM, ni - band * GM), q = b - band * GM * nj;
-    const int I = it.i0[g] + band * GM + q % hr, J = it.j0[g] + q / hr;
+    const int nj = it.nj[g], ni = it.ni[g], mj = (nj + TJ - 1) / TJ;
+    const int band = b / (GM * mj), hr = min(GM, ni - band * GM), q = b - band * GM * mj;
+    const int I = it.i0[g] + band * GM + q % hr, J = it.j0[g] + TJ * (q / hr);
+    const int cols_in = min(TJ, it.j0[g] + nj - J);  // tiles of this block present in the item
     const int Kb = it.Kb[g], nk = it.nk[g];
     // row metadata of A = tile(I, K) (consecutive tiles of block row I), in flight during the set-up
     const int64_t rowA = G(p.tbase)[I] + (Kb - G(p.J0)[I]);
@@ -348,112 +367,123 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 3 : 2) void k_trail(const SysD
       __syncthreads();
       gcur = g;
     }
-    // steps of the range reaching tile (I, J): a prefix, counted by a ballot
-    const bool reach = lane < nk && kI0[lane] <= I && kJ0[lane] <= J;
+    // steps of the range reaching the last tile of the block: a prefix, counted by a ballot
+    const bool reach = lane < nk && kI0[lane] <= I && kJ0[lane] <= J + cols_in - 1;
     const uint64_t rb = __ballot(reach);
     const int m = ~rb == 0 ? 64 : __builtin_ctzll(~rb);
     const int nch = m * (TB / KC);
     const gptr<const R> tiles = G(S.tiles);
     const gptr<const R> Ab = tiles + rowA * TE;
-    constexpr int NX = M3 ? 2 : 1;  // P2 accumulators only for the 3M form
-    v4 accr[2][2], acci[2][2], accx[NX][NX];
+    constexpr int NX = M3 ? NBJ : 1;  // P2 accumulators only for the 3M form
+    v4 accr[2][NBJ], acci[2][NBJ], accx[M3 ? 2 : 1][NX];
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
-      for (int bb = 0; bb < 2; bb++) {
+      for (int bb = 0; bb < NBJ; bb++) {
         accr[a][bb] = v4{0, 0, 0, 0};
         acci[a][bb] = v4{0, 0, 0, 0};
       }
 #pragma unroll
-    for (int a = 0; a < NX; a++)
+    for (int a = 0; a < (M3 ? 2 : 1); a++)
 #pragma unroll
       for (int bb = 0; bb < NX; bb++) accx[a][bb] = v4{0, 0, 0, 0};
-    // chunk c: step K = Kb + c/4, columns kc..kc+15 of A = tile(I,K), rows of B = tile(K,J).
-    // DEPTH register sets keep that many chunks' loads in flight while the
-    // MFMAs consume the LDS copy of the current one (two for FP32, whose
-    // register budget allows it; one for FP64).
-    constexpr int DEPTH = sizeof(R) == 4 ? 2 : 1;
-    r2 pa[DEPTH][PER], pb[DEPTH][PER];
-    R qa[DEPTH][PER], qb[DEPTH][PER];
+    // chunk c: step K = Kb + c/4, columns kc..kc+15 of A = tile(I,K), rows of
+    // B = tile(K,J..J+TJ-1).  DEPTH register sets keep that many chunks' loads
+    // in flight while the MFMAs consume the LDS copy of the current one (two
+    // for FP32 with one tile per workgroup, whose register budget allows it).
+    constexpr int DEPTH = (sizeof(R) == 4 && TJ == 1) ? 2 : 1;
+    r2 pa[DEPTH][PERA], pb[DEPTH][PERB];
+    R qa[DEPTH][PERA], qb[DEPTH][PERB];
     auto fetch = [&](int c, auto set) {
       constexpr int s = decltype(set)::value;
       const int st = c / (TB / KC), kc = (c % (TB / KC)) * KC;
       const gptr<const R> A = Ab + st * TE;
-      const gptr<const R> B = tiles + (ktb[st] + (J - kJ0[st])) * TE;
 #pragma unroll
-      for (int q = 0; q < PER; q++) {
+      for (int q = 0; q < PERA; q++) {
         const int e = tid + q * NT;
         const int ar = e / KC, ak = e % KC;  // A: 64 rows x 16 k
-        const int bk = e / TB, bc = e % TB;  // B: 16 k x 64 cols
-        if constexpr (C) {
-          pa[s][q] = ((gptr<const r2>)A)[ar * TB + kc + ak];
-          pb[s][q] = ((gptr<const r2>)B)[(kc + bk) * TB + bc];
-        } else {
-          qa[s][q] = A[ar * TB + kc + ak];
-          qb[s][q] = B[(kc + bk) * TB + bc];
-        }
+        if constexpr (C) pa[s][q] = ((gptr<const r2>)A)[ar * TB + kc + ak];
+        else qa[s][q] = A[ar * TB + kc + ak];
+      }
+#pragma unroll
+      for (int q = 0; q < PERB; q++) {
+        const int e = tid + q * NT;
+        const int bk = e / W, bc = e % W, tb = bc / TB;  // B: 16 k x W cols, tile J + tb
+        const bool ok = TJ == 1 || (tb < cols_in && kJ0[st] <= J + tb);
+        const gptr<const R> B = tiles + (ktb[st] + (J + tb - kJ0[st])) * TE;
+        if constexpr (C) pb[s][q] = ok ? ((gptr<const r2>)B)[(kc + bk) * TB + bc % TB] : r2{0, 0};
+        else qb[s][q] = ok ? B[(kc + bk) * TB + bc % TB] : R(0);
       }
     };
     auto step = [&](int c, auto set) {
       constexpr int s = decltype(set)::value;
       const int buf = c & 1;
 #pragma unroll
-      for (int q = 0; q < PER; q++) {
+      for (int q = 0; q < PERA; q++) {
         const int e = tid + q * NT;
-        const int ar = e / KC, ak = e % KC, bk = e / TB, bc = e % TB;
+        const int ar = e / KC, ak = e % KC;
         if constexpr (C) {
           As[buf][0][ak][ar ^ sw(ak)] = pa[s][q].x;
           As[buf][NC - 1][ak][ar ^ sw(ak)] = pa[s][q].y;
-          Bs[buf][0][bk][bc] = pb[s][q].x;
-          Bs[buf][NC - 1][bk][bc] = pb[s][q].y;
         } else {
           As[buf][0][ak][ar ^ sw(ak)] = qa[s][q];
-          Bs[buf][0][bk][bc] = qb[s][q];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < PERB; q++) {
+        const int e = tid + q * NT;
+        const int bk = e / W, bc = e % W;
+        if constexpr (C) {
+          Bs[buf][0][bk][bc ^ swb(bk)] = pb[s][q].x;
+          Bs[buf][NC - 1][bk][bc ^ swb(bk)] = pb[s][q].y;
+        } else {
+          Bs[buf][0][bk][bc ^ swb(bk)] = qb[s][q];
         }
       }
       __syncthreads();
       if (c + DEPTH < nch) fetch(c + DEPTH, set);
 #pragma unroll
-        for (int k4 = 0; k4 < KC; k4 += 4) {
-          R ar[2], ai[2], br[2], bi[2];
+      for (int k4 = 0; k4 < KC; k4 += 4) {
+        const int k = k4 + lk;
+        R ar[2], ai[2], br[NBJ], bi[NBJ];
 #pragma unroll
-          for (int a = 0; a < 2; a++) {
-            ar[a] = As[buf][0][k4 + lk][r0 + a * 16 + (li ^ sw(k4 + lk))];
-            if constexpr (C) ai[a] = As[buf][NC - 1][k4 + lk][r0 + a * 16 + (li ^ sw(k4 + lk))];
-          }
-#pragma unroll
-          for (int bb = 0; bb < 2; bb++) {
-            br[bb] = Bs[buf][0][k4 + lk][c0 + bb * 16 + li];
-            if constexpr (C) bi[bb] = Bs[buf][NC - 1][k4 + lk][c0 + bb * 16 + li];
-          }
-          if constexpr (M3) {
-            R as[2], bs[2];
-#pragma unroll
-            for (int a = 0; a < 2; a++) as[a] = ar[a] + ai[a];
-#pragma unroll
-            for (int bb = 0; bb < 2; bb++) bs[bb] = br[bb] + bi[bb];
-#pragma unroll
-            for (int a = 0; a < 2; a++)
-#pragma unroll
-              for (int bb = 0; bb < 2; bb++) {
-                accr[a][bb] = Mfma<R>::run(ar[a], br[bb], accr[a][bb]);   // P1
-                accx[a % NX][bb % NX] = Mfma<R>::run(ai[a], bi[bb], accx[a % NX][bb % NX]);  // P2
-                acci[a][bb] = Mfma<R>::run(as[a], bs[bb], acci[a][bb]);   // P3
-              }
-          } else {
-#pragma unroll
-            for (int a = 0; a < 2; a++)
-#pragma unroll
-              for (int bb = 0; bb < 2; bb++) {
-                accr[a][bb] = Mfma<R>::run(ar[a], br[bb], accr[a][bb]);
-                if constexpr (C) {
-                  accr[a][bb] = Mfma<R>::run(-ai[a], bi[bb], accr[a][bb]);
-                  acci[a][bb] = Mfma<R>::run(ar[a], bi[bb], acci[a][bb]);
-                  acci[a][bb] = Mfma<R>::run(ai[a], br[bb], acci[a][bb]);
-                }
-              }
-          }
+        for (int a = 0; a < 2; a++) {
+          ar[a] = As[buf][0][k][r0 + a * 16 + (li ^ sw(k))];
+          if constexpr (C) ai[a] = As[buf][NC - 1][k][r0 + a * 16 + (li ^ sw(k))];
         }
+#pragma unroll
+        for (int bb = 0; bb < NBJ; bb++) {
+          br[bb] = Bs[buf][0][k][(c0 + bb * 16 + li) ^ swb(k)];
+          if constexpr (C) bi[bb] = Bs[buf][NC - 1][k][(c0 + bb * 16 + li) ^ swb(k)];
+        }
+        if constexpr (M3) {
+          R as[2], bs[NBJ];
+#pragma unroll
+          for (int a = 0; a < 2; a++) as[a] = ar[a] + ai[a];
+#pragma unroll
+          for (int bb = 0; bb < NBJ; bb++) bs[bb] = br[bb] + bi[bb];
+#pragma unroll
+          for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int bb = 0; bb < NBJ; bb++) {
+              accr[a][bb] = Mfma<R>::run(ar[a], br[bb], accr[a][bb]);                       // P1
+              accx[a % 2][bb % NX] = Mfma<R>::run(ai[a], bi[bb], accx[a % 2][bb % NX]);  // P2
+              acci[a][bb] = Mfma<R>::run(as[a], bs[bb], acci[a][bb]);                       // P3
+            }
+        } else {
+#pragma unroll
+          for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int bb = 0; bb < NBJ; bb++) {
+              accr[a][bb] = Mfma<R>::run(ar[a], br[bb], accr[a][bb]);
+              if constexpr (C) {
+                accr[a][bb] = Mfma<R>::run(-ai[a], bi[bb], accr[a][bb]);
+                acci[a][bb] = Mfma<R>::run(ar[a], bi[bb], acci[a][bb]);
+                acci[a][bb] = Mfma<R>::run(ai[a], br[bb], acci[a][bb]);
+              }
+            }
+        }
+      }
     };
     using S0 = std::integral_constant<int, 0>;
     using S1 = std::integral_constant<int, DEPTH - 1>;
@@ -469,217 +499,25 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 3 : 2) void k_trail(const SysD
 #pragma unroll
       for (int a = 0; a < 2; a++)
 #pragma unroll
-        for (int bb = 0; bb < 2; bb++) {
-          acci[a][bb] = acci[a][bb] - accr[a][bb] - accx[a % NX][bb % NX];
-          accr[a][bb] = accr[a][bb] - accx[a % NX][bb % NX];
-        }
-    }
-    if (nch > 0) {
-      // all 16 loads of the read-modify-write first, then the stores
-      const gptr<R> Cg = tile_ptr<R, C>(p, S.tiles, I, J);
-      R cr[2][2][4], ci[2][2][4];
-#pragma unroll
-      for (int a = 0; a < 2; a++)
-#pragma unroll
-        for (int bb = 0; bb < 2; bb++)
-#pragma unroll
-          for (int v = 0; v < 4; v++) {
-            const int r = r0 + a * 16 + Mfma<R>::row(lk, v), cc = c0 + bb * 16 + li;
-            if constexpr (C) {
-              const r2 t = ((gptr<r2>)Cg)[r * TB + cc];
-              cr[a][bb][v] = t.x;
-              ci[a][bb][v] = t.y;
-            } else {
-              cr[a][bb][v] = Cg[r * TB + cc];
-            }
-          }
-#pragma unroll
-      for (int a = 0; a < 2; a++)
-#pragma unroll
-        for (int bb = 0; bb < 2; bb++)
-#pragma unroll
-          for (int v = 0; v < 4; v++) {
-            const int r = r0 + a * 16 + Mfma<R>::row(lk, v), cc = c0 + bb * 16 + li;
-            if constexpr (C)
-              ((gptr<r2>)Cg)[r * TB + cc] = r2{cr[a][bb][v] - accr[a][bb][v], ci[a][bb][v] - acci[a][bb][v]};
-            else
-              Cg[r * TB + cc] = cr[a][bb][v] - accr[a][bb][v];
-          }
-    }
-  }
-}
-
-// Macro-tiled trailing update: a workgroup of 8 waves updates a 2 x 2 block
-// of storage tiles (128 x 128 elements), so every U row chunk and L column
-// chunk staged in LDS feeds twice the MFMA work of the one-tile kernel and
-// the L2 traffic per flop halves.  Wave w owns the 32 x 64 patch at macro
-// rows 32*(w/2), columns 64*(w%2): 2 x 4 blocks of 16x16x4 MFMAs.  Items
-// count macro tiles (moff); at an item's last odd row or column the missing
-// storage tiles are treated as zero.  Within the macro tile the steps K of
-// the range that reach a storage tile (I0(K) <= I, J0(K) <= J) differ by
-// tile, so the chunk loads zero the rows of tiles (I+a, K) with I0(K) > I+a
-// and the columns of tiles (K, J+b) with J0(K) > J+b: the workgroup runs the
-// longest prefix and every tile receives exactly its own steps.
-constexpr int NT2 = 2 * NT;
-constexpr int GM2 = GM / 2;  // macro rows per band of the in-XCD order
-
-template <class R, bool C>
-__global__ __launch_bounds__(NT2, 1) void k_trail2(const SysDev<R> *__restrict__ sys, TrailItems it) {
-  constexpr int NC = Elem<R, C>::NC;
-  constexpr int MT = 2 * TB;
-  constexpr int LD = MT + 1;
-  constexpr int PER = MT * KC / NT2;  // elements per thread per chunk and operand (4)
-  __shared__ R As[2][NC][KC][LD];
-  __shared__ R Bs[2][NC][KC][LD];
-  typedef typename Mfma<R>::v4 v4;
-  typedef R r2 __attribute__((ext_vector_type(2)));
-  const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
-  const int r0 = (wave >> 1) * 32, c0 = (wave & 1) * 64;
-  const int li = lane & 15, lk = lane >> 4;
-  constexpr bool M3 = C && sizeof(R) == 4;
-  constexpr int NX = M3 ? 4 : 1;
-  const int total = it.moff[it.count];
-  const int per = (total + NXCD - 1) / NXCD, xcd = blockIdx.x % NXCD, lanes = gridDim.x / NXCD;
-  for (int jw = blockIdx.x / NXCD; jw < per; jw += lanes) {
-    const int wb = xcd * per + jw;
-    if (wb >= total) break;
-    int g = 0;
-    while (g + 1 < it.count && wb >= it.moff[g + 1]) g++;
-    const int b = wb - it.moff[g];
-    const SysDev<R> &S = sys[it.sys[g]];
-    const DevPlan &p = S.p;
-    const int ni = it.ni[g], nj = it.nj[g];
-    const int mi = (ni + 1) >> 1, mj = (nj + 1) >> 1;
-    const int band = b / (GM2 * mj), hr = min(GM2, mi - band * GM2), q = b - band * GM2 * mj;
-    const int ti = band * GM2 + q % hr, tj = q / hr;  // macro tile within the item
-    const int I = it.i0[g] + 2 * ti, J = it.j0[g] + 2 * tj;
-    const int rows_in = min(2, ni - 2 * ti), cols_in = min(2, nj - 2 * tj);
-    const int Il = I + rows_in - 1, Jl = J + cols_in - 1;  // last storage row / column present
-    int m = 0;
-    while (m < it.nk[g] && p.I0[it.Kb[g] + m] <= Il && p.J0[it.Kb[g] + m] <= Jl) m++;
-    const int nch = m * (TB / KC);
-    v4 accr[2][4], acci[2][4], accx[M3 ? 2 : 1][NX];
-#pragma unroll
-    for (int a = 0; a < 2; a++)
-#pragma unroll
-      for (int bb = 0; bb < 4; bb++) {
-        accr[a][bb] = v4{0, 0, 0, 0};
-        acci[a][bb] = v4{0, 0, 0, 0};
-      }
-#pragma unroll
-    for (int a = 0; a < (M3 ? 2 : 1); a++)
-#pragma unroll
-      for (int bb = 0; bb < NX; bb++) accx[a][bb] = v4{0, 0, 0, 0};
-    // chunk c: step K = Kb + c/4, k columns kc..kc+15.  Thread element e =
-    // tid + q*NT2: A row e/KC (storage tile I + row/64), B column e%MT
-    // (storage tile J + col/64).
-    r2 pa[PER], pb[PER];
-    R qa[PER], qb[PER];
-    auto fetch = [&](int c) {
-      const int K = it.Kb[g] + c / (TB / KC), kc = (c % (TB / KC)) * KC;
-#pragma unroll
-      for (int q = 0; q < PER; q++) {
-        const int e = tid + q * NT2;
-        const int ar = e / KC, ak = e % KC;
-        const int bk = e / MT, bc = e % MT;
-        const int ta = ar / TB, tb = bc / TB;
-        const bool oka = ta < rows_in && p.I0[K] <= I + ta;
-        const bool okb = tb < cols_in && p.J0[K] <= J + tb;
-        if constexpr (C) {
-          pa[q] = oka ? ((gptr<const r2>)tile_ptr<R, C>(p, S.tiles, I + ta, K))[(ar % TB) * TB + kc + ak]
-                      : r2{0, 0};
-          pb[q] = okb ? ((gptr<const r2>)tile_ptr<R, C>(p, S.tiles, K, J + tb))[(kc + bk) * TB + bc % TB]
-                      : r2{0, 0};
-        } else {
-          qa[q] = oka ? tile_ptr<R, C>(p, S.tiles, I + ta, K)[(ar % TB) * TB + kc + ak] : R(0);
-          qb[q] = okb ? tile_ptr<R, C>(p, S.tiles, K, J + tb)[(kc + bk) * TB + bc % TB] : R(0);
-        }
-      }
-    };
-    if (jw != (int)blockIdx.x / NXCD) __syncthreads();  // previous item done with LDS
-    if (nch > 0) fetch(0);
-    for (int c = 0; c < nch; c++) {
-      const int buf = c & 1;
-#pragma unroll
-      for (int q = 0; q < PER; q++) {
-        const int e = tid + q * NT2;
-        const int ar = e / KC, ak = e % KC, bk = e / MT, bc = e % MT;
-        if constexpr (C) {
-          As[buf][0][ak][ar] = pa[q].x;
-          As[buf][NC - 1][ak][ar] = pa[q].y;
-          Bs[buf][0][bk][bc] = pb[q].x;
-          Bs[buf][NC - 1][bk][bc] = pb[q].y;
-        } else {
-          As[buf][0][ak][ar] = qa[q];
-          Bs[buf][0][bk][bc] = qb[q];
-        }
-      }
-      __syncthreads();
-      if (c + 1 < nch) fetch(c + 1);
-#pragma unroll
-      for (int k4 = 0; k4 < KC; k4 += 4) {
-        R ar[2], ai[2], br[4], bi[4];
-#pragma unroll
-        for (int a = 0; a < 2; a++) {
-          ar[a] = As[buf][0][k4 + lk][r0 + a * 16 + li];
-          if constexpr (C) ai[a] = As[buf][NC - 1][k4 + lk][r0 + a * 16 + li];
-        }
-#pragma unroll
-        for (int bb = 0; bb < 4; bb++) {
-          br[bb] = Bs[buf][0][k4 + lk][c0 + bb * 16 + li];
-          if constexpr (C) bi[bb] = Bs[buf][NC - 1][k4 + lk][c0 + bb * 16 + li];
-        }
-        if constexpr (M3) {
-          R as[2], bs[4];
-#pragma unroll
-          for (int a = 0; a < 2; a++) as[a] = ar[a] + ai[a];
-#pragma unroll
-          for (int bb = 0; bb < 4; bb++) bs[bb] = br[bb] + bi[bb];
-#pragma unroll
-          for (int a = 0; a < 2; a++)
-#pragma unroll
-            for (int bb = 0; bb < 4; bb++) {
-              accr[a][bb] = Mfma<R>::run(ar[a], br[bb], accr[a][bb]);                           // P1
-              accx[a % 2][bb % NX] = Mfma<R>::run(ai[a], bi[bb], accx[a % 2][bb % NX]);  // P2
-              acci[a][bb] = Mfma<R>::run(as[a], bs[bb], acci[a][bb]);                           // P3
-            }
-        } else {
-#pragma unroll
-          for (int a = 0; a < 2; a++)
-#pragma unroll
-            for (int bb = 0; bb < 4; bb++) {
-              accr[a][bb] = Mfma<R>::run(ar[a], br[bb], accr[a][bb]);
-              if constexpr (C) {
-                accr[a][bb] = Mfma<R>::run(-ai[a], bi[bb], accr[a][bb]);
-                acci[a][bb] = Mfma<R>::run(ar[a], bi[bb], acci[a][bb]);
-                acci[a][bb] = Mfma<R>::run(ai[a], br[bb], acci[a][bb]);
-              }
-            }
-        }
-      }
-    }
-    if constexpr (M3) {
-#pragma unroll
-      for (int a = 0; a < 2; a++)
-#pragma unroll
-        for (int bb = 0; bb < 4; bb++) {
+        for (int bb = 0; bb < NBJ; bb++) {
           acci[a][bb] = acci[a][bb] - accr[a][bb] - accx[a % 2][bb % NX];
           accr[a][bb] = accr[a][bb] - accx[a % 2][bb % NX];
         }
     }
-    // this wave's patch lies in storage tile (I + r0/64, J + c0/64)
-    const int ta = r0 / TB, tb = c0 / TB;
-    if (nch > 0 && ta < rows_in && tb < cols_in) {
-      const gptr<R> Cg = tile_ptr<R, C>(p, S.tiles, I + ta, J + tb);
-      R cr[2][4][4], ci[2][4][4];
+    // this wave's columns c0 .. c0 + 32*TJ - 1 lie in tile J + c0/64
+    const int tb = c0 / TB;
+    if (nch > 0 && tb < cols_in) {
+      // all loads of the read-modify-write first, then the stores
+      const gptr<R> Cg = tile_ptr<R, C>(p, S.tiles, I, J + tb);
+      const int cl0 = c0 % TB;
+      R cr[2][NBJ][4], ci[2][NBJ][4];
 #pragma unroll
       for (int a = 0; a < 2; a++)
 #pragma unroll
-        for (int bb = 0; bb < 4; bb++)
+        for (int bb = 0; bb < NBJ; bb++)
 #pragma unroll
           for (int v = 0; v < 4; v++) {
-            const int r = (r0 % TB) + a * 16 + Mfma<R>::row(lk, v), cc = bb * 16 + li;
+            const int r = r0 + a * 16 + Mfma<R>::row(lk, v), cc = cl0 + bb * 16 + li;
             if constexpr (C) {
               const r2 t = ((gptr<r2>)Cg)[r * TB + cc];
               cr[a][bb][v] = t.x;
@@ -691,10 +529,10 @@ __global__ __launch_bounds__(NT2, 1) void k_trail2(const SysDev<R> *__restrict__
 #pragma unroll
       for (int a = 0; a < 2; a++)
 #pragma unroll
-        for (int bb = 0; bb < 4; bb++)
+        for (int bb = 0; bb < NBJ; bb++)
 #pragma unroll
           for (int v = 0; v < 4; v++) {
-            const int r = (r0 % TB) + a * 16 + Mfma<R>::row(lk, v), cc = bb * 16 + li;
+            const int r = r0 + a * 16 + Mfma<R>::row(lk, v), cc = cl0 + bb * 16 + li;
             if constexpr (C)
               ((gptr<r2>)Cg)[r * TB + cc] = r2{cr[a][bb][v] - accr[a][bb][v], ci[a][bb][v] - acci[a][bb][v]};
             else

@@ -71,7 +71,7 @@ struct Group {
   SysDev<R> *d_sys = nullptr;
   int persist_wg = 1024;     // workgroups of the persistent tile GEMMs
   int gk = 4;                // block steps per trailing update (0: one-step tile GEMM)
-  int trail_tile = 64;       // output tile edge of the group update: 64 (k_trail) or 128 (k_trail2)
+  int trail_tile = 64;       // output columns per workgroup of the group update: 64 or 128 (two tiles)
   bool persist_solve = true; // one dependency-driven launch per substitution
   int solve_wg = 512;        // workgroups of the persistent substitution
   int *ctl = nullptr;        // device: [0] task counter, [1] wait-timeout flag

@@ -243,7 +243,7 @@ inline void push_trail(TrailItems &it, int g, int Kb, int nk, int i0, int ni, in
   it.nj[c] = nj;
   it.ni[c] = ni;
   it.off[c + 1] = it.off[c] + ni * nj;
-  it.moff[c + 1] = it.moff[c] + ((ni + 1) / 2) * ((nj + 1) / 2);
+  it.moff[c + 1] = it.moff[c] + ni * ((nj + 1) / 2);
   it.count++;
 }
 }  // namespace
@@ -320,8 +320,8 @@ int Group<R, C>::factor() {
     }
     if (cross.count)
       timed([&]() {
-        hipLaunchKernelGGL((k_trail<R, C>), dim3(xcd_grid(cross.off[cross.count], persist_wg)), dim3(NT), 0, stream,
-                           (const SysDev<R> *)d_sys, cross);
+        hipLaunchKernelGGL((k_trail<R, C, 1>), dim3(xcd_grid(cross.off[cross.count], persist_wg)), dim3(NT), 0,
+                           stream, (const SysDev<R> *)d_sys, cross);
       });
     hipLaunchKernelGGL((k_panel<R, C>), dim3(ip.count), dim3(NT), 0, stream, (const SysDev<R> *)d_sys, ip);
     if (itr.count)
@@ -349,10 +349,10 @@ int Group<R, C>::factor() {
     if (trail.count)
       timed([&]() {
         if (trail_tile == 128)
-          hipLaunchKernelGGL((k_trail2<R, C>), dim3(xcd_grid(trail.moff[trail.count], persist_wg)), dim3(NT2), 0,
+          hipLaunchKernelGGL((k_trail<R, C, 2>), dim3(xcd_grid(trail.moff[trail.count], persist_wg)), dim3(NT), 0,
                              stream, (const SysDev<R> *)d_sys, trail);
         else
-          hipLaunchKernelGGL((k_trail<R, C>), dim3(xcd_grid(trail.off[trail.count], persist_wg)), dim3(NT), 0,
+          hipLaunchKernelGGL((k_trail<R, C, 1>), dim3(xcd_grid(trail.off[trail.count], persist_wg)), dim3(NT), 0,
                              stream, (const SysDev<R> *)d_sys, trail);
       });
   }

@@ -135,8 +135,8 @@ int fdmgpu_solver_options(fdmgpu_solver *s, int use_graphs, int time_updates, in
  * trailing update (effective panel width 64*gk, 0 = one-step updates),
  * workgroups of the persistent trailing-update grid, persistent (1) or
  * per-block (0) substitutions and their workgroup count, and the output
- * tile edge of the group update (64, or 128 for 2 x 2 macro tiles); <= 0
- * keeps a value.  Takes effect at the next solve (cached graphs are
+ * columns per workgroup of the group update (64, or 128: two tiles of a
+ * block row); <= 0 keeps a value.  Takes effect at the next solve (cached graphs are
  * rebuilt). */
 int fdmgpu_solver_tune(fdmgpu_solver *s, int gk, int trail_wg, int persist_solve, int solve_wg, int trail_tile);
 /* out[0] ms spent in trailing-update launches (when timed), out[1] their
