@@ -111,7 +111,7 @@ def traffic_from_profile():
         return None
 
 
-def roofline(sv32, slots32, vals, src, G):
+def roofline(sv32, slots32, vals, src, G, peak_key="fp32", kernel="k_trail<float,complex>"):
     import torch
     sv32.options(use_graphs=False, time_updates=True)
     grp = list(range(G))
@@ -125,8 +125,9 @@ def roofline(sv32, slots32, vals, src, G):
     share = st["tile_macs"] / (st["tile_macs"] + st["tile_other_macs"])
     alg_flops = 8.0 * st["profile_macs"] * share
     ach = alg_flops / upd_s / 1e12
-    roof = {"bound": "mfma", "achieved": ach, "peak": PEAK["fp32"], "unit": "TFLOP/s", "frac": ach / PEAK["fp32"],
-            "traffic": traffic_from_profile(), "kernel": "k_trail<float,complex> (trailing update: cross + group launches)",
+    roof = {"bound": "mfma", "achieved": ach, "peak": PEAK[peak_key], "unit": "TFLOP/s", "frac": ach / PEAK[peak_key],
+            "traffic": traffic_from_profile() if peak_key == "fp32" else None,
+            "kernel": f"{kernel} (trailing update: cross + group launches)",
             "launches": st["update_launches"], "avg_launch_us": 1e6 * upd_s / max(st["update_launches"], 1),
             "algorithmic_flops_per_launch": alg_flops / max(st["update_launches"], 1),
             "update_share_of_factorisation": share,
@@ -236,6 +237,9 @@ def main():
         torch.cuda.synchronize()
         t_adj = time.time() - a0
         t_fwd = wall64 / K
+        if not args.no_roofline:
+            r64 = roofline(sv64, slots64, vals, src, G, "fp64", "k_trail<double,complex>")
+            out["fp64_roofline"] = {k2: r64[k2] for k2 in ("achieved", "peak", "unit", "frac", "avg_launch_us")}
         out.update({"fp64_points_per_s": rate64, "mixed_over_fp64": rate32 / rate64,
                     "max_rel_sigma_err": float(err.max()), "adjoint_over_forward": t_adj / t_fwd,
                     "adjoint_s_per_group": t_adj, "fp64_ms_per_step": 1e3 * t_fwd})

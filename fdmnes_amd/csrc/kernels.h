@@ -442,6 +442,7 @@ __global__ __launch_bounds__(NT, (sizeof(R) == 4 && TJ == 1) ? 3 : 2) void k_tra
       }
       __syncthreads();
       if (c + DEPTH < nch) fetch(c + DEPTH, set);
+      __builtin_amdgcn_s_setprio(1);
 #pragma unroll
       for (int k4 = 0; k4 < KC; k4 += 4) {
         const int k = k4 + lk;
@@ -484,6 +485,7 @@ __global__ __launch_bounds__(NT, (sizeof(R) == 4 && TJ == 1) ? 3 : 2) void k_tra
             }
         }
       }
+      __builtin_amdgcn_s_setprio(0);
     };
     using S0 = std::integral_constant<int, 0>;
     using S1 = std::integral_constant<int, DEPTH - 1>;
