@@ -41,6 +41,8 @@ struct Solver {
   std::map<uint32_t, hipGraphExec_t> graphs;  // factorisation graphs by active-slot mask
   std::map<uint32_t, hipGraphExec_t> sweeps_g;  // refinement-sweep graphs by active-slot mask
   double *hnorm = nullptr;                       // pinned host copy of the norm partials
+  double *sig_buf = nullptr;                     // device: sigma partials and results
+  double *hsig = nullptr;                        // pinned host copy of the sigma results
   bool use_graphs = true;
   // refinement / fallback statistics of the last solve
   std::vector<int> last_sweeps;

@@ -8,26 +8,41 @@
 
 namespace fdmgpu {
 
-// sigma = -Im sum_i s_i x_i over n*nrhs entries (one workgroup per point,
-// fixed-order tree reduction: deterministic).
-template <bool C>
-__global__ __launch_bounds__(256) void k_sigma(const double *__restrict__ s, const double *__restrict__ x, size_t cnt,
-                                               double *__restrict__ out) {
-  __shared__ double sh[256];
-  double acc = 0;
-  for (size_t i = threadIdx.x; i < cnt; i += 256) {
-    if constexpr (C)
-      acc += s[2 * i] * x[2 * i + 1] + s[2 * i + 1] * x[2 * i];
-    else
-      acc += 0.0 * s[i] * x[i];
-  }
-  sh[threadIdx.x] = acc;
+// sigma = -Im sum_i s_i x_i over n*nrhs entries, in two deterministic
+// stages: workgroup b of SIG_PARTS sums a fixed contiguous range into
+// part[b] (fixed-order tree), then one workgroup adds the parts in order.
+constexpr int SIG_PARTS = 256;
+__device__ __forceinline__ double block_sum256(double v, double *sh) {
+  sh[threadIdx.x] = v;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) *out = -sh[0];
+  return sh[0];
+}
+
+template <bool C>
+__global__ __launch_bounds__(256) void k_sigma_part(const double *__restrict__ s, const double *__restrict__ x,
+                                                    size_t cnt, double *__restrict__ part) {
+  __shared__ double sh[256];
+  const size_t chunk = (cnt + SIG_PARTS - 1) / SIG_PARTS;
+  const size_t i0 = blockIdx.x * chunk, i1 = min(cnt, i0 + chunk);
+  double acc = 0;
+  for (size_t i = i0 + threadIdx.x; i < i1; i += 256) {
+    if constexpr (C)
+      acc += s[2 * i] * x[2 * i + 1] + s[2 * i + 1] * x[2 * i];
+    else
+      acc += 0.0 * s[i] * x[i];
+  }
+  const double t = block_sum256(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(256) void k_sigma_final(const double *__restrict__ part, double *__restrict__ out) {
+  __shared__ double sh[256];
+  const double t = block_sum256(part[threadIdx.x], sh);
+  if (threadIdx.x == 0) *out = -t;
 }
 
 // grad_i = Im sum_L lam[i][L] x[i][L]  (d sigma / d V_i for sigma = -Im s^T x,
@@ -50,6 +65,8 @@ Solver<R, C>::~Solver() {
   for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
   for (auto &kv : sweeps_g) (void)hipGraphExecDestroy(kv.second);
   if (hnorm) (void)hipHostFree(hnorm);
+  if (hsig) (void)hipHostFree(hsig);
+  if (sig_buf) (void)hipFree(sig_buf);
   grp.release();
   for (auto &s : slots) {
     if (s.row_ptr) (void)hipFree(s.row_ptr);
@@ -386,15 +403,24 @@ int Solver<R, C>::adjoint(int count, const int *ids, const double *const *src, c
 
 template <class R, bool C>
 int Solver<R, C>::sigma(int count, const int *ids, const double *const *src, const double *const *x, double *out) {
-  double *d = nullptr;
-  FDM_HIP(hipMalloc(&d, sizeof(double) * std::max(count, 1)));
+  if (count > MAXG) {
+    set_error("sigma: at most 16 points per call");
+    return -1;
+  }
+  // partials and results of up to MAXG points, allocated on first use
+  if (!sig_buf) FDM_HIP(hipMalloc(&sig_buf, sizeof(double) * MAXG * (SIG_PARTS + 1)));
+  if (!hsig) FDM_HIP(hipHostMalloc(&hsig, sizeof(double) * MAXG));
+  double *d = sig_buf + MAXG * SIG_PARTS;
   for (int k = 0; k < count; k++) {
     const int g = ids[k];
-    hipLaunchKernelGGL((k_sigma<C>), dim3(1), dim3(256), 0, stream, src[k], x[k], (size_t)slots[g].n * nrhs, d + k);
+    hipLaunchKernelGGL((k_sigma_part<C>), dim3(SIG_PARTS), dim3(256), 0, stream, src[k], x[k],
+                       (size_t)slots[g].n * nrhs, sig_buf + (size_t)k * SIG_PARTS);
+    hipLaunchKernelGGL(k_sigma_final, dim3(1), dim3(256), 0, stream, sig_buf + (size_t)k * SIG_PARTS, d + k);
   }
-  hipError_t e = hipMemcpyAsync(out, d, sizeof(double) * count, hipMemcpyDeviceToHost, stream);
+  hipError_t e = hipMemcpyAsync(hsig, d, sizeof(double) * count, hipMemcpyDeviceToHost, stream);
   if (e == hipSuccess) e = hipStreamSynchronize(stream);
-  (void)hipFree(d);
+  if (e == hipSuccess)
+    for (int k = 0; k < count; k++) out[k] = hsig[k];
   if (e != hipSuccess) {
     set_error(std::string("sigma: ") + hipGetErrorString(e));
     return -2;

@@ -137,10 +137,14 @@ __device__ __forceinline__ void stage_tile(P g, R (*s)[TB][LD]) {
   }
 }
 
+template <class R, bool C, int LD>
+__device__ void lower_inverse(const R (*P)[TB][LD], gptr<R> Li);
+
 // C(TB x TB) <- f(A * B), one output tile per work item; a persistent grid
 // strides over the work items of all grouped systems.
 //   G_UPDATE : tile(I,J) -= tile(I,K) * tile(K,J)   per item nI * nJ workgroups
-//   G_ROWTRSM: tile(K,J)  = Uinv(K)   * tile(K,J)   per item nJ          (L rows)
+//   G_ROWTRSM: tile(K,J)  = Uinv(K)   * tile(K,J)   per item nJ          (L rows),
+//              + MU(K) when tile (K,K+1) exists, + L_D^-1 of tile (K,K)
 //   G_COLTRSM: tile(I,K)  = tile(I,K) * Linv(K)     per item nI          (U columns)
 // Four waves; wave w owns the 32x32 quadrant (w/2, w%2), 2x2 MFMA blocks of
 // 16x16; a complex product is four real MFMAs (no 3M recombination).
@@ -169,6 +173,13 @@ __global__ __launch_bounds__(NT) void k_tile_gemm(const SysDev<R> *__restrict__ 
     B = tile_ptr<R, C>(p, S.tiles, K, J);
     Cg = tile_ptr<R, C>(p, S.tiles, I, J);
   } else if (mode == G_ROWTRSM) {
+    const int mu = K + 1 <= p.J1[K] ? 1 : 0;
+    if (b == nJ + mu) {  // last block of the item: L_D^-1 of the factored diagonal tile
+      stage_tile<R, C, LD>(tile_ptr<R, C>(p, S.tiles, K, K), As);
+      __syncthreads();
+      lower_inverse<R, C, LD>(As, dinv_tile<R, C>(S, K, D_LINV));
+      continue;
+    }
     A = dinv_tile<R, C>(S, K, D_UINV);
     if (b < nJ) {
       Cg = tile_ptr<R, C>(p, S.tiles, K, J0 + b);
@@ -547,7 +558,8 @@ __global__ __launch_bounds__(NT, (sizeof(R) == 4 && TJ == 1) ? 3 : 2) void k_tra
 // ---------------------------------------------------------------- panel
 // UL factorisation of the diagonal tile (K,K) without pivoting, from its last
 // row to its first as mat_solve_gaussian.f90:196-302 does row by row, then the
-// inverses U_D^-1 and L_D^-1 into dinv(K).  The pivot test is the reference's:
+// inverse U_D^-1 into dinv(K) (L_D^-1 is formed by the L-row launch that
+// follows, beside the L-row products, which need only U_D^-1).  The pivot test is the reference's:
 // |a|^2 < 1e-20 (complex) or |a| < 1e-10 (real); the largest failing row
 // (the first in elimination order) is recorded 1-based in *fail.  One
 // workgroup per item.
@@ -563,11 +575,80 @@ __global__ __launch_bounds__(NT, (sizeof(R) == 4 && TJ == 1) ? 3 : 2) void k_tra
 // four lanes per column j hold x_k (k = p + 4t) in registers; entries not yet
 // computed are zero, so each substitution step is a full 64-term dot product
 // with the published factor row, reduced across the four lanes by shuffles.
+// Lane moves within groups of four lanes through DPP quad permutations
+// (register to register; __shfl would go through LDS with ds_bpermute).
+template <int CTRL>
+__device__ __forceinline__ float quad_dpp(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double quad_dpp(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+  return __builtin_bit_cast(double, (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// Sum over the four lanes of a quad (every lane gets the sum).
 template <class R>
 __device__ __forceinline__ R lane_sum4(R v) {
-  v += __shfl_xor(v, 1);
-  v += __shfl_xor(v, 2);
+  v += quad_dpp<0xB1>(v);  // lanes 1,0,3,2
+  v += quad_dpp<0x4E>(v);  // lanes 2,3,0,1
   return v;
+}
+
+// Value of lane sel (0..3, the same in every lane) of each quad.
+template <class R>
+__device__ __forceinline__ R quad_bcast(R v, int sel) {
+  switch (sel) {
+    case 0: return quad_dpp<0x00>(v);
+    case 1: return quad_dpp<0x55>(v);
+    case 2: return quad_dpp<0xAA>(v);
+    default: return quad_dpp<0xFF>(v);
+  }
+}
+
+// X = L_D^-1 of the unit-lower part of a factored diagonal tile staged in
+// P (LDS), written to Li: four lanes per column j = tid/4 hold x_k
+// (k = p + 4t) in registers; column j from its diagonal down.
+template <class R, bool C, int LD>
+__device__ void lower_inverse(const R (*P)[TB][LD], gptr<R> Li) {
+  constexpr int NC = Elem<R, C>::NC;
+  constexpr int NQ = TB / 4;
+  const int tid = threadIdx.x, j = tid >> 2, part = tid & 3;
+  R xr[NQ], xi[NQ];
+#pragma unroll
+  for (int t = 0; t < NQ; t++) xr[t] = xi[t] = 0;
+#pragma unroll 1
+  for (int i = 0; i < TB; ++i) {
+    R sr = 0, si = 0;
+#pragma unroll
+    for (int t = 0; t < NQ; t++) {
+      const R lr = P[0][i][part + 4 * t];
+      if constexpr (C) {
+        const R li = P[NC - 1][i][part + 4 * t];
+        sr += lr * xr[t] - li * xi[t];
+        si += lr * xi[t] + li * xr[t];
+      } else {
+        sr += lr * xr[t];
+      }
+    }
+    sr = lane_sum4(sr);
+    if constexpr (C) si = lane_sum4(si);
+    const R nr = (i == j ? R(1) : R(0)) - sr, ni = -si;
+    const int ti = part == (i & 3) ? i >> 2 : -1;
+#pragma unroll
+    for (int t = 0; t < NQ; t++) {
+      xr[t] = t == ti ? nr : xr[t];
+      xi[t] = t == ti ? ni : xi[t];
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < NQ; t++) {
+    const int64_t e = (int64_t)((part + 4 * t) * TB + j) * NC;
+    Li[e] = xr[t];
+    if constexpr (C) Li[e + 1] = xi[t];
+  }
 }
 
 template <class R, bool C>
@@ -577,7 +658,7 @@ __global__ __launch_bounds__(NT) void k_panel(const SysDev<R> *__restrict__ sys,
   constexpr int NQ = TB / 4;
   __shared__ R P[NC][TB][LD];  // published rows, then the factored tile
   __shared__ R Vd[NC][TB];     // 1 / U_D(i,i)
-  const int tid = threadIdx.x, lane = tid & 63;
+  const int tid = threadIdx.x;
   const int myr = tid >> 2, part = tid & 3;
   const SysDev<R> &S = sys[it.sys[blockIdx.x]];
   const DevPlan &p = S.p;
@@ -592,15 +673,15 @@ __global__ __launch_bounds__(NT) void k_panel(const SysDev<R> *__restrict__ sys,
   }
 #pragma unroll 1
   for (int q = TB - 1; q >= 0; --q) {
-    const int src = (lane & ~3) | (q & 3), tq = q >> 2;
+    const int tq = q >> 2;
     R mr = 0, mi = 0;
 #pragma unroll
     for (int t = 0; t < NQ; t++) {
       mr = t == tq ? ar[t] : mr;
       mi = t == tq ? ai[t] : mi;
     }
-    const R fr = __shfl(mr, src);  // a_{myr,q}; the pivot in row q's group
-    const R fi = C ? __shfl(mi, src) : R(0);
+    const R fr = quad_bcast(mr, q & 3);  // a_{myr,q}; the pivot in row q's group
+    const R fi = C ? quad_bcast(mi, q & 3) : R(0);
     if (myr == q) {
       R vr, vi = 0;
       if constexpr (C) {
@@ -701,40 +782,6 @@ __global__ __launch_bounds__(NT) void k_panel(const SysDev<R> *__restrict__ sys,
     const int64_t e = (int64_t)((part + 4 * t) * TB + j) * NC;
     Ui[e] = xr[t];
     if constexpr (C) Ui[e + 1] = xi[t];
-  }
-  // X = L_D^-1 (unit lower): L X = I, column j from its diagonal down.
-#pragma unroll
-  for (int t = 0; t < NQ; t++) xr[t] = xi[t] = 0;
-#pragma unroll 1
-  for (int i = 0; i < TB; ++i) {
-    R sr = 0, si = 0;
-#pragma unroll
-    for (int t = 0; t < NQ; t++) {
-      const R lr = P[0][i][part + 4 * t];
-      if constexpr (C) {
-        const R li = P[NC - 1][i][part + 4 * t];
-        sr += lr * xr[t] - li * xi[t];
-        si += lr * xi[t] + li * xr[t];
-      } else {
-        sr += lr * xr[t];
-      }
-    }
-    sr = lane_sum4(sr);
-    if constexpr (C) si = lane_sum4(si);
-    const R nr = (i == j ? R(1) : R(0)) - sr, ni = -si;
-    const int ti = part == (i & 3) ? i >> 2 : -1;
-#pragma unroll
-    for (int t = 0; t < NQ; t++) {
-      xr[t] = t == ti ? nr : xr[t];
-      xi[t] = t == ti ? ni : xi[t];
-    }
-  }
-  const gptr<R> Li = dinv_tile<R, C>(S, K, D_LINV);
-#pragma unroll
-  for (int t = 0; t < NQ; t++) {
-    const int64_t e = (int64_t)((part + 4 * t) * TB + j) * NC;
-    Li[e] = xr[t];
-    if constexpr (C) Li[e + 1] = xi[t];
   }
 }
 

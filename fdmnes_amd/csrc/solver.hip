@@ -302,7 +302,7 @@ int Group<R, C>::factor() {
       // one more workgroup each for the substitutions' neighbour products
       // MU(K) (tile (K,K+1) exists) and ML(K) (tile (K,K-1) exists)
       const int mu = K + 1 <= pl.J1[K] ? 1 : 0, ml = K - 1 >= pl.J0[K] ? 1 : 0;
-      if (nJ + mu > 0) push(itr, g, K, nJ + mu);
+      push(itr, g, K, nJ + mu + 1);  // + one workgroup forming L_D^-1
       if (nI + ml > 0) push(itc, g, K, nI + ml);
       if (gk <= 0) {
         if (nI > 0 && nJ > 0) push(itu, g, K, nI * nJ);
