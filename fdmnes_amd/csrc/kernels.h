@@ -350,25 +350,47 @@ __global__ __launch_bounds__(NT, (sizeof(R) == 4 && TJ == 1) ? 3 : 2) void k_tra
   constexpr bool M3 = C && sizeof(R) == 4;
   const int total = TJ == 1 ? it.off[it.count] : it.moff[it.count];
   const int per = (total + NXCD - 1) / NXCD, xcd = blockIdx.x % NXCD, lanes = gridDim.x / NXCD;
-  int gcur = -1;
-  for (int jw = blockIdx.x / NXCD; jw < per; jw += lanes) {
-    const int wb = xcd * per + jw;
-    if (wb >= total) break;
+  // output block named by work index wb: item g, block row I, first tile
+  // column J, tiles present in the item; rowA = position of tile (I, Kb)
+  struct Blk {
+    int g, I, J, cols_in;
+    int64_t rowA;
+  };
+  auto decode = [&](int wb) {
+    Blk k;
     int g = 0;
     if constexpr (TJ == 1) g = find_trail(it, wb);
     else
       while (g + 1 < it.count && wb >= it.moff[g + 1]) g++;
     const int b = wb - (TJ == 1 ? it.off[g] : it.moff[g]);
-    const SysDev<R> &S = sys[it.sys[g]];
-    const DevPlan &p = S.p;
     const int nj = it.nj[g], ni = it.ni[g], mj = (nj + TJ - 1) / TJ;
     const int band = b / (GM * mj), hr = min(GM, ni - band * GM), q = b - band * GM * mj;
-    const int I = it.i0[g] + band * GM + q % hr, J = it.j0[g] + TJ * (q / hr);
-    const int cols_in = min(TJ, it.j0[g] + nj - J);  // tiles of this block present in the item
+    k.g = g;
+    k.I = it.i0[g] + band * GM + q % hr;
+    k.J = it.j0[g] + TJ * (q / hr);
+    k.cols_in = min(TJ, it.j0[g] + nj - k.J);
+    const DevPlan &pk = sys[it.sys[g]].p;
+    k.rowA = G(pk.tbase)[k.I] + (it.Kb[g] - G(pk.J0)[k.I]);
+    return k;
+  };
+  // DEPTH register sets keep that many chunks' loads in flight while the
+  // MFMAs consume the LDS copy of the current one (two for FP32 with one tile
+  // per workgroup, whose register budget allows it); they live across blocks
+  // because the next block's first chunks load during this block's epilogue
+  constexpr int DEPTH = (sizeof(R) == 4 && TJ == 1) ? 2 : 1;
+  r2 pa[DEPTH][PERA], pb[DEPTH][PERB];
+  R qa[DEPTH][PERA], qb[DEPTH][PERB];
+  int jw = blockIdx.x / NXCD;
+  if (jw >= per || xcd * per + jw >= total) return;
+  Blk cur = decode(xcd * per + jw);
+  int gcur = -1;
+  bool pre = false;  // the first chunks of cur are already in the prefetch registers
+  for (bool first = true;; first = false) {
+    const int g = cur.g, I = cur.I, Jc = cur.J, cin = cur.cols_in;
+    const SysDev<R> &S = sys[it.sys[g]];
+    const DevPlan &p = S.p;
     const int Kb = it.Kb[g], nk = it.nk[g];
-    // row metadata of A = tile(I, K) (consecutive tiles of block row I), in flight during the set-up
-    const int64_t rowA = G(p.tbase)[I] + (Kb - G(p.J0)[I]);
-    if (jw != (int)blockIdx.x / NXCD) __syncthreads();  // previous item done with LDS
+    if (!first) __syncthreads();  // previous block done with LDS
     if (g != gcur) {
       if (tid < nk) {
         kI0[tid] = G(p.I0)[Kb + tid];
@@ -378,13 +400,23 @@ __global__ __launch_bounds__(NT, (sizeof(R) == 4 && TJ == 1) ? 3 : 2) void k_tra
       __syncthreads();
       gcur = g;
     }
-    // steps of the range reaching the last tile of the block: a prefix, counted by a ballot
-    const bool reach = lane < nk && kI0[lane] <= I && kJ0[lane] <= J + cols_in - 1;
-    const uint64_t rb = __ballot(reach);
-    const int m = ~rb == 0 ? 64 : __builtin_ctzll(~rb);
+    // steps of the range reaching the last tile of a block: a prefix, counted by a ballot
+    auto steps = [&](int Ib, int Jb, int cb) {
+      const bool reach = lane < nk && kI0[lane] <= Ib && kJ0[lane] <= Jb + cb - 1;
+      const uint64_t rb = __ballot(reach);
+      return ~rb == 0 ? 64 : __builtin_ctzll(~rb);
+    };
+    const int m = steps(I, Jc, cin);
     const int nch = m * (TB / KC);
     const gptr<const R> tiles = G(S.tiles);
-    const gptr<const R> Ab = tiles + rowA * TE;
+    // the block the fetches below read: cur, then (before cur's epilogue) the next one
+    gptr<const R> Ab = tiles + cur.rowA * TE;
+    int J = Jc, cols_in = cin;
+    // the next block of this workgroup, its row metadata loading during cur's chunks
+    const int jn = jw + lanes;
+    const bool has_next = jn < per && xcd * per + jn < total;
+    Blk nxt{};
+    if (has_next) nxt = decode(xcd * per + jn);
     constexpr int NX = M3 ? NBJ : 1;  // P2 accumulators only for the 3M form
     v4 accr[2][NBJ], acci[2][NBJ], accx[M3 ? 2 : 1][NX];
 #pragma unroll
@@ -399,12 +431,7 @@ __global__ __launch_bounds__(NT, (sizeof(R) == 4 && TJ == 1) ? 3 : 2) void k_tra
 #pragma unroll
       for (int bb = 0; bb < NX; bb++) accx[a][bb] = v4{0, 0, 0, 0};
     // chunk c: step K = Kb + c/4, columns kc..kc+15 of A = tile(I,K), rows of
-    // B = tile(K,J..J+TJ-1).  DEPTH register sets keep that many chunks' loads
-    // in flight while the MFMAs consume the LDS copy of the current one (two
-    // for FP32 with one tile per workgroup, whose register budget allows it).
-    constexpr int DEPTH = (sizeof(R) == 4 && TJ == 1) ? 2 : 1;
-    r2 pa[DEPTH][PERA], pb[DEPTH][PERB];
-    R qa[DEPTH][PERA], qb[DEPTH][PERB];
+    // B = tile(K,J..J+TJ-1), loaded into the register sets declared above
     auto fetch = [&](int c, auto set) {
       constexpr int s = decltype(set)::value;
       const int st = c / (TB / KC), kc = (c % (TB / KC)) * KC;
@@ -500,9 +527,11 @@ __global__ __launch_bounds__(NT, (sizeof(R) == 4 && TJ == 1) ? 3 : 2) void k_tra
     };
     using S0 = std::integral_constant<int, 0>;
     using S1 = std::integral_constant<int, DEPTH - 1>;
-    if (nch > 0) fetch(0, S0{});
-    if constexpr (DEPTH == 2)
-      if (nch > 1) fetch(1, S1{});
+    if (!pre) {
+      if (nch > 0) fetch(0, S0{});
+      if constexpr (DEPTH == 2)
+        if (nch > 1) fetch(1, S1{});
+    }
     for (int c = 0; c < nch; c += DEPTH) {
       step(c, S0{});
       if constexpr (DEPTH == 2)
@@ -517,11 +546,26 @@ __global__ __launch_bounds__(NT, (sizeof(R) == 4 && TJ == 1) ? 3 : 2) void k_tra
           accr[a][bb] = accr[a][bb] - accx[a % 2][bb % NX];
         }
     }
-    // this wave's columns c0 .. c0 + 32*TJ - 1 lie in tile J + c0/64
+    // the next block's first chunks go in flight before this block's epilogue
+    // (same item: its step metadata is already in LDS)
+    pre = false;
+    if (has_next && nxt.g == g) {
+      const int mn = steps(nxt.I, nxt.J, nxt.cols_in);
+      if (mn > 0) {
+        Ab = tiles + nxt.rowA * TE;
+        J = nxt.J;
+        cols_in = nxt.cols_in;
+        fetch(0, S0{});
+        if constexpr (DEPTH == 2)
+          if (mn * (TB / KC) > 1) fetch(1, S1{});
+        pre = true;
+      }
+    }
+    // this wave's columns c0 .. c0 + 32*TJ - 1 lie in tile Jc + c0/64
     const int tb = c0 / TB;
-    if (nch > 0 && tb < cols_in) {
+    if (nch > 0 && tb < cin) {
       // all loads of the read-modify-write first, then the stores
-      const gptr<R> Cg = tile_ptr<R, C>(p, S.tiles, I, J + tb);
+      const gptr<R> Cg = tile_ptr<R, C>(p, S.tiles, I, Jc + tb);
       const int cl0 = c0 % TB;
       R cr[2][NBJ][4], ci[2][NBJ][4];
 #pragma unroll
@@ -552,6 +596,9 @@ __global__ __launch_bounds__(NT, (sizeof(R) == 4 && TJ == 1) ? 3 : 2) void k_tra
               Cg[r * TB + cc] = cr[a][bb][v] - accr[a][bb][v];
           }
     }
+    if (!has_next) break;
+    cur = nxt;
+    jw = jn;
   }
 }
 
