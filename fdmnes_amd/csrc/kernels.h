@@ -40,6 +40,22 @@ template <class T>
 __device__ __forceinline__ gptr<T> G(T *p) {
   return (gptr<T>)p;
 }
+// A pointer the caller knows to be the same in every lane, moved to scalar
+// registers: loads then take it as their scalar base plus a 32-bit lane
+// offset instead of a 64-bit address formed in every lane.
+template <class T>
+__device__ __forceinline__ gptr<T> uniform(gptr<T> p) {
+  const uint64_t v = (uint64_t)p;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return (gptr<T>)(((uint64_t)hi << 32) | lo);
+}
+// Element at byte offset off (32-bit, unsigned) from a scalar base: the form
+// global loads encode directly as base register pair + lane offset.
+template <class T, class P>
+__device__ __forceinline__ T load_at(gptr<const P> base, uint32_t off) {
+  return *(gptr<const T>)((gptr<const char>)base + off);
+}
 
 struct DevPlan {
   int n;   // real rows
@@ -409,14 +425,16 @@ __global__ __launch_bounds__(NT, (sizeof(R) == 4 && TJ == 1) ? 3 : 2) void k_tra
     const int m = steps(I, Jc, cin);
     const int nch = m * (TB / KC);
     const gptr<const R> tiles = G(S.tiles);
-    // the block the fetches below read: cur, then (before cur's epilogue) the next one
-    gptr<const R> Ab = tiles + cur.rowA * TE;
-    int J = Jc, cols_in = cin;
-    // the next block of this workgroup, its row metadata loading during cur's chunks
+    const gptr<const R> Ab = tiles + cur.rowA * TE;
+    // the next block of this workgroup; when it belongs to the same item (its
+    // step metadata already in LDS) its first chunks load during this block's
+    // last chunks
     const int jn = jw + lanes;
     const bool has_next = jn < per && xcd * per + jn < total;
     Blk nxt{};
     if (has_next) nxt = decode(xcd * per + jn);
+    const bool chain = has_next && nxt.g == g && steps(nxt.I, nxt.J, nxt.cols_in) > 0;
+    const gptr<const R> An = tiles + nxt.rowA * TE;
     constexpr int NX = M3 ? NBJ : 1;  // P2 accumulators only for the 3M form
     v4 accr[2][NBJ], acci[2][NBJ], accx[M3 ? 2 : 1][NX];
 #pragma unroll
@@ -431,31 +449,62 @@ __global__ __launch_bounds__(NT, (sizeof(R) == 4 && TJ == 1) ? 3 : 2) void k_tra
 #pragma unroll
       for (int bb = 0; bb < NX; bb++) accx[a][bb] = v4{0, 0, 0, 0};
     // chunk c: step K = Kb + c/4, columns kc..kc+15 of A = tile(I,K), rows of
-    // B = tile(K,J..J+TJ-1), loaded into the register sets declared above
-    auto fetch = [&](int c, auto set) {
+    // B = tile(K,J..J+TJ-1), loaded into the register sets declared above.
+    // src() forms the addresses (reading the step metadata in LDS), load()
+    // issues the global loads, so the loads can follow a barrier with no LDS
+    // wait in between.
+    struct Src {
+      gptr<const R> A, B[TJ];
+      uint32_t kc;  // unsigned lane offsets: loads take them as 32-bit offsets to the scalar base
+      bool ok[TJ];
+    };
+    // chunk c of the block whose tile (I, Kb) is at A0, output tiles J .. J + cols_in - 1
+    auto src = [&](gptr<const R> A0, int J, int cols_in, int c) {
+      Src x;
+      const int st = c / (TB / KC);
+      x.kc = (uint32_t)(c % (TB / KC)) * KC;
+      x.A = uniform(A0 + st * TE);
+#pragma unroll
+      for (int tb = 0; tb < TJ; tb++) {
+        x.ok[tb] = TJ == 1 || (tb < cols_in && kJ0[st] <= J + tb);
+        x.B[tb] = uniform(tiles + (ktb[st] + (J + tb - kJ0[st])) * TE);
+      }
+      return x;
+    };
+    auto load = [&](const Src &x, auto set) {
       constexpr int s = decltype(set)::value;
-      const int st = c / (TB / KC), kc = (c % (TB / KC)) * KC;
-      const gptr<const R> A = Ab + st * TE;
 #pragma unroll
       for (int q = 0; q < PERA; q++) {
         const int e = tid + q * NT;
-        const int ar = e / KC, ak = e % KC;  // A: 64 rows x 16 k
-        if constexpr (C) pa[s][q] = ((gptr<const r2>)A)[ar * TB + kc + ak];
-        else qa[s][q] = A[ar * TB + kc + ak];
+        const uint32_t ar = e / KC, ak = e % KC;  // A: 64 rows x 16 k
+        const uint32_t off = (ar * TB + x.kc + ak) * (uint32_t)(NC * sizeof(R));
+        if constexpr (C) pa[s][q] = load_at<r2>(x.A, off);
+        else qa[s][q] = load_at<R>(x.A, off);
       }
 #pragma unroll
       for (int q = 0; q < PERB; q++) {
         const int e = tid + q * NT;
-        const int bk = e / W, bc = e % W, tb = bc / TB;  // B: 16 k x W cols, tile J + tb
-        const bool ok = TJ == 1 || (tb < cols_in && kJ0[st] <= J + tb);
-        const gptr<const R> B = tiles + (ktb[st] + (J + tb - kJ0[st])) * TE;
-        if constexpr (C) pb[s][q] = ok ? ((gptr<const r2>)B)[(kc + bk) * TB + bc % TB] : r2{0, 0};
-        else qb[s][q] = ok ? B[(kc + bk) * TB + bc % TB] : R(0);
+        const uint32_t bk = e / W, bc = e % W, tb = bc / TB;  // B: 16 k x W cols, tile J + tb
+        const uint32_t off = ((x.kc + bk) * TB + bc % TB) * (uint32_t)(NC * sizeof(R));
+        if constexpr (C) pb[s][q] = x.ok[tb] ? load_at<r2>(x.B[tb], off) : r2{0, 0};
+        else qb[s][q] = x.ok[tb] ? load_at<R>(x.B[tb], off) : R(0);
       }
+    };
+    // the chunk loaded DEPTH chunks ahead: this block's chunk, else the next
+    // block's first chunks (chain), else this block's last chunk again
+    // (unused).  Every path issues the same loads, so the wait before a
+    // chunk's LDS staging counts only the loads issued before it and leaves
+    // the later set in flight.
+    auto ahead = [&](int c) {
+      const int cn = c + DEPTH;
+      const bool own = cn < nch, nb = !own && chain;
+      return src(own || !nb ? Ab : An, own || !nb ? Jc : nxt.J, own || !nb ? cin : nxt.cols_in,
+                 own ? cn : (nb ? cn - nch : nch - 1));
     };
     auto step = [&](int c, auto set) {
       constexpr int s = decltype(set)::value;
       const int buf = c & 1;
+      const Src nx = ahead(c);
 #pragma unroll
       for (int q = 0; q < PERA; q++) {
         const int e = tid + q * NT;
@@ -479,7 +528,7 @@ __global__ __launch_bounds__(NT, (sizeof(R) == 4 && TJ == 1) ? 3 : 2) void k_tra
         }
       }
       __syncthreads();
-      if (c + DEPTH < nch) fetch(c + DEPTH, set);
+      load(nx, set);
       __builtin_amdgcn_s_setprio(1);
 #pragma unroll
       for (int k4 = 0; k4 < KC; k4 += 4) {
@@ -527,16 +576,19 @@ __global__ __launch_bounds__(NT, (sizeof(R) == 4 && TJ == 1) ? 3 : 2) void k_tra
     };
     using S0 = std::integral_constant<int, 0>;
     using S1 = std::integral_constant<int, DEPTH - 1>;
-    if (!pre) {
-      if (nch > 0) fetch(0, S0{});
-      if constexpr (DEPTH == 2)
-        if (nch > 1) fetch(1, S1{});
+    static_assert((TB / KC) % 2 == 0, "chunks per step even: the loop below takes them in pairs");
+    if (!pre && nch > 0) {
+      load(src(Ab, Jc, cin, 0), S0{});
+      // keeps the two sets' loads in order, as the loop issues them: the
+      // waits at the loop head are counted over both entries to the loop
+      __builtin_amdgcn_sched_barrier(0);
+      if constexpr (DEPTH == 2) load(src(Ab, Jc, cin, 1), S1{});
     }
     for (int c = 0; c < nch; c += DEPTH) {
       step(c, S0{});
-      if constexpr (DEPTH == 2)
-        if (c + 1 < nch) step(c + 1, S1{});
+      if constexpr (DEPTH == 2) step(c + 1, S1{});
     }
+    pre = nch > 0 && chain;
     if constexpr (M3) {
 #pragma unroll
       for (int a = 0; a < 2; a++)
@@ -545,21 +597,6 @@ __global__ __launch_bounds__(NT, (sizeof(R) == 4 && TJ == 1) ? 3 : 2) void k_tra
           acci[a][bb] = acci[a][bb] - accr[a][bb] - accx[a % 2][bb % NX];
           accr[a][bb] = accr[a][bb] - accx[a % 2][bb % NX];
         }
-    }
-    // the next block's first chunks go in flight before this block's epilogue
-    // (same item: its step metadata is already in LDS)
-    pre = false;
-    if (has_next && nxt.g == g) {
-      const int mn = steps(nxt.I, nxt.J, nxt.cols_in);
-      if (mn > 0) {
-        Ab = tiles + nxt.rowA * TE;
-        J = nxt.J;
-        cols_in = nxt.cols_in;
-        fetch(0, S0{});
-        if constexpr (DEPTH == 2)
-          if (mn * (TB / KC) > 1) fetch(1, S1{});
-        pre = true;
-      }
     }
     // this wave's columns c0 .. c0 + 32*TJ - 1 lie in tile Jc + c0/64
     const int tb = c0 / TB;
