@@ -99,6 +99,52 @@ def dropin_rates(cl, lb1, lb2, V, s, w):
     return res
 
 
+def assembly_rates(cl, V, w, G):
+    """Per-energy row assembly: the device assembler (fdmgpu_assembler_run,
+    G points per call, HIP events; fresh buffers, and buffers updated in
+    place) beside the host assembler (fdmgpu_cluster_assemble, one core) on
+    the same points.  Algorithmic HBM
+    bytes per point: every value written (16 B) and its base entry read
+    (8 B), the diagonal positions and V read (16 B per grid point), the
+    outer-sphere tables read once per call."""
+    import torch
+    from fdmnes_amd import configs as cfg
+    from fdmnes_amd.assemble import DeviceAssembler
+    t = cl.assembly_tables()
+    E = [float(w.energies_Ry[(k * 13) % len(w.energies_Ry)]) for k in range(G)]
+    ens = [(e, w.eimag_Ry, cfg.ecinetic_out(e)) for e in E]
+    stream = torch.cuda.current_stream()
+    a = DeviceAssembler(cl, max_points=min(G, 16))
+    Vd = torch.from_numpy(V).cuda()
+    vals = [torch.empty(cl.nnz, dtype=torch.complex128, device="cuda") for _ in range(G)]
+    a.run([Vd] * G, ens, True, False, vals)
+    reps = 20
+
+    def timed(update_only):
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            a.run([Vd] * G, ens, True, False, vals, update_only=update_only)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / 1e3 / (reps * G)
+
+    dev_s, upd_s = timed(False), timed(True)
+    h0 = time.time()
+    nh = 3
+    for k in range(nh):
+        cl.assemble(V, *ens[k], True, False)
+    host_s = (time.time() - h0) / nh
+    nbytes = cl.nnz * 24 + cl.npoint * 16 + (t["nrow_out"] * 12 + t["nlink"] * 12 + t["nb"] * cl.nlmso * 8) / G
+    res = {"device_us_per_point": dev_s * 1e6, "device_update_us_per_point": upd_s * 1e6,
+           "host_ms_per_point": host_s * 1e3, "host_cores": 1,
+           "bytes_per_point": nbytes, "achieved_GBps": nbytes / dev_s / 1e9,
+           "frac_hbm": nbytes / dev_s / 8e12, "points_per_call": G}
+    log(f"assembly: device {dev_s * 1e6:.1f} us/point ({upd_s * 1e6:.1f} updating), host {host_s * 1e3:.1f} ms/point")
+    return res
+
+
 def traffic_from_profile():
     """HBM bytes per k_trail launch measured by rocprofv3 PMC runs (FETCH_SIZE,
     WRITE_SIZE) of this benchmark, scaled by the same counters on a plain
@@ -149,6 +195,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-roofline", action="store_true", help="skip the per-launch timed pass (profiler runs)")
+    ap.add_argument("--no-assembly", action="store_true", help="skip the per-energy assembly timing")
     ap.add_argument("--dropin", action="store_true",
                     help="also time the host-buffer drop-in entry points (PCIe and set-up included), one point each")
     args = ap.parse_args()
@@ -244,6 +291,8 @@ def main():
                     "max_rel_sigma_err": float(err.max()), "adjoint_over_forward": t_adj / t_fwd,
                     "adjoint_s_per_group": t_adj, "fp64_ms_per_step": 1e3 * t_fwd})
         del sv64, xs64, grads
+    if not args.no_assembly:
+        out["assembly"] = assembly_rates(cl, V, w, G)
     if args.dropin:
         out.update(dropin_rates(cl, lb1, lb2, V, s, w))
     cpu = None

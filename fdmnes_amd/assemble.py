@@ -1,0 +1,124 @@
+"""Per-energy row assembly on the device (``fdmgpu_assembler_*`` in
+include/fdmgpu.h): the energy-independent tables of a cluster are copied to
+the GPU once, then each call writes the compressed-row values (and the
+incoming-wave RHS) of up to 16 energy points straight into device buffers
+the batched solver reads.  Stands in for the per-row calcMatRow calls of
+mat.f90:1034-1727 (grid rows :1142-1232, outer columns :1312-1398, outer
+rows :1401-1511) for a cluster without atoms."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from .batch import Ordered, library_stream
+
+MAX_POINTS = 16
+_V = C.c_void_p
+
+
+class Tables(C.Structure):
+    _fields_ = [("npoint", C.c_int), ("nligne", C.c_int), ("nlmso", C.c_int), ("lmaxso", C.c_int),
+                ("nsortf", C.c_int), ("nb", C.c_int), ("nrow_out", C.c_int), ("nlink", C.c_int),
+                ("nnz", C.c_int64), ("clapl0", C.c_double), ("pw", C.c_double),
+                ("row_ptr", _V), ("base", _V), ("qdiag", _V), ("orow", _V), ("oq", _V), ("optr", _V),
+                ("ob", _V), ("ow", _V), ("ylm", _V), ("brad", _V)]
+
+
+def _sigs(L):
+    if getattr(L, "_asm_sigs", False):
+        return
+    P = C.POINTER
+    L.fdmgpu_assembler_create.argtypes = [P(Tables), C.c_int, _V, P(_V)]
+    L.fdmgpu_assembler_destroy.argtypes = [_V]
+    L.fdmgpu_assembler_destroy.restype = None
+    L.fdmgpu_assembler_run.argtypes = [_V, C.c_int, P(C.c_double), P(C.c_double), P(C.c_double), C.c_int, C.c_int,
+                                       C.c_int, P(_V), P(_V), P(_V)]
+    L._asm_sigs = True
+
+
+def _dptrs(ts):
+    arr = (_V * len(ts))()
+    for k, t in enumerate(ts):
+        if t is None:
+            arr[k] = None
+            continue
+        if not t.is_contiguous():
+            raise ValueError("device buffers must be contiguous")
+        arr[k] = t.data_ptr()
+    return arr
+
+
+class DeviceAssembler:
+    """Assembles the systems of one cluster (``fdmnes_amd.cluster.Cluster``)
+    for many energies on the GPU.  ``run`` fills complex128 (or float64 in
+    real arithmetic) value tensors of length nnz in the cluster's compressed
+    row order, and optionally (nligne, nlmso) RHS tensors.  Runs on a stream
+    of its own, ordered after the caller's queued work and before its later
+    work (see ``fdmnes_amd.batch.Ordered``)."""
+
+    def __init__(self, cluster, max_points: int = MAX_POINTS, stream: torch.cuda.Stream | None = None):
+        L = _lib.lib()
+        _sigs(L)
+        self.L = L
+        t = cluster.assembly_tables()
+        self.nnz, self.nligne, self.nlmso, self.npoint = t["nnz"], t["nligne"], t["nlmso"], t["npoint"]
+        self._keep = {k: np.ascontiguousarray(t[k]) for k in
+                      ("row_ptr", "base", "qdiag", "orow", "oq", "optr", "ob", "ow", "ylm", "brad")}
+        tb = Tables(t["npoint"], t["nligne"], t["nlmso"], t["lmaxso"], t["nsortf"], t["nb"], t["nrow_out"],
+                    t["nlink"], t["nnz"], t["clapl0"], t["pw"],
+                    *[self._keep[k].ctypes.data_as(_V) for k in
+                      ("row_ptr", "base", "qdiag", "orow", "oq", "optr", "ob", "ow", "ylm", "brad")])
+        self.stream = library_stream(stream)
+        h = _V()
+        _lib.check(L.fdmgpu_assembler_create(C.byref(tb), int(max_points), _V(self.stream.cuda_stream), C.byref(h)))
+        self._h = h
+        self.max_points = int(max_points)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self.L.fdmgpu_assembler_destroy(self._h)
+            self._h = None
+
+    def run(self, V, energies, E_comp: bool, basereel: bool, vals, rhs=None, update_only: bool = False):
+        """V: list of float64 device tensors (npoint); energies: list of
+        (Enervide, Eimag, Ecinetic_out) in Ry; vals (and rhs) lists of
+        contiguous device tensors, complex128 when the arithmetic is complex
+        (E_comp or not basereel), float64 otherwise.  update_only: the
+        buffers hold an earlier call's output for this cluster (same
+        arithmetic branch); only the energy- and V-dependent entries are
+        rewritten."""
+        n = len(energies)
+        if n == 0:
+            return
+        if not (len(V) == len(vals) == n) or (rhs is not None and len(rhs) != n):
+            raise ValueError("one V, vals (and rhs) per energy point")
+        cal = bool(E_comp) or not basereel
+        for t in list(vals) + list(rhs or []):
+            if t.is_complex() != cal:
+                raise ValueError("value and RHS buffers must be complex exactly when the arithmetic is")
+        for t in vals:
+            if t.numel() != self.nnz:
+                raise ValueError(f"value buffers hold nnz = {self.nnz} entries")
+        for t in rhs or []:
+            if t.numel() != self.nligne * self.nlmso:
+                raise ValueError("RHS buffers are (nligne, nlmso)")
+        for t in V:
+            if t.dtype != torch.float64 or t.numel() != self.npoint:
+                raise ValueError("V: float64, npoint entries")
+        E = np.array(energies, np.float64).reshape(n, 3)
+        D = C.POINTER(C.c_double)
+        ev = [np.ascontiguousarray(E[:, k]) for k in range(3)]
+        with Ordered(self.stream):
+            for k0 in range(0, n, self.max_points):
+                k1 = min(n, k0 + self.max_points)
+                rc = self.L.fdmgpu_assembler_run(
+                    self._h, k1 - k0, *[np.ascontiguousarray(a[k0:k1]).ctypes.data_as(D) for a in ev],
+                    int(bool(E_comp)), int(bool(basereel)), int(bool(update_only)), _dptrs(V[k0:k1]),
+                    _dptrs(vals[k0:k1]),
+                    _dptrs(rhs[k0:k1]) if rhs is not None else None)
+                if rc:
+                    break
+        _lib.check(rc)

@@ -1,7 +1,10 @@
 """Device-resident batched solver (``fdmgpu_solver_*`` in include/fdmgpu.h).
 
-Device memory comes from torch tensors; the library launches on the torch
-stream current at creation, so torch events bracket its work."""
+Device memory comes from torch tensors.  The library launches on a stream of
+its own (graph capture needs one; torch's default stream is the legacy null
+stream); every call is ordered after the work already queued on the caller's
+current torch stream and before the work queued there afterwards, so torch
+events on the caller's stream bracket the library's kernels."""
 from __future__ import annotations
 
 import ctypes as C
@@ -42,6 +45,33 @@ def _ptrs(ts):
     return arr
 
 
+def library_stream(stream: torch.cuda.Stream | None) -> torch.cuda.Stream:
+    """The stream a library object launches on: the given one unless it is
+    the null stream, else a new one (default priority)."""
+    if stream is not None and stream.cuda_stream != 0:
+        return stream
+    return torch.cuda.Stream()
+
+
+class Ordered:
+    """Context of one library call on stream s: s waits for the caller's
+    current stream first, and the caller's stream waits for s afterwards."""
+
+    def __init__(self, s: torch.cuda.Stream):
+        self.s = s
+
+    def __enter__(self):
+        self.cur = torch.cuda.current_stream()
+        if self.cur != self.s:
+            self.s.wait_stream(self.cur)
+        return self
+
+    def __exit__(self, *exc):
+        if self.cur != self.s:
+            self.cur.wait_stream(self.s)
+        return False
+
+
 def as_real(t: torch.Tensor) -> torch.Tensor:
     """Interleaved float64 view of a complex128 tensor (float64 passes through)."""
     return torch.view_as_real(t) if t.is_complex() else t
@@ -55,9 +85,9 @@ class DeviceSolver:
         self.cal = bool(cal_comp)
         self.nrhs = nrhs
         self.precision = precision
-        st = (stream or torch.cuda.current_stream()).cuda_stream
+        self.stream = library_stream(stream)
         h = _V()
-        _lib.check(L.fdmgpu_solver_create(precision, int(self.cal), nrhs, _V(st), C.byref(h)))
+        _lib.check(L.fdmgpu_solver_create(precision, int(self.cal), nrhs, _V(self.stream.cuda_stream), C.byref(h)))
         self._h = h
         self.sizes = []
 
@@ -89,9 +119,10 @@ class DeviceSolver:
         fr = (C.c_int * n)()
         sw = (C.c_int * n)()
         rel = (C.c_double * n)()
-        rc = self.L.fdmgpu_solver_solve(self._h, n, ids, _ptrs([as_real(v) for v in vals]),
-                                        _ptrs([as_real(t) for t in b]), _ptrs([as_real(t) for t in x]),
-                                        int(max_sweeps), float(tol), fr, sw, rel)
+        with Ordered(self.stream):
+            rc = self.L.fdmgpu_solver_solve(self._h, n, ids, _ptrs([as_real(v) for v in vals]),
+                                            _ptrs([as_real(t) for t in b]), _ptrs([as_real(t) for t in x]),
+                                            int(max_sweeps), float(tol), fr, sw, rel)
         if rc == 1 and not raise_on_pivot:
             return list(fr), list(sw), list(rel)
         _lib.check(rc, max(fr))
@@ -100,15 +131,19 @@ class DeviceSolver:
     def adjoint(self, slots, src, x, grad):
         n = len(slots)
         ids = (C.c_int * n)(*slots)
-        _lib.check(self.L.fdmgpu_solver_adjoint(self._h, n, ids, _ptrs([as_real(t) for t in src]),
-                                                _ptrs([as_real(t) for t in x]), _ptrs(grad)))
+        with Ordered(self.stream):
+            rc = self.L.fdmgpu_solver_adjoint(self._h, n, ids, _ptrs([as_real(t) for t in src]),
+                                              _ptrs([as_real(t) for t in x]), _ptrs(grad))
+        _lib.check(rc)
 
     def sigma(self, slots, src, x) -> np.ndarray:
         n = len(slots)
         ids = (C.c_int * n)(*slots)
         out = (C.c_double * n)()
-        _lib.check(self.L.fdmgpu_solver_sigma(self._h, n, ids, _ptrs([as_real(t) for t in src]),
-                                              _ptrs([as_real(t) for t in x]), out))
+        with Ordered(self.stream):
+            rc = self.L.fdmgpu_solver_sigma(self._h, n, ids, _ptrs([as_real(t) for t in src]),
+                                            _ptrs([as_real(t) for t in x]), out)
+        _lib.check(rc)
         return np.array(out[:])
 
     def stats(self) -> dict:

@@ -28,6 +28,11 @@ def _L():
         L.fdmgpu_cluster_assemble.argtypes = [C.c_void_p, _lib.dp, C.c_double, C.c_double, C.c_double, C.c_int,
                                               C.c_int, _lib.lp, _lib.ip, _lib.dp, C.c_void_p, C.c_void_p,
                                               C.c_void_p]
+        L.fdmgpu_cluster_assembly_sizes.restype = C.c_int
+        L.fdmgpu_cluster_assembly_sizes.argtypes = [C.c_void_p, _lib.lp]
+        L.fdmgpu_cluster_assembly_tables.restype = C.c_int
+        L.fdmgpu_cluster_assembly_tables.argtypes = [C.c_void_p, _lib.dp, _lib.lp, _lib.ip, _lib.lp, _lib.ip,
+                                                     _lib.ip, _lib.dp, _lib.dp, _lib.dp, _lib.dp]
         L._cluster_sigs = True
     return L
 
@@ -85,3 +90,25 @@ class Cluster:
         vals = re + 1j * im if cal else re
         rhs = (rr + 1j * ri).reshape(n, nl) if cal else rr.reshape(n, nl)
         return rp, col, vals, rhs
+
+    def assembly_tables(self) -> dict:
+        """Energy-independent tables of the per-energy assembly
+        (fdmgpu_cluster_assembly_tables): numpy arrays plus the sizes."""
+        L = _L()
+        sz = np.zeros(4, np.int64)
+        L.fdmgpu_cluster_assembly_sizes(self._h, sz)
+        nrow, nlink, nb, nnz = map(int, sz)
+        t = dict(nrow_out=nrow, nlink=nlink, nb=nb, nnz=nnz, npoint=self.npoint, nligne=self.nligne,
+                 nlmso=self.nlmso, lmaxso=self.lmaxso, nsortf=self.nsortf,
+                 base=np.zeros(nnz), qdiag=np.zeros(self.npoint, np.int64), orow=np.zeros(max(nrow, 1), np.int32),
+                 oq=np.zeros(max(nrow, 1), np.int64), optr=np.zeros(nrow + 1, np.int32),
+                 ob=np.zeros(max(nlink, 1), np.int32), ow=np.zeros(max(nlink, 1)),
+                 ylm=np.zeros(nb * self.nlmso), brad=np.zeros(nb), par=np.zeros(2))
+        rc = L.fdmgpu_cluster_assembly_tables(self._h, t["base"], t["qdiag"], t["orow"], t["oq"], t["optr"], t["ob"],
+                                              t["ow"], t["ylm"], t["brad"], t["par"])
+        if rc:
+            raise _lib.FdmGpuError(f"fdmgpu_cluster_assembly_tables failed ({rc})")
+        rp, _, _, _ = self.assemble(np.zeros(self.npoint), 0.0, 0.0, 1.0, False, True)
+        t["row_ptr"] = rp
+        t["clapl0"], t["pw"] = float(t["par"][0]), float(t["par"][1])
+        return t

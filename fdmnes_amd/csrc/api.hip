@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "../../include/fdmgpu.h"
+#include "assemble.h"
 #include "batch.h"
 
 using namespace fdmgpu;
@@ -345,6 +346,50 @@ int fdmgpu_solver_stats(const fdmgpu_solver *s, double *out) {
     else fill((Solver<float, false> *)s->impl);
   }
   return 0;
+}
+
+struct fdmgpu_assembler {
+  Assembler a;
+};
+
+int fdmgpu_assembler_create(const fdmgpu_assembly_tables *t, int max_points, void *hip_stream,
+                            fdmgpu_assembler **out) {
+  if (!t || !out) {
+    set_error("fdmgpu_assembler_create: invalid argument");
+    return FDMGPU_EINVAL;
+  }
+  int rc = ensure_init();
+  if (rc) return rc;
+  std::unique_ptr<fdmgpu_assembler> h(new (std::nothrow) fdmgpu_assembler());
+  if (!h) return FDMGPU_ENOMEM;
+  rc = h->a.init(t, max_points, hip_stream ? (hipStream_t)hip_stream : g_lib.stream);
+  if (rc) return rc;
+  *out = h.release();
+  return 0;
+}
+
+void fdmgpu_assembler_destroy(fdmgpu_assembler *a) { delete a; }
+
+int fdmgpu_assembler_run(fdmgpu_assembler *a, int count, const double *Enervide, const double *Eimag,
+                         const double *Ecinetic_out, int E_comp, int basereel, int update_only,
+                         const double *const *d_V, double *const *d_vals, double *const *d_rhs) {
+  if (!a || !Enervide || !Eimag || !Ecinetic_out || !d_V || !d_vals || count < 1 || count > ASM_MAXP) {
+    set_error("fdmgpu_assembler_run: invalid argument");
+    return FDMGPU_EINVAL;
+  }
+  AsmPoints P{};
+  for (int k = 0; k < count; k++) {
+    if (!d_V[k] || !d_vals[k]) {
+      set_error("fdmgpu_assembler_run: null device buffer for point " + std::to_string(k));
+      return FDMGPU_EINVAL;
+    }
+    P.e[k] = {Enervide[k], Eimag[k], Ecinetic_out[k]};
+    P.V[k] = d_V[k];
+    P.vals[k] = d_vals[k];
+    P.rhs[k] = d_rhs ? d_rhs[k] : nullptr;
+  }
+  FDM_HIP(hipSetDevice(g_lib.device));
+  return a->a.run(count, P, E_comp != 0, basereel != 0, update_only != 0);
 }
 
 }  // extern "C"

@@ -1,0 +1,44 @@
+// Per-energy assembly of a cluster's compressed rows on the device.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/fdmgpu.h"
+
+namespace fdmgpu {
+
+constexpr int ASM_MAXP = 16;  // energy points per assembly call
+
+// Energy of one point: Enervide (grid diagonal), Eimag (broadening), and the
+// outer-sphere kinetic energy Ecinetic_out (mat.f90:771-792).
+struct AsmEnergy {
+  double ener, eimag, ecin;
+};
+struct AsmPoints {
+  AsmEnergy e[ASM_MAXP];
+  const double *V[ASM_MAXP];  // device, npoint
+  double *vals[ASM_MAXP];     // device, nnz (interleaved when complex)
+  double *rhs[ASM_MAXP];      // device, (nligne, nlmso) row-major, or null
+};
+
+struct Assembler {
+  int npoint = 0, nligne = 0, nlmso = 0, lmaxso = 0, nsortf = 0, nb = 0, nrow_out = 0, nlink = 0;
+  int64_t nnz = 0;
+  double clapl0 = 0, pw = 0;
+  int max_points = 0;
+  hipStream_t stream = nullptr;
+  // device copies of the energy-independent tables
+  int64_t *row_ptr = nullptr, *qdiag = nullptr, *oq = nullptr;
+  int *orow = nullptr, *optr = nullptr, *ob = nullptr;
+  double *base = nullptr, *ow = nullptr, *ylm = nullptr, *brad = nullptr;
+  double *fun = nullptr;   // Bessel/Neumann table: [point][nb][lmaxso+1] x {B, N} complex
+  double *part = nullptr;  // outer-row partial sums: [point][run][nlmso^2] x {block, RHS} complex
+
+  ~Assembler();
+  int init(const fdmgpu_assembly_tables *t, int max_points, hipStream_t s);
+  // update_only: vals/rhs already hold an earlier call's output for this
+  // structure; only the energy-dependent entries are rewritten
+  int run(int count, const AsmPoints &pts, bool e_comp, bool basereel, bool update_only);
+};
+
+}  // namespace fdmgpu

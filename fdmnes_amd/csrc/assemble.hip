@@ -1,0 +1,386 @@
+// Per-energy assembly of a cluster's compressed rows on the device: the
+// energy-dependent entries of calcMatRow (mat.f90:1142-1511) for a cluster
+// without atoms, written into the value array the solver reads.  The tables
+// of everything that does not change with the energy are uploaded once; a
+// call then computes, for up to ASM_MAXP points at a time,
+//   1. the outer-sphere functions fnorm j_l(k r), fnorm n_l(k r) at every
+//      outer point (phiso, mat.f90:771-792, :975-994; recurrence of
+//      sphere.f90:7226-7248),
+//   2. every entry's energy-independent value (copy of the base table;
+//      skipped when the caller's buffers already hold an earlier point's
+//      values for this structure, which differ only in the entries below),
+//   3. the grid diagonals V - E - clapl(0) (- i Eimag)   (mat.f90:1147-1150),
+//   4. the outer columns of grid rows with neighbours outside the sphere and
+//      their incoming-wave RHS                         (mat.f90:1312-1398),
+//   5. the outer rows' outer-column block and RHS       (mat.f90:1401-1511).
+// Multiply-adds are not fused (contraction off) and the grid rows' sums run
+// in the reference's term order; the outer rows' sums over the boundary
+// points are taken in runs of 256 points, then over the runs, in order.  The entries agree with a
+// host assembly of the same rows to the last bits of the Bessel/Neumann
+// values and of those sums.
+#include <algorithm>
+#include <cmath>
+#include <string>
+
+#include "assemble.h"
+#include "solver.h"
+
+#pragma clang fp contract(off)
+
+namespace fdmgpu {
+namespace {
+
+struct cz {
+  double re, im;
+};
+__device__ inline cz add(cz a, cz b) { return {a.re + b.re, a.im + b.im}; }
+__device__ inline cz sub(cz a, cz b) { return {a.re - b.re, a.im - b.im}; }
+__device__ inline cz mul(cz a, cz b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+__device__ inline cz scale(cz a, double s) { return {a.re * s, a.im * s}; }
+// a / b by Smith's algorithm (R. L. Smith, CACM 5(8):435, 1962)
+__device__ inline cz cdiv(cz a, cz b) {
+  if (fabs(b.re) < fabs(b.im)) {
+    const double r = b.re / b.im, d = b.re * r + b.im;
+    return {(a.re * r + a.im) / d, (a.im * r - a.re) / d};
+  }
+  const double r = b.im / b.re, d = b.im * r + b.re;
+  return {(a.im * r + a.re) / d, (a.im - a.re * r) / d};
+}
+__device__ inline cz csin(cz z) { return {sin(z.re) * cosh(z.im), cos(z.re) * sinh(z.im)}; }
+__device__ inline cz ccos(cz z) { return {cos(z.re) * cosh(z.im), -(sin(z.re) * sinh(z.im))}; }
+// principal square root
+__device__ inline cz csqrt(cz z) {
+  if (z.re == 0 && z.im == 0) return {0, z.im};
+  const double t = sqrt((fabs(z.re) + hypot(z.re, z.im)) * 0.5);
+  if (z.re >= 0) return {t, z.im / (2 * t)};
+  return {fabs(z.im) / (2 * t), copysign(t, z.im)};
+}
+
+__device__ inline int lof(int lm) {  // l of the harmonic index l*l + l + m
+  int l = 0;
+  while ((l + 1) * (l + 1) <= lm) l++;
+  return l;
+}
+
+// 1. fun[((g*nb + b)*(L+1) + l)*2 + {0, 1}] = fnorm {j_l, n_l}(k brad[b])
+__global__ void k_asm_fun(int nb, int L, const double *__restrict__ brad, AsmPoints P, int e_comp,
+                          cz *__restrict__ fun) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x, g = blockIdx.y;
+  if (b >= nb) return;
+  const AsmEnergy E = P.e[g];
+  cz *f = fun + ((int64_t)g * nb + b) * (L + 1) * 2;
+  cz fn, z;
+  if (e_comp) {
+    cz k = csqrt({E.ecin, E.eimag});
+    if (hypot(k.re, k.im) < 1e-10) k = {1e-10, 0};
+    fn = csqrt({k.re / 3.1415926535897932384626433832, k.im / 3.1415926535897932384626433832});
+    z = scale(k, brad[b]);
+  } else {
+    const double k = sqrt(E.ecin);
+    fn = {sqrt(k / 3.1415926535897932384626433832), 0};
+    z = {k * brad[b], 0};
+  }
+  cz bl2, nl2, bl1, nl1;
+  if (e_comp) {
+    nl1 = cdiv(mul({-fn.re, -fn.im}, ccos(z)), z);
+    bl1 = cdiv(mul(fn, csin(z)), z);
+  } else {  // real arithmetic, as the host's real branch
+    nl1 = {-fn.re * cos(z.re) / z.re, 0};
+    bl1 = {fn.re * sin(z.re) / z.re, 0};
+  }
+  f[0] = bl1;
+  f[1] = nl1;
+  if (L < 1) return;
+  if (e_comp) {
+    bl2 = add(cdiv(bl1, z), nl1);
+    nl2 = sub(cdiv(nl1, z), bl1);
+  } else {
+    bl2 = {bl1.re / z.re + nl1.re, 0};
+    nl2 = {nl1.re / z.re - bl1.re, 0};
+  }
+  f[2] = bl2;
+  f[3] = nl2;
+  for (int l = 2; l <= L; l++) {
+    cz bn, nn;
+    const double c = 2 * l - 1;
+    if (e_comp) {
+      nn = sub(cdiv(scale(nl2, c), z), nl1);
+      bn = sub(cdiv(scale(bl2, c), z), bl1);
+    } else {
+      nn = {c * nl2.re / z.re - nl1.re, 0};
+      bn = {c * bl2.re / z.re - bl1.re, 0};
+    }
+    f[2 * l] = bn;
+    f[2 * l + 1] = nn;
+    bl1 = bl2;
+    nl1 = nl2;
+    bl2 = bn;
+    nl2 = nn;
+  }
+}
+
+// outer-column factor of outer point b (calcMatRow, mat.f90:1330-1346): the
+// Neumann function in the real basis, j + i n in the complex one
+__device__ inline cz cfac(const cz *f, int l, bool basereel) {
+  const cz B = f[2 * l], N = f[2 * l + 1];
+  if (basereel) return N;
+  return {B.re - N.im, B.im + N.re};
+}
+
+// 2. values from the base table; RHS cleared
+template <bool CAL>
+__global__ void k_asm_base(int64_t nnz, int64_t nrhs_el, const double *__restrict__ base, AsmPoints P) {
+  const int g = blockIdx.y;
+  double *v = P.vals[g], *r = P.rhs[g];
+  const int64_t n = nnz > nrhs_el ? nnz : nrhs_el;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+    if (q < nnz) {
+      if constexpr (CAL) {
+        v[2 * q] = base[q];
+        v[2 * q + 1] = 0;
+      } else {
+        v[q] = base[q];
+      }
+    }
+    if (r && q < nrhs_el) {
+      if constexpr (CAL) {
+        r[2 * q] = 0;
+        r[2 * q + 1] = 0;
+      } else {
+        r[q] = 0;
+      }
+    }
+  }
+}
+
+// 3. grid diagonals
+template <bool CAL>
+__global__ void k_asm_diag(int npoint, const int64_t *__restrict__ qdiag, double clapl0, AsmPoints P, int e_comp) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, g = blockIdx.y;
+  if (i >= npoint) return;
+  const AsmEnergy E = P.e[g];
+  const int64_t q = qdiag[i];
+  const double d = P.V[g][i] - E.ener - clapl0;
+  if constexpr (CAL) {
+    P.vals[g][2 * q] = d;
+    P.vals[g][2 * q + 1] = e_comp ? -E.eimag : 0.0;
+  } else {
+    P.vals[g][q] = d;
+  }
+}
+
+// 4. outer columns of the grid rows touching the outside, and their RHS:
+// one thread per (row, harmonic), outside neighbours in stencil order
+template <bool CAL>
+__global__ void k_asm_outcols(int nrow_out, int nl, int nb, int L, const int *__restrict__ orow,
+                              const int64_t *__restrict__ oq, const int *__restrict__ optr,
+                              const int *__restrict__ ob, const double *__restrict__ ow,
+                              const double *__restrict__ ylm, const cz *__restrict__ fun, AsmPoints P,
+                              int basereel) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x, g = blockIdx.y;
+  if (t >= nrow_out * nl) return;
+  const int r = t / nl, lm = t % nl, l = lof(lm);
+  const cz *fg = fun + (int64_t)g * nb * (L + 1) * 2;
+  cz row = {0, 0}, rrow = {0, 0};
+  for (int k = optr[r]; k < optr[r + 1]; k++) {
+    const int b = ob[k];
+    const double w = ow[k], y = ylm[(int64_t)b * nl + lm];
+    const cz *f = fg + (int64_t)b * (L + 1) * 2;
+    row = add(row, scale(scale(cfac(f, l, basereel), y), w));
+    rrow = sub(rrow, scale(scale(f[2 * l], y), w));
+  }
+  const int64_t q = oq[r] + lm, qr = (int64_t)orow[r] * nl + lm;
+  double *v = P.vals[g], *rh = P.rhs[g];
+  if constexpr (CAL) {
+    v[2 * q] = row.re;
+    v[2 * q + 1] = row.im;
+    if (rh) {
+      rh[2 * qr] = rrow.re;
+      rh[2 * qr + 1] = rrow.im;
+    }
+  } else {
+    v[q] = row.re;
+    if (rh) rh[qr] = rrow.re;
+  }
+}
+
+// 5. outer rows: block (lm, lp) = sum_b pw Y_lm(b) Y_lp(b) cfac(b, l(lp)) and
+// RHS (lm, lf) = -sum_b pw Y_lm(b) Y_lf(b) B(b, l(lf)), over the nsortf inner
+// outer points.  First pass: one workgroup per (point, run of ASM_CB points
+// b, 256 (lm, lp) pairs), each thread summing its pair over the run in
+// order; second pass: one thread per (point, pair) adds the runs' partial
+// sums in run order.  The result does not depend on timing.
+constexpr int ASM_CB = 256;
+__global__ __launch_bounds__(256) void k_asm_outrows_part(int nl, int nb, int L, int nsortf, double pw,
+                                                           const double *__restrict__ ylm,
+                                                           const cz *__restrict__ fun, int basereel,
+                                                           cz *__restrict__ part) {
+  const int run = blockIdx.x, nrun = gridDim.x, g = blockIdx.z;
+  const int pr = blockIdx.y * blockDim.x + threadIdx.x;
+  if (pr >= nl * nl) return;
+  const int lm = pr / nl, lp = pr % nl, l = lof(lp);
+  const cz *fg = fun + (int64_t)g * nb * (L + 1) * 2;
+  cz s = {0, 0}, sr = {0, 0};
+  const int b1 = min(nsortf, (run + 1) * ASM_CB);
+#pragma unroll 8
+  for (int b = run * ASM_CB; b < b1; b++) {
+    const double c = pw * ylm[(int64_t)b * nl + lm] * ylm[(int64_t)b * nl + lp];
+    const cz *f = fg + (int64_t)b * (L + 1) * 2;
+    s = add(s, scale(cfac(f, l, basereel), c));
+    sr = add(sr, scale(f[2 * l], c));
+  }
+  cz *o = part + (((int64_t)g * nrun + run) * nl * nl + pr) * 2;
+  o[0] = s;
+  o[1] = sr;
+}
+template <bool CAL>
+__global__ void k_asm_outrows_sum(int npoint, int nl, int nsortf, int nrun, const int64_t *__restrict__ row_ptr,
+                                  const cz *__restrict__ part, AsmPoints P) {
+  const int pr = blockIdx.x * blockDim.x + threadIdx.x, g = blockIdx.y;
+  if (pr >= nl * nl) return;
+  const int lm = pr / nl, lp = pr % nl;
+  cz t = {0, 0}, tr = {0, 0};
+  for (int run = 0; run < nrun; run++) {
+    const cz *o = part + (((int64_t)g * nrun + run) * nl * nl + pr) * 2;
+    t = add(t, o[0]);
+    tr = add(tr, o[1]);
+  }
+  const int ii = npoint + lm;
+  const int64_t q = row_ptr[ii] + nsortf + lp, qr = (int64_t)ii * nl + lp;
+  double *v = P.vals[g], *rh = P.rhs[g];
+  if constexpr (CAL) {
+    v[2 * q] = t.re;
+    v[2 * q + 1] = t.im;
+    if (rh) {
+      rh[2 * qr] = -tr.re;
+      rh[2 * qr + 1] = -tr.im;
+    }
+  } else {
+    v[q] = t.re;
+    if (rh) rh[qr] = -tr.re;
+  }
+}
+
+template <class T>
+int upload(T **d, const T *h, int64_t n) {
+  FDM_HIP(hipMalloc(d, sizeof(T) * (n > 0 ? n : 1)));
+  if (n > 0) FDM_HIP(hipMemcpy(*d, h, sizeof(T) * n, hipMemcpyHostToDevice));
+  return 0;
+}
+
+}  // namespace
+
+Assembler::~Assembler() {
+  for (void *p : {(void *)row_ptr, (void *)qdiag, (void *)oq, (void *)orow, (void *)optr, (void *)ob, (void *)base,
+                  (void *)ow, (void *)ylm, (void *)brad, (void *)fun, (void *)part})
+    if (p) (void)hipFree(p);
+}
+
+int Assembler::init(const fdmgpu_assembly_tables *t, int maxp, hipStream_t s) {
+  if (t->npoint <= 0 || t->nligne != t->npoint + t->nlmso || t->lmaxso < 0 || t->lmaxso > 40 ||
+      t->nlmso != (t->lmaxso + 1) * (t->lmaxso + 1) || t->nsortf <= 0 || t->nb < t->nsortf || t->nrow_out < 0 ||
+      t->nlink < t->nrow_out || t->nnz <= 0 || maxp < 1 || maxp > ASM_MAXP) {
+    set_error("fdmgpu_assembler_create: inconsistent table sizes");
+    return FDMGPU_EINVAL;
+  }
+  if (!t->row_ptr || !t->base || !t->qdiag || !t->ylm || !t->brad ||
+      (t->nrow_out > 0 && (!t->orow || !t->oq || !t->optr || !t->ob || !t->ow))) {
+    set_error("fdmgpu_assembler_create: null table");
+    return FDMGPU_EINVAL;
+  }
+  // every position written per energy must lie in the value array
+  if (t->row_ptr[0] != 0 || t->row_ptr[t->nligne] != t->nnz) {
+    set_error("fdmgpu_assembler_create: row_ptr does not span nnz");
+    return FDMGPU_EINVAL;
+  }
+  for (int i = 0; i < t->npoint; i++)
+    if (t->qdiag[i] < t->row_ptr[i] || t->qdiag[i] >= t->row_ptr[i + 1]) {
+      set_error("fdmgpu_assembler_create: diagonal position outside row " + std::to_string(i + 1));
+      return FDMGPU_EINVAL;
+    }
+  for (int r = 0; r < t->nrow_out; r++) {
+    const int i = t->orow[r];
+    if (i < 0 || i >= t->npoint || t->oq[r] < t->row_ptr[i] || t->oq[r] + t->nlmso > t->row_ptr[i + 1] ||
+        t->optr[r + 1] < t->optr[r] || t->optr[r + 1] > t->nlink) {
+      set_error("fdmgpu_assembler_create: outer-column table inconsistent at entry " + std::to_string(r));
+      return FDMGPU_EINVAL;
+    }
+  }
+  if (t->nrow_out > 0 && (t->optr[0] != 0 || t->optr[t->nrow_out] != t->nlink)) {
+    set_error("fdmgpu_assembler_create: optr does not span nlink");
+    return FDMGPU_EINVAL;
+  }
+  for (int k = 0; k < t->nlink; k++)
+    if (t->ob[k] < 0 || t->ob[k] >= t->nb) {
+      set_error("fdmgpu_assembler_create: outer point index out of range");
+      return FDMGPU_EINVAL;
+    }
+  for (int lm = 0; lm < t->nlmso; lm++)
+    if (t->row_ptr[t->npoint + lm + 1] - t->row_ptr[t->npoint + lm] != t->nsortf + t->nlmso) {
+      set_error("fdmgpu_assembler_create: outer row " + std::to_string(lm) + " is not nsortf + nlmso long");
+      return FDMGPU_EINVAL;
+    }
+  npoint = t->npoint;
+  nligne = t->nligne;
+  nlmso = t->nlmso;
+  lmaxso = t->lmaxso;
+  nsortf = t->nsortf;
+  nb = t->nb;
+  nrow_out = t->nrow_out;
+  nlink = t->nlink;
+  nnz = t->nnz;
+  clapl0 = t->clapl0;
+  pw = t->pw;
+  max_points = maxp;
+  stream = s;
+  if (upload(&row_ptr, t->row_ptr, nligne + 1) || upload(&base, t->base, nnz) || upload(&qdiag, t->qdiag, npoint) ||
+      upload(&orow, t->orow, nrow_out) || upload(&oq, t->oq, nrow_out) || upload(&optr, t->optr, nrow_out + 1) ||
+      upload(&ob, t->ob, nlink) || upload(&ow, t->ow, nlink) || upload(&ylm, t->ylm, (int64_t)nb * nlmso) ||
+      upload(&brad, t->brad, nb))
+    return FDMGPU_EHIP;
+  FDM_HIP(hipMalloc(&fun, sizeof(cz) * (size_t)maxp * nb * (lmaxso + 1) * 2));
+  const size_t nrun = (nsortf + ASM_CB - 1) / ASM_CB;
+  FDM_HIP(hipMalloc(&part, sizeof(cz) * 2 * (size_t)maxp * nrun * nlmso * nlmso));
+  return 0;
+}
+
+int Assembler::run(int count, const AsmPoints &P, bool e_comp, bool basereel, bool update_only) {
+  if (count < 1 || count > max_points) {
+    set_error("fdmgpu_assembler_run: count must be in 1.." + std::to_string(max_points));
+    return FDMGPU_EINVAL;
+  }
+  const bool cal = e_comp || !basereel;
+  cz *F = (cz *)fun;
+  const int NT = 256;
+  hipLaunchKernelGGL(k_asm_fun, dim3((nb + NT - 1) / NT, count), dim3(NT), 0, stream, nb, lmaxso, brad, P, (int)e_comp,
+                     F);
+  const int64_t nrhs_el = (int64_t)nligne * nlmso;
+  const int64_t n = nnz > nrhs_el ? nnz : nrhs_el;
+  const int gb = (int)std::min<int64_t>((n + NT - 1) / NT, 2048);
+  const int gd = (npoint + NT - 1) / NT, go = (nrow_out * nlmso + NT - 1) / NT;
+  const int nrun = (nsortf + ASM_CB - 1) / ASM_CB, npr = (nlmso * nlmso + NT - 1) / NT;
+  const dim3 gp(nrun, npr, count), gs(npr, count);
+  cz *Q = (cz *)part;
+  if (cal) {
+    if (!update_only) hipLaunchKernelGGL(k_asm_base<true>, dim3(gb, count), dim3(NT), 0, stream, nnz, nrhs_el, base, P);
+    hipLaunchKernelGGL(k_asm_diag<true>, dim3(gd, count), dim3(NT), 0, stream, npoint, qdiag, clapl0, P, (int)e_comp);
+    if (nrow_out > 0)
+      hipLaunchKernelGGL(k_asm_outcols<true>, dim3(go, count), dim3(NT), 0, stream, nrow_out, nlmso, nb, lmaxso, orow,
+                         oq, optr, ob, ow, ylm, F, P, (int)basereel);
+    hipLaunchKernelGGL(k_asm_outrows_part, gp, dim3(256), 0, stream, nlmso, nb, lmaxso, nsortf, pw, ylm, F,
+                       (int)basereel, Q);
+    hipLaunchKernelGGL(k_asm_outrows_sum<true>, gs, dim3(256), 0, stream, npoint, nlmso, nsortf, nrun, row_ptr, Q, P);
+  } else {
+    if (!update_only) hipLaunchKernelGGL(k_asm_base<false>, dim3(gb, count), dim3(NT), 0, stream, nnz, nrhs_el, base, P);
+    hipLaunchKernelGGL(k_asm_diag<false>, dim3(gd, count), dim3(NT), 0, stream, npoint, qdiag, clapl0, P, 0);
+    if (nrow_out > 0)
+      hipLaunchKernelGGL(k_asm_outcols<false>, dim3(go, count), dim3(NT), 0, stream, nrow_out, nlmso, nb, lmaxso, orow,
+                         oq, optr, ob, ow, ylm, F, P, 1);
+    hipLaunchKernelGGL(k_asm_outrows_part, gp, dim3(256), 0, stream, nlmso, nb, lmaxso, nsortf, pw, ylm, F, 1, Q);
+    hipLaunchKernelGGL(k_asm_outrows_sum<false>, gs, dim3(256), 0, stream, npoint, nlmso, nsortf, nrun, row_ptr, Q, P);
+  }
+  FDM_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace fdmgpu

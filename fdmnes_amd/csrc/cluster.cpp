@@ -401,4 +401,65 @@ int fdmgpu_cluster_assemble(const fdmgpu_cluster *c, const double *V, double Ene
   return 0;
 }
 
+int fdmgpu_cluster_assembly_sizes(const fdmgpu_cluster *c, int64_t *sizes) {
+  int64_t nrow = 0, nlink = 0;
+  for (int i = 0; i < c->npoint; i++) {
+    int k = 0;
+    for (int d = 0; d < 12; d++) k += c->nbr[(size_t)i * 12 + d] >= c->npoint;
+    nrow += k > 0;
+    nlink += k;
+  }
+  sizes[0] = nrow;
+  sizes[1] = nlink;
+  sizes[2] = (int64_t)c->bnd.size();
+  sizes[3] = (int64_t)c->cols.size();
+  return 0;
+}
+
+int fdmgpu_cluster_assembly_tables(const fdmgpu_cluster *c, double *base, int64_t *qdiag, int *orow,
+                                   int64_t *oq, int *optr, int *ob, double *ow, double *ylm, double *brad,
+                                   double *par) {
+  const int np = c->npoint, nl = c->nlmso;
+  std::fill(base, base + c->cols.size(), 0.0);
+  int r = 0, k = 0;
+  optr[0] = 0;
+  for (int i = 0; i < np; i++) {
+    const int64_t q0 = c->rp[i], q1 = c->rp[i + 1];
+    int64_t q = q0;
+    for (; q < q1 && c->cols[q] <= np; q++) {
+      const int j = c->cols[q] - 1;
+      if (j == i) {
+        qdiag[i] = q;
+        continue;
+      }
+      for (int d = 0; d < 12; d++)
+        if (c->nbr[(size_t)i * 12 + d] == j) base[q] = -c->clapl[d + 1];
+    }
+    bool out = false;
+    for (int d = 0; d < 12; d++) {
+      const int j = c->nbr[(size_t)i * 12 + d];
+      if (j < np) continue;
+      ob[k] = c->slot[j];
+      ow[k] = -c->clapl[d + 1];
+      k++;
+      out = true;
+    }
+    if (out) {
+      orow[r] = i;
+      oq[r] = q;
+      optr[++r] = k;
+    }
+  }
+  const double pw = 4 * kPi / c->nsortf;
+  for (int lm = 0; lm < nl; lm++) {
+    const int64_t q = c->rp[np + lm];
+    for (int b = 0; b < c->nsortf; b++) base[q + b] = -pw * c->ylm[(size_t)b * nl + lm];
+  }
+  std::memcpy(ylm, c->ylm.data(), sizeof(double) * c->ylm.size());
+  for (size_t b = 0; b < c->bnd.size(); b++) brad[b] = c->rad[c->bnd[b]];
+  par[0] = c->clapl[0];
+  par[1] = pw;
+  return 0;
+}
+
 }  // extern "C"

@@ -150,6 +150,52 @@ int fdmgpu_solver_tune(fdmgpu_solver *s, int gk, int trail_wg, int persist_solve
  * written once, every U and L tile of the step range read once). */
 int fdmgpu_solver_stats(const fdmgpu_solver *s, double *out);
 
+/*
+ * Per-energy row assembly on the device, for the energy loop over one
+ * structure.  Replaces, for a cluster without atoms, what calcMatRow
+ * (mat.f90:1034-1727) computes row by row on the CPU for every energy:
+ * the grid diagonal V - E - clapl(0) - i Eimag (mat.f90:1147-1150), the
+ * outer columns and incoming-wave RHS of grid rows touching the outside
+ * (:1312-1398), and the outer rows (:1401-1511), from the outer-sphere
+ * functions of phiso (mat.f90:739-1028, recurrence sphere.f90:7226-7248).
+ * The energy-independent tables (fdmgpu_cluster_assembly_tables in
+ * fdmgpu_cluster.h fills them) are copied to the device once; each call
+ * then writes the value arrays (and RHS) of up to 16 energy points.
+ */
+typedef struct fdmgpu_assembler fdmgpu_assembler;
+/* Host pointers; sizes as fdmgpu_cluster_assembly_sizes reports them. */
+typedef struct {
+  int npoint, nligne, nlmso, lmaxso, nsortf, nb, nrow_out, nlink;
+  int64_t nnz;
+  double clapl0, pw;        /* stencil centre weight, outer-row quadrature weight */
+  const int64_t *row_ptr;   /* [nligne+1] */
+  const double *base;       /* [nnz] energy-independent part of every entry */
+  const int64_t *qdiag;     /* [npoint] position of grid row i's diagonal */
+  const int *orow;          /* [nrow_out] grid rows with outside neighbours (0-based) */
+  const int64_t *oq;        /* [nrow_out] position of each one's first outer column */
+  const int *optr;          /* [nrow_out+1] offsets of its neighbours in ob/ow */
+  const int *ob;            /* [nlink] outer point of each outside neighbour */
+  const double *ow;         /* [nlink] its stencil weight -clapl(v) */
+  const double *ylm;        /* [nb*nlmso] real harmonics at the outer points */
+  const double *brad;       /* [nb] radii of the outer points (bohr) */
+} fdmgpu_assembly_tables;
+/* max_points (1..16): energy points per call; hip_stream may be NULL (library
+ * stream).  The tables are checked for consistency and copied. */
+int fdmgpu_assembler_create(const fdmgpu_assembly_tables *t, int max_points, void *hip_stream,
+                            fdmgpu_assembler **out);
+void fdmgpu_assembler_destroy(fdmgpu_assembler *a);
+/* count points: host arrays Enervide/Eimag/Ecinetic_out[count] (Ry), device
+ * d_V[k] (npoint, Ry), outputs d_vals[k] (nnz values in row_ptr order,
+ * interleaved (re, im) when complex) and d_rhs[k] ((nligne, nlmso) row-major,
+ * interleaved when complex; NULL array or entry = not written).  Complex
+ * arithmetic when E_comp or !basereel (mat.f90:96-104).  update_only != 0:
+ * the buffers already hold the output of an earlier call of this assembler
+ * with the same E_comp/basereel (any energy, any V); only the entries that
+ * depend on the energy or V are rewritten. */
+int fdmgpu_assembler_run(fdmgpu_assembler *a, int count, const double *Enervide, const double *Eimag,
+                         const double *Ecinetic_out, int E_comp, int basereel, int update_only,
+                         const double *const *d_V, double *const *d_vals, double *const *d_rhs);
+
 #ifdef __cplusplus
 }
 #endif

@@ -47,6 +47,33 @@ int fdmgpu_cluster_assemble(const fdmgpu_cluster *c, const double *V, double Ene
                             double Ecinetic_out, int E_comp, int basereel, int64_t *row_ptr, int *col,
                             double *re, double *im, double *rhs_re, double *rhs_im);
 
+/* E-independent tables of one structure's per-energy assembly: the parts of
+ * calcMatRow (mat.f90:1142-1232 grid rows, :1312-1398 outer columns,
+ * :1405-1511 outer rows) that do not change with the energy, for a device
+ * assembler (fdmgpu_assembler_create) to hold while only the energy-
+ * dependent entries are recomputed per point.  Sizes:
+ *   sizes[0] = nrow_out  grid rows with a stencil neighbour outside the sphere
+ *   sizes[1] = nlink     (row, outside neighbour) couplings of those rows
+ *   sizes[2] = nb        outer-sphere points carrying Bessel/Neumann functions
+ *                        (the nsortf inner ones first, then the outside ones)
+ *   sizes[3] = nnz */
+int fdmgpu_cluster_assembly_sizes(const fdmgpu_cluster *c, int64_t *sizes);
+/* base[nnz]      value of every stored entry with the energy terms left out:
+ *                -clapl(v) for interior neighbours, -pw*Ylm for the outer
+ *                rows' boundary columns, 0 where the value depends on E
+ * qdiag[npoint]  position of grid row i's diagonal
+ * orow[nrow_out] grid row (0-based) of each row with outside neighbours,
+ * oq[nrow_out]   position of its first outer column (nlmso columns follow),
+ * optr[nrow_out+1], ob[nlink], ow[nlink]: its outside neighbours in the
+ *                reference's stencil order, as outer-point index and weight
+ *                -clapl(v)
+ * ylm[nb*nlmso]  real harmonics at the outer-sphere points
+ * brad[nb]       their radii (bohr)
+ * par[2]         clapl(0), outer-row quadrature weight pw = 4 pi / nsortf */
+int fdmgpu_cluster_assembly_tables(const fdmgpu_cluster *c, double *base, int64_t *qdiag, int *orow,
+                                   int64_t *oq, int *optr, int *ob, double *ow, double *ylm, double *brad,
+                                   double *par);
+
 #ifdef __cplusplus
 }
 #endif

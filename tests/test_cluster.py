@@ -51,3 +51,65 @@ def test_sources_and_potential():
     y = cfg.real_ylm(3, pts[1:])
     for p, yy in zip(pts[1:], y):
         assert np.abs(yy - O.ylm_real(3, p, float(np.linalg.norm(p)))).max() < 1e-14
+
+
+def _from_tables(t, o, V, E, Ei, Ec, E_comp, basereel):
+    """One energy point rebuilt from the builder's energy-independent tables
+    and the oracle's outer-sphere functions (phiso), in numpy."""
+    cal = E_comp or not basereel
+    B, N = o.phiso(E_comp, Ec, Ei)  # (nsort, lmax+1)
+    cf = N if basereel else B + 1j * N
+    nl, npn = t["nlmso"], t["npoint"]
+    lof = np.array([int(np.floor(np.sqrt(lm) + 1e-9)) for lm in range(nl)])
+    Y = t["ylm"].reshape(t["nb"], nl)
+    vals = t["base"].astype(np.complex128)
+    rhs = np.zeros((t["nligne"], nl), np.complex128)
+    vals[t["qdiag"]] = V - E - t["clapl0"] + (-1j * Ei if E_comp else 0)
+    for r in range(t["nrow_out"]):
+        k0, k1 = t["optr"][r], t["optr"][r + 1]
+        b, w = t["ob"][k0:k1], t["ow"][k0:k1]
+        for lm in range(nl):
+            vals[t["oq"][r] + lm] = np.sum(cf[b, lof[lm]] * Y[b, lm] * w)
+            rhs[t["orow"][r], lm] = -np.sum(Y[b, lm] * B[b, lof[lm]] * w)
+    ns = t["nsortf"]
+    for lm in range(nl):
+        q = t["row_ptr"][npn + lm] + ns
+        for lp in range(nl):
+            c = t["pw"] * Y[:ns, lm] * Y[:ns, lp]
+            vals[q + lp] = np.sum(c * cf[:ns, lof[lp]])
+            rhs[npn + lm, lp] = -np.sum(c * B[:ns, lof[lp]])
+    return (vals, rhs) if cal else (vals.real, rhs.real)
+
+
+@pytest.mark.parametrize("R,h,lm", [(1.2, 0.25, 2), (2.0, 0.2, 3)])
+def test_assembly_tables_rebuild_oracle_rows(R, h, lm):
+    """The tables a device assembler holds, plus the per-energy outer-sphere
+    functions, reproduce the oracle's compressed rows and RHS (calcMatRow,
+    mat.f90:1142-1511) for the three arithmetic branches."""
+    from fdmnes_amd import cluster as CL
+    c = CL.Cluster(R / BOHR, h / BOHR, lm)
+    o = O.Cluster(R / BOHR, h / BOHR, lm)
+    t = c.assembly_tables()
+    assert t["nb"] == o.nsort and t["nnz"] == c.nnz
+    assert np.array_equal(np.sort(t["orow"]), t["orow"]) and t["optr"][-1] == t["nlink"]
+    V = np.random.default_rng(2).uniform(-1, 0, c.npoint)
+    for E_comp, basereel in [(True, False), (False, True), (False, False), (True, True)]:
+        E, Ei, Ec = 0.7, (0.04 if E_comp else 0.0), 0.9
+        cal = E_comp or not basereel
+        orp, ocol, ovals, orhs = o.assemble_csr(V, E, Ei, Ec, E_comp, cal, basereel)
+        vals, rhs = _from_tables(t, o, V, E, Ei, Ec, E_comp, basereel)
+        assert np.array_equal(t["row_ptr"], orp)
+        assert np.abs(vals - ovals).max() <= 1e-14 * np.abs(ovals).max()
+        assert np.abs(rhs - orhs).max() <= 1e-13 * np.abs(orhs).max()
+        # the energy-independent entries are the host assembler's, bit for bit
+        fixed = np.ones(c.nnz, bool)
+        fixed[t["qdiag"]] = False
+        for r in range(t["nrow_out"]):
+            fixed[t["oq"][r]: t["oq"][r] + c.nlmso] = False
+        for l in range(c.nlmso):
+            q = t["row_ptr"][c.npoint + l] + c.nsortf
+            fixed[q: q + c.nlmso] = False
+        _, _, hvals, _ = c.assemble(V, E, Ei, Ec, E_comp, basereel)
+        assert np.array_equal(np.real(hvals)[fixed], t["base"][fixed])
+        if cal:
+            assert not np.any(np.imag(hvals)[fixed])
