@@ -1,5 +1,6 @@
 """Autotuner of the tile schedule: picks the trailing-update depth gk
-(effective panel width 64*gk) and the persistent grid sizes for a system
+(effective panel width 64*gk) and the grid sizes (persistent, or one
+workgroup per output block) for a system
 structure by timing candidate schedules on one group of energy points, and
 caches the choice in a JSON file keyed by structure, precision and group
 size."""
@@ -14,11 +15,16 @@ import numpy as np
 
 CACHE = os.environ.get("FDMGPU_TUNE_CACHE", os.path.join(os.path.dirname(os.path.abspath(__file__)),
                                                          "tune_cache.json"))
+# trail_wg at least the number of output blocks of a launch gives one
+# workgroup per block (the hardware schedules them as slots free up) instead
+# of a persistent grid striding over the blocks
+ONE_PER_BLOCK = 1 << 24
 CANDIDATES = [
-    {"gk": 8, "trail_wg": 768, "solve_wg": 512, "trail_tile": 64},
     {"gk": 8, "trail_wg": 1024, "solve_wg": 512, "trail_tile": 64},
     {"gk": 12, "trail_wg": 768, "solve_wg": 512, "trail_tile": 64},
-    {"gk": 16, "trail_wg": 768, "solve_wg": 512, "trail_tile": 64},
+    {"gk": 8, "trail_wg": ONE_PER_BLOCK, "solve_wg": 512, "trail_tile": 64},
+    {"gk": 12, "trail_wg": ONE_PER_BLOCK, "solve_wg": 512, "trail_tile": 64},
+    {"gk": 16, "trail_wg": ONE_PER_BLOCK, "solve_wg": 512, "trail_tile": 64},
     {"gk": 12, "trail_wg": 512, "solve_wg": 512, "trail_tile": 128},
 ]
 KNOBS = ("gk", "trail_wg", "solve_wg", "trail_tile")
