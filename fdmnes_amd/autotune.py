@@ -25,6 +25,8 @@ CANDIDATES = [
     {"gk": 8, "trail_wg": ONE_PER_BLOCK, "solve_wg": 512, "trail_tile": 64},
     {"gk": 12, "trail_wg": ONE_PER_BLOCK, "solve_wg": 512, "trail_tile": 64},
     {"gk": 16, "trail_wg": ONE_PER_BLOCK, "solve_wg": 512, "trail_tile": 64},
+    {"gk": 20, "trail_wg": ONE_PER_BLOCK, "solve_wg": 512, "trail_tile": 64},
+    {"gk": 24, "trail_wg": ONE_PER_BLOCK, "solve_wg": 512, "trail_tile": 64},
     {"gk": 12, "trail_wg": 512, "solve_wg": 512, "trail_tile": 128},
 ]
 KNOBS = ("gk", "trail_wg", "solve_wg", "trail_tile")

@@ -297,8 +297,8 @@ int fdmgpu_solver_options(fdmgpu_solver *s, int use_graphs, int time_updates, in
 }
 
 int fdmgpu_solver_tune(fdmgpu_solver *s, int gk, int trail_wg, int persist_solve, int solve_wg, int trail_tile) {
-  if (!s || gk > 16 || (trail_tile > 0 && trail_tile != 64 && trail_tile != 128)) {
-    set_error("fdmgpu_solver_tune: gk must be <= 16 and trail_tile 64 or 128");
+  if (!s || gk > MAXGK || (trail_tile > 0 && trail_tile != 64 && trail_tile != 128)) {
+    set_error("fdmgpu_solver_tune: gk must be <= 32 and trail_tile 64 or 128");
     return FDMGPU_EINVAL;
   }
   auto apply = [&](auto *p) {

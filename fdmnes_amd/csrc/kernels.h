@@ -282,6 +282,7 @@ __global__ __launch_bounds__(NT) void k_tile_gemm(const SysDev<R> *__restrict__ 
 // range whose envelope reaches it (I0(K) <= I and J0(K) <= J; I0, J0 are
 // non-decreasing in K so these steps are a prefix of the range).
 constexpr int MAXT = 2 * MAXG;
+constexpr int MAXGK = 32;  // block steps per trailing-update group
 struct TrailItems {
   int count;
   int sys[MAXT];
@@ -352,8 +353,8 @@ __global__ __launch_bounds__(NT, (sizeof(R) == 4 && TJ == 1) ? 3 : 2) void k_tra
   // per step t of the current item (K = Kb + t): I0(K), J0(K), tbase(K),
   // loaded once per item; the tiles of the item then need no global metadata
   // beyond tbase(I), J0(I) of their row
-  __shared__ int kI0[16], kJ0[16];
-  __shared__ int64_t ktb[16];
+  __shared__ int kI0[MAXGK], kJ0[MAXGK];
+  __shared__ int64_t ktb[MAXGK];
   typedef typename Mfma<R>::v4 v4;
   typedef R r2 __attribute__((ext_vector_type(2)));
   const int tid = threadIdx.x;

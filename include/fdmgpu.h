@@ -132,7 +132,7 @@ int fdmgpu_solver_sigma(fdmgpu_solver *s, int count, const int *slots, const dou
  * launches on/off (disables graphs), workgroups of the persistent GEMM. */
 int fdmgpu_solver_options(fdmgpu_solver *s, int use_graphs, int time_updates, int persist_wg);
 /* Tile-schedule parameters (the autotuner's knobs): gk block steps per
- * trailing update (effective panel width 64*gk, 0 = one-step updates),
+ * trailing update (at most 32; effective panel width 64*gk, 0 = one-step updates),
  * workgroups of the persistent trailing-update grid, persistent (1) or
  * per-block (0) substitutions and their workgroup count, and the output
  * columns per workgroup of the group update (64, or 128: two tiles of a
