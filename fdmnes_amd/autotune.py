@@ -27,13 +27,16 @@ CANDIDATES = [
     {"gk": 16, "trail_wg": ONE_PER_BLOCK, "solve_wg": 512, "trail_tile": 64},
     {"gk": 20, "trail_wg": ONE_PER_BLOCK, "solve_wg": 512, "trail_tile": 64},
     {"gk": 24, "trail_wg": ONE_PER_BLOCK, "solve_wg": 512, "trail_tile": 64},
+    {"gk": 20, "trail_wg": ONE_PER_BLOCK, "solve_wg": 512, "trail_tile": 64, "lookahead": 1},
     {"gk": 12, "trail_wg": 512, "solve_wg": 512, "trail_tile": 128},
 ]
-KNOBS = ("gk", "trail_wg", "solve_wg", "trail_tile")
+KNOBS = ("gk", "trail_wg", "solve_wg", "trail_tile", "lookahead")
+DEFAULTS = {"trail_tile": 64, "lookahead": 0}
 
 
 def apply(solver, c: dict) -> None:
-    solver.tune(c["gk"], c["trail_wg"], 1, c["solve_wg"], c.get("trail_tile", 64))
+    solver.tune(c["gk"], c["trail_wg"], 1, c["solve_wg"], c.get("trail_tile", DEFAULTS["trail_tile"]),
+                c.get("lookahead", DEFAULTS["lookahead"]))
 
 
 def key(lb1, lb2, precision: int, group: int) -> str:
@@ -76,7 +79,7 @@ def tune(solver, slots, vals, b, x, lb1, lb2, precision: int, candidates=CANDIDA
             log(f"autotune {c}: {dt:.3f}s")
     best = min(results, key=lambda r: r["seconds"])
     apply(solver, best)
-    entry = {"best": {k2: best[k2] for k2 in KNOBS}, "trials": results}
+    entry = {"best": {k2: best.get(k2, DEFAULTS.get(k2)) for k2 in KNOBS}, "trials": results}
     cache[k] = entry
     try:
         with open(CACHE, "w") as f:

@@ -32,7 +32,7 @@ def _sigs(L):
     L.fdmgpu_solver_sigma.argtypes = [_V, C.c_int, P(C.c_int), P(_V), P(_V), P(C.c_double)]
     L.fdmgpu_solver_options.argtypes = [_V, C.c_int, C.c_int, C.c_int]
     L.fdmgpu_solver_stats.argtypes = [_V, P(C.c_double)]
-    L.fdmgpu_solver_tune.argtypes = [_V, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+    L.fdmgpu_solver_tune.argtypes = [_V, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
     L._solver_sigs = True
 
 
@@ -108,9 +108,9 @@ class DeviceSolver:
     def options(self, use_graphs=True, time_updates=False, persist_wg=0):
         _lib.check(self.L.fdmgpu_solver_options(self._h, int(use_graphs), int(time_updates), int(persist_wg)))
 
-    def tune(self, gk=-1, trail_wg=0, persist_solve=-1, solve_wg=0, trail_tile=0):
+    def tune(self, gk=-1, trail_wg=0, persist_solve=-1, solve_wg=0, trail_tile=0, lookahead=-1):
         _lib.check(self.L.fdmgpu_solver_tune(self._h, int(gk), int(trail_wg), int(persist_solve), int(solve_wg),
-                                             int(trail_tile)))
+                                             int(trail_tile), int(lookahead)))
 
     def solve(self, slots, vals, b, x, max_sweeps=5, tol=1e-12, raise_on_pivot=True):
         """vals/b/x: lists of device tensors (complex128 or float64)."""

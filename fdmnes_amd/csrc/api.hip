@@ -296,7 +296,8 @@ int fdmgpu_solver_options(fdmgpu_solver *s, int use_graphs, int time_updates, in
   return 0;
 }
 
-int fdmgpu_solver_tune(fdmgpu_solver *s, int gk, int trail_wg, int persist_solve, int solve_wg, int trail_tile) {
+int fdmgpu_solver_tune(fdmgpu_solver *s, int gk, int trail_wg, int persist_solve, int solve_wg, int trail_tile,
+                       int lookahead) {
   if (!s || gk > MAXGK || (trail_tile > 0 && trail_tile != 64 && trail_tile != 128)) {
     set_error("fdmgpu_solver_tune: gk must be <= 32 and trail_tile 64 or 128");
     return FDMGPU_EINVAL;
@@ -307,6 +308,7 @@ int fdmgpu_solver_tune(fdmgpu_solver *s, int gk, int trail_wg, int persist_solve
     if (trail_wg > 0) p->grp.persist_wg = trail_wg;
     if (persist_solve >= 0) p->grp.persist_solve = persist_solve != 0;
     if (solve_wg > 0) p->grp.solve_wg = solve_wg;
+    if (lookahead >= 0) p->grp.lookahead = lookahead;
     p->drop_graphs();
   };
   if (s->prec == FDMGPU_FP64) {

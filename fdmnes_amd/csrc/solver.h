@@ -76,6 +76,14 @@ struct Group {
   int solve_wg = 512;        // workgroups of the persistent substitution
   int *ctl = nullptr;        // device: [0] task counter, [1] wait-timeout flag
   hipStream_t stream = nullptr;
+  // Lookahead: at the end of a group the update of the block rows and
+  // columns of the next group runs on `stream`, the rest of the trailing
+  // matrix on `side`, beside the next group's per-step chain.
+  // 0 off; 1 on, the rest on the trailing update's grid; > 1 on, the rest on
+  // a persistent grid of that many workgroups (slots left for the chain)
+  int lookahead = 0;
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fact = nullptr, ev_rest = nullptr;  // stream -> side, side -> stream
   int maxparts = 1;
   // per-kernel timing (optional): cumulative ms of the G_UPDATE launches
   bool time_updates = false;

@@ -166,6 +166,9 @@ int Group<R, C>::finalize(hipStream_t s) {
     FDM_HIP(hipMalloc(&ctl, sizeof(int) * 2));
     FDM_HIP(hipMemset(ctl, 0, sizeof(int) * 2));
   }
+  if (!side) FDM_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+  if (!ev_fact) FDM_HIP(hipEventCreateWithFlags(&ev_fact, hipEventDisableTiming));
+  if (!ev_rest) FDM_HIP(hipEventCreateWithFlags(&ev_rest, hipEventDisableTiming));
   FDM_HIP(hipMalloc(&d_sys, sizeof(SysDev<R>) * h.size()));
   FDM_HIP(hipMemcpy(d_sys, h.data(), sizeof(SysDev<R>) * h.size(), hipMemcpyHostToDevice));
   return 0;
@@ -191,6 +194,11 @@ void Group<R, C>::release() {
   ctl = nullptr;
   for (auto e : ev) (void)hipEventDestroy(e);
   ev.clear();
+  if (ev_fact) (void)hipEventDestroy(ev_fact);
+  if (ev_rest) (void)hipEventDestroy(ev_rest);
+  if (side) (void)hipStreamDestroy(side);
+  ev_fact = ev_rest = nullptr;
+  side = nullptr;
 }
 
 template <class R, bool C>
@@ -263,6 +271,14 @@ static int xcd_grid(int items, int persist_wg) {
 // columns below the group) receives the whole group's rank-(gk*TB) update in
 // one pass at the group's last step.  Without groups (gk = 0) every step
 // updates its whole trailing block right-looking.
+//
+// Lookahead: the next group's chain (its left-looking launches, panels and
+// triangular products) reads and writes only the next group's block rows and
+// columns and tiles outside the current group's rectangle.  So at a group's
+// last step the rectangle is split: its part in the next group's block rows
+// and columns is updated on `stream` before the next chain, the rest on
+// `side` beside it (started after that part); the next group's rectangle
+// waits for the rest.
 template <class R, bool C>
 int Group<R, C>::factor() {
   int maxNB = 0;
@@ -270,27 +286,38 @@ int Group<R, C>::factor() {
     if (active[g]) maxNB = std::max(maxNB, m[g].plan->NB);
   if (time_updates) {
     for (auto e : ev) (void)hipEventDestroy(e);
-    ev.assign(4 * (size_t)maxNB, nullptr);
+    ev.assign(6 * (size_t)maxNB, nullptr);
     for (auto &e : ev) FDM_HIP(hipEventCreate(&e));
   }
   int nev = 0;
   // A start event queued behind unfinished kernels would be stamped before
-  // they complete; drain the stream first so the pair brackets one launch.
-  auto timed = [&](auto launch) -> int {
+  // they complete; drain the launch's stream first so the pair brackets one
+  // launch (work on the other stream may still run beside it).
+  auto timed = [&](hipStream_t st, auto launch) -> int {
     if (time_updates) {
-      FDM_HIP(hipStreamSynchronize(stream));
-      FDM_HIP(hipEventRecord(ev[nev], stream));
+      FDM_HIP(hipStreamSynchronize(st));
+      FDM_HIP(hipEventRecord(ev[nev], st));
     }
     launch();
     if (time_updates) {
-      FDM_HIP(hipEventRecord(ev[nev + 1], stream));
+      FDM_HIP(hipEventRecord(ev[nev + 1], st));
       nev += 2;
     }
     return 0;
   };
+  auto launch_trail = [&](hipStream_t st, const TrailItems &it, int wg) {
+    if (trail_tile == 128)
+      hipLaunchKernelGGL((k_trail<R, C, 2>), dim3(xcd_grid(it.moff[it.count], wg)), dim3(NT), 0, st,
+                         (const SysDev<R> *)d_sys, it);
+    else
+      hipLaunchKernelGGL((k_trail<R, C, 1>), dim3(xcd_grid(it.off[it.count], wg)), dim3(NT), 0, st,
+                         (const SysDev<R> *)d_sys, it);
+  };
+  const int rest_wg = lookahead > 1 ? lookahead : persist_wg;
+  bool rest_pending = false;  // a remainder on `side` not yet waited for by `stream`
   for (int s = 0; s < maxNB; s++) {
     Items ip{}, itr{}, itc{}, itu{};
-    TrailItems cross{}, trail{};
+    TrailItems cross{}, trail{}, rest{};
     for (int g = 0; g < (int)m.size(); g++) {
       if (!active[g]) continue;
       const Plan &pl = *m[g].plan;
@@ -315,14 +342,20 @@ int Group<R, C>::factor() {
         // block column K, rows I0(K)..K-1
         push_trail(cross, g, K + 1, Khi - K, pl.I0[K], nI, K, 1);
       }
-      if (K == Klo)
-        push_trail(trail, g, Klo, Khi - Klo + 1, pl.I0[Klo], Klo - pl.I0[Klo], pl.J0[Klo], Klo - pl.J0[Klo]);
+      if (K == Klo) {
+        const int nk = Khi - Klo + 1, i0 = pl.I0[Klo], j0 = pl.J0[Klo];
+        if (lookahead && Klo > 0) {
+          // the next group's steps are Klo2..Klo-1
+          const int Klo2 = std::max(Klo - gk, 0), a = std::max(i0, Klo2), b = std::max(j0, Klo2);
+          push_trail(trail, g, Klo, nk, a, Klo - a, j0, Klo - j0);      // its block rows
+          push_trail(trail, g, Klo, nk, i0, a - i0, b, Klo - b);        // its block columns above them
+          push_trail(rest, g, Klo, nk, i0, a - i0, j0, b - j0);         // the rest
+        } else {
+          push_trail(trail, g, Klo, nk, i0, Klo - i0, j0, Klo - j0);
+        }
+      }
     }
-    if (cross.count)
-      timed([&]() {
-        hipLaunchKernelGGL((k_trail<R, C, 1>), dim3(xcd_grid(cross.off[cross.count], persist_wg)), dim3(NT), 0,
-                           stream, (const SysDev<R> *)d_sys, cross);
-      });
+    if (cross.count) timed(stream, [&]() { launch_trail(stream, cross, persist_wg); });
     hipLaunchKernelGGL((k_panel<R, C>), dim3(ip.count), dim3(NT), 0, stream, (const SysDev<R> *)d_sys, ip);
     if (itr.count)
       hipLaunchKernelGGL((k_tile_gemm<R, C>), dim3(std::min(itr.off[itr.count], persist_wg)), dim3(NT), 0, stream,
@@ -337,27 +370,38 @@ int Group<R, C>::factor() {
         const int K = itu.K[c], nI = K - pl.I0[K], nJ = K - pl.J0[K];
         update_bytes += tb * (2.0 * nI * nJ + nI + nJ);
       }
-      for (const TrailItems *ti : {&cross, &trail})
+      for (const TrailItems *ti : {&cross, &trail, &rest})
         for (int c = 0; c < ti->count; c++)
           update_bytes += tb * (2.0 * ti->ni[c] * ti->nj[c] + (double)(ti->ni[c] + ti->nj[c]) * ti->nk[c]);
     }
     if (itu.count)
-      timed([&]() {
+      timed(stream, [&]() {
         hipLaunchKernelGGL((k_tile_gemm<R, C>), dim3(std::min(itu.off[itu.count], persist_wg)), dim3(NT), 0, stream,
                            (const SysDev<R> *)d_sys, itu, (int)G_UPDATE);
       });
-    if (trail.count)
-      timed([&]() {
-        if (trail_tile == 128)
-          hipLaunchKernelGGL((k_trail<R, C, 2>), dim3(xcd_grid(trail.moff[trail.count], persist_wg)), dim3(NT), 0,
-                             stream, (const SysDev<R> *)d_sys, trail);
-        else
-          hipLaunchKernelGGL((k_trail<R, C, 1>), dim3(xcd_grid(trail.off[trail.count], persist_wg)), dim3(NT), 0,
-                             stream, (const SysDev<R> *)d_sys, trail);
-      });
+    if (trail.count) {
+      // the rectangle (or its part in the next group's rows and columns)
+      // overlaps the remainders of earlier groups
+      if (rest_pending) {
+        FDM_HIP(hipStreamWaitEvent(stream, ev_rest, 0));
+        rest_pending = false;
+      }
+      timed(stream, [&]() { launch_trail(stream, trail, persist_wg); });
+    }
+    if (rest.count) {
+      // the remainder starts once the next group's rows and columns are
+      // updated, so it does not hold back the next group's first panel
+      FDM_HIP(hipEventRecord(ev_fact, stream));
+      FDM_HIP(hipStreamWaitEvent(side, ev_fact, 0));
+      timed(side, [&]() { launch_trail(side, rest, rest_wg); });
+      FDM_HIP(hipEventRecord(ev_rest, side));
+      rest_pending = true;
+    }
   }
+  if (rest_pending) FDM_HIP(hipStreamWaitEvent(stream, ev_rest, 0));  // join before the substitutions
   FDM_HIP(hipGetLastError());
   if (time_updates) {
+    FDM_HIP(hipStreamSynchronize(side));
     FDM_HIP(hipStreamSynchronize(stream));
     for (int e = 0; e < nev; e += 2) {
       float ms = 0;

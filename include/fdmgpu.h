@@ -136,9 +136,14 @@ int fdmgpu_solver_options(fdmgpu_solver *s, int use_graphs, int time_updates, in
  * workgroups of the persistent trailing-update grid, persistent (1) or
  * per-block (0) substitutions and their workgroup count, and the output
  * columns per workgroup of the group update (64, or 128: two tiles of a
- * block row); <= 0 keeps a value.  Takes effect at the next solve (cached graphs are
- * rebuilt). */
-int fdmgpu_solver_tune(fdmgpu_solver *s, int gk, int trail_wg, int persist_solve, int solve_wg, int trail_tile);
+ * block row); lookahead 1: at the end of a group, the update of the trailing
+ * matrix outside the next group's block rows and columns runs on a second
+ * stream beside the next group's panels (0: one update launch per group;
+ * n > 1: as 1, that update on a persistent grid of n workgroups);
+ * <= 0 keeps a value (< 0 for lookahead).  Takes effect at the next solve
+ * (cached graphs are rebuilt). */
+int fdmgpu_solver_tune(fdmgpu_solver *s, int gk, int trail_wg, int persist_solve, int solve_wg, int trail_tile,
+                       int lookahead);
 /* out[0] ms spent in trailing-update launches (when timed), out[1] their
  * count, out[2] multiply-adds of the trailing updates of one factorisation
  * of every slot at tile granularity, out[3] multiply-adds of the whole

@@ -47,20 +47,25 @@ def test_grouped_uneven_clusters_fp64():
         assert np.abs(x - xo).max() / np.abs(xo).max() <= 1e-10
 
 
+@pytest.mark.parametrize("lookahead", [0, 1, 96])
 @pytest.mark.parametrize("precision", [0, 1])
 @pytest.mark.parametrize("trail_tile,gk,trail_wg", [(64, 0, 64), (64, 3, 64), (128, 3, 64), (128, 8, 64),
-                                                     (128, 16, 64), (64, 24, 1 << 24), (64, 32, 1 << 24)])
-def test_schedules_match_oracle(precision, trail_tile, gk, trail_wg):
+                                                     (128, 16, 64), (64, 2, 1 << 24), (64, 24, 1 << 24),
+                                                     (64, 32, 1 << 24)])
+def test_schedules_match_oracle(precision, trail_tile, gk, trail_wg, lookahead):
     """Every tile schedule (one-step updates, two-level blocking with 64 x 64
     or 2 x 2 macro output tiles, odd item edges, persistent grids or one
-    workgroup per output block, groups up to the 32-step limit) gives the
-    oracle's solution."""
+    workgroup per output block, groups up to the 32-step limit, with and
+    without the lookahead split of each group's update over two streams)
+    gives the oracle's solution; clusters of uneven size end their groups
+    at different steps of the grouped launches."""
     systems = [small_system(R=1.2, seed=11, E=0.4), small_system(R=1.9, seed=12, E=1.1),
                small_system(R=2.2, seed=13, E=2.3)]
-    print("precision", precision, "tile", trail_tile, "gk", gk, "wg", trail_wg, "sizes",
+    print("precision", precision, "tile", trail_tile, "gk", gk, "wg", trail_wg, "lookahead", lookahead, "sizes",
           [len(s["lb1"]) for s in systems], flush=True)
     _, _, xs, sw, _, _, _ = _run(precision, systems, repeats=2,
-                                  tune=dict(gk=gk, trail_wg=trail_wg, trail_tile=trail_tile, persist_solve=1))
+                                  tune=dict(gk=gk, trail_wg=trail_wg, trail_tile=trail_tile, persist_solve=1,
+                                            lookahead=lookahead))
     for s, x in zip(systems, xs):
         xo = oracle_solution(s)
         assert np.abs(x - xo).max() / np.abs(xo).max() <= 1e-10
