@@ -57,7 +57,10 @@ class Cluster:
 
     def __del__(self):
         if getattr(self, "_h", None):
-            _L().fdmgpu_cluster_destroy(self._h)
+            try:
+                _L().fdmgpu_cluster_destroy(self._h)
+            except (TypeError, AttributeError):  # interpreter shutdown: module globals already cleared
+                pass
             self._h = None
 
     @property

@@ -69,8 +69,10 @@ struct Group {
   std::vector<Member> m;
   std::vector<char> active;  // members taking part in factor()/solve()
   SysDev<R> *d_sys = nullptr;
-  int persist_wg = 1024;     // workgroups of the persistent tile GEMMs
-  int gk = 4;                // block steps per trailing update (0: one-step tile GEMM)
+  // grid cap of the tile GEMM launches: a persistent grid of that many
+  // workgroups, or (at least the launch's block count) one per output block
+  int persist_wg = 1 << 24;
+  int gk = 8;                // block steps per trailing update (0: one-step tile GEMM)
   int trail_tile = 64;       // output columns per workgroup of the group update: 64 or 128 (two tiles)
   bool persist_solve = true; // one dependency-driven launch per substitution
   int solve_wg = 512;        // workgroups of the persistent substitution
