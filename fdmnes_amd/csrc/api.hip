@@ -39,13 +39,17 @@ struct DBuf {
   T *p = nullptr;
   size_t n = 0;
   int alloc(size_t count) {
+    release();
     n = count;
     FDM_HIP(hipMalloc(&p, sizeof(T) * (count ? count : 1)));
     return 0;
   }
-  ~DBuf() {
+  void release() {
     if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
   }
+  ~DBuf() { release(); }
 };
 
 int check_csr(int n, const int *lb1, const int *lb2, const int64_t *row_ptr, const int *col) {
@@ -68,20 +72,87 @@ int check_csr(int n, const int *lb1, const int *lb2, const int64_t *row_ptr, con
   return 0;
 }
 
-// Host-pointer drop-in on top of a one-slot device solver.
+// The drop-in keeps its one-slot device solver between calls: FDMNES calls
+// mat_solve once per energy point with the same structure, so the block plan,
+// the factor storage, the pattern on the device, the staging buffers and the
+// captured launch graphs are reused while the precision, the arithmetic, the
+// sizes, the profile and the pattern stay the same (compared exactly).  Any
+// change rebuilds it; fdmgpu_finalize and a device change release it.
+template <class R, bool C>
+struct DropinCache {
+  std::unique_ptr<Solver<R, C>> sv;
+  int n = 0, nrhs = 0, slot = -1;
+  std::vector<int> lb1, lb2, col;
+  std::vector<int64_t> row_ptr;
+  DBuf<double> dv, db, dx;
+  void reset() {
+    sv.reset();
+    dv.release();
+    db.release();
+    dx.release();
+    n = nrhs = 0;
+    slot = -1;
+    lb1.clear();
+    lb2.clear();
+    col.clear();
+    row_ptr.clear();
+  }
+  bool matches(int n_, int nrhs_, const int *l1, const int *l2, const int64_t *rp, const int *cl) const {
+    if (!sv || n_ != n || nrhs_ != nrhs || rp[n_] != row_ptr[n]) return false;
+    return std::equal(l1, l1 + n_, lb1.begin()) && std::equal(l2, l2 + n_, lb2.begin()) &&
+           std::equal(rp, rp + n_ + 1, row_ptr.begin()) && std::equal(cl, cl + rp[n_], col.begin());
+  }
+};
+// heap objects: never destroyed at exit, when the HIP runtime may be gone
+template <class R, bool C>
+DropinCache<R, C> &dropin_cache() {
+  static DropinCache<R, C> *c = new DropinCache<R, C>();
+  return *c;
+}
+void release_dropin_caches() {
+  dropin_cache<double, true>().reset();
+  dropin_cache<double, false>().reset();
+  dropin_cache<float, true>().reset();
+  dropin_cache<float, false>().reset();
+}
+
+// Host-pointer drop-in on top of the cached one-slot device solver.
 template <class R, bool C>
 int host_solve(int n, int nrhs, const int *lb1, const int *lb2, const int64_t *row_ptr, const int *col,
                const double *a_re, const double *a_im, double *smr, double *smi, int max_sweeps, double tol,
                int *sweeps, double *final_rel, int *fail_row, double *t_fill, double *t_tria) {
   constexpr int NC = C ? 2 : 1;
   const double t0 = now();
-  Solver<R, C> sv;
-  sv.use_graphs = false;
-  int slot = -1;
-  if (sv.init(nrhs, g_lib.stream)) return FDMGPU_EHIP;
-  int rc = sv.add_system(n, lb1, lb2, row_ptr, col, &slot);
-  if (rc) return rc == -1 ? FDMGPU_EINVAL : FDMGPU_EHIP;
+  auto &c = dropin_cache<R, C>();
   const int64_t nnz = row_ptr[n];
+  if (!c.matches(n, nrhs, lb1, lb2, row_ptr, col)) {
+    c.reset();
+    int rc = check_csr(n, lb1, lb2, row_ptr, col);
+    if (rc) return rc;
+    c.sv.reset(new (std::nothrow) Solver<R, C>());
+    if (!c.sv) return FDMGPU_ENOMEM;
+    if (c.sv->init(nrhs, g_lib.stream)) {
+      c.reset();
+      return FDMGPU_EHIP;
+    }
+    rc = c.sv->add_system(n, lb1, lb2, row_ptr, col, &c.slot);
+    if (rc) {
+      c.reset();
+      return rc == -1 ? FDMGPU_EINVAL : FDMGPU_EHIP;
+    }
+    if (c.dv.alloc((size_t)std::max<int64_t>(nnz, 1) * NC) || c.db.alloc((size_t)n * nrhs * NC) ||
+        c.dx.alloc((size_t)n * nrhs * NC)) {
+      c.reset();
+      return FDMGPU_ENOMEM;
+    }
+    c.n = n;
+    c.nrhs = nrhs;
+    c.lb1.assign(lb1, lb1 + n);
+    c.lb2.assign(lb2, lb2 + n);
+    c.row_ptr.assign(row_ptr, row_ptr + n + 1);
+    c.col.assign(col, col + nnz);
+  }
+  Solver<R, C> &sv = *c.sv;
   std::vector<double> hv((size_t)std::max<int64_t>(nnz, 1) * NC), hb((size_t)n * nrhs * NC);
   for (int64_t q = 0; q < nnz; q++) {
     hv[q * NC] = a_re[q];
@@ -91,22 +162,24 @@ int host_solve(int n, int nrhs, const int *lb1, const int *lb2, const int64_t *r
     hb[i * NC] = smr[i];
     if (C) hb[i * NC + 1] = smi[i];
   }
-  DBuf<double> dv, db, dx;
-  if (dv.alloc(hv.size()) || db.alloc(hb.size()) || dx.alloc(hb.size())) return FDMGPU_ENOMEM;
-  FDM_HIP(hipMemcpy(dv.p, hv.data(), sizeof(double) * hv.size(), hipMemcpyHostToDevice));
-  FDM_HIP(hipMemcpy(db.p, hb.data(), sizeof(double) * hb.size(), hipMemcpyHostToDevice));
+  FDM_HIP(hipMemcpy(c.dv.p, hv.data(), sizeof(double) * hv.size(), hipMemcpyHostToDevice));
+  FDM_HIP(hipMemcpy(c.db.p, hb.data(), sizeof(double) * hb.size(), hipMemcpyHostToDevice));
   const double t1 = now();
-  const double *vp = dv.p, *bp = db.p;
-  double *xp = dx.p;
+  const double *vp = c.dv.p, *bp = c.db.p;
+  double *xp = c.dx.p;
   int fr = 0, sw = 0;
   double rel = 0;
-  if (sv.solve(1, &slot, &vp, &bp, &xp, max_sweeps, tol, &fr, &sw, &rel)) return FDMGPU_EHIP;
+  const int slot = c.slot;
+  if (sv.solve(1, &slot, &vp, &bp, &xp, max_sweeps, tol, &fr, &sw, &rel)) {
+    c.reset();  // a failed launch may leave the solver in any state
+    return FDMGPU_EHIP;
+  }
   const double t2 = now();
   if (t_fill) *t_fill += t1 - t0;
   if (t_tria) *t_tria += t2 - t1;
   if (fail_row) *fail_row = fr;
   if (fr) return FDMGPU_PIVOT;
-  FDM_HIP(hipMemcpy(hb.data(), dx.p, sizeof(double) * hb.size(), hipMemcpyDeviceToHost));
+  FDM_HIP(hipMemcpy(hb.data(), c.dx.p, sizeof(double) * hb.size(), hipMemcpyDeviceToHost));
   for (int64_t i = 0; i < (int64_t)n * nrhs; i++) {
     smr[i] = hb[i * NC];
     if (C) smi[i] = hb[i * NC + 1];
@@ -131,6 +204,10 @@ int fdmgpu_init(int device) {
     set_error("no HIP device " + std::to_string(device));
     return FDMGPU_EINVAL;
   }
+  if (g_lib.ready) {  // another device: the cached drop-in solvers belong to the old one
+    FDM_HIP(hipSetDevice(g_lib.device));
+    release_dropin_caches();
+  }
   FDM_HIP(hipSetDevice(device));
   if (g_lib.stream) (void)hipStreamDestroy(g_lib.stream);
   FDM_HIP(hipStreamCreateWithFlags(&g_lib.stream, hipStreamNonBlocking));
@@ -140,6 +217,10 @@ int fdmgpu_init(int device) {
 }
 
 int fdmgpu_finalize(void) {
+  if (g_lib.ready) {
+    (void)hipSetDevice(g_lib.device);
+    release_dropin_caches();
+  }
   if (g_lib.stream) (void)hipStreamDestroy(g_lib.stream);
   g_lib = Lib();
   return 0;

@@ -48,7 +48,7 @@ extern "C" {
 /* Library set-up on HIP device `device` (no reference counterpart: the
  * Fortran program has no device).  Idempotent. */
 int fdmgpu_init(int device);
-/* Releases cached device buffers. */
+/* Releases cached device buffers and the drop-in's kept solvers. */
 int fdmgpu_finalize(void);
 const char *fdmgpu_last_error(void);
 /* Name of the solver, 5 characters, as getSolverParams returns it
@@ -63,6 +63,9 @@ void fdmgpu_solver_name(char out[5]);
  * t_fill / t_tria (may be NULL) are incremented by the seconds spent in
  * device assembly and in factorisation + substitution, like the reference's
  * Time_fill / Time_tria accumulators (mat_solve_gaussian.f90:198-315).
+ * The device solver of the last structure (profile and pattern compared
+ * exactly) is kept for the next call; fdmgpu_finalize releases it.  Not
+ * reentrant across threads (neither is mat_solve).
  */
 int fdmgpu_mat_solve(int nligne, int nlmso, int cal_comp, const int *lb1, const int *lb2,
                      const int64_t *row_ptr, const int *col, const double *a_re, const double *a_im,

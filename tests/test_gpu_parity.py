@@ -46,3 +46,28 @@ def test_mixed_matches_oracle():
     rel = np.abs(x - x_or).max() / np.abs(x_or).max()
     assert sweeps >= 0 and rel_res <= 1e-12
     assert rel <= 1e-10, rel
+
+
+def test_dropin_reuses_its_solver_across_energies():
+    """The drop-in keeps its device solver while the structure stays the same
+    (FDMNES's energy loop): consecutive energies of one cluster, another
+    cluster in between, and the first energy again all give the oracle's
+    solutions, and a repeated call gives the same bits."""
+    from fdmnes_amd import solver
+    cases = [_case(2.0, 0.25, 3, True, True, False, E=e) for e in (0.4, 1.1)]
+    other = _case(1.6, 0.25, 2, True, True, False, E=0.8)
+    seq = [cases[0], cases[1], other, cases[0], cases[1]]
+    first = None
+    for k, (cl, lb1, lb2, rp, col, vals, rhs, x_or) in enumerate(seq):
+        print(f"call {k}: n={cl.nligne}", flush=True)
+        x, _, _ = solver.mat_solve(lb1, lb2, rp, col, vals, rhs, True)
+        rel = np.abs(x - x_or).max() / np.abs(x_or).max()
+        assert rel <= 1e-10, (k, rel)
+        if k == 0:
+            first = x.copy()
+        if k == 3:
+            assert np.array_equal(x, first)
+    # mixed precision over the same sequence (its own cached solver)
+    for k, (cl, lb1, lb2, rp, col, vals, rhs, x_or) in enumerate(seq[:3]):
+        x, sweeps, rel_res = solver.mat_solve_mixed(lb1, lb2, rp, col, vals, rhs, True)
+        assert rel_res <= 1e-12 and np.abs(x - x_or).max() / np.abs(x_or).max() <= 1e-10, k
