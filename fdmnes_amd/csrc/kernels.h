@@ -1373,26 +1373,33 @@ __global__ void k_fill(R *__restrict__ a, size_t count, R v) {
 }
 
 // Scatter compressed rows (1-based columns) into the tiles; padded rows get a
-// unit diagonal.  Values are interleaved (re, im) when complex.  One thread
-// per row.
+// unit diagonal.  Values are interleaved (re, im) when complex.  One wave per
+// row, its lanes striding over the row's entries (the outer-sphere rows hold
+// thousands), the row's block metadata loaded once.
+constexpr int SCATTER_ROWS = 4;  // rows (waves) per workgroup
 template <class R, bool C>
-__global__ void k_scatter_csr(DevPlan p, R *__restrict__ tiles, const int64_t *__restrict__ row_ptr,
-                              const int *__restrict__ col, const double *__restrict__ vals) {
+__global__ __launch_bounds__(64 * SCATTER_ROWS) void k_scatter_csr(DevPlan p, R *__restrict__ tiles,
+                                                                   const int64_t *__restrict__ row_ptr,
+                                                                   const int *__restrict__ col,
+                                                                   const double *__restrict__ vals) {
   constexpr int NC = Elem<R, C>::NC;
-  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  constexpr int64_t TE = (int64_t)TB * TB * NC;
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * SCATTER_ROWS + (threadIdx.x >> 6);
   if (row >= p.NB * TB) return;
   const int I = row / TB, rr = row % TB;
   if (row >= p.n) {
-    const gptr<R> T = tile_ptr<R, C>(p, tiles, I, I);
-    T[(int64_t)(rr * TB + rr) * NC] = R(1);
+    if (lane == 0) tile_ptr<R, C>(p, tiles, I, I)[(int64_t)(rr * TB + rr) * NC] = R(1);
     return;
   }
-  for (int64_t q = row_ptr[row]; q < row_ptr[row + 1]; q++) {
-    const int c = col[q] - 1;
-    const gptr<R> T = tile_ptr<R, C>(p, tiles, I, c / TB);
-    const int64_t e = (int64_t)(rr * TB + (c % TB)) * NC;
-    T[e] = R(vals[q * NC]);
-    if constexpr (C) T[e + 1] = R(vals[q * NC + 1]);
+  const gptr<R> T0 = G(tiles) + G(p.tbase)[I] * TE;  // tile (I, J0(I))
+  const int j0 = G(p.J0)[I];
+  const int64_t q1 = G(row_ptr)[row + 1];
+  for (int64_t q = G(row_ptr)[row] + lane; q < q1; q += 64) {
+    const int c = G(col)[q] - 1;
+    const int64_t e = (c / TB - j0) * TE + (int64_t)(rr * TB + (c % TB)) * NC;
+    T0[e] = R(G(vals)[q * NC]);
+    if constexpr (C) T0[e + 1] = R(G(vals)[q * NC + 1]);
   }
 }
 

@@ -222,8 +222,8 @@ template <class R, bool C>
 int Group<R, C>::scatter_csr(int g, const int64_t *d_row_ptr, const int *d_col, const double *d_vals) {
   auto &mb = m[g];
   const int rows = mb.plan->NB * TB;
-  hipLaunchKernelGGL((k_scatter_csr<R, C>), dim3((rows + 255) / 256), dim3(256), 0, stream, mb.plan->dev(), mb.tiles,
-                     d_row_ptr, d_col, d_vals);
+  hipLaunchKernelGGL((k_scatter_csr<R, C>), dim3((rows + SCATTER_ROWS - 1) / SCATTER_ROWS), dim3(64 * SCATTER_ROWS), 0,
+                     stream, mb.plan->dev(), mb.tiles, d_row_ptr, d_col, d_vals);
   FDM_HIP(hipGetLastError());
   return 0;
 }
