@@ -128,7 +128,7 @@ template <> struct Mfma<float> {
   static __device__ __forceinline__ int row(int lk, int v) { return 4 * lk + v; }
 };
 
-enum GemmMode { G_UPDATE = 0, G_ROWTRSM = 1, G_COLTRSM = 2 };
+enum GemmMode { G_UPDATE = 0, G_TRSM = 1, G_NEIGH = 2 };
 
 // Stage one TB x TB tile (global, interleaved) into split LDS planes.
 template <class R, bool C, int LD, class P>
@@ -153,15 +153,16 @@ __device__ __forceinline__ void stage_tile(P g, R (*s)[TB][LD]) {
   }
 }
 
-template <class R, bool C, int LD>
-__device__ void lower_inverse(const R (*P)[TB][LD], gptr<R> Li);
-
 // C(TB x TB) <- f(A * B), one output tile per work item; a persistent grid
 // strides over the work items of all grouped systems.
-//   G_UPDATE : tile(I,J) -= tile(I,K) * tile(K,J)   per item nI * nJ workgroups
-//   G_ROWTRSM: tile(K,J)  = Uinv(K)   * tile(K,J)   per item nJ          (L rows),
-//              + MU(K) when tile (K,K+1) exists, + L_D^-1 of tile (K,K)
-//   G_COLTRSM: tile(I,K)  = tile(I,K) * Linv(K)     per item nI          (U columns)
+//   G_UPDATE: tile(I,J) -= tile(I,K) * tile(K,J)   per item nI * nJ workgroups
+//   G_TRSM:   the products of block step K once the panel has formed U_D^-1
+//             and L_D^-1, per item nJ + nI workgroups:
+//     tile(K,J) = Uinv(K) * tile(K,J), J0(K) <= J < K      (L row)
+//     tile(I,K) = tile(I,K) * Linv(K), I0(K) <= I < K      (U column)
+//   G_NEIGH:  after the last block step, the substitutions' neighbour
+//             products of every block K, per item 2 NB workgroups:
+//     MU(K) = Uinv(K) * U(K,K+1), ML(K) = Linv(K) * L(K,K-1) (where the tiles exist)
 // Four waves; wave w owns the 32x32 quadrant (w/2, w%2), 2x2 MFMA blocks of
 // 16x16; a complex product is four real MFMAs (no 3M recombination).
 template <class R, bool C>
@@ -188,31 +189,28 @@ __global__ __launch_bounds__(NT) void k_tile_gemm(const SysDev<R> *__restrict__ 
     A = tile_ptr<R, C>(p, S.tiles, I, K);
     B = tile_ptr<R, C>(p, S.tiles, K, J);
     Cg = tile_ptr<R, C>(p, S.tiles, I, J);
-  } else if (mode == G_ROWTRSM) {
-    const int mu = K + 1 <= p.J1[K] ? 1 : 0;
-    if (b == nJ + mu) {  // last block of the item: L_D^-1 of the factored diagonal tile
-      stage_tile<R, C, LD>(tile_ptr<R, C>(p, S.tiles, K, K), As);
-      __syncthreads();
-      lower_inverse<R, C, LD>(As, dinv_tile<R, C>(S, K, D_LINV));
-      continue;
-    }
-    A = dinv_tile<R, C>(S, K, D_UINV);
+  } else if (mode == G_TRSM) {
     if (b < nJ) {
       Cg = tile_ptr<R, C>(p, S.tiles, K, J0 + b);
+      A = dinv_tile<R, C>(S, K, D_UINV);
       B = Cg;
-    } else {  // MU(K) = Uinv(K) U(K,K+1); U(K,K+1) was finished at step K+1
-      B = tile_ptr<R, C>(p, S.tiles, K, K + 1);
-      Cg = dinv_tile<R, C>(S, K, D_MU);
-    }
-  } else {
-    if (b < K - I0) {
-      Cg = tile_ptr<R, C>(p, S.tiles, I0 + b, K);
+    } else {
+      Cg = tile_ptr<R, C>(p, S.tiles, I0 + (b - nJ), K);
       A = Cg;
       B = dinv_tile<R, C>(S, K, D_LINV);
-    } else {  // ML(K) = Linv(K) L(K,K-1); L(K,K-1) was finished by this step's ROWTRSM launch
-      A = dinv_tile<R, C>(S, K, D_LINV);
-      B = tile_ptr<R, C>(p, S.tiles, K, K - 1);
-      Cg = dinv_tile<R, C>(S, K, D_ML);
+    }
+  } else {
+    const int Kn = b >> 1;
+    if (b & 1) {
+      if (Kn - 1 < p.J0[Kn]) continue;  // no tile (K, K-1): block row K starts at K
+      A = dinv_tile<R, C>(S, Kn, D_LINV);
+      B = tile_ptr<R, C>(p, S.tiles, Kn, Kn - 1);
+      Cg = dinv_tile<R, C>(S, Kn, D_ML);
+    } else {
+      if (Kn + 1 > p.J1[Kn]) continue;  // no tile (K, K+1)
+      A = dinv_tile<R, C>(S, Kn, D_UINV);
+      B = tile_ptr<R, C>(p, S.tiles, Kn, Kn + 1);
+      Cg = dinv_tile<R, C>(S, Kn, D_MU);
     }
   }
   stage_tile<R, C, LD>(A, As);
@@ -224,38 +222,42 @@ __global__ __launch_bounds__(NT) void k_tile_gemm(const SysDev<R> *__restrict__ 
   const int r0 = (wave >> 1) * 32, c0 = (wave & 1) * 32;
   const int li = lane & 15, lk = lane >> 4;
   v4 accr[2][2], acci[2][2];
-#pragma unroll
-  for (int a = 0; a < 2; a++)
-#pragma unroll
-    for (int bb = 0; bb < 2; bb++) {
-      accr[a][bb] = v4{0, 0, 0, 0};
-      acci[a][bb] = v4{0, 0, 0, 0};
-    }
-#pragma unroll 4
-  for (int k0 = 0; k0 < TB; k0 += 4) {
-    R ar[2], ai[2], br[2], bi[2];
-#pragma unroll
-    for (int a = 0; a < 2; a++) {
-      ar[a] = As[0][r0 + a * 16 + li][k0 + lk];
-      if constexpr (C) ai[a] = As[1][r0 + a * 16 + li][k0 + lk];
-    }
-#pragma unroll
-    for (int bb = 0; bb < 2; bb++) {
-      br[bb] = Bs[0][k0 + lk][c0 + bb * 16 + li];
-      if constexpr (C) bi[bb] = Bs[1][k0 + lk][c0 + bb * 16 + li];
-    }
+  // acc = As * Bs
+  auto product = [&]() {
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
       for (int bb = 0; bb < 2; bb++) {
-        accr[a][bb] = Mfma<R>::run(ar[a], br[bb], accr[a][bb]);
-        if constexpr (C) {
-          accr[a][bb] = Mfma<R>::run(-ai[a], bi[bb], accr[a][bb]);
-          acci[a][bb] = Mfma<R>::run(ar[a], bi[bb], acci[a][bb]);
-          acci[a][bb] = Mfma<R>::run(ai[a], br[bb], acci[a][bb]);
-        }
+        accr[a][bb] = v4{0, 0, 0, 0};
+        acci[a][bb] = v4{0, 0, 0, 0};
       }
-  }
+#pragma unroll 4
+    for (int k0 = 0; k0 < TB; k0 += 4) {
+      R ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+      for (int a = 0; a < 2; a++) {
+        ar[a] = As[0][r0 + a * 16 + li][k0 + lk];
+        if constexpr (C) ai[a] = As[1][r0 + a * 16 + li][k0 + lk];
+      }
+#pragma unroll
+      for (int bb = 0; bb < 2; bb++) {
+        br[bb] = Bs[0][k0 + lk][c0 + bb * 16 + li];
+        if constexpr (C) bi[bb] = Bs[1][k0 + lk][c0 + bb * 16 + li];
+      }
+#pragma unroll
+      for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int bb = 0; bb < 2; bb++) {
+          accr[a][bb] = Mfma<R>::run(ar[a], br[bb], accr[a][bb]);
+          if constexpr (C) {
+            accr[a][bb] = Mfma<R>::run(-ai[a], bi[bb], accr[a][bb]);
+            acci[a][bb] = Mfma<R>::run(ar[a], bi[bb], acci[a][bb]);
+            acci[a][bb] = Mfma<R>::run(ai[a], br[bb], acci[a][bb]);
+          }
+        }
+    }
+  };
+  product();
 #pragma unroll
   for (int a = 0; a < 2; a++)
 #pragma unroll
@@ -697,10 +699,10 @@ __device__ __forceinline__ R quad_bcast(R v, int sel) {
 // P (LDS), written to Li: four lanes per column j = tid/4 hold x_k
 // (k = p + 4t) in registers; column j from its diagonal down.
 template <class R, bool C, int LD>
-__device__ void lower_inverse(const R (*P)[TB][LD], gptr<R> Li) {
+__device__ void lower_inverse(const R (*P)[TB][LD], gptr<R> Li, int tid) {
   constexpr int NC = Elem<R, C>::NC;
   constexpr int NQ = TB / 4;
-  const int tid = threadIdx.x, j = tid >> 2, part = tid & 3;
+  const int j = tid >> 2, part = tid & 3;
   R xr[NQ], xi[NQ];
 #pragma unroll
   for (int t = 0; t < NQ; t++) xr[t] = xi[t] = 0;
@@ -737,13 +739,16 @@ __device__ void lower_inverse(const R (*P)[TB][LD], gptr<R> Li) {
 }
 
 template <class R, bool C>
-__global__ __launch_bounds__(NT) void k_panel(const SysDev<R> *__restrict__ sys, Items it) {
+__global__ __launch_bounds__(2 * NT) void k_panel(const SysDev<R> *__restrict__ sys, Items it) {
   constexpr int NC = Elem<R, C>::NC;
   constexpr int LD = TB + 1;
   constexpr int NQ = TB / 4;
   __shared__ R P[NC][TB][LD];  // published rows, then the factored tile
   __shared__ R Vd[NC][TB];     // 1 / U_D(i,i)
   const int tid = threadIdx.x;
+  // threads 0..NT-1 eliminate (row tid/4, columns tid%4 + 4t) and then form
+  // U_D^-1; threads NT..2NT-1 wait at the barriers and form L_D^-1 beside it
+  const bool el = tid < NT;
   const int myr = tid >> 2, part = tid & 3;
   const SysDev<R> &S = sys[it.sys[blockIdx.x]];
   const DevPlan &p = S.p;
@@ -753,8 +758,8 @@ __global__ __launch_bounds__(NT) void k_panel(const SysDev<R> *__restrict__ sys,
 #pragma unroll
   for (int t = 0; t < NQ; t++) {
     const int64_t e = (int64_t)(myr * TB + part + 4 * t) * NC;
-    ar[t] = T[e];
-    ai[t] = C ? T[e + 1] : R(0);
+    ar[t] = el ? T[e] : R(0);
+    ai[t] = (C && el) ? T[e + 1] : R(0);
   }
 #pragma unroll 1
   for (int q = TB - 1; q >= 0; --q) {
@@ -814,16 +819,22 @@ __global__ __launch_bounds__(NT) void k_panel(const SysDev<R> *__restrict__ sys,
     }
   }
   __syncthreads();  // every read of the published rows done
+  if (el) {
 #pragma unroll
-  for (int t = 0; t < NQ; t++) {
-    const int k = part + 4 * t;
-    const int64_t e = (int64_t)(myr * TB + k) * NC;
-    T[e] = ar[t];
-    if constexpr (C) T[e + 1] = ai[t];
-    P[0][myr][k] = ar[t];
-    if constexpr (C) P[NC - 1][myr][k] = ai[t];
+    for (int t = 0; t < NQ; t++) {
+      const int k = part + 4 * t;
+      const int64_t e = (int64_t)(myr * TB + k) * NC;
+      T[e] = ar[t];
+      if constexpr (C) T[e + 1] = ai[t];
+      P[0][myr][k] = ar[t];
+      if constexpr (C) P[NC - 1][myr][k] = ai[t];
+    }
   }
   __syncthreads();
+  if (!el) {  // L_D^-1 of the factored tile (unit lower part)
+    lower_inverse<R, C, LD>(P, dinv_tile<R, C>(S, K, D_LINV), tid - NT);
+    return;
+  }
   const int j = tid >> 2;  // column of the inverses
   R xr[NQ], xi[NQ];
   // X = U_D^-1: U X = I, column j from its last row up.

@@ -316,7 +316,7 @@ int Group<R, C>::factor() {
   const int rest_wg = lookahead > 1 ? lookahead : persist_wg;
   bool rest_pending = false;  // a remainder on `side` not yet waited for by `stream`
   for (int s = 0; s < maxNB; s++) {
-    Items ip{}, itr{}, itc{}, itu{};
+    Items ip{}, itx{}, itu{};
     TrailItems cross{}, trail{}, rest{};
     for (int g = 0; g < (int)m.size(); g++) {
       if (!active[g]) continue;
@@ -326,11 +326,7 @@ int Group<R, C>::factor() {
       const int K = pl.NB - 1 - t;
       const int nJ = K - pl.J0[K], nI = K - pl.I0[K];
       push(ip, g, K, 1);
-      // one more workgroup each for the substitutions' neighbour products
-      // MU(K) (tile (K,K+1) exists) and ML(K) (tile (K,K-1) exists)
-      const int mu = K + 1 <= pl.J1[K] ? 1 : 0, ml = K - 1 >= pl.J0[K] ? 1 : 0;
-      push(itr, g, K, nJ + mu + 1);  // + one workgroup forming L_D^-1
-      if (nI + ml > 0) push(itc, g, K, nI + ml);
+      if (nJ + nI > 0) push(itx, g, K, nJ + nI);  // L row and U column in one launch
       if (gk <= 0) {
         if (nI > 0 && nJ > 0) push(itu, g, K, nI * nJ);
         continue;
@@ -356,13 +352,10 @@ int Group<R, C>::factor() {
       }
     }
     if (cross.count) timed(stream, [&]() { launch_trail(stream, cross, persist_wg); });
-    hipLaunchKernelGGL((k_panel<R, C>), dim3(ip.count), dim3(NT), 0, stream, (const SysDev<R> *)d_sys, ip);
-    if (itr.count)
-      hipLaunchKernelGGL((k_tile_gemm<R, C>), dim3(std::min(itr.off[itr.count], persist_wg)), dim3(NT), 0, stream,
-                         (const SysDev<R> *)d_sys, itr, (int)G_ROWTRSM);
-    if (itc.count)
-      hipLaunchKernelGGL((k_tile_gemm<R, C>), dim3(std::min(itc.off[itc.count], persist_wg)), dim3(NT), 0, stream,
-                         (const SysDev<R> *)d_sys, itc, (int)G_COLTRSM);
+    hipLaunchKernelGGL((k_panel<R, C>), dim3(ip.count), dim3(2 * NT), 0, stream, (const SysDev<R> *)d_sys, ip);
+    if (itx.count)
+      hipLaunchKernelGGL((k_tile_gemm<R, C>), dim3(std::min(itx.off[itx.count], persist_wg)), dim3(NT), 0, stream,
+                         (const SysDev<R> *)d_sys, itx, (int)G_TRSM);
     if (time_updates) {
       const double tb = (double)TB * TB * NC * sizeof(R);
       for (int c = 0; c < itu.count; c++) {
@@ -399,6 +392,14 @@ int Group<R, C>::factor() {
     }
   }
   if (rest_pending) FDM_HIP(hipStreamWaitEvent(stream, ev_rest, 0));  // join before the substitutions
+  // the substitutions' neighbour products MU(K), ML(K) of every block, off
+  // the per-step chain: their tiles are final once steps K and K+1 are done
+  Items nb{};
+  for (int g = 0; g < (int)m.size(); g++)
+    if (active[g]) push(nb, g, 0, 2 * m[g].plan->NB);
+  if (nb.count)
+    hipLaunchKernelGGL((k_tile_gemm<R, C>), dim3(std::min(nb.off[nb.count], persist_wg)), dim3(NT), 0, stream,
+                       (const SysDev<R> *)d_sys, nb, (int)G_NEIGH);
   FDM_HIP(hipGetLastError());
   if (time_updates) {
     FDM_HIP(hipStreamSynchronize(side));
