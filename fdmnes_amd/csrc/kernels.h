@@ -645,111 +645,113 @@ __global__ __launch_bounds__(NT, (sizeof(R) == 4 && TJ == 1) ? 3 : 2) void k_tra
 // ---------------------------------------------------------------- panel
 // UL factorisation of the diagonal tile (K,K) without pivoting, from its last
 // row to its first as mat_solve_gaussian.f90:196-302 does row by row, then the
-// inverse U_D^-1 into dinv(K) (L_D^-1 is formed by the L-row launch that
-// follows, beside the L-row products, which need only U_D^-1).  The pivot test is the reference's:
-// |a|^2 < 1e-20 (complex) or |a| < 1e-10 (real); the largest failing row
-// (the first in elimination order) is recorded 1-based in *fail.  One
-// workgroup per item.
+// inverses U_D^-1 and L_D^-1 into dinv(K).  The pivot test is the
+// reference's: |a|^2 < 1e-20 (complex) or |a| < 1e-10 (real); the largest
+// failing row (the first in elimination order) is recorded 1-based in *fail.
+// One workgroup of PNT threads per item.
 //
-// Thread (r = tid/4, p = tid%4) keeps the 16 elements (r, p + 4t) of the tile
-// in registers; the step loops stay rolled (a short loop body keeps the
-// instruction cache warm), so the element of the current step is picked from
-// the register array by selects over the constant index t.  Elimination step q: the four threads of row q take the
-// pivot from their group (lane shuffle), normalise the row's lower part (the
-// stored L row, a_qk / a_qq) and publish it to LDS with zeros from column q
-// on; after one barrier every row r < q subtracts a_rq * row q, a_rq coming
-// from its own group by shuffle.  The inverses then run without barriers:
-// four lanes per column j hold x_k (k = p + 4t) in registers; entries not yet
-// computed are zero, so each substitution step is a full 64-term dot product
-// with the published factor row, reduced across the four lanes by shuffles.
-// Lane moves within groups of four lanes through DPP quad permutations
-// (register to register; __shfl would go through LDS with ds_bpermute).
+// Elimination: PL lanes per row; lane p of row r keeps the elements
+// (r, p + PL t) in registers (the step loop stays rolled, so the element of
+// the current step is picked by selects over t).  Step q: the lanes of row q
+// take the pivot from their row (lane shuffle), normalise the row's lower part
+// (the stored L row, a_qk / a_qq) and publish it to LDS with zeros from column
+// q on; after one barrier every row r < q subtracts a_rq * row q, a_rq coming
+// from its own row's lanes.  Every element receives the reference's updates in
+// its order.  Inverses: the two halves of the workgroup form U_D^-1 and L_D^-1
+// at the same time, IL lanes per column j holding x_k (k = p + IL t); entries
+// not yet computed are zero, so each substitution step is a full 64-term dot
+// product with the published factor row, reduced across the IL lanes.
+constexpr int PNT = 1024;  // panel workgroup
+constexpr int PL = 16;     // lanes per row in the elimination
+constexpr int IL = 8;      // lanes per column in the inverses
+
 template <int CTRL>
-__device__ __forceinline__ float quad_dpp(float v) {
+__device__ __forceinline__ float dpp(float v) {
   return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xf, 0xf, false));
 }
 template <int CTRL>
-__device__ __forceinline__ double quad_dpp(double v) {
+__device__ __forceinline__ double dpp(double v) {
   const long long b = __builtin_bit_cast(long long, v);
   const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), CTRL, 0xf, 0xf, false);
   const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
   return __builtin_bit_cast(double, (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 
-// Sum over the four lanes of a quad (every lane gets the sum).
+// Sum over each group of eight lanes (every lane gets its group's sum):
+// within quads by DPP quad permutations, then across the two quads of the
+// group by the half-row mirror (lane i and lane 7 - i).
 template <class R>
-__device__ __forceinline__ R lane_sum4(R v) {
-  v += quad_dpp<0xB1>(v);  // lanes 1,0,3,2
-  v += quad_dpp<0x4E>(v);  // lanes 2,3,0,1
+__device__ __forceinline__ R lane_sum8(R v) {
+  v += dpp<0xB1>(v);   // lanes 1,0,3,2
+  v += dpp<0x4E>(v);   // lanes 2,3,0,1
+  v += dpp<0x141>(v);  // half-row mirror
   return v;
 }
 
-// Value of lane sel (0..3, the same in every lane) of each quad.
-template <class R>
-__device__ __forceinline__ R quad_bcast(R v, int sel) {
-  switch (sel) {
-    case 0: return quad_dpp<0x00>(v);
-    case 1: return quad_dpp<0x55>(v);
-    case 2: return quad_dpp<0xAA>(v);
-    default: return quad_dpp<0xFF>(v);
-  }
-}
-
-// X = L_D^-1 of the unit-lower part of a factored diagonal tile staged in
-// P (LDS), written to Li: four lanes per column j = tid/4 hold x_k
-// (k = p + 4t) in registers; column j from its diagonal down.
-template <class R, bool C, int LD>
-__device__ void lower_inverse(const R (*P)[TB][LD], gptr<R> Li, int tid) {
+// X = U_D^-1 (UPPER: from the last row up, Vd[i] = 1 / U(i,i)) or L_D^-1
+// (the unit-lower part, from the first row down) of the factored tile in P,
+// written to X; thread t of the IL * TB taking part: column t / IL.
+template <class R, bool C, int LD, bool UPPER>
+__device__ void tri_inverse(const R (*P)[TB][LD], const R (*Vd)[TB], gptr<R> X, int t) {
   constexpr int NC = Elem<R, C>::NC;
-  constexpr int NQ = TB / 4;
-  const int j = tid >> 2, part = tid & 3;
-  R xr[NQ], xi[NQ];
+  constexpr int IQ = TB / IL;
+  const int j = t / IL, part = t % IL;
+  R xr[IQ], xi[IQ];
 #pragma unroll
-  for (int t = 0; t < NQ; t++) xr[t] = xi[t] = 0;
+  for (int u = 0; u < IQ; u++) xr[u] = xi[u] = 0;
 #pragma unroll 1
-  for (int i = 0; i < TB; ++i) {
+  for (int s = 0; s < TB; ++s) {
+    const int i = UPPER ? TB - 1 - s : s;
     R sr = 0, si = 0;
 #pragma unroll
-    for (int t = 0; t < NQ; t++) {
-      const R lr = P[0][i][part + 4 * t];
+    for (int u = 0; u < IQ; u++) {
+      const R fr = P[0][i][part + IL * u];
       if constexpr (C) {
-        const R li = P[NC - 1][i][part + 4 * t];
-        sr += lr * xr[t] - li * xi[t];
-        si += lr * xi[t] + li * xr[t];
+        const R fi = P[NC - 1][i][part + IL * u];
+        sr += fr * xr[u] - fi * xi[u];
+        si += fr * xi[u] + fi * xr[u];
       } else {
-        sr += lr * xr[t];
+        sr += fr * xr[u];
       }
     }
-    sr = lane_sum4(sr);
-    if constexpr (C) si = lane_sum4(si);
-    const R nr = (i == j ? R(1) : R(0)) - sr, ni = -si;
-    const int ti = part == (i & 3) ? i >> 2 : -1;
+    sr = lane_sum8(sr);
+    if constexpr (C) si = lane_sum8(si);
+    const R br = (i == j ? R(1) : R(0)) - sr, bi = -si;
+    R nr = br, ni = bi;
+    if constexpr (UPPER) {
+      if constexpr (C) {
+        const R vr = Vd[0][i], vi = Vd[NC - 1][i];
+        nr = br * vr - bi * vi;
+        ni = br * vi + bi * vr;
+      } else {
+        nr = br * Vd[0][i];
+      }
+    }
+    const int ti = part == i % IL ? i / IL : -1;
 #pragma unroll
-    for (int t = 0; t < NQ; t++) {
-      xr[t] = t == ti ? nr : xr[t];
-      xi[t] = t == ti ? ni : xi[t];
+    for (int u = 0; u < IQ; u++) {
+      xr[u] = u == ti ? nr : xr[u];
+      xi[u] = u == ti ? ni : xi[u];
     }
   }
 #pragma unroll
-  for (int t = 0; t < NQ; t++) {
-    const int64_t e = (int64_t)((part + 4 * t) * TB + j) * NC;
-    Li[e] = xr[t];
-    if constexpr (C) Li[e + 1] = xi[t];
+  for (int u = 0; u < IQ; u++) {
+    const int64_t e = (int64_t)((part + IL * u) * TB + j) * NC;
+    X[e] = xr[u];
+    if constexpr (C) X[e + 1] = xi[u];
   }
 }
 
 template <class R, bool C>
-__global__ __launch_bounds__(2 * NT) void k_panel(const SysDev<R> *__restrict__ sys, Items it) {
+__global__ __launch_bounds__(PNT) void k_panel(const SysDev<R> *__restrict__ sys, Items it) {
+  static_assert(PNT == TB * PL && PNT == 2 * TB * IL, "one lane group per row, two inverses of IL lanes per column");
   constexpr int NC = Elem<R, C>::NC;
   constexpr int LD = TB + 1;
-  constexpr int NQ = TB / 4;
+  constexpr int NQ = TB / PL;
   __shared__ R P[NC][TB][LD];  // published rows, then the factored tile
   __shared__ R Vd[NC][TB];     // 1 / U_D(i,i)
   const int tid = threadIdx.x;
-  // threads 0..NT-1 eliminate (row tid/4, columns tid%4 + 4t) and then form
-  // U_D^-1; threads NT..2NT-1 wait at the barriers and form L_D^-1 beside it
-  const bool el = tid < NT;
-  const int myr = tid >> 2, part = tid & 3;
+  const int myr = tid / PL, part = tid % PL;
   const SysDev<R> &S = sys[it.sys[blockIdx.x]];
   const DevPlan &p = S.p;
   const int K = it.K[blockIdx.x];
@@ -757,21 +759,21 @@ __global__ __launch_bounds__(2 * NT) void k_panel(const SysDev<R> *__restrict__ 
   R ar[NQ], ai[NQ];
 #pragma unroll
   for (int t = 0; t < NQ; t++) {
-    const int64_t e = (int64_t)(myr * TB + part + 4 * t) * NC;
-    ar[t] = el ? T[e] : R(0);
-    ai[t] = (C && el) ? T[e + 1] : R(0);
+    const int64_t e = (int64_t)(myr * TB + part + PL * t) * NC;
+    ar[t] = T[e];
+    ai[t] = C ? T[e + 1] : R(0);
   }
 #pragma unroll 1
   for (int q = TB - 1; q >= 0; --q) {
-    const int tq = q >> 2;
+    const int tq = q / PL;
     R mr = 0, mi = 0;
 #pragma unroll
     for (int t = 0; t < NQ; t++) {
       mr = t == tq ? ar[t] : mr;
       mi = t == tq ? ai[t] : mi;
     }
-    const R fr = quad_bcast(mr, q & 3);  // a_{myr,q}; the pivot in row q's group
-    const R fi = C ? quad_bcast(mi, q & 3) : R(0);
+    const R fr = __shfl(mr, q % PL, PL);  // a_{myr,q}; the pivot in row q's lanes
+    const R fi = C ? __shfl(mi, q % PL, PL) : R(0);
     if (myr == q) {
       R vr, vi = 0;
       if constexpr (C) {
@@ -785,7 +787,7 @@ __global__ __launch_bounds__(2 * NT) void k_panel(const SysDev<R> *__restrict__ 
       }
 #pragma unroll
       for (int t = 0; t < NQ; t++) {
-        const int k = part + 4 * t;
+        const int k = part + PL * t;
         if (k < q) {  // normalise the stored row, :274-280 / :296-300
           const R lr = ar[t] * vr - ai[t] * vi, li = ai[t] * vr + ar[t] * vi;
           ar[t] = lr;
@@ -806,7 +808,7 @@ __global__ __launch_bounds__(2 * NT) void k_panel(const SysDev<R> *__restrict__ 
     if (myr < q) {  // eliminate column q from row myr
 #pragma unroll
       for (int t = 0; t < NQ; t++) {
-        const int k = part + 4 * t;
+        const int k = part + PL * t;
         const R lr = P[0][q][k];
         if constexpr (C) {
           const R li = P[NC - 1][q][k];
@@ -819,66 +821,20 @@ __global__ __launch_bounds__(2 * NT) void k_panel(const SysDev<R> *__restrict__ 
     }
   }
   __syncthreads();  // every read of the published rows done
-  if (el) {
-#pragma unroll
-    for (int t = 0; t < NQ; t++) {
-      const int k = part + 4 * t;
-      const int64_t e = (int64_t)(myr * TB + k) * NC;
-      T[e] = ar[t];
-      if constexpr (C) T[e + 1] = ai[t];
-      P[0][myr][k] = ar[t];
-      if constexpr (C) P[NC - 1][myr][k] = ai[t];
-    }
-  }
-  __syncthreads();
-  if (!el) {  // L_D^-1 of the factored tile (unit lower part)
-    lower_inverse<R, C, LD>(P, dinv_tile<R, C>(S, K, D_LINV), tid - NT);
-    return;
-  }
-  const int j = tid >> 2;  // column of the inverses
-  R xr[NQ], xi[NQ];
-  // X = U_D^-1: U X = I, column j from its last row up.
-#pragma unroll
-  for (int t = 0; t < NQ; t++) xr[t] = xi[t] = 0;
-#pragma unroll 1
-  for (int i = TB - 1; i >= 0; --i) {
-    R sr = 0, si = 0;
-#pragma unroll
-    for (int t = 0; t < NQ; t++) {
-      const R ur = P[0][i][part + 4 * t];
-      if constexpr (C) {
-        const R ui = P[NC - 1][i][part + 4 * t];
-        sr += ur * xr[t] - ui * xi[t];
-        si += ur * xi[t] + ui * xr[t];
-      } else {
-        sr += ur * xr[t];
-      }
-    }
-    sr = lane_sum4(sr);
-    if constexpr (C) si = lane_sum4(si);
-    const R br = (i == j ? R(1) : R(0)) - sr, bi = -si;
-    R nr, ni = 0;
-    if constexpr (C) {
-      const R vr = Vd[0][i], vi = Vd[NC - 1][i];
-      nr = br * vr - bi * vi;
-      ni = br * vi + bi * vr;
-    } else {
-      nr = br * Vd[0][i];
-    }
-    const int ti = part == (i & 3) ? i >> 2 : -1;
-#pragma unroll
-    for (int t = 0; t < NQ; t++) {
-      xr[t] = t == ti ? nr : xr[t];
-      xi[t] = t == ti ? ni : xi[t];
-    }
-  }
-  const gptr<R> Ui = dinv_tile<R, C>(S, K, D_UINV);
 #pragma unroll
   for (int t = 0; t < NQ; t++) {
-    const int64_t e = (int64_t)((part + 4 * t) * TB + j) * NC;
-    Ui[e] = xr[t];
-    if constexpr (C) Ui[e + 1] = xi[t];
+    const int k = part + PL * t;
+    const int64_t e = (int64_t)(myr * TB + k) * NC;
+    T[e] = ar[t];
+    if constexpr (C) T[e + 1] = ai[t];
+    P[0][myr][k] = ar[t];
+    if constexpr (C) P[NC - 1][myr][k] = ai[t];
   }
+  __syncthreads();
+  if (tid < TB * IL)
+    tri_inverse<R, C, LD, true>(P, Vd, dinv_tile<R, C>(S, K, D_UINV), tid);
+  else
+    tri_inverse<R, C, LD, false>(P, Vd, dinv_tile<R, C>(S, K, D_LINV), tid - TB * IL);
 }
 
 // ---------------------------------------------------------------- solves

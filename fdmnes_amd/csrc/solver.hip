@@ -352,7 +352,7 @@ int Group<R, C>::factor() {
       }
     }
     if (cross.count) timed(stream, [&]() { launch_trail(stream, cross, persist_wg); });
-    hipLaunchKernelGGL((k_panel<R, C>), dim3(ip.count), dim3(2 * NT), 0, stream, (const SysDev<R> *)d_sys, ip);
+    hipLaunchKernelGGL((k_panel<R, C>), dim3(ip.count), dim3(PNT), 0, stream, (const SysDev<R> *)d_sys, ip);
     if (itx.count)
       hipLaunchKernelGGL((k_tile_gemm<R, C>), dim3(std::min(itx.off[itx.count], persist_wg)), dim3(NT), 0, stream,
                          (const SysDev<R> *)d_sys, itx, (int)G_TRSM);
