@@ -131,6 +131,27 @@ template <> struct Mfma<float> {
 enum GemmMode { G_UPDATE = 0, G_TRSM = 1, G_NEIGH = 2 };
 
 // Stage one TB x TB tile (global, interleaved) into split LDS planes.
+// Stage the ROWS x COLS block at (row0, col0) of one TB x TB tile (global,
+// interleaved) into split LDS planes.
+template <class R, bool C, int LD, int ROWS, int COLS, class P>
+__device__ __forceinline__ void stage_block(P g, int row0, int col0, R (*s)[ROWS][LD]) {
+  static_assert(ROWS * COLS % NT == 0, "whole rounds of the workgroup");
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < ROWS * COLS / NT; q++) {
+    const int e = tid + q * NT, r = e / COLS, c = e % COLS;
+    const int64_t o = (int64_t)(row0 + r) * TB + col0 + c;
+    if constexpr (C) {
+      typedef R r2 __attribute__((ext_vector_type(2)));
+      const r2 v = ((gptr<const r2>)g)[o];
+      s[0][r][c] = v.x;
+      s[1][r][c] = v.y;
+    } else {
+      s[0][r][c] = g[o];
+    }
+  }
+}
+
 template <class R, bool C, int LD, class P>
 __device__ __forceinline__ void stage_tile(P g, R (*s)[TB][LD]) {
   const int tid = threadIdx.x;
@@ -164,13 +185,14 @@ __device__ __forceinline__ void stage_tile(P g, R (*s)[TB][LD]) {
 //             products of every block K, per item 2 NB workgroups:
 //     MU(K) = Uinv(K) * U(K,K+1), ML(K) = Linv(K) * L(K,K-1) (where the tiles exist)
 // Four waves; wave w owns the 32x32 quadrant (w/2, w%2), 2x2 MFMA blocks of
-// 16x16; a complex product is four real MFMAs (no 3M recombination).
+// 16x16.  The k dimension is staged in two halves (half the LDS).
 template <class R, bool C>
 __global__ __launch_bounds__(NT) void k_tile_gemm(const SysDev<R> *__restrict__ sys, Items it, int mode) {
   constexpr int NC = Elem<R, C>::NC;
-  constexpr int LD = TB + 1;
-  __shared__ R As[NC][TB][LD];
-  __shared__ R Bs[NC][TB][LD];
+  constexpr int KH = TB / 2;  // k staged in two halves: A columns / B rows kh .. kh+31
+  constexpr int LDA = KH + 1, LDB = TB + 1;
+  __shared__ R As[NC][TB][LDA];
+  __shared__ R Bs[NC][KH][LDB];
   const int total = it.off[it.count];
   for (int wb = blockIdx.x; wb < total; wb += gridDim.x) {
   if (wb != (int)blockIdx.x) __syncthreads();  // LDS reuse across work items
@@ -213,26 +235,30 @@ __global__ __launch_bounds__(NT) void k_tile_gemm(const SysDev<R> *__restrict__ 
       Cg = dinv_tile<R, C>(S, Kn, D_MU);
     }
   }
-  stage_tile<R, C, LD>(A, As);
-  stage_tile<R, C, LD>(B, Bs);
-  __syncthreads();
-
   typedef typename Mfma<R>::v4 v4;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int r0 = (wave >> 1) * 32, c0 = (wave & 1) * 32;
   const int li = lane & 15, lk = lane >> 4;
-  v4 accr[2][2], acci[2][2];
-  // acc = As * Bs
-  auto product = [&]() {
+  // FP32 complex products use three real MFMAs (P1 = ar br, P2 = ai bi,
+  // P3 = (ar + ai)(br + bi); re = P1 - P2, im = P3 - P1 - P2); FP64 keeps four
+  constexpr bool M3 = C && sizeof(R) == 4;
+  v4 accr[2][2], acci[2][2], accx[M3 ? 2 : 1][M3 ? 2 : 1];
 #pragma unroll
-    for (int a = 0; a < 2; a++)
+  for (int a = 0; a < 2; a++)
 #pragma unroll
-      for (int bb = 0; bb < 2; bb++) {
-        accr[a][bb] = v4{0, 0, 0, 0};
-        acci[a][bb] = v4{0, 0, 0, 0};
-      }
+    for (int bb = 0; bb < 2; bb++) {
+      accr[a][bb] = v4{0, 0, 0, 0};
+      acci[a][bb] = v4{0, 0, 0, 0};
+      if constexpr (M3) accx[a][bb] = v4{0, 0, 0, 0};
+    }
+#pragma unroll 1
+  for (int kh = 0; kh < TB; kh += KH) {
+    if (kh) __syncthreads();  // every read of the first halves done
+    stage_block<R, C, LDA, TB, KH>(A, 0, kh, As);
+    stage_block<R, C, LDB, KH, TB>(B, kh, 0, Bs);
+    __syncthreads();
 #pragma unroll 4
-    for (int k0 = 0; k0 < TB; k0 += 4) {
+    for (int k0 = 0; k0 < KH; k0 += 4) {
       R ar[2], ai[2], br[2], bi[2];
 #pragma unroll
       for (int a = 0; a < 2; a++) {
@@ -248,16 +274,30 @@ __global__ __launch_bounds__(NT) void k_tile_gemm(const SysDev<R> *__restrict__ 
       for (int a = 0; a < 2; a++)
 #pragma unroll
         for (int bb = 0; bb < 2; bb++) {
-          accr[a][bb] = Mfma<R>::run(ar[a], br[bb], accr[a][bb]);
-          if constexpr (C) {
-            accr[a][bb] = Mfma<R>::run(-ai[a], bi[bb], accr[a][bb]);
-            acci[a][bb] = Mfma<R>::run(ar[a], bi[bb], acci[a][bb]);
-            acci[a][bb] = Mfma<R>::run(ai[a], br[bb], acci[a][bb]);
+          if constexpr (M3) {
+            accr[a][bb] = Mfma<R>::run(ar[a], br[bb], accr[a][bb]);
+            accx[a][bb] = Mfma<R>::run(ai[a], bi[bb], accx[a][bb]);
+            acci[a][bb] = Mfma<R>::run(ar[a] + ai[a], br[bb] + bi[bb], acci[a][bb]);
+          } else {
+            accr[a][bb] = Mfma<R>::run(ar[a], br[bb], accr[a][bb]);
+            if constexpr (C) {
+              accr[a][bb] = Mfma<R>::run(-ai[a], bi[bb], accr[a][bb]);
+              acci[a][bb] = Mfma<R>::run(ar[a], bi[bb], acci[a][bb]);
+              acci[a][bb] = Mfma<R>::run(ai[a], br[bb], acci[a][bb]);
+            }
           }
         }
     }
-  };
-  product();
+  }
+  if constexpr (M3) {
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int bb = 0; bb < 2; bb++) {
+        acci[a][bb] = acci[a][bb] - accr[a][bb] - accx[a][bb];
+        accr[a][bb] = accr[a][bb] - accx[a][bb];
+      }
+  }
 #pragma unroll
   for (int a = 0; a < 2; a++)
 #pragma unroll
