@@ -1364,8 +1364,11 @@ __global__ __launch_bounds__(NT) void k_solve_persist(const SysDev<R> *__restric
       }
       __syncthreads();
     }
-    // publish x_K to every XCD, then the flag
-    __threadfence();
+    // publish x_K to every XCD, then the flag: each wave waits for its own
+    // stores to complete (vmcnt(0); expcnt and lgkmcnt left at their
+    // maxima), and the flag's agent-scope release after the barrier writes
+    // the L2 back once for the whole workgroup
+    __builtin_amdgcn_s_waitcnt(0x0F70);
     __syncthreads();
     if (tid == 0) __hip_atomic_store(&S.done[K], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
   }
