@@ -346,8 +346,11 @@ __device__ __forceinline__ int find_trail(const TrailItems &it, int b) {
 
 constexpr int KC = 16;  // k-chunk of the LDS pipeline
 
-// tile(I,J) -= sum_K tile(I,K) tile(K,J).  Persistent grid over the work
-// items of all systems.  A workgroup updates TJ consecutive tiles of a block
+// tile(I,J) -= sum_K tile(I,K) tile(K,J) over the work items of all
+// systems, either by a persistent grid striding over the output blocks or
+// (grid at least the block count) by one workgroup per output block; the
+// blocks a workgroup takes in turn get the next block's first chunks loaded
+// during the current one's last.  A workgroup updates TJ consecutive tiles of a block
 // row (a 64 x 64*TJ output: TJ = 1 or 2); the K dimension (m steps of 64)
 // streams through a two-buffer LDS pipeline in chunks of KC, the next chunks'
 // global loads in flight while the current chunk feeds the MFMAs.  Four waves,
