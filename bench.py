@@ -145,6 +145,37 @@ def assembly_rates(cl, V, w, G):
     return res
 
 
+def tau_rates(xs, cl, G):
+    """Post-solve reduction of the last group's solutions to tau
+    (fdmgpu_tau_reduce, HIP events): the workload's complex basis (the
+    contraction only), and the real-basis path (invcomp of the outer-sphere
+    block, then the contraction) on the same data.  One atom block of nlmso
+    rows; only tau (nlmso^2 complex entries per point) would leave the GPU
+    instead of the solution."""
+    import torch
+    from fdmnes_amd.tau import tau_reduce
+    atoms = [(0, cl.nlmso)]
+    nso = cl.nligne - cl.nlmso
+    stream = torch.cuda.current_stream()
+    reps = 20
+    res = {"points_per_call": G, "atom_rows": cl.nlmso}
+    for tag, basereel in (("complex_basis", False), ("real_basis", True)):
+        tau_reduce(xs, nso, basereel, False, atoms, check=False)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            tau_reduce(xs, nso, basereel, False, atoms, check=False)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        res[f"{tag}_us_per_point"] = e0.elapsed_time(e1) * 1e3 / (reps * G)
+    res["bytes_out_per_point"] = cl.nlmso * cl.nlmso * 16
+    res["solution_bytes_per_point"] = cl.nligne * cl.nlmso * 16
+    log(f"tau: {res['complex_basis_us_per_point']:.1f} us/point (complex basis), "
+        f"{res['real_basis_us_per_point']:.1f} us/point (real basis with invcomp)")
+    return res
+
+
 def traffic_from_profile():
     """HBM bytes per k_trail launch measured by rocprofv3 PMC runs (FETCH_SIZE,
     WRITE_SIZE) of this benchmark, scaled by the same counters on a plain
@@ -196,6 +227,7 @@ def main():
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-roofline", action="store_true", help="skip the per-launch timed pass (profiler runs)")
     ap.add_argument("--no-assembly", action="store_true", help="skip the per-energy assembly timing")
+    ap.add_argument("--no-tau", action="store_true", help="skip the post-solve tau reduction timing")
     ap.add_argument("--dropin", action="store_true",
                     help="also time the host-buffer drop-in entry points (PCIe and set-up included), one point each")
     args = ap.parse_args()
@@ -290,6 +322,8 @@ def main():
         out.update({"fp64_points_per_s": rate64, "mixed_over_fp64": rate32 / rate64,
                     "max_rel_sigma_err": float(err.max()), "adjoint_over_forward": t_adj / t_fwd,
                     "adjoint_s_per_group": t_adj, "fp64_ms_per_step": 1e3 * t_fwd})
+        if not args.no_tau:
+            out["tau"] = tau_rates(xs64, cl, G)
         del sv64, xs64, grads
     if not args.no_assembly:
         out["assembly"] = assembly_rates(cl, V, w, G)

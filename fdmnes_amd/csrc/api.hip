@@ -9,6 +9,7 @@
 #include "../../include/fdmgpu.h"
 #include "assemble.h"
 #include "batch.h"
+#include "tau.h"
 
 using namespace fdmgpu;
 
@@ -473,6 +474,70 @@ int fdmgpu_assembler_run(fdmgpu_assembler *a, int count, const double *Enervide,
   }
   FDM_HIP(hipSetDevice(g_lib.device));
   return a->a.run(count, P, E_comp != 0, basereel != 0, update_only != 0);
+}
+
+int fdmgpu_tau_reduce(const fdmgpu_tau_desc *d, int count, const double *const *d_x, double *const *d_tau,
+                      double *const *d_norm, int accumulate, int *d_info, void *hip_stream) {
+  if (!d || count < 0 || (count > 0 && !d_x) || d->natome < 0 || (d->natome > 0 && (!d->ianew || !d->nlmsa)) ||
+      (d->natome > 0 && count > 0 && !d_tau)) {
+    set_error("fdmgpu_tau_reduce: invalid argument");
+    return FDMGPU_EINVAL;
+  }
+  const int n = d->nlmso;
+  if (n < 1 || n > TAU_NT || d->nligne < n || (d->basereel && (d->nligneso < 0 || d->nligneso + n > d->nligne))) {
+    set_error("fdmgpu_tau_reduce: need 1 <= nlmso <= 256 and the outer-sphere rows inside the system");
+    return FDMGPU_EINVAL;
+  }
+  int na_max = 0;
+  for (int a = 0; a < d->natome; a++) {
+    if (d->nlmsa[a] < 0 || d->ianew[a] < 0 || d->ianew[a] + d->nlmsa[a] > d->nligne) {
+      set_error("fdmgpu_tau_reduce: rows of atom " + std::to_string(a) + " outside the system");
+      return FDMGPU_EINVAL;
+    }
+    na_max = std::max(na_max, d->nlmsa[a]);
+  }
+  if (tau_lds_bytes(n, na_max) > TAU_MAX_LDS) {
+    set_error("fdmgpu_tau_reduce: nlmso^2 + nlmso*(max nlmsa + 1) complex entries exceed the 160 KiB LDS");
+    return FDMGPU_EINVAL;
+  }
+  for (int k = 0; k < count; k++)
+    if (!d_x[k] || (d->natome > 0 && !d_tau[k])) {
+      set_error("fdmgpu_tau_reduce: null device buffer for point " + std::to_string(k));
+      return FDMGPU_EINVAL;
+    }
+  if (count == 0) return 0;
+  int rc = ensure_init();
+  if (rc) return rc;
+  hipStream_t s = hip_stream ? (hipStream_t)hip_stream : g_lib.stream;
+  // every point's norm and info are written by its first atom chunk; with no
+  // atoms one launch of one (empty) atom still inverts
+  const int natome = d->natome;
+  for (int p0 = 0; p0 < count; p0 += TAU_MAXP) {
+    const int np = std::min(TAU_MAXP, count - p0);
+    int64_t off = 0;
+    for (int a0 = 0; a0 < std::max(natome, 1); a0 += TAU_MAXA) {
+      const int na = natome ? std::min(TAU_MAXA, natome - a0) : 1;
+      TauJob J{};
+      for (int k = 0; k < np; k++) {
+        J.x[k] = d_x[p0 + k];
+        J.tau[k] = natome ? d_tau[p0 + k] : nullptr;
+        J.norm[k] = (a0 == 0 && d_norm) ? d_norm[p0 + k] : nullptr;
+      }
+      J.info = (a0 == 0 && d_info) ? d_info + p0 : nullptr;
+      int chunk_max = 0;
+      for (int a = 0; a < na; a++) {
+        J.row[a] = natome ? d->ianew[a0 + a] : 0;
+        J.na[a] = natome ? d->nlmsa[a0 + a] : 0;
+        J.off[a] = off;
+        off += (int64_t)J.na[a] * J.na[a];
+        chunk_max = std::max(chunk_max, J.na[a]);
+      }
+      rc = tau_launch(J, np, na, chunk_max, n, d->nligneso, d->cal_comp != 0, d->basereel != 0, d->fdm_comp != 0,
+                      accumulate != 0, s);
+      if (rc) return rc;
+    }
+  }
+  return 0;
 }
 
 }  // extern "C"

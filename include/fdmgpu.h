@@ -204,6 +204,43 @@ int fdmgpu_assembler_run(fdmgpu_assembler *a, int count, const double *Enervide,
                          const double *Ecinetic_out, int E_comp, int basereel, int update_only,
                          const double *const *d_V, double *const *d_vals, double *const *d_rhs);
 
+/*
+ * Post-solve reduction to the atom amplitude products (mat.f90:297-405,
+ * regular solution, nspino = 1, one representation), on solutions already
+ * on the device.  Replaces the CPU loop after mat_solve in mat():
+ *   - real basis (basereel): norm(Lm2, Lm) = delta + i x(nligneso + Lm, Lm2)
+ *     (mat.f90:314-328), inverted by invcomp's ZGETRF/ZGETRI sequence
+ *     (mat.f90:1927-1987, is = 0; sub_util.f:11149, 11382);
+ *   - amplitudes a(Lmf, ii) = sum_k norm^-1(Lmf, k) x(ii, k) (norm_t,
+ *     mat.f90:340, :368-376), or x(ii, Lmf) in the complex basis;
+ *   - tau_ia(Lm1, Lm2) (+)= -i sum_Lmf a(Lmf, ianew+Lm1) conj(a(Lmf,
+ *     ianew+Lm2)), or -i a a without the conjugate when fdm_comp
+ *     (mat.f90:392-399; Taull(Lm01, is1, Lm02, is2, ia) with Lm01/Lm02 the
+ *     caller's map of the atom row order).
+ */
+typedef struct {
+  int nligne, nlmso;
+  int nligneso;      /* 0-based first outer-sphere row (nligne - nlmso for the regular solution) */
+  int cal_comp;      /* solutions complex (interleaved) or real */
+  int basereel;      /* renormalise by the inverted outer-sphere block */
+  int fdm_comp;      /* no conjugate in the contraction */
+  int natome;
+  const int *ianew;  /* host [natome]: 0-based first row of each atom's block */
+  const int *nlmsa;  /* host [natome]: its row count */
+} fdmgpu_tau_desc;
+/* count energy points (any number; launched 16 at a time): d_x[k] device
+ * solution (nligne, nlmso) row-major; d_tau[k] device output, per atom
+ * nlmsa^2 complex entries [Lm1][Lm2] (interleaved), atoms concatenated,
+ * added to when accumulate != 0 (Taull accumulates over representations and
+ * spins); d_norm NULL or d_norm[k] NULL, else the inverted norm (nlmso^2
+ * complex, column-major, as invcomp leaves it); d_info NULL or a device
+ * array of count ints: 0, or the 1-based zero pivot of ZGETF2 at which the
+ * reference stops (that point's tau and norm are then not written).
+ * Asynchronous on hip_stream (NULL = library stream).  Limits: nlmso <= 256
+ * and (nlmso^2 + nlmso*(max nlmsa + 1)) * 16 + 4*nlmso bytes <= 160 KiB. */
+int fdmgpu_tau_reduce(const fdmgpu_tau_desc *d, int count, const double *const *d_x, double *const *d_tau,
+                      double *const *d_norm, int accumulate, int *d_info, void *hip_stream);
+
 #ifdef __cplusplus
 }
 #endif

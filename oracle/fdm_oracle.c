@@ -1037,3 +1037,203 @@ int orc_dense_solve(int n, int nrhs, double *a_re, double *a_im, double *b_re, d
   free(a); free(b);
   return st;
 }
+
+/* ------------------------------------------------------------- invcomp */
+/* Restated along the unblocked paths of the reference's vendored LAPACK
+ * (sub_util.f): ZGETRF -> ZGETF2, ZTRTRI -> ZTRTI2, and ZGETRI, which
+ * invcomp's lwork = n keeps unblocked.  Each helper follows the reference
+ * BLAS loop order, operation by operation. */
+#define M_(a, n, i, j) (a)[(size_t)((j)-1) * (n) + (i)-1]
+
+/* ONE / z as the reference build evaluates complex division (flang -O2:
+ * (a c + b d) / (c^2 + d^2), no scaling); with it the restatement reproduces
+ * the compiled reference invcomp bit for bit for n <= 64 */
+static double complex orc_crecip(double complex z) {
+  double c = creal(z), d = cimag(z), den = c * c + d * d;
+  return CMPLX((1.0 * c + 0.0 * d) / den, (0.0 * c - 1.0 * d) / den);
+}
+
+/* IZAMAX sub_util.f:23889: first index of the largest |re| + |im| */
+static int orc_izamax(int n, const double complex *x) {
+  int imax = 1;
+  double dmax = fabs(creal(x[0])) + fabs(cimag(x[0]));
+  for (int i = 2; i <= n; i++) {
+    double d = fabs(creal(x[i - 1])) + fabs(cimag(x[i - 1]));
+    if (d > dmax) { imax = i; dmax = d; }
+  }
+  return imax;
+}
+
+/* ZGETF2 sub_util.f:15877-15940 (with ZSWAP, ZSCAL, ZGERU) */
+static int orc_zgetf2(int n, double complex *a, int *ipiv) {
+  int info = 0;
+  const double sfmin = 2.2250738585072014e-308; /* DLAMCH('S') */
+  for (int j = 1; j <= n; j++) {
+    int jp = j - 1 + orc_izamax(n - j + 1, &M_(a, n, j, j));
+    ipiv[j - 1] = jp;
+    if (M_(a, n, jp, j) != 0) {
+      if (jp != j)
+        for (int k = 1; k <= n; k++) {
+          double complex t = M_(a, n, j, k);
+          M_(a, n, j, k) = M_(a, n, jp, k);
+          M_(a, n, jp, k) = t;
+        }
+      if (j < n) {
+        if (cabs(M_(a, n, j, j)) >= sfmin) {
+          double complex r = orc_crecip(M_(a, n, j, j));
+          for (int i = j + 1; i <= n; i++) M_(a, n, i, j) = r * M_(a, n, i, j);
+        } else {
+          for (int i = j + 1; i <= n; i++) M_(a, n, i, j) = M_(a, n, i, j) / M_(a, n, j, j);
+        }
+      }
+    } else if (info == 0) {
+      info = j;
+    }
+    if (j < n)
+      for (int k = j + 1; k <= n; k++) {
+        double complex y = M_(a, n, j, k);
+        if (y == 0) continue;
+        double complex temp = -1.0 * y;
+        for (int i = j + 1; i <= n; i++) M_(a, n, i, k) = M_(a, n, i, k) + M_(a, n, i, j) * temp;
+      }
+  }
+  return info;
+}
+
+/* ZTRTRI sub_util.f:15442 (singularity test) -> ZTRTI2 sub_util.f:18626
+ * (upper, non-unit) with ZTRMV's column loop and ZSCAL */
+static int orc_ztrtri_upper(int n, double complex *a) {
+  for (int j = 1; j <= n; j++)
+    if (M_(a, n, j, j) == 0) return j;
+  for (int j = 1; j <= n; j++) {
+    M_(a, n, j, j) = orc_crecip(M_(a, n, j, j));
+    double complex ajj = -M_(a, n, j, j);
+    double complex *x = &M_(a, n, 1, j);
+    for (int jj = 1; jj <= j - 1; jj++) {  /* x := T x, T = inv(U)(1:j-1, 1:j-1) */
+      if (x[jj - 1] == 0) continue;
+      double complex temp = x[jj - 1];
+      for (int i = 1; i <= jj - 1; i++) x[i - 1] = x[i - 1] + temp * M_(a, n, i, jj);
+      x[jj - 1] = x[jj - 1] * M_(a, n, jj, jj);
+    }
+    for (int i = 1; i <= j - 1; i++) x[i - 1] = ajj * x[i - 1];
+  }
+  return 0;
+}
+
+/* ZGETRI sub_util.f:11382-11470, unblocked branch (with ZGEMV and ZSWAP) */
+static int orc_zgetri(int n, double complex *a, const int *ipiv) {
+  int info = orc_ztrtri_upper(n, a);
+  if (info) return info;
+  double complex *work = malloc(sizeof(double complex) * (size_t)n);
+  for (int j = n; j >= 1; j--) {
+    for (int i = j + 1; i <= n; i++) {
+      work[i - 1] = M_(a, n, i, j);
+      M_(a, n, i, j) = 0;
+    }
+    if (j < n)  /* A(:, j) += -1 * A(:, j+1:n) work(j+1:n) */
+      for (int jj = j + 1; jj <= n; jj++) {
+        if (work[jj - 1] == 0) continue;
+        double complex temp = -1.0 * work[jj - 1];
+        for (int i = 1; i <= n; i++) M_(a, n, i, j) = M_(a, n, i, j) + temp * M_(a, n, i, jj);
+      }
+  }
+  for (int j = n - 1; j >= 1; j--) {
+    int jp = ipiv[j - 1];
+    if (jp != j)
+      for (int i = 1; i <= n; i++) {
+        double complex t = M_(a, n, i, j);
+        M_(a, n, i, j) = M_(a, n, i, jp);
+        M_(a, n, i, jp) = t;
+      }
+  }
+  free(work);
+  return 0;
+}
+
+/* invcomp mat.f90:1927-1987 with is = 0: ZGETRF, stop on INFO, ZGETRI */
+static int orc_invcomp_c(int n, double complex *a) {
+  int *ipiv = malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+  int info = orc_zgetf2(n, a, ipiv);
+  if (!info) info = orc_zgetri(n, a, ipiv);
+  free(ipiv);
+  return info;
+}
+
+int orc_invcomp(int n, double *a_re, double *a_im) {
+  double complex *a = malloc(sizeof(double complex) * (size_t)n * n + 1);
+  for (size_t k = 0; k < (size_t)n * n; k++) a[k] = CMPLX(a_re[k], a_im[k]);
+  int info = orc_invcomp_c(n, a);
+  for (size_t k = 0; k < (size_t)n * n; k++) { a_re[k] = creal(a[k]); a_im[k] = cimag(a[k]); }
+  free(a);
+  return info;
+}
+
+/* ----------------------------------------------------------------- tau */
+/* mat.f90:297-405, regular branch (n_RI = 1), nspino = 1 (is1 = is2 =
+ * ispinin), Lm01/Lm02 left as the atom row order Lm1/Lm2. */
+int orc_tau(int nligne, int nlmso, int nligneso, int cal_comp, int basereel, int fdm_comp,
+            const double *x_re, const double *x_im, int natome, const int *ianew, const int *nlmsa,
+            double *tau_re, double *tau_im, double *norm_re, double *norm_im) {
+  (void)nligne;
+  const int n = nlmso;
+  const double complex img = CMPLX(0.0, 1.0);
+#define SMR(lm, ii) x_re[(size_t)((ii)-1) * n + (lm)-1]
+#define SMI(lm, ii) x_im[(size_t)((ii)-1) * n + (lm)-1]
+  double complex *norm = malloc(sizeof(double complex) * (size_t)n * n + 1);
+  int info = 0;
+  if (basereel && n > 0) {
+    /* :314-328 (nligneso 1-based = the 0-based first outer row) */
+    for (int lm = 1; lm <= n; lm++) {
+      int ii = nligneso + lm;
+      for (int lm2 = 1; lm2 <= n; lm2++)
+        M_(norm, n, lm2, lm) = cal_comp ? CMPLX(-SMI(lm2, ii), SMR(lm2, ii)) : CMPLX(0.0, SMR(lm2, ii));
+    }
+    for (int lm = 1; lm <= n; lm++) M_(norm, n, lm, lm) = 1.0 + M_(norm, n, lm, lm);
+    info = orc_invcomp_c(n, norm);  /* :338 */
+    if (norm_re)
+      for (size_t k = 0; k < (size_t)n * n; k++) { norm_re[k] = creal(norm[k]); norm_im[k] = cimag(norm[k]); }
+    if (info) { free(norm); return info; }
+  }
+  /* norm_t(k, Lmf) = norm(Lmf, k) (:340) */
+  size_t off = 0;
+  for (int ia = 0; ia < natome; ia++) {
+    int na = nlmsa[ia];
+    for (int lm1 = 1; lm1 <= na; lm1++) {
+      int ii = ianew[ia] + lm1;
+      for (int lm2 = 1; lm2 <= na; lm2++) {
+        int jj = ianew[ia] + lm2;
+        double complex cfac = 0;
+        for (int lmf = 1; lmf <= n; lmf++) {
+          double complex a1, a2;
+          if (basereel) {
+            a1 = 0; a2 = 0;
+            for (int k = 1; k <= n; k++) {
+              double complex nt = M_(norm, n, lmf, k);
+              if (cal_comp) {
+                a1 = a1 + nt * CMPLX(SMR(k, ii), SMI(k, ii));
+                a2 = a2 + nt * CMPLX(SMR(k, jj), SMI(k, jj));
+              } else {
+                a1 = a1 + nt * SMR(k, ii);
+                a2 = a2 + nt * SMR(k, jj);
+              }
+            }
+          } else {
+            a1 = CMPLX(SMR(lmf, ii), cal_comp ? SMI(lmf, ii) : 0.0);
+            a2 = CMPLX(SMR(lmf, jj), cal_comp ? SMI(lmf, jj) : 0.0);
+          }
+          if (fdm_comp) cfac = cfac - img * a1 * a2;
+          else cfac = cfac - img * a1 * conj(a2);
+        }
+        size_t q = off + (size_t)(lm1 - 1) * na + (lm2 - 1);
+        tau_re[q] += creal(cfac);
+        tau_im[q] += cimag(cfac);
+      }
+    }
+    off += (size_t)na * na;
+  }
+#undef SMR
+#undef SMI
+  free(norm);
+  return info;
+}
+#undef M_

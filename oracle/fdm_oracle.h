@@ -98,6 +98,28 @@ int orc_mat_solve_csr(int nligne, int nlmso, int cal_comp, const int *lb1, const
  * Returns 0 or 1 if singular. */
 int orc_dense_solve(int n, int nrhs, double *a_re, double *a_im, double *b_re, double *b_im);
 
+/* invcomp(n, mat, n, n, 0, Stop_job) (mat.f90:1927-1987): in-place complex
+ * inverse by the reference LAPACK's ZGETRF then ZGETRI.  a_re/a_im n*n
+ * column-major.  Returns 0, or the 1-based index of the first exactly zero
+ * pivot (ZGETF2's INFO, which makes the reference stop). */
+int orc_invcomp(int n, double *a_re, double *a_im);
+
+/* Post-solve reduction of one energy point to the atom amplitude products
+ * (mat.f90:297-405: regular solution, nspino = 1, one representation).
+ *   x_re/x_im  solution, row-major nligne x nlmso: x[ii][Lm] = smr(Lm, ii)
+ *              (x_im NULL when cal_comp == 0)
+ *   nligneso   0-based first outer-sphere row (the outer unknowns follow)
+ *   basereel   renormalise by the inverted outer-sphere block (:312-343)
+ *   fdm_comp   cfac = -i a1 a2 instead of -i a1 conj(a2) (:392-397)
+ *   ianew[ia]  0-based first row of atom ia, nlmsa[ia] its row count
+ *   tau        per atom nlmsa^2 entries [Lm1][Lm2], atoms concatenated;
+ *              accumulated (+=) like Taull
+ *   norm       NULL or nlmso^2 column-major: norm after invcomp
+ * Returns orc_invcomp's status (0 when basereel == 0). */
+int orc_tau(int nligne, int nlmso, int nligneso, int cal_comp, int basereel, int fdm_comp,
+            const double *x_re, const double *x_im, int natome, const int *ianew, const int *nlmsa,
+            double *tau_re, double *tau_im, double *norm_re, double *norm_im);
+
 /* Spherical Bessel/Neumann recurrences (sphere.f90:7226-7248, 7356-7378). */
 void orc_cbessneu(double fn_re, double fn_im, double z_re, double z_im, int lmax,
                   double *bes, double *neu);

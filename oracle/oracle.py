@@ -64,6 +64,11 @@ def lib():
         L.orc_cbessneu.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, _dp, _dp]
         L.orc_cbessneur.argtypes = [C.c_double, C.c_double, C.c_int, _dp, _dp]
         L.orc_ylm_real.argtypes = [C.c_int, _dp, C.c_double, _dp]
+        L.orc_invcomp.restype = C.c_int
+        L.orc_invcomp.argtypes = [C.c_int, _dp, _dp]
+        L.orc_tau.restype = C.c_int
+        L.orc_tau.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _dp, C.c_void_p, C.c_int,
+                              _ip, _ip, _dp, _dp, C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
 
@@ -248,3 +253,44 @@ def ylm_real(lmax: int, v, r: float):
     y = np.zeros((lmax + 1) ** 2)
     lib().orc_ylm_real(lmax, np.ascontiguousarray(v, np.float64), r, y)
     return y
+
+
+def invcomp(A):
+    """invcomp restatement (mat.f90:1927, ZGETRF + ZGETRI).  Returns
+    (inverse, info); info = 1-based first zero pivot, 0 on success."""
+    a = np.asfortranarray(np.asarray(A, np.complex128))
+    n = a.shape[0]
+    re = np.ascontiguousarray(a.real.T).reshape(-1).copy()  # column-major
+    im = np.ascontiguousarray(a.imag.T).reshape(-1).copy()
+    info = lib().orc_invcomp(n, re, im)
+    return (re + 1j * im).reshape(n, n).T.copy(), info
+
+
+def tau(x, nligneso: int, basereel: bool, fdm_comp: bool, atoms, tau0=None, want_norm=False):
+    """Post-solve reduction of one point (mat.f90:297-405).  x (nligne,
+    nlmso) real or complex solution; atoms = [(first_row, nrows), ...]
+    (0-based).  Returns (list of per-atom (na, na) complex tau, norm or None,
+    info); tau0 (same structure) is accumulated into like Taull."""
+    x = np.asarray(x)
+    cal = np.iscomplexobj(x)
+    n, nl = x.shape
+    xr = np.ascontiguousarray(x.real, np.float64).reshape(-1)
+    xi = np.ascontiguousarray(x.imag, np.float64).reshape(-1) if cal else None
+    ianew = np.array([a[0] for a in atoms], np.int32)
+    nlmsa = np.array([a[1] for a in atoms], np.int32)
+    tot = int(sum(int(k) * int(k) for k in nlmsa))
+    tr, ti = np.zeros(max(tot, 1)), np.zeros(max(tot, 1))
+    if tau0 is not None:
+        flat = np.concatenate([np.asarray(t, np.complex128).reshape(-1) for t in tau0]) if tot else np.zeros(0)
+        tr[:tot], ti[:tot] = flat.real, flat.imag
+    nr = np.zeros(nl * nl) if want_norm else None
+    ni = np.zeros(nl * nl) if want_norm else None
+    info = lib().orc_tau(n, nl, int(nligneso), int(cal), int(bool(basereel)), int(bool(fdm_comp)), xr, _ptr(xi),
+                         len(atoms), ianew, nlmsa, tr, ti, _ptr(nr), _ptr(ni))
+    out, off = [], 0
+    for k in nlmsa:
+        k = int(k)
+        out.append((tr[off:off + k * k] + 1j * ti[off:off + k * k]).reshape(k, k))
+        off += k * k
+    norm = (nr + 1j * ni).reshape(nl, nl).T.copy() if want_norm else None
+    return out, norm, info

@@ -204,3 +204,14 @@ def mat_solve(cl, V, Enervide, Eimag, bes, neu, E_comp, cal_comp, basereel, rhs=
     if cal_comp:
         x = x + 1j * smi.T
     return x
+
+
+def invcomp(A):
+    """Reference ``invcomp`` (mat.f90:1927) with is = 0: the vendored
+    ZGETRF/ZGETRI.  A vanishing pivot makes it STOP, so callers pass only
+    non-singular matrices here."""
+    A_ = _Args()
+    a = np.asfortranarray(np.asarray(A, np.complex128)).copy(order="F")
+    n = a.shape[0]
+    lib().invcomp_(A_.i(n), a.ctypes.data_as(C.c_void_p), A_.i(n), A_.i(n), A_.i(0), A_.b(True))
+    return a
