@@ -177,3 +177,47 @@ def test_full_size_copper_basis_identity():
     assert _rel(a, b) <= 2e-10
     ot, _ = _oracle_tau(xr.cpu().numpy(), nso, True, False, atoms)
     assert _rel(a, ot) <= 1e-13
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("E_comp,basereel", [(True, False), (False, True)])
+def test_device_pipeline_matches_oracle_chain(prec, E_comp, basereel):
+    """The whole per-energy path on the GPU (device assembly, grouped solve,
+    tau reduction) against the oracle's chain (calcMatRow and mat_solve
+    restatements, then the reduction) for three energies of a small
+    cluster.  Bounds: 1e-10 in the complex basis (the solver's parity bar);
+    1e-9 in the real basis, where the reduction inverts I + i X (its
+    unpivoted solve leaves 2e-10 against a pivoted one on such clusters)."""
+    import torch
+    from fdmnes_amd import cluster as CL
+    from fdmnes_amd.assemble import DeviceAssembler
+    from fdmnes_amd.batch import DeviceSolver
+    from fdmnes_amd.tau import tau_reduce
+    print(f"pipeline precision {prec} E_comp {E_comp} basereel {basereel}", flush=True)
+    R, h, lm = 1.6 / BOHR, 0.25 / BOHR, 2
+    c, o = CL.Cluster(R, h, lm), O.Cluster(R, h, lm)
+    cal = E_comp or not basereel
+    rng = np.random.default_rng(21)
+    ens = [(-0.2 + 0.35 * k, 0.0367 if E_comp else 0.0, 0.3 + 0.4 * k) for k in range(3)]
+    V = [rng.uniform(-1.5, 0.0, c.npoint) for _ in ens]
+    dt = torch.complex128 if cal else torch.float64
+    vals = [torch.zeros(c.nnz, dtype=dt, device="cuda") for _ in ens]
+    rhs = [torch.zeros((c.nligne, c.nlmso), dtype=dt, device="cuda") for _ in ens]
+    DeviceAssembler(c).run([_dev(v) for v in V], ens, E_comp, basereel, vals, rhs)
+    lb1, lb2 = c.profile()
+    rp, col, _, _ = c.assemble(V[0], *ens[0], E_comp, basereel)
+    sv = DeviceSolver(prec, cal, c.nlmso)
+    slots = [sv.add_system(lb1, lb2, rp, col) for _ in ens]
+    xs = [torch.zeros_like(b) for b in rhs]
+    sv.solve(slots, vals, rhs, xs)
+    nso = c.nligne - c.nlmso
+    atoms = [(0, 9), (200, 4), (nso - 3, 3)]
+    tau, _, info = tau_reduce(xs, nso, basereel, False, atoms)
+    bound = 1e-9 if basereel else 1e-10
+    for k, (E, Ei, Ec) in enumerate(ens):
+        st, xo, *_ = o.solve(V[k], E, Ei, Ec, E_comp, cal, basereel)
+        assert st == 0
+        ot, _ = _oracle_tau(xo, nso, basereel, False, atoms)
+        d = _rel(tau[k].cpu().numpy(), ot)
+        print("point", k, "tau vs oracle chain", d, flush=True)
+        assert d <= bound
