@@ -18,6 +18,10 @@ def _L():
     if not hasattr(L, "_cluster_sigs"):
         L.fdmgpu_cluster_create.restype = C.c_int
         L.fdmgpu_cluster_create.argtypes = [C.c_double, C.c_double, C.c_int, C.POINTER(C.c_void_p)]
+        L.fdmgpu_cluster_create_mmm.restype = C.c_int
+        L.fdmgpu_cluster_create_mmm.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
+        L.fdmgpu_cluster_rvol.argtypes = [C.c_void_p, _lib.dp]
+        L.fdmgpu_cluster_harmonics.argtypes = [C.c_void_p, _lib.ip]
         L.fdmgpu_cluster_destroy.argtypes = [C.c_void_p]
         L.fdmgpu_cluster_sizes.argtypes = [C.c_void_p, _lib.ip]
         L.fdmgpu_cluster_points.argtypes = [C.c_void_p, _lib.dp]
@@ -39,12 +43,14 @@ def _L():
 
 class Cluster:
     """A spherical cluster of radius ``rsort`` and grid step ``adimp``
-    (bohr) with ``(lmaxso+1)**2`` outer-sphere unknowns."""
+    (bohr) with ``(lmaxso+1)**2`` outer-sphere unknowns; with ``irrep``
+    (0..7) the system of one representation of the mirror group mmm on the
+    wedge x, y, z >= 0 (see fdmgpu_cluster_create_mmm)."""
 
-    def __init__(self, rsort: float, adimp: float, lmaxso: int):
+    def __init__(self, rsort: float, adimp: float, lmaxso: int, irrep: int = -1):
         L = _L()
         h = C.c_void_p()
-        rc = L.fdmgpu_cluster_create(rsort, adimp, lmaxso, C.byref(h))
+        rc = L.fdmgpu_cluster_create_mmm(rsort, adimp, lmaxso, int(irrep), C.byref(h))
         if rc:
             raise _lib.FdmGpuError(f"fdmgpu_cluster_create failed ({rc})")
         self._h = h
@@ -52,6 +58,7 @@ class Cluster:
         L.fdmgpu_cluster_sizes(h, s)
         self.npoint, self.npso, self.nligne, self.nlmso, self.nsortf, self.nsort = map(int, s)
         self.lmaxso = lmaxso
+        self.irrep = int(irrep)
         self.rsort, self.adimp = rsort, adimp
         self.nnz = int(L.fdmgpu_cluster_nnz(h))
 
@@ -68,6 +75,18 @@ class Cluster:
         a = np.zeros(3 * self.npoint)
         _L().fdmgpu_cluster_points(self._h, a)
         return a.reshape(self.npoint, 3)
+
+    @property
+    def rvol(self) -> np.ndarray:
+        a = np.zeros(self.npoint)
+        _L().fdmgpu_cluster_rvol(self._h, a)
+        return a
+
+    @property
+    def harmonics(self) -> np.ndarray:
+        a = np.zeros(self.nlmso, np.int32)
+        _L().fdmgpu_cluster_harmonics(self._h, a)
+        return a
 
     def profile(self):
         lb1 = np.zeros(self.nligne, np.int32)
@@ -97,6 +116,8 @@ class Cluster:
     def assembly_tables(self) -> dict:
         """Energy-independent tables of the per-energy assembly
         (fdmgpu_cluster_assembly_tables): numpy arrays plus the sizes."""
+        if self.irrep >= 0:
+            raise _lib.FdmGpuError("assembly tables describe P1 clusters (no mirror representation)")
         L = _L()
         sz = np.zeros(4, np.int64)
         L.fdmgpu_cluster_assembly_sizes(self._h, sz)

@@ -87,10 +87,18 @@ void bessel_neumann(T fnorm, T z, int lmax, T *bes, T *neu) {
 struct fdmgpu_cluster {
   double rsort = 0, adimp = 0;
   int lmaxso = 0, nlmso = 0;
+  // -1: the whole cluster (point group P1).  0..7: one representation of the
+  // mirror group mmm (bit a set = odd under the mirror normal to axis a);
+  // the grid is then the wedge x, y, z >= 0
+  int irrep = -1;
   int npoint = 0, npso = 0, nligne = 0;
   std::vector<GridPoint> pt;   // npso, radial order
   std::vector<double> rad;     // npso, |index| * adimp
   std::vector<int> nbr;        // npso * 12, 0-based point or -1
+  std::vector<unsigned char> flip;  // npso * 12: mirrors taking the neighbour into the wedge (bit per axis)
+  std::vector<double> rvol;    // npso: points of the whole grid a wedge point stands for, / 8 (1 for P1)
+  std::vector<double> wso;     // nsortf: outer-sphere quadrature weights 4 pi rvol / sum(rvol)
+  std::vector<int> lmidx;      // nlmso: harmonic index l*l + l + m of each outer unknown
   std::vector<int> bnd;        // boundary points: inner (nsortf) then outer
   int nsortf = 0;
   std::vector<int> slot;       // per point: index in bnd of an outer point, -1 otherwise
@@ -103,7 +111,23 @@ struct fdmgpu_cluster {
 
 namespace {
 
+// Parities of the real harmonic l*l + l + m under the three mirrors, as the
+// representation bits: cos(m phi) P_l^m (m >= 0) is (-1)^m in x, even in y,
+// (-1)^(l+m) in z; sin(|m| phi) P_l^|m| is (-1)^(|m|+1) in x, odd in y,
+// (-1)^(l+|m|) in z.
+int harmonic_irrep(int l, int m) {
+  const int am = m < 0 ? -m : m;
+  const int ox = (m >= 0) ? (am & 1) : ((am + 1) & 1);
+  const int oy = (m < 0) ? 1 : 0;
+  const int oz = (l + am) & 1;
+  return ox | (oy << 1) | (oz << 2);
+}
+
+// character of the mirrors in mask for representation g: -1 per odd axis flipped
+inline double character(int g, int mask) { return (__builtin_popcount(g & mask) & 1) ? -1.0 : 1.0; }
+
 int build(fdmgpu_cluster *c) {
+  const bool sym = c->irrep >= 0;
   // Grid points inside rsort/adimp + sqrt(2)*iord/2 (reseau, general.f90:9476)
   // visited in (ix, iy, iz) lexicographic order, then ordered by radius with
   // the reference's exchange sort (:9528-9543): a later point is swapped in
@@ -116,6 +140,9 @@ int build(fdmgpu_cluster *c) {
       for (int iz = -nx; iz <= nx; iz++) {
         const double d = std::sqrt((double)ix * ix + (double)iy * iy + (double)iz * iz);
         if (d > rmax) continue;
+        // with mmm symmetry only the wedge (posequiv's region of mmm,
+        // general.f90:8892) is kept (reseau, general.f90:9512)
+        if (sym && (ix < 0 || iy < 0 || iz < 0)) continue;
         c->pt.push_back({ix, iy, iz});
         c->rad.push_back(c->adimp * d);
       }
@@ -135,16 +162,37 @@ int build(fdmgpu_cluster *c) {
   int np = 0;
   while (np < n && !(r[np] > c->rsort + kEpsPos)) np++;
   c->npoint = np;
-  // Neighbour table (laplac, general.f90:9839-9871)
+  // Point weights (reseau, general.f90:9582-9651): the number of grid points
+  // a wedge point stands for, over the largest such number
+  c->rvol.assign(n, 1.0);
+  if (sym) {
+    double rvmax = 1.0;
+    for (int i = 0; i < n; i++) {
+      c->rvol[i] = (double)(1 << ((p[i].ix != 0) + (p[i].iy != 0) + (p[i].iz != 0)));
+      rvmax = std::max(rvmax, c->rvol[i]);
+    }
+    for (int i = 0; i < n; i++) c->rvol[i] /= rvmax;
+  }
+  // Neighbour table (laplac, general.f90:9839-9871); with symmetry a
+  // neighbour outside the wedge is its mirror image, and flip records the
+  // mirrors (isvois)
   const int w = 2 * nx + 1;
   std::vector<int> where((size_t)w * w * w, -1);
   auto key = [&](int x, int y, int z) { return ((size_t)(x + nx) * w + (y + nx)) * w + (z + nx); };
   for (int i = 0; i < n; i++) where[key(p[i].ix, p[i].iy, p[i].iz)] = i;
   c->nbr.assign((size_t)n * 12, -1);
+  c->flip.assign((size_t)n * 12, 0);
   for (int i = 0; i < n; i++)
     for (int v = 0; v < 12; v++) {
-      const int x = p[i].ix + kDir[v][0], y = p[i].iy + kDir[v][1], z = p[i].iz + kDir[v][2];
-      if (std::abs(x) <= nx && std::abs(y) <= nx && std::abs(z) <= nx) c->nbr[(size_t)i * 12 + v] = where[key(x, y, z)];
+      int x = p[i].ix + kDir[v][0], y = p[i].iy + kDir[v][1], z = p[i].iz + kDir[v][2];
+      if (!(std::abs(x) <= nx && std::abs(y) <= nx && std::abs(z) <= nx)) continue;
+      if (sym) {
+        c->flip[(size_t)i * 12 + v] = (unsigned char)((x < 0) | ((y < 0) << 1) | ((z < 0) << 2));
+        x = std::abs(x);
+        y = std::abs(y);
+        z = std::abs(z);
+      }
+      c->nbr[(size_t)i * 12 + v] = where[key(x, y, z)];
     }
   const double c1 = (4.0 / 3.0) / (c->adimp * c->adimp), c2 = (-1.0 / 16) * c1;  // general.f90:9754-9776
   double s = 0;
@@ -165,6 +213,10 @@ int build(fdmgpu_cluster *c) {
       }
     }
   c->nsortf = (int)c->bnd.size();
+  double pdsort = 0;  // bordure, general.f90:10006, :10073-10075
+  for (int b = 0; b < c->nsortf; b++) pdsort += c->rvol[c->bnd[b]];
+  c->wso.resize(c->nsortf);
+  for (int b = 0; b < c->nsortf; b++) c->wso[b] = 4 * kPi * c->rvol[c->bnd[b]] / pdsort;
   c->slot.assign(n, -1);
   for (int i = np; i < n; i++)
     for (int v = 0; v < 12; v++) {
@@ -176,11 +228,20 @@ int build(fdmgpu_cluster *c) {
       }
     }
   // Real harmonics at the outer-sphere points (Ylmpt, general.f90:11545-11568)
-  c->nlmso = (c->lmaxso + 1) * (c->lmaxso + 1);
+  // (one representation: only its harmonics, in increasing l*l + l + m)
+  const int nall = (c->lmaxso + 1) * (c->lmaxso + 1);
+  c->lmidx.clear();
+  for (int l = 0; l <= c->lmaxso; l++)
+    for (int m = -l; m <= l; m++)
+      if (!sym || harmonic_irrep(l, m) == c->irrep) c->lmidx.push_back(l * l + l + m);
+  c->nlmso = (int)c->lmidx.size();
+  if (c->nlmso == 0) return -1;
   c->ylm.assign(c->bnd.size() * c->nlmso, 0.0);
+  std::vector<double> yall(nall);
   for (size_t b = 0; b < c->bnd.size(); b++) {
     const GridPoint &q = p[c->bnd[b]];
-    real_ylm(c->lmaxso, q.ix * c->adimp, q.iy * c->adimp, q.iz * c->adimp, &c->ylm[b * c->nlmso]);
+    real_ylm(c->lmaxso, q.ix * c->adimp, q.iy * c->adimp, q.iz * c->adimp, yall.data());
+    for (int k = 0; k < c->nlmso; k++) c->ylm[b * c->nlmso + k] = yall[c->lmidx[k]];
   }
   // Rows and profile (newind, mat.f90:1812-1884)
   const int nl = np + c->nlmso;
@@ -226,6 +287,7 @@ int build(fdmgpu_cluster *c) {
         outside = true;
     }
     std::sort(tmp.begin(), tmp.end());
+    tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());  // mirror images can coincide
     if (outside)
       for (int l = 0; l < c->nlmso; l++) tmp.push_back(np + l + 1);
     c->cols.insert(c->cols.end(), tmp.begin(), tmp.end());
@@ -249,18 +311,33 @@ inline int lof(int lm) {  // l of the harmonic index l*l + l + m
 
 extern "C" {
 
-int fdmgpu_cluster_create(double rsort, double adimp, int lmaxso, fdmgpu_cluster **out) {
-  if (!out || !(rsort > 0) || !(adimp > 0) || lmaxso < 0 || lmaxso > 40) return -1;
+int fdmgpu_cluster_create_mmm(double rsort, double adimp, int lmaxso, int irrep, fdmgpu_cluster **out) {
+  if (!out || !(rsort > 0) || !(adimp > 0) || lmaxso < 0 || lmaxso > 40 || irrep < -1 || irrep > 7) return -1;
   fdmgpu_cluster *c = new (std::nothrow) fdmgpu_cluster();
   if (!c) return -3;
   c->rsort = rsort;
   c->adimp = adimp;
   c->lmaxso = lmaxso;
+  c->irrep = irrep;
   if (build(c)) {
     delete c;
     return -1;
   }
   *out = c;
+  return 0;
+}
+
+int fdmgpu_cluster_create(double rsort, double adimp, int lmaxso, fdmgpu_cluster **out) {
+  return fdmgpu_cluster_create_mmm(rsort, adimp, lmaxso, -1, out);
+}
+
+int fdmgpu_cluster_rvol(const fdmgpu_cluster *c, double *rvol) {
+  std::memcpy(rvol, c->rvol.data(), sizeof(double) * c->npoint);
+  return 0;
+}
+
+int fdmgpu_cluster_harmonics(const fdmgpu_cluster *c, int *lm) {
+  std::memcpy(lm, c->lmidx.data(), sizeof(int) * c->nlmso);
   return 0;
 }
 
@@ -336,17 +413,21 @@ int fdmgpu_cluster_assemble(const fdmgpu_cluster *c, const double *V, double Ene
     std::fill(row.begin(), row.end(), cd(0));
     std::fill(rrow.begin(), rrow.end(), cd(0));
     const cd diag(V[i] - Enervide - c->clapl[0], E_comp ? -Eimag : 0.0);  // mat.f90:1147-1150
-    // interior entries in column order; the stencil weight of a neighbour is -clapl(v)
+    // stencil term of direction d: -clapl(v) times the character of the
+    // mirrors that bring the neighbour into the wedge, times rvol(i)
+    // (mat.f90:1164-1222; 1 and 1 without symmetry)
+    auto stencil = [&](int d) {
+      return (character(c->irrep, c->flip[(size_t)i * 12 + d]) * -c->clapl[d + 1]) * c->rvol[i];
+    };
+    // interior entries in column order, each neighbour's terms over rvol(j)
+    // added in direction order (:1225-1231); a neighbour that is the point's
+    // own mirror image adds to the diagonal
     int64_t q = q0;
     for (; q < q1 && col[q] <= np; q++) {
       const int j = col[q] - 1;
-      cd v = 0;
-      if (j == i) {
-        v = diag;
-      } else {
-        for (int d = 0; d < 12; d++)
-          if (c->nbr[(size_t)i * 12 + d] == j) v = -c->clapl[d + 1];
-      }
+      cd v = (j == i) ? diag : cd(0);
+      for (int d = 0; d < 12; d++)
+        if (c->nbr[(size_t)i * 12 + d] == j) v += stencil(d) / c->rvol[j];
       re[q] = v.real();
       if (cal) im[q] = v.imag();
     }
@@ -354,9 +435,9 @@ int fdmgpu_cluster_assemble(const fdmgpu_cluster *c, const double *V, double Ene
       const int j = c->nbr[(size_t)i * 12 + d];
       if (j < np) continue;
       const int b = c->slot[j];
-      const double w = -c->clapl[d + 1];
+      const double w = stencil(d);
       for (int lm = 0; lm < nl; lm++) {
-        const int l = lof(lm);
+        const int l = lof(c->lmidx[lm]);
         const double y = c->ylm[(size_t)b * nl + lm];
         row[lm] += cfac(b, l) * y * w;
         rrow[lm] -= y * B[(size_t)b * (L + 1) + l] * w;
@@ -372,28 +453,27 @@ int fdmgpu_cluster_assemble(const fdmgpu_cluster *c, const double *V, double Ene
         if (cal && rhs_im) rhs_im[(size_t)i * nl + lm] = rrow[lm].imag();
       }
   }
-  const double pw = 4 * kPi / c->nsortf;
-  for (int lm = 0; lm < nl; lm++) {  // outer rows, mat.f90:1401-1511
+  for (int lm = 0; lm < nl; lm++) {  // outer rows, mat.f90:1401-1511, weights poidso (bordure)
     const int ii = np + lm;
     int64_t q = c->rp[ii];
-    for (int b = 0; b < c->nsortf; b++, q++) {
-      re[q] = -pw * c->ylm[(size_t)b * nl + lm];
+    for (int b = 0; b < c->nsortf; b++, q++) {  // :1476-1483
+      re[q] = -(c->wso[b] * c->ylm[(size_t)b * nl + lm]) / c->rvol[c->bnd[b]];
       if (cal) im[q] = 0;
     }
     for (int lp = 0; lp < nl; lp++, q++) {
-      const int l = lof(lp);
+      const int l = lof(c->lmidx[lp]);
       cd s = 0;
       for (int b = 0; b < c->nsortf; b++)
-        s += (pw * c->ylm[(size_t)b * nl + lm] * c->ylm[(size_t)b * nl + lp]) * cfac(b, l);
+        s += (c->wso[b] * c->ylm[(size_t)b * nl + lm] * c->ylm[(size_t)b * nl + lp]) * cfac(b, l);
       re[q] = s.real();
       if (cal) im[q] = s.imag();
     }
     if (rhs_re)
       for (int lf = 0; lf < nl; lf++) {
-        const int l = lof(lf);
+        const int l = lof(c->lmidx[lf]);
         cd s = 0;
         for (int b = 0; b < c->nsortf; b++)
-          s += (pw * c->ylm[(size_t)b * nl + lm] * c->ylm[(size_t)b * nl + lf]) * B[(size_t)b * (L + 1) + l];
+          s += (c->wso[b] * c->ylm[(size_t)b * nl + lm] * c->ylm[(size_t)b * nl + lf]) * B[(size_t)b * (L + 1) + l];
         rhs_re[(size_t)ii * nl + lf] = -s.real();
         if (cal && rhs_im) rhs_im[(size_t)ii * nl + lf] = -s.imag();
       }
@@ -402,6 +482,7 @@ int fdmgpu_cluster_assemble(const fdmgpu_cluster *c, const double *V, double Ene
 }
 
 int fdmgpu_cluster_assembly_sizes(const fdmgpu_cluster *c, int64_t *sizes) {
+  if (c->irrep >= 0) return -1;  // the device assembler's tables describe P1 clusters
   int64_t nrow = 0, nlink = 0;
   for (int i = 0; i < c->npoint; i++) {
     int k = 0;
@@ -419,6 +500,7 @@ int fdmgpu_cluster_assembly_sizes(const fdmgpu_cluster *c, int64_t *sizes) {
 int fdmgpu_cluster_assembly_tables(const fdmgpu_cluster *c, double *base, int64_t *qdiag, int *orow,
                                    int64_t *oq, int *optr, int *ob, double *ow, double *ylm, double *brad,
                                    double *par) {
+  if (c->irrep >= 0) return -1;
   const int np = c->npoint, nl = c->nlmso;
   std::fill(base, base + c->cols.size(), 0.0);
   int r = 0, k = 0;

@@ -30,6 +30,25 @@ typedef struct fdmgpu_cluster fdmgpu_cluster;
 /* Sphere radius rsort and grid step adimp (bohr), outer expansion lmaxso
  * (nlmso = (lmaxso+1)^2 outer rows). */
 int fdmgpu_cluster_create(double rsort, double adimp, int lmaxso, fdmgpu_cluster **out);
+/* One representation of a cluster with the symmetry of the three mirror
+ * planes x = 0, y = 0, z = 0 (point group mmm; the potential must have it):
+ * the system of the igrph loop (fdm.f90:3775) for that representation.
+ * irrep bits 0, 1, 2 set = odd under the mirror normal to x, y, z (-1 = the
+ * whole cluster, as fdmgpu_cluster_create).  The grid is the wedge
+ * x, y, z >= 0 (reseau with posequiv, general.f90:9512, :8892), each point
+ * weighted by rvol = its number of mirror images / 8 (general.f90:9582-9651);
+ * a neighbour outside the wedge is its mirror image with the
+ * representation's character (laplac's isvois, general.f90:9839-9871;
+ * calcMatRow, mat.f90:1168-1231); the outer-sphere weights are
+ * 4 pi rvol / sum(rvol) (bordure, general.f90:10073-10075) and the outer
+ * unknowns are the representation's harmonics in increasing l*l + l + m.
+ * The unknown of grid row i is rvol(i) times the wave function there
+ * (mat.f90:2132-2143).  Such clusters have no device-assembler tables. */
+int fdmgpu_cluster_create_mmm(double rsort, double adimp, int lmaxso, int irrep, fdmgpu_cluster **out);
+/* rvol of the npoint interior points (1 for P1) */
+int fdmgpu_cluster_rvol(const fdmgpu_cluster *c, double *rvol);
+/* harmonic index l*l + l + m of each of the nlmso outer unknowns */
+int fdmgpu_cluster_harmonics(const fdmgpu_cluster *c, int *lm);
 void fdmgpu_cluster_destroy(fdmgpu_cluster *c);
 /* sizes[0..5] = npoint, npso, nligne, nlmso, nsortf, nsort */
 int fdmgpu_cluster_sizes(const fdmgpu_cluster *c, int *sizes);

@@ -1,0 +1,121 @@
+"""Symmetry-reduced systems (fdmgpu_cluster_create_mmm): the eight
+representations of the mirror group mmm, each solved on the wedge
+x, y, z >= 0, reproduce the solution of the whole cluster (the oracle's P1
+restatement) for a potential with that symmetry.  With u = rvol * psi on the
+wedge (mat.f90:2132-2143), the reduced rows are the whole cluster's rows
+restricted to one representation: u / rvol equals the whole solution at the
+wedge points and the outer amplitudes are equal, for every right-hand side
+(incoming wave) of the representation.  A wrong character, rvol factor,
+quadrature weight or harmonic assignment breaks it."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+BOHR = 0.52917721067
+
+
+def _potential(pts):
+    # even in each coordinate separately, but not cubic
+    x, y, z = np.abs(pts[:, 0]), np.abs(pts[:, 1]), np.abs(pts[:, 2])
+    r2 = x * x + y * y + z * z
+    return -1.3 * np.exp(-r2 / 6.0) + 0.3 * np.cos(0.9 * x) * np.cos(0.6 * y) * np.cos(1.3 * z) - 0.2
+
+
+def _rep(CL, R, h, lm, g):
+    """The representation's system, or None when no harmonic up to lm has
+    its parities (irrep 7, odd in x, y and z, needs l >= 3)."""
+    from fdmnes_amd import _lib
+    try:
+        return CL.Cluster(R, h, lm, irrep=g)
+    except _lib.FdmGpuError:
+        assert lm < 3 and g == 7
+        return None
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def test_harmonic_parities():
+    """The representation bits assigned to each real harmonic are its
+    parities under the three mirrors (checked on the harmonics themselves)."""
+    from fdmnes_amd import cluster as CL
+    lmax = 4
+    v = np.array([0.31, 0.52, 0.79])
+    y0 = O.ylm_real(lmax, v, float(np.linalg.norm(v)))
+    seen = []
+    for g in range(8):
+        c = CL.Cluster(1.2 / BOHR, 0.25 / BOHR, lmax, irrep=g)
+        for lm in c.harmonics:
+            for a in range(3):
+                w = v.copy()
+                w[a] = -w[a]
+                ya = O.ylm_real(lmax, w, float(np.linalg.norm(w)))[lm]
+                odd = (g >> a) & 1
+                assert abs(ya - (-1) ** odd * y0[lm]) <= 1e-12 * max(1.0, abs(y0[lm])), (g, lm, a)
+        seen += list(c.harmonics)
+    assert sorted(seen) == list(range((lmax + 1) ** 2))
+
+
+def test_wedge_grid_and_weights():
+    from fdmnes_amd import cluster as CL
+    full = CL.Cluster(1.6 / BOHR, 0.25 / BOHR, 2)
+    c = CL.Cluster(1.6 / BOHR, 0.25 / BOHR, 2, irrep=0)
+    pts, fp = c.points, full.points
+    assert (pts >= 0).all()
+    # every interior point of the whole cluster is the image of exactly one wedge point
+    imgs = set()
+    for p, rv in zip(pts, c.rvol):
+        orbit = {(sx * p[0], sy * p[1], sz * p[2]) for sx in (1, -1) for sy in (1, -1) for sz in (1, -1)}
+        assert rv == len(orbit) / 8
+        imgs |= {tuple(np.round(np.array(q) / full.adimp).astype(int)) for q in orbit}
+    assert imgs == {tuple(q) for q in np.round(fp / full.adimp).astype(int)}
+    # rows in radial order, as reseau's exchange sort leaves them
+    r = np.linalg.norm(pts, axis=1)
+    assert (np.diff(r) > -1e-4 * full.adimp).all()
+
+
+@pytest.mark.parametrize("E_comp,basereel", [(True, False), (False, True)])
+def test_representations_reproduce_whole_cluster(E_comp, basereel):
+    from fdmnes_amd import cluster as CL
+    R, h, lm = 1.6 / BOHR, 0.25 / BOHR, 2
+    o = O.Cluster(R, h, lm)
+    cal = E_comp or not basereel
+    E, Ei, Ec = 0.35, (0.0367 if E_comp else 0.0), 0.55
+    xyz = o.xyz[:o.npoint, :3]
+    st, xf, *_ = o.solve(_potential(xyz), E, Ei, Ec, E_comp, cal, basereel)
+    assert st == 0
+    row = {tuple(k): i for i, k in enumerate(np.round(xyz / h).astype(int))}
+    nsof = o.nligne - o.nlmso
+    scale = np.abs(xf).max()
+    bound = 1e-9 if basereel else 1e-10
+    covered = []
+    for g in range(8):
+        c = _rep(CL, R, h, lm, g)
+        if c is None:
+            continue
+        pts = c.points
+        lb1, lb2 = c.profile()
+        rp, col, vals, rhs = c.assemble(_potential(pts), E, Ei, Ec, E_comp, basereel)
+        st, u, _ = O.mat_solve_csr(lb1, lb2, rp, col, vals, rhs, cal)
+        assert st == 0
+        hm = c.harmonics
+        covered += list(hm)
+        rows = [row[tuple(k)] for k in np.round(pts / h).astype(int)]
+        psi = u[:c.npoint] / c.rvol[:, None]
+        d_grid = np.abs(psi - xf[rows][:, hm]).max() / scale
+        d_out = np.abs(u[c.npoint:] - xf[nsof + hm][:, hm]).max() / scale
+        print("irrep", g, "harmonics", list(hm), "rows", c.nligne, "grid", d_grid, "outer", d_out)
+        assert d_grid <= bound and d_out <= bound
+    assert sorted(covered) == list(range(o.nlmso))
+    # the whole solution has no outer amplitude across representations
+    gof = {}
+    for g in range(8):
+        c = _rep(CL, R, h, lm, g)
+        for k in ([] if c is None else c.harmonics):
+            gof[int(k)] = g
+    for L in range(o.nlmso):
+        for Lp in range(o.nlmso):
+            if gof[L] != gof[Lp]:
+                assert abs(xf[nsof + Lp, L]) <= bound * scale
