@@ -176,6 +176,104 @@ def tau_rates(xs, cl, G):
     return res
 
 
+def mmm_rates(w, G):
+    """Mirror-symmetric potential (point group mmm): the whole cluster's
+    mixed-precision grouped solve of G energies against the same energies
+    as the eight representations' systems on the wedge x, y, z >= 0
+    (fdmgpu_cluster_create_mmm), 16 systems per grouped solve (two
+    energies of all eight), both with calcMatRow's incoming-wave RHS, HIP
+    events around the timed solves (one warm-up group each).  Also the
+    largest difference between the two, u / rvol against the whole
+    solution, over every wedge row and harmonic of the first energy."""
+    import torch
+    from fdmnes_amd import _lib
+    from fdmnes_amd import batch as B
+    from fdmnes_amd import cluster as CL
+    from fdmnes_amd import configs as cfg
+    R, h = w.R / cfg.BOHR, w.h / cfg.BOHR
+    full = CL.Cluster(R, h, w.lmaxso)
+    E = [float(w.energies_Ry[(k * 13) % len(w.energies_Ry)]) for k in range(G)]
+    ens = [(e, w.eimag_Ry, cfg.ecinetic_out(e)) for e in E]
+    stream = torch.cuda.current_stream()
+
+    def timed(fn):
+        fn()  # warm-up: builds the graphs
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / 1e3
+
+    V = cfg.potential(w, full.points, mirror_symmetric=True)
+    lb1, lb2 = full.profile()
+    sv = B.DeviceSolver(B.FP32, True, full.nlmso)
+    slots, vals, rhs = [], [], []
+    for e in ens:
+        rp, col, v, b = full.assemble(V, *e, True, False)
+        slots.append(sv.add_system(lb1, lb2, rp, col))
+        vals.append(torch.from_numpy(v).cuda())
+        rhs.append(torch.from_numpy(b).cuda())
+    from fdmnes_amd import autotune
+    xs = [torch.zeros_like(b) for b in rhs]
+    autotune.tune(sv, slots, vals, rhs, xs, lb1, lb2, B.FP32, log=log)
+    t_full = timed(lambda: sv.solve(slots, vals, rhs, xs))
+    x0 = xs[0].cpu().numpy()
+    del sv, vals, rhs, xs
+    torch.cuda.empty_cache()
+    reps = []
+    for g in range(8):
+        try:
+            reps.append(CL.Cluster(R, h, w.lmaxso, irrep=g))
+        except _lib.FdmGpuError:
+            pass
+    nrhs = max(c.nlmso for c in reps)
+    per = 16 // len(reps)  # energies per grouped solve
+    sv = B.DeviceSolver(B.FP32, True, nrhs)
+    prof = [c.profile() for c in reps]
+    pots = [cfg.potential(w, c.points, mirror_symmetric=True) for c in reps]
+    groups = []
+    rslots = None
+    for e0 in range(0, G, per):
+        vals, rhs = [], []
+        for c, (l1, l2), Vc in zip(reps, prof, pots):
+            for e in ens[e0:e0 + per]:
+                rp, col, v, b = c.assemble(Vc, *e, True, False)
+                bp = np.zeros((c.nligne, nrhs), np.complex128)
+                bp[:, :c.nlmso] = b
+                vals.append(torch.from_numpy(v).cuda())
+                rhs.append(torch.from_numpy(bp).cuda())
+        if rslots is None:  # slot ci * per + j: representation ci, j-th energy of a group
+            rslots = [sv.add_system(l1, l2, *c.assemble(Vc, *ens[0], True, False)[:2])
+                      for c, (l1, l2), Vc in zip(reps, prof, pots) for _ in range(per)]
+        k = len(ens[e0:e0 + per])
+        gslots = [rslots[ci * per + j] for ci in range(len(reps)) for j in range(k)]
+        groups.append((gslots, vals, rhs, [torch.zeros_like(b) for b in rhs]))
+
+    def reduced():
+        for gslots, vals, rhs, xs in groups:
+            sv.solve(gslots, vals, rhs, xs)
+
+    t_red = timed(reduced)
+    row = {tuple(k): i for i, k in enumerate(np.round(full.points / h).astype(int))}
+    nsof = full.nligne - full.nlmso
+    dmax = 0.0
+    for ci, c in enumerate(reps):
+        u = groups[0][3][ci * min(per, G)].cpu().numpy()
+        hm = c.harmonics
+        rows = [row[tuple(k)] for k in np.round(c.points / h).astype(int)]
+        dmax = max(dmax, np.abs(u[:c.npoint, :c.nlmso] / c.rvol[:, None] - x0[rows][:, hm]).max(),
+                   np.abs(u[c.npoint:, :c.nlmso] - x0[nsof + hm][:, hm]).max())
+    res = {"points": G, "whole_points_per_s": G / t_full, "reduced_points_per_s": G / t_red,
+           "speedup": t_full / t_red, "max_rel_diff": dmax / np.abs(x0).max(),
+           "representations": [int(c.irrep) for c in reps], "rows_per_representation": [c.nligne for c in reps],
+           "whole_rows": full.nligne}
+    log(f"mmm: whole {G / t_full:.3f} points/s, eight representations {G / t_red:.3f} points/s, "
+        f"max rel diff {res['max_rel_diff']:.2e}")
+    return res
+
+
 def traffic_from_profile():
     """HBM bytes per k_trail launch measured by rocprofv3 PMC runs (FETCH_SIZE,
     WRITE_SIZE) of this benchmark, scaled by the same counters on a plain
@@ -228,6 +326,8 @@ def main():
     ap.add_argument("--no-roofline", action="store_true", help="skip the per-launch timed pass (profiler runs)")
     ap.add_argument("--no-assembly", action="store_true", help="skip the per-energy assembly timing")
     ap.add_argument("--no-tau", action="store_true", help="skip the post-solve tau reduction timing")
+    ap.add_argument("--mmm", action="store_true",
+                    help="also time the mirror-symmetric workload: whole cluster against its eight representations")
     ap.add_argument("--dropin", action="store_true",
                     help="also time the host-buffer drop-in entry points (PCIe and set-up included), one point each")
     args = ap.parse_args()
@@ -329,6 +429,8 @@ def main():
         out["assembly"] = assembly_rates(cl, V, w, G)
     if args.dropin:
         out.update(dropin_rates(cl, lb1, lb2, V, s, w))
+    if args.mmm:
+        out["mmm"] = mmm_rates(w, G)
     cpu = None
     if not args.no_cpu:
         procs = args.cpu_procs or min(16, os.cpu_count() or 1)

@@ -93,10 +93,13 @@ def lattice_sites(kind: str, a: float, radius: float) -> np.ndarray:
     return pts[order]
 
 
-def potential(w: Workload, points_bohr: np.ndarray, shifts: np.ndarray | None = None) -> np.ndarray:
+def potential(w: Workload, points_bohr: np.ndarray, shifts: np.ndarray | None = None,
+              mirror_symmetric: bool = False) -> np.ndarray:
     """V on the interior points (Ry).  ``shifts`` (per atom, Ry) adds
     shift_a * exp(-d/lambda) around atom a (the per-atom parameters of the
-    adjoint workload)."""
+    adjoint workload).  ``mirror_symmetric``: the noise term is drawn per
+    (|ix|, |iy|, |iz|), so V keeps the lattice's mirror planes x, y, z = 0
+    (point group mmm) and its values on the wedge x, y, z >= 0 define it."""
     sites = lattice_sites(w.lattice, w.a, w.atoms_radius) / BOHR
     lam = w.screening / BOHR
     dmin = 0.5 * w.h / BOHR
@@ -108,7 +111,12 @@ def potential(w: Workload, points_bohr: np.ndarray, shifts: np.ndarray | None = 
         if shifts is not None:
             V += shifts[a] * e
     rng = np.random.Generator(np.random.PCG64(w.seed))
-    V += rng.uniform(-w.noise_Ry, w.noise_Ry, len(points_bohr))
+    if mirror_symmetric:
+        idx = np.abs(np.rint(points_bohr / (w.h / BOHR)).astype(np.int64))
+        n = int(np.ceil(w.R / w.h)) + 3
+        V += rng.uniform(-w.noise_Ry, w.noise_Ry, (n, n, n))[idx[:, 0], idx[:, 1], idx[:, 2]]
+    else:
+        V += rng.uniform(-w.noise_Ry, w.noise_Ry, len(points_bohr))
     return V
 
 
