@@ -24,7 +24,7 @@ class Tables(C.Structure):
                 ("nsortf", C.c_int), ("nb", C.c_int), ("nrow_out", C.c_int), ("nlink", C.c_int),
                 ("nnz", C.c_int64), ("clapl0", C.c_double), ("pw", C.c_double),
                 ("row_ptr", _V), ("base", _V), ("qdiag", _V), ("orow", _V), ("oq", _V), ("optr", _V),
-                ("ob", _V), ("ow", _V), ("ylm", _V), ("brad", _V)]
+                ("ob", _V), ("ow", _V), ("ylm", _V), ("brad", _V), ("lharm", _V), ("wso", _V), ("dself", _V)]
 
 
 def _sigs(L):
@@ -65,12 +65,10 @@ class DeviceAssembler:
         self.L = L
         t = cluster.assembly_tables()
         self.nnz, self.nligne, self.nlmso, self.npoint = t["nnz"], t["nligne"], t["nlmso"], t["npoint"]
-        self._keep = {k: np.ascontiguousarray(t[k]) for k in
-                      ("row_ptr", "base", "qdiag", "orow", "oq", "optr", "ob", "ow", "ylm", "brad")}
+        names = ("row_ptr", "base", "qdiag", "orow", "oq", "optr", "ob", "ow", "ylm", "brad", "lharm", "wso", "dself")
+        self._keep = {k: np.ascontiguousarray(t[k]) for k in names}
         tb = Tables(t["npoint"], t["nligne"], t["nlmso"], t["lmaxso"], t["nsortf"], t["nb"], t["nrow_out"],
-                    t["nlink"], t["nnz"], t["clapl0"], t["pw"],
-                    *[self._keep[k].ctypes.data_as(_V) for k in
-                      ("row_ptr", "base", "qdiag", "orow", "oq", "optr", "ob", "ow", "ylm", "brad")])
+                    t["nlink"], t["nnz"], t["clapl0"], t["pw"], *[self._keep[k].ctypes.data_as(_V) for k in names])
         self.stream = library_stream(stream)
         h = _V()
         _lib.check(L.fdmgpu_assembler_create(C.byref(tb), int(max_points), _V(self.stream.cuda_stream), C.byref(h)))

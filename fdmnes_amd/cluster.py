@@ -21,6 +21,8 @@ def _L():
         L.fdmgpu_cluster_create_mmm.restype = C.c_int
         L.fdmgpu_cluster_create_mmm.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
         L.fdmgpu_cluster_rvol.argtypes = [C.c_void_p, _lib.dp]
+        L.fdmgpu_cluster_assembly_weights.restype = C.c_int
+        L.fdmgpu_cluster_assembly_weights.argtypes = [C.c_void_p, _lib.ip, _lib.dp, _lib.dp]
         L.fdmgpu_cluster_harmonics.argtypes = [C.c_void_p, _lib.ip]
         L.fdmgpu_cluster_destroy.argtypes = [C.c_void_p]
         L.fdmgpu_cluster_sizes.argtypes = [C.c_void_p, _lib.ip]
@@ -116,8 +118,6 @@ class Cluster:
     def assembly_tables(self) -> dict:
         """Energy-independent tables of the per-energy assembly
         (fdmgpu_cluster_assembly_tables): numpy arrays plus the sizes."""
-        if self.irrep >= 0:
-            raise _lib.FdmGpuError("assembly tables describe P1 clusters (no mirror representation)")
         L = _L()
         sz = np.zeros(4, np.int64)
         L.fdmgpu_cluster_assembly_sizes(self._h, sz)
@@ -132,6 +132,12 @@ class Cluster:
                                               t["ow"], t["ylm"], t["brad"], t["par"])
         if rc:
             raise _lib.FdmGpuError(f"fdmgpu_cluster_assembly_tables failed ({rc})")
+        t["lharm"] = np.zeros(self.nlmso, np.int32)
+        t["wso"] = np.zeros(self.nsortf)
+        t["dself"] = np.zeros(3 * self.npoint)
+        rc = L.fdmgpu_cluster_assembly_weights(self._h, t["lharm"], t["wso"], t["dself"])
+        if rc:
+            raise _lib.FdmGpuError(f"fdmgpu_cluster_assembly_weights failed ({rc})")
         rp, _, _, _ = self.assemble(np.zeros(self.npoint), 0.0, 0.0, 1.0, False, True)
         t["row_ptr"] = rp
         t["clapl0"], t["pw"] = float(t["par"][0]), float(t["par"][1])

@@ -31,6 +31,8 @@ struct Assembler {
   int64_t *row_ptr = nullptr, *qdiag = nullptr, *oq = nullptr;
   int *orow = nullptr, *optr = nullptr, *ob = nullptr;
   double *base = nullptr, *ow = nullptr, *ylm = nullptr, *brad = nullptr;
+  int *lharm = nullptr;                          // l of each outer unknown
+  double *wso = nullptr, *dself = nullptr;       // outer weights; diagonal mirror terms (null: none)
   double *fun = nullptr;   // Bessel/Neumann table: [point][nb][lmaxso+1] x {B, N} complex
   double *part = nullptr;  // outer-row partial sums: [point][run][nlmso^2] x {block, RHS} complex
 

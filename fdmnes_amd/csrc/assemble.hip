@@ -19,6 +19,7 @@
 // host assembly of the same rows to the last bits of the Bessel/Neumann
 // values and of those sums.
 #include <algorithm>
+#include <vector>
 #include <cmath>
 #include <string>
 
@@ -155,12 +156,18 @@ __global__ void k_asm_base(int64_t nnz, int64_t nrhs_el, const double *__restric
 
 // 3. grid diagonals
 template <bool CAL>
-__global__ void k_asm_diag(int npoint, const int64_t *__restrict__ qdiag, double clapl0, AsmPoints P, int e_comp) {
+__global__ void k_asm_diag(int npoint, const int64_t *__restrict__ qdiag, double clapl0,
+                           const double *__restrict__ dself, AsmPoints P, int e_comp) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x, g = blockIdx.y;
   if (i >= npoint) return;
   const AsmEnergy E = P.e[g];
   const int64_t q = qdiag[i];
-  const double d = P.V[g][i] - E.ener - clapl0;
+  double d = P.V[g][i] - E.ener - clapl0;
+  if (dself) {  // mirror images of the point itself, added in axis order (mat.f90:1225-1231)
+    d = d + dself[3 * i];
+    d = d + dself[3 * i + 1];
+    d = d + dself[3 * i + 2];
+  }
   if constexpr (CAL) {
     P.vals[g][2 * q] = d;
     P.vals[g][2 * q + 1] = e_comp ? -E.eimag : 0.0;
@@ -175,11 +182,11 @@ template <bool CAL>
 __global__ void k_asm_outcols(int nrow_out, int nl, int nb, int L, const int *__restrict__ orow,
                               const int64_t *__restrict__ oq, const int *__restrict__ optr,
                               const int *__restrict__ ob, const double *__restrict__ ow,
-                              const double *__restrict__ ylm, const cz *__restrict__ fun, AsmPoints P,
-                              int basereel) {
+                              const double *__restrict__ ylm, const int *__restrict__ lharm,
+                              const cz *__restrict__ fun, AsmPoints P, int basereel) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x, g = blockIdx.y;
   if (t >= nrow_out * nl) return;
-  const int r = t / nl, lm = t % nl, l = lof(lm);
+  const int r = t / nl, lm = t % nl, l = lharm[lm];
   const cz *fg = fun + (int64_t)g * nb * (L + 1) * 2;
   cz row = {0, 0}, rrow = {0, 0};
   for (int k = optr[r]; k < optr[r + 1]; k++) {
@@ -204,27 +211,29 @@ __global__ void k_asm_outcols(int nrow_out, int nl, int nb, int L, const int *__
   }
 }
 
-// 5. outer rows: block (lm, lp) = sum_b pw Y_lm(b) Y_lp(b) cfac(b, l(lp)) and
-// RHS (lm, lf) = -sum_b pw Y_lm(b) Y_lf(b) B(b, l(lf)), over the nsortf inner
+// 5. outer rows: block (lm, lp) = sum_b w_b Y_lm(b) Y_lp(b) cfac(b, l(lp)) and
+// RHS (lm, lf) = -sum_b w_b Y_lm(b) Y_lf(b) B(b, l(lf)), over the nsortf inner
 // outer points.  First pass: one workgroup per (point, run of ASM_CB points
 // b, 256 (lm, lp) pairs), each thread summing its pair over the run in
 // order; second pass: one thread per (point, pair) adds the runs' partial
 // sums in run order.  The result does not depend on timing.
 constexpr int ASM_CB = 256;
-__global__ __launch_bounds__(256) void k_asm_outrows_part(int nl, int nb, int L, int nsortf, double pw,
+__global__ __launch_bounds__(256) void k_asm_outrows_part(int nl, int nb, int L, int nsortf,
+                                                           const double *__restrict__ wso,
                                                            const double *__restrict__ ylm,
+                                                           const int *__restrict__ lharm,
                                                            const cz *__restrict__ fun, int basereel,
                                                            cz *__restrict__ part) {
   const int run = blockIdx.x, nrun = gridDim.x, g = blockIdx.z;
   const int pr = blockIdx.y * blockDim.x + threadIdx.x;
   if (pr >= nl * nl) return;
-  const int lm = pr / nl, lp = pr % nl, l = lof(lp);
+  const int lm = pr / nl, lp = pr % nl, l = lharm[lp];
   const cz *fg = fun + (int64_t)g * nb * (L + 1) * 2;
   cz s = {0, 0}, sr = {0, 0};
   const int b1 = min(nsortf, (run + 1) * ASM_CB);
 #pragma unroll 8
   for (int b = run * ASM_CB; b < b1; b++) {
-    const double c = pw * ylm[(int64_t)b * nl + lm] * ylm[(int64_t)b * nl + lp];
+    const double c = wso[b] * ylm[(int64_t)b * nl + lm] * ylm[(int64_t)b * nl + lp];
     const cz *f = fg + (int64_t)b * (L + 1) * 2;
     s = add(s, scale(cfac(f, l, basereel), c));
     sr = add(sr, scale(f[2 * l], c));
@@ -272,13 +281,16 @@ int upload(T **d, const T *h, int64_t n) {
 
 Assembler::~Assembler() {
   for (void *p : {(void *)row_ptr, (void *)qdiag, (void *)oq, (void *)orow, (void *)optr, (void *)ob, (void *)base,
-                  (void *)ow, (void *)ylm, (void *)brad, (void *)fun, (void *)part})
+                  (void *)ow, (void *)ylm, (void *)brad, (void *)lharm, (void *)wso, (void *)dself, (void *)fun,
+                  (void *)part})
     if (p) (void)hipFree(p);
 }
 
 int Assembler::init(const fdmgpu_assembly_tables *t, int maxp, hipStream_t s) {
+  const int nall = (t->lmaxso + 1) * (t->lmaxso + 1);  // a representation of mmm has a subset (lharm)
   if (t->npoint <= 0 || t->nligne != t->npoint + t->nlmso || t->lmaxso < 0 || t->lmaxso > 40 ||
-      t->nlmso != (t->lmaxso + 1) * (t->lmaxso + 1) || t->nsortf <= 0 || t->nb < t->nsortf || t->nrow_out < 0 ||
+      (t->lharm ? (t->nlmso < 1 || t->nlmso > nall) : t->nlmso != nall) || t->nsortf <= 0 || t->nb < t->nsortf ||
+      t->nrow_out < 0 ||
       t->nlink < t->nrow_out || t->nnz <= 0 || maxp < 1 || maxp > ASM_MAXP) {
     set_error("fdmgpu_assembler_create: inconsistent table sizes");
     return FDMGPU_EINVAL;
@@ -306,6 +318,12 @@ int Assembler::init(const fdmgpu_assembly_tables *t, int maxp, hipStream_t s) {
       return FDMGPU_EINVAL;
     }
   }
+  if (t->lharm)
+    for (int k = 0; k < t->nlmso; k++)
+      if (t->lharm[k] < 0 || t->lharm[k] > t->lmaxso) {
+        set_error("fdmgpu_assembler_create: lharm out of range");
+        return FDMGPU_EINVAL;
+      }
   if (t->nrow_out > 0 && (t->optr[0] != 0 || t->optr[t->nrow_out] != t->nlink)) {
     set_error("fdmgpu_assembler_create: optr does not span nlink");
     return FDMGPU_EINVAL;
@@ -338,6 +356,18 @@ int Assembler::init(const fdmgpu_assembly_tables *t, int maxp, hipStream_t s) {
       upload(&ob, t->ob, nlink) || upload(&ow, t->ow, nlink) || upload(&ylm, t->ylm, (int64_t)nb * nlmso) ||
       upload(&brad, t->brad, nb))
     return FDMGPU_EHIP;
+  // P1 defaults: l of l*l + l + m, the uniform weight pw, no mirror terms
+  std::vector<int> hl(nlmso);
+  std::vector<double> hw(nsortf, pw);
+  for (int k = 0; k < nlmso; k++) {
+    int l = 0;
+    while ((l + 1) * (l + 1) <= k) l++;
+    hl[k] = t->lharm ? t->lharm[k] : l;
+  }
+  if (t->wso) std::copy(t->wso, t->wso + nsortf, hw.begin());
+  if (upload(&lharm, hl.data(), nlmso) || upload(&wso, hw.data(), nsortf) ||
+      (t->dself && upload(&dself, t->dself, 3 * (int64_t)npoint)))
+    return FDMGPU_EHIP;
   FDM_HIP(hipMalloc(&fun, sizeof(cz) * (size_t)maxp * nb * (lmaxso + 1) * 2));
   const size_t nrun = (nsortf + ASM_CB - 1) / ASM_CB;
   FDM_HIP(hipMalloc(&part, sizeof(cz) * 2 * (size_t)maxp * nrun * nlmso * nlmso));
@@ -363,20 +393,22 @@ int Assembler::run(int count, const AsmPoints &P, bool e_comp, bool basereel, bo
   cz *Q = (cz *)part;
   if (cal) {
     if (!update_only) hipLaunchKernelGGL(k_asm_base<true>, dim3(gb, count), dim3(NT), 0, stream, nnz, nrhs_el, base, P);
-    hipLaunchKernelGGL(k_asm_diag<true>, dim3(gd, count), dim3(NT), 0, stream, npoint, qdiag, clapl0, P, (int)e_comp);
+    hipLaunchKernelGGL(k_asm_diag<true>, dim3(gd, count), dim3(NT), 0, stream, npoint, qdiag, clapl0, dself, P,
+                       (int)e_comp);
     if (nrow_out > 0)
       hipLaunchKernelGGL(k_asm_outcols<true>, dim3(go, count), dim3(NT), 0, stream, nrow_out, nlmso, nb, lmaxso, orow,
-                         oq, optr, ob, ow, ylm, F, P, (int)basereel);
-    hipLaunchKernelGGL(k_asm_outrows_part, gp, dim3(256), 0, stream, nlmso, nb, lmaxso, nsortf, pw, ylm, F,
-                       (int)basereel, Q);
+                         oq, optr, ob, ow, ylm, lharm, F, P, (int)basereel);
+    hipLaunchKernelGGL(k_asm_outrows_part, gp, dim3(256), 0, stream, nlmso, nb, lmaxso, nsortf, wso, ylm, lharm,
+                       F, (int)basereel, Q);
     hipLaunchKernelGGL(k_asm_outrows_sum<true>, gs, dim3(256), 0, stream, npoint, nlmso, nsortf, nrun, row_ptr, Q, P);
   } else {
     if (!update_only) hipLaunchKernelGGL(k_asm_base<false>, dim3(gb, count), dim3(NT), 0, stream, nnz, nrhs_el, base, P);
-    hipLaunchKernelGGL(k_asm_diag<false>, dim3(gd, count), dim3(NT), 0, stream, npoint, qdiag, clapl0, P, 0);
+    hipLaunchKernelGGL(k_asm_diag<false>, dim3(gd, count), dim3(NT), 0, stream, npoint, qdiag, clapl0, dself, P, 0);
     if (nrow_out > 0)
       hipLaunchKernelGGL(k_asm_outcols<false>, dim3(go, count), dim3(NT), 0, stream, nrow_out, nlmso, nb, lmaxso, orow,
-                         oq, optr, ob, ow, ylm, F, P, 1);
-    hipLaunchKernelGGL(k_asm_outrows_part, gp, dim3(256), 0, stream, nlmso, nb, lmaxso, nsortf, pw, ylm, F, 1, Q);
+                         oq, optr, ob, ow, ylm, lharm, F, P, 1);
+    hipLaunchKernelGGL(k_asm_outrows_part, gp, dim3(256), 0, stream, nlmso, nb, lmaxso, nsortf, wso, ylm, lharm,
+                       F, 1, Q);
     hipLaunchKernelGGL(k_asm_outrows_sum<false>, gs, dim3(256), 0, stream, npoint, nlmso, nsortf, nrun, row_ptr, Q, P);
   }
   FDM_HIP(hipGetLastError());

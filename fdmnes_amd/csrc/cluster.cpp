@@ -307,6 +307,13 @@ inline int lof(int lm) {  // l of the harmonic index l*l + l + m
   return l;
 }
 
+// stencil term of grid point i towards direction d: -clapl(v) times the
+// character of the mirrors that bring the neighbour into the wedge, times
+// rvol(i) (calcMatRow, mat.f90:1164-1222; -clapl(v) without symmetry)
+inline double stencil(const fdmgpu_cluster *c, int i, int d) {
+  return (character(c->irrep, c->flip[(size_t)i * 12 + d]) * -c->clapl[d + 1]) * c->rvol[i];
+}
+
 }  // namespace
 
 extern "C" {
@@ -416,9 +423,6 @@ int fdmgpu_cluster_assemble(const fdmgpu_cluster *c, const double *V, double Ene
     // stencil term of direction d: -clapl(v) times the character of the
     // mirrors that bring the neighbour into the wedge, times rvol(i)
     // (mat.f90:1164-1222; 1 and 1 without symmetry)
-    auto stencil = [&](int d) {
-      return (character(c->irrep, c->flip[(size_t)i * 12 + d]) * -c->clapl[d + 1]) * c->rvol[i];
-    };
     // interior entries in column order, each neighbour's terms over rvol(j)
     // added in direction order (:1225-1231); a neighbour that is the point's
     // own mirror image adds to the diagonal
@@ -427,7 +431,7 @@ int fdmgpu_cluster_assemble(const fdmgpu_cluster *c, const double *V, double Ene
       const int j = col[q] - 1;
       cd v = (j == i) ? diag : cd(0);
       for (int d = 0; d < 12; d++)
-        if (c->nbr[(size_t)i * 12 + d] == j) v += stencil(d) / c->rvol[j];
+        if (c->nbr[(size_t)i * 12 + d] == j) v += stencil(c, i, d) / c->rvol[j];
       re[q] = v.real();
       if (cal) im[q] = v.imag();
     }
@@ -435,7 +439,7 @@ int fdmgpu_cluster_assemble(const fdmgpu_cluster *c, const double *V, double Ene
       const int j = c->nbr[(size_t)i * 12 + d];
       if (j < np) continue;
       const int b = c->slot[j];
-      const double w = stencil(d);
+      const double w = stencil(c, i, d);
       for (int lm = 0; lm < nl; lm++) {
         const int l = lof(c->lmidx[lm]);
         const double y = c->ylm[(size_t)b * nl + lm];
@@ -482,7 +486,6 @@ int fdmgpu_cluster_assemble(const fdmgpu_cluster *c, const double *V, double Ene
 }
 
 int fdmgpu_cluster_assembly_sizes(const fdmgpu_cluster *c, int64_t *sizes) {
-  if (c->irrep >= 0) return -1;  // the device assembler's tables describe P1 clusters
   int64_t nrow = 0, nlink = 0;
   for (int i = 0; i < c->npoint; i++) {
     int k = 0;
@@ -500,7 +503,6 @@ int fdmgpu_cluster_assembly_sizes(const fdmgpu_cluster *c, int64_t *sizes) {
 int fdmgpu_cluster_assembly_tables(const fdmgpu_cluster *c, double *base, int64_t *qdiag, int *orow,
                                    int64_t *oq, int *optr, int *ob, double *ow, double *ylm, double *brad,
                                    double *par) {
-  if (c->irrep >= 0) return -1;
   const int np = c->npoint, nl = c->nlmso;
   std::fill(base, base + c->cols.size(), 0.0);
   int r = 0, k = 0;
@@ -510,19 +512,19 @@ int fdmgpu_cluster_assembly_tables(const fdmgpu_cluster *c, double *base, int64_
     int64_t q = q0;
     for (; q < q1 && c->cols[q] <= np; q++) {
       const int j = c->cols[q] - 1;
-      if (j == i) {
+      if (j == i) {  // E-dependent; a point's own mirror images are in dself
         qdiag[i] = q;
         continue;
       }
       for (int d = 0; d < 12; d++)
-        if (c->nbr[(size_t)i * 12 + d] == j) base[q] = -c->clapl[d + 1];
+        if (c->nbr[(size_t)i * 12 + d] == j) base[q] += stencil(c, i, d) / c->rvol[j];
     }
     bool out = false;
     for (int d = 0; d < 12; d++) {
       const int j = c->nbr[(size_t)i * 12 + d];
       if (j < np) continue;
       ob[k] = c->slot[j];
-      ow[k] = -c->clapl[d + 1];
+      ow[k] = stencil(c, i, d);
       k++;
       out = true;
     }
@@ -535,12 +537,24 @@ int fdmgpu_cluster_assembly_tables(const fdmgpu_cluster *c, double *base, int64_
   const double pw = 4 * kPi / c->nsortf;
   for (int lm = 0; lm < nl; lm++) {
     const int64_t q = c->rp[np + lm];
-    for (int b = 0; b < c->nsortf; b++) base[q + b] = -pw * c->ylm[(size_t)b * nl + lm];
+    for (int b = 0; b < c->nsortf; b++) base[q + b] = -(c->wso[b] * c->ylm[(size_t)b * nl + lm]) / c->rvol[c->bnd[b]];
   }
   std::memcpy(ylm, c->ylm.data(), sizeof(double) * c->ylm.size());
   for (size_t b = 0; b < c->bnd.size(); b++) brad[b] = c->rad[c->bnd[b]];
   par[0] = c->clapl[0];
   par[1] = pw;
+  return 0;
+}
+
+int fdmgpu_cluster_assembly_weights(const fdmgpu_cluster *c, int *lharm, double *wso, double *dself) {
+  for (int k = 0; k < c->nlmso; k++) lharm[k] = lof(c->lmidx[k]);
+  std::memcpy(wso, c->wso.data(), sizeof(double) * c->nsortf);
+  for (int i = 0; i < c->npoint; i++) {
+    double *s = dself + 3 * (size_t)i;
+    s[0] = s[1] = s[2] = 0.0;
+    for (int d = 0; d < 12; d++)  // only the second-shell steps towards a mirror can land on the point
+      if (c->nbr[(size_t)i * 12 + d] == i) s[(d % 6) / 2] += stencil(c, i, d) / c->rvol[i];
+  }
   return 0;
 }
 

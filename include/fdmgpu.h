@@ -186,6 +186,11 @@ typedef struct {
   const double *ow;         /* [nlink] its stencil weight -clapl(v) */
   const double *ylm;        /* [nb*nlmso] real harmonics at the outer points */
   const double *brad;       /* [nb] radii of the outer points (bohr) */
+  /* symmetry-reduced systems (fdmgpu_cluster_assembly_weights); NULL = P1:
+   * l of l*l + l + m, pw for every point, no self terms */
+  const int *lharm;         /* [nlmso] l of each outer unknown */
+  const double *wso;        /* [nsortf] outer-row quadrature weights */
+  const double *dself;      /* [3*npoint] mirror-image terms added to the diagonals */
 } fdmgpu_assembly_tables;
 /* max_points (1..16): energy points per call; hip_stream may be NULL (library
  * stream).  The tables are checked for consistency and copied. */

@@ -88,10 +88,21 @@ int fdmgpu_cluster_assembly_sizes(const fdmgpu_cluster *c, int64_t *sizes);
  *                -clapl(v)
  * ylm[nb*nlmso]  real harmonics at the outer-sphere points
  * brad[nb]       their radii (bohr)
- * par[2]         clapl(0), outer-row quadrature weight pw = 4 pi / nsortf */
+ * par[2]         clapl(0), outer-row quadrature weight pw = 4 pi / nsortf
+ * For a representation of mmm the stencil weights carry the character and
+ * rvol factors (-clapl(v) chi rvol(i), over rvol(j) in base) and the outer
+ * rows' boundary columns are -poidso Ylm / rvol(j). */
 int fdmgpu_cluster_assembly_tables(const fdmgpu_cluster *c, double *base, int64_t *qdiag, int *orow,
                                    int64_t *oq, int *optr, int *ob, double *ow, double *ylm, double *brad,
                                    double *par);
+/* The rest of a device assembler's tables:
+ * lharm[nlmso]    l of each outer unknown
+ * wso[nsortf]     outer-row quadrature weight of each inner boundary point
+ *                 (poidso, bordure general.f90:10073-10075; pw for P1)
+ * dself[3*npoint] per axis, the stencil term of a point's own mirror image
+ *                 (second-shell neighbour across a mirror plane), which adds
+ *                 to its diagonal (0 for P1) */
+int fdmgpu_cluster_assembly_weights(const fdmgpu_cluster *c, int *lharm, double *wso, double *dself);
 
 #ifdef __cplusplus
 }

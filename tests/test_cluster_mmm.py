@@ -119,3 +119,65 @@ def test_representations_reproduce_whole_cluster(E_comp, basereel):
         for Lp in range(o.nlmso):
             if gof[L] != gof[Lp]:
                 assert abs(xf[nsof + Lp, L]) <= bound * scale
+
+
+@pytest.mark.parametrize("E_comp,basereel", [(True, False), (False, True)])
+def test_assembly_tables_rebuild_representation_rows(E_comp, basereel):
+    """The device assembler's tables of each representation (stencil weights
+    with characters and rvol, per-point outer weights, mirror terms of the
+    diagonal, l of each outer unknown) rebuild the host assembler's rows:
+    energy-independent entries and diagonals bit for bit, the outer-sphere
+    entries and RHS to the last bits of the Bessel/Neumann values."""
+    from fdmnes_amd import cluster as CL
+    R, h, lm = 2.0 / BOHR, 0.25 / BOHR, 3
+    cal = E_comp or not basereel
+    E, Ei, Ec = 0.45, (0.0367 if E_comp else 0.0), 0.7
+    for g in range(8):
+        c = CL.Cluster(R, h, lm, irrep=g)
+        t = c.assembly_tables()
+        V = _potential(c.points)
+        _, _, hv, hr = c.assemble(V, E, Ei, Ec, E_comp, basereel)
+        nl, npn, ns = t["nlmso"], t["npoint"], t["nsortf"]
+        fixed = np.ones(c.nnz, bool)
+        fixed[t["qdiag"]] = False
+        for r in range(t["nrow_out"]):
+            fixed[t["oq"][r]: t["oq"][r] + nl] = False
+        for k in range(nl):
+            q = t["row_ptr"][npn + k] + ns
+            fixed[q: q + nl] = False
+        assert np.array_equal(np.real(hv)[fixed], t["base"][fixed])
+        d = V - E - t["clapl0"]
+        ds = t["dself"].reshape(npn, 3)
+        for a in range(3):
+            d = d + ds[:, a]
+        assert np.array_equal(np.real(hv)[t["qdiag"]], d)
+        # outer-sphere entries from the tables
+        k = np.sqrt(complex(Ec, Ei)) if E_comp else np.sqrt(Ec)
+        fn = np.sqrt(k / np.pi)
+        Bv = np.zeros((t["nb"], lm + 1), complex)
+        Nv = np.zeros((t["nb"], lm + 1), complex)
+        for b in range(t["nb"]):
+            if E_comp:
+                Bv[b], Nv[b] = O.cbessneu(fn, k * t["brad"][b], lm)
+            else:
+                Bv[b], Nv[b] = O.cbessneur(fn, k * t["brad"][b], lm)
+        cf = Nv if basereel else Bv + 1j * Nv
+        Y = t["ylm"].reshape(t["nb"], nl)
+        L = t["lharm"]
+        vals = np.array(hv, complex)
+        rhs = np.zeros((t["nligne"], nl), complex)
+        for r in range(t["nrow_out"]):
+            k0, k1 = t["optr"][r], t["optr"][r + 1]
+            b, w = t["ob"][k0:k1], t["ow"][k0:k1]
+            for m in range(nl):
+                vals[t["oq"][r] + m] = np.sum(cf[b, L[m]] * Y[b, m] * w)
+                rhs[t["orow"][r], m] = -np.sum(Y[b, m] * Bv[b, L[m]] * w)
+        for m in range(nl):
+            q = t["row_ptr"][npn + m] + ns
+            for p in range(nl):
+                cc = t["wso"] * Y[:ns, m] * Y[:ns, p]
+                vals[q + p] = np.sum(cc * cf[:ns, L[p]])
+                rhs[npn + m, p] = -np.sum(cc * Bv[:ns, L[p]])
+        assert np.abs(vals - hv).max() <= 1e-13 * np.abs(hv).max()
+        assert np.abs(rhs - hr).max() <= 1e-13 * np.abs(hr).max()
+    assert (t["dself"] != 0).any()

@@ -124,3 +124,65 @@ def test_iron_cluster_reduced_equals_whole_on_device():
                 np.abs(x[c.npoint:, :c.nlmso] - xf[nsof + hm][:, hm]).max()) / np.abs(xf).max()
         print("irrep", c.irrep, "rows", c.nligne, "rel", d, flush=True)
         assert d <= 1e-10
+
+
+@pytest.mark.parametrize("E_comp,basereel", [(True, False), (False, True)])
+def test_device_assembly_of_representations(E_comp, basereel):
+    """fdmgpu_assembler_* on each representation's tables writes the host
+    assembler's rows: energy-independent entries and diagonals bit for bit,
+    outer-sphere entries and RHS to 1e-13 (the device's Bessel/Neumann
+    values), fresh and updated in place."""
+    import torch
+    from fdmnes_amd import cluster as CL
+    from fdmnes_amd.assemble import DeviceAssembler
+    R, h, lm = 2.0 / BOHR, 0.25 / BOHR, 3
+    cal = E_comp or not basereel
+    dt = torch.complex128 if cal else torch.float64
+    ens = [(0.45, 0.0367 if E_comp else 0.0, 0.7), (0.9, 0.0367 if E_comp else 0.0, 1.2)]
+    for c in _reps(CL, R, h, lm):
+        print("device assembly irrep", c.irrep, flush=True)
+        t = c.assembly_tables()
+        V = _sym_potential(c.points)
+        a = DeviceAssembler(c)
+        vals = [torch.full((c.nnz,), 5.0, dtype=dt, device="cuda") for _ in ens]
+        rhs = [torch.full((c.nligne, c.nlmso), 5.0, dtype=dt, device="cuda") for _ in ens]
+        Vd = [torch.from_numpy(V).cuda()] * len(ens)
+        a.run(Vd, ens, E_comp, basereel, vals, rhs)
+        a.run(Vd, ens[::-1], E_comp, basereel, vals, rhs, update_only=True)  # each buffer: the other energy
+        torch.cuda.synchronize()
+        fixed = np.ones(c.nnz, bool)
+        fixed[t["qdiag"]] = False
+        for r in range(t["nrow_out"]):
+            fixed[t["oq"][r]: t["oq"][r] + c.nlmso] = False
+        for k in range(c.nlmso):
+            q = t["row_ptr"][c.npoint + k] + c.nsortf
+            fixed[q: q + c.nlmso] = False
+        for k, e in enumerate(ens[::-1]):
+            _, _, hv, hr = c.assemble(V, *e, E_comp, basereel)
+            dv, dr = vals[k].cpu().numpy(), rhs[k].cpu().numpy()
+            assert np.array_equal(dv[fixed], hv[fixed]) and np.array_equal(dv[t["qdiag"]], hv[t["qdiag"]])
+            assert np.abs(dv - hv).max() <= 1e-13 * np.abs(hv).max()
+            assert np.abs(dr - hr).max() <= 1e-13 * np.abs(hr).max()
+
+
+def test_energy_loop_per_representation_matches_oracle():
+    """One EnergyLoop (device assembly, grouped solve) per representation;
+    u / rvol against the oracle's whole-cluster solution, three energies."""
+    from fdmnes_amd import cluster as CL
+    from fdmnes_amd.energy_loop import EnergyLoop
+    R, h, lm = 2.0 / BOHR, 0.25 / BOHR, 3
+    ens = [(0.2 + 0.3 * k, 0.0367, 0.5 + 0.3 * k) for k in range(3)]
+    o = O.Cluster(R, h, lm)
+    xyz = o.xyz[:o.npoint, :3]
+    row = {tuple(k): i for i, k in enumerate(np.round(xyz / h).astype(int))}
+    nsof = o.nligne - o.nlmso
+    full = [o.solve(_sym_potential(xyz), *e, True, True, False)[1] for e in ens]
+    for c in _reps(CL, R, h, lm):
+        out = EnergyLoop(c, precision=1, group=2).run(_sym_potential(c.points), ens, keep_solutions=True)
+        hm = c.harmonics
+        rows = [row[tuple(k)] for k in np.round(c.points / h).astype(int)]
+        for x, xf in zip(out["x"], full):
+            d = max(np.abs(x[:c.npoint] / c.rvol[:, None] - xf[rows][:, hm]).max(),
+                    np.abs(x[c.npoint:] - xf[nsof + hm][:, hm]).max()) / np.abs(xf).max()
+            print("irrep", c.irrep, "loop rel", d, flush=True)
+            assert d <= 1e-10
