@@ -180,7 +180,7 @@ def mmm_rates(w, G):
     """Mirror-symmetric potential (point group mmm): the whole cluster's
     mixed-precision grouped solve of G energies against the same energies
     as the eight representations' systems on the wedge x, y, z >= 0
-    (fdmgpu_cluster_create_mmm), 16 systems per grouped solve (two
+    (fdmgpu_cluster_create_mmm), 32 systems per grouped solve (four
     energies of all eight), both with calcMatRow's incoming-wave RHS, HIP
     events around the timed solves (one warm-up group each).  Also the
     largest difference between the two, u / rvol against the whole
@@ -229,7 +229,7 @@ def mmm_rates(w, G):
         except _lib.FdmGpuError:
             pass
     nrhs = max(c.nlmso for c in reps)
-    per = 16 // len(reps)  # energies per grouped solve
+    per = 32 // len(reps)  # energies per grouped solve of up to 32 systems
     sv = B.DeviceSolver(B.FP32, True, nrhs)
     prof = [c.profile() for c in reps]
     pots = [cfg.potential(w, c.points, mirror_symmetric=True) for c in reps]
@@ -251,6 +251,11 @@ def mmm_rates(w, G):
         gslots = [rslots[ci * per + j] for ci in range(len(reps)) for j in range(k)]
         groups.append((gslots, vals, rhs, [torch.zeros_like(b) for b in rhs]))
 
+    # schedule for the reduced group, tuned (or taken from the cache) on the
+    # first group, keyed by the first representation's profile
+    gs0, v0, r0, x0r = groups[0]
+    ent = autotune.tune(sv, gs0, v0, r0, x0r, prof[0][0], prof[0][1], B.FP32, log=log)
+
     def reduced():
         for gslots, vals, rhs, xs in groups:
             sv.solve(gslots, vals, rhs, xs)
@@ -268,7 +273,7 @@ def mmm_rates(w, G):
     res = {"points": G, "whole_points_per_s": G / t_full, "reduced_points_per_s": G / t_red,
            "speedup": t_full / t_red, "max_rel_diff": dmax / np.abs(x0).max(),
            "representations": [int(c.irrep) for c in reps], "rows_per_representation": [c.nligne for c in reps],
-           "whole_rows": full.nligne}
+           "whole_rows": full.nligne, "reduced_schedule": ent["best"]}
     log(f"mmm: whole {G / t_full:.3f} points/s, eight representations {G / t_red:.3f} points/s, "
         f"max rel diff {res['max_rel_diff']:.2e}")
     return res

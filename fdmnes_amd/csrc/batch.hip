@@ -101,7 +101,7 @@ int Solver<R, C>::add_system(int n, const int *lb1, const int *lb2, const int64_
     return -1;
   }
   if ((int)slots.size() >= MAXG) {
-    set_error("a solver holds at most 16 systems");
+    set_error("a solver holds at most " + std::to_string(MAXG) + " systems");
     return -1;
   }
   if (row_ptr[0] != 0) {
@@ -404,7 +404,7 @@ int Solver<R, C>::adjoint(int count, const int *ids, const double *const *src, c
 template <class R, bool C>
 int Solver<R, C>::sigma(int count, const int *ids, const double *const *src, const double *const *x, double *out) {
   if (count > MAXG) {
-    set_error("sigma: at most 16 points per call");
+    set_error("sigma: at most " + std::to_string(MAXG) + " points per call");
     return -1;
   }
   // partials and results of up to MAXG points, allocated on first use

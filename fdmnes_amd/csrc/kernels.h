@@ -28,7 +28,7 @@ namespace fdmgpu {
 constexpr int TB = 64;    // tile edge = panel width
 constexpr int NT = 256;   // threads per workgroup
 constexpr int RC = 16;    // right-hand sides per solve chunk
-constexpr int MAXG = 16;  // systems per grouped launch
+constexpr int MAXG = 32;  // systems per grouped launch
 
 // Device pointers held in structs are generic (flat) to the compiler, and
 // flat loads count in lgkmcnt as well as vmcnt, so a wait for LDS results

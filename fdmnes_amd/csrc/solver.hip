@@ -125,7 +125,7 @@ double Plan::tile_update_macs() const {
 template <class R, bool C>
 int Group<R, C>::add(Plan *plan, int nrhs, R *X) {
   if ((int)m.size() >= MAXG) {
-    set_error("a group holds at most 16 systems");
+    set_error("a group holds at most " + std::to_string(MAXG) + " systems");
     return -1;
   }
   Member mb;

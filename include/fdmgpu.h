@@ -90,7 +90,7 @@ int fdmgpu_mat_solve_mixed(int nligne, int nlmso, int cal_comp, const int *lb1, 
 /*
  * Device-resident solver for the energy loop (fdm.f90:3493-4192 calls mat
  * once per energy point; here many points are solved together).  A solver
- * holds up to 16 system slots of one precision and arithmetic; each slot is
+ * holds up to 32 system slots of one precision and arithmetic; each slot is
  * a structure (profile lb1/lb2 + compressed-row pattern) with its own device
  * factor storage.  A solve call factorises and solves the listed slots
  * together — every kernel launch is grouped over them, so points of clusters

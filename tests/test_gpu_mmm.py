@@ -1,6 +1,6 @@
 """Symmetry-reduced systems on the GPU: the representations of the mirror
 group mmm of several energies, of uneven size and right-hand-side count,
-solved together in one grouped launch sequence (right-hand sides padded
+solved together in one grouped launch sequence (up to 32 systems) (right-hand sides padded
 with zero columns to the largest count), reproduce the whole cluster's
 solution, against the oracle on a small cluster and against the device's
 own whole-cluster solve on an iron cluster of 14k unknowns.  Bound 1e-10:
@@ -47,7 +47,7 @@ def _solve_reduced(reps, Vfun, ens, E_comp, basereel, precision):
             rhs.append(torch.from_numpy(bp).to(dt).cuda())
             xs.append(torch.zeros_like(rhs[-1]))
             keys.append((c, e))
-    assert len(slots) <= 16
+    assert len(slots) <= 32
     sv.solve(slots, vals, rhs, xs)
     torch.cuda.synchronize()
     out = [(c, e, x.cpu().numpy()) for (c, e), x in zip(keys, xs)]
@@ -61,14 +61,16 @@ def _sym_potential(pts):
     return -1.3 * np.exp(-(x * x + y * y + z * z) / 6.0) + 0.3 * np.cos(0.9 * x) * np.cos(0.6 * y) * np.cos(1.3 * z) - 0.2
 
 
-@pytest.mark.parametrize("precision", [0, 1])
-def test_grouped_representations_match_oracle(precision):
+@pytest.mark.parametrize("precision,nE", [(0, 2), (1, 2), (1, 4)])
+def test_grouped_representations_match_oracle(precision, nE):
+    """nE energies of all eight representations: 16 or 32 systems (the
+    grouped launch's limit) in one solve."""
     from fdmnes_amd import cluster as CL
-    print(f"mmm grouped, precision {precision}", flush=True)
+    print(f"mmm grouped, precision {precision}, {8 * nE} systems", flush=True)
     R, h, lm = 2.0 / BOHR, 0.25 / BOHR, 3
     reps = _reps(CL, R, h, lm)
     assert len(reps) == 8
-    ens = [(0.35, 0.0367, 0.55), (0.8, 0.0367, 1.0)]
+    ens = [(0.35, 0.0367, 0.55), (0.8, 0.0367, 1.0), (0.1, 0.0367, 0.3), (1.3, 0.0367, 1.5)][:nE]
     out = _solve_reduced(reps, _sym_potential, ens, True, False, precision)
     o = O.Cluster(R, h, lm)
     xyz = o.xyz[:o.npoint, :3]
