@@ -3,9 +3,9 @@
  *
  * A plain-C, single-threaded restatement of the FDMNES finite-difference path
  * (grid -> stencil -> outer-sphere boundary -> row assembly -> Gaussian band
- * solver).  It is the checker for the HIP library in fdmnes_amd/: only tests/,
- * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and
- * never as the thing measured or shipped.
+ * solver).  It checks the results of the HIP library in fdmnes_amd/: only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
+ * it, and never as the thing measured or shipped.
  *
  * Every function cites the reference file:line it follows (paths relative to
  * the reference's source/ directory).  Indices are 1-based wherever the

@@ -3,7 +3,7 @@
 ctypes wrapper over ``oracle/liboracle.so``, the plain-C restatement of the
 FDMNES finite-difference path (see ``fdm_oracle.c`` for the file:line map).
 Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s cpu_baseline
-leg may import this module, and only as the checker.
+leg may import this module, and only to check results.
 """
 from __future__ import annotations
 
