@@ -15,7 +15,7 @@ from .assemble import DeviceAssembler
 from .batch import FP32, FP64, DeviceSolver
 from .tau import split, tau_reduce
 
-__all__ = ["EnergyLoop", "FP32", "FP64"]
+__all__ = ["EnergyLoop", "RepresentationLoop", "FP32", "FP64"]
 
 
 class EnergyLoop:
@@ -83,6 +83,81 @@ class EnergyLoop:
             if keep_solutions:
                 xs += [xi.cpu().numpy() for xi in self.x[:k]]
         out = {"sigma": sigma, "tau": tau, "sweeps": sweeps}
+        if keep_solutions:
+            out["x"] = xs
+        return out
+
+
+class RepresentationLoop:
+    """The energy loop of a cluster split into representations (the igrph
+    loop, fdm.f90:3775, e.g. the eight systems of ``Cluster(..., irrep=g)``
+    for the mirror group mmm): each representation's rows are assembled on
+    the device, and every grouped solve takes as many energies of all the
+    representations as fit in one launch of up to 32 systems, right-hand
+    sides padded with zero columns to the largest count.  sigma(E) is
+    returned per (energy, representation)."""
+
+    MAX_SYSTEMS = 32
+
+    def __init__(self, clusters, precision: int = FP32, E_comp: bool = True, basereel: bool = False):
+        self.cls = list(clusters)
+        if not self.cls or len(self.cls) > self.MAX_SYSTEMS:
+            raise ValueError(f"1..{self.MAX_SYSTEMS} representations")
+        self.E_comp, self.basereel = bool(E_comp), bool(basereel)
+        self.cal = self.E_comp or not self.basereel
+        self.per = self.MAX_SYSTEMS // len(self.cls)  # energies per grouped solve
+        self.nrhs = max(c.nlmso for c in self.cls)
+        self.asm = [DeviceAssembler(c, max_points=min(self.per, 16)) for c in self.cls]
+        self.sv = DeviceSolver(precision, self.cal, self.nrhs)
+        dt = torch.complex128 if self.cal else torch.float64
+        self.slots, self.vals, self.rhs, self.rhs_pad, self.x = [], [], [], [], []
+        for c in self.cls:
+            lb1, lb2 = c.profile()
+            rp, col, _, _ = c.assemble(np.zeros(c.npoint), 0.0, 0.0, 1.0, self.E_comp, self.basereel)
+            self.slots.append([self.sv.add_system(lb1, lb2, rp, col) for _ in range(self.per)])
+            self.vals.append([torch.zeros(c.nnz, dtype=dt, device="cuda") for _ in range(self.per)])
+            self.rhs.append([torch.zeros((c.nligne, c.nlmso), dtype=dt, device="cuda") for _ in range(self.per)])
+            self.rhs_pad.append([torch.zeros((c.nligne, self.nrhs), dtype=dt, device="cuda") for _ in range(self.per)])
+            self.x.append([torch.zeros((c.nligne, self.nrhs), dtype=dt, device="cuda") for _ in range(self.per)])
+        self._filled = 0
+
+    def run(self, V, energies, keep_solutions: bool = False) -> dict:
+        """V: one potential per representation (its ``points``, Ry);
+        energies: [(Enervide, Eimag, Ecinetic_out), ...] in Ry.  Returns
+        {"sigma": (nE, nrep), "sweeps": (nE, nrep), and "x": per energy a
+        list of per-representation solutions (nligne, nlmso) when
+        keep_solutions}."""
+        if len(V) != len(self.cls):
+            raise ValueError("one potential per representation")
+        Vd = [torch.as_tensor(np.ascontiguousarray(v, np.float64)).cuda() for v in V]
+        nE, nr = len(energies), len(self.cls)
+        sigma = np.zeros((nE, nr))
+        sweeps = np.zeros((nE, nr), np.int64)
+        xs = [[None] * nr for _ in range(nE)] if keep_solutions else None
+        for e0 in range(0, nE, self.per):
+            e1 = min(nE, e0 + self.per)
+            k = e1 - e0
+            upd = self._filled >= k
+            slots, vals, rhs, x = [], [], [], []
+            for r, c in enumerate(self.cls):
+                self.asm[r].run([Vd[r]] * k, energies[e0:e1], self.E_comp, self.basereel, self.vals[r][:k],
+                                self.rhs[r][:k], update_only=upd)
+                for j in range(k):
+                    self.rhs_pad[r][j][:, :c.nlmso].copy_(self.rhs[r][j])
+                slots += self.slots[r][:k]
+                vals += self.vals[r][:k]
+                rhs += self.rhs_pad[r][:k]
+                x += self.x[r][:k]
+            self._filled = max(self._filled, k)
+            _, sw, _ = self.sv.solve(slots, vals, rhs, x)
+            sg = self.sv.sigma(slots, rhs, x)
+            for r in range(nr):
+                for j in range(k):
+                    sigma[e0 + j, r] = sg[r * k + j]
+                    sweeps[e0 + j, r] = sw[r * k + j]
+                    if keep_solutions:
+                        xs[e0 + j][r] = self.x[r][j][:, :self.cls[r].nlmso].cpu().numpy()
+        out = {"sigma": sigma, "sweeps": sweeps}
         if keep_solutions:
             out["x"] = xs
         return out

@@ -188,3 +188,36 @@ def test_energy_loop_per_representation_matches_oracle():
                     np.abs(x[c.npoint:] - xf[nsof + hm][:, hm]).max()) / np.abs(xf).max()
             print("irrep", c.irrep, "loop rel", d, flush=True)
             assert d <= 1e-10
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_representation_loop_matches_oracle(precision):
+    """RepresentationLoop: all eight representations, five energies (one
+    solve of 4 x 8 = 32 systems, then a partial one), device assembly;
+    u / rvol against the oracle's whole-cluster solution, and sigma per
+    system against -Im sum b.x from the host assembler's RHS."""
+    from fdmnes_amd import cluster as CL
+    from fdmnes_amd.energy_loop import RepresentationLoop
+    R, h, lm = 2.0 / BOHR, 0.25 / BOHR, 3
+    reps = _reps(CL, R, h, lm)
+    ens = [(0.15 + 0.25 * k, 0.0367, 0.45 + 0.25 * k) for k in range(5)]
+    loop = RepresentationLoop(reps, precision=precision)
+    assert loop.per == 4
+    out = loop.run([_sym_potential(c.points) for c in reps], ens, keep_solutions=True)
+    o = O.Cluster(R, h, lm)
+    xyz = o.xyz[:o.npoint, :3]
+    row = {tuple(k): i for i, k in enumerate(np.round(xyz / h).astype(int))}
+    nsof = o.nligne - o.nlmso
+    for k, e in enumerate(ens):
+        xf = o.solve(_sym_potential(xyz), *e, True, True, False)[1]
+        for r, c in enumerate(reps):
+            x = out["x"][k][r]
+            hm = c.harmonics
+            rows = [row[tuple(q)] for q in np.round(c.points / h).astype(int)]
+            d = max(np.abs(x[:c.npoint] / c.rvol[:, None] - xf[rows][:, hm]).max(),
+                    np.abs(x[c.npoint:] - xf[nsof + hm][:, hm]).max()) / np.abs(xf).max()
+            b = c.assemble(_sym_potential(c.points), *e, True, False)[3]
+            sig = -np.imag(np.sum(b * x))
+            print("energy", k, "irrep", c.irrep, "rel", d, "sigma", out["sigma"][k, r], sig, flush=True)
+            assert d <= 1e-10
+            assert abs(out["sigma"][k, r] - sig) <= 1e-10 * max(abs(sig), np.abs(b).max() * np.abs(x).max())
