@@ -11,7 +11,7 @@ import ctypes as C
 import torch
 
 from . import _lib
-from .batch import Ordered, library_stream
+from .batch import Ordered, as_real, library_stream
 
 _V = C.c_void_p
 _stream = None
@@ -80,10 +80,12 @@ def tau_reduce(xs, nligneso: int, basereel: bool, fdm_comp: bool, atoms, tau=Non
     nlmsa = (C.c_int * max(len(atoms), 1))(*[int(k) for _, k in atoms])
     d = TauDesc(int(nligne), int(nlmso), int(nligneso), int(cal), int(bool(basereel)), int(bool(fdm_comp)),
                 len(atoms), ianew, nlmsa)
-    if _stream is None:
-        _stream = library_stream(stream)
-    s = library_stream(stream) if stream is not None else _stream
-    from .batch import as_real
+    if stream is not None:
+        s = library_stream(stream)
+    else:
+        if _stream is None:  # the module's own stream for calls that name none
+            _stream = library_stream(None)
+        s = _stream
     with Ordered(s):
         _lib.check(L.fdmgpu_tau_reduce(C.byref(d), len(xs), _dptrs([as_real(x) if cal else x for x in xs]),
                                        _dptrs([as_real(t) for t in tau]),
