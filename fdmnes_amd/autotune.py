@@ -20,6 +20,9 @@ CACHE = os.environ.get("FDMGPU_TUNE_CACHE", os.path.join(os.path.dirname(os.path
 # of a persistent grid striding over the blocks
 ONE_PER_BLOCK = 1 << 24
 CANDIDATES = [
+    # shallow groups for small systems, whose time is mostly the per-step chain
+    {"gk": 4, "trail_wg": ONE_PER_BLOCK, "solve_wg": 512, "trail_tile": 64},
+    {"gk": 6, "trail_wg": ONE_PER_BLOCK, "solve_wg": 512, "trail_tile": 64},
     {"gk": 8, "trail_wg": 1024, "solve_wg": 512, "trail_tile": 64},
     {"gk": 12, "trail_wg": 768, "solve_wg": 512, "trail_tile": 64},
     {"gk": 8, "trail_wg": ONE_PER_BLOCK, "solve_wg": 512, "trail_tile": 64},

@@ -30,15 +30,15 @@ class EnergyLoop:
 
     def __init__(self, cluster, precision: int = FP32, group: int = 12, E_comp: bool = True,
                  basereel: bool = False, atoms=None, fdm_comp: bool = False):
-        if not 1 <= group <= 16:
-            raise ValueError("group: 1..16 energy points per grouped solve")
+        if not 1 <= group <= 32:
+            raise ValueError("group: 1..32 energy points per grouped solve")
         self.cl = cluster
         self.G = int(group)
         self.E_comp, self.basereel, self.fdm_comp = bool(E_comp), bool(basereel), bool(fdm_comp)
         self.cal = self.E_comp or not self.basereel
         self.atoms = list(atoms or [])
         self.nso = cluster.nligne - cluster.nlmso
-        self.asm = DeviceAssembler(cluster, max_points=self.G)
+        self.asm = DeviceAssembler(cluster, max_points=min(self.G, 16))
         self.sv = DeviceSolver(precision, self.cal, cluster.nlmso)
         lb1, lb2 = cluster.profile()
         rp, col, _, _ = cluster.assemble(np.zeros(cluster.npoint), 0.0, 0.0, 1.0, self.E_comp, self.basereel)
