@@ -39,6 +39,7 @@ class EnergyLoop:
         self.atoms = list(atoms or [])
         self.nso = cluster.nligne - cluster.nlmso
         self.asm = DeviceAssembler(cluster, max_points=min(self.G, 16))
+        self.precision = precision
         self.sv = DeviceSolver(precision, self.cal, cluster.nlmso)
         lb1, lb2 = cluster.profile()
         rp, col, _, _ = cluster.assemble(np.zeros(cluster.npoint), 0.0, 0.0, 1.0, self.E_comp, self.basereel)
@@ -50,12 +51,17 @@ class EnergyLoop:
         self.x = [torch.zeros((n, nl), dtype=dt, device="cuda") for _ in range(self.G)]
         self._filled = 0  # buffers holding an earlier group's assembly (update_only is valid for them)
 
-    def run(self, V, energies, keep_solutions: bool = False) -> dict:
+    def run(self, V, energies, keep_solutions: bool = False, gradient: bool = False) -> dict:
         """V: one potential (npoint, Ry) for every energy, or one per energy;
         energies: [(Enervide, Eimag, Ecinetic_out), ...] in Ry.  Returns
         {"sigma": (nE,), "tau": per energy a list of per-atom (nrows, nrows)
-        complex arrays, "sweeps": refinement sweeps per energy, and
-        "x": the solutions (host arrays) when keep_solutions}."""
+        complex arrays, "sweeps": refinement sweeps per energy, "x": the
+        solutions (host arrays) when keep_solutions, and "dsigma_dV": (nE,
+        npoint) when gradient: d sigma(E) / d V_i by the adjoint (the
+        transposed substitutions with each group's factors, FP64 complex
+        loops only)."""
+        if gradient and (self.precision != FP64 or not self.cal):
+            raise ValueError("the gradient needs an FP64 loop in complex arithmetic")
         nE = len(energies)
         per_energy = isinstance(V, (list, tuple))
         if per_energy and len(V) != nE:
@@ -64,6 +70,7 @@ class EnergyLoop:
             else torch.as_tensor(np.ascontiguousarray(V, np.float64)).cuda()
         sigma = np.zeros(nE)
         tau, sweeps, xs = [], [], []
+        grad = np.zeros((nE, self.cl.npoint)) if gradient else None
         for e0 in range(0, nE, self.G):
             e1 = min(nE, e0 + self.G)
             k = e1 - e0
@@ -77,6 +84,11 @@ class EnergyLoop:
             _, sw, _ = self.sv.solve(self.slots[:k], self.vals[:k], self.rhs[:k], self.x[:k])
             sweeps += list(sw)
             sigma[e0:e1] = self.sv.sigma(self.slots[:k], self.rhs[:k], self.x[:k])
+            if gradient:  # the rows' values depend on V only through the diagonals
+                g = [torch.zeros(self.cl.nligne, dtype=torch.float64, device="cuda") for _ in range(k)]
+                self.sv.adjoint(self.slots[:k], self.rhs[:k], self.x[:k], g)
+                for j in range(k):
+                    grad[e0 + j] = g[j][:self.cl.npoint].cpu().numpy()
             if self.atoms:
                 t, _, _ = tau_reduce(self.x[:k], self.nso, self.basereel, self.fdm_comp, self.atoms)
                 tau += [[b.cpu().numpy() for b in split(ti, self.atoms)] for ti in t]
@@ -85,6 +97,8 @@ class EnergyLoop:
         out = {"sigma": sigma, "tau": tau, "sweeps": sweeps}
         if keep_solutions:
             out["x"] = xs
+        if gradient:
+            out["dsigma_dV"] = grad
         return out
 
 

@@ -68,3 +68,35 @@ def test_energy_loop_potential_per_energy_and_reuse():
     for t1, t2 in zip(b["tau"], fresh["tau"]):
         assert _rel(t1[0], t2[0]) <= 1e-13
     assert np.allclose(a["sigma"], b["sigma"][::-1], rtol=1e-12, atol=0)
+
+
+def test_energy_loop_gradient():
+    """dsigma/dV from the FP64 loop's adjoint against central differences of
+    sigma through the same loop (grid points near the centre, the boundary
+    and in between) and against a dense CPU adjoint on the oracle's rows."""
+    from fdmnes_amd import cluster as CL
+    from fdmnes_amd.energy_loop import FP64, EnergyLoop
+    c = CL.Cluster(1.6 / BOHR, 0.25 / BOHR, 2)
+    o = O.Cluster(1.6 / BOHR, 0.25 / BOHR, 2)
+    rng = np.random.default_rng(41)
+    ens = [(0.1 + 0.3 * k, 0.0367, 0.4 + 0.3 * k) for k in range(3)]
+    V = rng.uniform(-1.5, 0.0, c.npoint)
+    loop = EnergyLoop(c, precision=FP64, group=2)
+    out = loop.run(V, ens, gradient=True)
+    g = out["dsigma_dV"]
+    assert g.shape == (3, c.npoint)
+    h = 1e-4
+    for i in (0, c.npoint // 2, c.npoint - 1):
+        vp, vm = V.copy(), V.copy()
+        vp[i] += h
+        vm[i] -= h
+        fd = (loop.run(vp, ens)["sigma"] - loop.run(vm, ens)["sigma"]) / (2 * h)
+        print("point", i, "adjoint", g[:, i], "central differences", fd, flush=True)
+        assert np.abs(g[:, i] - fd).max() <= 1e-6 * np.abs(g).max()
+    # dense CPU adjoint of the oracle's rows: A^T lam = b, dsigma/dV_i = Im sum_L lam_iL x_iL
+    for k, e in enumerate(ens):
+        A, b = o.assemble_dense(V, *e, True, True, False)
+        x = np.linalg.solve(A, b)
+        lam = np.linalg.solve(A.T, b)
+        ref = np.imag(np.sum(lam * x, axis=1))[:c.npoint]
+        assert np.abs(g[k] - ref).max() <= 1e-9 * np.abs(ref).max()
