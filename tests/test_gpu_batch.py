@@ -47,6 +47,20 @@ def test_grouped_uneven_clusters_fp64():
         assert np.abs(x - xo).max() / np.abs(xo).max() <= 1e-10
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_grouped_launch_of_24_uneven_systems(precision):
+    """24 systems of six cluster sizes (R = 1.0 to 2.0 A) and four energies in
+    one grouped solve, above the 16 of a half-full launch: every solution is
+    the oracle's, and a replay of the captured graphs reproduces it."""
+    radii = [1.0, 1.2, 1.4, 1.6, 1.8, 2.0]
+    systems = [small_system(R=R, seed=10 + k, E=0.3 + 0.35 * (k % 4)) for k, R in enumerate(radii * 4)]
+    print("24 systems, sizes", sorted({len(s["lb1"]) for s in systems}), flush=True)
+    _, _, xs, sw, _, _, _ = _run(precision, systems, repeats=2)
+    for s, x in zip(systems, xs):
+        xo = oracle_solution(s)
+        assert np.abs(x - xo).max() / np.abs(xo).max() <= 1e-10
+
+
 @pytest.mark.parametrize("lookahead", [0, 1, 96])
 @pytest.mark.parametrize("precision", [0, 1])
 @pytest.mark.parametrize("trail_tile,gk,trail_wg", [(64, 0, 64), (64, 3, 64), (128, 3, 64), (128, 8, 64),
