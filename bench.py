@@ -331,6 +331,9 @@ def main():
     ap.add_argument("--no-roofline", action="store_true", help="skip the per-launch timed pass (profiler runs)")
     ap.add_argument("--no-assembly", action="store_true", help="skip the per-energy assembly timing")
     ap.add_argument("--no-tau", action="store_true", help="skip the post-solve tau reduction timing")
+    ap.add_argument("--no-graphs", action="store_true", help="launch eagerly instead of replaying HIP graphs")
+    ap.add_argument("--overlap", type=int, default=None, choices=(0, 1),
+                    help="per-step overlap of the trailing update (fdmgpu_solver_overlap); default: the library's")
     ap.add_argument("--mmm", action="store_true",
                     help="also time the mirror-symmetric workload: whole cluster against its eight representations")
     ap.add_argument("--dropin", action="store_true",
@@ -377,6 +380,10 @@ def main():
             ent = autotune.tune(sv, slots, [vals[i] for i in range(G)], [src] * G, xs, lb1, lb2, precision,
                                 log=log)
             tuned[tag] = ent["best"]
+        if args.overlap is not None:
+            sv.overlap(args.overlap)
+        if args.no_graphs:
+            sv.options(use_graphs=False)
         sig = np.zeros(n_pts)
         sweeps = []
         stream = torch.cuda.current_stream()

@@ -33,6 +33,7 @@ def _sigs(L):
     L.fdmgpu_solver_options.argtypes = [_V, C.c_int, C.c_int, C.c_int]
     L.fdmgpu_solver_stats.argtypes = [_V, P(C.c_double)]
     L.fdmgpu_solver_tune.argtypes = [_V, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+    L.fdmgpu_solver_overlap.argtypes = [_V, C.c_int]
     L._solver_sigs = True
 
 
@@ -107,6 +108,11 @@ class DeviceSolver:
 
     def options(self, use_graphs=True, time_updates=False, persist_wg=0):
         _lib.check(self.L.fdmgpu_solver_options(self._h, int(use_graphs), int(time_updates), int(persist_wg)))
+
+    def overlap(self, on: bool):
+        """The left-looking update of the next block row and column on a
+        second stream beside the current step's panel (fdmgpu_solver_overlap)."""
+        _lib.check(self.L.fdmgpu_solver_overlap(self._h, int(bool(on))))
 
     def tune(self, gk=-1, trail_wg=0, persist_solve=-1, solve_wg=0, trail_tile=0, lookahead=-1):
         _lib.check(self.L.fdmgpu_solver_tune(self._h, int(gk), int(trail_wg), int(persist_solve), int(solve_wg),

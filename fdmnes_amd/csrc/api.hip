@@ -378,6 +378,25 @@ int fdmgpu_solver_options(fdmgpu_solver *s, int use_graphs, int time_updates, in
   return 0;
 }
 
+int fdmgpu_solver_overlap(fdmgpu_solver *s, int on) {
+  if (!s || on < 0 || on > 1) {
+    set_error("fdmgpu_solver_overlap: invalid argument");
+    return FDMGPU_EINVAL;
+  }
+  auto apply = [&](auto *p) {
+    p->grp.overlap = on;
+    p->drop_graphs();
+  };
+  if (s->prec == FDMGPU_FP64) {
+    if (s->cal) apply((Solver<double, true> *)s->impl);
+    else apply((Solver<double, false> *)s->impl);
+  } else {
+    if (s->cal) apply((Solver<float, true> *)s->impl);
+    else apply((Solver<float, false> *)s->impl);
+  }
+  return 0;
+}
+
 int fdmgpu_solver_tune(fdmgpu_solver *s, int gk, int trail_wg, int persist_solve, int solve_wg, int trail_tile,
                        int lookahead) {
   if (!s || gk > MAXGK || (trail_tile > 0 && trail_tile != 64 && trail_tile != 128)) {

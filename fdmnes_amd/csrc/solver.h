@@ -84,8 +84,13 @@ struct Group {
   // 0 off; 1 on, the rest on the trailing update's grid; > 1 on, the rest on
   // a persistent grid of that many workgroups (slots left for the chain)
   int lookahead = 0;
+  // Overlap inside a group: the left-looking update of block row and column
+  // K-1 from steps K+1..Khi runs on `side` beside step K's panel and
+  // products; `stream` adds step K's own contribution afterwards
+  int overlap = 0;
   hipStream_t side = nullptr;
   hipEvent_t ev_fact = nullptr, ev_rest = nullptr;  // stream -> side, side -> stream
+  hipEvent_t ev_step = nullptr, ev_left = nullptr;  // the same for the overlap
   int maxparts = 1;
   // per-kernel timing (optional): cumulative ms of the G_UPDATE launches
   bool time_updates = false;

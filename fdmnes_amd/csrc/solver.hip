@@ -169,6 +169,8 @@ int Group<R, C>::finalize(hipStream_t s) {
   if (!side) FDM_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
   if (!ev_fact) FDM_HIP(hipEventCreateWithFlags(&ev_fact, hipEventDisableTiming));
   if (!ev_rest) FDM_HIP(hipEventCreateWithFlags(&ev_rest, hipEventDisableTiming));
+  if (!ev_step) FDM_HIP(hipEventCreateWithFlags(&ev_step, hipEventDisableTiming));
+  if (!ev_left) FDM_HIP(hipEventCreateWithFlags(&ev_left, hipEventDisableTiming));
   FDM_HIP(hipMalloc(&d_sys, sizeof(SysDev<R>) * h.size()));
   FDM_HIP(hipMemcpy(d_sys, h.data(), sizeof(SysDev<R>) * h.size(), hipMemcpyHostToDevice));
   return 0;
@@ -196,6 +198,9 @@ void Group<R, C>::release() {
   ev.clear();
   if (ev_fact) (void)hipEventDestroy(ev_fact);
   if (ev_rest) (void)hipEventDestroy(ev_rest);
+  if (ev_step) (void)hipEventDestroy(ev_step);
+  if (ev_left) (void)hipEventDestroy(ev_left);
+  ev_step = ev_left = nullptr;
   if (side) (void)hipStreamDestroy(side);
   ev_fact = ev_rest = nullptr;
   side = nullptr;
@@ -315,9 +320,10 @@ int Group<R, C>::factor() {
   };
   const int rest_wg = lookahead > 1 ? lookahead : persist_wg;
   bool rest_pending = false;  // a remainder on `side` not yet waited for by `stream`
+  bool left_pending = false;  // an overlapped left-looking update on `side` not yet joined
   for (int s = 0; s < maxNB; s++) {
     Items ip{}, itx{}, itu{};
-    TrailItems cross{}, trail{}, rest{};
+    TrailItems cross{}, trail{}, rest{}, left_next{};
     for (int g = 0; g < (int)m.size(); g++) {
       if (!active[g]) continue;
       const Plan &pl = *m[g].plan;
@@ -333,10 +339,21 @@ int Group<R, C>::factor() {
       }
       const int Khi = pl.NB - 1 - gk * (t / gk), Klo = std::max(Khi - gk + 1, 0);
       if (K < Khi) {
-        // block row K, columns J0(K)..K (diagonal tile included), steps K+1..Khi
-        push_trail(cross, g, K + 1, Khi - K, K, 1, pl.J0[K], nJ + 1);
+        // block row K, columns J0(K)..K (diagonal tile included), steps
+        // K+1..Khi; with the overlap only step K+1 here, the others having
+        // run on `side` beside step K+1's panel
+        const int nk = overlap ? 1 : Khi - K;
+        push_trail(cross, g, K + 1, nk, K, 1, pl.J0[K], nJ + 1);
         // block column K, rows I0(K)..K-1
-        push_trail(cross, g, K + 1, Khi - K, pl.I0[K], nI, K, 1);
+        push_trail(cross, g, K + 1, nk, pl.I0[K], nI, K, 1);
+      }
+      if (overlap && K < Khi && K > Klo) {
+        // block row and column K-1 from steps K+1..Khi: they read only those
+        // steps' finished products and write tiles that step K's panel,
+        // products and single-step update neither read nor write
+        const int K2 = K - 1, nJ2 = K2 - pl.J0[K2], nI2 = K2 - pl.I0[K2];
+        push_trail(left_next, g, K + 1, Khi - K, K2, 1, pl.J0[K2], nJ2 + 1);
+        push_trail(left_next, g, K + 1, Khi - K, pl.I0[K2], nI2, K2, 1);
       }
       if (K == Klo) {
         const int nk = Khi - Klo + 1, i0 = pl.I0[Klo], j0 = pl.J0[Klo];
@@ -351,7 +368,18 @@ int Group<R, C>::factor() {
         }
       }
     }
+    if (left_next.count) FDM_HIP(hipEventRecord(ev_step, stream));  // step K+1 done: its products are final
+    if (left_pending) {  // the earlier steps' part of this step's update
+      FDM_HIP(hipStreamWaitEvent(stream, ev_left, 0));
+      left_pending = false;
+    }
     if (cross.count) timed(stream, [&]() { launch_trail(stream, cross, persist_wg); });
+    if (left_next.count) {
+      FDM_HIP(hipStreamWaitEvent(side, ev_step, 0));
+      timed(side, [&]() { launch_trail(side, left_next, persist_wg); });
+      FDM_HIP(hipEventRecord(ev_left, side));
+      left_pending = true;
+    }
     hipLaunchKernelGGL((k_panel<R, C>), dim3(ip.count), dim3(PNT), 0, stream, (const SysDev<R> *)d_sys, ip);
     if (itx.count)
       hipLaunchKernelGGL((k_tile_gemm<R, C>), dim3(std::min(itx.off[itx.count], persist_wg)), dim3(NT), 0, stream,
@@ -392,6 +420,7 @@ int Group<R, C>::factor() {
     }
   }
   if (rest_pending) FDM_HIP(hipStreamWaitEvent(stream, ev_rest, 0));  // join before the substitutions
+  if (left_pending) FDM_HIP(hipStreamWaitEvent(stream, ev_left, 0));
   // the substitutions' neighbour products MU(K), ML(K) of every block, off
   // the per-step chain: their tiles are final once steps K and K+1 are done
   Items nb{};

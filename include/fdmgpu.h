@@ -145,6 +145,13 @@ int fdmgpu_solver_options(fdmgpu_solver *s, int use_graphs, int time_updates, in
  * n > 1: as 1, that update on a persistent grid of n workgroups);
  * <= 0 keeps a value (< 0 for lookahead).  Takes effect at the next solve
  * (cached graphs are rebuilt). */
+/* Overlap inside a trailing-update group (1 on, 0 off): the left-looking
+ * update of block row and column K-1 from the group's steps K+1.. runs on a
+ * second stream beside step K's panel and L-row/U-column products, and only
+ * step K's own contribution follows them on the solver's stream.  No
+ * counterpart in the reference (it has no such schedule).  Takes effect at
+ * the next solve. */
+int fdmgpu_solver_overlap(fdmgpu_solver *s, int on);
 int fdmgpu_solver_tune(fdmgpu_solver *s, int gk, int trail_wg, int persist_solve, int solve_wg, int trail_tile,
                        int lookahead);
 /* out[0] ms spent in trailing-update launches (when timed), out[1] their

@@ -13,7 +13,7 @@ def _dev(a):
     return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
 
 
-def _run(precision, systems, repeats=1, max_sweeps=5, tol=1e-12, tune=None):
+def _run(precision, systems, repeats=1, max_sweeps=5, tol=1e-12, tune=None, overlap=None):
     import torch
     from fdmnes_amd.batch import DeviceSolver
     nrhs = systems[0]["rhs"].shape[1]
@@ -22,6 +22,8 @@ def _run(precision, systems, repeats=1, max_sweeps=5, tol=1e-12, tune=None):
     sv = DeviceSolver(precision, cal, nrhs)
     if tune:
         sv.tune(**tune)
+    if overlap is not None:
+        sv.overlap(overlap)
     slots = [sv.add_system(s["lb1"], s["lb2"], s["rp"], s["col"]) for s in systems]
     vals = [_dev(s["vals"].astype(dt)) for s in systems]
     b = [_dev(s["rhs"].astype(dt)) for s in systems]
@@ -80,6 +82,28 @@ def test_schedules_match_oracle(precision, trail_tile, gk, trail_wg, lookahead):
     _, _, xs, sw, _, _, _ = _run(precision, systems, repeats=2,
                                   tune=dict(gk=gk, trail_wg=trail_wg, trail_tile=trail_tile, persist_solve=1,
                                             lookahead=lookahead))
+    for s, x in zip(systems, xs):
+        xo = oracle_solution(s)
+        assert np.abs(x - xo).max() / np.abs(xo).max() <= 1e-10
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("trail_tile,gk,trail_wg,lookahead", [(64, 3, 64, 0), (64, 8, 1 << 24, 0), (128, 8, 64, 0),
+                                                              (64, 20, 1 << 24, 0), (64, 20, 1 << 24, 1),
+                                                              (64, 32, 1 << 24, 96)])
+def test_overlap_schedules_match_oracle(precision, trail_tile, gk, trail_wg, lookahead):
+    """The per-step overlap (the next block row and column's left-looking
+    update on the second stream beside the current panel), alone and with
+    the group-level lookahead on the same stream, over groups of 3 to 32
+    steps and clusters of uneven size: the oracle's solution, and graph
+    replays that reproduce it."""
+    systems = [small_system(R=1.2, seed=11, E=0.4), small_system(R=1.9, seed=12, E=1.1),
+               small_system(R=2.2, seed=13, E=2.3)]
+    print("overlap: precision", precision, "tile", trail_tile, "gk", gk, "wg", trail_wg, "lookahead", lookahead,
+          flush=True)
+    _, _, xs, _, _, _, _ = _run(precision, systems, repeats=3, overlap=True,
+                                tune=dict(gk=gk, trail_wg=trail_wg, trail_tile=trail_tile, persist_solve=1,
+                                          lookahead=lookahead))
     for s, x in zip(systems, xs):
         xo = oracle_solution(s)
         assert np.abs(x - xo).max() / np.abs(xo).max() <= 1e-10
