@@ -9,7 +9,9 @@ stream), run in numpy on random skyline matrices with 16 x 16 tiles:
   schedule forms lands in a stored tile;
 - the overlap gives the same factors, and the tiles the second-stream launch
   reads and writes are disjoint from those the main stream writes and reads
-  while it runs (it is joined before the next left-looking launch)."""
+  while it runs (it is joined before the next left-looking launch);
+- so does the group-level lookahead, whose remainder of each group's
+  rectangle runs on the second stream beside the next group's chain."""
 import numpy as np
 import pytest
 
@@ -34,16 +36,16 @@ def _plan(lb1, lb2):
     return NB, J0, J1, I0
 
 
-def _matrix(n, seed):
+def _matrix(n, seed, low_min=2, low_max=45, up_max=70, diag=30.0):
     rng = np.random.default_rng(seed)
-    low = rng.integers(2, 45, n)
+    low = rng.integers(low_min, low_max, n)
     lb1 = np.maximum.accumulate(np.maximum(1, np.arange(1, n + 1) - low))
-    lb2 = np.minimum(n, np.arange(1, n + 1) + rng.integers(0, 70, n))
+    lb2 = np.minimum(n, np.arange(1, n + 1) + rng.integers(0, up_max, n))
     A = np.zeros((n, n), complex)
     for i in range(n):
         k = np.arange(lb1[i] - 1, lb2[i])
         A[i, k] = rng.normal(size=len(k)) + 1j * rng.normal(size=len(k))
-        A[i, i] += 30.0  # no pivoting: keep every pivot away from zero
+        A[i, i] += diag  # no pivoting: keep every pivot away from zero
     return A, lb1, lb2
 
 
@@ -103,16 +105,34 @@ class Model:
                 w.add((I, J))
         return r, w
 
-    def factor(self, gk, overlap=False, side_first=False):
+    def factor(self, gk, overlap=False, side_first=False, lookahead=False):
         """factor()'s sequence; with the overlap, the second-stream launch of
         each step is checked against the main stream's launches it runs
-        beside and applied before or after them (side_first)."""
+        beside and applied before or after them (side_first).  With the
+        lookahead a group's rectangle is split: the next group's block rows
+        and columns on the main stream, the rest on the second stream from
+        then until the main stream waits for it before the next group's
+        rectangle (checked against every main-stream launch of that window,
+        applied at its start or at its end).  Returns the number of tiles the
+        remainders updated."""
         NB, J0, I0 = self.NB, self.J0, self.I0
+        pending = None  # remainder on the second stream: [launch, reads, writes, result or None]
+        rest_tiles = 0
+
+        def join():
+            nonlocal pending
+            launch, R, W, res = pending
+            r, w = res if res is not None else launch()
+            assert not (w & (R | W)), "the remainder writes a tile the main stream uses"
+            assert not (r & W), "the remainder reads a tile the main stream writes"
+            pending = None
+
         for t in range(NB):
             K = NB - 1 - t
             nJ, nI = K - J0[K], K - I0[K]
-            stream = []  # launches after the fork point, in order
+            stream = []  # this step's main-stream launches, in order
             side = None
+            new_rest = None
             if gk <= 0:
                 stream += [lambda K=K: self.panel(K), lambda K=K: self.products(K)]
                 if nI > 0 and nJ > 0:
@@ -134,20 +154,42 @@ class Model:
                         return r1 | r2, w1 | w2
                 stream += [lambda K=K: self.panel(K), lambda K=K: self.products(K)]
                 if K == Klo and Klo > 0:
-                    nk = Khi - Klo + 1
-                    stream.append(lambda: self.update(Klo, nk, I0[Klo], Klo - I0[Klo], J0[Klo], Klo - J0[Klo]))
+                    nk, i0, j0 = Khi - Klo + 1, I0[Klo], J0[Klo]
+                    stream.append("join")  # the main stream waits for the earlier remainder
+                    if lookahead:
+                        Klo2 = max(Klo - gk, 0)  # the next group's steps are Klo2..Klo-1
+                        a, b = max(i0, Klo2), max(j0, Klo2)
+                        stream.append(lambda: self.update(Klo, nk, a, Klo - a, j0, Klo - j0))
+                        stream.append(lambda: self.update(Klo, nk, i0, a - i0, b, Klo - b))
+                        if a > i0 and b > j0:
+                            new_rest = lambda args=(Klo, nk, i0, a - i0, j0, b - j0): self.update(*args)
+                            rest_tiles += (a - i0) * (b - j0)
+                    else:
+                        stream.append(lambda: self.update(Klo, nk, i0, Klo - i0, j0, Klo - j0))
             if side is not None and side_first:
                 rs, ws = side()
             R, W = set(), set()
             for f in stream:
+                if isinstance(f, str):
+                    if pending is not None:
+                        join()
+                    continue
                 r, w = f()
                 R |= r
                 W |= w
+                if pending is not None:
+                    pending[1] |= r
+                    pending[2] |= w
             if side is not None:
                 if not side_first:
                     rs, ws = side()
                 assert not (ws & (R | W)), "the second stream writes a tile the main stream uses"
                 assert not (rs & W), "the second stream reads a tile the main stream writes"
+            if new_rest is not None:  # started once this step's main-stream launches are queued
+                pending = [new_rest, set(), set(), new_rest() if side_first else None]
+        if pending is not None:
+            join()
+        return rest_tiles
 
     def check(self):
         N = self.NB * TB
@@ -185,4 +227,21 @@ def test_overlap_is_race_free_and_exact(gk, seed):
         m.factor(gk, overlap=True, side_first=side_first)
         assert m.check() <= 1e-12
         for k in ref.T:  # the same factors up to the order of the accumulated terms
+            assert np.abs(m.T[k] - ref.T[k]).max() <= 1e-12 * max(1.0, np.abs(ref.T[k]).max())
+
+
+@pytest.mark.parametrize("gk", [3, 5])
+@pytest.mark.parametrize("overlap", [False, True])
+def test_lookahead_remainder_is_race_free(gk, overlap):
+    """Wide profiles (bands of up to 20 tiles), so that groups leave a
+    remainder beside the next group's chain: the same factors, and the
+    remainder never touches a tile the main stream uses meanwhile."""
+    A, lb1, lb2 = _matrix(560, 5, low_min=240, low_max=320, up_max=320, diag=400.0)
+    ref = Model(A, lb1, lb2)
+    ref.factor(gk)
+    for side_first in (False, True):
+        m = Model(A, lb1, lb2)
+        assert m.factor(gk, overlap=overlap, side_first=side_first, lookahead=True) > 0
+        assert m.check() <= 1e-12
+        for k in ref.T:
             assert np.abs(m.T[k] - ref.T[k]).max() <= 1e-12 * max(1.0, np.abs(ref.T[k]).max())
