@@ -101,7 +101,8 @@ int fdmgpu_mat_solve_mixed(int nligne, int nlmso, int cal_comp, const int *lb1, 
  *   d_b[k], d_x[k]: (nligne, nrhs) row-major, interleaved when complex
  *              (the smr/smi layout of mat_solve with re/im interleaved).
  * No counterpart in the reference (it solves one point at a time on the
- * CPU); callers: the energy-loop driver and the batched benchmark.
+ * CPU); callers: the energy loop (fdm.f90's loop over energy points,
+ * taken a group of points at a time) and the batched benchmark.
  */
 typedef struct fdmgpu_solver fdmgpu_solver;
 
