@@ -60,3 +60,32 @@ def test_python_signatures_cover_header():
     from fdmnes_amd import _lib
     for n in ["fdmgpu_init", "fdmgpu_mat_solve", "fdmgpu_mat_solve_mixed", "fdmgpu_last_error"]:
         assert n in _lib.SIGNATURES
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libfdmgpu.so not built")
+def test_entry_points_fail_loudly_without_gpu():
+    """Without a device the library reports an error status with a message
+    (no crash), and the Python drop-in raises instead of computing anything
+    on the host."""
+    import numpy as np
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present: the device path is tested by the GPU tests")
+    lib = ctypes.CDLL(LIB)
+    lib.fdmgpu_last_error.restype = ctypes.c_char_p
+    # argument checks come before any device call
+    rc = lib.fdmgpu_mat_solve(0, 1, 1, None, None, None, None, None, None, None, None, None, None, None)
+    assert rc == -1  # FDMGPU_EINVAL
+    assert b"invalid argument" in lib.fdmgpu_last_error()
+    rc = lib.fdmgpu_init(0)
+    assert rc < 0 and lib.fdmgpu_last_error()
+    from fdmnes_amd import _lib, solver
+    n, nrhs = 3, 1
+    lb1 = np.array([1, 1, 2], np.int32)
+    lb2 = np.array([2, 3, 3], np.int32)
+    row_ptr = np.array([0, 2, 5, 7], np.int64)
+    col = np.array([0, 1, 0, 1, 2, 1, 2], np.int32)
+    vals = np.array([4, 1, 1, 4, 1, 1, 4], np.complex128)
+    rhs = np.ones((n, nrhs), np.complex128)
+    with pytest.raises(_lib.FdmGpuError, match="device"):
+        solver.mat_solve(lb1, lb2, row_ptr, col, vals, rhs, True)
