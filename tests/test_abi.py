@@ -89,3 +89,32 @@ def test_entry_points_fail_loudly_without_gpu():
     rhs = np.ones((n, nrhs), np.complex128)
     with pytest.raises(_lib.FdmGpuError, match="device"):
         solver.mat_solve(lb1, lb2, row_ptr, col, vals, rhs, True)
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libfdmgpu.so not built")
+def test_tau_reduce_argument_checks_without_gpu():
+    """fdmgpu_tau_reduce rejects, before any device call, an nlmso outside
+    1..256, atom rows outside the system and blocks whose normalisation,
+    amplitudes and pivots would not fit one workgroup's 160 KiB of LDS."""
+    from fdmnes_amd.tau import TauDesc
+    lib = ctypes.CDLL(LIB)
+    lib.fdmgpu_last_error.restype = ctypes.c_char_p
+    dummy = (ctypes.c_void_p * 1)(1)  # never dereferenced: every case fails its checks first
+
+    def call(nligne, nlmso, atoms, count=1):
+        ianew = (ctypes.c_int * max(len(atoms), 1))(*[a for a, _ in atoms])
+        nlmsa = (ctypes.c_int * max(len(atoms), 1))(*[k for _, k in atoms])
+        d = TauDesc(nligne, nlmso, nligne - nlmso, 1, 1, 0, len(atoms), ianew, nlmsa)
+        rc = lib.fdmgpu_tau_reduce(ctypes.byref(d), count, dummy, dummy, None, 0, None, None)
+        return rc, lib.fdmgpu_last_error().decode()
+
+    rc, msg = call(1000, 300, [(0, 9)])
+    assert rc == -1 and "nlmso <= 256" in msg
+    rc, msg = call(1000, 16, [(995, 9)])
+    assert rc == -1 and "rows of atom 0" in msg
+    # 128^2 + 128 * (128 + 1) complex entries: 526 KiB of LDS
+    rc, msg = call(4000, 128, [(0, 128)])
+    assert rc == -1 and "160 KiB" in msg
+    # a block that fits (nlmso 64, atoms of 64 rows: 129 KiB); no points, so no device call
+    rc, msg = call(4000, 64, [(0, 64)], count=0)
+    assert rc == 0
