@@ -20,6 +20,9 @@ def _L():
         L.fdmgpu_cluster_create.argtypes = [C.c_double, C.c_double, C.c_int, C.POINTER(C.c_void_p)]
         L.fdmgpu_cluster_create_mmm.restype = C.c_int
         L.fdmgpu_cluster_create_mmm.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
+        L.fdmgpu_cluster_create_group.restype = C.c_int
+        L.fdmgpu_cluster_create_group.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
+                                                  C.POINTER(C.c_void_p)]
         L.fdmgpu_cluster_rvol.argtypes = [C.c_void_p, _lib.dp]
         L.fdmgpu_cluster_assembly_weights.restype = C.c_int
         L.fdmgpu_cluster_assembly_weights.argtypes = [C.c_void_p, _lib.ip, _lib.dp, _lib.dp]
@@ -45,14 +48,21 @@ def _L():
 
 class Cluster:
     """A spherical cluster of radius ``rsort`` and grid step ``adimp``
-    (bohr) with ``(lmaxso+1)**2`` outer-sphere unknowns; with ``irrep``
-    (0..7) the system of one representation of the mirror group mmm on the
-    wedge x, y, z >= 0 (see fdmgpu_cluster_create_mmm)."""
+    (bohr) with ``(lmaxso+1)**2`` outer-sphere unknowns; with ``irrep`` the
+    system of one representation of a mirror group on its wedge (see
+    fdmgpu_cluster_create_group): ``group`` is the reference's point-group
+    number, 8 = mmm (irrep 0..7, wedge x, y, z >= 0, the default), 6 = mm2
+    (irrep 0..3, wedge x, y >= 0), 3 = m (irrep 0 or 4, wedge z >= 0);
+    irrep bit a set = odd under the mirror normal to axis a."""
 
-    def __init__(self, rsort: float, adimp: float, lmaxso: int, irrep: int = -1):
+    GROUPS = {1: 0, 3: 4, 6: 3, 8: 7}  # point-group number -> mirrors (bit a = normal to axis a)
+
+    def __init__(self, rsort: float, adimp: float, lmaxso: int, irrep: int = -1, group: int = 8):
         L = _L()
         h = C.c_void_p()
-        rc = L.fdmgpu_cluster_create_mmm(rsort, adimp, lmaxso, int(irrep), C.byref(h))
+        if irrep < 0:
+            group = 1
+        rc = L.fdmgpu_cluster_create_group(rsort, adimp, lmaxso, int(group), int(irrep), C.byref(h))
         if rc:
             raise _lib.FdmGpuError(f"fdmgpu_cluster_create failed ({rc})")
         self._h = h
@@ -61,6 +71,7 @@ class Cluster:
         self.npoint, self.npso, self.nligne, self.nlmso, self.nsortf, self.nsort = map(int, s)
         self.lmaxso = lmaxso
         self.irrep = int(irrep)
+        self.group = int(group)
         self.rsort, self.adimp = rsort, adimp
         self.nnz = int(L.fdmgpu_cluster_nnz(h))
 

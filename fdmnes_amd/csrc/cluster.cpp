@@ -87,16 +87,19 @@ void bessel_neumann(T fnorm, T z, int lmax, T *bes, T *neu) {
 struct fdmgpu_cluster {
   double rsort = 0, adimp = 0;
   int lmaxso = 0, nlmso = 0;
-  // -1: the whole cluster (point group P1).  0..7: one representation of the
-  // mirror group mmm (bit a set = odd under the mirror normal to axis a);
-  // the grid is then the wedge x, y, z >= 0
+  // -1: the whole cluster (point group P1).  Otherwise one representation of
+  // the group of the mirrors in `mirrors` (bit a = the mirror normal to axis
+  // a; 4 = m, 3 = mm2, 7 = mmm): bit a of irrep set = odd under that mirror
+  // (a subset of mirrors); the grid is then the wedge with coordinate a >= 0
+  // on every mirror axis
   int irrep = -1;
+  int mirrors = 0;
   int npoint = 0, npso = 0, nligne = 0;
   std::vector<GridPoint> pt;   // npso, radial order
   std::vector<double> rad;     // npso, |index| * adimp
   std::vector<int> nbr;        // npso * 12, 0-based point or -1
   std::vector<unsigned char> flip;  // npso * 12: mirrors taking the neighbour into the wedge (bit per axis)
-  std::vector<double> rvol;    // npso: points of the whole grid a wedge point stands for, / 8 (1 for P1)
+  std::vector<double> rvol;    // npso: points of the whole grid a wedge point stands for, over the group order (1 for P1)
   std::vector<double> wso;     // nsortf: outer-sphere quadrature weights 4 pi rvol / sum(rvol)
   std::vector<int> lmidx;      // nlmso: harmonic index l*l + l + m of each outer unknown
   std::vector<int> bnd;        // boundary points: inner (nsortf) then outer
@@ -128,6 +131,7 @@ inline double character(int g, int mask) { return (__builtin_popcount(g & mask) 
 
 int build(fdmgpu_cluster *c) {
   const bool sym = c->irrep >= 0;
+  const int mx = sym ? c->mirrors : 0;
   // Grid points inside rsort/adimp + sqrt(2)*iord/2 (reseau, general.f90:9476)
   // visited in (ix, iy, iz) lexicographic order, then ordered by radius with
   // the reference's exchange sort (:9528-9543): a later point is swapped in
@@ -140,9 +144,9 @@ int build(fdmgpu_cluster *c) {
       for (int iz = -nx; iz <= nx; iz++) {
         const double d = std::sqrt((double)ix * ix + (double)iy * iy + (double)iz * iz);
         if (d > rmax) continue;
-        // with mmm symmetry only the wedge (posequiv's region of mmm,
-        // general.f90:8892) is kept (reseau, general.f90:9512)
-        if (sym && (ix < 0 || iy < 0 || iz < 0)) continue;
+        // with mirror symmetry only the wedge (posequiv's regions of m, mm2
+        // and mmm, general.f90:8961-8978) is kept (reseau, general.f90:9512)
+        if (((mx & 1) && ix < 0) || ((mx & 2) && iy < 0) || ((mx & 4) && iz < 0)) continue;
         c->pt.push_back({ix, iy, iz});
         c->rad.push_back(c->adimp * d);
       }
@@ -168,7 +172,8 @@ int build(fdmgpu_cluster *c) {
   if (sym) {
     double rvmax = 1.0;
     for (int i = 0; i < n; i++) {
-      c->rvol[i] = (double)(1 << ((p[i].ix != 0) + (p[i].iy != 0) + (p[i].iz != 0)));
+      c->rvol[i] = (double)(1 << (((mx & 1) && p[i].ix != 0) + ((mx & 2) && p[i].iy != 0) +
+                                  ((mx & 4) && p[i].iz != 0)));
       rvmax = std::max(rvmax, c->rvol[i]);
     }
     for (int i = 0; i < n; i++) c->rvol[i] /= rvmax;
@@ -187,10 +192,11 @@ int build(fdmgpu_cluster *c) {
       int x = p[i].ix + kDir[v][0], y = p[i].iy + kDir[v][1], z = p[i].iz + kDir[v][2];
       if (!(std::abs(x) <= nx && std::abs(y) <= nx && std::abs(z) <= nx)) continue;
       if (sym) {
-        c->flip[(size_t)i * 12 + v] = (unsigned char)((x < 0) | ((y < 0) << 1) | ((z < 0) << 2));
-        x = std::abs(x);
-        y = std::abs(y);
-        z = std::abs(z);
+        const int f = ((x < 0) | ((y < 0) << 1) | ((z < 0) << 2)) & mx;
+        c->flip[(size_t)i * 12 + v] = (unsigned char)f;
+        if (f & 1) x = -x;
+        if (f & 2) y = -y;
+        if (f & 4) z = -z;
       }
       c->nbr[(size_t)i * 12 + v] = where[key(x, y, z)];
     }
@@ -233,7 +239,7 @@ int build(fdmgpu_cluster *c) {
   c->lmidx.clear();
   for (int l = 0; l <= c->lmaxso; l++)
     for (int m = -l; m <= l; m++)
-      if (!sym || harmonic_irrep(l, m) == c->irrep) c->lmidx.push_back(l * l + l + m);
+      if (!sym || (harmonic_irrep(l, m) & mx) == c->irrep) c->lmidx.push_back(l * l + l + m);
   c->nlmso = (int)c->lmidx.size();
   if (c->nlmso == 0) return -1;
   c->ylm.assign(c->bnd.size() * c->nlmso, 0.0);
@@ -318,14 +324,27 @@ inline double stencil(const fdmgpu_cluster *c, int i, int d) {
 
 extern "C" {
 
-int fdmgpu_cluster_create_mmm(double rsort, double adimp, int lmaxso, int irrep, fdmgpu_cluster **out) {
-  if (!out || !(rsort > 0) || !(adimp > 0) || lmaxso < 0 || lmaxso > 40 || irrep < -1 || irrep > 7) return -1;
+int fdmgpu_cluster_create_group(double rsort, double adimp, int lmaxso, int group, int irrep, fdmgpu_cluster **out) {
+  // the reference's point-group numbers (general.f90:5207-5220) of the
+  // mirror groups, in posequiv's orientation: m's mirror is normal to z and
+  // mm2's two mirrors are normal to x and y (general.f90:8961-8978)
+  int mirrors;
+  switch (group) {
+    case 1: mirrors = 0; break;
+    case 3: mirrors = 4; break;
+    case 6: mirrors = 3; break;
+    case 8: mirrors = 7; break;
+    default: return -1;
+  }
+  if (!out || !(rsort > 0) || !(adimp > 0) || lmaxso < 0 || lmaxso > 40 || irrep < -1 || (irrep & ~mirrors) > 0)
+    return -1;
   fdmgpu_cluster *c = new (std::nothrow) fdmgpu_cluster();
   if (!c) return -3;
   c->rsort = rsort;
   c->adimp = adimp;
   c->lmaxso = lmaxso;
-  c->irrep = irrep;
+  c->irrep = mirrors ? irrep : -1;
+  c->mirrors = mirrors;
   if (build(c)) {
     delete c;
     return -1;
@@ -334,8 +353,12 @@ int fdmgpu_cluster_create_mmm(double rsort, double adimp, int lmaxso, int irrep,
   return 0;
 }
 
+int fdmgpu_cluster_create_mmm(double rsort, double adimp, int lmaxso, int irrep, fdmgpu_cluster **out) {
+  return fdmgpu_cluster_create_group(rsort, adimp, lmaxso, irrep < 0 ? 1 : 8, irrep, out);
+}
+
 int fdmgpu_cluster_create(double rsort, double adimp, int lmaxso, fdmgpu_cluster **out) {
-  return fdmgpu_cluster_create_mmm(rsort, adimp, lmaxso, -1, out);
+  return fdmgpu_cluster_create_group(rsort, adimp, lmaxso, 1, -1, out);
 }
 
 int fdmgpu_cluster_rvol(const fdmgpu_cluster *c, double *rvol) {

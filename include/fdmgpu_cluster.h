@@ -43,8 +43,20 @@ int fdmgpu_cluster_create(double rsort, double adimp, int lmaxso, fdmgpu_cluster
  * 4 pi rvol / sum(rvol) (bordure, general.f90:10073-10075) and the outer
  * unknowns are the representation's harmonics in increasing l*l + l + m.
  * The unknown of grid row i is rvol(i) times the wave function there
- * (mat.f90:2132-2143).  Such clusters have no device-assembler tables. */
+ * (mat.f90:2132-2143).  fdmgpu_cluster_assembly_tables and
+ * fdmgpu_cluster_assembly_weights give the device assembler's tables. */
 int fdmgpu_cluster_create_mmm(double rsort, double adimp, int lmaxso, int irrep, fdmgpu_cluster **out);
+/* The same for the mirror groups by the reference's point-group number
+ * (general.f90:5207-5220) in posequiv's orientation (general.f90:8961-8978):
+ * group 1 (P1, irrep ignored), 3 (m, the mirror z = 0: wedge z >= 0),
+ * 6 (mm2, the mirrors x = 0 and y = 0: wedge x, y >= 0) and 8 (mmm, as
+ * fdmgpu_cluster_create_mmm).  irrep: -1 = the whole cluster, else its bits
+ * (odd under the mirror normal to x, y, z) are a subset of the group's
+ * mirrors: 2 representations for m, 4 for mm2, 8 for mmm.  rvol = the
+ * point's number of images under the group / the group's order; the
+ * potential must have the group's symmetry.  Returns -1 for any other
+ * group or irrep. */
+int fdmgpu_cluster_create_group(double rsort, double adimp, int lmaxso, int group, int irrep, fdmgpu_cluster **out);
 /* rvol of the npoint interior points (1 for P1) */
 int fdmgpu_cluster_rvol(const fdmgpu_cluster *c, double *rvol);
 /* harmonic index l*l + l + m of each of the nlmso outer unknowns */
@@ -89,7 +101,7 @@ int fdmgpu_cluster_assembly_sizes(const fdmgpu_cluster *c, int64_t *sizes);
  * ylm[nb*nlmso]  real harmonics at the outer-sphere points
  * brad[nb]       their radii (bohr)
  * par[2]         clapl(0), outer-row quadrature weight pw = 4 pi / nsortf
- * For a representation of mmm the stencil weights carry the character and
+ * For a representation of a mirror group the stencil weights carry the character and
  * rvol factors (-clapl(v) chi rvol(i), over rvol(j) in base) and the outer
  * rows' boundary columns are -poidso Ylm / rvol(j). */
 int fdmgpu_cluster_assembly_tables(const fdmgpu_cluster *c, double *base, int64_t *qdiag, int *orow,

@@ -1,7 +1,9 @@
-"""Symmetry-reduced systems (fdmgpu_cluster_create_mmm): the eight
-representations of the mirror group mmm, each solved on the wedge
-x, y, z >= 0, reproduce the solution of the whole cluster (the oracle's P1
-restatement) for a potential with that symmetry.  With u = rvol * psi on the
+"""Symmetry-reduced systems (fdmgpu_cluster_create_mmm and
+fdmgpu_cluster_create_group): the representations of the mirror groups mmm
+(eight, wedge x, y, z >= 0), mm2 (four, wedge x, y >= 0) and m (two, wedge
+z >= 0), each solved on its wedge, reproduce the solution of the whole
+cluster (the oracle's P1 restatement) for a potential with exactly that
+symmetry (odd terms along the other axes).  With u = rvol * psi on the
 wedge (mat.f90:2132-2143), the reduced rows are the whole cluster's rows
 restricted to one representation: u / rvol equals the whole solution at the
 wedge points and the outer amplitudes are equal, for every right-hand side
@@ -15,19 +17,30 @@ from oracle import oracle as O
 BOHR = 0.52917721067
 
 
-def _potential(pts):
-    # even in each coordinate separately, but not cubic
-    x, y, z = np.abs(pts[:, 0]), np.abs(pts[:, 1]), np.abs(pts[:, 2])
+MIRRORS = {8: 7, 6: 3, 3: 4}  # point-group number -> mirrors (bit a = normal to axis a)
+GROUPS = sorted(MIRRORS)
+
+
+def _irreps(group):
+    return [g for g in range(8) if g & ~MIRRORS[group] == 0]
+
+
+def _potential(pts, group=8):
+    """Even under the group's mirrors only: not cubic, and along an axis
+    without a mirror it has an odd part."""
+    mk = MIRRORS[group]
+    x, y, z = (np.abs(pts[:, a]) if (mk >> a) & 1 else pts[:, a] for a in range(3))
     r2 = x * x + y * y + z * z
-    return -1.3 * np.exp(-r2 / 6.0) + 0.3 * np.cos(0.9 * x) * np.cos(0.6 * y) * np.cos(1.3 * z) - 0.2
+    v = -1.3 * np.exp(-r2 / 6.0) + 0.3 * np.cos(0.9 * x) * np.cos(0.6 * y) * np.cos(1.3 * z) - 0.2
+    return v + 0.25 * np.exp(-r2 / 8.0) * (np.tanh(0.5 * x) + 0.6 * np.sin(0.4 * y) + np.tanh(0.7 * z))
 
 
-def _rep(CL, R, h, lm, g):
+def _rep(CL, R, h, lm, g, group=8):
     """The representation's system, or None when no harmonic up to lm has
     its parities (irrep 7, odd in x, y and z, needs l >= 3)."""
     from fdmnes_amd import _lib
     try:
-        return CL.Cluster(R, h, lm, irrep=g)
+        return CL.Cluster(R, h, lm, irrep=g, group=group)
     except _lib.FdmGpuError:
         assert lm < 3 and g == 7
         return None
@@ -37,18 +50,20 @@ def _rel(a, b):
     return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
 
 
-def test_harmonic_parities():
+@pytest.mark.parametrize("group", GROUPS)
+def test_harmonic_parities(group):
     """The representation bits assigned to each real harmonic are its
-    parities under the three mirrors (checked on the harmonics themselves)."""
+    parities under the group's mirrors (checked on the harmonics themselves),
+    and the representations share out every harmonic once."""
     from fdmnes_amd import cluster as CL
     lmax = 4
     v = np.array([0.31, 0.52, 0.79])
     y0 = O.ylm_real(lmax, v, float(np.linalg.norm(v)))
     seen = []
-    for g in range(8):
-        c = CL.Cluster(1.2 / BOHR, 0.25 / BOHR, lmax, irrep=g)
+    for g in _irreps(group):
+        c = CL.Cluster(1.2 / BOHR, 0.25 / BOHR, lmax, irrep=g, group=group)
         for lm in c.harmonics:
-            for a in range(3):
+            for a in (a for a in range(3) if (MIRRORS[group] >> a) & 1):
                 w = v.copy()
                 w[a] = -w[a]
                 ya = O.ylm_real(lmax, w, float(np.linalg.norm(w)))[lm]
@@ -58,46 +73,58 @@ def test_harmonic_parities():
     assert sorted(seen) == list(range((lmax + 1) ** 2))
 
 
-def test_wedge_grid_and_weights():
+@pytest.mark.parametrize("group", GROUPS)
+def test_wedge_grid_and_weights(group):
     from fdmnes_amd import cluster as CL
+    mk = MIRRORS[group]
     full = CL.Cluster(1.6 / BOHR, 0.25 / BOHR, 2)
-    c = CL.Cluster(1.6 / BOHR, 0.25 / BOHR, 2, irrep=0)
+    c = CL.Cluster(1.6 / BOHR, 0.25 / BOHR, 2, irrep=0, group=group)
     pts, fp = c.points, full.points
-    assert (pts >= 0).all()
+    for a in range(3):
+        if (mk >> a) & 1:
+            assert (pts[:, a] >= 0).all()
+        else:
+            assert (pts[:, a] < 0).any()
+    order = 1 << bin(mk).count("1")
+    signs = [[(-1) ** ((f >> a) & 1) for a in range(3)] for f in range(8) if f & ~mk == 0]
+    assert len(signs) == order
     # every interior point of the whole cluster is the image of exactly one wedge point
     imgs = set()
     for p, rv in zip(pts, c.rvol):
-        orbit = {(sx * p[0], sy * p[1], sz * p[2]) for sx in (1, -1) for sy in (1, -1) for sz in (1, -1)}
-        assert rv == len(orbit) / 8
-        imgs |= {tuple(np.round(np.array(q) / full.adimp).astype(int)) for q in orbit}
+        orbit = {tuple(s * p) for s in np.array(signs)}
+        assert rv == len(orbit) / order
+        new = {tuple(np.round(np.array(q) / full.adimp).astype(int)) for q in orbit}
+        assert not (imgs & new)
+        imgs |= new
     assert imgs == {tuple(q) for q in np.round(fp / full.adimp).astype(int)}
     # rows in radial order, as reseau's exchange sort leaves them
     r = np.linalg.norm(pts, axis=1)
     assert (np.diff(r) > -1e-4 * full.adimp).all()
 
 
+@pytest.mark.parametrize("group", GROUPS)
 @pytest.mark.parametrize("E_comp,basereel", [(True, False), (False, True)])
-def test_representations_reproduce_whole_cluster(E_comp, basereel):
+def test_representations_reproduce_whole_cluster(E_comp, basereel, group):
     from fdmnes_amd import cluster as CL
     R, h, lm = 1.6 / BOHR, 0.25 / BOHR, 2
     o = O.Cluster(R, h, lm)
     cal = E_comp or not basereel
     E, Ei, Ec = 0.35, (0.0367 if E_comp else 0.0), 0.55
     xyz = o.xyz[:o.npoint, :3]
-    st, xf, *_ = o.solve(_potential(xyz), E, Ei, Ec, E_comp, cal, basereel)
+    st, xf, *_ = o.solve(_potential(xyz, group), E, Ei, Ec, E_comp, cal, basereel)
     assert st == 0
     row = {tuple(k): i for i, k in enumerate(np.round(xyz / h).astype(int))}
     nsof = o.nligne - o.nlmso
     scale = np.abs(xf).max()
     bound = 1e-9 if basereel else 1e-10
     covered = []
-    for g in range(8):
-        c = _rep(CL, R, h, lm, g)
+    for g in _irreps(group):
+        c = _rep(CL, R, h, lm, g, group)
         if c is None:
             continue
         pts = c.points
         lb1, lb2 = c.profile()
-        rp, col, vals, rhs = c.assemble(_potential(pts), E, Ei, Ec, E_comp, basereel)
+        rp, col, vals, rhs = c.assemble(_potential(pts, group), E, Ei, Ec, E_comp, basereel)
         st, u, _ = O.mat_solve_csr(lb1, lb2, rp, col, vals, rhs, cal)
         assert st == 0
         hm = c.harmonics
@@ -106,13 +133,13 @@ def test_representations_reproduce_whole_cluster(E_comp, basereel):
         psi = u[:c.npoint] / c.rvol[:, None]
         d_grid = np.abs(psi - xf[rows][:, hm]).max() / scale
         d_out = np.abs(u[c.npoint:] - xf[nsof + hm][:, hm]).max() / scale
-        print("irrep", g, "harmonics", list(hm), "rows", c.nligne, "grid", d_grid, "outer", d_out)
+        print("group", group, "irrep", g, "harmonics", list(hm), "rows", c.nligne, "grid", d_grid, "outer", d_out)
         assert d_grid <= bound and d_out <= bound
     assert sorted(covered) == list(range(o.nlmso))
     # the whole solution has no outer amplitude across representations
     gof = {}
-    for g in range(8):
-        c = _rep(CL, R, h, lm, g)
+    for g in _irreps(group):
+        c = _rep(CL, R, h, lm, g, group)
         for k in ([] if c is None else c.harmonics):
             gof[int(k)] = g
     for L in range(o.nlmso):
@@ -121,8 +148,9 @@ def test_representations_reproduce_whole_cluster(E_comp, basereel):
                 assert abs(xf[nsof + Lp, L]) <= bound * scale
 
 
+@pytest.mark.parametrize("group", GROUPS)
 @pytest.mark.parametrize("E_comp,basereel", [(True, False), (False, True)])
-def test_assembly_tables_rebuild_representation_rows(E_comp, basereel):
+def test_assembly_tables_rebuild_representation_rows(E_comp, basereel, group):
     """The device assembler's tables of each representation (stencil weights
     with characters and rvol, per-point outer weights, mirror terms of the
     diagonal, l of each outer unknown) rebuild the host assembler's rows:
@@ -132,10 +160,10 @@ def test_assembly_tables_rebuild_representation_rows(E_comp, basereel):
     R, h, lm = 2.0 / BOHR, 0.25 / BOHR, 3
     cal = E_comp or not basereel
     E, Ei, Ec = 0.45, (0.0367 if E_comp else 0.0), 0.7
-    for g in range(8):
-        c = CL.Cluster(R, h, lm, irrep=g)
+    for g in _irreps(group):
+        c = CL.Cluster(R, h, lm, irrep=g, group=group)
         t = c.assembly_tables()
-        V = _potential(c.points)
+        V = _potential(c.points, group)
         _, _, hv, hr = c.assemble(V, E, Ei, Ec, E_comp, basereel)
         nl, npn, ns = t["nlmso"], t["npoint"], t["nsortf"]
         fixed = np.ones(c.nnz, bool)
