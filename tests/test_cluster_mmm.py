@@ -209,3 +209,19 @@ def test_assembly_tables_rebuild_representation_rows(E_comp, basereel, group):
         assert np.abs(vals - hv).max() <= 1e-13 * np.abs(hv).max()
         assert np.abs(rhs - hr).max() <= 1e-13 * np.abs(hr).max()
     assert (t["dself"] != 0).any()
+
+
+def test_group_and_irrep_checks():
+    """Only the mirror groups are built, and a representation's bits must
+    be among the group's mirrors."""
+    from fdmnes_amd import _lib
+    from fdmnes_amd import cluster as CL
+    R, h = 1.2 / BOHR, 0.25 / BOHR
+    for group, irrep in [(6, 4), (3, 1), (3, 2), (5, 0), (2, 0), (8, 8)]:
+        with pytest.raises(_lib.FdmGpuError):
+            CL.Cluster(R, h, 2, irrep=irrep, group=group)
+    whole = CL.Cluster(R, h, 2)
+    for group in GROUPS:  # irrep -1: the whole cluster whatever the group
+        c = CL.Cluster(R, h, 2, irrep=-1, group=group)
+        assert (c.nligne, c.nnz) == (whole.nligne, whole.nnz)
+        assert np.array_equal(c.points, whole.points)
