@@ -34,6 +34,8 @@ def _sigs(L):
     L.fdmgpu_solver_stats.argtypes = [_V, P(C.c_double)]
     L.fdmgpu_solver_tune.argtypes = [_V, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
     L.fdmgpu_solver_overlap.argtypes = [_V, C.c_int]
+    L.fdmgpu_solver_snapshots.argtypes = [_V, C.c_int]
+    L.fdmgpu_solver_snapshot.argtypes = [_V, C.c_int, C.c_int, C.c_int, _V, C.c_int64, P(C.c_int64)]
     L._solver_sigs = True
 
 
@@ -151,6 +153,26 @@ class DeviceSolver:
                                             _ptrs([as_real(t) for t in x]), out)
         _lib.check(rc)
         return np.array(out[:])
+
+    def snapshots(self, on: bool):
+        """Keep host copies of each stage of the next solves (test hook,
+        fdmgpu_solver_snapshots)."""
+        _lib.check(self.L.fdmgpu_solver_snapshots(self._h, int(bool(on))))
+
+    def snapshot(self, slot: int) -> dict:
+        """The last solve's stages of one slot: raw bytes of the factor and
+        inverse tiles, and per refinement sweep of the FP32 correction and
+        the FP64 x after it."""
+        def get(what, sweep=0):
+            n = C.c_int64(0)
+            _lib.check(self.L.fdmgpu_solver_snapshot(self._h, slot, what, sweep, None, 0, C.byref(n)))
+            if what == 3:
+                return n.value
+            buf = (C.c_char * n.value)()
+            _lib.check(self.L.fdmgpu_solver_snapshot(self._h, slot, what, sweep, buf, n.value, C.byref(n)))
+            return bytes(buf)
+        nsw = get(3)
+        return {"factor": get(0), "d": [get(1, k) for k in range(nsw)], "x": [get(2, k) for k in range(nsw)]}
 
     def stats(self) -> dict:
         out = (C.c_double * 8)()

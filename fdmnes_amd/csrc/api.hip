@@ -364,6 +364,8 @@ int fdmgpu_solver_sigma(fdmgpu_solver *s, int count, const int *slots, const dou
 int fdmgpu_solver_options(fdmgpu_solver *s, int use_graphs, int time_updates, int persist_wg) {
   if (!s) return FDMGPU_EINVAL;
   auto apply = [&](auto *p) {
+    // captured graphs hold the old grid sizes: rebuild them on a change
+    if ((use_graphs != 0) != p->use_graphs || (persist_wg > 0 && persist_wg != p->grp.persist_wg)) p->drop_graphs();
     p->use_graphs = use_graphs != 0;
     p->grp.time_updates = time_updates != 0;
     if (persist_wg > 0) p->grp.persist_wg = persist_wg;
@@ -449,6 +451,58 @@ int fdmgpu_solver_stats(const fdmgpu_solver *s, double *out) {
     else fill((Solver<float, false> *)s->impl);
   }
   return 0;
+}
+
+int fdmgpu_solver_snapshots(fdmgpu_solver *s, int on) {
+  if (!s || on < 0 || on > 1) {
+    set_error("fdmgpu_solver_snapshots: invalid argument");
+    return FDMGPU_EINVAL;
+  }
+  auto apply = [&](auto *p) { p->snapshots = on != 0; };
+  if (s->prec == FDMGPU_FP64) {
+    if (s->cal) apply((Solver<double, true> *)s->impl);
+    else apply((Solver<double, false> *)s->impl);
+  } else {
+    if (s->cal) apply((Solver<float, true> *)s->impl);
+    else apply((Solver<float, false> *)s->impl);
+  }
+  return 0;
+}
+
+int fdmgpu_solver_snapshot(const fdmgpu_solver *s, int slot, int what, int sweep, void *out, int64_t capacity,
+                           int64_t *bytes) {
+  if (!s || !bytes || what < 0 || what > 3 || slot < 0) {
+    set_error("fdmgpu_solver_snapshot: invalid argument");
+    return FDMGPU_EINVAL;
+  }
+  auto get = [&](auto *p) -> int {
+    if (slot >= (int)p->snap_factor.size() || p->snap_factor[slot].empty()) {
+      set_error("fdmgpu_solver_snapshot: no snapshot of this slot (snapshots off, or the slot was not solved)");
+      return FDMGPU_ESTATE;
+    }
+    const int nsw = (int)p->snap_d[slot].size();
+    if (what == 3) {
+      *bytes = nsw;
+      return 0;
+    }
+    if (what > 0 && (sweep < 0 || sweep >= nsw)) {
+      set_error("fdmgpu_solver_snapshot: sweep out of range");
+      return FDMGPU_EINVAL;
+    }
+    const std::vector<char> &v = what == 0 ? p->snap_factor[slot] : (what == 1 ? p->snap_d[slot][sweep]
+                                                                                 : p->snap_x[slot][sweep]);
+    *bytes = (int64_t)v.size();
+    if (out) {
+      if (capacity < (int64_t)v.size()) {
+        set_error("fdmgpu_solver_snapshot: buffer too small");
+        return FDMGPU_EINVAL;
+      }
+      memcpy(out, v.data(), v.size());
+    }
+    return 0;
+  };
+  if (s->prec == FDMGPU_FP64) return s->cal ? get((Solver<double, true> *)s->impl) : get((Solver<double, false> *)s->impl);
+  return s->cal ? get((Solver<float, true> *)s->impl) : get((Solver<float, false> *)s->impl);
 }
 
 struct fdmgpu_assembler {

@@ -47,6 +47,15 @@ struct Solver {
   // refinement / fallback statistics of the last solve
   std::vector<int> last_sweeps;
   std::vector<double> last_rel;
+  // FP64 groups of one slot each for the points whose FP32 refinement fails,
+  // allocated on a slot's first fallback and kept (scheduled like grp)
+  std::vector<std::unique_ptr<Group<double, C>>> fb;
+  // Test hook: host copies, from the last solve, of each solved slot's factor
+  // and inverse tiles after the factorisation and, per refinement sweep, of
+  // the FP32 correction d and the FP64 x after it (by slot, by sweep).
+  bool snapshots = false;
+  std::vector<std::vector<char>> snap_factor;
+  std::vector<std::vector<std::vector<char>>> snap_d, snap_x;
 
   ~Solver();
   int init(int nrhs, hipStream_t s);
@@ -64,6 +73,8 @@ struct Solver {
   int factor_phase(uint32_t mask);
   int sweep_phase(uint32_t mask);
   int fallback_fp64(int slot, int *fail_row);
+  int snap_factors(int count, const int *ids);
+  int snap_sweep(const std::vector<int> &slot_ids);
 };
 
 }  // namespace fdmgpu

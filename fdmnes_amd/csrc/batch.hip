@@ -62,6 +62,7 @@ __global__ void k_adjoint_grad(const double *__restrict__ lam, const double *__r
 
 template <class R, bool C>
 Solver<R, C>::~Solver() {
+  fb.clear();
   for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
   for (auto &kv : sweeps_g) (void)hipGraphExecDestroy(kv.second);
   if (hnorm) (void)hipHostFree(hnorm);
@@ -263,8 +264,18 @@ int Solver<R, C>::sweep_phase(uint32_t mask) {
 template <class R, bool C>
 int Solver<R, C>::fallback_fp64(int g, int *fail_row) {
   Slot &sl = slots[g];
-  Group<double, C> f;
-  if (f.add(sl.plan.get(), nrhs) || f.finalize(stream)) return -2;
+  if (fb.size() < slots.size()) fb.resize(slots.size());
+  if (!fb[g]) {
+    std::unique_ptr<Group<double, C>> f(new Group<double, C>());
+    f->gk = grp.gk;
+    f->persist_wg = grp.persist_wg;
+    f->trail_tile = grp.trail_tile;
+    f->persist_solve = grp.persist_solve;
+    f->solve_wg = grp.solve_wg;
+    if (f->add(sl.plan.get(), nrhs) || f->finalize(stream)) return -2;
+    fb[g] = std::move(f);
+  }
+  Group<double, C> &f = *fb[g];
   const size_t tot = (size_t)sl.plan->NB * TB * nrhs * NC;
   if (f.zero(0) || f.scatter_csr(0, sl.row_ptr, sl.col, sl.vals)) return -2;
   hipLaunchKernelGGL((k_load_rhs<double>), dim3(1024), dim3(256), 0, stream, f.m[0].X, (const double *)sl.B, tot, tot);
@@ -274,6 +285,38 @@ int Solver<R, C>::fallback_fp64(int g, int *fail_row) {
   *fail_row = fr[0];
   FDM_HIP(hipMemcpyAsync(sl.Xd, f.m[0].X, sizeof(double) * tot, hipMemcpyDeviceToDevice, stream));
   FDM_HIP(hipStreamSynchronize(stream));
+  return 0;
+}
+
+template <class R, bool C>
+int Solver<R, C>::snap_factors(int count, const int *ids) {
+  FDM_HIP(hipStreamSynchronize(stream));
+  snap_factor.resize(slots.size());
+  snap_d.assign(slots.size(), {});
+  snap_x.assign(slots.size(), {});
+  const size_t te = (size_t)TB * TB * NC * sizeof(R);
+  for (int k = 0; k < count; k++) {
+    const int g = ids[k];
+    const Plan &pl = *slots[g].plan;
+    const size_t bt = te * (size_t)pl.ntiles, bd = te * DSLOTS * (size_t)pl.NB;
+    snap_factor[g].resize(bt + bd);
+    FDM_HIP(hipMemcpy(snap_factor[g].data(), grp.m[g].tiles, bt, hipMemcpyDeviceToHost));
+    FDM_HIP(hipMemcpy(snap_factor[g].data() + bt, grp.m[g].dinv, bd, hipMemcpyDeviceToHost));
+  }
+  return 0;
+}
+
+template <class R, bool C>
+int Solver<R, C>::snap_sweep(const std::vector<int> &slot_ids) {
+  FDM_HIP(hipStreamSynchronize(stream));
+  for (int g : slot_ids) {
+    const size_t tot = (size_t)slots[g].plan->NB * TB * nrhs * NC;
+    std::vector<char> d(tot * sizeof(R)), x(tot * sizeof(double));
+    FDM_HIP(hipMemcpy(d.data(), grp.m[g].X, d.size(), hipMemcpyDeviceToHost));
+    FDM_HIP(hipMemcpy(x.data(), slots[g].Xd, x.size(), hipMemcpyDeviceToHost));
+    snap_d[g].push_back(std::move(d));
+    snap_x[g].push_back(std::move(x));
+  }
   return 0;
 }
 
@@ -293,6 +336,7 @@ int Solver<R, C>::solve(int count, const int *ids, const double *const *vals, co
   if (factor_phase(mask)) return -2;
   std::vector<int> fr;
   if (grp.read_fail(fr)) return -2;
+  if (snapshots && snap_factors(count, ids)) return -2;
   last_sweeps.assign(count, 0);
   last_rel.assign(count, 0.0);
   if (std::is_same<R, double>::value) {
@@ -339,6 +383,7 @@ int Solver<R, C>::solve(int count, const int *ids, const double *const *vals, co
     for (int k : act)
       FDM_HIP(hipMemcpyAsync(hnorm + k * np, slots[ids[k]].scr, sizeof(double) * np, hipMemcpyDeviceToHost, stream));
     FDM_HIP(hipStreamSynchronize(stream));
+    if (snapshots && snap_sweep(ids_act)) return -2;
     for (int k : act) {
       sum_norm_partials(hnorm + k * np, nrhs, rn[k].data());
       double mx = 0;
@@ -434,6 +479,7 @@ void Solver<R, C>::drop_graphs() {
   for (auto &kv : sweeps_g) (void)hipGraphExecDestroy(kv.second);
   graphs.clear();
   sweeps_g.clear();
+  fb.clear();  // rebuilt with the new schedule on their next use
 }
 
 template struct Solver<double, true>;

@@ -1038,6 +1038,23 @@ __global__ __launch_bounds__(NT) void k_solve_finish(const SysDev<R> *__restrict
 // bounded: past SPIN_LIMIT polls the task records an error and goes on, so
 // the kernel always terminates.
 constexpr int SPIN_LIMIT = 1 << 22;
+// s_waitcnt immediate: vmcnt(0), expcnt and lgkmcnt at their maxima (no wait)
+constexpr int WAIT_VM0 = 0x0F70;
+
+// Publishes done[K] = 1 after the workgroup's stores of x_K: one thread,
+// after a barrier at which every wave has waited for its own stores.  The
+// agent-scope release fence writes the XCD's L2 back to memory (buffer_wbl2);
+// the explicit vmcnt(0) makes that write-back complete before the flag store
+// leaves.  An atomic release store alone compiles to buffer_wbl2 followed at
+// once by the flag store when no store is pending (the wait the memory
+// model places after the write-back is dropped), so a consumer on another
+// XCD could see the flag while x_K's lines are still in this XCD's L2 and
+// read the block's old contents from memory.
+__device__ __forceinline__ void publish_flag(int *flag) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __builtin_amdgcn_s_waitcnt(WAIT_VM0);
+  __hip_atomic_store(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // Systems of one persistent solve: system sys[i] runs its NB[i] block steps
 // at global steps off[i] .. off[i] + NB[i] - 1 (aligned at the end).  Tasks
@@ -1070,7 +1087,11 @@ __device__ __forceinline__ int ready_run(const int *done, int base, int dir, int
                     __hip_atomic_load(&done[base + dir * (q + lane)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
     const uint64_t ready = __ballot(ok);
     if (ready & 1) {
+      // the acquire invalidates this CU's L1 and this XCD's L2; the wait
+      // lets the invalidation complete before the caller's barrier releases
+      // the other waves to load the published x blocks
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      __builtin_amdgcn_s_waitcnt(WAIT_VM0);
       return ~ready == 0 ? 64 : __builtin_ctzll(~ready);
     }
     if (__builtin_amdgcn_readfirstlane(__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) != 0)
@@ -1368,12 +1389,11 @@ __global__ __launch_bounds__(NT) void k_solve_persist(const SysDev<R> *__restric
       __syncthreads();
     }
     // publish x_K to every XCD, then the flag: each wave waits for its own
-    // stores to complete (vmcnt(0); expcnt and lgkmcnt left at their
-    // maxima), and the flag's agent-scope release after the barrier writes
-    // the L2 back once for the whole workgroup
-    __builtin_amdgcn_s_waitcnt(0x0F70);
+    // stores to complete, and one write-back of the L2 after the barrier
+    // covers the whole workgroup
+    __builtin_amdgcn_s_waitcnt(WAIT_VM0);
     __syncthreads();
-    if (tid == 0) __hip_atomic_store(&S.done[K], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (tid == 0) publish_flag(&S.done[K]);
   }
 }
 

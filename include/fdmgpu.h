@@ -165,6 +165,19 @@ int fdmgpu_solver_tune(fdmgpu_solver *s, int gk, int trail_wg, int persist_solve
  * of the timed trailing-update launches (every output tile read and
  * written once, every U and L tile of the step range read once). */
 int fdmgpu_solver_stats(const fdmgpu_solver *s, double *out);
+/* Test hooks, no counterpart in the reference: with snapshots on (1), each
+ * solve keeps host copies of every solved slot's factor tiles and inverse
+ * tiles after the factorisation and, per refinement sweep, of the FP32
+ * correction and of the FP64 x after it (the last solve only; the copies
+ * synchronise the stream, so a solve is slower).  fdmgpu_solver_snapshot
+ * copies one of them to out (capacity bytes) and sets *bytes to its size
+ * (out may be NULL to ask for the size): what 0 = factor then inverse tiles,
+ * 1 = correction of sweep `sweep`, 2 = x after sweep `sweep`, 3 = *bytes
+ * set to the number of sweeps kept.  Callers: determinism tests that
+ * compare graph replays stage by stage. */
+int fdmgpu_solver_snapshots(fdmgpu_solver *s, int on);
+int fdmgpu_solver_snapshot(const fdmgpu_solver *s, int slot, int what, int sweep, void *out, int64_t capacity,
+                           int64_t *bytes);
 
 /*
  * Per-energy row assembly on the device, for the energy loop over one

@@ -28,14 +28,26 @@ def _run(precision, systems, repeats=1, max_sweeps=5, tol=1e-12, tune=None, over
     vals = [_dev(s["vals"].astype(dt)) for s in systems]
     b = [_dev(s["rhs"].astype(dt)) for s in systems]
     x = [torch.zeros_like(t) for t in b]
-    out = None
-    for _ in range(repeats):  # graph replays must reproduce the first run
+    out = stages = sweeps = None
+    if repeats > 1:
+        sv.snapshots(True)
+    for rep in range(repeats):  # graph replays must reproduce the first run stage by stage
         fr, sw, rel = sv.solve(slots, vals, b, x, max_sweeps=max_sweeps, tol=tol)
         cur = [t.cpu().numpy() for t in x]
+        st = [sv.snapshot(k) for k in slots] if repeats > 1 else None
         if out is not None:
+            assert sw == sweeps, (rep, sw, sweeps)
+            for k, (p, q) in enumerate(zip(stages, st)):
+                assert p["factor"] == q["factor"], f"replay {rep}: factor tiles of slot {k} differ"
+                assert len(p["d"]) == len(q["d"])
+                for j in range(len(p["d"])):
+                    assert p["d"][j] == q["d"][j], f"replay {rep}: sweep {j} correction of slot {k} differs"
+                    assert p["x"][j] == q["x"][j], f"replay {rep}: x after sweep {j} of slot {k} differs"
             for a, c in zip(out, cur):
                 assert np.array_equal(a, c)
-        out = cur
+        out, stages, sweeps = cur, st, sw
+    if repeats > 1:
+        sv.snapshots(False)
     return sv, slots, out, sw, rel, b, x
 
 
