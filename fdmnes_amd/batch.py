@@ -30,6 +30,7 @@ def _sigs(L):
                                       P(C.c_int), P(C.c_int), P(C.c_double)]
     L.fdmgpu_solver_adjoint.argtypes = [_V, C.c_int, P(C.c_int), P(_V), P(_V), P(_V)]
     L.fdmgpu_solver_sigma.argtypes = [_V, C.c_int, P(C.c_int), P(_V), P(_V), P(C.c_double)]
+    L.fdmgpu_solver_sigma_noalloc.argtypes = [_V, C.c_int, P(C.c_int), P(_V), P(_V), _V]
     L.fdmgpu_solver_options.argtypes = [_V, C.c_int, C.c_int, C.c_int]
     L.fdmgpu_solver_stats.argtypes = [_V, P(C.c_double)]
     L.fdmgpu_solver_tune.argtypes = [_V, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
@@ -153,6 +154,18 @@ class DeviceSolver:
                                             _ptrs([as_real(t) for t in x]), out)
         _lib.check(rc)
         return np.array(out[:])
+
+    def sigma_noalloc(self, slots, src, x, out: torch.Tensor):
+        """sigma into the float64 device tensor out (len(slots) entries),
+        allocating nothing (fdmgpu_solver_sigma_noalloc)."""
+        if out.dtype != torch.float64 or not out.is_cuda or out.numel() < len(slots) or not out.is_contiguous():
+            raise ValueError("out: contiguous float64 device tensor with one entry per point")
+        n = len(slots)
+        ids = (C.c_int * n)(*slots)
+        with Ordered(self.stream):
+            rc = self.L.fdmgpu_solver_sigma_noalloc(self._h, n, ids, _ptrs([as_real(t) for t in src]),
+                                                    _ptrs([as_real(t) for t in x]), _V(out.data_ptr()))
+        _lib.check(rc)
 
     def snapshots(self, on: bool):
         """Keep host copies of each stage of the next solves (test hook,

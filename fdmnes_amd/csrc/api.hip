@@ -22,6 +22,14 @@ struct Lib {
 };
 Lib g_lib;
 
+// Scratch of fdmgpu_solver_sigma_noalloc: the partial sums of up to MAXG
+// points, SIG_PARTS doubles each (64 KiB).  Each device holds its own copy
+// until the process ends; host code uses only the device address that
+// hipGetSymbolAddress gives for it (per device, taken in fdmgpu_init).
+constexpr int SCRATCH_DEVICES = 64;
+static __device__ __align__(256) double scratch[MAXG * SIG_PARTS];
+double *g_scratch[SCRATCH_DEVICES] = {};
+
 int ensure_init() {
   if (g_lib.ready) {
     FDM_HIP(hipSetDevice(g_lib.device));
@@ -210,6 +218,15 @@ int fdmgpu_init(int device) {
     release_dropin_caches();
   }
   FDM_HIP(hipSetDevice(device));
+  if (device < SCRATCH_DEVICES && !g_scratch[device]) {
+    double *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error(std::string("fdmgpu_init: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+      return FDMGPU_EHIP;
+    }
+    g_scratch[device] = p;
+  }
   if (g_lib.stream) (void)hipStreamDestroy(g_lib.stream);
   FDM_HIP(hipStreamCreateWithFlags(&g_lib.stream, hipStreamNonBlocking));
   g_lib.device = device;
@@ -359,6 +376,26 @@ int fdmgpu_solver_sigma(fdmgpu_solver *s, int count, const int *slots, const dou
     return FDMGPU_EINVAL;
   }
   return map_rc(FDM_DISPATCH(s, sigma(count, slots, d_src, d_x, sigma)));
+}
+
+int fdmgpu_solver_sigma_noalloc(fdmgpu_solver *s, int count, const int *slots, const double *const *d_src,
+                                const double *const *d_x, double *d_sigma) {
+  if (!s || count <= 0 || !slots || !d_src || !d_x || !d_sigma) {
+    set_error("fdmgpu_solver_sigma_noalloc: invalid argument");
+    return FDMGPU_EINVAL;
+  }
+  if (count > MAXG) {
+    set_error("fdmgpu_solver_sigma_noalloc: at most " + std::to_string(MAXG) +
+              " points per call (the scratch holds their partial sums)");
+    return FDMGPU_EINVAL;
+  }
+  // the solver exists, so the library was initialised on this device
+  if (!g_lib.ready || g_lib.device >= SCRATCH_DEVICES || !g_scratch[g_lib.device]) {
+    set_error("fdmgpu_solver_sigma_noalloc: library not initialised on a device with a scratch address");
+    return FDMGPU_ESTATE;
+  }
+  FDM_HIP(hipSetDevice(g_lib.device));
+  return map_rc(FDM_DISPATCH(s, sigma_into(count, slots, d_src, d_x, g_scratch[g_lib.device], d_sigma)));
 }
 
 int fdmgpu_solver_options(fdmgpu_solver *s, int use_graphs, int time_updates, int persist_wg) {

@@ -9,6 +9,9 @@
 
 namespace fdmgpu {
 
+// Workgroups of the first stage of the sigma reduction (partial sums per point).
+constexpr int SIG_PARTS = 256;
+
 // One system slot: structure (profile + compressed-row pattern) and the FP64
 // buffers its energy points are staged into.
 struct Slot {
@@ -67,6 +70,11 @@ struct Solver {
             int max_sweeps, double tol, int *fail_rows, int *sweeps, double *rel);
   int adjoint(int count, const int *ids, const double *const *src, const double *const *x, double *const *grad);
   int sigma(int count, const int *ids, const double *const *src, const double *const *x, double *out);
+  // The same reduction on the solver's stream with the partial sums in part
+  // (count * SIG_PARTS doubles) and sigma written to d_out (device, count
+  // doubles): no allocation, no synchronisation.
+  int sigma_into(int count, const int *ids, const double *const *src, const double *const *x, double *part,
+                 double *d_out);
   void drop_graphs();  // after a schedule change
 
  private:

@@ -11,7 +11,6 @@ namespace fdmgpu {
 // sigma = -Im sum_i s_i x_i over n*nrhs entries, in two deterministic
 // stages: workgroup b of SIG_PARTS sums a fixed contiguous range into
 // part[b] (fixed-order tree), then one workgroup adds the parts in order.
-constexpr int SIG_PARTS = 256;
 __device__ __forceinline__ double block_sum256(double v, double *sh) {
   sh[threadIdx.x] = v;
   __syncthreads();
@@ -447,21 +446,35 @@ int Solver<R, C>::adjoint(int count, const int *ids, const double *const *src, c
 }
 
 template <class R, bool C>
-int Solver<R, C>::sigma(int count, const int *ids, const double *const *src, const double *const *x, double *out) {
-  if (count > MAXG) {
-    set_error("sigma: at most " + std::to_string(MAXG) + " points per call");
+int Solver<R, C>::sigma_into(int count, const int *ids, const double *const *src, const double *const *x,
+                             double *part, double *d_out) {
+  if (count < 1 || count > MAXG) {
+    set_error("sigma: 1 to " + std::to_string(MAXG) + " points per call");
     return -1;
   }
+  for (int k = 0; k < count; k++)
+    if (ids[k] < 0 || ids[k] >= (int)slots.size() || !src[k] || !x[k]) {
+      set_error("sigma: bad slot id or null buffer for point " + std::to_string(k));
+      return -1;
+    }
+  for (int k = 0; k < count; k++) {
+    const int g = ids[k];
+    hipLaunchKernelGGL((k_sigma_part<C>), dim3(SIG_PARTS), dim3(256), 0, stream, src[k], x[k],
+                       (size_t)slots[g].n * nrhs, part + (size_t)k * SIG_PARTS);
+    hipLaunchKernelGGL(k_sigma_final, dim3(1), dim3(256), 0, stream, part + (size_t)k * SIG_PARTS, d_out + k);
+  }
+  FDM_HIP(hipGetLastError());
+  return 0;
+}
+
+template <class R, bool C>
+int Solver<R, C>::sigma(int count, const int *ids, const double *const *src, const double *const *x, double *out) {
   // partials and results of up to MAXG points, allocated on first use
   if (!sig_buf) FDM_HIP(hipMalloc(&sig_buf, sizeof(double) * MAXG * (SIG_PARTS + 1)));
   if (!hsig) FDM_HIP(hipHostMalloc(&hsig, sizeof(double) * MAXG));
   double *d = sig_buf + MAXG * SIG_PARTS;
-  for (int k = 0; k < count; k++) {
-    const int g = ids[k];
-    hipLaunchKernelGGL((k_sigma_part<C>), dim3(SIG_PARTS), dim3(256), 0, stream, src[k], x[k],
-                       (size_t)slots[g].n * nrhs, sig_buf + (size_t)k * SIG_PARTS);
-    hipLaunchKernelGGL(k_sigma_final, dim3(1), dim3(256), 0, stream, sig_buf + (size_t)k * SIG_PARTS, d + k);
-  }
+  const int rc = sigma_into(count, ids, src, x, sig_buf, d);
+  if (rc) return rc;
   hipError_t e = hipMemcpyAsync(hsig, d, sizeof(double) * count, hipMemcpyDeviceToHost, stream);
   if (e == hipSuccess) e = hipStreamSynchronize(stream);
   if (e == hipSuccess)

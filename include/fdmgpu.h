@@ -132,6 +132,17 @@ int fdmgpu_solver_adjoint(fdmgpu_solver *s, int count, const int *slots, const d
 /* sigma[k] = -Im sum_{i,L} src_{i,L} x_{i,L} (host output). */
 int fdmgpu_solver_sigma(fdmgpu_solver *s, int count, const int *slots, const double *const *d_src,
                         const double *const *d_x, double *sigma);
+/* The same sigma, bit for bit, written to device memory d_sigma[0..count-1]
+ * with no allocation and no synchronisation: the kernels run on the solver's
+ * stream (ordered after its solves) and keep their partial sums in a static
+ * scratch array of the library (64 KiB per device), so count is at most 32
+ * (FDMGPU_EINVAL beyond).  Every call on one device shares that scratch:
+ * calls that overlap on one device (solvers on different streams, other
+ * threads) race and must be ordered by the caller.  No counterpart in the
+ * reference; callers: those that cannot allocate (work captured into a HIP
+ * graph, tight serving loops). */
+int fdmgpu_solver_sigma_noalloc(fdmgpu_solver *s, int count, const int *slots, const double *const *d_src,
+                                const double *const *d_x, double *d_sigma);
 /* Tuning: HIP-graph capture on/off, event timing of the trailing-update
  * launches on/off (disables graphs), workgroups of the persistent GEMM. */
 int fdmgpu_solver_options(fdmgpu_solver *s, int use_graphs, int time_updates, int persist_wg);
