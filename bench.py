@@ -80,6 +80,236 @@ def _cpu_worker(wname, k, target_s):
                       f"rate scaled by that fraction")
 
 
+def _cu_worker(k):
+    """One whole energy point of the copper cluster (real arithmetic,
+    Basereel, calcMatRow's incoming-wave right-hand sides) through the
+    oracle's restatement of mat_solve, then its reduction to tau over the
+    block of the nlmso innermost grid rows; returns (k, seconds, sigma_tau)."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from fdmnes_amd import configs as cfg
+    from oracle import oracle as O
+    w = cfg.CU_REAL
+    cl = O.Cluster(w.R / cfg.BOHR, w.h / cfg.BOHR, w.lmaxso)
+    V = cfg.potential(w, cl.xyz[: cl.npoint, :3])
+    E = float(w.energies_Ry[k])
+    t0 = time.time()
+    st, x, fr, tf, tt, _ = cl.solve(V, E, 0.0, cfg.ecinetic_out(E), False, False, True)
+    t = time.time() - t0
+    if st:
+        return k, t, float("nan")
+    taus, _, info = O.tau(x, cl.nligne - cl.nlmso, True, False, [(0, cl.nlmso)])
+    return k, t, float(-np.imag(np.trace(taus[0])))
+
+
+def copper_cpu_start(procs: int):
+    """Starts the copper cluster's CPU sample in the background: procs
+    processes, one whole energy point each, spread over the 91 energies."""
+    import multiprocessing as mp
+    from fdmnes_amd import configs as cfg
+    n = cfg.CU_REAL.n_energies
+    ks = sorted({int(round(i * (n - 1) / max(procs - 1, 1))) for i in range(procs)})
+    pool = mp.get_context("spawn").Pool(len(ks))
+    return pool, pool.map_async(_cu_worker, ks)
+
+
+def copper_gpu(cpu_job):
+    """The copper cluster (fcc Cu, R = 3.0 A, h = 0.24 A, l <= 2, real
+    arithmetic, Basereel, no broadening) at all 91 energies on the GPU:
+    grouped FP64 solves of calcMatRow's incoming-wave right-hand sides (the
+    reference's RHS, nlmso = 9), then tau in the real basis over the block of
+    the nlmso innermost grid rows (invcomp of the outer-sphere
+    normalisation, fdmgpu_tau_reduce); sigma_tau(E) = -Im sum_L tau_LL, the
+    density of states the reference reads from Taull (mat.f90:393-396).
+    With the CPU sample (the oracle, whole energy points) its rate and the
+    per-energy difference of sigma_tau at the sampled energies."""
+    import torch
+    from fdmnes_amd import batch as B
+    from fdmnes_amd import cluster as CL
+    from fdmnes_amd import configs as cfg
+    from fdmnes_amd.tau import tau_reduce
+    w = cfg.CU_REAL
+    cl = CL.Cluster(w.R / cfg.BOHR, w.h / cfg.BOHR, w.lmaxso)
+    lb1, lb2 = cl.profile()
+    V = cfg.potential(w, cl.points)
+    E = w.energies_Ry
+    vals, rhs = [], []
+    for e in E:
+        rp, col, v, b = cl.assemble(V, float(e), 0.0, cfg.ecinetic_out(float(e)), False, True)
+        vals.append(torch.from_numpy(np.ascontiguousarray(v)).cuda())
+        rhs.append(torch.from_numpy(np.ascontiguousarray(b)).cuda())
+    G = 32
+    sv = B.DeviceSolver(B.FP64, False, cl.nlmso)
+    slots = [sv.add_system(lb1, lb2, rp, col) for _ in range(G)]
+    xs = [torch.zeros_like(b) for b in rhs]
+    nso = cl.nligne - cl.nlmso
+    sig = np.zeros(len(E))
+
+    def one_pass():
+        for g0 in range(0, len(E), G):
+            idx = list(range(g0, min(g0 + G, len(E))))
+            sv.solve(slots[: len(idx)], [vals[i] for i in idx], [rhs[i] for i in idx], [xs[i] for i in idx])
+            tau, _, _ = tau_reduce([xs[i] for i in idx], nso, True, False, [(0, cl.nlmso)])
+            for i, t in zip(idx, tau):
+                sig[i] = -float(torch.diagonal(t.view(cl.nlmso, cl.nlmso)).imag.sum())
+
+    one_pass()  # captures the graphs
+    torch.cuda.synchronize()
+    t0 = time.time()
+    one_pass()
+    torch.cuda.synchronize()
+    t_gpu = time.time() - t0
+    res = {"workload": w.name, "nligne": cl.nligne, "nrhs": cl.nlmso, "energies": len(E),
+           "gpu_fp64_points_per_s": len(E) / t_gpu, "sigma_tau": [float(v) for v in sig],
+           "energies_eV": [float(e * cfg.RYDB) for e in E]}
+    log(f"copper: {len(E)} energies on the GPU in {t_gpu:.2f}s ({len(E) / t_gpu:.1f} points/s)")
+    del sv, xs, vals, rhs
+    torch.cuda.empty_cache()
+    if cpu_job is not None:
+        pool, job = cpu_job
+        out = job.get()
+        pool.close()
+        pool.join()
+        ts = [t for _, t, _ in out]
+        rel = [abs(sig[k] - sc) / abs(sc) for k, _, sc in out]
+        res["cpu"] = {"kind": "port", "cores": len(out), "points": len(out),
+                      "mean_s_per_point": float(np.mean(ts)), "points_per_s": len(out) / float(np.mean(ts)),
+                      "sample": f"{len(out)} of the {len(E)} energies, one whole point (assembly and mat_solve "
+                                f"restatement) per process, concurrent",
+                      "energies_eV": [float(E[k] * cfg.RYDB) for k, _, _ in out],
+                      "sigma_tau_rel_diff_gpu_vs_oracle": [float(r) for r in rel]}
+        res["gpu_over_cpu"] = res["gpu_fp64_points_per_s"] / res["cpu"]["points_per_s"]
+        log(f"copper CPU: {len(out)} points, {np.mean(ts):.1f} s each; sigma_tau GPU vs oracle max rel "
+            f"{max(rel):.2e}")
+    return res
+
+
+def batch_waves(budget_gb: float):
+    """The three iron clusters of uneven size (R = 3, 4, 5 A, h = 0.2 A;
+    86 + 85 + 85 = 256 energy points, FP64, 16 right-hand sides each) in
+    HBM-sized waves of grouped solves: the solver holds n_c slots of cluster
+    c, chosen so the factor storage fits the budget, at most 32 slots, with
+    the fewest waves; each wave is one grouped graph-captured solve of the
+    next n_c points of every cluster.  Host assembly and upload of all
+    points precede the timed region."""
+    import itertools as it
+    import torch
+    from fdmnes_amd import batch as B
+    from fdmnes_amd import cluster as CL
+    from fdmnes_amd import configs as cfg
+    cls, pts = [], []
+    for ci, w in enumerate(cfg.BATCH):
+        cl = CL.Cluster(w.R / cfg.BOHR, w.h / cfg.BOHR, w.lmaxso)
+        lb1, lb2 = cl.profile()
+        V = cfg.potential(w, cl.points)
+        s = torch.from_numpy(cfg.sources(w, cl.points, cl.nligne).astype(np.complex128)).cuda()
+        vals = []
+        for e in w.energies_Ry:
+            rp, col, v, _ = cl.assemble(V, float(e), w.eimag_Ry, cfg.ecinetic_out(float(e)), True, False)
+            vals.append(torch.from_numpy(v.astype(np.complex128)).cuda())
+        cls.append(dict(w=w, cl=cl, lb1=lb1, lb2=lb2, rp=rp, col=col, s=s, vals=vals))
+        pts.append(len(vals))
+    # factor bytes per slot of each cluster
+    nrhs = cls[0]["s"].shape[1]
+    per = []
+    for c in cls:
+        sv = B.DeviceSolver(B.FP64, True, nrhs)
+        sv.add_system(c["lb1"], c["lb2"], c["rp"], c["col"])
+        per.append(sv.stats()["factor_bytes"] / 1e9)
+        del sv
+    torch.cuda.empty_cache()
+    best = None
+    for n in it.product(*[range(1, 33)] * len(cls)):
+        if sum(n) > 32 or sum(a * b for a, b in zip(n, per)) > budget_gb:
+            continue
+        waves = max(-(-p // k) for p, k in zip(pts, n))
+        key = (waves, sum(a * b for a, b in zip(n, per)))
+        if best is None or key < best[0]:
+            best = (key, n)
+    (waves, gb), nslot = best
+    sv = B.DeviceSolver(B.FP64, True, nrhs)
+    slots = [[sv.add_system(c["lb1"], c["lb2"], c["rp"], c["col"]) for _ in range(k)] for c, k in zip(cls, nslot)]
+    xs = [[torch.zeros_like(c["s"]) for _ in c["vals"]] for c in cls]
+    sig = [np.zeros(p) for p in pts]
+
+    def wave(v):
+        sl, va, b, x, where = [], [], [], [], []
+        for ci, c in enumerate(cls):
+            for j in range(nslot[ci]):
+                k = v * nslot[ci] + j
+                if k < pts[ci]:
+                    sl.append(slots[ci][j])
+                    va.append(c["vals"][k])
+                    b.append(c["s"])
+                    x.append(xs[ci][k])
+                    where.append((ci, k))
+        sv.solve(sl, va, b, x)
+        for (ci, k), val in zip(where, sv.sigma(sl, b, x)):
+            sig[ci][k] = val
+
+    wave(0)  # warm-up: captures the first wave's graphs (later masks are captured in the timed run)
+    torch.cuda.synchronize()
+    t0 = time.time()
+    for v in range(waves):
+        wave(v)
+    torch.cuda.synchronize()
+    t = time.time() - t0
+    res = {"points": int(sum(pts)), "points_per_cluster": pts, "nligne": [c["cl"].nligne for c in cls],
+           "slots_per_cluster": list(nslot), "waves": waves, "factor_GB_in_flight": gb,
+           "factor_GB_per_slot": per, "seconds": t, "points_per_s": sum(pts) / t,
+           "workloads": [c["w"].name for c in cls]}
+    log(f"batch: {sum(pts)} points of {len(cls)} clusters in {waves} waves (slots {nslot}, {gb:.0f} GB), "
+        f"{t:.1f}s -> {sum(pts) / t:.2f} points/s")
+    del sv, xs, cls
+    torch.cuda.empty_cache()
+    return res
+
+
+def adjoint_recompute(w, n_energies: int, slots: int):
+    """Adjoint of sigma(E) over n_energies energies of the iron cluster
+    through the torch.autograd.Function (fdmnes_amd.adjoint): forward solves
+    in groups of `slots` (FP64, the last group's factors stay resident, the
+    checkpoint), then the backward pass: transposed substitutions on the
+    resident factors, refactorisation of every other point (recompute)
+    before its transposed substitutions.  Host assembly is inside both
+    passes.  Reports backward / forward time, and from the per-point costs
+    the same ratio for 200 energies with the same checkpoint."""
+    import torch
+    from fdmnes_amd import adjoint as AD
+    from fdmnes_amd import cluster as CL
+    from fdmnes_amd import configs as cfg
+    cl = CL.Cluster(w.R / cfg.BOHR, w.h / cfg.BOHR, w.lmaxso)
+    s = cfg.sources(w, cl.points, cl.nligne)
+    eng = AD.SigmaEngine(cl, s, w.eimag_Ry, slots=slots)
+    E = w.energies_Ry[np.linspace(0, len(w.energies_Ry) - 1, n_energies).round().astype(int)]
+    V = torch.tensor(cfg.potential(w, cl.points), dtype=torch.float64, requires_grad=True)
+    eng.forward(V.detach().numpy(), E[:slots])  # warm-up: graphs
+    torch.cuda.synchronize()
+    t0 = time.time()
+    sig = AD.SigmaFunction.apply(V, eng, E)
+    torch.cuda.synchronize()
+    t_f = time.time() - t0
+    eng.recomputed = 0
+    t0 = time.time()
+    sig.sum().backward()
+    torch.cuda.synchronize()
+    t_b = time.time() - t0
+    nrec = eng.recomputed
+    nres = n_energies - nrec
+    t_fp = t_f / n_energies
+    # 200 energies with the same checkpoint: 200 - slots points recomputed;
+    # the whole backward time is charged to the recomputed points (an upper
+    # bound: the resident points' transposed solves are in it too)
+    proj = (max(200 - slots, 0) * t_b / max(nrec, 1)) / (200 * t_fp)
+    res = {"energies": n_energies, "checkpoint_slots": slots, "recomputed": nrec, "resident": nres,
+           "forward_s": t_f, "backward_s": t_b, "adjoint_over_forward": t_b / t_f,
+           "projected_adjoint_over_forward_200": proj, "grad_norm": float(V.grad.norm())}
+    log(f"adjoint: {n_energies} energies, {slots} checkpointed, {nrec} recomputed: forward {t_f:.1f}s, "
+        f"backward {t_b:.1f}s, ratio {t_b / t_f:.3f} (200 energies: {proj:.3f})")
+    del eng, V, sig
+    torch.cuda.empty_cache()
+    return res
+
+
 def dropin_rates(cl, lb1, lb2, V, s, w):
     """Seconds per energy point through fdmgpu_mat_solve / fdmgpu_mat_solve_mixed
     (host buffers in and out, device set-up per call), the Fortran drop-in's
@@ -338,6 +568,16 @@ def main():
                     help="also time the mirror-symmetric workload: whole cluster against its eight representations")
     ap.add_argument("--dropin", action="store_true",
                     help="also time the host-buffer drop-in entry points (PCIe and set-up included), one point each")
+    ap.add_argument("--fp64-steps", type=int, default=4,
+                    help="timed groups of the FP64 comparison leg (at most --steps), after one warm-up group")
+    ap.add_argument("--adjoint-energies", type=int, default=24,
+                    help="energies of the adjoint leg (forward, then backward with checkpoint/recompute)")
+    ap.add_argument("--adjoint-slots", type=int, default=12, help="checkpointed factor slots of the adjoint leg")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the three-cluster 256-point wave leg")
+    ap.add_argument("--batch-budget-gb", type=float, default=240.0)
+    ap.add_argument("--no-copper", action="store_true", help="skip the copper cluster leg (GPU and CPU sample)")
+    ap.add_argument("--copper-cpu-procs", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs shard energy points "
@@ -371,7 +611,7 @@ def main():
     src = torch.from_numpy(s).to(dev)
     log(f"assembled and uploaded {n_pts} energy points in {time.time() - t0:.1f}s")
 
-    def run(precision, tag):
+    def run(precision, tag, K, W):
         sv = B.DeviceSolver(precision, True, s.shape[1])
         slots = [sv.add_system(lb1, lb2, rp, col) for _ in range(G)]
         xs = [torch.zeros_like(src) for _ in range(G)]
@@ -407,7 +647,7 @@ def main():
 
     tuned = {}
     # mixed precision (headline)
-    sv32, slots32, _, sig32, wall32, sweeps32 = run(B.FP32, "mixed")
+    sv32, slots32, _, sig32, wall32, sweeps32 = run(B.FP32, "mixed", K, W)
     st32 = sv32.stats()
     rate32 = K * G / wall32
     # roofline of the dominant kernel: the FP32 trailing update, timed per
@@ -416,10 +656,22 @@ def main():
     del sv32  # its factor storage makes room for the FP64 solver's
     torch.cuda.empty_cache()
     out = {}
+    # the copper cluster's CPU sample runs on the host cores beside the FP64 legs
+    cu_job = None
+    if not args.no_copper and not args.no_cpu:
+        cu_procs = args.copper_cpu_procs or max(1, min(15, (os.cpu_count() or 2) - 1))
+        log(f"copper CPU sample on {cu_procs} processes (background)")
+        cu_job = copper_cpu_start(cu_procs)
     if not args.no_fp64:
-        sv64, slots64, xs64, sig64, wall64, _ = run(B.FP64, "fp64")
-        rate64 = K * G / wall64
-        err = np.abs(sig32 - sig64) / np.maximum(np.abs(sig64), 1e-300)
+        K64, W64 = min(K, args.fp64_steps), min(W, 1)
+        sv64, slots64, xs64, sig64, wall64, _ = run(B.FP64, "fp64", K64, W64)
+        rate64 = K64 * G / wall64
+        n64 = (W64 + K64) * G  # the FP64 leg solved the first n64 of the mixed leg's points
+        err = np.abs(sig32[:n64] - sig64[:n64]) / np.maximum(np.abs(sig64[:n64]), 1e-300)
+        out["sigma_rel_err_per_energy"] = {"reference": "FP64 GPU solver (pinned to the oracle at <= 1e-10)",
+                                           "energies_eV": [float(energies[k] * cfg.RYDB) for k in idx[:n64]],
+                                           "rel_err": [float(e) for e in err]}
+        out["fp64_steps"], out["fp64_warmup"] = K64, W64
         # adjoint: transposed solves with the stored FP64 factors of the last group
         grads = [torch.zeros(cl.nligne, dtype=torch.float64, device=dev) for _ in range(G)]
         torch.cuda.synchronize()
@@ -427,16 +679,24 @@ def main():
         sv64.adjoint(slots64, [src] * G, xs64, grads)
         torch.cuda.synchronize()
         t_adj = time.time() - a0
-        t_fwd = wall64 / K
+        t_fwd = wall64 / K64
         if not args.no_roofline:
             r64 = roofline(sv64, slots64, vals, src, G, "fp64", "k_trail<double,complex>")
             out["fp64_roofline"] = {k2: r64[k2] for k2 in ("achieved", "peak", "unit", "frac", "avg_launch_us")}
         out.update({"fp64_points_per_s": rate64, "mixed_over_fp64": rate32 / rate64,
-                    "max_rel_sigma_err": float(err.max()), "adjoint_over_forward": t_adj / t_fwd,
+                    "max_rel_sigma_err": float(err.max()), "adjoint_resident_over_forward": t_adj / t_fwd,
                     "adjoint_s_per_group": t_adj, "fp64_ms_per_step": 1e3 * t_fwd})
         if not args.no_tau:
             out["tau"] = tau_rates(xs64, cl, G)
         del sv64, xs64, grads
+        torch.cuda.empty_cache()
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_recompute(w, args.adjoint_energies, args.adjoint_slots)
+        out["adjoint_over_forward"] = out["adjoint"]["adjoint_over_forward"]
+    if not args.no_batch:
+        out["batch"] = batch_waves(args.batch_budget_gb)
+    if not args.no_copper:
+        out["copper"] = copper_gpu(cu_job)
     if not args.no_assembly:
         out["assembly"] = assembly_rates(cl, V, w, G)
     if args.dropin:

@@ -38,6 +38,13 @@ int ensure_init() {
   return fdmgpu_init(0);
 }
 
+// Correction sweeps of the FP64 drop-in: the blocked factorisation applies
+// explicit inverses of its 64 x 64 diagonal tiles, whose rounding on an
+// ill-conditioned undamped system (real arithmetic) exceeds the row-by-row
+// elimination's; refinement with the same factors restores the accuracy.
+constexpr int DROPIN_FP64_SWEEPS = 2;
+constexpr double DROPIN_FP64_TOL = 1e-12;
+
 double now() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -258,10 +265,14 @@ int fdmgpu_mat_solve(int nligne, int nlmso, int cal_comp, const int *lb1, const 
   }
   int rc = ensure_init();
   if (rc) return rc;
-  return cal_comp ? host_solve<double, true>(nligne, nlmso, lb1, lb2, row_ptr, col, a_re, a_im, smr, smi, 0, 0.0,
-                                             nullptr, nullptr, fail_row, t_fill, t_tria)
-                  : host_solve<double, false>(nligne, nlmso, lb1, lb2, row_ptr, col, a_re, a_im, smr, smi, 0, 0.0,
-                                              nullptr, nullptr, fail_row, t_fill, t_tria);
+  // the direct solve, then at most DROPIN_FP64_SWEEPS corrections with the
+  // same FP64 factors while max_l ||r_l|| / ||b_l|| > DROPIN_FP64_TOL
+  return cal_comp ? host_solve<double, true>(nligne, nlmso, lb1, lb2, row_ptr, col, a_re, a_im, smr, smi,
+                                             DROPIN_FP64_SWEEPS, DROPIN_FP64_TOL, nullptr, nullptr, fail_row, t_fill,
+                                             t_tria)
+                  : host_solve<double, false>(nligne, nlmso, lb1, lb2, row_ptr, col, a_re, a_im, smr, smi,
+                                              DROPIN_FP64_SWEEPS, DROPIN_FP64_TOL, nullptr, nullptr, fail_row, t_fill,
+                                              t_tria);
 }
 
 int fdmgpu_mat_solve_mixed(int nligne, int nlmso, int cal_comp, const int *lb1, const int *lb2,

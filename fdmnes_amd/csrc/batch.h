@@ -80,7 +80,7 @@ struct Solver {
  private:
   int factor_phase(uint32_t mask);
   int sweep_phase(uint32_t mask);
-  int fallback_fp64(int slot, int *fail_row);
+  int fallback_fp64(int slot, int *fail_row, int max_sweeps, double tol, double *rel);
   int snap_factors(int count, const int *ids);
   int snap_sweep(const std::vector<int> &slot_ids);
 };

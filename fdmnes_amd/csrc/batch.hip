@@ -139,12 +139,11 @@ int Solver<R, C>::add_system(int n, const int *lb1, const int *lb2, const int64_
     FDM_HIP(hipMalloc(&sl.long_rows, sizeof(int) * lr.size()));
     FDM_HIP(hipMemcpy(sl.long_rows, lr.data(), sizeof(int) * lr.size(), hipMemcpyHostToDevice));
   }
-  if (std::is_same<R, float>::value) {
-    FDM_HIP(hipMalloc(&sl.Xd, sizeof(double) * xe));
-    FDM_HIP(hipMalloc(&sl.Rr, sizeof(double) * xe));
-    FDM_HIP(hipMemset(sl.Rr, 0, sizeof(double) * xe));
-    FDM_HIP(hipMalloc(&sl.scr, sizeof(double) * NORM_GRID * std::max(nrhs, 1)));
-  }
+  // refinement buffers (FP32 factors always refine; FP64 ones on request)
+  FDM_HIP(hipMalloc(&sl.Xd, sizeof(double) * xe));
+  FDM_HIP(hipMalloc(&sl.Rr, sizeof(double) * xe));
+  FDM_HIP(hipMemset(sl.Rr, 0, sizeof(double) * xe));
+  FDM_HIP(hipMalloc(&sl.scr, sizeof(double) * NORM_GRID * std::max(nrhs, 1)));
   if (grp.add(sl.plan.get(), nrhs)) return -2;
   *slot = (int)slots.size();
   slots.push_back(std::move(sl));
@@ -260,8 +259,12 @@ int Solver<R, C>::sweep_phase(uint32_t mask) {
   return 0;
 }
 
+// FP64 factors for one point whose FP32 factorisation or refinement failed:
+// the direct solve, then (as the FP64 solver) at most max_sweeps corrections
+// with the same factors while max_l ||r_l||/||b_l|| > tol; *rel gets the
+// final residual.
 template <class R, bool C>
-int Solver<R, C>::fallback_fp64(int g, int *fail_row) {
+int Solver<R, C>::fallback_fp64(int g, int *fail_row, int max_sweeps, double tol, double *rel) {
   Slot &sl = slots[g];
   if (fb.size() < slots.size()) fb.resize(slots.size());
   if (!fb[g]) {
@@ -282,7 +285,31 @@ int Solver<R, C>::fallback_fp64(int g, int *fail_row) {
   std::vector<int> fr;
   if (f.read_fail(fr)) return -2;
   *fail_row = fr[0];
+  *rel = NAN;
   FDM_HIP(hipMemcpyAsync(sl.Xd, f.m[0].X, sizeof(double) * tot, hipMemcpyDeviceToDevice, stream));
+  if (fr[0]) {
+    FDM_HIP(hipStreamSynchronize(stream));
+    return 0;
+  }
+  const DevCsr a{sl.n, sl.nnz, sl.row_ptr, sl.col, sl.vals, sl.long_rows, sl.nlong};
+  const int scr = NORM_GRID * std::max(nrhs, 1);
+  std::vector<double> bn(nrhs), rn(nrhs);
+  if (col_norms2<C>(sl.B, sl.n, nrhs, bn.data(), sl.scr, scr, stream)) return -2;
+  auto max_rel = [&]() {
+    double mx = 0;
+    for (int l = 0; l < nrhs; l++) mx = std::max(mx, bn[l] > 0 ? std::sqrt(rn[l] / bn[l]) : std::sqrt(rn[l]));
+    return mx;
+  };
+  for (int it = 0;; it++) {
+    if (residual<C>(a, sl.B, sl.Xd, sl.Rr, nrhs, stream) || col_norms2<C>(sl.Rr, sl.n, nrhs, rn.data(), sl.scr, scr,
+                                                                          stream))
+      return -2;
+    *rel = max_rel();
+    if (*rel <= tol || *rel != *rel || it >= max_sweeps) break;
+    if (convert<double>(sl.Rr, f.m[0].X, tot, stream) || f.solve(S_UPPER) || f.solve(S_LOWER) ||
+        accumulate<double>(sl.Xd, f.m[0].X, tot, stream))
+      return -2;
+  }
   FDM_HIP(hipStreamSynchronize(stream));
   return 0;
 }
@@ -338,7 +365,8 @@ int Solver<R, C>::solve(int count, const int *ids, const double *const *vals, co
   if (snapshots && snap_factors(count, ids)) return -2;
   last_sweeps.assign(count, 0);
   last_rel.assign(count, 0.0);
-  if (std::is_same<R, double>::value) {
+  constexpr bool F64 = std::is_same<R, double>::value;
+  if (F64 && max_sweeps <= 0) {  // the direct FP64 solve only
     for (int k = 0; k < count; k++) {
       const int g = ids[k];
       fail_rows[k] = fr[g];
@@ -351,26 +379,54 @@ int Solver<R, C>::solve(int count, const int *ids, const double *const *vals, co
     FDM_HIP(hipStreamSynchronize(stream));
     return 0;
   }
-  // FP32 factors: refinement in FP64 over the members whose factorisation held.
+  // Refinement in FP64 over the members whose factorisation held.  FP32
+  // factors: x = 0, r = b, and every sweep (the first is the direct solve)
+  // solves with the FP32 factors.  FP64 factors: the direct solve already
+  // ran with the factorisation; x = that solution, r = b - A x, and the
+  // sweeps are corrections with the FP64 factors.
   std::vector<int> act;
   for (int k = 0; k < count; k++)
     if (fr[ids[k]] == 0) act.push_back(k);
-  if (!hnorm) FDM_HIP(hipHostMalloc(&hnorm, sizeof(double) * MAXG * NORM_GRID * std::max(nrhs, 1)));
+  if (!hnorm) FDM_HIP(hipHostMalloc(&hnorm, sizeof(double) * 2 * MAXG * NORM_GRID * std::max(nrhs, 1)));
   const size_t np = (size_t)NORM_GRID * nrhs;
+  double *hres = hnorm + (size_t)MAXG * np;  // second half: residual norms of the direct FP64 solve
   std::vector<std::vector<double>> bn(count, std::vector<double>(nrhs)), rn(count, std::vector<double>(nrhs));
   std::vector<double> r(count, INFINITY);
   std::vector<int> sw(count, 0);
+  auto max_rel = [&](int k) {
+    double mx = 0;
+    for (int l = 0; l < nrhs; l++)
+      mx = std::max(mx, bn[k][l] > 0 ? std::sqrt(rn[k][l] / bn[k][l]) : std::sqrt(rn[k][l]));
+    return mx;
+  };
   for (int k : act) {
     Slot &sl = slots[ids[k]];
     const size_t tot = (size_t)sl.plan->NB * TB * nrhs * NC;
     if (launch_norms2<C>(sl.B, sl.n, nrhs, sl.scr, stream)) return -2;
     FDM_HIP(hipMemcpyAsync(hnorm + k * np, sl.scr, sizeof(double) * np, hipMemcpyDeviceToHost, stream));
-    FDM_HIP(hipMemsetAsync(sl.Xd, 0, sizeof(double) * tot, stream));
-    FDM_HIP(hipMemcpyAsync(sl.Rr, sl.B, sizeof(double) * tot, hipMemcpyDeviceToDevice, stream));
+    if (F64) {
+      FDM_HIP(hipMemcpyAsync(sl.Xd, grp.m[ids[k]].X, sizeof(double) * tot, hipMemcpyDeviceToDevice, stream));
+      DevCsr a{sl.n, sl.nnz, sl.row_ptr, sl.col, sl.vals, sl.long_rows, sl.nlong};
+      if (residual<C>(a, sl.B, sl.Xd, sl.Rr, nrhs, stream) || launch_norms2<C>(sl.Rr, sl.n, nrhs, sl.scr, stream))
+        return -2;
+      FDM_HIP(hipMemcpyAsync(hres + k * np, sl.scr, sizeof(double) * np, hipMemcpyDeviceToHost, stream));
+    } else {
+      FDM_HIP(hipMemsetAsync(sl.Xd, 0, sizeof(double) * tot, stream));
+      FDM_HIP(hipMemcpyAsync(sl.Rr, sl.B, sizeof(double) * tot, hipMemcpyDeviceToDevice, stream));
+    }
   }
   FDM_HIP(hipStreamSynchronize(stream));
   for (int k : act) sum_norm_partials(hnorm + k * np, nrhs, bn[k].data());
-  for (int it = 0; !act.empty(); it++) {
+  if (F64) {
+    std::vector<int> keep;
+    for (int k : act) {
+      sum_norm_partials(hres + k * np, nrhs, rn[k].data());
+      r[k] = max_rel(k);
+      if (!(r[k] <= tol) && r[k] == r[k]) keep.push_back(k);
+    }
+    act.swap(keep);
+  }
+  for (int it = F64 ? 1 : 0; !act.empty(); it++) {
     std::vector<int> ids_act;
     uint32_t m = 0;
     for (int k : act) {
@@ -385,10 +441,7 @@ int Solver<R, C>::solve(int count, const int *ids, const double *const *vals, co
     if (snapshots && snap_sweep(ids_act)) return -2;
     for (int k : act) {
       sum_norm_partials(hnorm + k * np, nrhs, rn[k].data());
-      double mx = 0;
-      for (int l = 0; l < nrhs; l++)
-        mx = std::max(mx, bn[k][l] > 0 ? std::sqrt(rn[k][l] / bn[k][l]) : std::sqrt(rn[k][l]));
-      r[k] = mx;
+      r[k] = max_rel(k);
       sw[k] = it;
     }
     std::vector<int> keep;
@@ -396,14 +449,29 @@ int Solver<R, C>::solve(int count, const int *ids, const double *const *vals, co
       if (!(r[k] <= tol) && r[k] == r[k] && it < max_sweeps) keep.push_back(k);
     act.swap(keep);
   }
+  if (F64) {  // no fallback: the refined FP64 solution, with its last residual
+    for (int k = 0; k < count; k++) {
+      const int g = ids[k];
+      fail_rows[k] = fr[g];
+      if (fr[g] == 0)
+        FDM_HIP(hipMemcpyAsync(x[k], slots[g].Xd, sizeof(double) * (size_t)slots[g].n * nrhs * NC,
+                               hipMemcpyDeviceToDevice, stream));
+      if (sweeps) sweeps[k] = sw[k];
+      if (rel) rel[k] = fr[g] == 0 ? r[k] : NAN;
+    }
+    FDM_HIP(hipStreamSynchronize(stream));
+    if (grp.check_solve()) return -2;
+    last_sweeps = sw;
+    last_rel = r;
+    return 0;
+  }
   // members that did not converge (or whose FP32 pivots failed) fall back to FP64
   for (int k = 0; k < count; k++) {
     const int g = ids[k];
     int frow = 0;
     if (fr[g] != 0 || !(r[k] <= tol)) {
-      if (fallback_fp64(g, &frow)) return -2;
+      if (fallback_fp64(g, &frow, max_sweeps, tol, &r[k])) return -2;
       sw[k] = -(std::max(sw[k], 1));
-      r[k] = NAN;
     }
     fail_rows[k] = frow;
     if (frow == 0)

@@ -57,3 +57,9 @@ def mat_solve_mixed(lb1, lb2, row_ptr, col, vals, rhs, cal_comp: bool, max_sweep
     check(rc, fr.value)
     x = smr + 1j * smi if cal_comp else smr
     return x, sw.value, rel.value
+
+
+def finalize():
+    """Releases the drop-in's kept device solvers (fdmgpu_finalize); the next
+    call sets the library up again."""
+    check(_lib.lib().fdmgpu_finalize())

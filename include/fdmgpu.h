@@ -60,6 +60,12 @@ void fdmgpu_solver_name(char out[5]);
  * symbolic pass, the bottom-up elimination with the pivot test and the
  * forward substitution, in FP64, on the GPU.  Solves A x = b for all nlmso
  * right-hand sides; b is read from smr/smi and x written back in place.
+ * The direct solve is followed by at most two correction sweeps with the
+ * same FP64 factors while max_l ||r_l||_2/||b_l||_2 > 1e-12 (residual in
+ * FP64 from the compressed rows): the blocked factorisation's explicit
+ * 64 x 64 diagonal-tile inverses round worse than the reference's row by
+ * row elimination on ill-conditioned real systems, and the corrections
+ * bring the solution back to the reference's accuracy or better.
  * t_fill / t_tria (may be NULL) are incremented by the seconds spent in
  * device assembly and in factorisation + substitution, like the reference's
  * Time_fill / Time_tria accumulators (mat_solve_gaussian.f90:198-315).
@@ -78,7 +84,8 @@ int fdmgpu_mat_solve(int nligne, int nlmso, int cal_comp, const int *lb1, const 
  * FP32 factors.  At most max_sweeps refinement sweeps; stops when
  * max over right-hand sides of ||r||_2/||b||_2 <= tol.  If refinement does
  * not reach tol (growth without pivoting can ruin FP32 factors), the system
- * is refactorised in FP64 and *sweeps is returned negative (-(sweeps done)).
+ * is refactorised in FP64, refined with those factors in the same way, and
+ * *sweeps is returned negative (-(FP32 sweeps done)).
  * No counterpart in the reference (it is FP64 only); same callers as
  * mat_solve.  *final_rel (may be NULL) receives the last residual ratio.
  */
@@ -115,7 +122,12 @@ int fdmgpu_solver_add_system(fdmgpu_solver *s, int nligne, const int *lb1, const
                              const int *col, int *slot);
 /* Solves count points on distinct slots.  FP32 solvers refine in FP64 (at
  * most max_sweeps sweeps to max_l ||r_l||/||b_l|| <= tol) and fall back to
- * FP64 factors per point when refinement does not converge (sweeps[k] < 0).
+ * FP64 factors per point when refinement does not converge (sweeps[k] < 0;
+ * the FP64 solution is refined with its own factors to the same tol).
+ * FP64 solvers with max_sweeps > 0 follow the direct solve by at most
+ * max_sweeps correction sweeps with the same factors while
+ * max_l ||r_l||/||b_l|| > tol (no fallback; sweeps[k] = corrections taken,
+ * rel[k] = the final residual); max_sweeps <= 0: the direct solve only.
  * fail_rows[k] gets the reference's failing row (0 = none); returns
  * FDMGPU_PIVOT if any point failed.  sweeps/rel may be NULL. */
 int fdmgpu_solver_solve(fdmgpu_solver *s, int count, const int *slots, const double *const *d_vals,

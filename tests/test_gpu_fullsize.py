@@ -95,3 +95,44 @@ def test_three_iron_clusters_grouped():
         assert r64 <= 1e-10 and r32 <= 1e-10
     rel = np.abs(out[B.FP32][1] - out[B.FP64][1]) / np.abs(out[B.FP64][1])
     assert rel.max() <= 1e-8, rel
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_copper_tau_matches_oracle_at_four_energies(mixed):
+    """Config 0's observable: the copper cluster's solutions for calcMatRow's
+    incoming-wave right-hand sides, through the host-buffer drop-in (FP64, or
+    FP32 factors with FP64 refinement), reduced on the GPU to tau in the real
+    basis over the nlmso innermost grid rows; tau and sigma_tau = -Im tr tau
+    against the oracle's (tests/golden/cu13_real_tau.npz) at -5, 10, 25 and
+    40 eV, within 1e-10.  The oracle itself is within 1.4e-11 of a pivoted
+    dense solve there (also in the fixture)."""
+    import os
+    import torch
+    from fdmnes_amd import cluster as CL
+    from fdmnes_amd import configs as cfg
+    from fdmnes_amd import solver as SV
+    from fdmnes_amd.tau import tau_reduce
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "cu13_real_tau.npz"))
+    w = cfg.CU_REAL
+    cl = CL.Cluster(w.R / cfg.BOHR, w.h / cfg.BOHR, w.lmaxso)
+    lb1, lb2 = cl.profile()
+    V = cfg.potential(w, cl.points)
+    nso = cl.nligne - cl.nlmso
+    for k, E in zip(g["energy_index"], g["E"]):
+        assert float(w.energies_Ry[k]) == float(E)
+        rp, col, vals, binc = cl.assemble(V, float(E), 0.0, cfg.ecinetic_out(float(E)), False, True)
+        if mixed:
+            x, sweeps, rel = SV.mat_solve_mixed(lb1, lb2, rp, col, vals, binc, False)
+        else:
+            x, _, _ = SV.mat_solve(lb1, lb2, rp, col, vals, binc, False)
+        tau, _, _ = tau_reduce([torch.from_numpy(np.ascontiguousarray(x)).cuda()], nso, True, False,
+                               [(0, cl.nlmso)])
+        t = tau[0].view(cl.nlmso, cl.nlmso).cpu().numpy()
+        ref = g["tau"][list(g["energy_index"]).index(k)]
+        sig = -np.imag(np.trace(t))
+        sref = -np.imag(np.trace(ref))
+        print(f"E={E * cfg.RYDB:.1f} eV mixed={mixed} sigma_tau {sig:.15e} oracle {sref:.15e} "
+              f"rel {abs(sig - sref) / abs(sref):.2e} tau {np.abs(t - ref).max() / np.abs(ref).max():.2e}",
+              flush=True)
+        assert abs(sig - sref) <= 1e-10 * abs(sref)
+        assert np.abs(t - ref).max() <= 1e-10 * np.abs(ref).max()
