@@ -84,7 +84,10 @@ def _cu_worker(k):
     """One whole energy point of the copper cluster (real arithmetic,
     Basereel, calcMatRow's incoming-wave right-hand sides) through the
     oracle's restatement of mat_solve, then its reduction to tau over the
-    block of the nlmso innermost grid rows; returns (k, seconds, sigma_tau)."""
+    block of the nlmso innermost grid rows; also the same reduction of a
+    partial-pivoting dense solve (LAPACK through numpy, untimed), the
+    accuracy reference on this undamped system.  Returns (k, seconds,
+    sigma_tau of the oracle, sigma_tau of the dense solve)."""
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     from fdmnes_amd import configs as cfg
     from oracle import oracle as O
@@ -96,9 +99,12 @@ def _cu_worker(k):
     st, x, fr, tf, tt, _ = cl.solve(V, E, 0.0, cfg.ecinetic_out(E), False, False, True)
     t = time.time() - t0
     if st:
-        return k, t, float("nan")
-    taus, _, info = O.tau(x, cl.nligne - cl.nlmso, True, False, [(0, cl.nlmso)])
-    return k, t, float(-np.imag(np.trace(taus[0])))
+        return k, t, float("nan"), float("nan")
+    nso = cl.nligne - cl.nlmso
+    taus, _, info = O.tau(x, nso, True, False, [(0, cl.nlmso)])
+    A, b = cl.assemble_dense(V, E, 0.0, cfg.ecinetic_out(E), False, False, True)
+    taud, _, _ = O.tau(np.linalg.solve(A, b), nso, True, False, [(0, cl.nlmso)])
+    return k, t, float(-np.imag(np.trace(taus[0]))), float(-np.imag(np.trace(taud[0])))
 
 
 def copper_cpu_start(procs: int):
@@ -108,6 +114,8 @@ def copper_cpu_start(procs: int):
     from fdmnes_amd import configs as cfg
     n = cfg.CU_REAL.n_energies
     ks = sorted({int(round(i * (n - 1) / max(procs - 1, 1))) for i in range(procs)})
+    for v in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS"):
+        os.environ[v] = "1"  # one core per worker process (inherited at spawn)
     pool = mp.get_context("spawn").Pool(len(ks))
     return pool, pool.map_async(_cu_worker, ks)
 
@@ -169,17 +177,21 @@ def copper_gpu(cpu_job):
         out = job.get()
         pool.close()
         pool.join()
-        ts = [t for _, t, _ in out]
-        rel = [abs(sig[k] - sc) / abs(sc) for k, _, sc in out]
+        ts = [t for _, t, _, _ in out]
+        rel = [abs(sig[k] - sc) / abs(sc) for k, _, sc, _ in out]
+        rel_d = [abs(sig[k] - sd) / abs(sd) for k, _, _, sd in out]
+        rel_od = [abs(sc - sd) / abs(sd) for k, _, sc, sd in out]
         res["cpu"] = {"kind": "port", "cores": len(out), "points": len(out),
                       "mean_s_per_point": float(np.mean(ts)), "points_per_s": len(out) / float(np.mean(ts)),
                       "sample": f"{len(out)} of the {len(E)} energies, one whole point (assembly and mat_solve "
                                 f"restatement) per process, concurrent",
-                      "energies_eV": [float(E[k] * cfg.RYDB) for k, _, _ in out],
-                      "sigma_tau_rel_diff_gpu_vs_oracle": [float(r) for r in rel]}
+                      "energies_eV": [float(E[k] * cfg.RYDB) for k, _, _, _ in out],
+                      "sigma_tau_rel_diff_gpu_vs_oracle": [float(r) for r in rel],
+                      "sigma_tau_rel_diff_gpu_vs_pivoted_dense": [float(r) for r in rel_d],
+                      "sigma_tau_rel_diff_oracle_vs_pivoted_dense": [float(r) for r in rel_od]}
         res["gpu_over_cpu"] = res["gpu_fp64_points_per_s"] / res["cpu"]["points_per_s"]
-        log(f"copper CPU: {len(out)} points, {np.mean(ts):.1f} s each; sigma_tau GPU vs oracle max rel "
-            f"{max(rel):.2e}")
+        log(f"copper CPU: {len(out)} points, {np.mean(ts):.1f} s each; sigma_tau max rel diff: GPU vs oracle "
+            f"{max(rel):.2e}, GPU vs pivoted dense {max(rel_d):.2e}, oracle vs pivoted dense {max(rel_od):.2e}")
     return res
 
 

@@ -26,7 +26,9 @@ FLAGS = [
 ]
 FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
 SHIM_LIB = os.path.join(HERE, "libfdmnes_shim.so")
-PROBE_LIB = os.path.join(HERE, "libfdmgpu_probe.so")  # MFMA lane-layout probe for the tests
+# test-only native code: the MFMA lane-layout and XCC probe of tests/test_gpu_mfma_layout.py
+TESTS = os.path.join(HERE, "..", "tests")
+PROBE_LIB = os.path.join(TESTS, "libfdmgpu_probe.so")
 SHIM_SRC = os.path.join(HERE, "fortran", "mat_solve_hip.f90")
 # the shim is compiled inside FDMNES against its module `declarations`;
 # here against the module file of the reference build (oracle/_ref)
@@ -55,7 +57,7 @@ def build(verbose: bool = False) -> str:
         raise RuntimeError("g++ failed building libfdmgpu_host.so")
     os.replace(tmp, HOST_LIB)
     tmp = PROBE_LIB + ".tmp"
-    cmd = [HIPCC, *FLAGS[:-1], "-o", tmp, os.path.join(CSRC, "probe.hip")]
+    cmd = [HIPCC, *FLAGS[:-1], "-o", tmp, os.path.join(TESTS, "probe.hip")]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)

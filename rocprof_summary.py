@@ -4,9 +4,9 @@ that ``rocprofv3 -d DIR -o run`` writes): per-kernel time table of a
 and the HBM traffic of one kernel from a FETCH_SIZE run and a WRITE_SIZE run
 calibrated on a plain copy of known size.
 
-    python -m fdmnes_amd.rocprof_summary trace DIR/run_results.db
-    python -m fdmnes_amd.rocprof_summary pmc DIR/run_results.db KERNEL_SUBSTRING
-    python -m fdmnes_amd.rocprof_summary traffic FETCH_DB WRITE_DB COPY_FETCH_DB COPY_WRITE_DB COPY_BYTES KERNEL
+    python rocprof_summary.py trace DIR/run_results.db
+    python rocprof_summary.py pmc DIR/run_results.db KERNEL_SUBSTRING
+    python rocprof_summary.py traffic FETCH_DB WRITE_DB COPY_FETCH_DB COPY_WRITE_DB COPY_BYTES KERNEL
 """
 from __future__ import annotations
 

@@ -14,7 +14,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 @pytest.mark.parametrize("prec", [0, 1])
 def test_mfma_output_layout(prec):
     import torch  # noqa: F401  (one HIP runtime for the process)
-    lib = ctypes.CDLL(os.path.join(ROOT, "fdmnes_amd", "libfdmgpu_probe.so"))
+    lib = ctypes.CDLL(os.path.join(ROOT, "tests", "libfdmgpu_probe.so"))
     out = np.zeros(256)
     assert lib.fdmgpu_probe_mfma(prec, out.ctypes.data_as(ctypes.c_void_p)) == 0
     A = np.array([[i + 16 * k + 1 for k in range(4)] for i in range(16)], float)
