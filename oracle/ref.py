@@ -215,3 +215,64 @@ def invcomp(A):
     n = a.shape[0]
     lib().invcomp_(A_.i(n), a.ctypes.data_as(C.c_void_p), A_.i(n), A_.i(n), A_.i(0), A_.b(True))
     return a
+
+
+def grid(rsort: float, adimp: float, iord: int = 4, igrpt: int = 1, iopsymr=None):
+    """The reference's grid builders for a cubic grid centred on the
+    absorber, with no atoms (natome = 0): reseau (general.f90:9439: points,
+    radial order, rvol), laplac (:9732: stencil weights and neighbour table)
+    and bordure (:9895: outer-sphere boundary points and weights), whose
+    lists come back through recup_bordure (:10140, which reads the scratch
+    unit bordure writes).  igrpt / iopsymr select the point group
+    (posequiv, :8892); the default is group 1 (P1, the identity only).
+    rsort, adimp in bohr.  Returns a dict of the reference's arrays
+    (0-based point numbers where an index is stored, -1 for none)."""
+    A = _Args()
+    p = lambda a: a.ctypes.data_as(C.c_void_p)
+    eps = 1.0e-4
+    rmax = rsort / adimp + np.sqrt(2.0) * iord / 2 + eps
+    nx = int(np.ceil(rmax)) + 1
+    r = np.arange(-nx, nx + 1)
+    g = np.sqrt(r[:, None, None] ** 2 + r[None, :, None] ** 2 + r[None, None, :] ** 2)
+    npsom = int((g <= rmax).sum()) + 8
+    if iopsymr is None:
+        iopsymr = np.zeros(NOPSM, np.int32)
+        iopsymr[0] = 1
+    indice = np.zeros((npsom, 3), np.int32, order="F")
+    xyz = np.zeros((4, npsom), order="F")
+    rvol = np.zeros(npsom)
+    numia = np.zeros(npsom, np.int32)
+    mpres = np.zeros((2 * nx + 1,) * 3, np.int32, order="F")
+    npoint, npr, npso = np.zeros(1, np.int32), np.zeros(1, np.int32), np.zeros(1, np.int32)
+    lib().reseau_(A.d(adimp), A.b(False), A.d(0.0), A.b(False), A.i(1), A.i(0), A.i(igrpt), p(indice),
+                  A.arr(iopsymr, np.int32), A.i(iord), A.arr([0], np.int32), A.b(False), A.i(0), p(mpres), A.i(0),
+                  A.i(1), A.i(npsom), p(npoint), p(npr), p(npso), A.i(npsom), A.i(0), p(numia), A.i(nx),
+                  A.arr(np.zeros((3, 1)), np.float64), A.arr(np.zeros((3, 1)), np.float64),
+                  A.arr(np.zeros(1), np.float64), A.d(rsort), p(rvol), p(xyz))
+    npoint, npso = int(npoint[0]), int(npso[0])
+    nvois = 3 * iord
+    cgrad = np.zeros(nvois)
+    clapl = np.zeros(nvois + 1)
+    ivois = np.zeros((npsom, nvois), np.int32, order="F")
+    isvois = np.zeros((npsom, nvois), np.int32, order="F")
+    lib().laplac_(A.d(adimp), A.b(False), p(cgrad), p(clapl), A.i(0), A.i(igrpt), p(indice),
+                  A.arr(iopsymr, np.int32), A.i(iord), p(ivois), p(isvois), A.i(0), p(mpres), A.i(npso), A.i(npsom),
+                  A.i(nvois), A.i(nx))
+    nbm, nbtm, nsm, nstm = (np.zeros(1, np.int32) for _ in range(4))
+    unit = 77
+    lib().bordure_(A.b(False), A.i(0), A.arr(iopsymr, np.int32), A.i(iord), A.i(unit), p(ivois), A.i(0), A.i(0),
+                   p(nbm), p(nbtm), A.i(npsom), A.i(npoint), A.i(npso), A.i(npsom), p(nsm), p(nstm), p(numia),
+                   A.i(nvois), A.arr(np.zeros((3, 1)), np.float64), p(rvol), p(xyz))
+    nsm_, nstm_ = max(int(nsm[0]), 1), max(int(nstm[0]), 1)
+    isrt = np.zeros(nstm_, np.int32)
+    poidso = np.zeros(nsm_)
+    nsort, nsortf = np.zeros(1, np.int32), np.zeros(1, np.int32)
+    lib().recup_bordure_(A.arr(np.zeros((1, 1), np.int32), np.int32), A.arr(np.zeros((1, 1), np.int32), np.int32),
+                         A.i(unit), p(isrt), A.i(1), A.i(0), A.i(0), A.arr([0], np.int32), A.arr([0], np.int32),
+                         A.i(max(int(nbm[0]), 1)), A.i(max(int(nbtm[0]), 1)), A.i(nsm_), p(nsort), p(nsortf),
+                         A.i(nstm_), A.arr(np.zeros((1, 1)), np.float64), p(poidso))
+    nsort, nsortf = int(nsort[0]), int(nsortf[0])
+    return dict(npoint=npoint, npso=npso, indice=indice[:npso].copy(), radius=xyz[3, :npso].copy(),
+                xyz=xyz[:3, :npso].T.copy(), rvol=rvol[:npoint].copy(), numia=numia[:npso].copy(),
+                clapl=clapl, cgrad=cgrad, ivois=ivois[:npso].copy() - 1, isvois=isvois[:npso].copy(),
+                nsortf=nsortf, nsort=nsort, isrt=isrt[:nsort].copy() - 1, poidso=poidso[:nsortf].copy())

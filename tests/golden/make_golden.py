@@ -9,6 +9,12 @@ clusters and stores inputs and outputs as .npz fixtures:
                           -> solution smr/smi, for the three branch sets
                              (complex energy; real; complex arithmetic at real E)
   cbessneu / cbessneur (sphere.f90:7226/7356), cylm + ylmcr (general.f90:11606/11692)
+  reseau / laplac / bordure (general.f90:9439/9732/9895, lists read back by
+  recup_bordure)          -> point order, neighbour table, stencil weights,
+                             outer-sphere boundary points and weights, for
+                             point group 1 and for the mirror groups m, mm2
+                             and mmm (wedge, rvol, symmetry operation of
+                             each neighbour)
 The cluster arrays fed to the reference (grid, neighbours, boundary lists,
 Ylmso) come from the oracle's restatement; the fixtures store the profile
 and matrix-relevant inputs so the tests can rebuild each case.
@@ -39,6 +45,34 @@ CASES = [
 ]
 
 
+GRID_CASES = [(1.2, 0.25), (1.6, 0.3), (2.0, 0.25)]
+# point group -> its operations in the reference's numbering (opsym,
+# general.f90:4587): identity 1, two-fold rotations about x, y, z 22-24,
+# inversion 25, mirrors normal to x, y, z 40-42
+GROUP_OPS = {3: [1, 42], 6: [1, 24, 40, 41], 8: [1, 22, 23, 24, 25, 40, 41, 42]}
+
+
+def grid_fixtures():
+    for Ra, h in GRID_CASES:
+        g = R.grid(Ra / BOHR, h / BOHR)
+        np.savez_compressed(os.path.join(HERE, f"grid_p1_R{Ra}_h{h}.npz"), R=Ra, h=h, npoint=g["npoint"],
+                            npso=g["npso"], indice=g["indice"], radius=g["radius"], ivois=g["ivois"],
+                            clapl=g["clapl"], nsortf=g["nsortf"], nsort=g["nsort"], isrt=g["isrt"],
+                            poidso=g["poidso"])
+        print("grid P1", Ra, h, g["npoint"], g["npso"], g["nsortf"], g["nsort"])
+    for grp, ops in GROUP_OPS.items():
+        io = np.zeros(R.NOPSM, np.int32)
+        io[np.array(ops) - 1] = 1
+        Ra, h = 1.6, 0.25
+        g = R.grid(Ra / BOHR, h / BOHR, igrpt=grp, iopsymr=io)
+        npt = g["npoint"]
+        np.savez_compressed(os.path.join(HERE, f"grid_g{grp}_R{Ra}_h{h}.npz"), R=Ra, h=h, group=grp, ops=np.array(ops),
+                            npoint=npt, npso=g["npso"], indice=g["indice"], radius=g["radius"], rvol=g["rvol"],
+                            ivois=g["ivois"][:npt], isvois=g["isvois"][:npt], clapl=g["clapl"], nsortf=g["nsortf"],
+                            nsort=g["nsort"], isrt=g["isrt"], poidso=g["poidso"])
+        print("grid group", grp, npt, g["npso"], g["nsortf"], g["nsort"])
+
+
 def main():
     if not R.available():
         raise SystemExit("oracle/_ref/libfdmnes_ref.so is missing: make -f oracle/Makefile.ref")
@@ -67,6 +101,7 @@ def main():
                         bes_r=np.array([b for b, _ in br]), neu_r=np.array([n for _, n in br]), dirs=dirs,
                         ylm=np.array(ylm))
     print("special functions written")
+    grid_fixtures()
 
 
 if __name__ == "__main__":
