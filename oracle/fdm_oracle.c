@@ -3,14 +3,15 @@
  *
  * Plain-C restatement of the FDMNES finite-difference path, 1-based like the
  * reference so each loop reads against its Fortran source:
- *   reseau    general.f90:9439-9561   grid points, radial exchange sort
+ *   reseau    general.f90:9439-9587   grid points, radial exchange sort, atom points
  *   laplac    general.f90:9732-9871   stencil coefficients, neighbour table
- *   bordure   general.f90:9895-10076  outer-sphere point lists and weights
+ *   bordure   general.f90:9895-10076  outer-sphere and atom border lists, weights
  *   Ylmpt     general.f90:11505-11570, cylm :11606-11688, ylmcr :11692-11720
  *   phiso     mat.f90:739-1028 (non-Rydberg branch :975-994)
  *   cbessneu  sphere.f90:7226-7248, cbessneur sphere.f90:7356-7378
- *   newind    mat.f90:1761-1921
- *   calcMatRow mat.f90:1034-1511 (grid rows :1142-1398, outer rows :1401-1511)
+ *   newind    mat.f90:1761-1921 (atom blocks :1790-1810, :1836-1860)
+ *   calcMatRow mat.f90:1034-1632 (grid rows :1142-1398, atom neighbours
+ *             :1233-1309, outer rows :1401-1511, atom rows :1515-1632)
  *   mat_solve mat_solve_gaussian.f90:67-358
  * Constants follow the reference's module declarations (main.f90:36-83).
  */
@@ -47,7 +48,21 @@ struct orc_cluster {
   int *lso, *mso, *iso; /* (nlmso) */
   int *newp;       /* new(0:npoint) */
   int *lb1, *lb2, *newinv; /* (nligne) */
+  /* atoms (muffin-tin spheres), point group P1, one spin, regular solution */
+  int natome, npr, nbtm, nlmmax;
+  double *posi;    /* posi(3,natome), bohr, absorber at the origin */
+  double *rmt;     /* (natome) */
+  int *lmaxa, *nlmsa, *ianew, *nbord, *nbordf; /* (natome) */
+  int *ibord;      /* ibord(nbtm,natome) */
+  double *poidsa;  /* poidsa(nbtm,natome), the first nbordf(ia) used */
+  double *ylmato;  /* Ylmato(nbtm,nlmmax,natome) */
+  double *phi_re, *phi_im; /* phiato(nbtm,nlmmax,natome) set by orc_cluster_set_phiato */
+  int phi_cplx;    /* nphiato7 == 2 */
 };
+#define IBORD(c, ib, ia) ((c)->ibord[((ia)-1) * (c)->nbtm + (ib)-1])
+#define POIDSA(c, ib, ia) ((c)->poidsa[((ia)-1) * (c)->nbtm + (ib)-1])
+#define YLMATO(c, ib, lm, ia) ((c)->ylmato[(((size_t)(ia)-1) * (c)->nlmmax + (lm)-1) * (c)->nbtm + (ib)-1])
+#define PHIATO(a, c, ib, lm, ia) ((a)[(((size_t)(ia)-1) * (c)->nlmmax + (lm)-1) * (c)->nbtm + (ib)-1])
 
 #define XYZ(c, k, i) ((c)->xyz[4 * ((i)-1) + (k)-1])
 #define IND(c, i, k) ((c)->indice[((k)-1) * (c)->npso + (i)-1])
@@ -116,6 +131,18 @@ static int orc_reseau(orc_cluster *c) {
   /* rvol: with P1 every point is counted once and rvmax = 1, :9582-9651 */
   c->rvol = malloc(sizeof(double) * (c->npoint + 1));
   for (int p = 1; p <= c->npoint; p++) c->rvol[p - 1] = 1.0;
+  /* points inside an atom's sphere, :9563-9577 (posrel with the identity
+   * only: the distance to the atom) */
+  for (int ia = 1; ia <= c->natome; ia++)
+    for (int i = 1; i <= c->npoint; i++) {
+      double u[3] = {XYZ(c, 1, i) - c->posi[3 * (ia - 1)], XYZ(c, 2, i) - c->posi[3 * (ia - 1) + 1],
+                     XYZ(c, 3, i) - c->posi[3 * (ia - 1) + 2]};
+      double dr = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+      if (dr < c->rmt[ia - 1] + EPSPOS) c->numia[i - 1] = ia;
+    }
+  c->npr = c->npoint;
+  for (int i = 1; i <= c->npoint; i++)
+    if (c->numia[i - 1] != 0) c->npr--;
   return 0;
 }
 
@@ -166,19 +193,39 @@ static int orc_laplac(orc_cluster *c) {
 }
 
 /* ---------------------------------------------------------------- bordure */
-/* general.f90:9895-10076 with natome = 0, Green = F. */
+/* general.f90:9895-10076, Green = F, point group P1 (isbord = 1). */
 static int orc_bordure(orc_cluster *c) {
-  const int nv = 2 * c->nvois / c->iord;
+  const int nv = 2 * c->nvois / c->iord, na = c->natome;
   int nsortf = 0, iss = 0;
+  int *ibp = calloc((size_t)na + 1, sizeof(int));
+  c->nbordf = calloc((size_t)na + 1, sizeof(int));
+  c->nbord = calloc((size_t)na + 1, sizeof(int));
+  if (!ibp || !c->nbordf || !c->nbord) return -1;
   for (int i = 1; i <= c->npoint; i++) { /* :9924-9938 */
     if (c->numia[i - 1] != 0) continue;
     for (int iv = 1; iv <= nv; iv++) {
       int j = IVOIS(c, i, iv);
       if (j == 0) continue;
-      if (c->numia[j - 1] == -2) {
+      int ia = c->numia[j - 1];
+      if (ia == -2) {
         if (iss != i) nsortf++;
         iss = i;
+      } else if (ia > 0) {
+        if (ibp[ia - 1] != i) c->nbordf[ia - 1]++;
+        ibp[ia - 1] = i;
       }
+    }
+  }
+  for (int ia = 1; ia <= na; ia++) c->nbord[ia - 1] = c->nbordf[ia - 1];
+  for (int i = 1; i <= c->npoint; i++) { /* :9940-9959 */
+    int ia = c->numia[i - 1];
+    if (ia < 1) continue;
+    for (int iv = 1; iv <= c->nvois; iv++) {
+      int j = IVOIS(c, i, iv);
+      if (j == 0) return -1; /* the reference stops */
+      if (c->numia[j - 1] != 0) continue;
+      c->nbord[ia - 1]++;
+      break;
     }
   }
   int nsort = nsortf;
@@ -190,24 +237,55 @@ static int orc_bordure(orc_cluster *c) {
       nsort++;
       break;
     }
+  c->nbtm = 1;
+  for (int ia = 1; ia <= na; ia++)
+    if (c->nbord[ia - 1] > c->nbtm) c->nbtm = c->nbord[ia - 1];
   c->isrt = calloc((size_t)nsort + 1, sizeof(int));
   c->poidso = calloc((size_t)nsortf + 1, sizeof(double));
-  /* second pass, :9987-10013 (iss is not reset, as in the reference) */
+  c->ibord = calloc((size_t)c->nbtm * (na > 0 ? na : 1), sizeof(int));
+  c->poidsa = calloc((size_t)c->nbtm * (na > 0 ? na : 1), sizeof(double));
+  double *pdharm = calloc((size_t)na + 1, sizeof(double));
+  if (!c->isrt || !c->poidso || !c->ibord || !c->poidsa || !pdharm) return -1;
+  /* second pass, :9987-10039 (iss is not reset, as in the reference) */
   nsortf = 0;
   double pdsort = 0;
+  for (int ia = 1; ia <= na; ia++) {
+    c->nbordf[ia - 1] = 0;
+    ibp[ia - 1] = 0;
+  }
   for (int i = 1; i <= c->npoint; i++) {
     if (c->numia[i - 1] != 0) continue;
     for (int iv = 1; iv <= nv; iv++) {
       int j = IVOIS(c, i, iv);
       if (j == 0) continue;
-      if (c->numia[j - 1] == -2) {
+      int ia = c->numia[j - 1];
+      if (ia == -2) {
         if (iss != i) {
           pdsort += c->rvol[i - 1];
           nsortf++;
           c->isrt[nsortf - 1] = i;
         }
         iss = i;
+      } else if (ia > 0) {
+        if (ibp[ia - 1] != i) {
+          pdharm[ia - 1] += c->rvol[i - 1];
+          c->nbordf[ia - 1]++;
+          IBORD(c, c->nbordf[ia - 1], ia) = i;
+        }
+        ibp[ia - 1] = i;
       }
+    }
+  }
+  for (int ia = 1; ia <= na; ia++) c->nbord[ia - 1] = c->nbordf[ia - 1];
+  for (int i = 1; i <= c->npoint; i++) { /* :10015-10039 */
+    int ia = c->numia[i - 1];
+    if (ia < 1) continue;
+    for (int iv = 1; iv <= c->nvois; iv++) {
+      int j = IVOIS(c, i, iv);
+      if (c->numia[j - 1] != 0) continue;
+      c->nbord[ia - 1]++;
+      IBORD(c, c->nbord[ia - 1], ia) = i;
+      break;
     }
   }
   nsort = nsortf;
@@ -220,16 +298,31 @@ static int orc_bordure(orc_cluster *c) {
       c->isrt[nsort - 1] = i;
       break;
     }
+  for (int ia = 1; ia <= na; ia++) { /* :10063-10068 */
+    if (c->nbordf[ia - 1] == 0) continue;
+    double fac = 4 * PI / pdharm[ia - 1];
+    for (int ib = 1; ib <= c->nbordf[ia - 1]; ib++) POIDSA(c, ib, ia) = fac * c->rvol[IBORD(c, ib, ia) - 1];
+  }
   if (nsortf > 0) { /* :10070-10076 */
     double fac = 4 * PI / pdsort;
     for (int ib = 1; ib <= nsortf; ib++) c->poidso[ib - 1] = fac * c->rvol[c->isrt[ib - 1] - 1];
   }
   c->nsortf = nsortf;
   c->nsort = nsort;
+  free(ibp);
+  free(pdharm);
   return 0;
 }
 
 /* ---------------------------------------------------------------- Ylm */
+/* cmplx(a, b) / d with d real: Fortran promotes d to complex and the
+ * reference build (flang -O2) divides as (a c + b e, b c - a e) / (c^2 + e^2)
+ * with c = d, e = 0, two roundings where a/d, b/d would take one. */
+static double complex fdiv_by_real(double a, double b, double d) {
+  const double den = d * d + 0.0 * 0.0;
+  return CMPLX((a * d + b * 0.0) / den, (b * d - a * 0.0) / den);
+}
+
 /* cylm, general.f90:11606-11688: complex Y(L,m), m >= 0, index L(L+1)/2+1+m. */
 static void orc_cylm(int lmax, const double *v, double r, double complex *ylmc) {
   double cos_t, sin_t = 0, cot_t = 0;
@@ -237,9 +330,9 @@ static void orc_cylm(int lmax, const double *v, double r, double complex *ylmc) 
   cos_t = (r > EPS6) ? v[2] / r : 1.0;
   double den = sqrt(v[0] * v[0] + v[1] * v[1]);
   if (den > EPS6) {
-    exphi = CMPLX(v[0], v[1]) / den;
+    exphi = fdiv_by_real(v[0], v[1], den);
     sin_t = sqrt(1 - cos_t * cos_t);
-    sin_t_exp_phi = CMPLX(v[0], v[1]) / r;
+    sin_t_exp_phi = fdiv_by_real(v[0], v[1], r);
     cot_t = cos_t / sin_t;
   } else {
     exphi = 1.0;
@@ -315,13 +408,33 @@ static void orc_ylmpt(orc_cluster *c) {
     for (int lm = 1; lm <= nlmr; lm++) YLMSO(c, ib, lm) = y[lm - 1];
   }
   free(y);
+  /* the atoms (ia > 0), :11533-11568: every border point, its direction
+   * from the atom (posrel with the identity, rot_atom = 1) */
+  c->nlmmax = 1;
+  for (int ia = 1; ia <= c->natome; ia++)
+    if (c->nlmsa[ia - 1] > c->nlmmax) c->nlmmax = c->nlmsa[ia - 1];
+  c->ylmato = calloc((size_t)c->nbtm * c->nlmmax * (c->natome > 0 ? c->natome : 1), sizeof(double));
+  y = malloc(sizeof(double) * c->nlmmax);
+  for (int ia = 1; ia <= c->natome; ia++)
+    for (int ib = 1; ib <= c->nbord[ia - 1]; ib++) {
+      int i = IBORD(c, ib, ia);
+      double v[3] = {XYZ(c, 1, i) - c->posi[3 * (ia - 1)], XYZ(c, 2, i) - c->posi[3 * (ia - 1) + 1],
+                     XYZ(c, 3, i) - c->posi[3 * (ia - 1) + 2]};
+      double r = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+      orc_ylm_real(c->lmaxa[ia - 1], v, r, y);
+      for (int lm = 1; lm <= c->nlmsa[ia - 1]; lm++) YLMATO(c, ib, lm, ia) = y[lm - 1];
+    }
+  free(y);
 }
 
 /* ---------------------------------------------------------------- newind */
-/* mat.f90:1761-1921, natome = 0, nspino = 1, regular solution. */
+/* mat.f90:1761-1921, nspino = 1, regular solution (n_ato_RI = 1); the atom
+ * blocks are placed in the radial order at their distance distai(ia) from
+ * the absorber (:1790-1810). */
 static void orc_newind(orc_cluster *c) {
-  const int nspino = 1;
-  c->newp = calloc((size_t)c->npoint + 1, sizeof(int));
+  const int nspino = 1, na = c->natome;
+  c->newp = calloc((size_t)c->npoint + 2, sizeof(int));
+  c->ianew = calloc((size_t)na + 1, sizeof(int));
   int ii = 0;
   for (int i = 1; i <= c->npoint; i++) {
     if (c->numia[i - 1] == 0) {
@@ -330,6 +443,22 @@ static void orc_newind(orc_cluster *c) {
     } else {
       c->newp[i] = 0;
     }
+  }
+  int npf = 1, npf1 = 0;
+  for (int ia = 1; ia <= na; ia++) { /* :1790-1810 */
+    const double *p = c->posi + 3 * (ia - 1);
+    const double distai = sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+    int i;
+    for (i = npf; i <= c->npoint; i++)
+      if (c->numia[i - 1] == 0 && XYZ(c, 4, i) >= distai) break;
+    npf = i;
+    if (npf1 == npf)
+      c->ianew[ia - 1] = c->ianew[ia - 2] + c->nlmsa[ia - 2];
+    else
+      c->ianew[ia - 1] = c->newp[npf] - nspino;
+    npf1 = npf;
+    for (i = npf; i <= c->npoint; i++)
+      if (c->numia[i - 1] == 0) c->newp[i] += c->nlmsa[ia - 1];
   }
   const int nligne = c->nligne;
   c->lb1 = malloc(sizeof(int) * (nligne + 1));
@@ -340,7 +469,7 @@ static void orc_newind(orc_cluster *c) {
     c->lb2[ii] = ii;
   }
   const int n = nspino - 1;
-  for (int i = 1; i <= c->npoint; i++) {
+  for (int i = 1; i <= c->npoint; i++) { /* :1820-1861 */
     if (c->numia[i - 1] != 0) continue;
     ii = c->newp[i];
     for (int iv = 1; iv <= c->nvois; iv++) {
@@ -350,12 +479,28 @@ static void orc_newind(orc_cluster *c) {
         int nj = c->newp[j];
         if (nj - n < c->lb1[ii]) c->lb1[ii] = nj - n;
         if (nj > c->lb2[ii]) c->lb2[ii] = nj;
+      } else if (ia > 0) {
+        if (c->ianew[ia - 1] + 1 < c->lb1[ii]) c->lb1[ii] = c->ianew[ia - 1] + 1;
+        if (c->ianew[ia - 1] + c->nlmsa[ia - 1] > c->lb2[ii]) c->lb2[ii] = c->ianew[ia - 1] + c->nlmsa[ia - 1];
       } else if (ia == -2) {
         c->lb2[ii] = nligne;
       }
     }
   }
-  int lb11 = nligne - c->nlmso + 1; /* :1871-1880 */
+  for (int ia = 1; ia <= na; ia++) { /* :1863-1875 */
+    int lb11 = c->ianew[ia - 1] + 1, lb22 = c->ianew[ia - 1] + c->nlmsa[ia - 1];
+    for (int ib = 1; ib <= c->nbordf[ia - 1]; ib++) {
+      int r = c->newp[IBORD(c, ib, ia)];
+      if (r - n < lb11) lb11 = r - n;
+      if (r > lb22) lb22 = r;
+    }
+    for (int L = 1; L <= c->nlmsa[ia - 1]; L++) {
+      ii = c->ianew[ia - 1] + L;
+      c->lb1[ii] = lb11;
+      c->lb2[ii] = lb22;
+    }
+  }
+  int lb11 = nligne - c->nlmso + 1; /* :1877-1886 */
   for (int ib = 1; ib <= c->nsortf; ib++) {
     int r = c->newp[c->isrt[ib - 1]] - n;
     if (r < lb11) lb11 = r;
@@ -364,21 +509,39 @@ static void orc_newind(orc_cluster *c) {
     c->lb1[ii] = lb11;
     c->lb2[ii] = nligne;
   }
-  for (ii = nligne - 1; ii >= 1; ii--) /* :1882-1884 */
+  for (ii = nligne - 1; ii >= 1; ii--) /* :1888-1890 */
     if (c->lb1[ii + 1] < c->lb1[ii]) c->lb1[ii] = c->lb1[ii + 1];
   for (int i = 1; i <= c->npoint; i++) {
     ii = c->newp[i];
     if (ii != 0) c->newinv[ii] = i;
   }
+  for (int ia = 1; ia <= na; ia++)
+    for (int Lm = 1; Lm <= c->nlmsa[ia - 1]; Lm++) c->newinv[c->ianew[ia - 1] + Lm] = -ia;
 }
 
-orc_cluster *orc_cluster_new(double rsort, double adimp, int iord, int lmaxso) {
+orc_cluster *orc_cluster_new_atoms(double rsort, double adimp, int iord, int lmaxso, int natome, const double *posi,
+                                   const double *rmt, const int *lmaxa) {
   orc_cluster *c = calloc(1, sizeof(orc_cluster));
   if (!c) return NULL;
   c->rsort = rsort;
   c->adimp = adimp;
   c->iord = iord;
   c->lmaxso = lmaxso;
+  c->natome = natome;
+  c->posi = calloc((size_t)3 * natome + 1, sizeof(double));
+  c->rmt = calloc((size_t)natome + 1, sizeof(double));
+  c->lmaxa = calloc((size_t)natome + 1, sizeof(int));
+  c->nlmsa = calloc((size_t)natome + 1, sizeof(int));
+  if (!c->posi || !c->rmt || !c->lmaxa || !c->nlmsa) {
+    orc_cluster_free(c);
+    return NULL;
+  }
+  for (int ia = 0; ia < natome; ia++) {
+    for (int k = 0; k < 3; k++) c->posi[3 * ia + k] = posi[3 * ia + k];
+    c->rmt[ia] = rmt[ia];
+    c->lmaxa[ia] = lmaxa[ia];
+    c->nlmsa[ia] = (lmaxa[ia] + 1) * (lmaxa[ia] + 1); /* one spin: all (L, m) of L <= lmax */
+  }
   if (orc_reseau(c) || orc_laplac(c) || orc_bordure(c)) {
     orc_cluster_free(c);
     return NULL;
@@ -397,10 +560,16 @@ orc_cluster *orc_cluster_new(double rsort, double adimp, int iord, int lmaxso) {
       c->mso[lm] = m;
       c->iso[lm] = 1;
     }
-  c->nligne = c->npoint + c->nlmso; /* mat.f90:150, npr = npoint */
+  int nat = 0; /* mat.f90:150 */
+  for (int ia = 0; ia < natome; ia++) nat += c->nlmsa[ia];
+  c->nligne = c->npr + nat + c->nlmso;
   c->nligneso = c->nligne - c->nlmso;
   orc_newind(c);
   return c;
+}
+
+orc_cluster *orc_cluster_new(double rsort, double adimp, int iord, int lmaxso) {
+  return orc_cluster_new_atoms(rsort, adimp, iord, lmaxso, 0, NULL, NULL, NULL);
 }
 
 void orc_cluster_free(orc_cluster *c) {
@@ -408,12 +577,45 @@ void orc_cluster_free(orc_cluster *c) {
   free(c->xyz); free(c->indice); free(c->numia); free(c->rvol); free(c->ivois);
   free(c->isrt); free(c->poidso); free(c->ylmso); free(c->lso); free(c->mso); free(c->iso);
   free(c->newp); free(c->lb1); free(c->lb2); free(c->newinv);
+  free(c->posi); free(c->rmt); free(c->lmaxa); free(c->nlmsa); free(c->ianew); free(c->nbord); free(c->nbordf);
+  free(c->ibord); free(c->poidsa); free(c->ylmato); free(c->phi_re); free(c->phi_im);
   free(c);
 }
 
 void orc_cluster_sizes(const orc_cluster *c, int *s) {
   s[0] = c->npoint; s[1] = c->npso; s[2] = c->nligne; s[3] = c->nlmso;
   s[4] = c->nsortf; s[5] = c->nsort; s[6] = c->nvois;
+}
+void orc_cluster_atom_sizes(const orc_cluster *c, int *s) {
+  s[0] = c->natome; s[1] = c->npr; s[2] = c->nbtm; s[3] = c->nlmmax;
+}
+void orc_cluster_atom(const orc_cluster *c, int ia, int *info, int *ibord, double *poidsa, double *ylmato) {
+  info[0] = c->nbord[ia - 1]; info[1] = c->nbordf[ia - 1]; info[2] = c->nlmsa[ia - 1]; info[3] = c->ianew[ia - 1];
+  for (int ib = 1; ib <= c->nbord[ia - 1]; ib++) {
+    if (ibord) ibord[ib - 1] = IBORD(c, ib, ia);
+    if (poidsa) poidsa[ib - 1] = ib <= c->nbordf[ia - 1] ? POIDSA(c, ib, ia) : 0.0;
+    if (ylmato)
+      for (int lm = 1; lm <= c->nlmsa[ia - 1]; lm++)
+        ylmato[(size_t)(ib - 1) * c->nlmsa[ia - 1] + lm - 1] = YLMATO(c, ib, lm, ia);
+  }
+}
+void orc_cluster_numia(const orc_cluster *c, int *numia) { memcpy(numia, c->numia, sizeof(int) * c->npso); }
+int orc_cluster_set_phiato(orc_cluster *c, int cplx, const double *re, const double *im) {
+  const size_t n = (size_t)c->nbtm * c->nlmmax * (c->natome > 0 ? c->natome : 1);
+  free(c->phi_re);
+  free(c->phi_im);
+  c->phi_re = calloc(n, sizeof(double));
+  c->phi_im = calloc(n, sizeof(double));
+  if (!c->phi_re || !c->phi_im) return -1;
+  c->phi_cplx = cplx;
+  size_t q = 0; /* input: per atom, per border point ib, nlmsa(ia) harmonics */
+  for (int ia = 1; ia <= c->natome; ia++)
+    for (int ib = 1; ib <= c->nbord[ia - 1]; ib++)
+      for (int lm = 1; lm <= c->nlmsa[ia - 1]; lm++, q++) {
+        PHIATO(c->phi_re, c, ib, lm, ia) = re[q];
+        PHIATO(c->phi_im, c, ib, lm, ia) = (cplx && im) ? im[q] : 0.0;
+      }
+  return 0;
 }
 void orc_cluster_xyz(const orc_cluster *c, double *xyz) { memcpy(xyz, c->xyz, sizeof(double) * 4 * c->npso); }
 void orc_cluster_indice(const orc_cluster *c, int *ind) { memcpy(ind, c->indice, sizeof(int) * 3 * c->npso); }
@@ -551,6 +753,25 @@ static void orc_calcmatrow(const row_ctx *x, int ii, double *abvr, double *abvi,
         int jj = c->newp[j];
         abvr[jj] = abvr[jj] + cfvs_r / c->rvol[j - 1];
         if (cal) abvi[jj] = abvi[jj] + cfvs_i / c->rvol[j - 1];
+      } else if (ia > 0) { /* neighbour inside atom ia, :1233-1309 (isym = 1) */
+        int ib;
+        for (ib = c->nbordf[ia - 1] + 1; ib <= c->nbord[ia - 1]; ib++)
+          if (j == IBORD(c, ib, ia)) break;
+        for (int Lm = 1; Lm <= c->nlmsa[ia - 1]; Lm++) {
+          int jjj = c->ianew[ia - 1] + Lm;
+          /* Lato/mato in (L, m) order: Lm0 = L**2 + L + 1 + m = Lm */
+          const double pr = PHIATO(c->phi_re, c, ib, Lm, ia), pi = PHIATO(c->phi_im, c, ib, Lm, ia);
+          if (c->phi_cplx) {
+            double complex cfac = CMPLX(cfvs_r, cfvs_i) * CMPLX(pr, pi);
+            abvr[jjj] = abvr[jjj] + creal(cfac);
+            abvi[jjj] = abvi[jjj] + cimag(cfac);
+          } else if (cal) {
+            abvr[jjj] = abvr[jjj] + cfvs_r * pr;
+            abvi[jjj] = abvi[jjj] + cfvs_i * pr;
+          } else {
+            abvr[jjj] = abvr[jjj] + cfvs_r * pr;
+          }
+        }
       } else if (ia == -2) { /* :1312-1398 */
         int ib;
         for (ib = c->nsortf + 1; ib <= c->nsort; ib++)
@@ -649,6 +870,40 @@ static void orc_calcmatrow(const row_ctx *x, int ii, double *abvr, double *abvi,
         }
       }
     }
+  } else {
+    /* Expansion in atom ia, :1515-1632 (regular solution, Ylm_comp = F,
+     * Kar = 1, one spin: ispt = isol = 1) */
+    const int ia = -i;
+    const int Lm = ii - c->ianew[ia - 1]; /* Lm0 = Lm with (L, m) ordering */
+    for (int ib = 1; ib <= c->nbordf[ia - 1]; ib++) {
+      int j = IBORD(c, ib, ia);
+      int jj = c->newp[j];
+      if (cal) {
+        double complex cfac = POIDSA(c, ib, ia) * conj(CMPLX(YLMATO(c, ib, Lm, ia), 0.0)) * 1.0 / c->rvol[j - 1];
+        abvr[jj] = abvr[jj] - creal(cfac);
+        abvi[jj] = abvi[jj] - cimag(cfac);
+      } else {
+        double f = POIDSA(c, ib, ia) * 1.0 / c->rvol[j - 1];
+        abvr[jj] = abvr[jj] - f * YLMATO(c, ib, Lm, ia);
+      }
+    }
+    for (int Lmp = 1; Lmp <= c->nlmsa[ia - 1]; Lmp++) {
+      double charmr = 0, charmi = 0;
+      for (int ib = 1; ib <= c->nbordf[ia - 1]; ib++) {
+        const double pr = PHIATO(c->phi_re, c, ib, Lmp, ia), pi = PHIATO(c->phi_im, c, ib, Lmp, ia);
+        if (cal) {
+          double complex cfac = c->phi_cplx ? CMPLX(pr, pi) : CMPLX(pr, 0.0);
+          cfac = cfac * YLMATO(c, ib, Lm, ia);
+          charmr = charmr + POIDSA(c, ib, ia) * creal(cfac);
+          charmi = charmi + POIDSA(c, ib, ia) * cimag(cfac);
+        } else {
+          charmr = charmr + POIDSA(c, ib, ia) * YLMATO(c, ib, Lm, ia) * pr;
+        }
+      }
+      int ljj = c->ianew[ia - 1] + Lmp;
+      abvr[ljj] = abvr[ljj] + charmr;
+      if (cal) abvi[ljj] = abvi[ljj] + charmi;
+    }
   }
 }
 
@@ -663,10 +918,16 @@ static void orc_mark_row(const orc_cluster *c, int ii, unsigned char *colnz) {
       int ia = c->numia[j - 1];
       if (ia == 0) {
         colnz[c->newp[j]] = 1;
+      } else if (ia > 0) {
+        for (int k = c->ianew[ia - 1] + 1; k <= c->ianew[ia - 1] + c->nlmsa[ia - 1]; k++) colnz[k] = 1;
       } else if (ia == -2) {
         for (int k = c->nligneso + 1; k <= c->nligneso + c->nlmso; k++) colnz[k] = 1;
       }
     }
+  } else if (i < 0) { /* mat_solve_gaussian.f90:123-137 */
+    const int ia = -i;
+    for (int k = c->ianew[ia - 1] + 1; k <= c->ianew[ia - 1] + c->nlmsa[ia - 1]; k++) colnz[k] = 1;
+    for (int ib = 1; ib <= c->nbordf[ia - 1]; ib++) colnz[c->newp[IBORD(c, ib, ia)]] = 1;
   } else if (i == 0) {
     for (int k = c->nligneso + 1; k <= c->nligneso + c->nlmso; k++) colnz[k] = 1;
     int lm = ii - c->nligneso;

@@ -31,6 +31,28 @@ typedef struct orc_cluster orc_cluster;
  * grid step adimp (bohr), finite-difference order iord (2 or 4) and outer
  * expansion lmaxso.  Returns NULL on failure. */
 orc_cluster *orc_cluster_new(double rsort, double adimp, int iord, int lmaxso);
+/* The same cluster with natome muffin-tin atoms (point group P1, one spin,
+ * regular solution): posi[3*natome] positions (bohr, absorber at the
+ * origin, sorted by distance), rmt[natome] sphere radii (bohr), lmaxa[natome]
+ * their expansions ((lmax+1)^2 unknowns each, (L, m) order).  Grid points
+ * within rmt of an atom are inside it (reseau :9563-9577); border lists and
+ * weights follow bordure, the Ylm at the border points Ylmpt, the rows
+ * newind with the atom blocks in the radial order.  Before a solve or an
+ * assembly the atoms' phiato must be set (orc_cluster_set_phiato). */
+orc_cluster *orc_cluster_new_atoms(double rsort, double adimp, int iord, int lmaxso, int natome, const double *posi,
+                                   const double *rmt, const int *lmaxa);
+/* s[0..3] = natome, npr (grid rows), nbtm (largest border), nlmmax */
+void orc_cluster_atom_sizes(const orc_cluster *c, int *s);
+/* atom ia (1-based): info[0..3] = nbord, nbordf, nlmsa, ianew; ibord[nbord]
+ * (1-based points: the nbordf outside points next to the sphere, then the
+ * inside points next to the outside), poidsa[nbord] (0 past nbordf),
+ * ylmato[nbord * nlmsa] (Ylmato(ib, Lm), Lm fastest).  Arrays may be NULL. */
+void orc_cluster_atom(const orc_cluster *c, int ia, int *info, int *ibord, double *poidsa, double *ylmato);
+/* numia[npso]: 0 interstitial, ia inside atom ia, -2 outside the sphere */
+void orc_cluster_numia(const orc_cluster *c, int *numia);
+/* phiato of every atom: per atom, per border point ib (its nbord points),
+ * nlmsa values (Lm fastest): re[], and im[] when cplx (nphiato7 = 2). */
+int orc_cluster_set_phiato(orc_cluster *c, int cplx, const double *re, const double *im);
 void orc_cluster_free(orc_cluster *c);
 
 /* sizes: [0]=npoint [1]=npso [2]=nligne [3]=nlmso [4]=nsortf [5]=nsort [6]=nvois */

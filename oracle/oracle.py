@@ -35,6 +35,13 @@ def lib():
         L = C.CDLL(LIB_PATH)
         L.orc_cluster_new.restype = C.c_void_p
         L.orc_cluster_new.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int]
+        L.orc_cluster_new_atoms.restype = C.c_void_p
+        L.orc_cluster_new_atoms.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, _dp, _dp, _ip]
+        L.orc_cluster_atom_sizes.argtypes = [C.c_void_p, _ip]
+        L.orc_cluster_atom.argtypes = [C.c_void_p, C.c_int, _ip, _ip, _dp, _dp]
+        L.orc_cluster_numia.argtypes = [C.c_void_p, _ip]
+        L.orc_cluster_set_phiato.restype = C.c_int
+        L.orc_cluster_set_phiato.argtypes = [C.c_void_p, C.c_int, _dp, C.c_void_p]
         L.orc_cluster_free.argtypes = [C.c_void_p]
         L.orc_cluster_sizes.argtypes = [C.c_void_p, _ip]
         L.orc_cluster_xyz.argtypes = [C.c_void_p, _dp]
@@ -82,9 +89,20 @@ class Cluster:
 
     ``rsort`` and ``adimp`` in bohr (reseau, general.f90:9439)."""
 
-    def __init__(self, rsort: float, adimp: float, lmaxso: int, iord: int = 4):
+    def __init__(self, rsort: float, adimp: float, lmaxso: int, iord: int = 4, atoms=None):
+        """atoms: None, or (posi (natome, 3) bohr sorted by distance, rmt
+        (natome,) bohr, lmax (natome,)) for muffin-tin atoms (P1)."""
         L = lib()
-        self._h = L.orc_cluster_new(rsort, adimp, iord, lmaxso)
+        if atoms is None:
+            self._h = L.orc_cluster_new(rsort, adimp, iord, lmaxso)
+            self.natome = 0
+        else:
+            posi, rmt, lmax = atoms
+            posi = np.ascontiguousarray(posi, np.float64).reshape(-1, 3)
+            self.natome = len(posi)
+            self._h = L.orc_cluster_new_atoms(rsort, adimp, iord, lmaxso, self.natome, posi.reshape(-1),
+                                              np.ascontiguousarray(rmt, np.float64),
+                                              np.ascontiguousarray(lmax, np.int32))
         if not self._h:
             raise MemoryError("orc_cluster_new failed")
         s = np.zeros(7, np.int32)
@@ -97,6 +115,44 @@ class Cluster:
         if getattr(self, "_h", None):
             lib().orc_cluster_free(self._h)
             self._h = None
+
+    @property
+    def numia(self):
+        a = np.zeros(self.npso, np.int32)
+        lib().orc_cluster_numia(self._h, a)
+        return a
+
+    def atom(self, ia: int) -> dict:
+        """Atom ia (1-based): nbord, nbordf, nlmsa, ianew, ibord (1-based
+        points), poidsa, ylmato (nbord, nlmsa)."""
+        L = lib()
+        s = np.zeros(4, np.int32)
+        L.orc_cluster_atom_sizes(self._h, s)
+        nbtm, nlmmax = int(s[2]), int(s[3])
+        info = np.zeros(4, np.int32)
+        ib = np.zeros(nbtm, np.int32)
+        pa = np.zeros(nbtm)
+        ya = np.zeros(nbtm * nlmmax)
+        L.orc_cluster_atom(self._h, ia, info, ib, pa, ya)
+        nb, nbf, nl, ianew = map(int, info)
+        return dict(nbord=nb, nbordf=nbf, nlmsa=nl, ianew=ianew, ibord=ib[:nb].copy(), poidsa=pa[:nb].copy(),
+                    ylmato=ya[: nb * nl].reshape(nb, nl).copy())
+
+    @property
+    def npr(self):
+        s = np.zeros(4, np.int32)
+        lib().orc_cluster_atom_sizes(self._h, s)
+        return int(s[1])
+
+    def set_phiato(self, phi):
+        """phi: list over atoms of (nbord, nlmsa) arrays, complex (nphiato7 = 2)
+        or real."""
+        cplx = any(np.iscomplexobj(p) for p in phi)
+        flat = np.concatenate([np.asarray(p).reshape(-1) for p in phi]) if phi else np.zeros(1)
+        re = np.ascontiguousarray(np.real(flat), np.float64)
+        im = np.ascontiguousarray(np.imag(flat), np.float64) if cplx else None
+        if lib().orc_cluster_set_phiato(self._h, int(cplx), re, _ptr(im)):
+            raise MemoryError("orc_cluster_set_phiato failed")
 
     @property
     def xyz(self):

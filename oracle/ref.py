@@ -276,3 +276,188 @@ def grid(rsort: float, adimp: float, iord: int = 4, igrpt: int = 1, iopsymr=None
                 xyz=xyz[:3, :npso].T.copy(), rvol=rvol[:npoint].copy(), numia=numia[:npso].copy(),
                 clapl=clapl, cgrad=cgrad, ivois=ivois[:npso].copy() - 1, isvois=isvois[:npso].copy(),
                 nsortf=nsortf, nsort=nsort, isrt=isrt[:nsort].copy() - 1, poidso=poidso[:nsortf].copy())
+
+
+def atom_system(rsort: float, adimp: float, lmaxso: int, posi, rmt, lmax, iord: int = 4):
+    """A cluster with muffin-tin atoms (point group 1, one spin, regular
+    solution) built entirely by the reference: reseau with the atoms
+    (general.f90:9439, points inside a sphere marked), laplac, bordure and
+    recup_bordure (outer and atom border lists, poidso, poidsa), Ylmpt
+    (general.f90:11505: Ylmso, Ylmato) and newind (mat.f90:1761: atom blocks
+    in the radial order, lb1/lb2, new, newinv).  posi (natome, 3) bohr, the
+    absorber at the origin, sorted by distance; rmt (natome,) bohr; lmax
+    (natome,).  Returns a dict of the reference's arrays (1-based point and
+    row numbers, Fortran shapes)."""
+    A = _Args()
+    p = lambda a: a.ctypes.data_as(C.c_void_p)
+    posi = np.asfortranarray(np.asarray(posi, np.float64).reshape(-1, 3).T)  # (3, natome)
+    natome = posi.shape[1]
+    rmt = np.asarray(rmt, np.float64)
+    lmax = np.asarray(lmax, np.int32)
+    eps = 1.0e-4
+    rmax = rsort / adimp + np.sqrt(2.0) * iord / 2 + eps
+    nx = int(np.ceil(rmax)) + 1
+    r = np.arange(-nx, nx + 1)
+    g = np.sqrt(r[:, None, None] ** 2 + r[None, :, None] ** 2 + r[None, None, :] ** 2)
+    npsom = int((g <= rmax).sum()) + 8
+    iopsymr = np.zeros(NOPSM, np.int32)
+    iopsymr[0] = 1
+    indice = np.zeros((npsom, 3), np.int32, order="F")
+    xyz = np.zeros((4, npsom), order="F")
+    rvol = np.zeros(npsom)
+    numia = np.zeros(npsom, np.int32)
+    mpres = np.zeros((2 * nx + 1,) * 3, np.int32, order="F")
+    npoint, npr, npso = np.zeros(1, np.int32), np.zeros(1, np.int32), np.zeros(1, np.int32)
+    rmt_t = np.concatenate([[0.0], rmt])  # rmt(0:ntype), one type per atom
+    lib().reseau_(A.d(adimp), A.b(False), A.d(0.0), A.b(False), A.i(1), A.i(0), A.i(1), p(indice),
+                  A.arr(iopsymr, np.int32), A.i(iord), A.arr(np.arange(1, natome + 1), np.int32), A.b(False), A.i(0),
+                  p(mpres), A.i(natome), A.i(1), A.i(npsom), p(npoint), p(npr), p(npso), A.i(npsom), A.i(natome),
+                  p(numia), A.i(nx), A.arr(np.zeros((3, 1)), np.float64), p(posi), A.arr(rmt_t, np.float64),
+                  A.d(rsort), p(rvol), p(xyz))
+    npoint, npr, npso = int(npoint[0]), int(npr[0]), int(npso[0])
+    nvois = 3 * iord
+    cgrad = np.zeros(nvois)
+    clapl = np.zeros(nvois + 1)
+    ivois = np.zeros((npsom, nvois), np.int32, order="F")
+    isvois = np.zeros((npsom, nvois), np.int32, order="F")
+    lib().laplac_(A.d(adimp), A.b(False), p(cgrad), p(clapl), A.i(0), A.i(1), p(indice), A.arr(iopsymr, np.int32),
+                  A.i(iord), p(ivois), p(isvois), A.i(0), p(mpres), A.i(npso), A.i(npsom), A.i(nvois), A.i(nx))
+    nbm, nbtm, nsm, nstm = (np.zeros(1, np.int32) for _ in range(4))
+    unit = 78
+    lib().bordure_(A.b(False), A.i(0), A.arr(iopsymr, np.int32), A.i(iord), A.i(unit), p(ivois), A.i(0), A.i(natome),
+                   p(nbm), p(nbtm), A.i(npsom), A.i(npoint), A.i(npso), A.i(npsom), p(nsm), p(nstm), p(numia),
+                   A.i(nvois), p(posi), p(rvol), p(xyz))
+    nbm, nbtm, nsm, nstm = (max(int(v[0]), 1) for v in (nbm, nbtm, nsm, nstm))
+    ibord = np.zeros((nbtm, natome), np.int32, order="F")
+    isbord = np.zeros((nbtm, natome), np.int32, order="F")
+    nbord = np.zeros(natome, np.int32)
+    nbordf = np.zeros(natome, np.int32)
+    poidsa = np.zeros((nbm, natome), order="F")
+    isrt = np.zeros(nstm, np.int32)
+    poidso = np.zeros(nsm)
+    nsort, nsortf = np.zeros(1, np.int32), np.zeros(1, np.int32)
+    lib().recup_bordure_(p(ibord), p(isbord), A.i(unit), p(isrt), A.i(1), A.i(0), A.i(natome), p(nbord), p(nbordf),
+                         A.i(nbm), A.i(nbtm), A.i(nsm), p(nsort), p(nsortf), A.i(nstm), p(poidsa), p(poidso))
+    nsort, nsortf = int(nsort[0]), int(nsortf[0])
+    nlmsa = ((lmax + 1) ** 2).astype(np.int32)
+    nlmmax = int(nlmsa.max())
+    nlmso = (lmaxso + 1) ** 2
+    ylmato = np.zeros((nbtm, nlmmax, natome), order="F")
+    ylmso = np.zeros((nsort, nlmso), order="F")
+    rot = np.zeros((3, 3, natome), order="F")
+    for ia in range(natome):
+        rot[:, :, ia] = np.eye(3)
+    lib().ylmpt_(A.arr(np.arange(1, natome + 1), np.int32), p(ibord), A.i(0), A.arr(iopsymr, np.int32), p(isrt),
+                 A.arr(np.concatenate([[lmaxso], lmax]), np.int32), A.i(lmaxso), A.i(natome), A.i(natome), p(nbord),
+                 A.i(nbtm), A.i(nlmmax), A.i(nlmso), A.i(nsort), A.i(nstm), A.i(npsom), p(posi), p(rot), p(xyz),
+                 p(ylmato), p(ylmso))
+    nligne = npr + int(nlmsa.sum()) + nlmso  # mat.f90:150
+    lb1 = np.zeros(nligne, np.int32)
+    lb2 = np.zeros(nligne, np.int32)
+    newinv = np.zeros(nligne, np.int32)
+    newp = np.zeros(npoint + 1, np.int32)
+    ianew = np.zeros(natome, np.int32)
+    distai = np.sqrt((posi ** 2).sum(axis=0))
+    lib().newind_(A.arr(distai, np.float64), p(ianew), p(ibord), A.i(0), A.b(False), p(isrt), p(ivois), p(lb1),
+                  p(lb2), A.i(0), A.i(1), A.i(natome), p(nbordf), A.i(nbtm), p(newp), p(newinv), A.i(nligne),
+                  A.arr(nlmsa, np.int32), A.i(nlmso), A.i(npoint), A.i(nsortf), A.i(1), A.i(npsom), A.i(nstm),
+                  p(numia), A.i(nvois), p(xyz))
+    return dict(npoint=npoint, npr=npr, npso=npso, npsom=npsom, natome=natome, posi=posi, rmt=rmt, lmax=lmax,
+                lmaxso=lmaxso, nlmso=nlmso, nligne=nligne, nligneso=nligne - nlmso, indice=indice, xyz=xyz, rvol=rvol,
+                numia=numia, clapl=clapl, ivois=ivois, isvois=isvois, nsort=nsort, nsortf=nsortf, nsm=nsm, nstm=nstm,
+                isrt=isrt, poidso=poidso, nbm=nbm, nbtm=nbtm, nbord=nbord, nbordf=nbordf, ibord=ibord, isbord=isbord,
+                poidsa=poidsa, nlmsa=nlmsa, nlmmax=nlmmax, ylmato=ylmato, ylmso=ylmso, ianew=ianew, lb1=lb1, lb2=lb2,
+                newinv=newinv, new=newp)
+
+
+def phiso_sys(s, E_comp: bool, Ecinetic_out: float, Eimag: float, Enervide: float = 0.0):
+    """Reference phiso (non-Rydberg branch) on an atom_system dict."""
+    A = _Args()
+    ns, lm = s["nsort"], s["lmaxso"]
+    if E_comp:
+        bes = np.zeros((ns, lm + 1), np.complex128, order="F")
+        neu = np.zeros((ns, lm + 1), np.complex128, order="F")
+        besr, neur = np.zeros(1), np.zeros(1)
+        nsc, nsr = ns, 0
+    else:
+        bes, neu = np.zeros(1, np.complex128), np.zeros(1, np.complex128)
+        besr = np.zeros((ns, lm + 1), order="F")
+        neur = np.zeros((ns, lm + 1), order="F")
+        nsc, nsr = 0, ns
+    p = lambda a: a.ctypes.data_as(C.c_void_p)
+    lib().phiso_(A.d(0.0), p(bes), p(besr), A.b(E_comp), A.d(Ecinetic_out), A.d(0.0), A.d(Eimag), A.b(False),
+                 A.d(Enervide), A.i(0), A.i(1), A.arr(s["isrt"], np.int32), A.i(lm), A.i(0), p(neu), p(neur),
+                 A.i(s["npsom"]), A.i(ns), A.i(nsc), A.i(nsr), A.i(1), A.i(ns), A.d(0.0), A.d(0.0), A.b(False),
+                 A.d(0.0), A.arr(s["xyz"], np.float64))
+    return (bes, neu) if E_comp else (besr, neur)
+
+
+def mat_solve_sys(s, V, Enervide, Eimag, bes, neu, E_comp, cal_comp, basereel, phiato):
+    """Reference mat_solve (with its calcMatRow) on an atom_system dict.
+    phiato: list over atoms of (nbord(ia), nlmsa(ia)) arrays, complex
+    (nphiato7 = 2) or real.  Returns the solution (nligne, nlmso)."""
+    A = _Args()
+    n, nl, ns, lm = s["nligne"], s["nlmso"], s["nsort"], s["lmaxso"]
+    natome, nbtm, nlmmax = s["natome"], s["nbtm"], s["nlmmax"]
+    nvois = s["ivois"].shape[1]
+    cplx_phi = any(np.iscomplexobj(x) for x in phiato)
+    nphi7 = 2 if cplx_phi else 1
+    phi = np.zeros((nbtm, nlmmax, 1, 1, natome, nphi7), order="F")
+    for ia, x in enumerate(phiato):
+        nb, na = x.shape
+        phi[:nb, :na, 0, 0, ia, 0] = np.real(x)
+        if cplx_phi:
+            phi[:nb, :na, 0, 0, ia, 1] = np.imag(x)
+    nlmsam = nlmmax
+    lato = np.zeros((nlmsam, natome, 1), np.int32, order="F")
+    mato = np.zeros((nlmsam, natome, 1), np.int32, order="F")
+    iato = np.ones((nlmsam, natome, 1), np.int32, order="F")
+    for ia in range(natome):
+        k = 0
+        for L in range(int(s["lmax"][ia]) + 1):
+            for m in range(-L, L + 1):
+                lato[k, ia, 0], mato[k, ia, 0] = L, m
+                k += 1
+    smr = np.zeros((nl, n), order="F")
+    smi = np.zeros((nl, n), order="F") if cal_comp else np.zeros((1, 1), order="F")
+    if E_comp:
+        Bc, Nc = np.asfortranarray(bes, np.complex128), np.asfortranarray(neu, np.complex128)
+        Br, Nr = np.zeros(1), np.zeros(1)
+        nsc, nsr = ns, 0
+    else:
+        Bc, Nc = np.zeros(1, np.complex128), np.zeros(1, np.complex128)
+        Br, Nr = np.asfortranarray(bes, np.float64), np.asfortranarray(neu, np.float64)
+        nsc, nsr = 0, ns
+    kar = np.ones((NOPSM, 1), order="F")
+    kari = np.zeros((NOPSM, 1), order="F")
+    lso = np.array([L for L in range(lm + 1) for m in range(-L, L + 1)], np.int32)
+    mso = np.array([m for L in range(lm + 1) for m in range(-L, L + 1)], np.int32)
+    iso = np.ones(nl, np.int32)
+    tf, tt = np.zeros(1), np.zeros(1)
+    p = lambda a: a.ctypes.data_as(C.c_void_p)
+    npsom = s["npsom"]
+    args = [
+        A.b(False), A.b(basereel), p(Bc), p(Br), A.b(cal_comp), A.arr(np.zeros(nvois), np.float64),
+        A.arr(s["clapl"], np.float64), A.b(E_comp), A.d(Eimag), A.d(Enervide), A.arr(np.zeros(3), np.float64),
+        A.arr(s["ianew"], np.int32), p(iato), A.arr(s["ibord"], np.int32), A.i(0), A.i(1), A.i(1), A.i(0),
+        A.b(False), A.arr(s["isbord"], np.int32), A.arr(iso, np.int32), A.i(1), A.arr(s["isrt"], np.int32),
+        A.arr(s["isvois"], np.int32), A.arr(s["ivois"], np.int32), A.arr(kar, np.float64), A.arr(kari, np.float64),
+        p(lato), A.arr(s["lb1"], np.int32), A.arr(s["lb2"], np.int32), A.i(lm), A.arr(lso, np.int32),
+        p(mato), A.i(1), A.i(0), A.arr(mso, np.int32), A.i(1), A.i(1), A.i(natome), A.i(s["nbm"]),
+        A.arr(s["nbord"], np.int32), A.arr(s["nbordf"], np.int32), A.i(nbtm), p(Nc), p(Nr),
+        A.arr(s["new"], np.int32), A.arr(s["newinv"], np.int32), A.i(1), A.i(1), A.i(s["npoint"]), A.i(n),
+        A.i(n if cal_comp else 0), A.i(s["nligneso"]), A.i(nlmsam), A.i(nlmmax), A.i(nlmmax), A.i(nl),
+        A.arr(s["nlmsa"], np.int32), A.i(nl), A.i(nl if cal_comp else 0), A.i(nbtm), A.i(nphi7),
+        A.i(s["npoint"]), A.i(npsom), A.i(max(s["nsm"], 1)), A.i(nl), A.i(ns), A.i(nsc), A.i(nsr),
+        A.i(s["nsortf"]), A.i(1), A.i(1), A.i(1), A.i(1), A.i(s["nstm"]), A.arr(s["numia"], np.int32), A.i(nvois),
+        p(phi), A.arr(s["poidsa"], np.float64), A.arr(s["poidso"], np.float64), A.b(False), A.b(False),
+        A.arr(s["rvol"][: s["npoint"]], np.float64), p(smi), p(smr), A.b(False), A.i(0), p(tf), p(tt),
+        A.arr(np.asarray(V, np.float64), np.float64), A.b(False), A.arr(s["ylmato"], np.float64),
+        A.arr(s["ylmso"], np.float64),
+    ]
+    assert len(args) == 93
+    lib().mat_solve_(*args)
+    x = smr.T.copy()
+    if cal_comp:
+        x = x + 1j * smi.T
+    return x

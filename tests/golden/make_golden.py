@@ -15,6 +15,11 @@ clusters and stores inputs and outputs as .npz fixtures:
                              point group 1 and for the mirror groups m, mm2
                              and mmm (wedge, rvol, symmetry operation of
                              each neighbour)
+  the same with muffin-tin atoms, then newind and mat_solve with a synthetic
+  phiato (j_L(k r) Ylm at the atoms' border points)
+                          -> atom border lists, weights, Ylmato, atom blocks
+                             in the profile, and the solution for the three
+                             branch sets
 The cluster arrays fed to the reference (grid, neighbours, boundary lists,
 Ylmso) come from the oracle's restatement; the fixtures store the profile
 and matrix-relevant inputs so the tests can rebuild each case.
@@ -73,6 +78,55 @@ def grid_fixtures():
         print("grid group", grp, npt, g["npso"], g["nsortf"], g["nsort"])
 
 
+# a cluster with three atoms: the absorber and two at the same distance
+# (newind's shared insertion point), positions in Angstrom
+ATOMS = dict(R=1.9, h=0.25, lmaxso=2, posi=[[0.0, 0.0, 0.0], [1.0, 0.25, 0.0], [-1.0, -0.25, 0.0]],
+             rmt=[0.45, 0.4, 0.4], lmax=[2, 1, 1])
+ATOM_BRANCHES = [  # E (Ry), Eimag, E_comp, cal_comp, basereel
+    (0.7, 0.05, True, True, False), (0.7, 0.0, False, False, True), (0.7, 0.0, False, True, False)]
+ROW_STRIDE = 3
+
+
+def atom_phiato(s, E, Eimag, E_comp):
+    """Synthetic phiato: j_L(k r) Y_Lm at every border point of every atom,
+    k = sqrt(E + 1.5 (+ i Eimag)), as the radial solutions of a flat well."""
+    from scipy.special import spherical_jn
+    k = np.sqrt(E + 1.5 + (1j * Eimag if E_comp else 0))
+    out = []
+    for ia in range(s["natome"]):
+        nb, nl = int(s["nbord"][ia]), int(s["nlmsa"][ia])
+        pts = s["xyz"][:3, s["ibord"][:nb, ia] - 1].T - s["posi"][:, ia]
+        r = np.linalg.norm(pts, axis=1)
+        ls = np.array([L for L in range(int(s["lmax"][ia]) + 1) for m in range(-L, L + 1)])
+        phi = np.stack([spherical_jn(L, k * r) for L in ls], axis=1) * s["ylmato"][:nb, :nl, ia]
+        out.append(phi if E_comp else phi.real)
+    return out
+
+
+def atom_fixture():
+    a = ATOMS
+    posi = np.array(a["posi"]) / BOHR
+    s = R.atom_system(a["R"] / BOHR, a["h"] / BOHR, a["lmaxso"], posi, np.array(a["rmt"]) / BOHR, a["lmax"])
+    rng = np.random.default_rng(7)
+    V = rng.uniform(-0.3, 0.3, s["npoint"]) - 0.8
+    rows = np.arange(0, s["nligne"], ROW_STRIDE)
+    out = dict(R=a["R"], h=a["h"], lmaxso=a["lmaxso"], posi=np.array(a["posi"]), rmt=np.array(a["rmt"]),
+               lmax=np.array(a["lmax"]), V=V, rows=rows, npoint=s["npoint"], npr=s["npr"], nligne=s["nligne"],
+               numia=s["numia"][: s["npso"]], ianew=s["ianew"], nbord=s["nbord"], nbordf=s["nbordf"],
+               lb1=s["lb1"], lb2=s["lb2"], newinv=s["newinv"], new=s["new"])
+    for ia in range(s["natome"]):
+        nb, nl = int(s["nbord"][ia]), int(s["nlmsa"][ia])
+        out[f"ibord{ia}"] = s["ibord"][:nb, ia]
+        out[f"poidsa{ia}"] = s["poidsa"][: s["nbordf"][ia], ia]
+        out[f"ylmato{ia}"] = s["ylmato"][:nb, :nl, ia]
+    for b, (E, Ei, E_comp, cal, basereel) in enumerate(ATOM_BRANCHES):
+        bes, neu = R.phiso_sys(s, E_comp, E, Ei, E)
+        x = R.mat_solve_sys(s, V, E, Ei, bes, neu, E_comp, cal, basereel, atom_phiato(s, E, Ei, E_comp))
+        out[f"x{b}"] = x[rows]
+    np.savez_compressed(os.path.join(HERE, "atoms_R1.9_h0.25.npz"), **out)
+    print("atoms", s["npoint"], s["npr"], s["nligne"], s["nbord"], s["nbordf"], s["ianew"])
+
+
 def main():
     if not R.available():
         raise SystemExit("oracle/_ref/libfdmnes_ref.so is missing: make -f oracle/Makefile.ref")
@@ -102,6 +156,7 @@ def main():
                         ylm=np.array(ylm))
     print("special functions written")
     grid_fixtures()
+    atom_fixture()
 
 
 if __name__ == "__main__":
