@@ -42,6 +42,19 @@ def _L():
         L.fdmgpu_cluster_assembly_tables.restype = C.c_int
         L.fdmgpu_cluster_assembly_tables.argtypes = [C.c_void_p, _lib.dp, _lib.lp, _lib.ip, _lib.lp, _lib.ip,
                                                      _lib.ip, _lib.dp, _lib.dp, _lib.dp, _lib.dp]
+        L.fdmgpu_cluster_create_atoms.restype = C.c_int
+        L.fdmgpu_cluster_create_atoms.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int, _lib.dp, _lib.dp,
+                                                  _lib.ip, C.POINTER(C.c_void_p)]
+        L.fdmgpu_cluster_atom_sizes.restype = C.c_int
+        L.fdmgpu_cluster_atom_sizes.argtypes = [C.c_void_p, _lib.lp]
+        L.fdmgpu_cluster_atom.restype = C.c_int
+        L.fdmgpu_cluster_atom.argtypes = [C.c_void_p, C.c_int, _lib.ip, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.fdmgpu_cluster_point_rows.restype = C.c_int
+        L.fdmgpu_cluster_point_rows.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.fdmgpu_cluster_assemble_atoms.restype = C.c_int
+        L.fdmgpu_cluster_assemble_atoms.argtypes = [C.c_void_p, _lib.dp, C.c_double, C.c_double, C.c_double,
+                                                    C.c_int, C.c_int, _lib.dp, C.c_void_p, _lib.lp, _lib.ip,
+                                                    _lib.dp, C.c_void_p, C.c_void_p, C.c_void_p]
         L._cluster_sigs = True
     return L
 
@@ -53,16 +66,30 @@ class Cluster:
     fdmgpu_cluster_create_group): ``group`` is the reference's point-group
     number, 8 = mmm (irrep 0..7, wedge x, y, z >= 0, the default), 6 = mm2
     (irrep 0..3, wedge x, y >= 0), 3 = m (irrep 0 or 4, wedge z >= 0);
-    irrep bit a set = odd under the mirror normal to axis a."""
+    irrep bit a set = odd under the mirror normal to axis a.  With ``atoms``
+    = (posi (natome, 3) bohr sorted by distance from the absorber, rmt
+    (natome,) bohr, lmax (natome,)) a P1 cluster with muffin-tin atoms
+    (fdmgpu_cluster_create_atoms), assembled with :meth:`assemble_atoms`."""
 
     GROUPS = {1: 0, 3: 4, 6: 3, 8: 7}  # point-group number -> mirrors (bit a = normal to axis a)
 
-    def __init__(self, rsort: float, adimp: float, lmaxso: int, irrep: int = -1, group: int = 8):
+    def __init__(self, rsort: float, adimp: float, lmaxso: int, irrep: int = -1, group: int = 8, atoms=None):
         L = _L()
         h = C.c_void_p()
         if irrep < 0:
             group = 1
-        rc = L.fdmgpu_cluster_create_group(rsort, adimp, lmaxso, int(group), int(irrep), C.byref(h))
+        self.natome = 0
+        if atoms is not None:
+            if irrep >= 0:
+                raise ValueError("clusters with atoms are built in point group P1 only")
+            posi, rmt, lmax = atoms
+            posi = np.ascontiguousarray(posi, np.float64).reshape(-1, 3)
+            self.natome = len(posi)
+            rc = L.fdmgpu_cluster_create_atoms(rsort, adimp, lmaxso, self.natome, posi.reshape(-1),
+                                               np.ascontiguousarray(rmt, np.float64),
+                                               np.ascontiguousarray(lmax, np.int32), C.byref(h))
+        else:
+            rc = L.fdmgpu_cluster_create_group(rsort, adimp, lmaxso, int(group), int(irrep), C.byref(h))
         if rc:
             raise _lib.FdmGpuError(f"fdmgpu_cluster_create failed ({rc})")
         self._h = h
@@ -122,6 +149,68 @@ class Cluster:
                                           _lib.ptr(im), _lib.ptr(rr), _lib.ptr(ri))
         if rc:
             raise _lib.FdmGpuError(f"fdmgpu_cluster_assemble failed ({rc})")
+        vals = re + 1j * im if cal else re
+        rhs = (rr + 1j * ri).reshape(n, nl) if cal else rr.reshape(n, nl)
+        return rp, col, vals, rhs
+
+    @property
+    def npr(self) -> int:
+        """Interstitial grid points (rows of the grid)."""
+        s = np.zeros(4, np.int64)
+        _L().fdmgpu_cluster_atom_sizes(self._h, s)
+        return int(s[1]) if self.natome else self.npoint
+
+    @property
+    def numia(self) -> np.ndarray:
+        a = np.zeros(self.npoint, np.int32)
+        _L().fdmgpu_cluster_point_rows(self._h, _lib.ptr(a), None)
+        return a
+
+    @property
+    def point_rows(self) -> np.ndarray:
+        """0-based row of each interior point, -1 inside an atom."""
+        a = np.zeros(self.npoint, np.int32)
+        _L().fdmgpu_cluster_point_rows(self._h, None, _lib.ptr(a))
+        return a
+
+    def atom(self, a: int) -> dict:
+        """Atom a (0-based): nbord, nbordf, nlmsa, ianew (first row, 0-based),
+        ibord (0-based points), poidsa (nbordf,), ylmato (nbord, nlmsa)."""
+        L = _L()
+        info = np.zeros(4, np.int32)
+        if L.fdmgpu_cluster_atom(self._h, int(a), info, None, None, None):
+            raise IndexError(a)
+        nb, nbf, nl, ianew = map(int, info)
+        ib = np.zeros(nb, np.int32)
+        pa = np.zeros(nbf)
+        ya = np.zeros(nb * nl)
+        L.fdmgpu_cluster_atom(self._h, int(a), info, _lib.ptr(ib), _lib.ptr(pa), _lib.ptr(ya))
+        return dict(nbord=nb, nbordf=nbf, nlmsa=nl, ianew=ianew, ibord=ib, poidsa=pa, ylmato=ya.reshape(nb, nl))
+
+    def assemble_atoms(self, V, Enervide: float, Eimag: float, Ecinetic_out: float, E_comp: bool, basereel: bool,
+                       phiato):
+        """Compressed rows and incoming-wave RHS of one energy point of a
+        cluster with atoms; phiato: per atom an (nbord, nlmsa) array of the
+        radial solutions times Ylm at its border points, complex or real."""
+        cal = bool(E_comp) or not basereel
+        cplx = any(np.iscomplexobj(p) for p in phiato)
+        if cplx and not cal:
+            raise ValueError("complex radial solutions need complex arithmetic (E_comp or not basereel)")
+        flat = np.concatenate([np.asarray(p).reshape(-1) for p in phiato])
+        pr = np.ascontiguousarray(flat.real, np.float64)
+        pi = np.ascontiguousarray(flat.imag, np.float64) if cplx else None
+        n, nl = self.nligne, self.nlmso
+        rp = np.zeros(n + 1, np.int64)
+        col = np.zeros(self.nnz, np.int32)
+        re = np.zeros(self.nnz)
+        im = np.zeros(self.nnz) if cal else None
+        rr = np.zeros(n * nl)
+        ri = np.zeros(n * nl) if cal else None
+        rc = _L().fdmgpu_cluster_assemble_atoms(self._h, np.ascontiguousarray(V, np.float64), Enervide, Eimag,
+                                                Ecinetic_out, int(bool(E_comp)), int(bool(basereel)), pr,
+                                                _lib.ptr(pi), rp, col, re, _lib.ptr(im), _lib.ptr(rr), _lib.ptr(ri))
+        if rc:
+            raise _lib.FdmGpuError(f"fdmgpu_cluster_assemble_atoms failed ({rc})")
         vals = re + 1j * im if cal else re
         rhs = (rr + 1j * ri).reshape(n, nl) if cal else rr.reshape(n, nl)
         return rp, col, vals, rhs

@@ -110,6 +110,19 @@ struct fdmgpu_cluster {
   std::vector<int64_t> rp;     // compressed-row structure, nligne + 1
   std::vector<int> cols;       // 1-based
   double clapl[13] = {0};
+  // Muffin-tin atoms (point group P1 only): positions (bohr, absorber at
+  // the origin, sorted by distance), sphere radii, expansions
+  int natome = 0, npr = 0;
+  std::vector<double> apos, armt;
+  std::vector<int> almax, nlmsa, ianew;  // ianew: rows ianew .. ianew + nlmsa - 1 (0-based) are the atom's
+  std::vector<int> numia;                // npoint: 0, or 1 + the atom the point lies in
+  std::vector<int> row;                  // npoint: 0-based row of an interstitial point, -1 inside an atom
+  std::vector<std::vector<int>> abord;   // per atom: nbordf outside border points, then the inside ones
+  std::vector<int> nbordf;
+  std::vector<std::vector<double>> apoids;  // per atom, nbordf weights 4 pi rvol / sum(rvol)
+  std::vector<std::vector<double>> aylm;    // per atom, nbord x nlmsa real harmonics about the atom
+  std::vector<int> rkind, ridx;          // per row: 0 grid (point), 1 atom (atom, Lm packed), 2 outer (lm)
+  int nso = 0;                           // first outer row (0-based)
 };
 
 namespace {
@@ -128,6 +141,176 @@ int harmonic_irrep(int l, int m) {
 
 // character of the mirrors in mask for representation g: -1 per odd axis flipped
 inline double character(int g, int mask) { return (__builtin_popcount(g & mask) & 1) ? -1.0 : 1.0; }
+
+// The atom border lists, harmonics and weights (bordure, general.f90:
+// 9924-10068; Ylmpt :11533-11568), the rows with the atom blocks in the
+// radial order (newind, mat.f90:1790-1884) and the compressed-row structure.
+int build_atoms(fdmgpu_cluster *c) {
+  const int np = c->npoint, na = c->natome, nl = c->nlmso;
+  const GridPoint *p = c->pt.data();
+  c->abord.assign(na, {});
+  c->nbordf.assign(na, 0);
+  std::vector<int> ibp(na, -1);
+  for (int i = 0; i < np; i++) {  // outside border points: a first-shell neighbour in the atom
+    if (c->numia[i]) continue;
+    for (int v = 0; v < 6; v++) {
+      const int j = c->nbr[(size_t)i * 12 + v];
+      if (j < 0 || j >= np || !c->numia[j]) continue;
+      const int a = c->numia[j] - 1;
+      if (ibp[a] != i) c->abord[a].push_back(i);
+      ibp[a] = i;
+    }
+  }
+  for (int a = 0; a < na; a++) c->nbordf[a] = (int)c->abord[a].size();
+  for (int i = 0; i < np; i++) {  // inside border points: a neighbour in the interstitial grid
+    if (!c->numia[i]) continue;
+    for (int v = 0; v < 12; v++) {
+      const int j = c->nbr[(size_t)i * 12 + v];
+      if (j < 0) return -1;
+      if (j >= np || c->numia[j]) continue;
+      c->abord[c->numia[i] - 1].push_back(i);
+      break;
+    }
+  }
+  c->apoids.assign(na, {});
+  c->aylm.assign(na, {});
+  c->nlmsa.resize(na);
+  for (int a = 0; a < na; a++) {
+    if (c->nbordf[a] == 0) return -1;  // an atom the grid does not see
+    double pd = 0;
+    for (int b = 0; b < c->nbordf[a]; b++) pd += c->rvol[c->abord[a][b]];
+    for (int b = 0; b < c->nbordf[a]; b++) c->apoids[a].push_back(4 * kPi * c->rvol[c->abord[a][b]] / pd);
+    const int lm = (c->almax[a] + 1) * (c->almax[a] + 1);
+    c->nlmsa[a] = lm;
+    c->aylm[a].resize(c->abord[a].size() * lm);
+    for (size_t b = 0; b < c->abord[a].size(); b++) {
+      const GridPoint &q = p[c->abord[a][b]];
+      real_ylm(c->almax[a], q.ix * c->adimp - c->apos[3 * a], q.iy * c->adimp - c->apos[3 * a + 1],
+               q.iz * c->adimp - c->apos[3 * a + 2], &c->aylm[a][b * lm]);
+    }
+  }
+  // rows: interstitial points in order, each atom's block before the first
+  // interstitial point at least as far from the absorber as the atom
+  std::vector<int> nw(np + 1, 0);  // 1-based rows of the reference's new(i)
+  int ii = 0, npr = 0;
+  for (int i = 0; i < np; i++)
+    if (!c->numia[i]) nw[i] = ++ii, npr++;
+  c->npr = npr;
+  c->ianew.assign(na, 0);
+  int npf = 0, npf1 = -1;
+  for (int a = 0; a < na; a++) {
+    const double dist = std::sqrt(c->apos[3 * a] * c->apos[3 * a] + c->apos[3 * a + 1] * c->apos[3 * a + 1] +
+                                  c->apos[3 * a + 2] * c->apos[3 * a + 2]);
+    int i = npf;
+    while (i < np && !(c->numia[i] == 0 && c->rad[i] >= dist)) i++;
+    if (i >= np) return -1;  // no grid row beyond the atom
+    npf = i;
+    c->ianew[a] = (npf1 == npf) ? c->ianew[a - 1] + c->nlmsa[a - 1] : nw[npf] - 1;
+    npf1 = npf;
+    for (int k = npf; k < np; k++)
+      if (!c->numia[k]) nw[k] += c->nlmsa[a];
+  }
+  int nat = 0;
+  for (int a = 0; a < na; a++) nat += c->nlmsa[a];
+  const int nlig = npr + nat + nl;
+  c->nligne = nlig;
+  c->nso = nlig - nl;
+  c->row.assign(np, -1);
+  for (int i = 0; i < np; i++)
+    if (!c->numia[i]) c->row[i] = nw[i] - 1;
+  c->rkind.assign(nlig, 0);
+  c->ridx.assign(nlig, 0);
+  for (int i = 0; i < np; i++)
+    if (c->row[i] >= 0) c->ridx[c->row[i]] = i;
+  for (int a = 0; a < na; a++)
+    for (int lm = 0; lm < c->nlmsa[a]; lm++) {
+      c->rkind[c->ianew[a] + lm] = 1;
+      c->ridx[c->ianew[a] + lm] = (a << 16) | lm;
+    }
+  for (int lm = 0; lm < nl; lm++) {
+    c->rkind[c->nso + lm] = 2;
+    c->ridx[c->nso + lm] = lm;
+  }
+  // profile (newind, mat.f90:1812-1884), 1-based
+  c->lb1.resize(nlig);
+  c->lb2.resize(nlig);
+  for (int r = 1; r <= nlig; r++) c->lb1[r - 1] = c->lb2[r - 1] = r;
+  for (int i = 0; i < np; i++) {
+    if (c->row[i] < 0) continue;
+    int &l1 = c->lb1[c->row[i]], &l2 = c->lb2[c->row[i]];
+    for (int v = 0; v < 12; v++) {
+      const int j = c->nbr[(size_t)i * 12 + v];
+      if (j < 0) return -1;
+      if (j >= np) {
+        l2 = nlig;
+      } else if (c->numia[j]) {
+        const int a = c->numia[j] - 1;
+        l1 = std::min(l1, c->ianew[a] + 1);
+        l2 = std::max(l2, c->ianew[a] + c->nlmsa[a]);
+      } else {
+        l1 = std::min(l1, c->row[j] + 1);
+        l2 = std::max(l2, c->row[j] + 1);
+      }
+    }
+  }
+  for (int a = 0; a < na; a++) {
+    int l1 = c->ianew[a] + 1, l2 = c->ianew[a] + c->nlmsa[a];
+    for (int b = 0; b < c->nbordf[a]; b++) {
+      l1 = std::min(l1, c->row[c->abord[a][b]] + 1);
+      l2 = std::max(l2, c->row[c->abord[a][b]] + 1);
+    }
+    for (int lm = 0; lm < c->nlmsa[a]; lm++) {
+      c->lb1[c->ianew[a] + lm] = l1;
+      c->lb2[c->ianew[a] + lm] = l2;
+    }
+  }
+  int lo = c->nso + 1;
+  for (int b = 0; b < c->nsortf; b++) lo = std::min(lo, c->row[c->bnd[b]] + 1);
+  for (int r = c->nso + 1; r <= nlig; r++) {
+    c->lb1[r - 1] = lo;
+    c->lb2[r - 1] = nlig;
+  }
+  for (int r = nlig - 1; r >= 1; r--) c->lb1[r - 1] = std::min(c->lb1[r - 1], c->lb1[r]);
+  // compressed rows, columns ascending: grid rows (diagonal, interstitial
+  // neighbours, the blocks of atoms a neighbour lies in, outer columns),
+  // atom rows (the outside border points, the atom's block), outer rows
+  c->rp.assign(nlig + 1, 0);
+  c->cols.clear();
+  std::vector<int> tmp;
+  for (int r = 0; r < nlig; r++) {
+    tmp.clear();
+    if (c->rkind[r] == 0) {
+      const int i = c->ridx[r];
+      tmp.push_back(r + 1);
+      bool outside = false;
+      for (int v = 0; v < 12; v++) {
+        const int j = c->nbr[(size_t)i * 12 + v];
+        if (j >= np) {
+          outside = true;
+        } else if (c->numia[j]) {
+          const int a = c->numia[j] - 1;
+          for (int lm = 0; lm < c->nlmsa[a]; lm++) tmp.push_back(c->ianew[a] + lm + 1);
+        } else {
+          tmp.push_back(c->row[j] + 1);
+        }
+      }
+      if (outside)
+        for (int l = 0; l < nl; l++) tmp.push_back(c->nso + l + 1);
+    } else if (c->rkind[r] == 1) {
+      const int a = c->ridx[r] >> 16;
+      for (int b = 0; b < c->nbordf[a]; b++) tmp.push_back(c->row[c->abord[a][b]] + 1);
+      for (int lm = 0; lm < c->nlmsa[a]; lm++) tmp.push_back(c->ianew[a] + lm + 1);
+    } else {
+      for (int b = 0; b < c->nsortf; b++) tmp.push_back(c->row[c->bnd[b]] + 1);
+      for (int k = 0; k < nl; k++) tmp.push_back(c->nso + k + 1);
+    }
+    std::sort(tmp.begin(), tmp.end());
+    tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+    c->cols.insert(c->cols.end(), tmp.begin(), tmp.end());
+    c->rp[r + 1] = (int64_t)c->cols.size();
+  }
+  return 0;
+}
 
 int build(fdmgpu_cluster *c) {
   const bool sym = c->irrep >= 0;
@@ -207,10 +390,21 @@ int build(fdmgpu_cluster *c) {
     s += c->clapl[v];
   }
   c->clapl[0] = -s;
-  // Outer-sphere point lists (bordure, general.f90:9918-10076): interior points
-  // with a first-shell neighbour outside, then outside points touching the
-  // interior; quadrature weight 4 pi / nsortf each (unit point volumes).
-  for (int i = 0; i < np; i++)
+  // Points inside an atom's sphere (reseau, general.f90:9563-9577; the
+  // distance to the atom, the identity being the only operation)
+  c->numia.assign(np, 0);
+  for (int a = 0; a < c->natome; a++)
+    for (int i = 0; i < np; i++) {
+      const double dx = p[i].ix * c->adimp - c->apos[3 * a], dy = p[i].iy * c->adimp - c->apos[3 * a + 1],
+                   dz = p[i].iz * c->adimp - c->apos[3 * a + 2];
+      if (std::sqrt(dx * dx + dy * dy + dz * dz) < c->armt[a] + kEpsPos) c->numia[i] = a + 1;
+    }
+  // Outer-sphere point lists (bordure, general.f90:9918-10076): interstitial
+  // points with a first-shell neighbour outside, then outside points touching
+  // an interstitial point; quadrature weight 4 pi / nsortf each (unit point
+  // volumes).
+  for (int i = 0; i < np; i++) {
+    if (c->numia[i]) continue;
     for (int v = 0; v < 6; v++) {
       const int j = c->nbr[(size_t)i * 12 + v];
       if (j >= np) {
@@ -218,6 +412,7 @@ int build(fdmgpu_cluster *c) {
         break;
       }
     }
+  }
   c->nsortf = (int)c->bnd.size();
   double pdsort = 0;  // bordure, general.f90:10006, :10073-10075
   for (int b = 0; b < c->nsortf; b++) pdsort += c->rvol[c->bnd[b]];
@@ -227,7 +422,7 @@ int build(fdmgpu_cluster *c) {
   for (int i = np; i < n; i++)
     for (int v = 0; v < 12; v++) {
       const int j = c->nbr[(size_t)i * 12 + v];
-      if (j >= 0 && j < np) {
+      if (j >= 0 && j < np && !c->numia[j]) {
         c->slot[i] = (int)c->bnd.size();
         c->bnd.push_back(i);
         break;
@@ -249,6 +444,7 @@ int build(fdmgpu_cluster *c) {
     real_ylm(c->lmaxso, q.ix * c->adimp, q.iy * c->adimp, q.iz * c->adimp, yall.data());
     for (int k = 0; k < c->nlmso; k++) c->ylm[b * c->nlmso + k] = yall[c->lmidx[k]];
   }
+  if (c->natome > 0) return build_atoms(c);
   // Rows and profile (newind, mat.f90:1812-1884)
   const int nl = np + c->nlmso;
   c->nligne = nl;
@@ -304,6 +500,10 @@ int build(fdmgpu_cluster *c) {
     for (int k = 0; k < c->nlmso; k++) c->cols.push_back(np + k + 1);
     c->rp[np + l + 1] = (int64_t)c->cols.size();
   }
+  c->npr = np;
+  c->nso = np;
+  c->row.resize(np);
+  for (int i = 0; i < np; i++) c->row[i] = i;
   return 0;
 }
 
@@ -318,6 +518,31 @@ inline int lof(int lm) {  // l of the harmonic index l*l + l + m
 // rvol(i) (calcMatRow, mat.f90:1164-1222; -clapl(v) without symmetry)
 inline double stencil(const fdmgpu_cluster *c, int i, int d) {
   return (character(c->irrep, c->flip[(size_t)i * 12 + d]) * -c->clapl[d + 1]) * c->rvol[i];
+}
+
+// Outer-sphere functions at every boundary point (phiso, non-Rydberg branch,
+// mat.f90:771-792, :975-994): k = sqrt(Ec + i Eimag), fnorm = sqrt(k/pi);
+// B, N: (nb, lmaxso + 1)
+void outer_functions(const fdmgpu_cluster *c, double Eimag, double Ecinetic_out, int E_comp, cd *B, cd *N) {
+  const int nb = (int)c->bnd.size(), L = c->lmaxso;
+  if (E_comp) {
+    cd k = std::sqrt(cd(Ecinetic_out, Eimag));
+    if (std::abs(k) < 1e-10) k = cd(1e-10, 0);
+    const cd fn = std::sqrt(k / kPi);
+    for (int b = 0; b < nb; b++)
+      bessel_neumann<cd>(fn, k * c->rad[c->bnd[b]], L, &B[(size_t)b * (L + 1)], &N[(size_t)b * (L + 1)]);
+  } else {
+    const double k = std::sqrt(Ecinetic_out);
+    const double fn = std::sqrt(k / kPi);
+    std::vector<double> br(L + 1), nr(L + 1);
+    for (int b = 0; b < nb; b++) {
+      bessel_neumann<double>(fn, k * c->rad[c->bnd[b]], L, br.data(), nr.data());
+      for (int l = 0; l <= L; l++) {
+        B[(size_t)b * (L + 1) + l] = br[l];
+        N[(size_t)b * (L + 1) + l] = nr[l];
+      }
+    }
+  }
 }
 
 }  // namespace
@@ -359,6 +584,71 @@ int fdmgpu_cluster_create_mmm(double rsort, double adimp, int lmaxso, int irrep,
 
 int fdmgpu_cluster_create(double rsort, double adimp, int lmaxso, fdmgpu_cluster **out) {
   return fdmgpu_cluster_create_group(rsort, adimp, lmaxso, 1, -1, out);
+}
+
+int fdmgpu_cluster_create_atoms(double rsort, double adimp, int lmaxso, int natome, const double *posi,
+                                const double *rmt, const int *lmax, fdmgpu_cluster **out) {
+  if (!out || !(rsort > 0) || !(adimp > 0) || lmaxso < 0 || lmaxso > 40 || natome < 1 || natome > 32767 || !posi ||
+      !rmt || !lmax)
+    return -1;
+  double prev = 0;
+  for (int a = 0; a < natome; a++) {
+    const double d = std::sqrt(posi[3 * a] * posi[3 * a] + posi[3 * a + 1] * posi[3 * a + 1] +
+                               posi[3 * a + 2] * posi[3 * a + 2]);
+    if (!(rmt[a] > 0) || lmax[a] < 0 || lmax[a] > 12 || d < prev) return -1;  // sorted by distance, as posi
+    prev = d;
+  }
+  fdmgpu_cluster *c = new (std::nothrow) fdmgpu_cluster();
+  if (!c) return -3;
+  c->rsort = rsort;
+  c->adimp = adimp;
+  c->lmaxso = lmaxso;
+  c->irrep = -1;
+  c->mirrors = 0;
+  c->natome = natome;
+  c->apos.assign(posi, posi + 3 * natome);
+  c->armt.assign(rmt, rmt + natome);
+  c->almax.assign(lmax, lmax + natome);
+  if (build(c)) {
+    delete c;
+    return -1;
+  }
+  *out = c;
+  return 0;
+}
+
+int fdmgpu_cluster_atom_sizes(const fdmgpu_cluster *c, int64_t *sizes) {
+  int64_t nbord = 0, nphi = 0;
+  for (int a = 0; a < c->natome; a++) {
+    nbord += (int64_t)c->abord[a].size();
+    nphi += (int64_t)c->abord[a].size() * c->nlmsa[a];
+  }
+  sizes[0] = c->natome;
+  sizes[1] = c->npr;
+  sizes[2] = nbord;
+  sizes[3] = nphi;
+  return 0;
+}
+
+int fdmgpu_cluster_atom(const fdmgpu_cluster *c, int a, int *info, int *ibord, double *poidsa, double *ylmato) {
+  if (a < 0 || a >= c->natome) return -1;
+  const int nb = (int)c->abord[a].size();
+  info[0] = nb;
+  info[1] = c->nbordf[a];
+  info[2] = c->nlmsa[a];
+  info[3] = c->ianew[a];
+  if (ibord) std::memcpy(ibord, c->abord[a].data(), sizeof(int) * nb);
+  if (poidsa) std::memcpy(poidsa, c->apoids[a].data(), sizeof(double) * c->nbordf[a]);
+  if (ylmato) std::memcpy(ylmato, c->aylm[a].data(), sizeof(double) * c->aylm[a].size());
+  return 0;
+}
+
+int fdmgpu_cluster_point_rows(const fdmgpu_cluster *c, int *numia, int *row) {
+  for (int i = 0; i < c->npoint; i++) {
+    if (numia) numia[i] = c->natome ? c->numia[i] : 0;
+    if (row) row[i] = c->row[i];
+  }
+  return 0;
 }
 
 int fdmgpu_cluster_rvol(const fdmgpu_cluster *c, double *rvol) {
@@ -403,29 +693,11 @@ int64_t fdmgpu_cluster_nnz(const fdmgpu_cluster *c) { return (int64_t)c->cols.si
 int fdmgpu_cluster_assemble(const fdmgpu_cluster *c, const double *V, double Enervide, double Eimag,
                             double Ecinetic_out, int E_comp, int basereel, int64_t *row_ptr, int *col, double *re,
                             double *im, double *rhs_re, double *rhs_im) {
+  if (c->natome > 0) return -2;  // fdmgpu_cluster_assemble_atoms
   const bool cal = E_comp || !basereel;
   const int np = c->npoint, nl = c->nlmso, nb = (int)c->bnd.size(), L = c->lmaxso;
-  // Outer-sphere functions at every boundary point (phiso, non-Rydberg
-  // branch, mat.f90:771-792, :975-994): k = sqrt(Ec + i Eimag), fnorm = sqrt(k/pi).
   std::vector<cd> B((size_t)nb * (L + 1)), N((size_t)nb * (L + 1));
-  if (E_comp) {
-    cd k = std::sqrt(cd(Ecinetic_out, Eimag));
-    if (std::abs(k) < 1e-10) k = cd(1e-10, 0);
-    const cd fn = std::sqrt(k / kPi);
-    for (int b = 0; b < nb; b++)
-      bessel_neumann<cd>(fn, k * c->rad[c->bnd[b]], L, &B[(size_t)b * (L + 1)], &N[(size_t)b * (L + 1)]);
-  } else {
-    const double k = std::sqrt(Ecinetic_out);
-    const double fn = std::sqrt(k / kPi);
-    std::vector<double> br(L + 1), nr(L + 1);
-    for (int b = 0; b < nb; b++) {
-      bessel_neumann<double>(fn, k * c->rad[c->bnd[b]], L, br.data(), nr.data());
-      for (int l = 0; l <= L; l++) {
-        B[(size_t)b * (L + 1) + l] = br[l];
-        N[(size_t)b * (L + 1) + l] = nr[l];
-      }
-    }
-  }
+  outer_functions(c, Eimag, Ecinetic_out, E_comp, B.data(), N.data());
   // Outer-column factor of an outside point (calcMatRow, mat.f90:1330-1346):
   // the Neumann function in the real basis, j + i n in the complex one.
   auto cfac = [&](int b, int l) -> cd {
@@ -508,7 +780,117 @@ int fdmgpu_cluster_assemble(const fdmgpu_cluster *c, const double *V, double Ene
   return 0;
 }
 
+int fdmgpu_cluster_assemble_atoms(const fdmgpu_cluster *c, const double *V, double Enervide, double Eimag,
+                                  double Ecinetic_out, int E_comp, int basereel, const double *phi_re,
+                                  const double *phi_im, int64_t *row_ptr, int *col, double *re, double *im,
+                                  double *rhs_re, double *rhs_im) {
+  const bool cal = E_comp || !basereel;
+  if (!c || !V || !phi_re || !row_ptr || !col || !re || (cal && !im) || (phi_im && !cal)) return -1;
+  const int np = c->npoint, nl = c->nlmso, nb = (int)c->bnd.size(), L = c->lmaxso, nlig = c->nligne;
+  std::vector<cd> B((size_t)nb * (L + 1)), N((size_t)nb * (L + 1));
+  outer_functions(c, Eimag, Ecinetic_out, E_comp, B.data(), N.data());
+  auto cfac = [&](int b, int l) -> cd {  // mat.f90:1330-1346
+    const cd bb = B[(size_t)b * (L + 1) + l], nn = N[(size_t)b * (L + 1) + l];
+    if (basereel) return nn;
+    return bb + cd(0, 1) * nn;
+  };
+  std::vector<size_t> phoff(c->natome + 1, 0);  // each atom's (nbord, nlmsa) block of phiato
+  for (int a = 0; a < c->natome; a++) phoff[a + 1] = phoff[a] + c->abord[a].size() * c->nlmsa[a];
+  std::memcpy(row_ptr, c->rp.data(), sizeof(int64_t) * (nlig + 1));
+  std::memcpy(col, c->cols.data(), sizeof(int) * c->cols.size());
+  std::fill(re, re + c->cols.size(), 0.0);
+  if (cal) std::fill(im, im + c->cols.size(), 0.0);
+  if (rhs_re) std::memset(rhs_re, 0, sizeof(double) * (size_t)nlig * nl);
+  if (rhs_im && cal) std::memset(rhs_im, 0, sizeof(double) * (size_t)nlig * nl);
+  int64_t q0 = 0, q1 = 0;
+  auto at = [&](int column) -> int64_t {  // stored position of a 1-based column in the current row
+    return std::lower_bound(c->cols.begin() + q0, c->cols.begin() + q1, column) - c->cols.begin();
+  };
+  auto add = [&](int column, double vr, double vi) {
+    const int64_t q = at(column);
+    re[q] += vr;
+    if (cal) im[q] += vi;
+  };
+  for (int r = 0; r < nlig; r++) {
+    q0 = c->rp[r];
+    q1 = c->rp[r + 1];
+    if (c->rkind[r] == 0) {  // grid row, mat.f90:1142-1398 (P1: rvol = 1, no character)
+      const int i = c->ridx[r];
+      add(r + 1, V[i] - Enervide - c->clapl[0], E_comp ? -Eimag : 0.0);
+      for (int d = 0; d < 12; d++) {
+        const int j = c->nbr[(size_t)i * 12 + d];
+        const double w = -c->clapl[d + 1];
+        if (j < np && !c->numia[j]) {  // :1227-1232
+          add(c->row[j] + 1, w, 0.0);
+        } else if (j < np) {  // a neighbour inside atom a couples to its expansion, :1233-1309
+          const int a = c->numia[j] - 1;
+          const std::vector<int> &bl = c->abord[a];
+          const int ib = (int)(std::find(bl.begin() + c->nbordf[a], bl.end(), j) - bl.begin());
+          const double *pr = phi_re + phoff[a] + (size_t)ib * c->nlmsa[a];
+          const double *pi = phi_im ? phi_im + phoff[a] + (size_t)ib * c->nlmsa[a] : nullptr;
+          for (int lm = 0; lm < c->nlmsa[a]; lm++) add(c->ianew[a] + lm + 1, w * pr[lm], pi ? w * pi[lm] : 0.0);
+        } else {  // outside the sphere: the outer unknowns and the incoming wave, :1312-1398
+          const int b = c->slot[j];
+          for (int lm = 0; lm < nl; lm++) {
+            const int l = lof(c->lmidx[lm]);
+            const double y = c->ylm[(size_t)b * nl + lm];
+            const cd v = cal ? cfac(b, l) * y * w : cd(w * (y * N[(size_t)b * (L + 1) + l].real()));
+            add(c->nso + lm + 1, v.real(), v.imag());
+            if (rhs_re) {
+              const cd s = cal ? y * B[(size_t)b * (L + 1) + l] * w : cd(w * (y * B[(size_t)b * (L + 1) + l].real()));
+              rhs_re[(size_t)r * nl + lm] -= s.real();
+              if (cal && rhs_im) rhs_im[(size_t)r * nl + lm] -= s.imag();
+            }
+          }
+        }
+      }
+    } else if (c->rkind[r] == 1) {  // expansion row Lm of atom a, mat.f90:1515-1632
+      const int a = c->ridx[r] >> 16, lm = c->ridx[r] & 0xffff, na = c->nlmsa[a];
+      const std::vector<double> &pw = c->apoids[a], &Y = c->aylm[a];
+      for (int b = 0; b < c->nbordf[a]; b++) add(c->row[c->abord[a][b]] + 1, -(pw[b] * Y[(size_t)b * na + lm]), 0.0);
+      for (int lp = 0; lp < na; lp++) {
+        double sr = 0, si = 0;
+        for (int b = 0; b < c->nbordf[a]; b++) {
+          const double y = Y[(size_t)b * na + lm], pr = phi_re[phoff[a] + (size_t)b * na + lp];
+          if (cal) {
+            sr += pw[b] * (pr * y);
+            if (phi_im) si += pw[b] * (phi_im[phoff[a] + (size_t)b * na + lp] * y);
+          } else {
+            sr += pw[b] * y * pr;
+          }
+        }
+        add(c->ianew[a] + lp + 1, sr, si);
+      }
+    } else {  // outer row lm, mat.f90:1401-1511
+      const int lm = c->ridx[r];
+      for (int lp = 0; lp < nl; lp++) {
+        const int l = lof(c->lmidx[lp]);
+        cd s = 0;
+        for (int b = 0; b < c->nsortf; b++) {
+          const double ym = c->ylm[(size_t)b * nl + lm], yp = c->ylm[(size_t)b * nl + lp];
+          s += cal ? (c->wso[b] * (ym * yp)) * cfac(b, l) : cd(c->wso[b] * ym * yp * N[(size_t)b * (L + 1) + l].real());
+        }
+        add(c->nso + lp + 1, s.real(), s.imag());
+      }
+      for (int b = 0; b < c->nsortf; b++) {
+        const double f = c->wso[b] * c->ylm[(size_t)b * nl + lm];
+        add(c->row[c->bnd[b]] + 1, -f, 0.0);
+        if (rhs_re)
+          for (int lf = 0; lf < nl; lf++) {
+            const int l = lof(c->lmidx[lf]);
+            const cd s = (f * c->ylm[(size_t)b * nl + lf]) * B[(size_t)b * (L + 1) + l];
+            rhs_re[(size_t)r * nl + lf] -= s.real();
+            if (cal && rhs_im) rhs_im[(size_t)r * nl + lf] -= s.imag();
+          }
+      }
+    }
+  }
+  return 0;
+}
+
+
 int fdmgpu_cluster_assembly_sizes(const fdmgpu_cluster *c, int64_t *sizes) {
+  if (c->natome > 0) return -2;  // the device assembler holds no atom rows
   int64_t nrow = 0, nlink = 0;
   for (int i = 0; i < c->npoint; i++) {
     int k = 0;
@@ -526,6 +908,7 @@ int fdmgpu_cluster_assembly_sizes(const fdmgpu_cluster *c, int64_t *sizes) {
 int fdmgpu_cluster_assembly_tables(const fdmgpu_cluster *c, double *base, int64_t *qdiag, int *orow,
                                    int64_t *oq, int *optr, int *ob, double *ow, double *ylm, double *brad,
                                    double *par) {
+  if (c->natome > 0) return -2;
   const int np = c->npoint, nl = c->nlmso;
   std::fill(base, base + c->cols.size(), 0.0);
   int r = 0, k = 0;
@@ -570,6 +953,7 @@ int fdmgpu_cluster_assembly_tables(const fdmgpu_cluster *c, double *base, int64_
 }
 
 int fdmgpu_cluster_assembly_weights(const fdmgpu_cluster *c, int *lharm, double *wso, double *dself) {
+  if (c->natome > 0) return -2;  // the device assembler holds no atom rows
   for (int k = 0; k < c->nlmso; k++) lharm[k] = lof(c->lmidx[k]);
   std::memcpy(wso, c->wso.data(), sizeof(double) * c->nsortf);
   for (int i = 0; i < c->npoint; i++) {

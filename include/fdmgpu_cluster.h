@@ -3,7 +3,8 @@
  * measured on: a cubic grid inside a sphere, ordered by radius, with the
  * 4th-order Laplacian stencil and the outer-sphere expansion rows, i.e. the
  * inputs the reference's mat_solve receives for a cluster without atoms in
- * point group P1.  It replaces the reference's set-up chain
+ * point group P1 or a mirror group, or with muffin-tin atoms in P1
+ * (fdmgpu_cluster_create_atoms).  It replaces the reference's set-up chain
  *   reseau   general.f90:9439   (grid and radial order)
  *   laplac   general.f90:9732   (stencil, neighbour table)
  *   bordure  general.f90:9895   (outer-sphere point lists and weights)
@@ -57,6 +58,44 @@ int fdmgpu_cluster_create_mmm(double rsort, double adimp, int lmaxso, int irrep,
  * potential must have the group's symmetry.  Returns -1 for any other
  * group or irrep. */
 int fdmgpu_cluster_create_group(double rsort, double adimp, int lmaxso, int group, int irrep, fdmgpu_cluster **out);
+/* A P1 cluster with natome muffin-tin atoms: posi[3*natome] (bohr, the
+ * absorber at the origin, in non-decreasing distance from it, as the
+ * reference sorts its cluster), sphere radii rmt[natome] (bohr), expansion
+ * lmax[natome] (nlmsa = (lmax+1)^2 unknowns each, lmax <= 12).  It replaces
+ * the atom branches of the set-up chain:
+ *   reseau   general.f90:9563-9577   a point within rmt + epspos of an atom
+ *                                    belongs to it (numia; the last atom in
+ *                                    posi order wins) and leaves the grid
+ *   bordure  general.f90:9924-10068  per atom, the nbordf interstitial points
+ *                                    with a first-shell neighbour inside it
+ *                                    (weights poidsa = 4 pi / nbordf), then the
+ *                                    atom's points with an interstitial neighbour
+ *   Ylmpt    general.f90:11533-11568 real harmonics about the atom at them
+ *   newind   mat.f90:1790-1884       each atom's nlmsa rows go before the
+ *                                    first interstitial point at least as far
+ *                                    from the absorber as the atom (two atoms
+ *                                    at one place follow each other), and the
+ *                                    profile of the grid, atom and outer rows
+ * Rows: the npr interstitial points in radial order with the atom blocks
+ * inserted, then the nlmso outer rows (nligne = npr + sum nlmsa + nlmso).
+ * Assemble with fdmgpu_cluster_assemble_atoms; fdmgpu_cluster_assemble and
+ * the device assembler's tables return -2 for such a cluster.  Returns -1 on
+ * bad arguments, unsorted atoms or an atom that touches no interstitial
+ * point. */
+int fdmgpu_cluster_create_atoms(double rsort, double adimp, int lmaxso, int natome, const double *posi,
+                                const double *rmt, const int *lmax, fdmgpu_cluster **out);
+/* sizes[0..3] = natome, npr (interstitial points), sum of nbord, and the
+ * length of phiato (sum of nbord * nlmsa) */
+int fdmgpu_cluster_atom_sizes(const fdmgpu_cluster *c, int64_t *sizes);
+/* Atom a (0-based): info[4] = nbord, nbordf, nlmsa, ianew (rows ianew ..
+ * ianew + nlmsa - 1, 0-based, are its expansion); ibord[nbord] its border
+ * points (0-based), the nbordf outside ones first; poidsa[nbordf];
+ * ylmato[nbord * nlmsa] row-major.  Any output may be NULL. */
+int fdmgpu_cluster_atom(const fdmgpu_cluster *c, int a, int *info, int *ibord, double *poidsa, double *ylmato);
+/* numia[npoint]: 0, or 1 + the atom a point lies in; row[npoint]: the
+ * 0-based row of an interstitial point, -1 for a point inside an atom.
+ * Either may be NULL. */
+int fdmgpu_cluster_point_rows(const fdmgpu_cluster *c, int *numia, int *row);
 /* rvol of the npoint interior points (1 for P1) */
 int fdmgpu_cluster_rvol(const fdmgpu_cluster *c, double *rvol);
 /* harmonic index l*l + l + m of each of the nlmso outer unknowns */
@@ -77,6 +116,20 @@ int64_t fdmgpu_cluster_nnz(const fdmgpu_cluster *c);
 int fdmgpu_cluster_assemble(const fdmgpu_cluster *c, const double *V, double Enervide, double Eimag,
                             double Ecinetic_out, int E_comp, int basereel, int64_t *row_ptr, int *col,
                             double *re, double *im, double *rhs_re, double *rhs_im);
+/* The same for a cluster with atoms (calcMatRow, mat.f90:1034-1632), given
+ * the atoms' radial solutions at their border points, phiato: for each atom
+ * in turn an (nbord, nlmsa) row-major block, phi_re and, for complex radial
+ * solutions (the reference's nphiato7 = 2), phi_im (NULL: real; must be NULL
+ * when the arithmetic is real).  Grid rows with a neighbour inside atom a
+ * couple to a's expansion with -clapl(v) phiato at that neighbour
+ * (:1233-1309); atom row Lm holds -poidsa Ylm in the columns of the nbordf
+ * outside border points and sum_ib poidsa Ylm phiato(ib, Lm') in its own
+ * block (:1515-1632); atom rows carry no incoming wave.  V[npoint] (the
+ * values at points inside atoms are not used). */
+int fdmgpu_cluster_assemble_atoms(const fdmgpu_cluster *c, const double *V, double Enervide, double Eimag,
+                                  double Ecinetic_out, int E_comp, int basereel, const double *phi_re,
+                                  const double *phi_im, int64_t *row_ptr, int *col, double *re, double *im,
+                                  double *rhs_re, double *rhs_im);
 
 /* E-independent tables of one structure's per-energy assembly: the parts of
  * calcMatRow (mat.f90:1142-1232 grid rows, :1312-1398 outer columns,

@@ -392,10 +392,11 @@ def phiso_sys(s, E_comp: bool, Ecinetic_out: float, Eimag: float, Enervide: floa
     return (bes, neu) if E_comp else (besr, neur)
 
 
-def mat_solve_sys(s, V, Enervide, Eimag, bes, neu, E_comp, cal_comp, basereel, phiato):
+def mat_solve_sys(s, V, Enervide, Eimag, bes, neu, E_comp, cal_comp, basereel, phiato, fn=None):
     """Reference mat_solve (with its calcMatRow) on an atom_system dict.
     phiato: list over atoms of (nbord(ia), nlmsa(ia)) arrays, complex
-    (nphiato7 = 2) or real.  Returns the solution (nligne, nlmso)."""
+    (nphiato7 = 2) or real.  Returns the solution (nligne, nlmso).  ``fn``
+    calls another procedure with the same argument surface instead."""
     A = _Args()
     n, nl, ns, lm = s["nligne"], s["nlmso"], s["nsort"], s["lmaxso"]
     natome, nbtm, nlmmax = s["natome"], s["nbtm"], s["nlmmax"]
@@ -456,7 +457,7 @@ def mat_solve_sys(s, V, Enervide, Eimag, bes, neu, E_comp, cal_comp, basereel, p
         A.arr(s["ylmso"], np.float64),
     ]
     assert len(args) == 93
-    lib().mat_solve_(*args)
+    (fn or lib().mat_solve_)(*args)
     x = smr.T.copy()
     if cal_comp:
         x = x + 1j * smi.T
