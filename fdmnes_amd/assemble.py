@@ -4,7 +4,9 @@ the GPU once, then each call writes the compressed-row values (and the
 incoming-wave RHS) of up to 16 energy points straight into device buffers
 the batched solver reads.  Stands in for the per-row calcMatRow calls of
 mat.f90:1034-1727 (grid rows :1142-1232, outer columns :1312-1398, outer
-rows :1401-1511) for a cluster without atoms."""
+rows :1401-1511; with muffin-tin atoms also the couplings to the atoms'
+expansions :1233-1309 and the atom rows :1515-1632, from the caller's
+phiato)."""
 from __future__ import annotations
 
 import ctypes as C
@@ -27,6 +29,12 @@ class Tables(C.Structure):
                 ("ob", _V), ("ow", _V), ("ylm", _V), ("brad", _V), ("lharm", _V), ("wso", _V), ("dself", _V)]
 
 
+class AtomTables(C.Structure):
+    _fields_ = [("natome", C.c_int), ("nar", C.c_int), ("nal", C.c_int), ("nphi", C.c_int64),
+                ("aq", _V), ("aa", _V), ("aptr", _V), ("alphi", _V), ("alw", _V), ("ianew", _V), ("nlmsa", _V),
+                ("nbordf", _V), ("phoff", _V), ("bq", _V), ("poidsa", _V), ("ylmato", _V)]
+
+
 def _sigs(L):
     if getattr(L, "_asm_sigs", False):
         return
@@ -36,6 +44,9 @@ def _sigs(L):
     L.fdmgpu_assembler_destroy.restype = None
     L.fdmgpu_assembler_run.argtypes = [_V, C.c_int, P(C.c_double), P(C.c_double), P(C.c_double), C.c_int, C.c_int,
                                        C.c_int, P(_V), P(_V), P(_V)]
+    L.fdmgpu_assembler_create_atoms.argtypes = [P(Tables), P(AtomTables), C.c_int, _V, P(_V)]
+    L.fdmgpu_assembler_run_atoms.argtypes = [_V, C.c_int, P(C.c_double), P(C.c_double), P(C.c_double), C.c_int,
+                                             C.c_int, C.c_int, P(_V), P(_V), C.c_int, P(_V), P(_V)]
     L._asm_sigs = True
 
 
@@ -71,7 +82,19 @@ class DeviceAssembler:
                     t["nlink"], t["nnz"], t["clapl0"], t["pw"], *[self._keep[k].ctypes.data_as(_V) for k in names])
         self.stream = library_stream(stream)
         h = _V()
-        _lib.check(L.fdmgpu_assembler_create(C.byref(tb), int(max_points), _V(self.stream.cuda_stream), C.byref(h)))
+        self.natome = int(getattr(cluster, "natome", 0))
+        if self.natome:
+            from .cluster import ATOM_TABLES
+            at = cluster.atom_assembly_tables()
+            self.nphi = at["nphi"]
+            self._keep_at = {k: np.ascontiguousarray(at[k]) for k in ATOM_TABLES}
+            atb = AtomTables(at["natome"], at["nar"], at["nal"], at["nphi"],
+                             *[self._keep_at[k].ctypes.data_as(_V) for k in ATOM_TABLES])
+            _lib.check(L.fdmgpu_assembler_create_atoms(C.byref(tb), C.byref(atb), int(max_points),
+                                                       _V(self.stream.cuda_stream), C.byref(h)))
+        else:
+            _lib.check(L.fdmgpu_assembler_create(C.byref(tb), int(max_points), _V(self.stream.cuda_stream),
+                                                 C.byref(h)))
         self._h = h
         self.max_points = int(max_points)
 
@@ -80,14 +103,16 @@ class DeviceAssembler:
             self.L.fdmgpu_assembler_destroy(self._h)
             self._h = None
 
-    def run(self, V, energies, E_comp: bool, basereel: bool, vals, rhs=None, update_only: bool = False):
+    def run(self, V, energies, E_comp: bool, basereel: bool, vals, rhs=None, update_only: bool = False, phi=None):
         """V: list of float64 device tensors (npoint); energies: list of
         (Enervide, Eimag, Ecinetic_out) in Ry; vals (and rhs) lists of
         contiguous device tensors, complex128 when the arithmetic is complex
         (E_comp or not basereel), float64 otherwise.  update_only: the
         buffers hold an earlier call's output for this cluster (same
         arithmetic branch); only the energy- and V-dependent entries are
-        rewritten."""
+        rewritten.  phi (clusters with atoms): per point a device tensor
+        of the nphi phiato values (Cluster.assemble_atoms's blocks,
+        concatenated), complex128 (only in complex arithmetic) or float64."""
         n = len(energies)
         if n == 0:
             return
@@ -106,16 +131,29 @@ class DeviceAssembler:
         for t in V:
             if t.dtype != torch.float64 or t.numel() != self.npoint:
                 raise ValueError("V: float64, npoint entries")
+        if (phi is not None) != bool(self.natome):
+            raise ValueError("phi is given exactly for a cluster with atoms")
+        phi_cplx = False
+        if phi is not None:
+            if len(phi) != n:
+                raise ValueError("one phi per energy point")
+            phi_cplx = phi[0].is_complex()
+            for t in phi:
+                if t.is_complex() != phi_cplx or t.numel() != self.nphi or t.dtype not in (torch.float64,
+                                                                                          torch.complex128):
+                    raise ValueError(f"phi: nphi = {self.nphi} float64 or complex128 entries, one kind for all")
+            if phi_cplx and not cal:
+                raise ValueError("complex radial solutions need complex arithmetic (E_comp or not basereel)")
         E = np.array(energies, np.float64).reshape(n, 3)
         D = C.POINTER(C.c_double)
         ev = [np.ascontiguousarray(E[:, k]) for k in range(3)]
         with Ordered(self.stream):
             for k0 in range(0, n, self.max_points):
                 k1 = min(n, k0 + self.max_points)
-                rc = self.L.fdmgpu_assembler_run(
+                rc = self.L.fdmgpu_assembler_run_atoms(
                     self._h, k1 - k0, *[np.ascontiguousarray(a[k0:k1]).ctypes.data_as(D) for a in ev],
                     int(bool(E_comp)), int(bool(basereel)), int(bool(update_only)), _dptrs(V[k0:k1]),
-                    _dptrs(vals[k0:k1]),
+                    _dptrs(phi[k0:k1]) if phi is not None else None, int(phi_cplx), _dptrs(vals[k0:k1]),
                     _dptrs(rhs[k0:k1]) if rhs is not None else None)
                 if rc:
                     break

@@ -55,6 +55,12 @@ def _L():
         L.fdmgpu_cluster_assemble_atoms.argtypes = [C.c_void_p, _lib.dp, C.c_double, C.c_double, C.c_double,
                                                     C.c_int, C.c_int, _lib.dp, C.c_void_p, _lib.lp, _lib.ip,
                                                     _lib.dp, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.fdmgpu_cluster_atom_assembly_sizes.restype = C.c_int
+        L.fdmgpu_cluster_atom_assembly_sizes.argtypes = [C.c_void_p, _lib.lp]
+        L.fdmgpu_cluster_atom_assembly_tables.restype = C.c_int
+        L.fdmgpu_cluster_atom_assembly_tables.argtypes = [C.c_void_p, _lib.lp, _lib.ip, _lib.ip, _lib.lp, _lib.dp,
+                                                          _lib.ip, _lib.ip, _lib.ip, _lib.lp, _lib.lp, _lib.dp,
+                                                          _lib.dp]
         L._cluster_sigs = True
     return L
 
@@ -238,7 +244,32 @@ class Cluster:
         rc = L.fdmgpu_cluster_assembly_weights(self._h, t["lharm"], t["wso"], t["dself"])
         if rc:
             raise _lib.FdmGpuError(f"fdmgpu_cluster_assembly_weights failed ({rc})")
-        rp, _, _, _ = self.assemble(np.zeros(self.npoint), 0.0, 0.0, 1.0, False, True)
+        if self.natome:
+            phi = [np.zeros((self.atom(a)["nbord"], self.atom(a)["nlmsa"])) for a in range(self.natome)]
+            rp, _, _, _ = self.assemble_atoms(np.zeros(self.npoint), 0.0, 0.0, 1.0, False, True, phi)
+        else:
+            rp, _, _, _ = self.assemble(np.zeros(self.npoint), 0.0, 0.0, 1.0, False, True)
         t["row_ptr"] = rp
         t["clapl0"], t["pw"] = float(t["par"][0]), float(t["par"][1])
         return t
+
+    def atom_assembly_tables(self) -> dict:
+        """The device assembler's atom tables (fdmgpu_cluster_atom_assembly_tables)."""
+        if not self.natome:
+            raise ValueError("the cluster has no atoms")
+        L = _L()
+        sz = np.zeros(7, np.int64)
+        L.fdmgpu_cluster_atom_assembly_sizes(self._h, sz)
+        na, nar, nal, nphi, nrow, nbf, nyl = map(int, sz)
+        t = dict(natome=na, nar=nar, nal=nal, nphi=nphi, aq=np.zeros(max(nar, 1), np.int64),
+                 aa=np.zeros(max(nar, 1), np.int32), aptr=np.zeros(nar + 1, np.int32),
+                 alphi=np.zeros(max(nal, 1), np.int64), alw=np.zeros(max(nal, 1)), ianew=np.zeros(na, np.int32),
+                 nlmsa=np.zeros(na, np.int32), nbordf=np.zeros(na, np.int32), phoff=np.zeros(na, np.int64),
+                 bq=np.zeros(nrow, np.int64), poidsa=np.zeros(nbf), ylmato=np.zeros(nyl))
+        rc = L.fdmgpu_cluster_atom_assembly_tables(self._h, *[t[k] for k in ATOM_TABLES])
+        if rc:
+            raise _lib.FdmGpuError(f"fdmgpu_cluster_atom_assembly_tables failed ({rc})")
+        return t
+
+
+ATOM_TABLES = ("aq", "aa", "aptr", "alphi", "alw", "ianew", "nlmsa", "nbordf", "phoff", "bq", "poidsa", "ylmato")

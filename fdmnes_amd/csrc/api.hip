@@ -557,8 +557,8 @@ struct fdmgpu_assembler {
   Assembler a;
 };
 
-int fdmgpu_assembler_create(const fdmgpu_assembly_tables *t, int max_points, void *hip_stream,
-                            fdmgpu_assembler **out) {
+int fdmgpu_assembler_create_atoms(const fdmgpu_assembly_tables *t, const fdmgpu_atom_tables *at, int max_points,
+                                  void *hip_stream, fdmgpu_assembler **out) {
   if (!t || !out) {
     set_error("fdmgpu_assembler_create: invalid argument");
     return FDMGPU_EINVAL;
@@ -567,24 +567,35 @@ int fdmgpu_assembler_create(const fdmgpu_assembly_tables *t, int max_points, voi
   if (rc) return rc;
   std::unique_ptr<fdmgpu_assembler> h(new (std::nothrow) fdmgpu_assembler());
   if (!h) return FDMGPU_ENOMEM;
-  rc = h->a.init(t, max_points, hip_stream ? (hipStream_t)hip_stream : g_lib.stream);
+  rc = h->a.init(t, at, max_points, hip_stream ? (hipStream_t)hip_stream : g_lib.stream);
   if (rc) return rc;
   *out = h.release();
   return 0;
 }
 
+int fdmgpu_assembler_create(const fdmgpu_assembly_tables *t, int max_points, void *hip_stream,
+                            fdmgpu_assembler **out) {
+  return fdmgpu_assembler_create_atoms(t, nullptr, max_points, hip_stream, out);
+}
+
 void fdmgpu_assembler_destroy(fdmgpu_assembler *a) { delete a; }
 
-int fdmgpu_assembler_run(fdmgpu_assembler *a, int count, const double *Enervide, const double *Eimag,
-                         const double *Ecinetic_out, int E_comp, int basereel, int update_only,
-                         const double *const *d_V, double *const *d_vals, double *const *d_rhs) {
+int fdmgpu_assembler_run_atoms(fdmgpu_assembler *a, int count, const double *Enervide, const double *Eimag,
+                               const double *Ecinetic_out, int E_comp, int basereel, int update_only,
+                               const double *const *d_V, const double *const *d_phi, int phi_cplx,
+                               double *const *d_vals, double *const *d_rhs) {
   if (!a || !Enervide || !Eimag || !Ecinetic_out || !d_V || !d_vals || count < 1 || count > ASM_MAXP) {
     set_error("fdmgpu_assembler_run: invalid argument");
     return FDMGPU_EINVAL;
   }
+  if ((a->a.natome > 0) != (d_phi != nullptr)) {
+    set_error(a->a.natome > 0 ? "fdmgpu_assembler_run: a cluster with atoms needs phiato (fdmgpu_assembler_run_atoms)"
+                              : "fdmgpu_assembler_run_atoms: the assembler holds no atoms");
+    return FDMGPU_EINVAL;
+  }
   AsmPoints P{};
   for (int k = 0; k < count; k++) {
-    if (!d_V[k] || !d_vals[k]) {
+    if (!d_V[k] || !d_vals[k] || (d_phi && !d_phi[k])) {
       set_error("fdmgpu_assembler_run: null device buffer for point " + std::to_string(k));
       return FDMGPU_EINVAL;
     }
@@ -592,9 +603,17 @@ int fdmgpu_assembler_run(fdmgpu_assembler *a, int count, const double *Enervide,
     P.V[k] = d_V[k];
     P.vals[k] = d_vals[k];
     P.rhs[k] = d_rhs ? d_rhs[k] : nullptr;
+    P.phi[k] = d_phi ? d_phi[k] : nullptr;
   }
   FDM_HIP(hipSetDevice(g_lib.device));
-  return a->a.run(count, P, E_comp != 0, basereel != 0, update_only != 0);
+  return a->a.run(count, P, E_comp != 0, basereel != 0, update_only != 0, phi_cplx != 0);
+}
+
+int fdmgpu_assembler_run(fdmgpu_assembler *a, int count, const double *Enervide, const double *Eimag,
+                         const double *Ecinetic_out, int E_comp, int basereel, int update_only,
+                         const double *const *d_V, double *const *d_vals, double *const *d_rhs) {
+  return fdmgpu_assembler_run_atoms(a, count, Enervide, Eimag, Ecinetic_out, E_comp, basereel, update_only, d_V,
+                                    nullptr, 0, d_vals, d_rhs);
 }
 
 int fdmgpu_tau_reduce(const fdmgpu_tau_desc *d, int count, const double *const *d_x, double *const *d_tau,

@@ -19,6 +19,7 @@ struct AsmPoints {
   const double *V[ASM_MAXP];  // device, npoint
   double *vals[ASM_MAXP];     // device, nnz (interleaved when complex)
   double *rhs[ASM_MAXP];      // device, (nligne, nlmso) row-major, or null
+  const double *phi[ASM_MAXP];  // device, phiato of a cluster with atoms (interleaved when complex)
 };
 
 struct Assembler {
@@ -35,12 +36,22 @@ struct Assembler {
   double *wso = nullptr, *dself = nullptr;       // outer weights; diagonal mirror terms (null: none)
   double *fun = nullptr;   // Bessel/Neumann table: [point][nb][lmaxso+1] x {B, N} complex
   double *part = nullptr;  // outer-row partial sums: [point][run][nlmso^2] x {block, RHS} complex
+  // muffin-tin atoms (natome = 0: none)
+  int natome = 0, nar = 0, nal = 0, nrow_atom = 0, nlmax = 0;
+  int64_t nphi = 0;
+  int64_t *aq = nullptr, *alphi = nullptr, *bq = nullptr, *phoff = nullptr, *poff = nullptr, *yoff = nullptr;
+  int *aa = nullptr, *aptr = nullptr, *ianew = nullptr, *nlmsa = nullptr, *nbordf = nullptr, *arow_atom = nullptr,
+      *arow_lm = nullptr;
+  double *alw = nullptr, *poidsa = nullptr, *ylmato = nullptr;
 
   ~Assembler();
-  int init(const fdmgpu_assembly_tables *t, int max_points, hipStream_t s);
+  // at: null for a cluster without atoms
+  int init(const fdmgpu_assembly_tables *t, const fdmgpu_atom_tables *at, int max_points, hipStream_t s);
+  int init_atoms(const fdmgpu_atom_tables *at, int64_t natrow, const int64_t *row_ptr_h);
   // update_only: vals/rhs already hold an earlier call's output for this
-  // structure; only the energy-dependent entries are rewritten
-  int run(int count, const AsmPoints &pts, bool e_comp, bool basereel, bool update_only);
+  // structure; only the energy-dependent entries are rewritten.  phi_cplx:
+  // the points' phiato are complex (atoms only)
+  int run(int count, const AsmPoints &pts, bool e_comp, bool basereel, bool update_only, bool phi_cplx = false);
 };
 
 }  // namespace fdmgpu

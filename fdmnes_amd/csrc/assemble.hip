@@ -1,6 +1,6 @@
 // Per-energy assembly of a cluster's compressed rows on the device: the
-// energy-dependent entries of calcMatRow (mat.f90:1142-1511) for a cluster
-// without atoms, written into the value array the solver reads.  The tables
+// energy-dependent entries of calcMatRow (mat.f90:1142-1632), written into
+// the value array the solver reads.  The tables
 // of everything that does not change with the energy are uploaded once; a
 // call then computes, for up to ASM_MAXP points at a time,
 //   1. the outer-sphere functions fnorm j_l(k r), fnorm n_l(k r) at every
@@ -12,7 +12,11 @@
 //   3. the grid diagonals V - E - clapl(0) (- i Eimag)   (mat.f90:1147-1150),
 //   4. the outer columns of grid rows with neighbours outside the sphere and
 //      their incoming-wave RHS                         (mat.f90:1312-1398),
-//   5. the outer rows' outer-column block and RHS       (mat.f90:1401-1511).
+//   5. the outer rows' outer-column block and RHS       (mat.f90:1401-1511),
+// and for a cluster with muffin-tin atoms, from the caller's radial
+// solutions phiato at the atoms' border points,
+//   6. the grid rows' couplings to an atom's expansion  (mat.f90:1233-1309),
+//   7. the atom rows' own blocks                         (mat.f90:1583-1632).
 // Multiply-adds are not fused (contraction off) and the grid rows' sums run
 // in the reference's term order; the outer rows' sums over the boundary
 // points are taken in runs of 256 points, then over the runs, in order.  The entries agree with a
@@ -162,6 +166,7 @@ __global__ void k_asm_diag(int npoint, const int64_t *__restrict__ qdiag, double
   if (i >= npoint) return;
   const AsmEnergy E = P.e[g];
   const int64_t q = qdiag[i];
+  if (q < 0) return;  // a point inside an atom has no row
   double d = P.V[g][i] - E.ener - clapl0;
   if (dself) {  // mirror images of the point itself, added in axis order (mat.f90:1225-1231)
     d = d + dself[3 * i];
@@ -243,7 +248,7 @@ __global__ __launch_bounds__(256) void k_asm_outrows_part(int nl, int nb, int L,
   o[1] = sr;
 }
 template <bool CAL>
-__global__ void k_asm_outrows_sum(int npoint, int nl, int nsortf, int nrun, const int64_t *__restrict__ row_ptr,
+__global__ void k_asm_outrows_sum(int nso, int nl, int nsortf, int nrun, const int64_t *__restrict__ row_ptr,
                                   const cz *__restrict__ part, AsmPoints P) {
   const int pr = blockIdx.x * blockDim.x + threadIdx.x, g = blockIdx.y;
   if (pr >= nl * nl) return;
@@ -254,7 +259,7 @@ __global__ void k_asm_outrows_sum(int npoint, int nl, int nsortf, int nrun, cons
     t = add(t, o[0]);
     tr = add(tr, o[1]);
   }
-  const int ii = npoint + lm;
+  const int ii = nso + lm;
   const int64_t q = row_ptr[ii] + nsortf + lp, qr = (int64_t)ii * nl + lp;
   double *v = P.vals[g], *rh = P.rhs[g];
   if constexpr (CAL) {
@@ -270,6 +275,77 @@ __global__ void k_asm_outrows_sum(int npoint, int nl, int nsortf, int nrun, cons
   }
 }
 
+// 6. a grid row's couplings to the expansion of an atom its stencil reaches:
+// the sum over its neighbours inside the atom, in stencil order, of
+// -clapl(v) phiato(neighbour, Lm); one thread per (row-atom pair, Lm)
+template <bool CAL, bool PHIC>
+__global__ void k_asm_atomcols(int nar, int nlmax, const int64_t *__restrict__ aq, const int *__restrict__ aa,
+                               const int *__restrict__ aptr, const int64_t *__restrict__ alphi,
+                               const double *__restrict__ alw, const int *__restrict__ nlmsa, AsmPoints P) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x, g = blockIdx.y;
+  if (t >= nar * nlmax) return;
+  const int r = t / nlmax, lm = t % nlmax;
+  if (lm >= nlmsa[aa[r]]) return;
+  const double *phi = P.phi[g];
+  double sr = 0, si = 0;
+  for (int k = aptr[r]; k < aptr[r + 1]; k++) {
+    const double w = alw[k];
+    const int64_t o = alphi[k] + lm;
+    if constexpr (PHIC) {
+      sr = sr + w * phi[2 * o];
+      si = si + w * phi[2 * o + 1];
+    } else {
+      sr = sr + w * phi[o];
+    }
+  }
+  const int64_t q = aq[r] + lm;
+  double *v = P.vals[g];
+  if constexpr (CAL) {
+    v[2 * q] = sr;
+    v[2 * q + 1] = si;
+  } else {
+    v[q] = sr;
+  }
+}
+
+// 7. atom row (a, Lm), column Lm' of its own block: the sum over the atom's
+// outside border points, in border order, of poidsa Ylmato(Lm) phiato(Lm');
+// one thread per (atom row, Lm')
+template <bool CAL, bool PHIC>
+__global__ void k_asm_atomrows(int nrow_atom, int nlmax, const int *__restrict__ arow_atom,
+                               const int *__restrict__ arow_lm, const int64_t *__restrict__ bq,
+                               const int *__restrict__ nlmsa, const int *__restrict__ nbordf,
+                               const int64_t *__restrict__ phoff, const int64_t *__restrict__ poff,
+                               const int64_t *__restrict__ yoff, const double *__restrict__ poidsa,
+                               const double *__restrict__ ylmato, AsmPoints P) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x, g = blockIdx.y;
+  if (t >= nrow_atom * nlmax) return;
+  const int k = t / nlmax, lp = t % nlmax, a = arow_atom[k], lm = arow_lm[k], na = nlmsa[a];
+  if (lp >= na) return;
+  const double *phi = P.phi[g], *pw = poidsa + poff[a], *Y = ylmato + yoff[a];
+  double sr = 0, si = 0;
+  for (int b = 0; b < nbordf[a]; b++) {
+    const double y = Y[(int64_t)b * na + lm];
+    const int64_t o = phoff[a] + (int64_t)b * na + lp;
+    if constexpr (!CAL) {
+      sr = sr + pw[b] * y * phi[o];
+    } else if constexpr (PHIC) {
+      sr = sr + pw[b] * (phi[2 * o] * y);
+      si = si + pw[b] * (phi[2 * o + 1] * y);
+    } else {
+      sr = sr + pw[b] * (phi[o] * y);
+    }
+  }
+  const int64_t q = bq[k] + lp;
+  double *v = P.vals[g];
+  if constexpr (CAL) {
+    v[2 * q] = sr;
+    v[2 * q + 1] = si;
+  } else {
+    v[q] = sr;
+  }
+}
+
 template <class T>
 int upload(T **d, const T *h, int64_t n) {
   FDM_HIP(hipMalloc(d, sizeof(T) * (n > 0 ? n : 1)));
@@ -282,13 +358,35 @@ int upload(T **d, const T *h, int64_t n) {
 Assembler::~Assembler() {
   for (void *p : {(void *)row_ptr, (void *)qdiag, (void *)oq, (void *)orow, (void *)optr, (void *)ob, (void *)base,
                   (void *)ow, (void *)ylm, (void *)brad, (void *)lharm, (void *)wso, (void *)dself, (void *)fun,
-                  (void *)part})
+                  (void *)part, (void *)aq, (void *)alphi, (void *)bq, (void *)phoff, (void *)poff, (void *)yoff,
+                  (void *)aa, (void *)aptr, (void *)ianew, (void *)nlmsa, (void *)nbordf, (void *)arow_atom,
+                  (void *)arow_lm, (void *)alw, (void *)poidsa, (void *)ylmato})
     if (p) (void)hipFree(p);
 }
 
-int Assembler::init(const fdmgpu_assembly_tables *t, int maxp, hipStream_t s) {
+int Assembler::init(const fdmgpu_assembly_tables *t, const fdmgpu_atom_tables *at, int maxp, hipStream_t s) {
   const int nall = (t->lmaxso + 1) * (t->lmaxso + 1);  // a representation of mmm has a subset (lharm)
-  if (t->npoint <= 0 || t->nligne != t->npoint + t->nlmso || t->lmaxso < 0 || t->lmaxso > 40 ||
+  int64_t natrow = 0;  // rows of the atoms' expansions
+  if (at) {
+    if (at->natome < 1 || at->nar < 0 || at->nal < at->nar || at->nphi < 1 || !at->ianew || !at->nlmsa ||
+        !at->nbordf || !at->phoff || !at->bq || !at->poidsa || !at->ylmato ||
+        (at->nar > 0 && (!at->aq || !at->aa || !at->aptr || !at->alphi || !at->alw))) {
+      set_error("fdmgpu_assembler_create_atoms: inconsistent atom table sizes or null table");
+      return FDMGPU_EINVAL;
+    }
+    for (int a = 0; a < at->natome; a++) {
+      if (at->nlmsa[a] < 1 || at->nlmsa[a] > 169 || at->nbordf[a] < 1 || at->ianew[a] < 0 ||
+          at->ianew[a] + at->nlmsa[a] > t->nligne - t->nlmso || at->phoff[a] < 0 ||
+          at->phoff[a] + (int64_t)at->nbordf[a] * at->nlmsa[a] > at->nphi) {
+        set_error("fdmgpu_assembler_create_atoms: atom " + std::to_string(a) + " outside the rows or phiato");
+        return FDMGPU_EINVAL;
+      }
+      natrow += at->nlmsa[a];
+    }
+  }
+  const int64_t npr = t->nligne - t->nlmso - natrow;  // interstitial grid rows
+  if (t->npoint <= 0 || (at ? (npr < 1 || npr > t->npoint) : t->nligne != t->npoint + t->nlmso) || t->lmaxso < 0 ||
+      t->lmaxso > 40 ||
       (t->lharm ? (t->nlmso < 1 || t->nlmso > nall) : t->nlmso != nall) || t->nsortf <= 0 || t->nb < t->nsortf ||
       t->nrow_out < 0 ||
       t->nlink < t->nrow_out || t->nnz <= 0 || maxp < 1 || maxp > ASM_MAXP) {
@@ -305,14 +403,16 @@ int Assembler::init(const fdmgpu_assembly_tables *t, int maxp, hipStream_t s) {
     set_error("fdmgpu_assembler_create: row_ptr does not span nnz");
     return FDMGPU_EINVAL;
   }
+  const int nso = t->nligne - t->nlmso;  // first outer row
   for (int i = 0; i < t->npoint; i++)
-    if (t->qdiag[i] < t->row_ptr[i] || t->qdiag[i] >= t->row_ptr[i + 1]) {
+    if (at ? (t->qdiag[i] < -1 || t->qdiag[i] >= t->nnz)
+           : (t->qdiag[i] < t->row_ptr[i] || t->qdiag[i] >= t->row_ptr[i + 1])) {
       set_error("fdmgpu_assembler_create: diagonal position outside row " + std::to_string(i + 1));
       return FDMGPU_EINVAL;
     }
   for (int r = 0; r < t->nrow_out; r++) {
     const int i = t->orow[r];
-    if (i < 0 || i >= t->npoint || t->oq[r] < t->row_ptr[i] || t->oq[r] + t->nlmso > t->row_ptr[i + 1] ||
+    if (i < 0 || i >= nso || t->oq[r] < t->row_ptr[i] || t->oq[r] + t->nlmso > t->row_ptr[i + 1] ||
         t->optr[r + 1] < t->optr[r] || t->optr[r + 1] > t->nlink) {
       set_error("fdmgpu_assembler_create: outer-column table inconsistent at entry " + std::to_string(r));
       return FDMGPU_EINVAL;
@@ -334,7 +434,7 @@ int Assembler::init(const fdmgpu_assembly_tables *t, int maxp, hipStream_t s) {
       return FDMGPU_EINVAL;
     }
   for (int lm = 0; lm < t->nlmso; lm++)
-    if (t->row_ptr[t->npoint + lm + 1] - t->row_ptr[t->npoint + lm] != t->nsortf + t->nlmso) {
+    if (t->row_ptr[nso + lm + 1] - t->row_ptr[nso + lm] != t->nsortf + t->nlmso) {
       set_error("fdmgpu_assembler_create: outer row " + std::to_string(lm) + " is not nsortf + nlmso long");
       return FDMGPU_EINVAL;
     }
@@ -371,15 +471,74 @@ int Assembler::init(const fdmgpu_assembly_tables *t, int maxp, hipStream_t s) {
   FDM_HIP(hipMalloc(&fun, sizeof(cz) * (size_t)maxp * nb * (lmaxso + 1) * 2));
   const size_t nrun = (nsortf + ASM_CB - 1) / ASM_CB;
   FDM_HIP(hipMalloc(&part, sizeof(cz) * 2 * (size_t)maxp * nrun * nlmso * nlmso));
+  return at ? init_atoms(at, natrow, t->row_ptr) : 0;
+}
+
+int Assembler::init_atoms(const fdmgpu_atom_tables *at, int64_t natrow, const int64_t *row_ptr_h) {
+  // every position and phiato offset the kernels use lies in its array
+  if (at->nar > 0 && (at->aptr[0] != 0 || at->aptr[at->nar] != at->nal)) {
+    set_error("fdmgpu_assembler_create_atoms: aptr does not span nal");
+    return FDMGPU_EINVAL;
+  }
+  for (int r = 0; r < at->nar; r++) {
+    const int a = at->aa[r];
+    if (a < 0 || a >= at->natome || at->aq[r] < 0 || at->aq[r] + at->nlmsa[a] > nnz ||
+        at->aptr[r + 1] < at->aptr[r]) {
+      set_error("fdmgpu_assembler_create_atoms: row-atom pair " + std::to_string(r) + " inconsistent");
+      return FDMGPU_EINVAL;
+    }
+    for (int k = at->aptr[r]; k < at->aptr[r + 1]; k++)
+      if (at->alphi[k] < 0 || at->alphi[k] + at->nlmsa[a] > at->nphi) {
+        set_error("fdmgpu_assembler_create_atoms: phiato offset out of range");
+        return FDMGPU_EINVAL;
+      }
+  }
+  std::vector<int> ra, rl;
+  std::vector<int64_t> po(at->natome), yo(at->natome);
+  int64_t pb = 0, yb = 0;
+  for (int a = 0; a < at->natome; a++) {
+    po[a] = pb;
+    yo[a] = yb;
+    pb += at->nbordf[a];
+    yb += (int64_t)at->nbordf[a] * at->nlmsa[a];
+    nlmax = std::max(nlmax, at->nlmsa[a]);
+    for (int lm = 0; lm < at->nlmsa[a]; lm++) {
+      const int row = at->ianew[a] + lm;
+      const int64_t q = at->bq[ra.size()];
+      if (q < row_ptr_h[row] || q + at->nlmsa[a] > row_ptr_h[row + 1]) {
+        set_error("fdmgpu_assembler_create_atoms: block of atom row " + std::to_string(row + 1) + " outside the row");
+        return FDMGPU_EINVAL;
+      }
+      ra.push_back(a);
+      rl.push_back(lm);
+    }
+  }
+  natome = at->natome;
+  nar = at->nar;
+  nal = at->nal;
+  nphi = at->nphi;
+  nrow_atom = (int)natrow;
+  if (upload(&aq, at->aq, nar) || upload(&aa, at->aa, nar) || upload(&aptr, at->aptr, nar + 1) ||
+      upload(&alphi, at->alphi, nal) || upload(&alw, at->alw, nal) || upload(&ianew, at->ianew, natome) ||
+      upload(&nlmsa, at->nlmsa, natome) || upload(&nbordf, at->nbordf, natome) ||
+      upload(&phoff, at->phoff, natome) || upload(&bq, at->bq, natrow) || upload(&poidsa, at->poidsa, pb) ||
+      upload(&ylmato, at->ylmato, yb) || upload(&poff, po.data(), natome) || upload(&yoff, yo.data(), natome) ||
+      upload(&arow_atom, ra.data(), natrow) || upload(&arow_lm, rl.data(), natrow))
+    return FDMGPU_EHIP;
   return 0;
 }
 
-int Assembler::run(int count, const AsmPoints &P, bool e_comp, bool basereel, bool update_only) {
+int Assembler::run(int count, const AsmPoints &P, bool e_comp, bool basereel, bool update_only, bool phi_cplx) {
   if (count < 1 || count > max_points) {
     set_error("fdmgpu_assembler_run: count must be in 1.." + std::to_string(max_points));
     return FDMGPU_EINVAL;
   }
   const bool cal = e_comp || !basereel;
+  if (phi_cplx && !cal) {
+    set_error("fdmgpu_assembler_run_atoms: complex phiato needs complex arithmetic (E_comp or !basereel)");
+    return FDMGPU_EINVAL;
+  }
+  const int nso = nligne - nlmso;
   cz *F = (cz *)fun;
   const int NT = 256;
   hipLaunchKernelGGL(k_asm_fun, dim3((nb + NT - 1) / NT, count), dim3(NT), 0, stream, nb, lmaxso, brad, P, (int)e_comp,
@@ -400,7 +559,7 @@ int Assembler::run(int count, const AsmPoints &P, bool e_comp, bool basereel, bo
                          oq, optr, ob, ow, ylm, lharm, F, P, (int)basereel);
     hipLaunchKernelGGL(k_asm_outrows_part, gp, dim3(256), 0, stream, nlmso, nb, lmaxso, nsortf, wso, ylm, lharm,
                        F, (int)basereel, Q);
-    hipLaunchKernelGGL(k_asm_outrows_sum<true>, gs, dim3(256), 0, stream, npoint, nlmso, nsortf, nrun, row_ptr, Q, P);
+    hipLaunchKernelGGL(k_asm_outrows_sum<true>, gs, dim3(256), 0, stream, nso, nlmso, nsortf, nrun, row_ptr, Q, P);
   } else {
     if (!update_only) hipLaunchKernelGGL(k_asm_base<false>, dim3(gb, count), dim3(NT), 0, stream, nnz, nrhs_el, base, P);
     hipLaunchKernelGGL(k_asm_diag<false>, dim3(gd, count), dim3(NT), 0, stream, npoint, qdiag, clapl0, dself, P, 0);
@@ -409,7 +568,18 @@ int Assembler::run(int count, const AsmPoints &P, bool e_comp, bool basereel, bo
                          oq, optr, ob, ow, ylm, lharm, F, P, 1);
     hipLaunchKernelGGL(k_asm_outrows_part, gp, dim3(256), 0, stream, nlmso, nb, lmaxso, nsortf, wso, ylm, lharm,
                        F, 1, Q);
-    hipLaunchKernelGGL(k_asm_outrows_sum<false>, gs, dim3(256), 0, stream, npoint, nlmso, nsortf, nrun, row_ptr, Q, P);
+    hipLaunchKernelGGL(k_asm_outrows_sum<false>, gs, dim3(256), 0, stream, nso, nlmso, nsortf, nrun, row_ptr, Q, P);
+  }
+  if (natome > 0) {
+    const dim3 gc((nar * nlmax + NT - 1) / NT, count), gr((nrow_atom * nlmax + NT - 1) / NT, count);
+    auto cols = cal ? (phi_cplx ? k_asm_atomcols<true, true> : k_asm_atomcols<true, false>)
+                    : k_asm_atomcols<false, false>;
+    auto rows = cal ? (phi_cplx ? k_asm_atomrows<true, true> : k_asm_atomrows<true, false>)
+                    : k_asm_atomrows<false, false>;
+    if (nar > 0)
+      hipLaunchKernelGGL(cols, gc, dim3(NT), 0, stream, nar, nlmax, aq, aa, aptr, alphi, alw, nlmsa, P);
+    hipLaunchKernelGGL(rows, gr, dim3(NT), 0, stream, nrow_atom, nlmax, arow_atom, arow_lm, bq, nlmsa, nbordf, phoff,
+                       poff, yoff, poidsa, ylmato, P);
   }
   FDM_HIP(hipGetLastError());
   return 0;

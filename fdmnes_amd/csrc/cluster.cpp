@@ -545,6 +545,66 @@ void outer_functions(const fdmgpu_cluster *c, double Eimag, double Ecinetic_out,
   }
 }
 
+// Stored position of a 1-based column in row r (0-based) of the compressed rows
+int64_t position(const fdmgpu_cluster *c, int r, int column) {
+  return std::lower_bound(c->cols.begin() + c->rp[r], c->cols.begin() + c->rp[r + 1], column) - c->cols.begin();
+}
+
+// The energy-independent tables of a cluster with atoms, as
+// fdmgpu_cluster_assembly_tables describes them, with rows in place of
+// points: base holds the grid rows' -clapl(v) towards interstitial
+// neighbours (mat.f90:1227-1232), the outer rows' -poidso Ylm (:1476-1483)
+// and the atom rows' -poidsa Ylm (:1560-1581) in the border points' columns;
+// a point inside an atom has no diagonal (qdiag -1).
+int atom_cluster_tables(const fdmgpu_cluster *c, double *base, int64_t *qdiag, int *orow, int64_t *oq, int *optr,
+                        int *ob, double *ow, double *ylm, double *brad, double *par) {
+  const int np = c->npoint, nl = c->nlmso;
+  std::fill(base, base + c->cols.size(), 0.0);
+  int n = 0, k = 0;
+  optr[0] = 0;
+  for (int i = 0; i < np; i++) {
+    if (c->numia[i]) {
+      qdiag[i] = -1;
+      continue;
+    }
+    const int r = c->row[i];
+    qdiag[i] = position(c, r, r + 1);
+    bool out = false;
+    for (int d = 0; d < 12; d++) {
+      const int j = c->nbr[(size_t)i * 12 + d];
+      if (j < np && !c->numia[j]) {
+        base[position(c, r, c->row[j] + 1)] += -c->clapl[d + 1];
+      } else if (j >= np) {
+        ob[k] = c->slot[j];
+        ow[k] = -c->clapl[d + 1];
+        k++;
+        out = true;
+      }
+    }
+    if (out) {
+      orow[n] = r;
+      oq[n] = position(c, r, c->nso + 1);
+      optr[++n] = k;
+    }
+  }
+  for (int lm = 0; lm < nl; lm++) {
+    const int r = c->nso + lm;
+    for (int b = 0; b < c->nsortf; b++)
+      base[position(c, r, c->row[c->bnd[b]] + 1)] = -(c->wso[b] * c->ylm[(size_t)b * nl + lm]);
+  }
+  for (int a = 0; a < c->natome; a++)
+    for (int lm = 0; lm < c->nlmsa[a]; lm++) {
+      const int r = c->ianew[a] + lm;
+      for (int b = 0; b < c->nbordf[a]; b++)
+        base[position(c, r, c->row[c->abord[a][b]] + 1)] = -(c->apoids[a][b] * c->aylm[a][(size_t)b * c->nlmsa[a] + lm]);
+    }
+  std::memcpy(ylm, c->ylm.data(), sizeof(double) * c->ylm.size());
+  for (size_t b = 0; b < c->bnd.size(); b++) brad[b] = c->rad[c->bnd[b]];
+  par[0] = c->clapl[0];
+  par[1] = 4 * kPi / c->nsortf;
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -890,9 +950,9 @@ int fdmgpu_cluster_assemble_atoms(const fdmgpu_cluster *c, const double *V, doub
 
 
 int fdmgpu_cluster_assembly_sizes(const fdmgpu_cluster *c, int64_t *sizes) {
-  if (c->natome > 0) return -2;  // the device assembler holds no atom rows
   int64_t nrow = 0, nlink = 0;
   for (int i = 0; i < c->npoint; i++) {
+    if (c->numia[i]) continue;
     int k = 0;
     for (int d = 0; d < 12; d++) k += c->nbr[(size_t)i * 12 + d] >= c->npoint;
     nrow += k > 0;
@@ -908,7 +968,7 @@ int fdmgpu_cluster_assembly_sizes(const fdmgpu_cluster *c, int64_t *sizes) {
 int fdmgpu_cluster_assembly_tables(const fdmgpu_cluster *c, double *base, int64_t *qdiag, int *orow,
                                    int64_t *oq, int *optr, int *ob, double *ow, double *ylm, double *brad,
                                    double *par) {
-  if (c->natome > 0) return -2;
+  if (c->natome > 0) return atom_cluster_tables(c, base, qdiag, orow, oq, optr, ob, ow, ylm, brad, par);
   const int np = c->npoint, nl = c->nlmso;
   std::fill(base, base + c->cols.size(), 0.0);
   int r = 0, k = 0;
@@ -953,7 +1013,6 @@ int fdmgpu_cluster_assembly_tables(const fdmgpu_cluster *c, double *base, int64_
 }
 
 int fdmgpu_cluster_assembly_weights(const fdmgpu_cluster *c, int *lharm, double *wso, double *dself) {
-  if (c->natome > 0) return -2;  // the device assembler holds no atom rows
   for (int k = 0; k < c->nlmso; k++) lharm[k] = lof(c->lmidx[k]);
   std::memcpy(wso, c->wso.data(), sizeof(double) * c->nsortf);
   for (int i = 0; i < c->npoint; i++) {
@@ -961,6 +1020,82 @@ int fdmgpu_cluster_assembly_weights(const fdmgpu_cluster *c, int *lharm, double 
     s[0] = s[1] = s[2] = 0.0;
     for (int d = 0; d < 12; d++)  // only the second-shell steps towards a mirror can land on the point
       if (c->nbr[(size_t)i * 12 + d] == i) s[(d % 6) / 2] += stencil(c, i, d) / c->rvol[i];
+  }
+  return 0;
+}
+
+int fdmgpu_cluster_atom_assembly_sizes(const fdmgpu_cluster *c, int64_t *sizes) {
+  int64_t nar = 0, nal = 0, nphi = 0, nrow = 0, nbf = 0, nyl = 0;
+  for (int i = 0; i < c->npoint; i++) {
+    if (c->numia[i]) continue;
+    std::vector<int> seen;
+    for (int d = 0; d < 12; d++) {
+      const int j = c->nbr[(size_t)i * 12 + d];
+      if (j >= c->npoint || !c->numia[j]) continue;
+      nal++;
+      if (std::find(seen.begin(), seen.end(), c->numia[j]) == seen.end()) seen.push_back(c->numia[j]);
+    }
+    nar += (int64_t)seen.size();
+  }
+  for (int a = 0; a < c->natome; a++) {
+    nphi += (int64_t)c->abord[a].size() * c->nlmsa[a];
+    nrow += c->nlmsa[a];
+    nbf += c->nbordf[a];
+    nyl += (int64_t)c->nbordf[a] * c->nlmsa[a];
+  }
+  sizes[0] = c->natome;
+  sizes[1] = nar;
+  sizes[2] = nal;
+  sizes[3] = nphi;
+  sizes[4] = nrow;
+  sizes[5] = nbf;
+  sizes[6] = nyl;
+  return 0;
+}
+
+int fdmgpu_cluster_atom_assembly_tables(const fdmgpu_cluster *c, int64_t *aq, int *aa, int *aptr, int64_t *alphi,
+                                        double *alw, int *ianew, int *nlmsa, int *nbordf, int64_t *phoff,
+                                        int64_t *bq, double *poidsa, double *ylmato) {
+  if (c->natome < 1) return -1;
+  std::vector<int64_t> off(c->natome + 1, 0);
+  for (int a = 0; a < c->natome; a++) off[a + 1] = off[a] + (int64_t)c->abord[a].size() * c->nlmsa[a];
+  int n = 0, k = 0;
+  aptr[0] = 0;
+  for (int i = 0; i < c->npoint; i++) {
+    if (c->numia[i]) continue;
+    std::vector<int> atoms;  // in the order the stencil first meets them
+    for (int d = 0; d < 12; d++) {
+      const int j = c->nbr[(size_t)i * 12 + d];
+      if (j < c->npoint && c->numia[j] &&
+          std::find(atoms.begin(), atoms.end(), c->numia[j] - 1) == atoms.end())
+        atoms.push_back(c->numia[j] - 1);
+    }
+    for (int a : atoms) {
+      for (int d = 0; d < 12; d++) {
+        const int j = c->nbr[(size_t)i * 12 + d];
+        if (j >= c->npoint || c->numia[j] != a + 1) continue;
+        const std::vector<int> &bl = c->abord[a];
+        const int ib = (int)(std::find(bl.begin() + c->nbordf[a], bl.end(), j) - bl.begin());
+        alphi[k] = off[a] + (int64_t)ib * c->nlmsa[a];
+        alw[k] = -c->clapl[d + 1];
+        k++;
+      }
+      aq[n] = position(c, c->row[i], c->ianew[a] + 1);
+      aa[n] = a;
+      aptr[++n] = k;
+    }
+  }
+  int64_t r = 0, pb = 0, yb = 0;
+  for (int a = 0; a < c->natome; a++) {
+    ianew[a] = c->ianew[a];
+    nlmsa[a] = c->nlmsa[a];
+    nbordf[a] = c->nbordf[a];
+    phoff[a] = off[a];
+    for (int lm = 0; lm < c->nlmsa[a]; lm++) bq[r++] = position(c, c->ianew[a] + lm, c->ianew[a] + 1);
+    for (int b = 0; b < c->nbordf[a]; b++) {
+      poidsa[pb++] = c->apoids[a][b];
+      for (int lm = 0; lm < c->nlmsa[a]; lm++) ylmato[yb++] = c->aylm[a][(size_t)b * c->nlmsa[a] + lm];
+    }
   }
   return 0;
 }

@@ -222,8 +222,8 @@ typedef struct {
   double clapl0, pw;        /* stencil centre weight, outer-row quadrature weight */
   const int64_t *row_ptr;   /* [nligne+1] */
   const double *base;       /* [nnz] energy-independent part of every entry */
-  const int64_t *qdiag;     /* [npoint] position of grid row i's diagonal */
-  const int *orow;          /* [nrow_out] grid rows with outside neighbours (0-based) */
+  const int64_t *qdiag;     /* [npoint] position of grid point i's diagonal (-1: inside an atom) */
+  const int *orow;          /* [nrow_out] grid rows with outside neighbours (0-based row numbers) */
   const int64_t *oq;        /* [nrow_out] position of each one's first outer column */
   const int *optr;          /* [nrow_out+1] offsets of its neighbours in ob/ow */
   const int *ob;            /* [nlink] outer point of each outside neighbour */
@@ -252,6 +252,53 @@ void fdmgpu_assembler_destroy(fdmgpu_assembler *a);
 int fdmgpu_assembler_run(fdmgpu_assembler *a, int count, const double *Enervide, const double *Eimag,
                          const double *Ecinetic_out, int E_comp, int basereel, int update_only,
                          const double *const *d_V, double *const *d_vals, double *const *d_rhs);
+
+/*
+ * The same for a cluster with muffin-tin atoms (point group P1), whose
+ * entries also depend on the atoms' radial solutions at their border points
+ * (phiato, computed per energy by the caller's radial solver): the grid rows'
+ * couplings to the expansion of an atom a stencil neighbour lies in,
+ * -clapl(v) phiato(ib, Lm) summed over such neighbours in stencil order
+ * (calcMatRow, mat.f90:1233-1309), and each atom row's own block
+ * sum_ib poidsa(ib) Ylmato(ib, Lm) phiato(ib, Lm') over the nbordf outside
+ * border points (:1583-1632).  With atoms the assembly tables' rows are the
+ * cluster's rows (fdmgpu_cluster_assembly_tables fills them so): qdiag is -1
+ * for a point inside an atom, orow holds row numbers, and the outer rows
+ * are the last nlmso.  fdmgpu_cluster_atom_assembly_tables
+ * (fdmgpu_cluster.h) fills this struct; sizes from
+ * fdmgpu_cluster_atom_assembly_sizes.
+ */
+typedef struct {
+  int natome;
+  int nar;                  /* (grid row, atom) pairs: a stencil neighbour of the row lies in the atom */
+  int nal;                  /* their (row, neighbour) links */
+  int64_t nphi;             /* length of one energy point's phiato: sum over atoms of nbord * nlmsa */
+  const int64_t *aq;        /* [nar] position of the atom block's first column in that grid row */
+  const int *aa;            /* [nar] the atom (0-based) */
+  const int *aptr;          /* [nar+1] offsets of its links in alphi/alw */
+  const int64_t *alphi;     /* [nal] offset of the neighbour's nlmsa values in phiato */
+  const double *alw;        /* [nal] stencil weight -clapl(v) */
+  const int *ianew;         /* [natome] first row of the atom's expansion (0-based) */
+  const int *nlmsa;         /* [natome] its unknowns (lm order, l*l + l + m) */
+  const int *nbordf;        /* [natome] outside border points */
+  const int64_t *phoff;     /* [natome] offset of its (nbord, nlmsa) row-major block in phiato */
+  const int64_t *bq;        /* [sum nlmsa] per atom row, position of the atom block's first column */
+  const double *poidsa;     /* [sum nbordf] border weights, atom after atom */
+  const double *ylmato;     /* [sum nbordf*nlmsa] Ylm about the atom at them, (nbordf, nlmsa) per atom */
+} fdmgpu_atom_tables;
+/* As fdmgpu_assembler_create, with the atom tables (checked and copied).
+ * nligne must be the interstitial rows + sum nlmsa + nlmso. */
+int fdmgpu_assembler_create_atoms(const fdmgpu_assembly_tables *t, const fdmgpu_atom_tables *at, int max_points,
+                                  void *hip_stream, fdmgpu_assembler **out);
+/* As fdmgpu_assembler_run, with d_phi[k] the device phiato of point k
+ * (nphi values; interleaved (re, im) when phi_cplx, the reference's
+ * nphiato7 = 2, which needs complex arithmetic).  An assembler with atoms is
+ * run only by this entry point (fdmgpu_assembler_run returns
+ * FDMGPU_EINVAL for it). */
+int fdmgpu_assembler_run_atoms(fdmgpu_assembler *a, int count, const double *Enervide, const double *Eimag,
+                               const double *Ecinetic_out, int E_comp, int basereel, int update_only,
+                               const double *const *d_V, const double *const *d_phi, int phi_cplx,
+                               double *const *d_vals, double *const *d_rhs);
 
 /*
  * Post-solve reduction to the atom amplitude products (mat.f90:297-405,

@@ -168,6 +168,18 @@ int fdmgpu_cluster_assembly_tables(const fdmgpu_cluster *c, double *base, int64_
  *                 (second-shell neighbour across a mirror plane), which adds
  *                 to its diagonal (0 for P1) */
 int fdmgpu_cluster_assembly_weights(const fdmgpu_cluster *c, int *lharm, double *wso, double *dself);
+/* For a cluster with atoms the three functions above describe rows, not
+ * points: qdiag[i] = -1 for a point inside an atom, orow holds row numbers,
+ * the outer rows are the last nlmso, and base also holds the atom rows'
+ * -poidsa Ylmato in the columns of their outside border points
+ * (mat.f90:1560-1581).  The entries that depend on phiato come from these
+ * tables (fdmgpu_atom_tables in fdmgpu.h):
+ * sizes[0..6] = natome, nar, nal, nphi, sum nlmsa, sum nbordf,
+ * sum nbordf * nlmsa */
+int fdmgpu_cluster_atom_assembly_sizes(const fdmgpu_cluster *c, int64_t *sizes);
+int fdmgpu_cluster_atom_assembly_tables(const fdmgpu_cluster *c, int64_t *aq, int *aa, int *aptr, int64_t *alphi,
+                                        double *alw, int *ianew, int *nlmsa, int *nbordf, int64_t *phoff,
+                                        int64_t *bq, double *poidsa, double *ylmato);
 
 #ifdef __cplusplus
 }

@@ -94,3 +94,70 @@ def test_complex_radial_solutions_need_complex_arithmetic():
         c.assemble_atoms(g["V"][: c.npoint], 0.7, 0.0, 0.7, False, True, phi)
     with pytest.raises(CL._lib.FdmGpuError):  # the plain assembler refuses a cluster with atoms
         c.assemble(g["V"][: c.npoint], 0.7, 0.0, 0.7, False, True)
+
+
+@pytest.mark.parametrize("branch", range(3))
+def test_atom_assembly_tables_rebuild_the_host_rows(branch):
+    """The device assembler's tables of a cluster with atoms, applied in
+    numpy as its kernels apply them (base values, diagonals, outer columns
+    and rows with the oracle's outer-sphere functions, the atom couplings and
+    atom rows from phiato), give the host assembly's rows: bit for bit where
+    no outer-sphere function enters, within 1e-14 elsewhere."""
+    g, c, o = _clusters()
+    E, Ei, E_comp, cal, basereel = BRANCHES[branch]
+    phi = phiato(o, g["posi"] / BOHR, g["lmax"], E, Ei, E_comp)
+    V = g["V"][: c.npoint]
+    _, _, hv, hr = c.assemble_atoms(V, E, Ei, E, E_comp, basereel, phi)
+    t, at = c.assembly_tables(), c.atom_assembly_tables()
+    flat = np.concatenate([p.reshape(-1) for p in phi])
+    assert len(flat) == at["nphi"]
+    B, N = o.phiso(E_comp, E, Ei)
+    cf = N if basereel else B + 1j * N
+    nl, nso = c.nlmso, c.nligne - c.nlmso
+    lof = np.array([int(np.floor(np.sqrt(k) + 1e-9)) for k in range(nl)])
+    Y = t["ylm"].reshape(t["nb"], nl)
+    vals = t["base"].astype(np.complex128)
+    rhs = np.zeros((c.nligne, nl), np.complex128)
+    inside = t["qdiag"] < 0
+    assert np.array_equal(inside, c.numia > 0)
+    vals[t["qdiag"][~inside]] = V[~inside] - E - t["clapl0"] + (-1j * Ei if E_comp else 0)
+    exact = np.ones(c.nnz, bool)
+    for r in range(t["nrow_out"]):
+        b, w = t["ob"][t["optr"][r]:t["optr"][r + 1]], t["ow"][t["optr"][r]:t["optr"][r + 1]]
+        for lm in range(nl):
+            vals[t["oq"][r] + lm] = np.sum(cf[b, lof[lm]] * Y[b, lm] * w)
+            rhs[t["orow"][r], lm] = -np.sum(Y[b, lm] * B[b, lof[lm]] * w)
+        exact[t["oq"][r]:t["oq"][r] + nl] = False
+    ns = t["nsortf"]
+    for lm in range(nl):
+        q = t["row_ptr"][nso + lm] + ns
+        for lp in range(nl):
+            f = t["wso"] * (Y[:ns, lm] * Y[:ns, lp])
+            vals[q + lp] = np.sum(f * cf[:ns, lof[lp]])
+            rhs[nso + lm, lp] = -np.sum(t["wso"] * Y[:ns, lm] * Y[:ns, lp] * B[:ns, lof[lp]])
+        exact[q:q + nl] = False
+    for r in range(at["nar"]):  # grid rows' couplings to an atom's expansion, summed in stencil order
+        na = at["nlmsa"][at["aa"][r]]
+        s = np.zeros(na, np.complex128)
+        for k in range(at["aptr"][r], at["aptr"][r + 1]):
+            s = s + at["alw"][k] * flat[at["alphi"][k]:at["alphi"][k] + na]
+        vals[at["aq"][r]:at["aq"][r] + na] = s
+    k = 0
+    po = np.concatenate([[0], np.cumsum(at["nbordf"])])
+    yo = np.concatenate([[0], np.cumsum(at["nbordf"] * at["nlmsa"])])
+    for a in range(at["natome"]):
+        na, nbf = at["nlmsa"][a], at["nbordf"][a]
+        pw = at["poidsa"][po[a]:po[a + 1]]
+        Ya = at["ylmato"][yo[a]:yo[a + 1]].reshape(nbf, na)
+        ph = flat[at["phoff"][a]:at["phoff"][a] + nbf * na].reshape(nbf, na)
+        for lm in range(na):
+            s = np.zeros(na, np.complex128)
+            for b in range(nbf):
+                s = s + (pw[b] * (ph[b] * Ya[b, lm]) if cal else pw[b] * Ya[b, lm] * ph[b])
+            vals[at["bq"][k]:at["bq"][k] + na] = s
+            k += 1
+    if not cal:
+        vals, rhs = vals.real, rhs.real
+    assert np.array_equal(vals[exact], hv[exact])
+    assert np.abs(vals - hv).max() <= 1e-14 * np.abs(hv).max()
+    assert np.abs(rhs - hr).max() <= 1e-14 * np.abs(hr).max()

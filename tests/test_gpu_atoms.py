@@ -97,3 +97,92 @@ def test_shim_with_atoms_matches_reference_mat_solve(branch):
     rel = np.abs(x_gpu - x_ref).max() / np.abs(x_ref).max()
     print("shim vs reference mat_solve", rel, flush=True)
     assert rel <= 1e-10
+
+
+def _energies(E_comp):
+    # (Enervide, Eimag, Ecinetic_out) in Ry
+    return [(0.3 + 0.35 * k, 0.05 if E_comp else 0.0, 0.3 + 0.35 * k) for k in range(3)]
+
+
+@pytest.mark.parametrize("branch", range(3))
+def test_device_assembly_with_atoms_matches_host(branch):
+    """fdmgpu_assembler_run_atoms at three energies, each with its own V and
+    phiato, against the host assembly (fdmgpu_cluster_assemble_atoms): bit
+    for bit where no outer-sphere function enters, within 1e-13 elsewhere;
+    then update_only on the same buffers equals a fresh call, and the
+    device-assembled systems solve to the host-assembled systems' solutions."""
+    import torch
+    from fdmnes_amd import batch as B
+    from fdmnes_amd.assemble import DeviceAssembler
+    g, c, o, _, _, cal = _system(branch)
+    _, _, E_comp, _, basereel = BRANCHES[branch]
+    ens = _energies(E_comp)
+    rng = np.random.default_rng(11 + branch)
+    Vs = [g["V"][: c.npoint] + rng.uniform(-0.05, 0.05, c.npoint) for _ in ens]
+    phis = [phiato(o, g["posi"] / BOHR, g["lmax"], E, Ei, E_comp) for E, Ei, _ in ens]
+    print(f"device assembly with atoms: branch {BRANCHES[branch]} nnz {c.nnz}", flush=True)
+    dt = torch.complex128 if cal else torch.float64
+    a = DeviceAssembler(c, max_points=2)  # three points in two calls
+    Vd = [torch.from_numpy(v).cuda() for v in Vs]
+    pd = [torch.from_numpy(np.concatenate([p.reshape(-1) for p in ph])).cuda() for ph in phis]
+    vals = [torch.full((c.nnz,), 7.0, dtype=dt, device="cuda") for _ in ens]
+    rhs = [torch.full((c.nligne, c.nlmso), 7.0, dtype=dt, device="cuda") for _ in ens]
+    a.run(Vd, ens, E_comp, basereel, vals, rhs, phi=pd)
+    torch.cuda.synchronize()
+    t = c.assembly_tables()
+    fixed = np.ones(c.nnz, bool)
+    for r in range(t["nrow_out"]):
+        fixed[t["oq"][r]:t["oq"][r] + c.nlmso] = False
+    nso = c.nligne - c.nlmso
+    for lm in range(c.nlmso):
+        q = t["row_ptr"][nso + lm] + c.nsortf
+        fixed[q:q + c.nlmso] = False
+    host = []
+    for k, (E, Ei, Ec) in enumerate(ens):
+        rp, col, hv, hr = c.assemble_atoms(Vs[k], E, Ei, Ec, E_comp, basereel, phis[k])
+        host.append((rp, col, hv, hr))
+        dv, dr = vals[k].cpu().numpy(), rhs[k].cpu().numpy()
+        assert np.array_equal(dv[fixed], hv[fixed])
+        assert np.abs(dv - hv).max() <= 1e-13 * np.abs(hv).max()
+        assert np.abs(dr - hr).max() <= 1e-13 * np.abs(hr).max()
+    # update_only: new potentials and phiato over the first call's output
+    Vd2 = [torch.from_numpy(v[::-1].copy()).cuda() for v in Vs]
+    a.run(Vd2, ens[::-1], E_comp, basereel, vals, rhs, update_only=True, phi=pd[::-1])
+    fv = [torch.full((c.nnz,), 3.0, dtype=dt, device="cuda") for _ in ens]
+    fr = [torch.full((c.nligne, c.nlmso), 3.0, dtype=dt, device="cuda") for _ in ens]
+    a.run(Vd2, ens[::-1], E_comp, basereel, fv, fr, phi=pd[::-1])
+    torch.cuda.synchronize()
+    for u, f in zip(vals + rhs, fv + fr):
+        assert torch.equal(u, f)
+    # device-assembled and host-assembled points through the grouped FP64 solver
+    a.run(Vd, ens, E_comp, basereel, vals, rhs, phi=pd)
+    lb1, lb2 = c.profile()
+    sv = B.DeviceSolver(B.FP64, True, c.nlmso)
+    slots = [sv.add_system(lb1, lb2, host[0][0], host[0][1]) for _ in ens]
+    cv = [v.to(torch.complex128) for v in vals]
+    cr = [r.to(torch.complex128) for r in rhs]
+    xd = [torch.zeros_like(r) for r in cr]
+    sv.solve(slots, cv, cr, xd)
+    hv = [torch.from_numpy(h[2].astype(np.complex128)).cuda() for h in host]
+    hr = [torch.from_numpy(np.ascontiguousarray(h[3], np.complex128)).cuda() for h in host]
+    xh = [torch.zeros_like(r) for r in hr]
+    sv.solve(slots, hv, hr, xh)
+    for x1, x2 in zip(xd, xh):
+        x1, x2 = x1.cpu().numpy(), x2.cpu().numpy()
+        rel = np.abs(x1 - x2).max() / np.abs(x2).max()
+        print("device- vs host-assembled solution", rel, flush=True)
+        assert rel <= 1e-10
+
+
+def test_atom_assembler_needs_phiato():
+    import torch
+    from fdmnes_amd.assemble import DeviceAssembler
+    g, c, o, _, _, _ = _system(0)
+    a = DeviceAssembler(c)
+    V = torch.from_numpy(g["V"][: c.npoint].copy()).cuda()
+    vals = [torch.zeros(c.nnz, dtype=torch.complex128, device="cuda")]
+    with pytest.raises(ValueError):
+        a.run([V], _energies(True)[:1], True, False, vals)
+    with pytest.raises(ValueError):  # complex phiato in real arithmetic
+        a.run([V], _energies(False)[:1], False, True, [torch.zeros(c.nnz, dtype=torch.float64, device="cuda")],
+              phi=[torch.zeros(a.nphi, dtype=torch.complex128, device="cuda")])
