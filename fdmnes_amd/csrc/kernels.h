@@ -1232,6 +1232,13 @@ struct XRegs {
 // the chain from one block step to the next carries only x_nb, one 64x64
 // tile product x_K = y_K - M(K) x_nb and the publication.  The transposed
 // substitutions stage T(nb,K)^T and D^-1 before the wait and apply both after.
+// Depth of the persistent solve's register ring: FP32 complex stays at 256
+// VGPRs with two sets (two workgroups per CU, as its LDS allows); FP64 runs
+// one workgroup per CU (its LDS) and holds three sets in 449 VGPRs
+template <class R>
+constexpr int solve_pd() {
+  return sizeof(R) == 4 ? 2 : 3;
+}
 template <class R, bool C>
 __global__ __launch_bounds__(NT) void k_solve_persist(const SysDev<R> *__restrict__ sys, SolveSet ss, int kind,
                                                       int *counter, int *err) {
@@ -1299,33 +1306,36 @@ __global__ __launch_bounds__(NT) void k_solve_persist(const SysDev<R> *__restric
         }
         __syncthreads();
         const int run = nrun;
-        if constexpr (sizeof(R) == 4) {
-          // FP32: the next dependency's tile and x block load into registers
-          // while the current pair is multiplied
-          TileRegs<R, C> tr;
-          XRegs<R, C> xr;
-          tr.load(tile(base + dir * q));
-          xr.load(G(S.X), base + dir * q, rc0, nrhs);
-          for (int u = 0; u < run; u++) {
-            tr.store(Ts);
-            xr.store(Xs);
-            __syncthreads();
-            if (u + 1 < run) {
-              const int Jn = base + dir * (q + u + 1);
-              tr.load(tile(Jn));
-              xr.load(G(S.X), Jn, rc0, nrhs);
-            }
-            tile_mfma<R, C, LD>(Ts, Xs, trans, accr, acci);
-            __syncthreads();
+        constexpr int PD = solve_pd<R>();
+        // the run's tiles and x blocks load into a ring of PD register
+        // sets, PD - 1 pairs ahead of the pair being multiplied, so that
+        // several tiles per workgroup are in flight (one alone leaves the
+        // stream bound by memory latency); every load is of a dependency
+        // whose flag this run has already seen
+        TileRegs<R, C> tr[PD];
+        XRegs<R, C> xr[PD];
+#pragma unroll
+        for (int s = 0; s < PD; s++)
+          if (s < run) {
+            tr[s].load(tile(base + dir * (q + s)));
+            xr[s].load(G(S.X), base + dir * (q + s), rc0, nrhs);
           }
-        } else {
-          for (int u = 0; u < run; u++) {
-            const int J = base + dir * (q + u);
-            stage_tile<R, C, LD>(tile(J), Ts);
-            stage_x<R, C>(G(S.X), J, rc0, nrhs, Xs);
-            __syncthreads();
-            tile_mfma<R, C, LD>(Ts, Xs, trans, accr, acci);
-            __syncthreads();
+        for (int u0 = 0; u0 < run; u0 += PD) {
+#pragma unroll
+          for (int s = 0; s < PD; s++) {
+            const int u = u0 + s;
+            if (u < run) {
+              tr[s].store(Ts);
+              xr[s].store(Xs);
+              __syncthreads();
+              if (u + PD < run) {
+                const int Jn = base + dir * (q + u + PD);
+                tr[s].load(tile(Jn));
+                xr[s].load(G(S.X), Jn, rc0, nrhs);
+              }
+              tile_mfma<R, C, LD>(Ts, Xs, trans, accr, acci);
+              __syncthreads();
+            }
           }
         }
         q += run;
