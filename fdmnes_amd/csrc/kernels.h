@@ -1203,14 +1203,18 @@ struct XRegs {
   static constexpr int NC = Elem<R, C>::NC;
   static constexpr int PER = TB * RC / NT;
   R re[PER], im[PER];
+  // every lane loads (columns past nrhs read column rc0 and are zeroed), so
+  // the loads sit on no branch
   __device__ __forceinline__ void load(gptr<const R> X, int J, int rc0, int nrhs) {
 #pragma unroll
     for (int q = 0; q < PER; q++) {
       const int e = threadIdx.x + q * NT, rr = e / RC, cc = e % RC;
-      const int64_t gi = ((int64_t)(J * TB + rr) * nrhs + rc0 + cc) * NC;
       const bool ok = rc0 + cc < nrhs;
-      re[q] = ok ? X[gi] : R(0);
-      im[q] = (C && ok) ? X[gi + NC - 1] : R(0);
+      const int64_t gi = ((int64_t)(J * TB + rr) * nrhs + rc0 + (ok ? cc : 0)) * NC;
+      const R a = X[gi];
+      const R b = C ? X[gi + NC - 1] : R(0);
+      re[q] = ok ? a : R(0);
+      im[q] = ok ? b : R(0);
     }
   }
   __device__ __forceinline__ void store(R (*Xs)[TB][RC + 1]) const {
@@ -1240,7 +1244,7 @@ constexpr int solve_pd() {
   return sizeof(R) == 4 ? 2 : 3;
 }
 template <class R, bool C>
-__global__ __launch_bounds__(NT) void k_solve_persist(const SysDev<R> *__restrict__ sys, SolveSet ss, int kind,
+__global__ __launch_bounds__(NT, sizeof(R) == 4 ? 2 : 1) void k_solve_persist(const SysDev<R> *__restrict__ sys, SolveSet ss, int kind,
                                                       int *counter, int *err) {
   constexpr int NC = Elem<R, C>::NC;
   constexpr int LD = TB + 1;
@@ -1248,6 +1252,11 @@ __global__ __launch_bounds__(NT) void k_solve_persist(const SysDev<R> *__restric
   __shared__ R Ds[NC][TB][LD];
   __shared__ R Xs[NC][TB][RC + 1];
   __shared__ int tk, nrun;
+  // tile index (tbase + column - J0 of its block row) of each dependency of
+  // the current run: formed once per run by wave 0, so the ring's tile
+  // addresses need no global load (a load's wait counts every load in flight
+  // before it, the ring's tiles included)
+  __shared__ int64_t toff[64];
   const int tid = threadIdx.x;
   const bool trans = (kind == S_LOWER_T || kind == S_UPPER_T);
   const bool upper = (kind == S_UPPER || kind == S_UPPER_T);
@@ -1298,10 +1307,16 @@ __global__ __launch_bounds__(NT) void k_solve_persist(const SysDev<R> *__restric
       v4 accr = {0, 0, 0, 0}, acci = {0, 0, 0, 0};
       const bool colok = rc0 + li < nrhs;
       const int nold = oe - ob, base = asc ? ob : oe - 1, dir = asc ? 1 : -1;
+      const gptr<const R> tiles0 = uniform(G((const R *)S.tiles));
+      constexpr int64_t TE = (int64_t)TB * TB * NC;
       for (int q = 0; q < nold;) {
         if (tid < 64) {
           int run = ready_run(S.done, base, dir, q, nold, err);
-          if (run == 0) run = nold - q;  // error: finish without waiting
+          if (run == 0) run = min(64, nold - q);  // error: finish without waiting
+          if (tid < run) {
+            const int J = base + dir * (q + tid);
+            toff[tid] = trans ? G(p.tbase)[J] + (K - G(p.J0)[J]) : G(p.tbase)[K] + (J - G(p.J0)[K]);
+          }
           if (tid == 0) nrun = run;
         }
         __syncthreads();
@@ -1312,30 +1327,41 @@ __global__ __launch_bounds__(NT) void k_solve_persist(const SysDev<R> *__restric
         // several tiles per workgroup are in flight (one alone leaves the
         // stream bound by memory latency); every load is of a dependency
         // whose flag this run has already seen
+        // Loads are issued on every path (past the end of the run they
+        // repeat its last pair, which the L2 holds), and the ring's body has
+        // no branch around them, so the compiler's wait before staging a
+        // pair counts only the older loads and the younger sets stay in
+        // flight; a conditional load would make it wait for all of them.
         TileRegs<R, C> tr[PD];
         XRegs<R, C> xr[PD];
 #pragma unroll
-        for (int s = 0; s < PD; s++)
-          if (s < run) {
-            tr[s].load(tile(base + dir * (q + s)));
-            xr[s].load(G(S.X), base + dir * (q + s), rc0, nrhs);
-          }
-        for (int u0 = 0; u0 < run; u0 += PD) {
+        for (int s = 0; s < PD; s++) {
+          const int v = min(s, run - 1);
+          tr[s].load(tiles0 + toff[v] * TE);
+          xr[s].load(G(S.X), base + dir * (q + v), rc0, nrhs);
+        }
+        const int full = run - run % PD;
+        for (int u0 = 0; u0 < full; u0 += PD) {
 #pragma unroll
           for (int s = 0; s < PD; s++) {
-            const int u = u0 + s;
-            if (u < run) {
-              tr[s].store(Ts);
-              xr[s].store(Xs);
-              __syncthreads();
-              if (u + PD < run) {
-                const int Jn = base + dir * (q + u + PD);
-                tr[s].load(tile(Jn));
-                xr[s].load(G(S.X), Jn, rc0, nrhs);
-              }
-              tile_mfma<R, C, LD>(Ts, Xs, trans, accr, acci);
-              __syncthreads();
-            }
+            tr[s].store(Ts);
+            xr[s].store(Xs);
+            __syncthreads();
+            const int v = min(u0 + s + PD, run - 1);
+            tr[s].load(tiles0 + toff[v] * TE);
+            xr[s].load(G(S.X), base + dir * (q + v), rc0, nrhs);
+            tile_mfma<R, C, LD>(Ts, Xs, trans, accr, acci);
+            __syncthreads();
+          }
+        }
+#pragma unroll
+        for (int s = 0; s < PD; s++) {
+          if (full + s < run) {
+            tr[s].store(Ts);
+            xr[s].store(Xs);
+            __syncthreads();
+            tile_mfma<R, C, LD>(Ts, Xs, trans, accr, acci);
+            __syncthreads();
           }
         }
         q += run;
