@@ -58,6 +58,11 @@ struct orc_cluster {
   double *ylmato;  /* Ylmato(nbtm,nlmmax,natome) */
   double *phi_re, *phi_im; /* phiato(nbtm,nlmmax,natome) set by orc_cluster_set_phiato */
   int phi_cplx;    /* nphiato7 == 2 */
+  /* irregular cluster solution (mat.f90:140-147, Test_reg = 0): n_ato_RI = 2
+   * blocks per atom, no outer-sphere rows; phi2 = the irregular atomic
+   * solutions (i_RI = 2 of phiato) */
+  int irregular, n_ato_RI;
+  double *phi2_re, *phi2_im;
 };
 #define IBORD(c, ib, ia) ((c)->ibord[((ia)-1) * (c)->nbtm + (ib)-1])
 #define POIDSA(c, ib, ia) ((c)->poidsa[((ia)-1) * (c)->nbtm + (ib)-1])
@@ -432,7 +437,7 @@ static void orc_ylmpt(orc_cluster *c) {
  * blocks are placed in the radial order at their distance distai(ia) from
  * the absorber (:1790-1810). */
 static void orc_newind(orc_cluster *c) {
-  const int nspino = 1, na = c->natome;
+  const int nspino = 1, na = c->natome, nri = c->n_ato_RI;
   c->newp = calloc((size_t)c->npoint + 2, sizeof(int));
   c->ianew = calloc((size_t)na + 1, sizeof(int));
   int ii = 0;
@@ -453,12 +458,12 @@ static void orc_newind(orc_cluster *c) {
       if (c->numia[i - 1] == 0 && XYZ(c, 4, i) >= distai) break;
     npf = i;
     if (npf1 == npf)
-      c->ianew[ia - 1] = c->ianew[ia - 2] + c->nlmsa[ia - 2];
+      c->ianew[ia - 1] = c->ianew[ia - 2] + nri * c->nlmsa[ia - 2];
     else
       c->ianew[ia - 1] = c->newp[npf] - nspino;
     npf1 = npf;
     for (i = npf; i <= c->npoint; i++)
-      if (c->numia[i - 1] == 0) c->newp[i] += c->nlmsa[ia - 1];
+      if (c->numia[i - 1] == 0) c->newp[i] += nri * c->nlmsa[ia - 1];
   }
   const int nligne = c->nligne;
   c->lb1 = malloc(sizeof(int) * (nligne + 1));
@@ -481,33 +486,35 @@ static void orc_newind(orc_cluster *c) {
         if (nj > c->lb2[ii]) c->lb2[ii] = nj;
       } else if (ia > 0) {
         if (c->ianew[ia - 1] + 1 < c->lb1[ii]) c->lb1[ii] = c->ianew[ia - 1] + 1;
-        if (c->ianew[ia - 1] + c->nlmsa[ia - 1] > c->lb2[ii]) c->lb2[ii] = c->ianew[ia - 1] + c->nlmsa[ia - 1];
+        if (c->ianew[ia - 1] + nri * c->nlmsa[ia - 1] > c->lb2[ii]) c->lb2[ii] = c->ianew[ia - 1] + nri * c->nlmsa[ia - 1];
       } else if (ia == -2) {
         c->lb2[ii] = nligne;
       }
     }
   }
   for (int ia = 1; ia <= na; ia++) { /* :1863-1875 */
-    int lb11 = c->ianew[ia - 1] + 1, lb22 = c->ianew[ia - 1] + c->nlmsa[ia - 1];
+    int lb11 = c->ianew[ia - 1] + 1, lb22 = c->ianew[ia - 1] + nri * c->nlmsa[ia - 1];
     for (int ib = 1; ib <= c->nbordf[ia - 1]; ib++) {
       int r = c->newp[IBORD(c, ib, ia)];
       if (r - n < lb11) lb11 = r - n;
       if (r > lb22) lb22 = r;
     }
-    for (int L = 1; L <= c->nlmsa[ia - 1]; L++) {
+    for (int L = 1; L <= nri * c->nlmsa[ia - 1]; L++) {
       ii = c->ianew[ia - 1] + L;
       c->lb1[ii] = lb11;
       c->lb2[ii] = lb22;
     }
   }
-  int lb11 = nligne - c->nlmso + 1; /* :1877-1886 */
-  for (int ib = 1; ib <= c->nsortf; ib++) {
-    int r = c->newp[c->isrt[ib - 1]] - n;
-    if (r < lb11) lb11 = r;
-  }
-  for (ii = nligne - c->nlmso + 1; ii <= nligne; ii++) {
-    c->lb1[ii] = lb11;
-    c->lb2[ii] = nligne;
+  if (!c->irregular) { /* :1877-1886, no scattered amplitude in the irregular solution */
+    int lb11 = nligne - c->nlmso + 1;
+    for (int ib = 1; ib <= c->nsortf; ib++) {
+      int r = c->newp[c->isrt[ib - 1]] - n;
+      if (r < lb11) lb11 = r;
+    }
+    for (ii = nligne - c->nlmso + 1; ii <= nligne; ii++) {
+      c->lb1[ii] = lb11;
+      c->lb2[ii] = nligne;
+    }
   }
   for (ii = nligne - 1; ii >= 1; ii--) /* :1888-1890 */
     if (c->lb1[ii + 1] < c->lb1[ii]) c->lb1[ii] = c->lb1[ii + 1];
@@ -516,13 +523,15 @@ static void orc_newind(orc_cluster *c) {
     if (ii != 0) c->newinv[ii] = i;
   }
   for (int ia = 1; ia <= na; ia++)
-    for (int Lm = 1; Lm <= c->nlmsa[ia - 1]; Lm++) c->newinv[c->ianew[ia - 1] + Lm] = -ia;
+    for (int Lm = 1; Lm <= nri * c->nlmsa[ia - 1]; Lm++) c->newinv[c->ianew[ia - 1] + Lm] = -ia;
 }
 
-orc_cluster *orc_cluster_new_atoms(double rsort, double adimp, int iord, int lmaxso, int natome, const double *posi,
-                                   const double *rmt, const int *lmaxa) {
+orc_cluster *orc_cluster_new_atoms_ri(double rsort, double adimp, int iord, int lmaxso, int natome,
+                                      const double *posi, const double *rmt, const int *lmaxa, int irregular) {
   orc_cluster *c = calloc(1, sizeof(orc_cluster));
   if (!c) return NULL;
+  c->irregular = irregular != 0;
+  c->n_ato_RI = c->irregular ? 2 : 1;
   c->rsort = rsort;
   c->adimp = adimp;
   c->iord = iord;
@@ -560,12 +569,22 @@ orc_cluster *orc_cluster_new_atoms(double rsort, double adimp, int iord, int lma
       c->mso[lm] = m;
       c->iso[lm] = 1;
     }
-  int nat = 0; /* mat.f90:150 */
+  int nat = 0; /* mat.f90:147, :150 */
   for (int ia = 0; ia < natome; ia++) nat += c->nlmsa[ia];
-  c->nligne = c->npr + nat + c->nlmso;
-  c->nligneso = c->nligne - c->nlmso;
+  if (c->irregular) {
+    c->nligne = c->npr + c->n_ato_RI * nat;
+    c->nligneso = c->nligne; /* mat.f90:178-179 */
+  } else {
+    c->nligne = c->npr + nat + c->nlmso;
+    c->nligneso = c->nligne - c->nlmso;
+  }
   orc_newind(c);
   return c;
+}
+
+orc_cluster *orc_cluster_new_atoms(double rsort, double adimp, int iord, int lmaxso, int natome, const double *posi,
+                                   const double *rmt, const int *lmaxa) {
+  return orc_cluster_new_atoms_ri(rsort, adimp, iord, lmaxso, natome, posi, rmt, lmaxa, 0);
 }
 
 orc_cluster *orc_cluster_new(double rsort, double adimp, int iord, int lmaxso) {
@@ -579,6 +598,7 @@ void orc_cluster_free(orc_cluster *c) {
   free(c->newp); free(c->lb1); free(c->lb2); free(c->newinv);
   free(c->posi); free(c->rmt); free(c->lmaxa); free(c->nlmsa); free(c->ianew); free(c->nbord); free(c->nbordf);
   free(c->ibord); free(c->poidsa); free(c->ylmato); free(c->phi_re); free(c->phi_im);
+  free(c->phi2_re); free(c->phi2_im);
   free(c);
 }
 
@@ -600,22 +620,28 @@ void orc_cluster_atom(const orc_cluster *c, int ia, int *info, int *ibord, doubl
   }
 }
 void orc_cluster_numia(const orc_cluster *c, int *numia) { memcpy(numia, c->numia, sizeof(int) * c->npso); }
-int orc_cluster_set_phiato(orc_cluster *c, int cplx, const double *re, const double *im) {
+/* i_RI = 1: the regular atomic solutions (phi_re/phi_im); 2: the irregular
+ * ones (phi2_re/phi2_im) */
+int orc_cluster_set_phiato_ri(orc_cluster *c, int i_ri, int cplx, const double *re, const double *im) {
   const size_t n = (size_t)c->nbtm * c->nlmmax * (c->natome > 0 ? c->natome : 1);
-  free(c->phi_re);
-  free(c->phi_im);
-  c->phi_re = calloc(n, sizeof(double));
-  c->phi_im = calloc(n, sizeof(double));
-  if (!c->phi_re || !c->phi_im) return -1;
-  c->phi_cplx = cplx;
+  double **pr = i_ri == 2 ? &c->phi2_re : &c->phi_re, **pi = i_ri == 2 ? &c->phi2_im : &c->phi_im;
+  free(*pr);
+  free(*pi);
+  *pr = calloc(n, sizeof(double));
+  *pi = calloc(n, sizeof(double));
+  if (!*pr || !*pi) return -1;
+  if (i_ri != 2) c->phi_cplx = cplx;
   size_t q = 0; /* input: per atom, per border point ib, nlmsa(ia) harmonics */
   for (int ia = 1; ia <= c->natome; ia++)
     for (int ib = 1; ib <= c->nbord[ia - 1]; ib++)
       for (int lm = 1; lm <= c->nlmsa[ia - 1]; lm++, q++) {
-        PHIATO(c->phi_re, c, ib, lm, ia) = re[q];
-        PHIATO(c->phi_im, c, ib, lm, ia) = (cplx && im) ? im[q] : 0.0;
+        PHIATO(*pr, c, ib, lm, ia) = re[q];
+        PHIATO(*pi, c, ib, lm, ia) = (cplx && im) ? im[q] : 0.0;
       }
   return 0;
+}
+int orc_cluster_set_phiato(orc_cluster *c, int cplx, const double *re, const double *im) {
+  return orc_cluster_set_phiato_ri(c, 1, cplx, re, im);
 }
 void orc_cluster_xyz(const orc_cluster *c, double *xyz) { memcpy(xyz, c->xyz, sizeof(double) * 4 * c->npso); }
 void orc_cluster_indice(const orc_cluster *c, int *ind) { memcpy(ind, c->indice, sizeof(int) * 3 * c->npso); }
@@ -757,19 +783,25 @@ static void orc_calcmatrow(const row_ctx *x, int ii, double *abvr, double *abvi,
         int ib;
         for (ib = c->nbordf[ia - 1] + 1; ib <= c->nbord[ia - 1]; ib++)
           if (j == IBORD(c, ib, ia)) break;
-        for (int Lm = 1; Lm <= c->nlmsa[ia - 1]; Lm++) {
-          int jjj = c->ianew[ia - 1] + Lm;
-          /* Lato/mato in (L, m) order: Lm0 = L**2 + L + 1 + m = Lm */
-          const double pr = PHIATO(c->phi_re, c, ib, Lm, ia), pi = PHIATO(c->phi_im, c, ib, Lm, ia);
-          if (c->phi_cplx) {
-            double complex cfac = CMPLX(cfvs_r, cfvs_i) * CMPLX(pr, pi);
-            abvr[jjj] = abvr[jjj] + creal(cfac);
-            abvi[jjj] = abvi[jjj] + cimag(cfac);
-          } else if (cal) {
-            abvr[jjj] = abvr[jjj] + cfvs_r * pr;
-            abvi[jjj] = abvi[jjj] + cfvs_i * pr;
-          } else {
-            abvr[jjj] = abvr[jjj] + cfvs_r * pr;
+        /* :1266-1307: regular (i_RI = 1) and, in the irregular solution,
+         * irregular (i_RI = 2) atomic solutions, Test_reg = 0 */
+        for (int i_ri = 1; i_ri <= c->n_ato_RI; i_ri++) {
+          const double *phr = i_ri == 2 ? c->phi2_re : c->phi_re, *phi = i_ri == 2 ? c->phi2_im : c->phi_im;
+          for (int Lm = 1; Lm <= c->nlmsa[ia - 1]; Lm++) {
+            int jjj = c->ianew[ia - 1] + Lm;
+            if (i_ri == 2 && c->n_ato_RI == 2) jjj += c->nlmsa[ia - 1];
+            /* Lato/mato in (L, m) order: Lm0 = L**2 + L + 1 + m = Lm */
+            const double pr = PHIATO(phr, c, ib, Lm, ia), pi = PHIATO(phi, c, ib, Lm, ia);
+            if (c->phi_cplx) {
+              double complex cfac = CMPLX(cfvs_r, cfvs_i) * CMPLX(pr, pi);
+              abvr[jjj] = abvr[jjj] + creal(cfac);
+              abvi[jjj] = abvi[jjj] + cimag(cfac);
+            } else if (cal) {
+              abvr[jjj] = abvr[jjj] + cfvs_r * pr;
+              abvi[jjj] = abvi[jjj] + cfvs_i * pr;
+            } else {
+              abvr[jjj] = abvr[jjj] + cfvs_r * pr;
+            }
           }
         }
       } else if (ia == -2) { /* :1312-1398 */
@@ -777,7 +809,8 @@ static void orc_calcmatrow(const row_ctx *x, int ii, double *abvr, double *abvi,
         for (ib = c->nsortf + 1; ib <= c->nsort; ib++)
           if (j == c->isrt[ib - 1]) break;
         double complex cfvs = CMPLX(cfvs_r, cfvs_i);
-        for (int lm = 1; lm <= nlmso; lm++) { /* regular solution: not irregular */
+        /* :1318-1361: no scattered wave in the irregular solution */
+        for (int lm = 1; lm <= (c->irregular ? 0 : nlmso); lm++) {
           int L = c->lso[lm], m = c->mso[lm];
           int lm0 = L * L + L + 1 + m;
           int jjj = nligneso + lm;
@@ -803,7 +836,11 @@ static void orc_calcmatrow(const row_ctx *x, int ii, double *abvr, double *abvi,
           int lm0 = L * L + L + 1 + m;
           if (cal) {
             double complex ycomp = CMPLX(YLMSO(c, ib, lm0), 0.0);
-            double complex cfv = ycomp * ctx_bes(x, ib, L) * cfvs;
+            double complex cfv;
+            if (c->irregular) /* :1379-1381 */
+              cfv = ycomp * (-I * ctx_bes(x, ib, L) + ctx_neu(x, ib, L)) * cfvs;
+            else
+              cfv = ycomp * ctx_bes(x, ib, L) * cfvs;
             smr_ii[lm - 1] = smr_ii[lm - 1] - creal(cfv);
             smi_ii[lm - 1] = smi_ii[lm - 1] - cimag(cfv);
           } else {
@@ -874,8 +911,21 @@ static void orc_calcmatrow(const row_ctx *x, int ii, double *abvr, double *abvi,
     /* Expansion in atom ia, :1515-1632 (regular solution, Ylm_comp = F,
      * Kar = 1, one spin: ispt = isol = 1) */
     const int ia = -i;
-    const int Lm = ii - c->ianew[ia - 1]; /* Lm0 = Lm with (L, m) ordering */
+    /* :1528-1545: in the irregular solution rows ianew + 1 .. + nlmsa match
+     * the grid on the first half of the border points (i_RI = 1), the next
+     * nlmsa rows on the second half (i_RI = 2) */
+    int Lm = ii - c->ianew[ia - 1]; /* Lm0 = Lm with (L, m) ordering */
+    int i_ri = 1;
+    if (c->n_ato_RI == 2 && Lm > c->nlmsa[ia - 1]) {
+      i_ri = 2;
+      Lm -= c->nlmsa[ia - 1];
+    }
+    const int nh = c->nbordf[ia - 1] / 2;
     for (int ib = 1; ib <= c->nbordf[ia - 1]; ib++) {
+      if (c->irregular && c->n_ato_RI == 2) {
+        if (i_ri == 1 && ib > nh) break;
+        if (i_ri == 2 && ib <= nh) continue;
+      }
       int j = IBORD(c, ib, ia);
       int jj = c->newp[j];
       if (cal) {
@@ -887,10 +937,16 @@ static void orc_calcmatrow(const row_ctx *x, int ii, double *abvr, double *abvi,
         abvr[jj] = abvr[jj] - f * YLMATO(c, ib, Lm, ia);
       }
     }
+    for (int j_ri = 1; j_ri <= (c->irregular ? 2 : 1); j_ri++) /* :1580-1630 */
     for (int Lmp = 1; Lmp <= c->nlmsa[ia - 1]; Lmp++) {
+      const double *phr = j_ri == 2 ? c->phi2_re : c->phi_re, *phi = j_ri == 2 ? c->phi2_im : c->phi_im;
       double charmr = 0, charmi = 0;
       for (int ib = 1; ib <= c->nbordf[ia - 1]; ib++) {
-        const double pr = PHIATO(c->phi_re, c, ib, Lmp, ia), pi = PHIATO(c->phi_im, c, ib, Lmp, ia);
+        if (c->irregular) {
+          if (j_ri == 1 && ib > nh) break;
+          if (j_ri == 2 && ib <= nh) continue;
+        }
+        const double pr = PHIATO(phr, c, ib, Lmp, ia), pi = PHIATO(phi, c, ib, Lmp, ia);
         if (cal) {
           double complex cfac = c->phi_cplx ? CMPLX(pr, pi) : CMPLX(pr, 0.0);
           cfac = cfac * YLMATO(c, ib, Lm, ia);
@@ -901,6 +957,7 @@ static void orc_calcmatrow(const row_ctx *x, int ii, double *abvr, double *abvi,
         }
       }
       int ljj = c->ianew[ia - 1] + Lmp;
+      if (j_ri == 2 && c->n_ato_RI == 2) ljj += c->nlmsa[ia - 1];
       abvr[ljj] = abvr[ljj] + charmr;
       if (cal) abvi[ljj] = abvi[ljj] + charmi;
     }
@@ -919,14 +976,14 @@ static void orc_mark_row(const orc_cluster *c, int ii, unsigned char *colnz) {
       if (ia == 0) {
         colnz[c->newp[j]] = 1;
       } else if (ia > 0) {
-        for (int k = c->ianew[ia - 1] + 1; k <= c->ianew[ia - 1] + c->nlmsa[ia - 1]; k++) colnz[k] = 1;
-      } else if (ia == -2) {
+        for (int k = c->ianew[ia - 1] + 1; k <= c->ianew[ia - 1] + c->n_ato_RI * c->nlmsa[ia - 1]; k++) colnz[k] = 1;
+      } else if (ia == -2 && !c->irregular) {
         for (int k = c->nligneso + 1; k <= c->nligneso + c->nlmso; k++) colnz[k] = 1;
       }
     }
   } else if (i < 0) { /* mat_solve_gaussian.f90:123-137 */
     const int ia = -i;
-    for (int k = c->ianew[ia - 1] + 1; k <= c->ianew[ia - 1] + c->nlmsa[ia - 1]; k++) colnz[k] = 1;
+    for (int k = c->ianew[ia - 1] + 1; k <= c->ianew[ia - 1] + c->n_ato_RI * c->nlmsa[ia - 1]; k++) colnz[k] = 1;
     for (int ib = 1; ib <= c->nbordf[ia - 1]; ib++) colnz[c->newp[IBORD(c, ib, ia)]] = 1;
   } else if (i == 0) {
     for (int k = c->nligneso + 1; k <= c->nligneso + c->nlmso; k++) colnz[k] = 1;
@@ -1498,3 +1555,86 @@ int orc_tau(int nligne, int nlmso, int nligneso, int cal_comp, int basereel, int
   return info;
 }
 #undef M_
+
+/* The symmetrisation of Taull at the end of mat (mat.f90:596-641), one spin,
+ * no spin-orbit: with complex harmonics (Ylm_comp) every pair (Lm1, Lm2) and
+ * its partner (L2 -m2, L1 -m1) become 0.5 (T + (-1)^(m1+m2) T_partner) and
+ * that value times the sign, in the reference's loop order; with real
+ * harmonics only i Im(T) is kept.  Not applied with FDM_comp (:600).
+ * tau: per atom (nlmsa x nlmsa), row Lm1, harmonics in (L, m) order. */
+void orc_tau_sym(int natome, const int *nlmsa, int ylm_comp, double *tau_re, double *tau_im) {
+  size_t off = 0;
+  for (int ia = 0; ia < natome; ia++) {
+    const int na = nlmsa[ia];
+    double complex *T = malloc(sizeof(double complex) * (size_t)na * na + 1);
+    for (size_t k = 0; k < (size_t)na * na; k++) T[k] = CMPLX(tau_re[off + k], tau_im[off + k]);
+#define TT(a, b) T[(size_t)((a)-1) * na + (b)-1]
+    int lm1 = 0;
+    for (int L1 = 0; lm1 < na; L1++)
+      for (int m1 = -L1; m1 <= L1 && lm1 < na; m1++) {
+        lm1++;
+        int lm2 = 0;
+        for (int L2 = 0; lm2 < na; L2++)
+          for (int m2 = -L2; m2 <= L2 && lm2 < na; m2++) {
+            lm2++;
+            if (ylm_comp) {
+              const double isg = ((m1 + m2) & 1) ? -1.0 : 1.0;
+              const int p2 = lm2 - 2 * m2, p1 = lm1 - 2 * m1;
+              const double complex cfac = 0.5 * (TT(lm1, lm2) + isg * TT(p2, p1));
+              TT(lm1, lm2) = cfac;
+              TT(p2, p1) = isg * cfac;
+            } else {
+              TT(lm1, lm2) = CMPLX(0.0, cimag(TT(lm1, lm2)));
+            }
+          }
+      }
+#undef TT
+    for (size_t k = 0; k < (size_t)na * na; k++) {
+      tau_re[off + k] = creal(T[k]);
+      tau_im[off + k] = cimag(T[k]);
+    }
+    free(T);
+    off += (size_t)na * na;
+  }
+}
+
+/* Tau_RI of mat.f90:229-295 (complex energy, Test_reg = 0): per atom,
+ * Ampl(f, Lm1) = the regular cluster solution at row ianew_reg + Lm1,
+ * Ampl_I(f, Lm) = the irregular solution at row ianew_irr + Lm for
+ * Lm = 1 .. n_ato_ri * nlmsa (with n_ato_ri = 1 only its first nlmsa
+ * columns are set, the second block stays zero);
+ * Tau_RI(Lm1, Lm2, i_RI) += sum_f Ampl(f, Lm1) Ampl_I(f, (i_RI-1) nlmsa + Lm2)
+ * for i_RI = 1, n_RI.  Solutions (nligne, nlmso) complex, row-major; out per
+ * atom (n_RI, nlmsa, nlmsa).  round_regular: the regular solution enters
+ * rounded to single precision, as the reference stores it (mat.f90:211). */
+void orc_tau_ri(int nlmso, const double *xr_re, const double *xr_im, const int *ianew_reg, const double *xi_re,
+                const double *xi_im, const int *ianew_irr, int natome, const int *nlmsa, int n_ato_ri, int n_ri,
+                int round_regular, double *out_re, double *out_im) {
+  size_t off = 0;
+  for (int ia = 0; ia < natome; ia++) {
+    const int na = nlmsa[ia];
+    for (int lm1 = 1; lm1 <= na; lm1++) {
+      const size_t ii = (size_t)(ianew_reg[ia] + lm1 - 1) * nlmso;
+      int lm = 0;
+      for (int i_ri = 1; i_ri <= n_ri; i_ri++)
+        for (int lm2 = 1; lm2 <= na; lm2++) {
+          lm++;
+          double complex acc = 0;
+          if (lm <= n_ato_ri * na) {
+            const size_t jj = (size_t)(ianew_irr[ia] + lm - 1) * nlmso;
+            for (int f = 0; f < nlmso; f++) {
+              /* mat.f90:211 keeps the regular solution as cmplx(smr, smi)
+               * of default kind: each part rounded to single precision */
+              const double ar = round_regular ? (double)(float)xr_re[ii + f] : xr_re[ii + f];
+              const double ai = round_regular ? (double)(float)xr_im[ii + f] : xr_im[ii + f];
+              acc = acc + CMPLX(ar, ai) * CMPLX(xi_re[jj + f], xi_im[jj + f]);
+            }
+          }
+          const size_t q = off + ((size_t)(i_ri - 1) * na + (lm1 - 1)) * na + (lm2 - 1);
+          out_re[q] += creal(acc);
+          out_im[q] += cimag(acc);
+        }
+    }
+    off += (size_t)n_ri * na * na;
+  }
+}

@@ -41,6 +41,14 @@ orc_cluster *orc_cluster_new(double rsort, double adimp, int iord, int lmaxso);
  * assembly the atoms' phiato must be set (orc_cluster_set_phiato). */
 orc_cluster *orc_cluster_new_atoms(double rsort, double adimp, int iord, int lmaxso, int natome, const double *posi,
                                    const double *rmt, const int *lmaxa);
+/* The same with irregular != 0: the rows of the irregular cluster solution
+ * (mat.f90:140-147 with Test_reg = 0; newind :1761-1921 with n_ato_RI = 2):
+ * two blocks of nlmsa unknowns per atom (regular and irregular atomic
+ * solutions), no outer-sphere rows (nligneso = nligne); calcMatRow's
+ * irregular branches (:1266-1307, :1318-1398, :1528-1630).  Both phiato
+ * sets must be set (orc_cluster_set_phiato_ri, i_ri 1 and 2). */
+orc_cluster *orc_cluster_new_atoms_ri(double rsort, double adimp, int iord, int lmaxso, int natome,
+                                      const double *posi, const double *rmt, const int *lmaxa, int irregular);
 /* s[0..3] = natome, npr (grid rows), nbtm (largest border), nlmmax */
 void orc_cluster_atom_sizes(const orc_cluster *c, int *s);
 /* atom ia (1-based): info[0..3] = nbord, nbordf, nlmsa, ianew; ibord[nbord]
@@ -53,6 +61,9 @@ void orc_cluster_numia(const orc_cluster *c, int *numia);
 /* phiato of every atom: per atom, per border point ib (its nbord points),
  * nlmsa values (Lm fastest): re[], and im[] when cplx (nphiato7 = 2). */
 int orc_cluster_set_phiato(orc_cluster *c, int cplx, const double *re, const double *im);
+/* i_ri = 1: the regular atomic solutions (as orc_cluster_set_phiato); 2: the
+ * irregular ones (phiato(.., i_RI = 2, ..)) of the irregular cluster solution */
+int orc_cluster_set_phiato_ri(orc_cluster *c, int i_ri, int cplx, const double *re, const double *im);
 void orc_cluster_free(orc_cluster *c);
 
 /* sizes: [0]=npoint [1]=npso [2]=nligne [3]=nlmso [4]=nsortf [5]=nsort [6]=nvois */
@@ -152,4 +163,16 @@ void orc_ylm_real(int lmax, const double *v, double r, double *ylmr);
 #ifdef __cplusplus
 }
 #endif
+
+/* mat.f90:596-641 (one spin, no spin-orbit): symmetrisation of Taull per
+ * atom, in place; tau as orc_tau writes it; ylm_comp selects the complex-
+ * harmonic branch, else i Im(tau) is kept. */
+void orc_tau_sym(int natome, const int *nlmsa, int ylm_comp, double *tau_re, double *tau_im);
+/* mat.f90:229-295: Tau_RI from the regular and irregular cluster solutions
+ * (complex, (nligne, nlmso) row-major each); out per atom (n_ri, nlmsa,
+ * nlmsa), accumulated; round_regular: the regular solution rounded to single
+ * precision first, as mat.f90:211 stores it. */
+void orc_tau_ri(int nlmso, const double *xr_re, const double *xr_im, const int *ianew_reg, const double *xi_re,
+                const double *xi_im, const int *ianew_irr, int natome, const int *nlmsa, int n_ato_ri, int n_ri,
+                int round_regular, double *out_re, double *out_im);
 #endif

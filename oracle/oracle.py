@@ -42,6 +42,11 @@ def lib():
         L.orc_cluster_numia.argtypes = [C.c_void_p, _ip]
         L.orc_cluster_set_phiato.restype = C.c_int
         L.orc_cluster_set_phiato.argtypes = [C.c_void_p, C.c_int, _dp, C.c_void_p]
+        L.orc_cluster_new_atoms_ri.restype = C.c_void_p
+        L.orc_cluster_new_atoms_ri.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, _dp, _dp, _ip,
+                                               C.c_int]
+        L.orc_cluster_set_phiato_ri.restype = C.c_int
+        L.orc_cluster_set_phiato_ri.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, C.c_void_p]
         L.orc_cluster_free.argtypes = [C.c_void_p]
         L.orc_cluster_sizes.argtypes = [C.c_void_p, _ip]
         L.orc_cluster_xyz.argtypes = [C.c_void_p, _dp]
@@ -76,6 +81,9 @@ def lib():
         L.orc_tau.restype = C.c_int
         L.orc_tau.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _dp, C.c_void_p, C.c_int,
                               _ip, _ip, _dp, _dp, C.c_void_p, C.c_void_p]
+        L.orc_tau_sym.argtypes = [C.c_int, _ip, C.c_int, _dp, _dp]
+        L.orc_tau_ri.argtypes = [C.c_int, _dp, _dp, _ip, _dp, _dp, _ip, C.c_int, _ip, C.c_int, C.c_int, C.c_int,
+                                 _dp, _dp]
         _lib = L
     return _lib
 
@@ -89,10 +97,12 @@ class Cluster:
 
     ``rsort`` and ``adimp`` in bohr (reseau, general.f90:9439)."""
 
-    def __init__(self, rsort: float, adimp: float, lmaxso: int, iord: int = 4, atoms=None):
+    def __init__(self, rsort: float, adimp: float, lmaxso: int, iord: int = 4, atoms=None, irregular=False):
         """atoms: None, or (posi (natome, 3) bohr sorted by distance, rmt
-        (natome,) bohr, lmax (natome,)) for muffin-tin atoms (P1)."""
+        (natome,) bohr, lmax (natome,)) for muffin-tin atoms (P1).
+        irregular: the rows of the irregular cluster solution (atoms only)."""
         L = lib()
+        self.irregular = bool(irregular)
         if atoms is None:
             self._h = L.orc_cluster_new(rsort, adimp, iord, lmaxso)
             self.natome = 0
@@ -100,16 +110,16 @@ class Cluster:
             posi, rmt, lmax = atoms
             posi = np.ascontiguousarray(posi, np.float64).reshape(-1, 3)
             self.natome = len(posi)
-            self._h = L.orc_cluster_new_atoms(rsort, adimp, iord, lmaxso, self.natome, posi.reshape(-1),
-                                              np.ascontiguousarray(rmt, np.float64),
-                                              np.ascontiguousarray(lmax, np.int32))
+            self._h = L.orc_cluster_new_atoms_ri(rsort, adimp, iord, lmaxso, self.natome, posi.reshape(-1),
+                                                 np.ascontiguousarray(rmt, np.float64),
+                                                 np.ascontiguousarray(lmax, np.int32), int(self.irregular))
         if not self._h:
             raise MemoryError("orc_cluster_new failed")
         s = np.zeros(7, np.int32)
         L.orc_cluster_sizes(self._h, s)
         (self.npoint, self.npso, self.nligne, self.nlmso, self.nsortf, self.nsort, self.nvois) = map(int, s)
         self.lmaxso = lmaxso
-        self.nligneso = self.nligne - self.nlmso
+        self.nligneso = self.nligne if self.irregular else self.nligne - self.nlmso
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -144,15 +154,18 @@ class Cluster:
         lib().orc_cluster_atom_sizes(self._h, s)
         return int(s[1])
 
-    def set_phiato(self, phi):
+    def set_phiato(self, phi, phi2=None):
         """phi: list over atoms of (nbord, nlmsa) arrays, complex (nphiato7 = 2)
-        or real."""
-        cplx = any(np.iscomplexobj(p) for p in phi)
-        flat = np.concatenate([np.asarray(p).reshape(-1) for p in phi]) if phi else np.zeros(1)
-        re = np.ascontiguousarray(np.real(flat), np.float64)
-        im = np.ascontiguousarray(np.imag(flat), np.float64) if cplx else None
-        if lib().orc_cluster_set_phiato(self._h, int(cplx), re, _ptr(im)):
-            raise MemoryError("orc_cluster_set_phiato failed")
+        or real; phi2 the same for the irregular atomic solutions (the
+        irregular cluster solution)."""
+        sets = [phi] if phi2 is None else [phi, phi2]
+        cplx = any(np.iscomplexobj(p) for xs in sets for p in xs)
+        for i_ri, xs in enumerate(sets, 1):
+            flat = np.concatenate([np.asarray(p).reshape(-1) for p in xs]) if xs else np.zeros(1)
+            re = np.ascontiguousarray(np.real(flat), np.float64)
+            im = np.ascontiguousarray(np.imag(flat), np.float64) if cplx else None
+            if lib().orc_cluster_set_phiato_ri(self._h, i_ri, int(cplx), re, _ptr(im)):
+                raise MemoryError("orc_cluster_set_phiato failed")
 
     @property
     def xyz(self):
@@ -350,3 +363,44 @@ def tau(x, nligneso: int, basereel: bool, fdm_comp: bool, atoms, tau0=None, want
         off += k * k
     norm = (nr + 1j * ni).reshape(nl, nl).T.copy() if want_norm else None
     return out, norm, info
+
+
+def tau_sym(taus, ylm_comp: bool = False):
+    """The symmetrisation of Taull at the end of mat (mat.f90:596-641, one
+    spin): taus a list of per-atom (na, na) complex arrays (harmonics in
+    (L, m) order); returns new arrays."""
+    nlmsa = np.array([t.shape[0] for t in taus], np.int32)
+    tr = np.ascontiguousarray(np.concatenate([np.asarray(t).real.reshape(-1) for t in taus]), np.float64)
+    ti = np.ascontiguousarray(np.concatenate([np.asarray(t).imag.reshape(-1) for t in taus]), np.float64)
+    lib().orc_tau_sym(len(taus), nlmsa, int(bool(ylm_comp)), tr, ti)
+    out, off = [], 0
+    for k in nlmsa:
+        k = int(k)
+        out.append((tr[off:off + k * k] + 1j * ti[off:off + k * k]).reshape(k, k))
+        off += k * k
+    return out
+
+
+def tau_ri(x_reg, x_irr, ianew_reg, ianew_irr, nlmsa, n_ato_ri: int = 2, n_ri: int = 2, round_regular=True):
+    """Tau_RI of mat.f90:229-295 from the regular and irregular cluster
+    solutions (complex (nligne, nlmso) each; ianew 0-based first rows of the
+    atom blocks).  Returns per atom (n_ri, na, na) complex arrays.  With
+    round_regular (the reference's arithmetic) the regular solution enters
+    rounded to single precision, as mat.f90:211 stores it."""
+    xr = np.asarray(x_reg, np.complex128)
+    xi = np.asarray(x_irr, np.complex128)
+    nl = xr.shape[1]
+    nlmsa = np.ascontiguousarray(nlmsa, np.int32)
+    tot = int(sum(n_ri * int(k) * int(k) for k in nlmsa))
+    o_r, o_i = np.zeros(tot), np.zeros(tot)
+    lib().orc_tau_ri(nl, np.ascontiguousarray(xr.real).reshape(-1), np.ascontiguousarray(xr.imag).reshape(-1),
+                     np.ascontiguousarray(ianew_reg, np.int32), np.ascontiguousarray(xi.real).reshape(-1),
+                     np.ascontiguousarray(xi.imag).reshape(-1), np.ascontiguousarray(ianew_irr, np.int32),
+                     len(nlmsa), nlmsa, int(n_ato_ri), int(n_ri), int(bool(round_regular)), o_r, o_i)
+    out, off = [], 0
+    for k in nlmsa:
+        k = int(k)
+        m = n_ri * k * k
+        out.append((o_r[off:off + m] + 1j * o_i[off:off + m]).reshape(n_ri, k, k))
+        off += m
+    return out

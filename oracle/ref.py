@@ -362,7 +362,8 @@ def atom_system(rsort: float, adimp: float, lmaxso: int, posi, rmt, lmax, iord: 
                   p(lb2), A.i(0), A.i(1), A.i(natome), p(nbordf), A.i(nbtm), p(newp), p(newinv), A.i(nligne),
                   A.arr(nlmsa, np.int32), A.i(nlmso), A.i(npoint), A.i(nsortf), A.i(1), A.i(npsom), A.i(nstm),
                   p(numia), A.i(nvois), p(xyz))
-    return dict(npoint=npoint, npr=npr, npso=npso, npsom=npsom, natome=natome, posi=posi, rmt=rmt, lmax=lmax,
+    return dict(rsort=rsort, adimp=adimp, npoint=npoint, npr=npr, npso=npso, npsom=npsom, natome=natome, posi=posi,
+                rmt=rmt, lmax=lmax,
                 lmaxso=lmaxso, nlmso=nlmso, nligne=nligne, nligneso=nligne - nlmso, indice=indice, xyz=xyz, rvol=rvol,
                 numia=numia, clapl=clapl, ivois=ivois, isvois=isvois, nsort=nsort, nsortf=nsortf, nsm=nsm, nstm=nstm,
                 isrt=isrt, poidso=poidso, nbm=nbm, nbtm=nbtm, nbord=nbord, nbordf=nbordf, ibord=ibord, isbord=isbord,
@@ -392,23 +393,47 @@ def phiso_sys(s, E_comp: bool, Ecinetic_out: float, Eimag: float, Enervide: floa
     return (bes, neu) if E_comp else (besr, neur)
 
 
-def mat_solve_sys(s, V, Enervide, Eimag, bes, neu, E_comp, cal_comp, basereel, phiato, fn=None):
+def irregular_layout(s):
+    """The reference newind (mat.f90:1761) for the irregular cluster solution
+    of an atom_system dict (mat.f90:140-147 with Test_reg = 0: n_ato_RI = 2,
+    no outer-sphere rows): nligne, ianew, lb1, lb2, newinv, new."""
+    A = _Args()
+    p = lambda a: a.ctypes.data_as(C.c_void_p)
+    natome = s["natome"]
+    nligne = s["npr"] + 2 * int(s["nlmsa"].sum())
+    lb1 = np.zeros(nligne, np.int32)
+    lb2 = np.zeros(nligne, np.int32)
+    newinv = np.zeros(nligne, np.int32)
+    newp = np.zeros(s["npoint"] + 1, np.int32)
+    ianew = np.zeros(natome, np.int32)
+    distai = np.sqrt((s["posi"] ** 2).sum(axis=0))
+    lib().newind_(A.arr(distai, np.float64), p(ianew), A.arr(s["ibord"], np.int32), A.i(0), A.b(True),
+                  A.arr(s["isrt"], np.int32), A.arr(s["ivois"], np.int32), p(lb1), p(lb2), A.i(0), A.i(2),
+                  A.i(natome), A.arr(s["nbordf"], np.int32), A.i(s["nbtm"]), p(newp), p(newinv), A.i(nligne),
+                  A.arr(s["nlmsa"], np.int32), A.i(s["nlmso"]), A.i(s["npoint"]), A.i(s["nsortf"]), A.i(1),
+                  A.i(s["npsom"]), A.i(s["nstm"]), A.arr(s["numia"], np.int32), A.i(s["ivois"].shape[1]),
+                  A.arr(s["xyz"], np.float64))
+    return dict(nligne=nligne, ianew=ianew, lb1=lb1, lb2=lb2, newinv=newinv, new=newp)
+
+
+def mat_solve_sys(s, V, Enervide, Eimag, bes, neu, E_comp, cal_comp, basereel, phiato, fn=None, phiato2=None):
     """Reference mat_solve (with its calcMatRow) on an atom_system dict.
     phiato: list over atoms of (nbord(ia), nlmsa(ia)) arrays, complex
     (nphiato7 = 2) or real.  Returns the solution (nligne, nlmso).  ``fn``
-    calls another procedure with the same argument surface instead."""
+    calls another procedure with the same argument surface instead.  With
+    phiato2 (the irregular atomic solutions) it solves the irregular cluster
+    solution instead (irregular, n_ato_RI = n_RI = 2, Test_reg = 0, the
+    irregular_layout rows)."""
     A = _Args()
+    irregular = phiato2 is not None
+    if irregular:
+        lay = irregular_layout(s)
+        s = dict(s, nligne=lay["nligne"], nligneso=lay["nligne"], ianew=lay["ianew"], lb1=lay["lb1"],
+                 lb2=lay["lb2"], newinv=lay["newinv"], new=lay["new"])
     n, nl, ns, lm = s["nligne"], s["nlmso"], s["nsort"], s["lmaxso"]
     natome, nbtm, nlmmax = s["natome"], s["nbtm"], s["nlmmax"]
     nvois = s["ivois"].shape[1]
-    cplx_phi = any(np.iscomplexobj(x) for x in phiato)
-    nphi7 = 2 if cplx_phi else 1
-    phi = np.zeros((nbtm, nlmmax, 1, 1, natome, nphi7), order="F")
-    for ia, x in enumerate(phiato):
-        nb, na = x.shape
-        phi[:nb, :na, 0, 0, ia, 0] = np.real(x)
-        if cplx_phi:
-            phi[:nb, :na, 0, 0, ia, 1] = np.imag(x)
+    phi, nphi7 = _phi_array(s, phiato, phiato2)
     nlmsam = nlmmax
     lato = np.zeros((nlmsam, natome, 1), np.int32, order="F")
     mato = np.zeros((nlmsam, natome, 1), np.int32, order="F")
@@ -441,10 +466,11 @@ def mat_solve_sys(s, V, Enervide, Eimag, bes, neu, E_comp, cal_comp, basereel, p
         A.b(False), A.b(basereel), p(Bc), p(Br), A.b(cal_comp), A.arr(np.zeros(nvois), np.float64),
         A.arr(s["clapl"], np.float64), A.b(E_comp), A.d(Eimag), A.d(Enervide), A.arr(np.zeros(3), np.float64),
         A.arr(s["ianew"], np.int32), p(iato), A.arr(s["ibord"], np.int32), A.i(0), A.i(1), A.i(1), A.i(0),
-        A.b(False), A.arr(s["isbord"], np.int32), A.arr(iso, np.int32), A.i(1), A.arr(s["isrt"], np.int32),
+        A.b(irregular), A.arr(s["isbord"], np.int32), A.arr(iso, np.int32), A.i(1), A.arr(s["isrt"], np.int32),
         A.arr(s["isvois"], np.int32), A.arr(s["ivois"], np.int32), A.arr(kar, np.float64), A.arr(kari, np.float64),
         p(lato), A.arr(s["lb1"], np.int32), A.arr(s["lb2"], np.int32), A.i(lm), A.arr(lso, np.int32),
-        p(mato), A.i(1), A.i(0), A.arr(mso, np.int32), A.i(1), A.i(1), A.i(natome), A.i(s["nbm"]),
+        p(mato), A.i(1), A.i(0), A.arr(mso, np.int32), A.i(2 if irregular else 1), A.i(2 if irregular else 1),
+        A.i(natome), A.i(s["nbm"]),
         A.arr(s["nbord"], np.int32), A.arr(s["nbordf"], np.int32), A.i(nbtm), p(Nc), p(Nr),
         A.arr(s["new"], np.int32), A.arr(s["newinv"], np.int32), A.i(1), A.i(1), A.i(s["npoint"]), A.i(n),
         A.i(n if cal_comp else 0), A.i(s["nligneso"]), A.i(nlmsam), A.i(nlmmax), A.i(nlmmax), A.i(nl),
@@ -462,3 +488,106 @@ def mat_solve_sys(s, V, Enervide, Eimag, bes, neu, E_comp, cal_comp, basereel, p
     if cal_comp:
         x = x + 1j * smi.T
     return x
+
+
+def _phi_array(s, phiato, phiato2=None):
+    """phiato(nphiato1, nlmagm, nspinp, nspino, n_RI, natome, nphiato7) from
+    per-atom (nbord, nlmsa) arrays of the regular (and irregular) atomic
+    solutions."""
+    sets = [phiato] if phiato2 is None else [phiato, phiato2]
+    cplx = any(np.iscomplexobj(x) for xs in sets for x in xs)
+    nphi7 = 2 if cplx else 1
+    phi = np.zeros((s["nbtm"], s["nlmmax"], 1, 1, len(sets), s["natome"], nphi7), order="F")
+    for r, xs in enumerate(sets):
+        for ia, x in enumerate(xs):
+            nb, na = x.shape
+            phi[:nb, :na, 0, 0, r, ia, 0] = np.real(x)
+            if cplx:
+                phi[:nb, :na, 0, 0, r, ia, 1] = np.imag(x)
+    return phi, nphi7
+
+
+MAT_ARGS = (
+    "Adimp Ampl_dft Atom_axe Axe_atom_grn Base_hexa Basereelt Cal_xanes cgrad clapl Dist_coop distai E_comp "
+    "Ecinetic_out Eclie_out Eimag Eneg Enervide FDM_comp Full_atom gradvr ia_coop iaabsi iaprotoi iato ibord icheck "
+    "ie igroupi igrph irep_util isbord iso ispinin isrt ivois isvois karact Lato Lmaxa lmaxso Lso mato mpinodes "
+    "mpirank0 mso n_atom_0 n_atom_coop n_atom_ind n_atom_proto n_RI nab_coop natome nbm nbord nbordf nbtm ngroup_m "
+    "ngrph nim nicm nlm_rixs nlmagm nlmmax nlmomax nlmsa nlmsam nlmso nphiato1 nphiato7 npoint npr npsom nsm nsort "
+    "nsortf nso1 nspin nspino nspinp nstm numia nvois phiato poidsa poidso Posi R_rydb Recop Relativiste Repres_comp "
+    "RIXS Rmtg Rsort rvol Rydberg Solsing Spinorbite State_all_r Sym_cubic Tau_ato Tau_coop Tau_RI Taull Test_reg "
+    "Time_fill Time_tria V0bd_out Vr xyz Ylm_comp Ylmato Ylmso").split()
+
+
+def mat_tau(s, adimp, V, E, Eimag, E_comp, fdm_comp, basereel, phiato, phiato2=None, ylm_comp=False):
+    """The reference ``mat`` (mat.f90:5) on an atom_system dict: phiso,
+    newind and mat_solve (with its calcMatRow) for the regular cluster
+    solution and, with FDM_comp (n_RI = 2, Test_reg = 0), the irregular one,
+    then the reduction to the atoms' amplitude products: Taull (mat.f90:297-
+    405, then the symmetrisation :596-641) or, with complex energy, Tau_RI
+    (:229-295).  Point group 1, one spin, every atom (Cal_xanes false).
+    phiato / phiato2: per atom (nbord, nlmsa) regular / irregular atomic
+    solutions.  Returns (Taull (natome, nlm, nlm), Tau_RI (natome, n_RI,
+    nlm, nlm)) with nlm = nlmmax."""
+    A = _Args()
+    p = lambda a: a.ctypes.data_as(C.c_void_p)
+    natome, nbtm, nlmmax, nl, lm = s["natome"], s["nbtm"], s["nlmmax"], s["nlmso"], s["lmaxso"]
+    nvois = s["ivois"].shape[1]
+    n_RI = 2 if fdm_comp else 1
+    phi, nphi7 = _phi_array(s, phiato, phiato2 if fdm_comp else None)
+    nrepm = 12
+    lato = np.zeros((nlmmax, natome, 1), np.int32, order="F")
+    mato = np.zeros((nlmmax, natome, 1), np.int32, order="F")
+    iato = np.ones((nlmmax, natome, 1), np.int32, order="F")
+    for ia in range(natome):
+        k = 0
+        for L in range(int(s["lmax"][ia]) + 1):
+            for m in range(-L, L + 1):
+                lato[k, ia, 0], mato[k, ia, 0] = L, m
+                k += 1
+    lso = np.array([L for L in range(lm + 1) for m in range(-L, L + 1)], np.int32)
+    mso = np.array([m for L in range(lm + 1) for m in range(-L, L + 1)], np.int32)
+    irep_util = np.zeros((nrepm, 2), np.int32, order="F")
+    irep_util[0, :] = 1
+    karact = np.zeros((NOPSM, nrepm), np.complex128, order="F")
+    karact[:, 0] = 1.0
+    taull = np.zeros((nlmmax, 1, nlmmax, 1, natome), np.complex128, order="F")
+    tau_ri = np.zeros((nlmmax, 1, nlmmax, 1, n_RI, natome), np.complex128, order="F")
+    tau_ato = np.zeros((nlmmax, 1, nlmmax, 1, 1), np.complex128, order="F")
+    dz = np.zeros(1, np.complex128)
+    tf, tt = np.zeros(1), np.zeros(1)
+    distai = np.sqrt((s["posi"] ** 2).sum(axis=0))
+    i32, f64 = np.int32, np.float64
+    # mat.f90:5-13, argument by argument
+    v = dict(
+        Adimp=A.d(adimp), Ampl_dft=p(dz), Atom_axe=A.arr(np.zeros(natome, i32), i32),
+        Axe_atom_grn=A.arr(np.zeros((3, 1)), f64), Base_hexa=A.b(False), Basereelt=A.b(basereel),
+        Cal_xanes=A.b(False), cgrad=A.arr(np.zeros(nvois), f64), clapl=A.arr(s["clapl"], f64),
+        Dist_coop=A.arr(np.zeros(2), f64), distai=A.arr(distai, f64), E_comp=A.b(E_comp),
+        Ecinetic_out=A.arr([E], f64), Eclie_out=A.d(0.0), Eimag=A.d(Eimag), Eneg=A.b(False), Enervide=A.d(E),
+        FDM_comp=A.b(fdm_comp), Full_atom=A.b(False), gradvr=A.arr(np.zeros(3), f64), ia_coop=A.arr([0], i32),
+        iaabsi=A.i(1), iaprotoi=A.arr(np.arange(1, natome + 1), i32), iato=p(iato), ibord=A.arr(s["ibord"], i32),
+        icheck=A.i(0), ie=A.i(1), igroupi=A.arr(np.arange(1, natome + 1), i32), igrph=A.i(1),
+        irep_util=A.arr(irep_util, i32), isbord=A.arr(s["isbord"], i32), iso=A.arr(np.ones(nl, i32), i32),
+        ispinin=A.i(1), isrt=A.arr(s["isrt"], i32), ivois=A.arr(s["ivois"], i32), isvois=A.arr(s["isvois"], i32),
+        karact=A.arr(karact, np.complex128), Lato=p(lato), Lmaxa=A.arr(s["lmax"], i32), lmaxso=A.i(lm),
+        Lso=A.arr(lso, i32), mato=p(mato), mpinodes=A.i(1), mpirank0=A.i(0), mso=A.arr(mso, i32),
+        n_atom_0=A.i(1), n_atom_coop=A.i(0), n_atom_ind=A.i(1), n_atom_proto=A.i(natome), n_RI=A.i(n_RI),
+        nab_coop=A.i(0), natome=A.i(natome), nbm=A.i(s["nbm"]), nbord=A.arr(s["nbord"], i32),
+        nbordf=A.arr(s["nbordf"], i32), nbtm=A.i(nbtm), ngroup_m=A.i(1), ngrph=A.i(1), nim=A.i(s["npoint"]),
+        nicm=A.i(1), nlm_rixs=A.i(1), nlmagm=A.i(nlmmax), nlmmax=A.i(nlmmax), nlmomax=A.i(nl),
+        nlmsa=A.arr(s["nlmsa"], i32), nlmsam=A.i(nlmmax), nlmso=A.i(nl), nphiato1=A.i(nbtm), nphiato7=A.i(nphi7),
+        npoint=A.i(s["npoint"]), npr=A.i(s["npr"]), npsom=A.i(s["npsom"]), nsm=A.i(max(s["nsm"], 1)),
+        nsort=A.i(s["nsort"]), nsortf=A.i(s["nsortf"]), nso1=A.i(nl), nspin=A.i(1), nspino=A.i(1), nspinp=A.i(1),
+        nstm=A.i(s["nstm"]), numia=A.arr(s["numia"], i32), nvois=A.i(nvois), phiato=p(phi),
+        poidsa=A.arr(s["poidsa"], f64), poidso=A.arr(s["poidso"], f64), Posi=A.arr(s["posi"], f64),
+        R_rydb=A.d(0.0), Recop=A.b(False), Relativiste=A.b(False), Repres_comp=A.b(False), RIXS=A.b(False),
+        Rmtg=A.arr(np.zeros(natome + 1), f64), Rsort=A.d(s["rsort"]), rvol=A.arr(s["rvol"][: s["npoint"]], f64),
+        Rydberg=A.b(False), Solsing=A.b(False), Spinorbite=A.b(False), State_all_r=A.b(False),
+        Sym_cubic=A.b(False), Tau_ato=p(tau_ato), Tau_coop=p(dz), Tau_RI=p(tau_ri), Taull=p(taull),
+        Test_reg=A.i(0), Time_fill=p(tf), Time_tria=p(tt), V0bd_out=A.arr([0.0], f64),
+        Vr=A.arr(np.asarray(V, f64), f64), xyz=A.arr(s["xyz"], f64), Ylm_comp=A.b(ylm_comp),
+        Ylmato=A.arr(s["ylmato"], f64), Ylmso=A.arr(s["ylmso"], f64))
+    args = [v[k] for k in MAT_ARGS]
+    lib().mat_(*args)
+    return (np.transpose(taull[:, 0, :, 0, :], (2, 0, 1)).copy(),
+            np.transpose(tau_ri[:, 0, :, 0, :, :], (3, 2, 0, 1)).copy())

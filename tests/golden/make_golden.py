@@ -20,6 +20,11 @@ clusters and stores inputs and outputs as .npz fixtures:
                           -> atom border lists, weights, Ylmato, atom blocks
                              in the profile, and the solution for the three
                              branch sets
+  mat (mat.f90:5) on that cluster
+                          -> Taull after its symmetrisation (real and complex
+                             basis), and with FDM_comp at complex energy the
+                             irregular cluster solution (newind and mat_solve
+                             with irregular, n_ato_RI = 2) and Tau_RI
 The cluster arrays fed to the reference (grid, neighbours, boundary lists,
 Ylmso) come from the oracle's restatement; the fixtures store the profile
 and matrix-relevant inputs so the tests can rebuild each case.
@@ -127,6 +132,53 @@ def atom_fixture():
     print("atoms", s["npoint"], s["npr"], s["nligne"], s["nbord"], s["nbordf"], s["ianew"])
 
 
+def atom_phiato_irregular(s, E, Eimag):
+    """Synthetic irregular atomic solutions: y_L(k r) Y_Lm at every border
+    point, k = sqrt(E + 1.5 + i Eimag), r clamped to rmt / 2 (the atom's
+    centre point can be a border point, where y_L diverges)."""
+    from scipy.special import spherical_yn
+    k = np.sqrt(E + 1.5 + 1j * Eimag)
+    out = []
+    for ia in range(s["natome"]):
+        nb, nl = int(s["nbord"][ia]), int(s["nlmsa"][ia])
+        pts = s["xyz"][:3, s["ibord"][:nb, ia] - 1].T - s["posi"][:, ia]
+        r = np.maximum(np.linalg.norm(pts, axis=1), 0.5 * s["rmt"][ia])
+        ls = np.array([L for L in range(int(s["lmax"][ia]) + 1) for m in range(-L, L + 1)])
+        out.append(np.stack([spherical_yn(L, k * r) for L in ls], axis=1) * s["ylmato"][:nb, :nl, ia])
+    return out
+
+
+def atom_tau_fixture():
+    """The reference mat (mat.f90:5) on the three-atom cluster: Taull after
+    its symmetrisation (real energy; real and complex basis) and, with
+    FDM_comp at complex energy, the irregular cluster solution and Tau_RI."""
+    a = ATOMS
+    posi = np.array(a["posi"]) / BOHR
+    s = R.atom_system(a["R"] / BOHR, a["h"] / BOHR, a["lmaxso"], posi, np.array(a["rmt"]) / BOHR, a["lmax"])
+    V = np.random.default_rng(7).uniform(-0.3, 0.3, s["npoint"]) - 0.8
+    out = dict(V=V)
+    E = 0.7
+    phi = atom_phiato(s, E, 0.0, False)
+    for basereel in (True, False):
+        T, _ = R.mat_tau(s, a["h"] / BOHR, V, E, 0.0, False, False, basereel, phi)
+        out[f"taull_basereel{int(basereel)}"] = T
+    out["E"], out["phi_real"] = E, np.concatenate([p.reshape(-1) for p in phi])
+    Ec, Ei = 0.7, 0.05
+    phi = atom_phiato(s, Ec, Ei, True)
+    phi2 = atom_phiato_irregular(s, Ec, Ei)
+    _, TR = R.mat_tau(s, a["h"] / BOHR, V, Ec, Ei, True, True, False, phi, phi2)
+    lay = R.irregular_layout(s)
+    bes, neu = R.phiso_sys(s, True, Ec, Ei, Ec)
+    xi = R.mat_solve_sys(s, V, Ec, Ei, bes, neu, True, True, False, phi, phiato2=phi2)
+    rows = np.arange(0, lay["nligne"], ROW_STRIDE)
+    out.update(Ec=Ec, Eimag=Ei, phi_c=np.concatenate([p.reshape(-1) for p in phi]),
+               phi2_c=np.concatenate([p.reshape(-1) for p in phi2]), tau_ri=TR, irr_rows=rows, x_irr=xi[rows],
+               irr_nligne=lay["nligne"], irr_ianew=lay["ianew"], irr_lb1=lay["lb1"], irr_lb2=lay["lb2"],
+               irr_newinv=lay["newinv"], irr_new=lay["new"])
+    np.savez_compressed(os.path.join(HERE, "atoms_tau_R1.9_h0.25.npz"), **out)
+    print("atom tau", lay["nligne"], np.abs(TR).max())
+
+
 def main():
     if not R.available():
         raise SystemExit("oracle/_ref/libfdmnes_ref.so is missing: make -f oracle/Makefile.ref")
@@ -157,6 +209,7 @@ def main():
     print("special functions written")
     grid_fixtures()
     atom_fixture()
+    atom_tau_fixture()
 
 
 if __name__ == "__main__":
