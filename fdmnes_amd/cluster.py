@@ -45,6 +45,14 @@ def _L():
         L.fdmgpu_cluster_create_atoms.restype = C.c_int
         L.fdmgpu_cluster_create_atoms.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int, _lib.dp, _lib.dp,
                                                   _lib.ip, C.POINTER(C.c_void_p)]
+        L.fdmgpu_cluster_create_atoms_ri.restype = C.c_int
+        L.fdmgpu_cluster_create_atoms_ri.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int, _lib.dp, _lib.dp,
+                                                     _lib.ip, C.c_int, C.POINTER(C.c_void_p)]
+        L.fdmgpu_cluster_assemble_atoms_ri.restype = C.c_int
+        L.fdmgpu_cluster_assemble_atoms_ri.argtypes = [C.c_void_p, _lib.dp, C.c_double, C.c_double, C.c_double,
+                                                       C.c_int, C.c_int, _lib.dp, C.c_void_p, C.c_void_p,
+                                                       C.c_void_p, _lib.lp, _lib.ip, _lib.dp, C.c_void_p,
+                                                       C.c_void_p, C.c_void_p]
         L.fdmgpu_cluster_atom_sizes.restype = C.c_int
         L.fdmgpu_cluster_atom_sizes.argtypes = [C.c_void_p, _lib.lp]
         L.fdmgpu_cluster_atom.restype = C.c_int
@@ -79,9 +87,16 @@ class Cluster:
 
     GROUPS = {1: 0, 3: 4, 6: 3, 8: 7}  # point-group number -> mirrors (bit a = normal to axis a)
 
-    def __init__(self, rsort: float, adimp: float, lmaxso: int, irrep: int = -1, group: int = 8, atoms=None):
+    def __init__(self, rsort: float, adimp: float, lmaxso: int, irrep: int = -1, group: int = 8, atoms=None,
+                 irregular: bool = False):
+        """irregular (with atoms): the rows of the irregular cluster solution
+        (fdmgpu_cluster_create_atoms_ri), assembled with
+        :meth:`assemble_atoms` given both atomic solutions."""
         L = _L()
         h = C.c_void_p()
+        self.irregular = bool(irregular)
+        if self.irregular and atoms is None:
+            raise ValueError("the irregular cluster solution needs atoms")
         if irrep < 0:
             group = 1
         self.natome = 0
@@ -91,9 +106,10 @@ class Cluster:
             posi, rmt, lmax = atoms
             posi = np.ascontiguousarray(posi, np.float64).reshape(-1, 3)
             self.natome = len(posi)
-            rc = L.fdmgpu_cluster_create_atoms(rsort, adimp, lmaxso, self.natome, posi.reshape(-1),
-                                               np.ascontiguousarray(rmt, np.float64),
-                                               np.ascontiguousarray(lmax, np.int32), C.byref(h))
+            rc = L.fdmgpu_cluster_create_atoms_ri(rsort, adimp, lmaxso, self.natome, posi.reshape(-1),
+                                                  np.ascontiguousarray(rmt, np.float64),
+                                                  np.ascontiguousarray(lmax, np.int32), int(self.irregular),
+                                                  C.byref(h))
         else:
             rc = L.fdmgpu_cluster_create_group(rsort, adimp, lmaxso, int(group), int(irrep), C.byref(h))
         if rc:
@@ -194,17 +210,21 @@ class Cluster:
         return dict(nbord=nb, nbordf=nbf, nlmsa=nl, ianew=ianew, ibord=ib, poidsa=pa, ylmato=ya.reshape(nb, nl))
 
     def assemble_atoms(self, V, Enervide: float, Eimag: float, Ecinetic_out: float, E_comp: bool, basereel: bool,
-                       phiato):
+                       phiato, phiato2=None):
         """Compressed rows and incoming-wave RHS of one energy point of a
         cluster with atoms; phiato: per atom an (nbord, nlmsa) array of the
-        radial solutions times Ylm at its border points, complex or real."""
+        radial solutions times Ylm at its border points, complex or real;
+        phiato2 the irregular atomic solutions (irregular clusters only)."""
         cal = bool(E_comp) or not basereel
-        cplx = any(np.iscomplexobj(p) for p in phiato)
+        sets = [phiato] if phiato2 is None else [phiato, phiato2]
+        cplx = any(np.iscomplexobj(p) for xs in sets for p in xs)
         if cplx and not cal:
             raise ValueError("complex radial solutions need complex arithmetic (E_comp or not basereel)")
-        flat = np.concatenate([np.asarray(p).reshape(-1) for p in phiato])
-        pr = np.ascontiguousarray(flat.real, np.float64)
-        pi = np.ascontiguousarray(flat.imag, np.float64) if cplx else None
+        flats = [np.concatenate([np.asarray(p).reshape(-1) for p in xs]) for xs in sets]
+        pr = np.ascontiguousarray(flats[0].real, np.float64)
+        pi = np.ascontiguousarray(flats[0].imag, np.float64) if cplx else None
+        p2r = np.ascontiguousarray(flats[1].real, np.float64) if phiato2 is not None else None
+        p2i = np.ascontiguousarray(flats[1].imag, np.float64) if (phiato2 is not None and cplx) else None
         n, nl = self.nligne, self.nlmso
         rp = np.zeros(n + 1, np.int64)
         col = np.zeros(self.nnz, np.int32)
@@ -212,9 +232,10 @@ class Cluster:
         im = np.zeros(self.nnz) if cal else None
         rr = np.zeros(n * nl)
         ri = np.zeros(n * nl) if cal else None
-        rc = _L().fdmgpu_cluster_assemble_atoms(self._h, np.ascontiguousarray(V, np.float64), Enervide, Eimag,
-                                                Ecinetic_out, int(bool(E_comp)), int(bool(basereel)), pr,
-                                                _lib.ptr(pi), rp, col, re, _lib.ptr(im), _lib.ptr(rr), _lib.ptr(ri))
+        rc = _L().fdmgpu_cluster_assemble_atoms_ri(self._h, np.ascontiguousarray(V, np.float64), Enervide, Eimag,
+                                                   Ecinetic_out, int(bool(E_comp)), int(bool(basereel)), pr,
+                                                   _lib.ptr(pi), _lib.ptr(p2r), _lib.ptr(p2i), rp, col, re,
+                                                   _lib.ptr(im), _lib.ptr(rr), _lib.ptr(ri))
         if rc:
             raise _lib.FdmGpuError(f"fdmgpu_cluster_assemble_atoms failed ({rc})")
         vals = re + 1j * im if cal else re

@@ -680,4 +680,103 @@ int fdmgpu_tau_reduce(const fdmgpu_tau_desc *d, int count, const double *const *
   return 0;
 }
 
+int fdmgpu_tau_ri(const fdmgpu_tau_ri_desc *d, int count, const double *const *d_xreg, const double *const *d_xirr,
+                  double *const *d_tau_ri, int accumulate, void *hip_stream) {
+  if (!d || count < 0 || d->natome < 0 || (d->natome > 0 && (!d->ianew_reg || !d->ianew_irr || !d->nlmsa)) ||
+      d->n_ri < 1 || d->n_ri > 2 || d->n_ato_ri < 1 || d->n_ato_ri > d->n_ri || d->nlmso < 1 ||
+      (count > 0 && d->natome > 0 && (!d_xreg || !d_xirr || !d_tau_ri))) {
+    set_error("fdmgpu_tau_ri: invalid argument");
+    return FDMGPU_EINVAL;
+  }
+  for (int a = 0; a < d->natome; a++) {
+    const int na = d->nlmsa[a];
+    if (na < 0 || d->ianew_reg[a] < 0 || d->ianew_reg[a] + na > d->nligne_reg || d->ianew_irr[a] < 0 ||
+        d->ianew_irr[a] + d->n_ato_ri * na > d->nligne_irr) {
+      set_error("fdmgpu_tau_ri: rows of atom " + std::to_string(a) + " outside a solution");
+      return FDMGPU_EINVAL;
+    }
+  }
+  for (int k = 0; k < count && d->natome > 0; k++)
+    if (!d_xreg[k] || !d_xirr[k] || !d_tau_ri[k]) {
+      set_error("fdmgpu_tau_ri: null device buffer for point " + std::to_string(k));
+      return FDMGPU_EINVAL;
+    }
+  if (count == 0 || d->natome == 0) return 0;
+  int rc = ensure_init();
+  if (rc) return rc;
+  hipStream_t s = hip_stream ? (hipStream_t)hip_stream : g_lib.stream;
+  for (int p0 = 0; p0 < count; p0 += TAU_MAXP) {
+    const int np = std::min(TAU_MAXP, count - p0);
+    int64_t off = 0;
+    for (int a0 = 0; a0 < d->natome; a0 += TAU_MAXA) {
+      const int na = std::min(TAU_MAXA, d->natome - a0);
+      TauRIJob J{};
+      for (int k = 0; k < np; k++) {
+        J.xr[k] = d_xreg[p0 + k];
+        J.xi[k] = d_xirr[p0 + k];
+        J.tau[k] = d_tau_ri[p0 + k];
+      }
+      for (int a = 0; a < na; a++) {
+        J.row_reg[a] = d->ianew_reg[a0 + a];
+        J.row_irr[a] = d->ianew_irr[a0 + a];
+        J.na[a] = d->nlmsa[a0 + a];
+        J.off[a] = off;
+        off += (int64_t)d->n_ri * J.na[a] * J.na[a];
+      }
+      rc = tau_ri_launch(J, np, na, d->nlmso, d->n_ato_ri, d->n_ri, accumulate != 0, s);
+      if (rc) return rc;
+    }
+  }
+  return 0;
+}
+
+int fdmgpu_tau_symmetrize(int natome, const int *nlmsa, int ylm_comp, int count, double *const *d_tau,
+                          void *hip_stream) {
+  if (natome < 0 || count < 0 || (natome > 0 && !nlmsa) || (count > 0 && natome > 0 && !d_tau)) {
+    set_error("fdmgpu_tau_symmetrize: invalid argument");
+    return FDMGPU_EINVAL;
+  }
+  int na_max = 0;
+  for (int a = 0; a < natome; a++) {
+    const int L = (int)std::lround(std::sqrt((double)nlmsa[a])) - 1;
+    if (nlmsa[a] < 1 || (L + 1) * (L + 1) != nlmsa[a]) {
+      set_error("fdmgpu_tau_symmetrize: nlmsa must be (lmax+1)^2 (harmonics in (L, m) order)");
+      return FDMGPU_EINVAL;
+    }
+    na_max = std::max(na_max, nlmsa[a]);
+  }
+  if ((size_t)na_max * na_max * 16 > TAU_MAX_LDS) {
+    set_error("fdmgpu_tau_symmetrize: nlmsa^2 complex entries exceed the 160 KiB LDS");
+    return FDMGPU_EINVAL;
+  }
+  for (int k = 0; k < count && natome > 0; k++)
+    if (!d_tau[k]) {
+      set_error("fdmgpu_tau_symmetrize: null device buffer for point " + std::to_string(k));
+      return FDMGPU_EINVAL;
+    }
+  if (count == 0 || natome == 0) return 0;
+  int rc = ensure_init();
+  if (rc) return rc;
+  hipStream_t s = hip_stream ? (hipStream_t)hip_stream : g_lib.stream;
+  for (int p0 = 0; p0 < count; p0 += TAU_MAXP) {
+    const int np = std::min(TAU_MAXP, count - p0);
+    int64_t off = 0;
+    for (int a0 = 0; a0 < natome; a0 += TAU_MAXA) {
+      const int na = std::min(TAU_MAXA, natome - a0);
+      TauSymJob J{};
+      for (int k = 0; k < np; k++) J.tau[k] = d_tau[p0 + k];
+      int chunk_max = 0;
+      for (int a = 0; a < na; a++) {
+        J.na[a] = nlmsa[a0 + a];
+        J.off[a] = off;
+        off += (int64_t)J.na[a] * J.na[a];
+        chunk_max = std::max(chunk_max, J.na[a]);
+      }
+      rc = tau_sym_launch(J, np, na, chunk_max, ylm_comp != 0, s);
+      if (rc) return rc;
+    }
+  }
+  return 0;
+}
+
 }  // extern "C"

@@ -123,6 +123,11 @@ struct fdmgpu_cluster {
   std::vector<std::vector<double>> aylm;    // per atom, nbord x nlmsa real harmonics about the atom
   std::vector<int> rkind, ridx;          // per row: 0 grid (point), 1 atom (atom, Lm packed), 2 outer (lm)
   int nso = 0;                           // first outer row (0-based)
+  // the irregular cluster solution (mat.f90:140-147, Test_reg = 0): two
+  // blocks per atom (regular then irregular atomic solutions, nri = 2), no
+  // outer-sphere rows (nso = nligne)
+  bool irregular = false;
+  int nri = 1;
 };
 
 namespace {
@@ -205,16 +210,17 @@ int build_atoms(fdmgpu_cluster *c) {
     while (i < np && !(c->numia[i] == 0 && c->rad[i] >= dist)) i++;
     if (i >= np) return -1;  // no grid row beyond the atom
     npf = i;
-    c->ianew[a] = (npf1 == npf) ? c->ianew[a - 1] + c->nlmsa[a - 1] : nw[npf] - 1;
+    c->ianew[a] = (npf1 == npf) ? c->ianew[a - 1] + c->nri * c->nlmsa[a - 1] : nw[npf] - 1;
     npf1 = npf;
     for (int k = npf; k < np; k++)
-      if (!c->numia[k]) nw[k] += c->nlmsa[a];
+      if (!c->numia[k]) nw[k] += c->nri * c->nlmsa[a];
   }
   int nat = 0;
-  for (int a = 0; a < na; a++) nat += c->nlmsa[a];
-  const int nlig = npr + nat + nl;
+  for (int a = 0; a < na; a++) nat += c->nri * c->nlmsa[a];
+  const int nout = c->irregular ? 0 : nl;  // no scattered wave in the irregular solution
+  const int nlig = npr + nat + nout;
   c->nligne = nlig;
-  c->nso = nlig - nl;
+  c->nso = nlig - nout;
   c->row.assign(np, -1);
   for (int i = 0; i < np; i++)
     if (!c->numia[i]) c->row[i] = nw[i] - 1;
@@ -223,11 +229,11 @@ int build_atoms(fdmgpu_cluster *c) {
   for (int i = 0; i < np; i++)
     if (c->row[i] >= 0) c->ridx[c->row[i]] = i;
   for (int a = 0; a < na; a++)
-    for (int lm = 0; lm < c->nlmsa[a]; lm++) {
+    for (int lm = 0; lm < c->nri * c->nlmsa[a]; lm++) {  // lm >= nlmsa: the irregular block's rows
       c->rkind[c->ianew[a] + lm] = 1;
       c->ridx[c->ianew[a] + lm] = (a << 16) | lm;
     }
-  for (int lm = 0; lm < nl; lm++) {
+  for (int lm = 0; lm < nout; lm++) {
     c->rkind[c->nso + lm] = 2;
     c->ridx[c->nso + lm] = lm;
   }
@@ -246,7 +252,7 @@ int build_atoms(fdmgpu_cluster *c) {
       } else if (c->numia[j]) {
         const int a = c->numia[j] - 1;
         l1 = std::min(l1, c->ianew[a] + 1);
-        l2 = std::max(l2, c->ianew[a] + c->nlmsa[a]);
+        l2 = std::max(l2, c->ianew[a] + c->nri * c->nlmsa[a]);
       } else {
         l1 = std::min(l1, c->row[j] + 1);
         l2 = std::max(l2, c->row[j] + 1);
@@ -254,21 +260,23 @@ int build_atoms(fdmgpu_cluster *c) {
     }
   }
   for (int a = 0; a < na; a++) {
-    int l1 = c->ianew[a] + 1, l2 = c->ianew[a] + c->nlmsa[a];
+    int l1 = c->ianew[a] + 1, l2 = c->ianew[a] + c->nri * c->nlmsa[a];
     for (int b = 0; b < c->nbordf[a]; b++) {
       l1 = std::min(l1, c->row[c->abord[a][b]] + 1);
       l2 = std::max(l2, c->row[c->abord[a][b]] + 1);
     }
-    for (int lm = 0; lm < c->nlmsa[a]; lm++) {
+    for (int lm = 0; lm < c->nri * c->nlmsa[a]; lm++) {
       c->lb1[c->ianew[a] + lm] = l1;
       c->lb2[c->ianew[a] + lm] = l2;
     }
   }
-  int lo = c->nso + 1;
-  for (int b = 0; b < c->nsortf; b++) lo = std::min(lo, c->row[c->bnd[b]] + 1);
-  for (int r = c->nso + 1; r <= nlig; r++) {
-    c->lb1[r - 1] = lo;
-    c->lb2[r - 1] = nlig;
+  if (!c->irregular) {
+    int lo = c->nso + 1;
+    for (int b = 0; b < c->nsortf; b++) lo = std::min(lo, c->row[c->bnd[b]] + 1);
+    for (int r = c->nso + 1; r <= nlig; r++) {
+      c->lb1[r - 1] = lo;
+      c->lb2[r - 1] = nlig;
+    }
   }
   for (int r = nlig - 1; r >= 1; r--) c->lb1[r - 1] = std::min(c->lb1[r - 1], c->lb1[r]);
   // compressed rows, columns ascending: grid rows (diagonal, interstitial
@@ -289,17 +297,20 @@ int build_atoms(fdmgpu_cluster *c) {
           outside = true;
         } else if (c->numia[j]) {
           const int a = c->numia[j] - 1;
-          for (int lm = 0; lm < c->nlmsa[a]; lm++) tmp.push_back(c->ianew[a] + lm + 1);
+          for (int lm = 0; lm < c->nri * c->nlmsa[a]; lm++) tmp.push_back(c->ianew[a] + lm + 1);
         } else {
           tmp.push_back(c->row[j] + 1);
         }
       }
       if (outside)
-        for (int l = 0; l < nl; l++) tmp.push_back(c->nso + l + 1);
+        for (int l = 0; l < nout; l++) tmp.push_back(c->nso + l + 1);
     } else if (c->rkind[r] == 1) {
       const int a = c->ridx[r] >> 16;
+      // every outside border point, as mat_solve's symbolic pass marks them
+      // (mat_solve_gaussian.f90:123-137); in the irregular solution each row
+      // block fills only its half of them (mat.f90:1553-1558)
       for (int b = 0; b < c->nbordf[a]; b++) tmp.push_back(c->row[c->abord[a][b]] + 1);
-      for (int lm = 0; lm < c->nlmsa[a]; lm++) tmp.push_back(c->ianew[a] + lm + 1);
+      for (int k = 0; k < c->nri * c->nlmsa[a]; k++) tmp.push_back(c->ianew[a] + k + 1);
     } else {
       for (int b = 0; b < c->nsortf; b++) tmp.push_back(c->row[c->bnd[b]] + 1);
       for (int k = 0; k < nl; k++) tmp.push_back(c->nso + k + 1);
@@ -648,8 +659,13 @@ int fdmgpu_cluster_create(double rsort, double adimp, int lmaxso, fdmgpu_cluster
 
 int fdmgpu_cluster_create_atoms(double rsort, double adimp, int lmaxso, int natome, const double *posi,
                                 const double *rmt, const int *lmax, fdmgpu_cluster **out) {
+  return fdmgpu_cluster_create_atoms_ri(rsort, adimp, lmaxso, natome, posi, rmt, lmax, 0, out);
+}
+
+int fdmgpu_cluster_create_atoms_ri(double rsort, double adimp, int lmaxso, int natome, const double *posi,
+                                   const double *rmt, const int *lmax, int irregular, fdmgpu_cluster **out) {
   if (!out || !(rsort > 0) || !(adimp > 0) || lmaxso < 0 || lmaxso > 40 || natome < 1 || natome > 32767 || !posi ||
-      !rmt || !lmax)
+      !rmt || !lmax || irregular < 0 || irregular > 1)
     return -1;
   double prev = 0;
   for (int a = 0; a < natome; a++) {
@@ -666,6 +682,8 @@ int fdmgpu_cluster_create_atoms(double rsort, double adimp, int lmaxso, int nato
   c->irrep = -1;
   c->mirrors = 0;
   c->natome = natome;
+  c->irregular = irregular != 0;
+  c->nri = c->irregular ? 2 : 1;
   c->apos.assign(posi, posi + 3 * natome);
   c->armt.assign(rmt, rmt + natome);
   c->almax.assign(lmax, lmax + natome);
@@ -844,8 +862,22 @@ int fdmgpu_cluster_assemble_atoms(const fdmgpu_cluster *c, const double *V, doub
                                   double Ecinetic_out, int E_comp, int basereel, const double *phi_re,
                                   const double *phi_im, int64_t *row_ptr, int *col, double *re, double *im,
                                   double *rhs_re, double *rhs_im) {
+  return fdmgpu_cluster_assemble_atoms_ri(c, V, Enervide, Eimag, Ecinetic_out, E_comp, basereel, phi_re, phi_im,
+                                          nullptr, nullptr, row_ptr, col, re, im, rhs_re, rhs_im);
+}
+
+int fdmgpu_cluster_assemble_atoms_ri(const fdmgpu_cluster *c, const double *V, double Enervide, double Eimag,
+                                     double Ecinetic_out, int E_comp, int basereel, const double *phi_re,
+                                     const double *phi_im, const double *phi2_re, const double *phi2_im,
+                                     int64_t *row_ptr, int *col, double *re, double *im, double *rhs_re,
+                                     double *rhs_im) {
   const bool cal = E_comp || !basereel;
   if (!c || !V || !phi_re || !row_ptr || !col || !re || (cal && !im) || (phi_im && !cal)) return -1;
+  // the irregular solution exists with complex energy only (FDM_comp), and
+  // needs the irregular atomic solutions; a regular cluster takes none
+  if (c->natome == 0 || (c->irregular && (!E_comp || basereel || !phi2_re || (phi_im && !phi2_im))) ||
+      (!c->irregular && phi2_re))
+    return -1;
   const int np = c->npoint, nl = c->nlmso, nb = (int)c->bnd.size(), L = c->lmaxso, nlig = c->nligne;
   std::vector<cd> B((size_t)nb * (L + 1)), N((size_t)nb * (L + 1));
   outer_functions(c, Eimag, Ecinetic_out, E_comp, B.data(), N.data());
@@ -886,9 +918,23 @@ int fdmgpu_cluster_assemble_atoms(const fdmgpu_cluster *c, const double *V, doub
           const int a = c->numia[j] - 1;
           const std::vector<int> &bl = c->abord[a];
           const int ib = (int)(std::find(bl.begin() + c->nbordf[a], bl.end(), j) - bl.begin());
-          const double *pr = phi_re + phoff[a] + (size_t)ib * c->nlmsa[a];
-          const double *pi = phi_im ? phi_im + phoff[a] + (size_t)ib * c->nlmsa[a] : nullptr;
-          for (int lm = 0; lm < c->nlmsa[a]; lm++) add(c->ianew[a] + lm + 1, w * pr[lm], pi ? w * pi[lm] : 0.0);
+          for (int ri = 0; ri < c->nri; ri++) {  // regular, then the irregular atomic solutions (:1266-1307)
+            const double *fr = ri ? phi2_re : phi_re, *fi = ri ? phi2_im : phi_im;
+            const double *pr = fr + phoff[a] + (size_t)ib * c->nlmsa[a];
+            const double *pi = fi ? fi + phoff[a] + (size_t)ib * c->nlmsa[a] : nullptr;
+            for (int lm = 0; lm < c->nlmsa[a]; lm++)
+              add(c->ianew[a] + ri * c->nlmsa[a] + lm + 1, w * pr[lm], pi ? w * pi[lm] : 0.0);
+          }
+        } else if (c->irregular) {  // no scattered wave; incoming -i j + n, :1318-1398
+          const int b = c->slot[j];
+          if (rhs_re)
+            for (int lm = 0; lm < nl; lm++) {
+              const int l = lof(c->lmidx[lm]);
+              const double y = c->ylm[(size_t)b * nl + lm];
+              const cd s = y * (cd(0, -1) * B[(size_t)b * (L + 1) + l] + N[(size_t)b * (L + 1) + l]) * w;
+              rhs_re[(size_t)r * nl + lm] -= s.real();
+              if (rhs_im) rhs_im[(size_t)r * nl + lm] -= s.imag();
+            }
         } else {  // outside the sphere: the outer unknowns and the incoming wave, :1312-1398
           const int b = c->slot[j];
           for (int lm = 0; lm < nl; lm++) {
@@ -905,21 +951,31 @@ int fdmgpu_cluster_assemble_atoms(const fdmgpu_cluster *c, const double *V, doub
         }
       }
     } else if (c->rkind[r] == 1) {  // expansion row Lm of atom a, mat.f90:1515-1632
-      const int a = c->ridx[r] >> 16, lm = c->ridx[r] & 0xffff, na = c->nlmsa[a];
+      const int a = c->ridx[r] >> 16, na = c->nlmsa[a];
+      const int ri = (c->ridx[r] & 0xffff) >= na;  // the irregular solution's second row block
+      const int lm = (c->ridx[r] & 0xffff) - ri * na;
+      const int nh = c->nbordf[a] / 2;  // irregular: the blocks match on the two halves of the border
       const std::vector<double> &pw = c->apoids[a], &Y = c->aylm[a];
-      for (int b = 0; b < c->nbordf[a]; b++) add(c->row[c->abord[a][b]] + 1, -(pw[b] * Y[(size_t)b * na + lm]), 0.0);
-      for (int lp = 0; lp < na; lp++) {
-        double sr = 0, si = 0;
-        for (int b = 0; b < c->nbordf[a]; b++) {
-          const double y = Y[(size_t)b * na + lm], pr = phi_re[phoff[a] + (size_t)b * na + lp];
-          if (cal) {
-            sr += pw[b] * (pr * y);
-            if (phi_im) si += pw[b] * (phi_im[phoff[a] + (size_t)b * na + lp] * y);
-          } else {
-            sr += pw[b] * y * pr;
+      for (int b = 0; b < c->nbordf[a]; b++) {
+        if (c->irregular && (ri ? b < nh : b >= nh)) continue;
+        add(c->row[c->abord[a][b]] + 1, -(pw[b] * Y[(size_t)b * na + lm]), 0.0);
+      }
+      for (int rj = 0; rj < c->nri; rj++) {  // :1580-1630, the columns of the regular then irregular block
+        const double *fr = rj ? phi2_re : phi_re, *fi = rj ? phi2_im : phi_im;
+        const int b0 = c->irregular && rj ? nh : 0, b1 = c->irregular && !rj ? nh : c->nbordf[a];
+        for (int lp = 0; lp < na; lp++) {
+          double sr = 0, si = 0;
+          for (int b = b0; b < b1; b++) {
+            const double y = Y[(size_t)b * na + lm], pr = fr[phoff[a] + (size_t)b * na + lp];
+            if (cal) {
+              sr += pw[b] * (pr * y);
+              if (fi) si += pw[b] * (fi[phoff[a] + (size_t)b * na + lp] * y);
+            } else {
+              sr += pw[b] * y * pr;
+            }
           }
+          add(c->ianew[a] + rj * na + lp + 1, sr, si);
         }
-        add(c->ianew[a] + lp + 1, sr, si);
       }
     } else {  // outer row lm, mat.f90:1401-1511
       const int lm = c->ridx[r];
@@ -1025,6 +1081,7 @@ int fdmgpu_cluster_assembly_weights(const fdmgpu_cluster *c, int *lharm, double 
 }
 
 int fdmgpu_cluster_atom_assembly_sizes(const fdmgpu_cluster *c, int64_t *sizes) {
+  if (c->irregular) return -2;  // the device tables describe regular solutions
   int64_t nar = 0, nal = 0, nphi = 0, nrow = 0, nbf = 0, nyl = 0;
   for (int i = 0; i < c->npoint; i++) {
     if (c->numia[i]) continue;
@@ -1057,6 +1114,7 @@ int fdmgpu_cluster_atom_assembly_tables(const fdmgpu_cluster *c, int64_t *aq, in
                                         double *alw, int *ianew, int *nlmsa, int *nbordf, int64_t *phoff,
                                         int64_t *bq, double *poidsa, double *ylmato) {
   if (c->natome < 1) return -1;
+  if (c->irregular) return -2;
   std::vector<int64_t> off(c->natome + 1, 0);
   for (int a = 0; a < c->natome; a++) off[a + 1] = off[a] + (int64_t)c->abord[a].size() * c->nlmsa[a];
   int n = 0, k = 0;

@@ -229,7 +229,99 @@ __global__ __launch_bounds__(TAU_NT) void k_tau(TauJob J, int n, int nso, int ba
 #undef MM
 }
 
+// Tau_RI (mat.f90:229-295, complex energy, FDM_comp, Test_reg = 0): one
+// workgroup per (point, atom); entry (i_RI, Lm1, Lm2) of the atom's block is
+// Tau_RI += sum_f Ampl(f, Lm1) Ampl_I(f, (i_RI-1) na + Lm2), each product
+// added straight into it in f order as :286-288 does, with Ampl the regular
+// solution's atom rows rounded to single precision (the reference keeps the
+// regular solution as cmplx(smr, smi) of default kind, :211) and Ampl_I the
+// irregular solution's (its first n_ato_RI * na rows of the atom's blocks).
+__global__ __launch_bounds__(TAU_NT) void k_tau_ri(TauRIJob J, int n, int n_ato_ri, int n_ri, int accumulate) {
+  const int p = blockIdx.x, at = blockIdx.y, na = J.na[at];
+  const double *xr = J.xr[p], *xi = J.xi[p];
+  double *t = J.tau[p] + 2 * J.off[at];
+  for (int e = threadIdx.x; e < n_ri * na * na; e += TAU_NT) {
+    const int i_ri = e / (na * na), l1 = (e / na) % na, l2 = e % na;
+    const int lm = i_ri * na + l2;  // 0-based column of Ampl_I
+    double tr = accumulate ? t[2 * e] : 0.0, ti = accumulate ? t[2 * e + 1] : 0.0;
+    if (lm < n_ato_ri * na) {
+      const size_t a = (size_t)(J.row_reg[at] + l1) * n, b = (size_t)(J.row_irr[at] + lm) * n;
+      for (int f = 0; f < n; f++) {
+        const double ar = (double)(float)xr[2 * (a + f)], ai = (double)(float)xr[2 * (a + f) + 1];
+        const double br = xi[2 * (b + f)], bi = xi[2 * (b + f) + 1];
+        tr = tr + (ar * br - ai * bi);
+        ti = ti + (ar * bi + ai * br);
+      }
+    }
+    t[2 * e] = tr;
+    t[2 * e + 1] = ti;
+  }
+}
+
+// The symmetrisation at the end of mat (mat.f90:596-641, one spin, no
+// spin-orbit) of each atom's nlmsa^2 block, harmonics in (L, m) order: with
+// complex harmonics (Ylm_comp) the reference visits (Lm1, Lm2) in order and
+// sets it to c = 0.5 (T(Lm1, Lm2) + s T(P)) and its partner P = (L2 -m2,
+// L1 -m1) to s c, s = (-1)^(m1+m2); a later visit of the partner finds the
+// pair consistent and rewrites the same values, so every entry follows
+// from the block before the sweep: the earlier of an entry and its partner
+// in that order supplies c.  With real harmonics the reference keeps
+// i Im(T).  One workgroup per (point, atom), the block staged in LDS.
+__global__ __launch_bounds__(TAU_NT) void k_tau_sym(TauSymJob J, int ylm_comp) {
+  extern __shared__ cz blk[];
+  const int p = blockIdx.x, at = blockIdx.y, na = J.na[at];
+  double *t = J.tau[p] + 2 * J.off[at];
+  for (int e = threadIdx.x; e < na * na; e += TAU_NT) blk[e] = cz{t[2 * e], t[2 * e + 1]};
+  __syncthreads();
+  auto lm_of = [](int k, int *m) {  // 0-based (L, m)-ordered index -> m
+    int L = (int)sqrt((double)k);
+    while (L * L > k) L--;
+    while ((L + 1) * (L + 1) <= k) L++;
+    *m = k - L * L - L;
+  };
+  for (int e = threadIdx.x; e < na * na; e += TAU_NT) {
+    const int a = e / na, b = e % na;
+    cz v;
+    if (ylm_comp) {
+      int m1, m2;
+      lm_of(a, &m1);
+      lm_of(b, &m2);
+      const int pa = b - 2 * m2, pb = a - 2 * m1, pe = pa * na + pb;
+      const double s = ((m1 + m2) & 1) ? -1.0 : 1.0;
+      if (e < pe) {
+        v = cz{0.5 * (blk[e].re + s * blk[pe].re), 0.5 * (blk[e].im + s * blk[pe].im)};
+      } else if (e > pe) {  // the partner came first: c = 0.5 (T(P) + s T(e)), this entry s c
+        const cz c = {0.5 * (blk[pe].re + s * blk[e].re), 0.5 * (blk[pe].im + s * blk[e].im)};
+        v = cz{s * c.re, s * c.im};
+      } else {  // its own partner: c then s c written to the same place
+        const cz c = {0.5 * (blk[e].re + s * blk[e].re), 0.5 * (blk[e].im + s * blk[e].im)};
+        v = cz{s * c.re, s * c.im};
+      }
+    } else {
+      v = cz{0.0, blk[e].im};
+    }
+    t[2 * e] = v.re;
+    t[2 * e + 1] = v.im;
+  }
+}
+
 }  // namespace
+
+int tau_ri_launch(const TauRIJob &J, int npts, int natoms, int n, int n_ato_ri, int n_ri, bool accumulate,
+                  hipStream_t s) {
+  hipLaunchKernelGGL(k_tau_ri, dim3(npts, natoms), dim3(TAU_NT), 0, s, J, n, n_ato_ri, n_ri, (int)accumulate);
+  FDM_HIP(hipGetLastError());
+  return 0;
+}
+
+int tau_sym_launch(const TauSymJob &J, int npts, int natoms, int na_max, bool ylm_comp, hipStream_t s) {
+  const size_t lds = sizeof(cz) * (size_t)na_max * na_max;
+  if (lds > 64 * 1024)
+    FDM_HIP(hipFuncSetAttribute((const void *)k_tau_sym, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(k_tau_sym, dim3(npts, natoms), dim3(TAU_NT), lds, s, J, (int)ylm_comp);
+  FDM_HIP(hipGetLastError());
+  return 0;
+}
 
 size_t tau_lds_bytes(int n, int na_max) {
   return sizeof(cz) * ((size_t)n * n + n + (size_t)na_max * n) + sizeof(int) * n;

@@ -23,12 +23,29 @@ class TauDesc(C.Structure):
                 ("ianew", C.POINTER(C.c_int)), ("nlmsa", C.POINTER(C.c_int))]
 
 
+class TauRIDesc(C.Structure):
+    _fields_ = [("nlmso", C.c_int), ("natome", C.c_int), ("ianew_reg", C.POINTER(C.c_int)),
+                ("ianew_irr", C.POINTER(C.c_int)), ("nlmsa", C.POINTER(C.c_int)), ("nligne_reg", C.c_int),
+                ("nligne_irr", C.c_int), ("n_ato_ri", C.c_int), ("n_ri", C.c_int)]
+
+
 def _sigs(L):
     if getattr(L, "_tau_sigs", False):
         return
     P = C.POINTER
     L.fdmgpu_tau_reduce.argtypes = [P(TauDesc), C.c_int, P(_V), P(_V), P(_V), C.c_int, _V, _V]
+    L.fdmgpu_tau_ri.argtypes = [P(TauRIDesc), C.c_int, P(_V), P(_V), P(_V), C.c_int, _V]
+    L.fdmgpu_tau_symmetrize.argtypes = [C.c_int, P(C.c_int), C.c_int, C.c_int, P(_V), _V]
     L._tau_sigs = True
+
+
+def _lib_stream(stream):
+    global _stream
+    if stream is not None:
+        return library_stream(stream)
+    if _stream is None:  # the module's own stream for calls that name none
+        _stream = library_stream(None)
+    return _stream
 
 
 def _dptrs(ts):
@@ -56,7 +73,6 @@ def tau_reduce(xs, nligneso: int, basereel: bool, fdm_comp: bool, atoms, tau=Non
     when ``want_norm`` and basereel, and the int32 status tensor.  With
     ``check`` a zero pivot of the normalisation raises FdmGpuError (the
     reference stops there)."""
-    global _stream
     L = _lib.lib()
     _sigs(L)
     if not xs:
@@ -80,12 +96,7 @@ def tau_reduce(xs, nligneso: int, basereel: bool, fdm_comp: bool, atoms, tau=Non
     nlmsa = (C.c_int * max(len(atoms), 1))(*[int(k) for _, k in atoms])
     d = TauDesc(int(nligne), int(nlmso), int(nligneso), int(cal), int(bool(basereel)), int(bool(fdm_comp)),
                 len(atoms), ianew, nlmsa)
-    if stream is not None:
-        s = library_stream(stream)
-    else:
-        if _stream is None:  # the module's own stream for calls that name none
-            _stream = library_stream(None)
-        s = _stream
+    s = _lib_stream(stream)
     with Ordered(s):
         _lib.check(L.fdmgpu_tau_reduce(C.byref(d), len(xs), _dptrs([as_real(x) if cal else x for x in xs]),
                                        _dptrs([as_real(t) for t in tau]),
@@ -105,3 +116,56 @@ def split(t: torch.Tensor, atoms):
         out.append(t[off:off + na * na].view(na, na))
         off += na * na
     return out
+
+
+def tau_ri(x_reg, x_irr, atoms_reg, atoms_irr, tau=None, accumulate: bool = False, n_ato_ri: int = 2,
+           n_ri: int = 2, stream: torch.cuda.Stream | None = None):
+    """Tau_RI (mat.f90:229-295, FDM_comp) of energy points from their regular
+    and irregular cluster solutions (lists of complex128 (nligne, nlmso)
+    device tensors; atoms_reg / atoms_irr: [(first_row, nlmsa), ...] of each
+    atom in the two systems).  Returns per point a flat complex128 tensor,
+    atoms concatenated, each block [i_RI][Lm1][Lm2] (n_ri x nlmsa x nlmsa),
+    added to ``tau`` when ``accumulate``."""
+    L = _lib.lib()
+    _sigs(L)
+    if not x_reg:
+        return []
+    if len(x_irr) != len(x_reg) or len(atoms_irr) != len(atoms_reg):
+        raise ValueError("one irregular solution per regular one, the same atoms")
+    if any(a[1] != b[1] for a, b in zip(atoms_reg, atoms_irr)):
+        raise ValueError("the atoms' nlmsa differ between the two systems")
+    for x in list(x_reg) + list(x_irr):
+        if x.dtype != torch.complex128 or x.dim() != 2 or x.shape[1] != x_reg[0].shape[1]:
+            raise ValueError("solutions: complex128 (nligne, nlmso), one nlmso")
+    tot = sum(n_ri * int(na) * int(na) for _, na in atoms_reg)
+    dev = x_reg[0].device
+    if tau is None:
+        tau = [torch.zeros(max(tot, 1), dtype=torch.complex128, device=dev) for _ in x_reg]
+    elif len(tau) != len(x_reg) or any(t.dtype != torch.complex128 or t.numel() < tot for t in tau):
+        raise ValueError(f"Tau_RI buffers: complex128, {tot} entries each")
+    na = max(len(atoms_reg), 1)
+    d = TauRIDesc(int(x_reg[0].shape[1]), len(atoms_reg), (C.c_int * na)(*[int(a) for a, _ in atoms_reg]),
+                  (C.c_int * na)(*[int(a) for a, _ in atoms_irr]), (C.c_int * na)(*[int(k) for _, k in atoms_reg]),
+                  int(x_reg[0].shape[0]), int(x_irr[0].shape[0]), int(n_ato_ri), int(n_ri))
+    s = _lib_stream(stream)
+    with Ordered(s):
+        _lib.check(L.fdmgpu_tau_ri(C.byref(d), len(x_reg), _dptrs([as_real(x) for x in x_reg]),
+                                   _dptrs([as_real(x) for x in x_irr]), _dptrs([as_real(t) for t in tau]),
+                                   int(bool(accumulate)), _V(s.cuda_stream)))
+    return tau
+
+
+def symmetrize(tau, nlmsa, ylm_comp: bool = False, stream: torch.cuda.Stream | None = None):
+    """The symmetrisation ending mat (mat.f90:596-641) in place on tau
+    buffers as tau_reduce writes them (per atom nlmsa^2, (L, m) order)."""
+    L = _lib.lib()
+    _sigs(L)
+    if not tau:
+        return tau
+    na = max(len(nlmsa), 1)
+    s = _lib_stream(stream)
+    with Ordered(s):
+        _lib.check(L.fdmgpu_tau_symmetrize(len(nlmsa), (C.c_int * na)(*[int(k) for k in nlmsa]),
+                                           int(bool(ylm_comp)), len(tau), _dptrs([as_real(t) for t in tau]),
+                                           _V(s.cuda_stream)))
+    return tau

@@ -337,6 +337,43 @@ typedef struct {
 int fdmgpu_tau_reduce(const fdmgpu_tau_desc *d, int count, const double *const *d_x, double *const *d_tau,
                       double *const *d_norm, int accumulate, int *d_info, void *hip_stream);
 
+/*
+ * Tau_RI with complex energy and FDM_comp (mat.f90:229-295, Test_reg = 0,
+ * one spin, one representation), from the regular cluster solution and the
+ * irregular one (the system of fdmgpu_cluster_create_atoms_ri), both complex
+ * and already on the device.  Replaces mat's loop after its two mat_solve
+ * calls: per atom, Tau_RI(Lm1, Lm2, i_RI) += sum_Lmf Ampl(Lmf, Lm1)
+ * Ampl_I(Lmf, (i_RI-1) nlmsa + Lm2), each product added in Lmf order as
+ * the reference adds it, with Ampl the regular solution's atom rows rounded
+ * to single precision (the reference keeps the regular solution as
+ * cmplx(smr, smi) of default kind, mat.f90:211) and Ampl_I the irregular
+ * solution's atom rows (the first n_ato_ri * nlmsa of the atom's blocks).
+ */
+typedef struct {
+  int nlmso;
+  int natome;
+  const int *ianew_reg;  /* host [natome]: 0-based first row of each atom in the regular solution */
+  const int *ianew_irr;  /* host [natome]: the same in the irregular solution */
+  const int *nlmsa;      /* host [natome] */
+  int nligne_reg, nligne_irr;
+  int n_ato_ri;          /* atom blocks of the irregular system per atom: 2 (Test_reg = 0) */
+  int n_ri;              /* 2 */
+} fdmgpu_tau_ri_desc;
+/* d_xreg[k], d_xirr[k]: (nligne, nlmso) interleaved complex; d_tau_ri[k]:
+ * per atom n_ri * nlmsa^2 complex entries [i_RI][Lm1][Lm2], atoms
+ * concatenated, added to when accumulate != 0.  Asynchronous on hip_stream
+ * (NULL = library stream). */
+int fdmgpu_tau_ri(const fdmgpu_tau_ri_desc *d, int count, const double *const *d_xreg, const double *const *d_xirr,
+                  double *const *d_tau_ri, int accumulate, void *hip_stream);
+/* The symmetrisation that ends mat (mat.f90:596-641, one spin, no
+ * spin-orbit, not with FDM_comp), in place on the per-atom blocks written
+ * by fdmgpu_tau_reduce: harmonics in (L, m) order (nlmsa = (lmax+1)^2);
+ * ylm_comp != 0: each pair and its partner (L2 -m2, L1 -m1) become
+ * c = 0.5 (T + (-1)^(m1+m2) T_partner) and (-1)^(m1+m2) c, as the
+ * reference's sweep leaves them; else i Im(T) (real harmonics). */
+int fdmgpu_tau_symmetrize(int natome, const int *nlmsa, int ylm_comp, int count, double *const *d_tau,
+                          void *hip_stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -84,6 +84,17 @@ int fdmgpu_cluster_create_group(double rsort, double adimp, int lmaxso, int grou
  * point. */
 int fdmgpu_cluster_create_atoms(double rsort, double adimp, int lmaxso, int natome, const double *posi,
                                 const double *rmt, const int *lmax, fdmgpu_cluster **out);
+/* The same cluster's rows for the irregular cluster solution (irregular = 1;
+ * 0 gives fdmgpu_cluster_create_atoms), as mat (mat.f90:135-217) builds them
+ * for FDM_comp with Test_reg = 0: newind (:1761-1921) with n_ato_RI = 2, so
+ * each atom holds two blocks of nlmsa rows (the regular then the irregular
+ * atomic solution's amplitudes, rows ianew .. ianew + 2 nlmsa - 1), and no
+ * outer-sphere rows (nligne = npr + 2 sum nlmsa); a grid row with an
+ * outside neighbour reaches to nligne in the profile, as newind sets it.
+ * Assemble with fdmgpu_cluster_assemble_atoms_ri; the device assembler's
+ * tables return -2 for it. */
+int fdmgpu_cluster_create_atoms_ri(double rsort, double adimp, int lmaxso, int natome, const double *posi,
+                                   const double *rmt, const int *lmax, int irregular, fdmgpu_cluster **out);
 /* sizes[0..3] = natome, npr (interstitial points), sum of nbord, and the
  * length of phiato (sum of nbord * nlmsa) */
 int fdmgpu_cluster_atom_sizes(const fdmgpu_cluster *c, int64_t *sizes);
@@ -130,6 +141,22 @@ int fdmgpu_cluster_assemble_atoms(const fdmgpu_cluster *c, const double *V, doub
                                   double Ecinetic_out, int E_comp, int basereel, const double *phi_re,
                                   const double *phi_im, int64_t *row_ptr, int *col, double *re, double *im,
                                   double *rhs_re, double *rhs_im);
+/* The same with the irregular atomic solutions phi2 (same layout as phi),
+ * for a cluster made with irregular = 1 (calcMatRow with irregular,
+ * n_ato_RI = 2, Test_reg = 0): grid rows couple to both blocks of an atom
+ * (:1266-1307); an outside neighbour adds no column and -(-i j_l + n_l) Ylm
+ * times the stencil weight to the right-hand side (:1318-1398); atom rows of
+ * the first block match the grid on the first nbordf/2 border points, those
+ * of the second on the rest, and both hold sum_ib poidsa Ylm phi over the
+ * first half in the regular block's columns and phi2 over the second half
+ * in the irregular block's (:1515-1630).  Complex energy only (E_comp = 1,
+ * basereel = 0).  A regular cluster takes phi2 = NULL (then it is
+ * fdmgpu_cluster_assemble_atoms). */
+int fdmgpu_cluster_assemble_atoms_ri(const fdmgpu_cluster *c, const double *V, double Enervide, double Eimag,
+                                     double Ecinetic_out, int E_comp, int basereel, const double *phi_re,
+                                     const double *phi_im, const double *phi2_re, const double *phi2_im,
+                                     int64_t *row_ptr, int *col, double *re, double *im, double *rhs_re,
+                                     double *rhs_im);
 
 /* E-independent tables of one structure's per-energy assembly: the parts of
  * calcMatRow (mat.f90:1142-1232 grid rows, :1312-1398 outer columns,

@@ -1619,7 +1619,9 @@ void orc_tau_ri(int nlmso, const double *xr_re, const double *xr_im, const int *
       for (int i_ri = 1; i_ri <= n_ri; i_ri++)
         for (int lm2 = 1; lm2 <= na; lm2++) {
           lm++;
-          double complex acc = 0;
+          const size_t q = off + ((size_t)(i_ri - 1) * na + (lm1 - 1)) * na + (lm2 - 1);
+          /* each product goes straight into Tau_RI, as :286-288 adds it */
+          double complex t = CMPLX(out_re[q], out_im[q]);
           if (lm <= n_ato_ri * na) {
             const size_t jj = (size_t)(ianew_irr[ia] + lm - 1) * nlmso;
             for (int f = 0; f < nlmso; f++) {
@@ -1627,12 +1629,12 @@ void orc_tau_ri(int nlmso, const double *xr_re, const double *xr_im, const int *
                * of default kind: each part rounded to single precision */
               const double ar = round_regular ? (double)(float)xr_re[ii + f] : xr_re[ii + f];
               const double ai = round_regular ? (double)(float)xr_im[ii + f] : xr_im[ii + f];
-              acc = acc + CMPLX(ar, ai) * CMPLX(xi_re[jj + f], xi_im[jj + f]);
+              const double br = xi_re[jj + f], bi = xi_im[jj + f];
+              t = CMPLX(creal(t) + (ar * br - ai * bi), cimag(t) + (ar * bi + ai * br));
             }
           }
-          const size_t q = off + ((size_t)(i_ri - 1) * na + (lm1 - 1)) * na + (lm2 - 1);
-          out_re[q] += creal(acc);
-          out_im[q] += cimag(acc);
+          out_re[q] = creal(t);
+          out_im[q] = cimag(t);
         }
     }
     off += (size_t)n_ri * na * na;

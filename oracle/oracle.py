@@ -381,18 +381,23 @@ def tau_sym(taus, ylm_comp: bool = False):
     return out
 
 
-def tau_ri(x_reg, x_irr, ianew_reg, ianew_irr, nlmsa, n_ato_ri: int = 2, n_ri: int = 2, round_regular=True):
+def tau_ri(x_reg, x_irr, ianew_reg, ianew_irr, nlmsa, n_ato_ri: int = 2, n_ri: int = 2, round_regular=True,
+           tau0=None):
     """Tau_RI of mat.f90:229-295 from the regular and irregular cluster
     solutions (complex (nligne, nlmso) each; ianew 0-based first rows of the
     atom blocks).  Returns per atom (n_ri, na, na) complex arrays.  With
     round_regular (the reference's arithmetic) the regular solution enters
-    rounded to single precision, as mat.f90:211 stores it."""
+    rounded to single precision, as mat.f90:211 stores it.  tau0 (the same
+    structure) is added to, as Tau_RI accumulates."""
     xr = np.asarray(x_reg, np.complex128)
     xi = np.asarray(x_irr, np.complex128)
     nl = xr.shape[1]
     nlmsa = np.ascontiguousarray(nlmsa, np.int32)
     tot = int(sum(n_ri * int(k) * int(k) for k in nlmsa))
     o_r, o_i = np.zeros(tot), np.zeros(tot)
+    if tau0 is not None:
+        flat = np.concatenate([np.asarray(t, np.complex128).reshape(-1) for t in tau0])
+        o_r, o_i = np.ascontiguousarray(flat.real), np.ascontiguousarray(flat.imag)
     lib().orc_tau_ri(nl, np.ascontiguousarray(xr.real).reshape(-1), np.ascontiguousarray(xr.imag).reshape(-1),
                      np.ascontiguousarray(ianew_reg, np.int32), np.ascontiguousarray(xi.real).reshape(-1),
                      np.ascontiguousarray(xi.imag).reshape(-1), np.ascontiguousarray(ianew_irr, np.int32),
