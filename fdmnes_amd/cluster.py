@@ -76,16 +76,22 @@ def _L():
 class Cluster:
     """A spherical cluster of radius ``rsort`` and grid step ``adimp``
     (bohr) with ``(lmaxso+1)**2`` outer-sphere unknowns; with ``irrep`` the
-    system of one representation of a mirror group on its wedge (see
-    fdmgpu_cluster_create_group): ``group`` is the reference's point-group
-    number, 8 = mmm (irrep 0..7, wedge x, y, z >= 0, the default), 6 = mm2
-    (irrep 0..3, wedge x, y >= 0), 3 = m (irrep 0 or 4, wedge z >= 0);
-    irrep bit a set = odd under the mirror normal to axis a.  With ``atoms``
+    system of one representation of a subgroup of mmm on its calculation
+    area (see fdmgpu_cluster_create_group): ``group`` is the reference's
+    point-group number, 8 = mmm (irrep 0..7, wedge x, y, z >= 0, the
+    default), 6 = mm2 (irrep 0..3, wedge x, y >= 0), 3 = m (irrep 0 or 4,
+    wedge z >= 0), 2 = -1 (irrep 0, 1), 4 = 2 about z (irrep 0, 1),
+    5 = 2/m (irrep 0, 1, 4, 5), 7 = 222 (irrep 0..3); irrep is a character
+    vector: the operation negating the coordinates in a set of axes has the
+    character (-1)^(number of those axes whose bit is set), and irrep is the
+    smallest such mask of the representation.  With ``atoms``
     = (posi (natome, 3) bohr sorted by distance from the absorber, rmt
     (natome,) bohr, lmax (natome,)) a P1 cluster with muffin-tin atoms
     (fdmgpu_cluster_create_atoms), assembled with :meth:`assemble_atoms`."""
 
-    GROUPS = {1: 0, 3: 4, 6: 3, 8: 7}  # point-group number -> mirrors (bit a = normal to axis a)
+    # point-group number -> the representation masks it takes
+    GROUPS = {1: (), 3: (0, 4), 6: (0, 1, 2, 3), 8: tuple(range(8)), 2: (0, 1), 4: (0, 1), 5: (0, 1, 4, 5),
+              7: (0, 1, 2, 3)}
 
     def __init__(self, rsort: float, adimp: float, lmaxso: int, irrep: int = -1, group: int = 8, atoms=None,
                  irregular: bool = False):

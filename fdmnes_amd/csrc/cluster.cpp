@@ -94,6 +94,18 @@ struct fdmgpu_cluster {
   // on every mirror axis
   int irrep = -1;
   int mirrors = 0;
+  // The point group as axis sign flips (bit a: coordinate a negated): every
+  // group built here is a subgroup of mmm.  group: the reference's number
+  // (general.f90:5207-5220): 1, -1 = 2, m = 3, 2 = 4, 2/m = 5, mm2 = 6,
+  // 222 = 7, mmm = 8.  ops: its operations in the reference's order of the
+  // symmetry operations (opsym: identity 1, two-fold axes x, y, z 22-24,
+  // inversion 25, mirrors normal to x, y, z 40-42), as flip masks.  For the
+  // groups with rotations or inversion the representation bits irrep are a
+  // character vector: the character of the operation with flips g is
+  // (-1)^popcount(g & irrep); irrep is the smallest mask giving the
+  // representation's characters on the group.
+  int group = 1;
+  std::vector<int> ops{0};
   int npoint = 0, npso = 0, nligne = 0;
   std::vector<GridPoint> pt;   // npso, radial order
   std::vector<double> rad;     // npso, |index| * adimp
@@ -146,6 +158,48 @@ int harmonic_irrep(int l, int m) {
 
 // character of the mirrors in mask for representation g: -1 per odd axis flipped
 inline double character(int g, int mask) { return (__builtin_popcount(g & mask) & 1) ? -1.0 : 1.0; }
+
+// The calculation area of each group, posequiv's regions (general.f90:
+// 8950-8978) on integer grid coordinates
+bool in_domain(int group, int x, int y, int z) {
+  switch (group) {
+    case 2: return z > 0 || (z == 0 && (x > 0 || (x == 0 && y >= 0)));  // -1
+    case 3: return z >= 0;                                               // m
+    case 4: return x > 0 || (x == 0 && y >= 0);                          // 2 (axis z)
+    case 5: return z >= 0 && (x > 0 || (x == 0 && y >= 0));              // 2/m
+    case 6: return x >= 0 && y >= 0;                                     // mm2
+    case 7: return (x > 0 && y > 0) || (x == 0 && y >= 0 && z >= 0) || (y == 0 && x >= 0 && z >= 0);  // 222
+    case 8: return x >= 0 && y >= 0 && z >= 0;                           // mmm
+    default: return true;
+  }
+}
+
+// group number -> its operations as flip masks, in opsym order:
+// 1, 22 (2x: y z), 23 (2y: x z), 24 (2z: x y), 25 (-1: x y z), 40 (m_x),
+// 41 (m_y), 42 (m_z)
+std::vector<int> group_ops(int group) {
+  switch (group) {
+    case 1: return {0};
+    case 2: return {0, 7};
+    case 3: return {0, 4};
+    case 4: return {0, 3};
+    case 5: return {0, 3, 7, 4};
+    case 6: return {0, 3, 1, 2};
+    case 7: return {0, 6, 5, 3};
+    case 8: return {0, 6, 5, 3, 7, 1, 2, 4};
+    default: return {};
+  }
+}
+
+inline bool is_mirror_group(int group) { return group == 3 || group == 6 || group == 8; }
+
+// Does the real harmonic with flip parities o belong to representation v of
+// the group: its parity under every operation g equals the character of g
+bool harmonic_in(const std::vector<int> &ops, int o, int v) {
+  for (int g : ops)
+    if (__builtin_popcount(g & (o ^ v)) & 1) return false;
+  return true;
+}
 
 // The atom border lists, harmonics and weights (bordure, general.f90:
 // 9924-10068; Ylmpt :11533-11568), the rows with the atom blocks in the
@@ -338,9 +392,9 @@ int build(fdmgpu_cluster *c) {
       for (int iz = -nx; iz <= nx; iz++) {
         const double d = std::sqrt((double)ix * ix + (double)iy * iy + (double)iz * iz);
         if (d > rmax) continue;
-        // with mirror symmetry only the wedge (posequiv's regions of m, mm2
-        // and mmm, general.f90:8961-8978) is kept (reseau, general.f90:9512)
-        if (((mx & 1) && ix < 0) || ((mx & 2) && iy < 0) || ((mx & 4) && iz < 0)) continue;
+        // with symmetry only the calculation area (posequiv's region of the
+        // group, general.f90:8950-8978) is kept (reseau, general.f90:9512)
+        if (sym && !in_domain(c->group, ix, iy, iz)) continue;
         c->pt.push_back({ix, iy, iz});
         c->rad.push_back(c->adimp * d);
       }
@@ -364,10 +418,20 @@ int build(fdmgpu_cluster *c) {
   // a wedge point stands for, over the largest such number
   c->rvol.assign(n, 1.0);
   if (sym) {
+    // the size of the point's orbit under the group (every image of an
+    // area point lies in the grid's symmetric sphere)
     double rvmax = 1.0;
     for (int i = 0; i < n; i++) {
-      c->rvol[i] = (double)(1 << (((mx & 1) && p[i].ix != 0) + ((mx & 2) && p[i].iy != 0) +
-                                  ((mx & 4) && p[i].iz != 0)));
+      int img[8], ni = 0;
+      for (int g : c->ops) {
+        const int x = (g & 1) ? -p[i].ix : p[i].ix, y = (g & 2) ? -p[i].iy : p[i].iy,
+                  z = (g & 4) ? -p[i].iz : p[i].iz;
+        const int k = ((x + 64) << 14) | ((y + 64) << 7) | (z + 64);
+        bool seen = false;
+        for (int t = 0; t < ni; t++) seen |= img[t] == k;
+        if (!seen) img[ni++] = k;
+      }
+      c->rvol[i] = (double)ni;
       rvmax = std::max(rvmax, c->rvol[i]);
     }
     for (int i = 0; i < n; i++) c->rvol[i] /= rvmax;
@@ -386,7 +450,16 @@ int build(fdmgpu_cluster *c) {
       int x = p[i].ix + kDir[v][0], y = p[i].iy + kDir[v][1], z = p[i].iz + kDir[v][2];
       if (!(std::abs(x) <= nx && std::abs(y) <= nx && std::abs(z) <= nx)) continue;
       if (sym) {
-        const int f = ((x < 0) | ((y < 0) << 1) | ((z < 0) << 2)) & mx;
+        int f = 0;
+        if (is_mirror_group(c->group)) {  // the mirrors of the negative coordinates
+          f = ((x < 0) | ((y < 0) << 1) | ((z < 0) << 2)) & mx;
+        } else {  // the first operation, in opsym order, that takes it into the area (posequiv)
+          for (int g : c->ops)
+            if (in_domain(c->group, (g & 1) ? -x : x, (g & 2) ? -y : y, (g & 4) ? -z : z)) {
+              f = g;
+              break;
+            }
+        }
         c->flip[(size_t)i * 12 + v] = (unsigned char)f;
         if (f & 1) x = -x;
         if (f & 2) y = -y;
@@ -445,7 +518,7 @@ int build(fdmgpu_cluster *c) {
   c->lmidx.clear();
   for (int l = 0; l <= c->lmaxso; l++)
     for (int m = -l; m <= l; m++)
-      if (!sym || (harmonic_irrep(l, m) & mx) == c->irrep) c->lmidx.push_back(l * l + l + m);
+      if (!sym || harmonic_in(c->ops, harmonic_irrep(l, m), c->irrep)) c->lmidx.push_back(l * l + l + m);
   c->nlmso = (int)c->lmidx.size();
   if (c->nlmso == 0) return -1;
   c->ylm.assign(c->bnd.size() * c->nlmso, 0.0);
@@ -624,23 +697,38 @@ int fdmgpu_cluster_create_group(double rsort, double adimp, int lmaxso, int grou
   // the reference's point-group numbers (general.f90:5207-5220) of the
   // mirror groups, in posequiv's orientation: m's mirror is normal to z and
   // mm2's two mirrors are normal to x and y (general.f90:8961-8978)
+  // The other subgroups of mmm (-1 = 2, 2 = 4 about z, 2/m = 5, 222 = 7)
+  // have rotations or the inversion, with posequiv's areas: their
+  // representations are again one-dimensional with characters +-1.
   int mirrors;
   switch (group) {
     case 1: mirrors = 0; break;
     case 3: mirrors = 4; break;
     case 6: mirrors = 3; break;
     case 8: mirrors = 7; break;
+    case 2: case 4: case 5: case 7: mirrors = 7; break;  // irrep: a character vector over the axes
     default: return -1;
   }
   if (!out || !(rsort > 0) || !(adimp > 0) || lmaxso < 0 || lmaxso > 40 || irrep < -1 || (irrep & ~mirrors) > 0)
     return -1;
+  const std::vector<int> ops = group_ops(group);
+  if (irrep >= 0 && group != 1) {
+    // irrep must be the smallest mask with its characters on the group
+    for (int v = 0; v < irrep; v++) {
+      bool same = true;
+      for (int g : ops) same &= ((__builtin_popcount(g & v) ^ __builtin_popcount(g & irrep)) & 1) == 0;
+      if (same) return -1;
+    }
+  }
   fdmgpu_cluster *c = new (std::nothrow) fdmgpu_cluster();
   if (!c) return -3;
   c->rsort = rsort;
   c->adimp = adimp;
   c->lmaxso = lmaxso;
-  c->irrep = mirrors ? irrep : -1;
+  c->irrep = group != 1 ? irrep : -1;
   c->mirrors = mirrors;
+  c->group = group;
+  c->ops = ops;
   if (build(c)) {
     delete c;
     return -1;

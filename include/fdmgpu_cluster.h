@@ -47,16 +47,27 @@ int fdmgpu_cluster_create(double rsort, double adimp, int lmaxso, fdmgpu_cluster
  * (mat.f90:2132-2143).  fdmgpu_cluster_assembly_tables and
  * fdmgpu_cluster_assembly_weights give the device assembler's tables. */
 int fdmgpu_cluster_create_mmm(double rsort, double adimp, int lmaxso, int irrep, fdmgpu_cluster **out);
-/* The same for the mirror groups by the reference's point-group number
- * (general.f90:5207-5220) in posequiv's orientation (general.f90:8961-8978):
- * group 1 (P1, irrep ignored), 3 (m, the mirror z = 0: wedge z >= 0),
- * 6 (mm2, the mirrors x = 0 and y = 0: wedge x, y >= 0) and 8 (mmm, as
- * fdmgpu_cluster_create_mmm).  irrep: -1 = the whole cluster, else its bits
- * (odd under the mirror normal to x, y, z) are a subset of the group's
- * mirrors: 2 representations for m, 4 for mm2, 8 for mmm.  rvol = the
- * point's number of images under the group / the group's order; the
- * potential must have the group's symmetry.  Returns -1 for any other
- * group or irrep. */
+/* The same for every subgroup of mmm by the reference's point-group number
+ * (general.f90:5207-5220), in posequiv's orientation and with its
+ * calculation areas (general.f90:8950-8978): group 1 (P1, irrep ignored),
+ * 2 (-1: z > 0, then the plane z = 0 split at x > 0 and the half line),
+ * 3 (m, the mirror z = 0: z >= 0), 4 (2, the two-fold axis z: x > 0, then
+ * the half plane x = 0, y >= 0), 5 (2/m: the area of 2 with z >= 0), 6 (mm2,
+ * the mirrors x = 0 and y = 0: x, y >= 0), 7 (222: x, y > 0, then the
+ * quarter planes x = 0 and y = 0 with z >= 0) and 8 (mmm, as
+ * fdmgpu_cluster_create_mmm).  Every operation negates some coordinates
+ * (opsym: 2-fold axes 22-24, inversion 25, mirrors 40-42); a neighbour
+ * outside the area is replaced by its image under the first operation, in
+ * opsym's order, that takes it inside (the mirror groups: under the mirrors
+ * of its negative coordinates), with that operation's character.  irrep:
+ * -1 = the whole cluster, else a character vector: the operation negating
+ * the coordinates of a set of axes has the character (-1)^(number of those
+ * axes whose bit is set), and irrep is the smallest mask with the
+ * representation's characters (for the mirror groups: the mirrors it is odd
+ * under, a subset of the group's; -1: 0, 1; 2: 0, 1; 2/m: 0, 1, 4, 5;
+ * 222: 0..3).  rvol = the point's number of images under the group / the
+ * group's order; the potential must have the group's symmetry.  Returns -1
+ * for any other group or irrep. */
 int fdmgpu_cluster_create_group(double rsort, double adimp, int lmaxso, int group, int irrep, fdmgpu_cluster **out);
 /* A P1 cluster with natome muffin-tin atoms: posi[3*natome] (bohr, the
  * absorber at the origin, in non-decreasing distance from it, as the

@@ -12,9 +12,9 @@ clusters and stores inputs and outputs as .npz fixtures:
   reseau / laplac / bordure (general.f90:9439/9732/9895, lists read back by
   recup_bordure)          -> point order, neighbour table, stencil weights,
                              outer-sphere boundary points and weights, for
-                             point group 1 and for the mirror groups m, mm2
-                             and mmm (wedge, rvol, symmetry operation of
-                             each neighbour)
+                             point group 1 and for the subgroups of mmm:
+                             m, mm2, mmm, -1, 2, 2/m, 222 (calculation area,
+                             rvol, symmetry operation of each neighbour)
   the same with muffin-tin atoms, then newind and mat_solve with a synthetic
   phiato (j_L(k r) Ylm at the atoms' border points)
                           -> atom border lists, weights, Ylmato, atom blocks
@@ -59,7 +59,8 @@ GRID_CASES = [(1.2, 0.25), (1.6, 0.3), (2.0, 0.25)]
 # point group -> its operations in the reference's numbering (opsym,
 # general.f90:4587): identity 1, two-fold rotations about x, y, z 22-24,
 # inversion 25, mirrors normal to x, y, z 40-42
-GROUP_OPS = {3: [1, 42], 6: [1, 24, 40, 41], 8: [1, 22, 23, 24, 25, 40, 41, 42]}
+GROUP_OPS = {3: [1, 42], 6: [1, 24, 40, 41], 8: [1, 22, 23, 24, 25, 40, 41, 42],
+             2: [1, 25], 4: [1, 24], 5: [1, 24, 25, 42], 7: [1, 22, 23, 24]}
 
 
 def grid_fixtures():

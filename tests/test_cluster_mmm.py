@@ -1,9 +1,12 @@
 """Symmetry-reduced systems (fdmgpu_cluster_create_mmm and
-fdmgpu_cluster_create_group): the representations of the mirror groups mmm
-(eight, wedge x, y, z >= 0), mm2 (four, wedge x, y >= 0) and m (two, wedge
-z >= 0), each solved on its wedge, reproduce the solution of the whole
-cluster (the oracle's P1 restatement) for a potential with exactly that
-symmetry (odd terms along the other axes).  With u = rvol * psi on the
+fdmgpu_cluster_create_group): the representations of the subgroups of mmm —
+the mirror groups mmm (eight, wedge x, y, z >= 0), mm2 (four, wedge
+x, y >= 0) and m (two, wedge z >= 0), and the groups with rotations or the
+inversion, -1 (two), 2 about z (two), 2/m (four) and 222 (four), whose
+calculation areas are posequiv's half-spaces and quarter-spaces with their
+boundary planes split (general.f90:8950-8978) — each solved on its area,
+reproduce the solution of the whole cluster (the oracle's P1 restatement)
+for a potential with exactly that symmetry.  With u = rvol * psi on the
 wedge (mat.f90:2132-2143), the reduced rows are the whole cluster's rows
 restricted to one representation: u / rvol equals the whole solution at the
 wedge points and the outer amplitudes are equal, for every right-hand side
@@ -18,31 +21,53 @@ BOHR = 0.52917721067
 
 
 MIRRORS = {8: 7, 6: 3, 3: 4}  # point-group number -> mirrors (bit a = normal to axis a)
-GROUPS = sorted(MIRRORS)
+# point-group number -> its operations as axis sign flips (bit a: coordinate a negated)
+OPS = {3: [0, 4], 6: [0, 3, 1, 2], 8: [0, 6, 5, 3, 7, 1, 2, 4], 2: [0, 7], 4: [0, 3], 5: [0, 3, 7, 4],
+       7: [0, 6, 5, 3]}
+GROUPS = [3, 6, 8, 2, 4, 5, 7]
+
+
+def _chars(group, v):
+    return tuple(bin(g & v).count("1") % 2 for g in OPS[group])
 
 
 def _irreps(group):
-    return [g for g in range(8) if g & ~MIRRORS[group] == 0]
+    """The representations, each named by the smallest mask v whose
+    characters (-1)^popcount(g & v) it has."""
+    out, seen = [], set()
+    for v in range(8):
+        c = _chars(group, v)
+        if c not in seen:
+            seen.add(c)
+            out.append(v)
+    assert len(out) == len(OPS[group])
+    return out
+
+
+def _flip(pts, g):
+    return pts * np.array([(-1) ** ((g >> a) & 1) for a in range(3)])
 
 
 def _potential(pts, group=8):
-    """Even under the group's mirrors only: not cubic, and along an axis
-    without a mirror it has an odd part."""
-    mk = MIRRORS[group]
-    x, y, z = (np.abs(pts[:, a]) if (mk >> a) & 1 else pts[:, a] for a in range(3))
-    r2 = x * x + y * y + z * z
-    v = -1.3 * np.exp(-r2 / 6.0) + 0.3 * np.cos(0.9 * x) * np.cos(0.6 * y) * np.cos(1.3 * z) - 0.2
-    return v + 0.25 * np.exp(-r2 / 8.0) * (np.tanh(0.5 * x) + 0.6 * np.sin(0.4 * y) + np.tanh(0.7 * z))
+    """Invariant under the group's operations and no others: a function with
+    no symmetry summed over the group."""
+    def f(q):
+        x, y, z = q[:, 0], q[:, 1], q[:, 2]
+        r2 = x * x + y * y + z * z
+        v = -1.3 * np.exp(-r2 / 6.0) + 0.3 * np.cos(0.9 * x) * np.cos(0.6 * y) * np.cos(1.3 * z) - 0.2
+        return v + 0.25 * np.exp(-r2 / 8.0) * (np.tanh(0.5 * x + 0.2) + 0.6 * np.sin(0.4 * y + 0.3 * x)
+                                               + np.tanh(0.7 * z - 0.1 * y))
+    return sum(f(_flip(pts, g)) for g in OPS[group]) / len(OPS[group])
 
 
 def _rep(CL, R, h, lm, g, group=8):
     """The representation's system, or None when no harmonic up to lm has
-    its parities (irrep 7, odd in x, y and z, needs l >= 3)."""
+    its parities (mmm's irrep 7, odd in x, y and z, needs l >= 3)."""
     from fdmnes_amd import _lib
     try:
         return CL.Cluster(R, h, lm, irrep=g, group=group)
     except _lib.FdmGpuError:
-        assert lm < 3 and g == 7
+        assert lm < 3 and g == 7 and group == 8
         return None
 
 
@@ -52,9 +77,10 @@ def _rel(a, b):
 
 @pytest.mark.parametrize("group", GROUPS)
 def test_harmonic_parities(group):
-    """The representation bits assigned to each real harmonic are its
-    parities under the group's mirrors (checked on the harmonics themselves),
-    and the representations share out every harmonic once."""
+    """Every real harmonic of a representation transforms under each of the
+    group's operations with the representation's character (checked on the
+    harmonics themselves), and the representations share out every harmonic
+    once."""
     from fdmnes_amd import cluster as CL
     lmax = 4
     v = np.array([0.31, 0.52, 0.79])
@@ -63,12 +89,11 @@ def test_harmonic_parities(group):
     for g in _irreps(group):
         c = CL.Cluster(1.2 / BOHR, 0.25 / BOHR, lmax, irrep=g, group=group)
         for lm in c.harmonics:
-            for a in (a for a in range(3) if (MIRRORS[group] >> a) & 1):
-                w = v.copy()
-                w[a] = -w[a]
+            for op in OPS[group]:
+                w = _flip(v[None, :], op)[0]
                 ya = O.ylm_real(lmax, w, float(np.linalg.norm(w)))[lm]
-                odd = (g >> a) & 1
-                assert abs(ya - (-1) ** odd * y0[lm]) <= 1e-12 * max(1.0, abs(y0[lm])), (g, lm, a)
+                chi = (-1) ** (bin(op & g).count("1") % 2)
+                assert abs(ya - chi * y0[lm]) <= 1e-12 * max(1.0, abs(y0[lm])), (g, lm, op)
         seen += list(c.harmonics)
     assert sorted(seen) == list(range((lmax + 1) ** 2))
 
@@ -76,18 +101,17 @@ def test_harmonic_parities(group):
 @pytest.mark.parametrize("group", GROUPS)
 def test_wedge_grid_and_weights(group):
     from fdmnes_amd import cluster as CL
-    mk = MIRRORS[group]
     full = CL.Cluster(1.6 / BOHR, 0.25 / BOHR, 2)
     c = CL.Cluster(1.6 / BOHR, 0.25 / BOHR, 2, irrep=0, group=group)
     pts, fp = c.points, full.points
-    for a in range(3):
-        if (mk >> a) & 1:
-            assert (pts[:, a] >= 0).all()
-        else:
-            assert (pts[:, a] < 0).any()
-    order = 1 << bin(mk).count("1")
-    signs = [[(-1) ** ((f >> a) & 1) for a in range(3)] for f in range(8) if f & ~mk == 0]
-    assert len(signs) == order
+    if group in MIRRORS:
+        for a in range(3):
+            if (MIRRORS[group] >> a) & 1:
+                assert (pts[:, a] >= 0).all()
+            else:
+                assert (pts[:, a] < 0).any()
+    order = len(OPS[group])
+    signs = [[(-1) ** ((f >> a) & 1) for a in range(3)] for f in OPS[group]]
     # every interior point of the whole cluster is the image of exactly one wedge point
     imgs = set()
     for p, rv in zip(pts, c.rvol):
@@ -212,12 +236,13 @@ def test_assembly_tables_rebuild_representation_rows(E_comp, basereel, group):
 
 
 def test_group_and_irrep_checks():
-    """Only the mirror groups are built, and a representation's bits must
-    be among the group's mirrors."""
+    """Only the subgroups of mmm are built, and a representation is named by
+    the smallest mask of its characters."""
     from fdmnes_amd import _lib
     from fdmnes_amd import cluster as CL
     R, h = 1.2 / BOHR, 0.25 / BOHR
-    for group, irrep in [(6, 4), (3, 1), (3, 2), (5, 0), (2, 0), (8, 8)]:
+    # a representation's mask must be the smallest with its characters; no group 9
+    for group, irrep in [(6, 4), (3, 1), (3, 2), (5, 2), (2, 2), (4, 4), (7, 4), (8, 8), (9, 0)]:
         with pytest.raises(_lib.FdmGpuError):
             CL.Cluster(R, h, 2, irrep=irrep, group=group)
     whole = CL.Cluster(R, h, 2)

@@ -29,7 +29,10 @@ from oracle import oracle as O
 BOHR = 0.52917721067
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 P1 = ["grid_p1_R1.2_h0.25", "grid_p1_R1.6_h0.3", "grid_p1_R2.0_h0.25"]
-GROUPS = ["grid_g3_R1.6_h0.25", "grid_g6_R1.6_h0.25", "grid_g8_R1.6_h0.25"]
+GROUPS = ["grid_g3_R1.6_h0.25", "grid_g6_R1.6_h0.25", "grid_g8_R1.6_h0.25", "grid_g2_R1.6_h0.25",
+          "grid_g4_R1.6_h0.25", "grid_g5_R1.6_h0.25", "grid_g7_R1.6_h0.25"]
+# the reference's operations as the axes they negate (opsym, general.f90:4587)
+OP_AXES = {1: (), 40: (0,), 41: (1,), 42: (2,), 22: (1, 2), 23: (0, 2), 24: (0, 1), 25: (0, 1, 2)}
 EPSPOS = 1e-4
 # stencil offsets in laplac's order (general.f90:9768-9781): per order io = 1, 2,
 # per axis x, y, z, the negative then the positive step
@@ -69,10 +72,10 @@ def test_product_grid_equals_reference(name):
 
 
 def _chi(op, irrep):
-    """Character of the reference's operation `op` in the mirror-group
-    representation `irrep` (bit a set: odd under the mirror normal to axis a)."""
-    mirrors = {1: (), 40: (0,), 41: (1,), 42: (2,), 22: (1, 2), 23: (0, 2), 24: (0, 1), 25: (0, 1, 2)}[abs(op)]
-    return (-1.0) ** sum((irrep >> a) & 1 for a in mirrors)
+    """Character of the reference's operation `op` in the representation
+    `irrep` (a character vector: the operation negating the axes in a set
+    has the character (-1)^(number of those axes whose bit is set))."""
+    return (-1.0) ** sum((irrep >> a) & 1 for a in OP_AXES[abs(op)])
 
 
 @pytest.mark.parametrize("name", GROUPS)
@@ -81,9 +84,8 @@ def test_product_mirror_group_builder_equals_reference(name):
     g = _load(name)
     grp, h = int(g["group"]), float(g["h"]) / BOHR
     npt = int(g["npoint"])
-    bits = CL.Cluster.GROUPS[grp]
-    irreps = [r for r in range(8) if r & ~bits == 0]
-    for irrep in irreps:
+    from .test_cluster_mmm import _irreps
+    for irrep in _irreps(grp):
         c = CL.Cluster(float(g["R"]) / BOHR, h, 3, irrep=irrep, group=grp)
         assert (c.npoint, c.npso, c.nsortf, c.nsort) == (npt, g["npso"], g["nsortf"], g["nsort"])
         assert np.array_equal(np.rint(c.points / h).astype(int), g["indice"][:npt])
@@ -93,16 +95,17 @@ def test_product_mirror_group_builder_equals_reference(name):
         # grid-row couplings between interior points, energy-independent
         rp, col, vals, _ = c.assemble(np.zeros(npt), 0.0, 0.0, 1.0, False, True)
         rvol, clapl = g["rvol"], g["clapl"]
-        # points on a mirror plane the representation is odd under carry a
-        # zero wave function: there two operations fold a neighbour onto the
-        # same image with different characters (the reference takes the
-        # first in its list, e.g. C2z where the product takes a mirror), and
-        # the couplings among such points are left out of the comparison
+        # points an operation of odd character leaves in place (a mirror
+        # plane, a two-fold axis, the centre of inversion) carry a zero wave
+        # function: there two operations fold a neighbour onto the same image
+        # with different characters (the reference takes the first in its
+        # list, e.g. C2z where the product takes a mirror), and the couplings
+        # among such points are left out of the comparison
         ind = g["indice"]
         zero = np.zeros(len(ind), bool)
-        for a in range(3):
-            if (irrep >> a) & 1:
-                zero |= ind[:, a] == 0
+        for op in g["ops"]:
+            if _chi(int(op), irrep) < 0:
+                zero |= np.all([ind[:, a] == 0 for a in OP_AXES[int(op)]], axis=0)
         worst = 0.0
         for i in range(npt):
             if zero[i]:
@@ -187,14 +190,11 @@ def test_neighbours_points_and_boundary(name):
     got = [tuple(p) for p in ind]
     assert len(set(got)) == len(got)
     if "group" in g:
-        ops = {3: [(1, 1, -1)], 6: [(-1, 1, 1), (1, -1, 1)], 8: [(-1, 1, 1), (1, -1, 1), (1, 1, -1)]}[int(g["group"])]
-        orbits = set()
-        for p in lattice:
-            q = np.array(p)
-            for o in ops:  # fold into the wedge of posequiv's orientation
-                q = np.where((np.array(o) < 0) & (q < 0), -q, q)
-            orbits.add(tuple(q))
-        assert set(got) == orbits
+        # one point per orbit of the group's operations
+        signs = [np.array([-1 if a in OP_AXES[int(op)] else 1 for a in range(3)]) for op in g["ops"]]
+        orbits = {frozenset(tuple(sg * np.array(p)) for sg in signs) for p in lattice}
+        assert len(got) == len(orbits)
+        assert {frozenset(tuple(sg * np.array(p)) for sg in signs) for p in got} == orbits
     else:
         assert set(got) == lattice
     # interior points: radius <= Rsort
