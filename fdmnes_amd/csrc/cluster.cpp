@@ -422,11 +422,12 @@ int build(fdmgpu_cluster *c) {
     // area point lies in the grid's symmetric sphere)
     double rvmax = 1.0;
     for (int i = 0; i < n; i++) {
+      // two operations give the same image when they differ only in the
+      // flips of zero coordinates
+      const int zero = (p[i].ix == 0) | ((p[i].iy == 0) << 1) | ((p[i].iz == 0) << 2);
       int img[8], ni = 0;
       for (int g : c->ops) {
-        const int x = (g & 1) ? -p[i].ix : p[i].ix, y = (g & 2) ? -p[i].iy : p[i].iy,
-                  z = (g & 4) ? -p[i].iz : p[i].iz;
-        const int k = ((x + 64) << 14) | ((y + 64) << 7) | (z + 64);
+        const int k = g & ~zero;
         bool seen = false;
         for (int t = 0; t < ni; t++) seen |= img[t] == k;
         if (!seen) img[ni++] = k;

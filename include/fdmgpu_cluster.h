@@ -66,7 +66,8 @@ int fdmgpu_cluster_create_mmm(double rsort, double adimp, int lmaxso, int irrep,
  * representation's characters (for the mirror groups: the mirrors it is odd
  * under, a subset of the group's; -1: 0, 1; 2: 0, 1; 2/m: 0, 1, 4, 5;
  * 222: 0..3).  rvol = the point's number of images under the group / the
- * group's order; the potential must have the group's symmetry.  Returns -1
+ * largest such number (the group's order once a grid point lies off every
+ * plane and axis); the potential must have the group's symmetry.  Returns -1
  * for any other group or irrep. */
 int fdmgpu_cluster_create_group(double rsort, double adimp, int lmaxso, int group, int irrep, fdmgpu_cluster **out);
 /* A P1 cluster with natome muffin-tin atoms: posi[3*natome] (bohr, the
