@@ -6,7 +6,8 @@ the batched solver reads.  Stands in for the per-row calcMatRow calls of
 mat.f90:1034-1727 (grid rows :1142-1232, outer columns :1312-1398, outer
 rows :1401-1511; with muffin-tin atoms also the couplings to the atoms'
 expansions :1233-1309 and the atom rows :1515-1632, from the caller's
-phiato)."""
+phiato; with scalar-relativistic rows the terms of :1155-1156, :1171-1185
+from the caller's potential gradient)."""
 from __future__ import annotations
 
 import ctypes as C
@@ -35,6 +36,13 @@ class AtomTables(C.Structure):
                 ("nbordf", _V), ("phoff", _V), ("bq", _V), ("poidsa", _V), ("ylmato", _V)]
 
 
+class RelTables(C.Structure):
+    _fields_ = [("npoint", C.c_int), ("nrow_out", C.c_int), ("nlink", C.c_int), ("nar", C.c_int), ("nal", C.c_int),
+                ("clapl", C.c_double * 13), ("cgrad", C.c_double * 13),
+                ("rq", _V), ("rfac", _V), ("rdiv", _V), ("opoint", _V), ("odir", _V), ("ofac", _V),
+                ("apoint", _V), ("adir", _V)]
+
+
 def _sigs(L):
     if getattr(L, "_asm_sigs", False):
         return
@@ -47,6 +55,9 @@ def _sigs(L):
     L.fdmgpu_assembler_create_atoms.argtypes = [P(Tables), P(AtomTables), C.c_int, _V, P(_V)]
     L.fdmgpu_assembler_run_atoms.argtypes = [_V, C.c_int, P(C.c_double), P(C.c_double), P(C.c_double), C.c_int,
                                              C.c_int, C.c_int, P(_V), P(_V), C.c_int, P(_V), P(_V)]
+    L.fdmgpu_assembler_set_relativistic.argtypes = [_V, P(RelTables)]
+    L.fdmgpu_assembler_run_rel.argtypes = [_V, C.c_int, P(C.c_double), P(C.c_double), P(C.c_double), C.c_int,
+                                           C.c_int, C.c_int, P(_V), P(_V), P(_V), C.c_int, P(_V), P(_V)]
     L._asm_sigs = True
 
 
@@ -70,7 +81,10 @@ class DeviceAssembler:
     of its own, ordered after the caller's queued work and before its later
     work (see ``fdmnes_amd.batch.Ordered``)."""
 
-    def __init__(self, cluster, max_points: int = MAX_POINTS, stream: torch.cuda.Stream | None = None):
+    def __init__(self, cluster, max_points: int = MAX_POINTS, stream: torch.cuda.Stream | None = None,
+                 relativistic: bool = False):
+        """relativistic: scalar-relativistic rows (the reference's
+        Relativiste); ``run`` then takes the potential's gradient G."""
         L = _lib.lib()
         _sigs(L)
         self.L = L
@@ -97,13 +111,23 @@ class DeviceAssembler:
                                                  C.byref(h)))
         self._h = h
         self.max_points = int(max_points)
+        self.relativistic = bool(relativistic)
+        if self.relativistic:
+            from .cluster import REL_TABLES
+            rt = cluster.rel_tables()
+            self._keep_rel = {k: np.ascontiguousarray(rt[k]) for k in REL_TABLES}
+            rtb = RelTables(rt["npoint"], rt["nrow_out"], rt["nlink"], rt["nar"], rt["nal"],
+                            (C.c_double * 13)(*rt["clapl"]), (C.c_double * 13)(*rt["cgrad"]),
+                            *[self._keep_rel[k].ctypes.data_as(_V) for k in REL_TABLES])
+            _lib.check(L.fdmgpu_assembler_set_relativistic(h, C.byref(rtb)))
 
     def __del__(self):
         if getattr(self, "_h", None):
             self.L.fdmgpu_assembler_destroy(self._h)
             self._h = None
 
-    def run(self, V, energies, E_comp: bool, basereel: bool, vals, rhs=None, update_only: bool = False, phi=None):
+    def run(self, V, energies, E_comp: bool, basereel: bool, vals, rhs=None, update_only: bool = False, phi=None,
+            G=None):
         """V: list of float64 device tensors (npoint); energies: list of
         (Enervide, Eimag, Ecinetic_out) in Ry; vals (and rhs) lists of
         contiguous device tensors, complex128 when the arithmetic is complex
@@ -112,7 +136,9 @@ class DeviceAssembler:
         arithmetic branch); only the energy- and V-dependent entries are
         rewritten.  phi (clusters with atoms): per point a device tensor
         of the nphi phiato values (Cluster.assemble_atoms's blocks,
-        concatenated), complex128 (only in complex arithmetic) or float64."""
+        concatenated), complex128 (only in complex arithmetic) or float64.
+        G (relativistic assemblers only): per point the float64 device
+        gradient of V, (npoint, 3)."""
         n = len(energies)
         if n == 0:
             return
@@ -133,6 +159,14 @@ class DeviceAssembler:
                 raise ValueError("V: float64, npoint entries")
         if (phi is not None) != bool(self.natome):
             raise ValueError("phi is given exactly for a cluster with atoms")
+        if (G is not None) != self.relativistic:
+            raise ValueError("G is given exactly for a relativistic assembler")
+        if G is not None:
+            if len(G) != n:
+                raise ValueError("one G per energy point")
+            for t in G:
+                if t.dtype != torch.float64 or t.numel() != 3 * self.npoint:
+                    raise ValueError("G: float64, (npoint, 3)")
         phi_cplx = False
         if phi is not None:
             if len(phi) != n:
@@ -150,9 +184,10 @@ class DeviceAssembler:
         with Ordered(self.stream):
             for k0 in range(0, n, self.max_points):
                 k1 = min(n, k0 + self.max_points)
-                rc = self.L.fdmgpu_assembler_run_atoms(
+                rc = self.L.fdmgpu_assembler_run_rel(
                     self._h, k1 - k0, *[np.ascontiguousarray(a[k0:k1]).ctypes.data_as(D) for a in ev],
                     int(bool(E_comp)), int(bool(basereel)), int(bool(update_only)), _dptrs(V[k0:k1]),
+                    _dptrs(G[k0:k1]) if G is not None else None,
                     _dptrs(phi[k0:k1]) if phi is not None else None, int(phi_cplx), _dptrs(vals[k0:k1]),
                     _dptrs(rhs[k0:k1]) if rhs is not None else None)
                 if rc:

@@ -37,6 +37,15 @@ def _L():
         L.fdmgpu_cluster_assemble.argtypes = [C.c_void_p, _lib.dp, C.c_double, C.c_double, C.c_double, C.c_int,
                                               C.c_int, _lib.lp, _lib.ip, _lib.dp, C.c_void_p, C.c_void_p,
                                               C.c_void_p]
+        L.fdmgpu_cluster_assemble_rel.restype = C.c_int
+        L.fdmgpu_cluster_assemble_rel.argtypes = [C.c_void_p, _lib.dp, C.c_void_p, C.c_double, C.c_double, C.c_double,
+                                                  C.c_int, C.c_int, _lib.lp, _lib.ip, _lib.dp, C.c_void_p, C.c_void_p,
+                                                  C.c_void_p]
+        L.fdmgpu_cluster_assemble_atoms_rel.restype = C.c_int
+        L.fdmgpu_cluster_assemble_atoms_rel.argtypes = [C.c_void_p, _lib.dp, C.c_void_p, C.c_double, C.c_double,
+                                                        C.c_double, C.c_int, C.c_int, _lib.dp, C.c_void_p,
+                                                        C.c_void_p, C.c_void_p, _lib.lp, _lib.ip, _lib.dp,
+                                                        C.c_void_p, C.c_void_p, C.c_void_p]
         L.fdmgpu_cluster_assembly_sizes.restype = C.c_int
         L.fdmgpu_cluster_assembly_sizes.argtypes = [C.c_void_p, _lib.lp]
         L.fdmgpu_cluster_assembly_tables.restype = C.c_int
@@ -63,6 +72,9 @@ def _L():
         L.fdmgpu_cluster_assemble_atoms.argtypes = [C.c_void_p, _lib.dp, C.c_double, C.c_double, C.c_double,
                                                     C.c_int, C.c_int, _lib.dp, C.c_void_p, _lib.lp, _lib.ip,
                                                     _lib.dp, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.fdmgpu_cluster_rel_tables.restype = C.c_int
+        L.fdmgpu_cluster_rel_tables.argtypes = [C.c_void_p, _lib.lp, _lib.dp, _lib.dp, _lib.ip, _lib.ip, _lib.dp,
+                                                _lib.ip, _lib.ip, _lib.dp, _lib.dp]
         L.fdmgpu_cluster_atom_assembly_sizes.restype = C.c_int
         L.fdmgpu_cluster_atom_assembly_sizes.argtypes = [C.c_void_p, _lib.lp]
         L.fdmgpu_cluster_atom_assembly_tables.restype = C.c_int
@@ -162,8 +174,19 @@ class Cluster:
         _L().fdmgpu_cluster_profile(self._h, lb1, lb2)
         return lb1, lb2
 
-    def assemble(self, V, Enervide: float, Eimag: float, Ecinetic_out: float, E_comp: bool, basereel: bool):
-        """Compressed rows and the incoming-wave RHS of one energy point."""
+    def _gradvr(self, gradvr):
+        if gradvr is None:
+            return None
+        g = np.ascontiguousarray(gradvr, np.float64)
+        if g.shape != (self.npoint, 3):
+            raise ValueError(f"gradvr must be (npoint, 3) = ({self.npoint}, 3), got {g.shape}")
+        return g
+
+    def assemble(self, V, Enervide: float, Eimag: float, Ecinetic_out: float, E_comp: bool, basereel: bool,
+                 gradvr=None):
+        """Compressed rows and the incoming-wave RHS of one energy point.
+        gradvr (npoint, 3): the potential's gradient (Ry/bohr) for
+        scalar-relativistic rows (the reference's Relativiste)."""
         cal = bool(E_comp) or not basereel
         n, nl = self.nligne, self.nlmso
         rp = np.zeros(n + 1, np.int64)
@@ -172,9 +195,10 @@ class Cluster:
         im = np.zeros(self.nnz) if cal else None
         rr = np.zeros(n * nl)
         ri = np.zeros(n * nl) if cal else None
-        rc = _L().fdmgpu_cluster_assemble(self._h, np.ascontiguousarray(V, np.float64), Enervide, Eimag,
-                                          Ecinetic_out, int(bool(E_comp)), int(bool(basereel)), rp, col, re,
-                                          _lib.ptr(im), _lib.ptr(rr), _lib.ptr(ri))
+        g = self._gradvr(gradvr)
+        rc = _L().fdmgpu_cluster_assemble_rel(self._h, np.ascontiguousarray(V, np.float64), _lib.ptr(g), Enervide,
+                                              Eimag, Ecinetic_out, int(bool(E_comp)), int(bool(basereel)), rp, col,
+                                              re, _lib.ptr(im), _lib.ptr(rr), _lib.ptr(ri))
         if rc:
             raise _lib.FdmGpuError(f"fdmgpu_cluster_assemble failed ({rc})")
         vals = re + 1j * im if cal else re
@@ -216,11 +240,12 @@ class Cluster:
         return dict(nbord=nb, nbordf=nbf, nlmsa=nl, ianew=ianew, ibord=ib, poidsa=pa, ylmato=ya.reshape(nb, nl))
 
     def assemble_atoms(self, V, Enervide: float, Eimag: float, Ecinetic_out: float, E_comp: bool, basereel: bool,
-                       phiato, phiato2=None):
+                       phiato, phiato2=None, gradvr=None):
         """Compressed rows and incoming-wave RHS of one energy point of a
         cluster with atoms; phiato: per atom an (nbord, nlmsa) array of the
         radial solutions times Ylm at its border points, complex or real;
-        phiato2 the irregular atomic solutions (irregular clusters only)."""
+        phiato2 the irregular atomic solutions (irregular clusters only);
+        gradvr (npoint, 3) for scalar-relativistic rows."""
         cal = bool(E_comp) or not basereel
         sets = [phiato] if phiato2 is None else [phiato, phiato2]
         cplx = any(np.iscomplexobj(p) for xs in sets for p in xs)
@@ -238,10 +263,12 @@ class Cluster:
         im = np.zeros(self.nnz) if cal else None
         rr = np.zeros(n * nl)
         ri = np.zeros(n * nl) if cal else None
-        rc = _L().fdmgpu_cluster_assemble_atoms_ri(self._h, np.ascontiguousarray(V, np.float64), Enervide, Eimag,
-                                                   Ecinetic_out, int(bool(E_comp)), int(bool(basereel)), pr,
-                                                   _lib.ptr(pi), _lib.ptr(p2r), _lib.ptr(p2i), rp, col, re,
-                                                   _lib.ptr(im), _lib.ptr(rr), _lib.ptr(ri))
+        g = self._gradvr(gradvr)
+        rc = _L().fdmgpu_cluster_assemble_atoms_rel(self._h, np.ascontiguousarray(V, np.float64), _lib.ptr(g),
+                                                    Enervide, Eimag, Ecinetic_out, int(bool(E_comp)),
+                                                    int(bool(basereel)), pr, _lib.ptr(pi), _lib.ptr(p2r),
+                                                    _lib.ptr(p2i), rp, col, re, _lib.ptr(im), _lib.ptr(rr),
+                                                    _lib.ptr(ri))
         if rc:
             raise _lib.FdmGpuError(f"fdmgpu_cluster_assemble_atoms failed ({rc})")
         vals = re + 1j * im if cal else re
@@ -299,4 +326,29 @@ class Cluster:
         return t
 
 
+    def rel_tables(self) -> dict:
+        """The device assembler's tables of scalar-relativistic rows
+        (fdmgpu_cluster_rel_tables)."""
+        L = _L()
+        sz = np.zeros(4, np.int64)
+        L.fdmgpu_cluster_assembly_sizes(self._h, sz)
+        nrow, nlink = int(sz[0]), int(sz[1])
+        nar = nal = 0
+        if self.natome:
+            sa = np.zeros(7, np.int64)
+            L.fdmgpu_cluster_atom_assembly_sizes(self._h, sa)
+            nar, nal = int(sa[1]), int(sa[2])
+        n12 = 12 * self.npoint
+        t = dict(npoint=self.npoint, nrow_out=nrow, nlink=nlink, nar=nar, nal=nal, rq=np.zeros(n12, np.int64),
+                 rfac=np.zeros(n12), rdiv=np.zeros(n12), opoint=np.zeros(max(nrow, 1), np.int32),
+                 odir=np.zeros(max(nlink, 1), np.int32), ofac=np.zeros(max(nlink, 1)),
+                 apoint=np.zeros(max(nar, 1), np.int32), adir=np.zeros(max(nal, 1), np.int32),
+                 clapl=np.zeros(13), cgrad=np.zeros(13))
+        rc = L.fdmgpu_cluster_rel_tables(self._h, *[t[k] for k in REL_TABLES], t["clapl"], t["cgrad"])
+        if rc:
+            raise _lib.FdmGpuError(f"fdmgpu_cluster_rel_tables failed ({rc})")
+        return t
+
+
 ATOM_TABLES = ("aq", "aa", "aptr", "alphi", "alw", "ianew", "nlmsa", "nbordf", "phoff", "bq", "poidsa", "ylmato")
+REL_TABLES = ("rq", "rfac", "rdiv", "opoint", "odir", "ofac", "apoint", "adir")

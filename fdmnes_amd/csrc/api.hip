@@ -580,10 +580,27 @@ int fdmgpu_assembler_create(const fdmgpu_assembly_tables *t, int max_points, voi
 
 void fdmgpu_assembler_destroy(fdmgpu_assembler *a) { delete a; }
 
+int fdmgpu_assembler_set_relativistic(fdmgpu_assembler *a, const fdmgpu_rel_tables *t) {
+  if (!a || !t) {
+    set_error("fdmgpu_assembler_set_relativistic: invalid argument");
+    return FDMGPU_EINVAL;
+  }
+  FDM_HIP(hipSetDevice(g_lib.device));
+  return a->a.init_rel(t);
+}
+
 int fdmgpu_assembler_run_atoms(fdmgpu_assembler *a, int count, const double *Enervide, const double *Eimag,
                                const double *Ecinetic_out, int E_comp, int basereel, int update_only,
                                const double *const *d_V, const double *const *d_phi, int phi_cplx,
                                double *const *d_vals, double *const *d_rhs) {
+  return fdmgpu_assembler_run_rel(a, count, Enervide, Eimag, Ecinetic_out, E_comp, basereel, update_only, d_V,
+                                  nullptr, d_phi, phi_cplx, d_vals, d_rhs);
+}
+
+int fdmgpu_assembler_run_rel(fdmgpu_assembler *a, int count, const double *Enervide, const double *Eimag,
+                             const double *Ecinetic_out, int E_comp, int basereel, int update_only,
+                             const double *const *d_V, const double *const *d_G, const double *const *d_phi,
+                             int phi_cplx, double *const *d_vals, double *const *d_rhs) {
   if (!a || !Enervide || !Eimag || !Ecinetic_out || !d_V || !d_vals || count < 1 || count > ASM_MAXP) {
     set_error("fdmgpu_assembler_run: invalid argument");
     return FDMGPU_EINVAL;
@@ -595,10 +612,11 @@ int fdmgpu_assembler_run_atoms(fdmgpu_assembler *a, int count, const double *Ene
   }
   AsmPoints P{};
   for (int k = 0; k < count; k++) {
-    if (!d_V[k] || !d_vals[k] || (d_phi && !d_phi[k])) {
+    if (!d_V[k] || !d_vals[k] || (d_phi && !d_phi[k]) || (d_G && !d_G[k])) {
       set_error("fdmgpu_assembler_run: null device buffer for point " + std::to_string(k));
       return FDMGPU_EINVAL;
     }
+    P.G[k] = d_G ? d_G[k] : nullptr;
     P.e[k] = {Enervide[k], Eimag[k], Ecinetic_out[k]};
     P.V[k] = d_V[k];
     P.vals[k] = d_vals[k];

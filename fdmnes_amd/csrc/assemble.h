@@ -20,6 +20,12 @@ struct AsmPoints {
   double *vals[ASM_MAXP];     // device, nnz (interleaved when complex)
   double *rhs[ASM_MAXP];      // device, (nligne, nlmso) row-major, or null
   const double *phi[ASM_MAXP];  // device, phiato of a cluster with atoms (interleaved when complex)
+  const double *G[ASM_MAXP];    // device, gradient of V (npoint x 3, point-major) for relativistic rows
+};
+// stencil and gradient coefficients of the relativistic terms, 1..12 in
+// the reference's neighbour order (clapl[0] unused)
+struct RelCoef {
+  double clapl[13], cgrad[13];
 };
 
 struct Assembler {
@@ -43,14 +49,25 @@ struct Assembler {
   int *aa = nullptr, *aptr = nullptr, *ianew = nullptr, *nlmsa = nullptr, *nbordf = nullptr, *arow_atom = nullptr,
       *arow_lm = nullptr;
   double *alw = nullptr, *poidsa = nullptr, *ylmato = nullptr;
+  // scalar-relativistic rows (init_rel): per grid point and direction the
+  // position its interior neighbour's term lands in (-1: none), chi rvol(i)
+  // and rvol(j); per outer-column row and row-atom pair the grid point, per
+  // link its direction and chi rvol(i)
+  bool rel = false;
+  RelCoef relc{};
+  int64_t *rq = nullptr;
+  double *rfac = nullptr, *rdiv = nullptr, *ofac = nullptr;
+  int *opoint = nullptr, *odir = nullptr, *apoint = nullptr, *adir = nullptr;
 
   ~Assembler();
   // at: null for a cluster without atoms
   int init(const fdmgpu_assembly_tables *t, const fdmgpu_atom_tables *at, int max_points, hipStream_t s);
   int init_atoms(const fdmgpu_atom_tables *at, int64_t natrow, const int64_t *row_ptr_h);
+  int init_rel(const fdmgpu_rel_tables *t);
   // update_only: vals/rhs already hold an earlier call's output for this
   // structure; only the energy-dependent entries are rewritten.  phi_cplx:
-  // the points' phiato are complex (atoms only)
+  // the points' phiato are complex (atoms only).  An assembler with
+  // relativistic tables needs pts.G for every point.
   int run(int count, const AsmPoints &pts, bool e_comp, bool basereel, bool update_only, bool phi_cplx = false);
 };
 

@@ -16,7 +16,11 @@
 // and for a cluster with muffin-tin atoms, from the caller's radial
 // solutions phiato at the atoms' border points,
 //   6. the grid rows' couplings to an atom's expansion  (mat.f90:1233-1309),
-//   7. the atom rows' own blocks                         (mat.f90:1583-1632).
+//   7. the atom rows' own blocks                         (mat.f90:1583-1632),
+// and with scalar-relativistic rows (mat.f90:1155-1156, :1171-1185) the
+// diagonal's -a2s4 (V - E)^2 and every stencil coefficient of a grid row,
+// -clapl(v) + bder cgrad(v) dV/dx_k, in the interior entries (8.), the outer
+// columns (4.) and the atom couplings (6.).
 // Multiply-adds are not fused (contraction off) and the grid rows' sums run
 // in the reference's term order; the outer rows' sums over the boundary
 // points are taken in runs of 256 points, then over the runs, in order.  The entries agree with a
@@ -59,6 +63,19 @@ __device__ inline cz csqrt(cz z) {
   const double t = sqrt((fabs(z.re) + hypot(z.re, z.im)) * 0.5);
   if (z.re >= 0) return {t, z.im / (2 * t)};
   return {fabs(z.im) / (2 * t), copysign(t, z.im)};
+}
+
+constexpr double ASM_A2S4 = 0.25 * (0.0072973525664 * 0.0072973525664);  // alpha^2 / 4 (mat.f90:1123)
+
+// bder of grid point i (mat.f90:1156)
+__device__ inline double rel_bder(const AsmPoints &P, int g, int i) {
+  const double vme = P.V[g][i] - P.e[g].ener;
+  return -ASM_A2S4 / (1 - ASM_A2S4 * vme);
+}
+// stencil coefficient of direction d (0..11) of grid point i with the
+// relativistic term, before the character and rvol factors (mat.f90:1164-1185)
+__device__ inline double rel_coef(const RelCoef &rc, double bder, const double *Gi, int d) {
+  return -rc.clapl[d + 1] + bder * rc.cgrad[d + 1] * Gi[(d >> 1) % 3];
 }
 
 __device__ inline int lof(int lm) {  // l of the harmonic index l*l + l + m
@@ -161,14 +178,17 @@ __global__ void k_asm_base(int64_t nnz, int64_t nrhs_el, const double *__restric
 // 3. grid diagonals
 template <bool CAL>
 __global__ void k_asm_diag(int npoint, const int64_t *__restrict__ qdiag, double clapl0,
-                           const double *__restrict__ dself, AsmPoints P, int e_comp) {
+                           const double *__restrict__ dself, AsmPoints P, int e_comp, int rel) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x, g = blockIdx.y;
   if (i >= npoint) return;
   const AsmEnergy E = P.e[g];
   const int64_t q = qdiag[i];
   if (q < 0) return;  // a point inside an atom has no row
   double d = P.V[g][i] - E.ener - clapl0;
-  if (dself) {  // mirror images of the point itself, added in axis order (mat.f90:1225-1231)
+  if (rel) {  // mat.f90:1155; the point's own images follow in k_asm_rel
+    const double vme = P.V[g][i] - E.ener;
+    d = d - ASM_A2S4 * (vme * vme);
+  } else if (dself) {  // mirror images of the point itself, added in axis order (mat.f90:1225-1231)
     d = d + dself[3 * i];
     d = d + dself[3 * i + 1];
     d = d + dself[3 * i + 2];
@@ -183,20 +203,36 @@ __global__ void k_asm_diag(int npoint, const int64_t *__restrict__ qdiag, double
 
 // 4. outer columns of the grid rows touching the outside, and their RHS:
 // one thread per (row, harmonic), outside neighbours in stencil order
-template <bool CAL>
+// (relativistic rows: the weight of each link recomputed from its
+// direction, chi rvol(i) and the row's point)
+template <bool CAL, bool REL>
 __global__ void k_asm_outcols(int nrow_out, int nl, int nb, int L, const int *__restrict__ orow,
                               const int64_t *__restrict__ oq, const int *__restrict__ optr,
                               const int *__restrict__ ob, const double *__restrict__ ow,
                               const double *__restrict__ ylm, const int *__restrict__ lharm,
-                              const cz *__restrict__ fun, AsmPoints P, int basereel) {
+                              const cz *__restrict__ fun, AsmPoints P, int basereel,
+                              const int *__restrict__ opoint, const int *__restrict__ odir,
+                              const double *__restrict__ ofac, RelCoef rc) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x, g = blockIdx.y;
   if (t >= nrow_out * nl) return;
   const int r = t / nl, lm = t % nl, l = lharm[lm];
   const cz *fg = fun + (int64_t)g * nb * (L + 1) * 2;
+  double bder = 0;
+  const double *Gi = nullptr;
+  if constexpr (REL) {
+    const int i = opoint[r];
+    bder = rel_bder(P, g, i);
+    Gi = P.G[g] + 3 * (int64_t)i;
+  }
   cz row = {0, 0}, rrow = {0, 0};
   for (int k = optr[r]; k < optr[r + 1]; k++) {
     const int b = ob[k];
-    const double w = ow[k], y = ylm[(int64_t)b * nl + lm];
+    double w;
+    if constexpr (REL)
+      w = rel_coef(rc, bder, Gi, odir[k]) * ofac[k];
+    else
+      w = ow[k];
+    const double y = ylm[(int64_t)b * nl + lm];
     const cz *f = fg + (int64_t)b * (L + 1) * 2;
     row = add(row, scale(scale(cfac(f, l, basereel), y), w));
     rrow = sub(rrow, scale(scale(f[2 * l], y), w));
@@ -278,18 +314,32 @@ __global__ void k_asm_outrows_sum(int nso, int nl, int nsortf, int nrun, const i
 // 6. a grid row's couplings to the expansion of an atom its stencil reaches:
 // the sum over its neighbours inside the atom, in stencil order, of
 // -clapl(v) phiato(neighbour, Lm); one thread per (row-atom pair, Lm)
-template <bool CAL, bool PHIC>
+// (relativistic rows: each link's weight recomputed from its direction and
+// the row's point; atoms are built in P1, where chi rvol(i) = 1)
+template <bool CAL, bool PHIC, bool REL>
 __global__ void k_asm_atomcols(int nar, int nlmax, const int64_t *__restrict__ aq, const int *__restrict__ aa,
                                const int *__restrict__ aptr, const int64_t *__restrict__ alphi,
-                               const double *__restrict__ alw, const int *__restrict__ nlmsa, AsmPoints P) {
+                               const double *__restrict__ alw, const int *__restrict__ nlmsa, AsmPoints P,
+                               const int *__restrict__ apoint, const int *__restrict__ adir, RelCoef rc) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x, g = blockIdx.y;
   if (t >= nar * nlmax) return;
   const int r = t / nlmax, lm = t % nlmax;
   if (lm >= nlmsa[aa[r]]) return;
   const double *phi = P.phi[g];
+  double bder = 0;
+  const double *Gi = nullptr;
+  if constexpr (REL) {
+    const int i = apoint[r];
+    bder = rel_bder(P, g, i);
+    Gi = P.G[g] + 3 * (int64_t)i;
+  }
   double sr = 0, si = 0;
   for (int k = aptr[r]; k < aptr[r + 1]; k++) {
-    const double w = alw[k];
+    double w;
+    if constexpr (REL)
+      w = rel_coef(rc, bder, Gi, adir[k]);
+    else
+      w = alw[k];
     const int64_t o = alphi[k] + lm;
     if constexpr (PHIC) {
       sr = sr + w * phi[2 * o];
@@ -346,6 +396,41 @@ __global__ void k_asm_atomrows(int nrow_atom, int nlmax, const int *__restrict__
   }
 }
 
+// 8. relativistic grid rows: the entries towards interior neighbours (and
+// the point's own images, on the diagonal k_asm_diag wrote) recomputed with
+// the relativistic coefficient, neighbours in the reference's order
+// (mat.f90:1164-1232); one thread per grid point, which alone writes its row
+template <bool CAL>
+__global__ void k_asm_rel(int npoint, const int64_t *__restrict__ qdiag, const int64_t *__restrict__ rq,
+                          const double *__restrict__ rfac, const double *__restrict__ rdiv, RelCoef rc,
+                          AsmPoints P) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, g = blockIdx.y;
+  if (i >= npoint) return;
+  const int64_t qd = qdiag[i];
+  if (qd < 0) return;  // inside an atom: no row
+  double *v = P.vals[g];
+  const int64_t *q = rq + 12 * (int64_t)i;
+  for (int d = 0; d < 12; d++) {
+    if (q[d] < 0 || q[d] == qd) continue;
+    if constexpr (CAL) {
+      v[2 * q[d]] = 0;
+      v[2 * q[d] + 1] = 0;
+    } else {
+      v[q[d]] = 0;
+    }
+  }
+  const double bder = rel_bder(P, g, i);
+  const double *Gi = P.G[g] + 3 * (int64_t)i;
+  for (int d = 0; d < 12; d++) {
+    if (q[d] < 0) continue;
+    const double t = (rel_coef(rc, bder, Gi, d) * rfac[12 * (int64_t)i + d]) / rdiv[12 * (int64_t)i + d];
+    if constexpr (CAL)
+      v[2 * q[d]] = v[2 * q[d]] + t;
+    else
+      v[q[d]] = v[q[d]] + t;
+  }
+}
+
 template <class T>
 int upload(T **d, const T *h, int64_t n) {
   FDM_HIP(hipMalloc(d, sizeof(T) * (n > 0 ? n : 1)));
@@ -360,7 +445,8 @@ Assembler::~Assembler() {
                   (void *)ow, (void *)ylm, (void *)brad, (void *)lharm, (void *)wso, (void *)dself, (void *)fun,
                   (void *)part, (void *)aq, (void *)alphi, (void *)bq, (void *)phoff, (void *)poff, (void *)yoff,
                   (void *)aa, (void *)aptr, (void *)ianew, (void *)nlmsa, (void *)nbordf, (void *)arow_atom,
-                  (void *)arow_lm, (void *)alw, (void *)poidsa, (void *)ylmato})
+                  (void *)arow_lm, (void *)alw, (void *)poidsa, (void *)ylmato, (void *)rq, (void *)rfac,
+                  (void *)rdiv, (void *)ofac, (void *)opoint, (void *)odir, (void *)apoint, (void *)adir})
     if (p) (void)hipFree(p);
 }
 
@@ -528,11 +614,78 @@ int Assembler::init_atoms(const fdmgpu_atom_tables *at, int64_t natrow, const in
   return 0;
 }
 
+int Assembler::init_rel(const fdmgpu_rel_tables *t) {
+  if (rel) {
+    set_error("fdmgpu_assembler_set_relativistic: the assembler already holds relativistic tables");
+    return FDMGPU_EINVAL;
+  }
+  if (t->npoint != npoint || t->nrow_out != nrow_out || t->nlink != nlink || t->nar != nar || t->nal != nal ||
+      !t->rq || !t->rfac || !t->rdiv || (nrow_out > 0 && (!t->opoint || !t->odir || !t->ofac)) ||
+      (nar > 0 && (!t->apoint || !t->adir))) {
+    set_error("fdmgpu_assembler_set_relativistic: table sizes differ from the assembler's, or a null table");
+    return FDMGPU_EINVAL;
+  }
+  // every position and index the kernels use lies in its array; a target
+  // lies in its point's row, positions are written only by their point
+  std::vector<int64_t> rp(nligne + 1), qd(npoint);
+  FDM_HIP(hipMemcpy(rp.data(), row_ptr, sizeof(int64_t) * (nligne + 1), hipMemcpyDeviceToHost));
+  FDM_HIP(hipMemcpy(qd.data(), qdiag, sizeof(int64_t) * npoint, hipMemcpyDeviceToHost));
+  for (int i = 0; i < npoint; i++)
+    for (int d = 0; d < 12; d++) {
+      const int64_t q = t->rq[12 * (int64_t)i + d];
+      if (q < 0) continue;
+      const int64_t qq = qd[i];
+      int row = -1;  // the point's row: the one holding its diagonal
+      if (qq >= 0) row = (int)(std::upper_bound(rp.begin(), rp.end(), qq) - rp.begin()) - 1;
+      if (qq < 0 || q < rp[row] || q >= rp[row + 1] || !(t->rdiv[12 * (int64_t)i + d] > 0)) {
+        set_error("fdmgpu_assembler_set_relativistic: entry of point " + std::to_string(i) + " outside its row");
+        return FDMGPU_EINVAL;
+      }
+    }
+  for (int r = 0; r < nrow_out; r++)
+    if (t->opoint[r] < 0 || t->opoint[r] >= npoint) {
+      set_error("fdmgpu_assembler_set_relativistic: opoint out of range");
+      return FDMGPU_EINVAL;
+    }
+  for (int k = 0; k < nlink; k++)
+    if (t->odir[k] < 0 || t->odir[k] > 11) {
+      set_error("fdmgpu_assembler_set_relativistic: odir out of range");
+      return FDMGPU_EINVAL;
+    }
+  for (int r = 0; r < nar; r++)
+    if (t->apoint[r] < 0 || t->apoint[r] >= npoint) {
+      set_error("fdmgpu_assembler_set_relativistic: apoint out of range");
+      return FDMGPU_EINVAL;
+    }
+  for (int k = 0; k < nal; k++)
+    if (t->adir[k] < 0 || t->adir[k] > 11) {
+      set_error("fdmgpu_assembler_set_relativistic: adir out of range");
+      return FDMGPU_EINVAL;
+    }
+  const int64_t n12 = 12 * (int64_t)npoint;
+  if (upload(&rq, t->rq, n12) || upload(&rfac, t->rfac, n12) || upload(&rdiv, t->rdiv, n12) ||
+      upload(&opoint, t->opoint, nrow_out) || upload(&odir, t->odir, nlink) || upload(&ofac, t->ofac, nlink) ||
+      upload(&apoint, t->apoint, nar) || upload(&adir, t->adir, nal))
+    return FDMGPU_EHIP;
+  for (int v = 0; v < 13; v++) {
+    relc.clapl[v] = t->clapl[v];
+    relc.cgrad[v] = t->cgrad[v];
+  }
+  rel = true;
+  return 0;
+}
+
 int Assembler::run(int count, const AsmPoints &P, bool e_comp, bool basereel, bool update_only, bool phi_cplx) {
   if (count < 1 || count > max_points) {
     set_error("fdmgpu_assembler_run: count must be in 1.." + std::to_string(max_points));
     return FDMGPU_EINVAL;
   }
+  for (int k = 0; k < count; k++)
+    if (rel != (P.G[k] != nullptr)) {
+      set_error(rel ? "fdmgpu_assembler_run: relativistic rows need the potential's gradient (fdmgpu_assembler_run_rel)"
+                    : "fdmgpu_assembler_run_rel: the assembler holds no relativistic tables");
+      return FDMGPU_EINVAL;
+    }
   const bool cal = e_comp || !basereel;
   if (phi_cplx && !cal) {
     set_error("fdmgpu_assembler_run_atoms: complex phiato needs complex arithmetic (E_comp or !basereel)");
@@ -553,31 +706,42 @@ int Assembler::run(int count, const AsmPoints &P, bool e_comp, bool basereel, bo
   if (cal) {
     if (!update_only) hipLaunchKernelGGL(k_asm_base<true>, dim3(gb, count), dim3(NT), 0, stream, nnz, nrhs_el, base, P);
     hipLaunchKernelGGL(k_asm_diag<true>, dim3(gd, count), dim3(NT), 0, stream, npoint, qdiag, clapl0, dself, P,
-                       (int)e_comp);
+                       (int)e_comp, (int)rel);
+    if (rel)
+      hipLaunchKernelGGL(k_asm_rel<true>, dim3(gd, count), dim3(NT), 0, stream, npoint, qdiag, rq, rfac, rdiv, relc, P);
+    auto oc = rel ? k_asm_outcols<true, true> : k_asm_outcols<true, false>;
     if (nrow_out > 0)
-      hipLaunchKernelGGL(k_asm_outcols<true>, dim3(go, count), dim3(NT), 0, stream, nrow_out, nlmso, nb, lmaxso, orow,
-                         oq, optr, ob, ow, ylm, lharm, F, P, (int)basereel);
+      hipLaunchKernelGGL(oc, dim3(go, count), dim3(NT), 0, stream, nrow_out, nlmso, nb, lmaxso, orow, oq, optr, ob, ow,
+                         ylm, lharm, F, P, (int)basereel, opoint, odir, ofac, relc);
     hipLaunchKernelGGL(k_asm_outrows_part, gp, dim3(256), 0, stream, nlmso, nb, lmaxso, nsortf, wso, ylm, lharm,
                        F, (int)basereel, Q);
     hipLaunchKernelGGL(k_asm_outrows_sum<true>, gs, dim3(256), 0, stream, nso, nlmso, nsortf, nrun, row_ptr, Q, P);
   } else {
     if (!update_only) hipLaunchKernelGGL(k_asm_base<false>, dim3(gb, count), dim3(NT), 0, stream, nnz, nrhs_el, base, P);
-    hipLaunchKernelGGL(k_asm_diag<false>, dim3(gd, count), dim3(NT), 0, stream, npoint, qdiag, clapl0, dself, P, 0);
+    hipLaunchKernelGGL(k_asm_diag<false>, dim3(gd, count), dim3(NT), 0, stream, npoint, qdiag, clapl0, dself, P, 0,
+                       (int)rel);
+    if (rel)
+      hipLaunchKernelGGL(k_asm_rel<false>, dim3(gd, count), dim3(NT), 0, stream, npoint, qdiag, rq, rfac, rdiv, relc,
+                         P);
+    auto oc = rel ? k_asm_outcols<false, true> : k_asm_outcols<false, false>;
     if (nrow_out > 0)
-      hipLaunchKernelGGL(k_asm_outcols<false>, dim3(go, count), dim3(NT), 0, stream, nrow_out, nlmso, nb, lmaxso, orow,
-                         oq, optr, ob, ow, ylm, lharm, F, P, 1);
+      hipLaunchKernelGGL(oc, dim3(go, count), dim3(NT), 0, stream, nrow_out, nlmso, nb, lmaxso, orow, oq, optr, ob, ow,
+                         ylm, lharm, F, P, 1, opoint, odir, ofac, relc);
     hipLaunchKernelGGL(k_asm_outrows_part, gp, dim3(256), 0, stream, nlmso, nb, lmaxso, nsortf, wso, ylm, lharm,
                        F, 1, Q);
     hipLaunchKernelGGL(k_asm_outrows_sum<false>, gs, dim3(256), 0, stream, nso, nlmso, nsortf, nrun, row_ptr, Q, P);
   }
   if (natome > 0) {
     const dim3 gc((nar * nlmax + NT - 1) / NT, count), gr((nrow_atom * nlmax + NT - 1) / NT, count);
-    auto cols = cal ? (phi_cplx ? k_asm_atomcols<true, true> : k_asm_atomcols<true, false>)
-                    : k_asm_atomcols<false, false>;
+    auto cols = rel ? (cal ? (phi_cplx ? k_asm_atomcols<true, true, true> : k_asm_atomcols<true, false, true>)
+                           : k_asm_atomcols<false, false, true>)
+                    : (cal ? (phi_cplx ? k_asm_atomcols<true, true, false> : k_asm_atomcols<true, false, false>)
+                           : k_asm_atomcols<false, false, false>);
     auto rows = cal ? (phi_cplx ? k_asm_atomrows<true, true> : k_asm_atomrows<true, false>)
                     : k_asm_atomrows<false, false>;
     if (nar > 0)
-      hipLaunchKernelGGL(cols, gc, dim3(NT), 0, stream, nar, nlmax, aq, aa, aptr, alphi, alw, nlmsa, P);
+      hipLaunchKernelGGL(cols, gc, dim3(NT), 0, stream, nar, nlmax, aq, aa, aptr, alphi, alw, nlmsa, P, apoint, adir,
+                         relc);
     hipLaunchKernelGGL(rows, gr, dim3(NT), 0, stream, nrow_atom, nlmax, arow_atom, arow_lm, bq, nlmsa, nbordf, phoff,
                        poff, yoff, poidsa, ylmato, P);
   }

@@ -14,6 +14,7 @@ namespace {
 
 typedef std::complex<double> cd;
 constexpr double kPi = 3.1415926535897932384626433832;
+constexpr double kAlfaSf = 0.0072973525664;  // fine-structure constant (main.f90:58)
 constexpr double kEpsPos = 1e-4;  // tolerance on point positions (main.f90:83)
 
 struct GridPoint {
@@ -122,6 +123,7 @@ struct fdmgpu_cluster {
   std::vector<int64_t> rp;     // compressed-row structure, nligne + 1
   std::vector<int> cols;       // 1-based
   double clapl[13] = {0};
+  double cgrad[13] = {0};  // gradient stencil (laplac), 1..12 in kDir order
   // Muffin-tin atoms (point group P1 only): positions (bohr, absorber at
   // the origin, sorted by distance), sphere radii, expansions
   int natome = 0, npr = 0;
@@ -475,6 +477,10 @@ int build(fdmgpu_cluster *c) {
     s += c->clapl[v];
   }
   c->clapl[0] = -s;
+  // gradient coefficients (general.f90:9791-9822): is * coeford * cd with
+  // is = -1, +1 along each axis, coeford 1 and -1/8 for the two shells
+  const double cdg = (4.0 / 3.0) / (2 * c->adimp);
+  for (int v = 1; v <= 12; v++) c->cgrad[v] = ((v & 1) ? -1.0 : 1.0) * (v <= 6 ? 1.0 : -1.0 / 8) * cdg;
   // Points inside an atom's sphere (reseau, general.f90:9563-9577; the
   // distance to the atom, the identity being the only operation)
   c->numia.assign(np, 0);
@@ -603,6 +609,32 @@ inline int lof(int lm) {  // l of the harmonic index l*l + l + m
 // rvol(i) (calcMatRow, mat.f90:1164-1222; -clapl(v) without symmetry)
 inline double stencil(const fdmgpu_cluster *c, int i, int d) {
   return (character(c->irrep, c->flip[(size_t)i * 12 + d]) * -c->clapl[d + 1]) * c->rvol[i];
+}
+
+// Scalar-relativistic terms of grid point i (Relativiste, mat.f90:1155-1156,
+// :1171-1185): with the potential's gradient G (npoint x 3, point-major),
+// the diagonal loses a2s4 (V - E)^2 and the stencil term of direction d
+// gains bder cgrad(v) dV/dx_k (k the axis of d), bder = -a2s4 / (1 - a2s4
+// (V - E)), a2s4 = alpha^2 / 4; without G both are the plain terms.
+struct RelPoint {
+  double diag;  // V - E - clapl(0) (- a2s4 (V - E)^2)
+  double bder;
+};
+inline RelPoint rel_point(const fdmgpu_cluster *c, double V, double Enervide, const double *G) {
+  const double vme = V - Enervide;
+  RelPoint r{vme - c->clapl[0], 0.0};
+  if (G) {
+    const double a2s4 = 0.25 * (kAlfaSf * kAlfaSf);
+    r.diag = r.diag - a2s4 * (vme * vme);
+    r.bder = -a2s4 / (1 - a2s4 * vme);
+  }
+  return r;
+}
+// stencil term with the relativistic gradient term (the plain term without G)
+inline double stencil_rel(const fdmgpu_cluster *c, int i, int d, double bder, const double *G) {
+  if (!G) return stencil(c, i, d);
+  const double cf = -c->clapl[d + 1] + bder * c->cgrad[d + 1] * G[(size_t)i * 3 + (d >> 1) % 3];
+  return (character(c->irrep, c->flip[(size_t)i * 12 + d]) * cf) * c->rvol[i];
 }
 
 // Outer-sphere functions at every boundary point (phiso, non-Rydberg branch,
@@ -860,7 +892,15 @@ int64_t fdmgpu_cluster_nnz(const fdmgpu_cluster *c) { return (int64_t)c->cols.si
 int fdmgpu_cluster_assemble(const fdmgpu_cluster *c, const double *V, double Enervide, double Eimag,
                             double Ecinetic_out, int E_comp, int basereel, int64_t *row_ptr, int *col, double *re,
                             double *im, double *rhs_re, double *rhs_im) {
+  return fdmgpu_cluster_assemble_rel(c, V, nullptr, Enervide, Eimag, Ecinetic_out, E_comp, basereel, row_ptr, col, re,
+                                     im, rhs_re, rhs_im);
+}
+
+int fdmgpu_cluster_assemble_rel(const fdmgpu_cluster *c, const double *V, const double *gradvr, double Enervide,
+                                double Eimag, double Ecinetic_out, int E_comp, int basereel, int64_t *row_ptr,
+                                int *col, double *re, double *im, double *rhs_re, double *rhs_im) {
   if (c->natome > 0) return -2;  // fdmgpu_cluster_assemble_atoms
+  const double *G = gradvr;
   const bool cal = E_comp || !basereel;
   const int np = c->npoint, nl = c->nlmso, nb = (int)c->bnd.size(), L = c->lmaxso;
   std::vector<cd> B((size_t)nb * (L + 1)), N((size_t)nb * (L + 1));
@@ -881,7 +921,8 @@ int fdmgpu_cluster_assemble(const fdmgpu_cluster *c, const double *V, double Ene
     const int64_t q0 = c->rp[i], q1 = c->rp[i + 1];
     std::fill(row.begin(), row.end(), cd(0));
     std::fill(rrow.begin(), rrow.end(), cd(0));
-    const cd diag(V[i] - Enervide - c->clapl[0], E_comp ? -Eimag : 0.0);  // mat.f90:1147-1150
+    const RelPoint rp = rel_point(c, V[i], Enervide, G);
+    const cd diag(rp.diag, E_comp ? -Eimag : 0.0);  // mat.f90:1147-1150, :1155
     // stencil term of direction d: -clapl(v) times the character of the
     // mirrors that bring the neighbour into the wedge, times rvol(i)
     // (mat.f90:1164-1222; 1 and 1 without symmetry)
@@ -893,7 +934,7 @@ int fdmgpu_cluster_assemble(const fdmgpu_cluster *c, const double *V, double Ene
       const int j = col[q] - 1;
       cd v = (j == i) ? diag : cd(0);
       for (int d = 0; d < 12; d++)
-        if (c->nbr[(size_t)i * 12 + d] == j) v += stencil(c, i, d) / c->rvol[j];
+        if (c->nbr[(size_t)i * 12 + d] == j) v += stencil_rel(c, i, d, rp.bder, G) / c->rvol[j];
       re[q] = v.real();
       if (cal) im[q] = v.imag();
     }
@@ -901,7 +942,7 @@ int fdmgpu_cluster_assemble(const fdmgpu_cluster *c, const double *V, double Ene
       const int j = c->nbr[(size_t)i * 12 + d];
       if (j < np) continue;
       const int b = c->slot[j];
-      const double w = stencil(c, i, d);
+      const double w = stencil_rel(c, i, d, rp.bder, G);
       for (int lm = 0; lm < nl; lm++) {
         const int l = lof(c->lmidx[lm]);
         const double y = c->ylm[(size_t)b * nl + lm];
@@ -960,6 +1001,16 @@ int fdmgpu_cluster_assemble_atoms_ri(const fdmgpu_cluster *c, const double *V, d
                                      const double *phi_im, const double *phi2_re, const double *phi2_im,
                                      int64_t *row_ptr, int *col, double *re, double *im, double *rhs_re,
                                      double *rhs_im) {
+  return fdmgpu_cluster_assemble_atoms_rel(c, V, nullptr, Enervide, Eimag, Ecinetic_out, E_comp, basereel, phi_re,
+                                           phi_im, phi2_re, phi2_im, row_ptr, col, re, im, rhs_re, rhs_im);
+}
+
+int fdmgpu_cluster_assemble_atoms_rel(const fdmgpu_cluster *c, const double *V, const double *gradvr,
+                                      double Enervide, double Eimag, double Ecinetic_out, int E_comp, int basereel,
+                                      const double *phi_re, const double *phi_im, const double *phi2_re,
+                                      const double *phi2_im, int64_t *row_ptr, int *col, double *re, double *im,
+                                      double *rhs_re, double *rhs_im) {
+  const double *G = gradvr;
   const bool cal = E_comp || !basereel;
   if (!c || !V || !phi_re || !row_ptr || !col || !re || (cal && !im) || (phi_im && !cal)) return -1;
   // the irregular solution exists with complex energy only (FDM_comp), and
@@ -997,10 +1048,11 @@ int fdmgpu_cluster_assemble_atoms_ri(const fdmgpu_cluster *c, const double *V, d
     q1 = c->rp[r + 1];
     if (c->rkind[r] == 0) {  // grid row, mat.f90:1142-1398 (P1: rvol = 1, no character)
       const int i = c->ridx[r];
-      add(r + 1, V[i] - Enervide - c->clapl[0], E_comp ? -Eimag : 0.0);
+      const RelPoint rp = rel_point(c, V[i], Enervide, G);
+      add(r + 1, rp.diag, E_comp ? -Eimag : 0.0);
       for (int d = 0; d < 12; d++) {
         const int j = c->nbr[(size_t)i * 12 + d];
-        const double w = -c->clapl[d + 1];
+        const double w = stencil_rel(c, i, d, rp.bder, G);
         if (j < np && !c->numia[j]) {  // :1227-1232
           add(c->row[j] + 1, w, 0.0);
         } else if (j < np) {  // a neighbour inside atom a couples to its expansion, :1233-1309
@@ -1165,6 +1217,65 @@ int fdmgpu_cluster_assembly_weights(const fdmgpu_cluster *c, int *lharm, double 
     s[0] = s[1] = s[2] = 0.0;
     for (int d = 0; d < 12; d++)  // only the second-shell steps towards a mirror can land on the point
       if (c->nbr[(size_t)i * 12 + d] == i) s[(d % 6) / 2] += stencil(c, i, d) / c->rvol[i];
+  }
+  return 0;
+}
+
+int fdmgpu_cluster_rel_tables(const fdmgpu_cluster *c, int64_t *rq, double *rfac, double *rdiv, int *opoint,
+                              int *odir, double *ofac, int *apoint, int *adir, double *clapl, double *cgrad) {
+  if (!c || !rq || !rfac || !rdiv || !clapl || !cgrad) return -1;
+  if (c->irregular) return -2;  // the device tables describe regular solutions
+  const int np = c->npoint;
+  int r = 0, k = 0, n = 0, ka = 0;
+  for (int i = 0; i < np; i++) {
+    for (int d = 0; d < 12; d++) {
+      rq[12 * (size_t)i + d] = -1;
+      rfac[12 * (size_t)i + d] = 0;
+      rdiv[12 * (size_t)i + d] = 1;
+    }
+    if (c->numia[i]) continue;
+    const double chi_r = c->rvol[i];
+    bool out = false;
+    for (int d = 0; d < 12; d++) {  // the grid row's own entries, as in the assemblers
+      const int j = c->nbr[(size_t)i * 12 + d];
+      const double f = character(c->irrep, c->flip[(size_t)i * 12 + d]) * chi_r;
+      if (j < np && !c->numia[j]) {
+        rq[12 * (size_t)i + d] = position(c, c->row[i], c->row[j] + 1);
+        rfac[12 * (size_t)i + d] = f;
+        rdiv[12 * (size_t)i + d] = c->natome ? 1.0 : c->rvol[j];
+      } else if (j >= np) {  // the outer-column links, in the assembly tables' order
+        if (opoint && odir && ofac) {
+          odir[k] = d;
+          ofac[k] = f;
+        }
+        k++;
+        out = true;
+      }
+    }
+    if (out && opoint) opoint[r] = i;
+    r += out;
+    if (c->natome == 0) continue;
+    std::vector<int> atoms;  // the atom couplings, in the atom tables' order
+    for (int d = 0; d < 12; d++) {
+      const int j = c->nbr[(size_t)i * 12 + d];
+      if (j < np && c->numia[j] && std::find(atoms.begin(), atoms.end(), c->numia[j]) == atoms.end())
+        atoms.push_back(c->numia[j]);
+    }
+    for (int a : atoms) {
+      for (int d = 0; d < 12; d++) {
+        const int j = c->nbr[(size_t)i * 12 + d];
+        if (j < np && c->numia[j] == a) {
+          if (adir) adir[ka] = d;
+          ka++;
+        }
+      }
+      if (apoint) apoint[n] = i;
+      n++;
+    }
+  }
+  for (int v = 0; v < 13; v++) {
+    clapl[v] = c->clapl[v];
+    cgrad[v] = c->cgrad[v];
   }
   return 0;
 }

@@ -301,6 +301,43 @@ int fdmgpu_assembler_run_atoms(fdmgpu_assembler *a, int count, const double *Ene
                                double *const *d_vals, double *const *d_rhs);
 
 /*
+ * Scalar-relativistic rows (the reference's Relativiste = T; calcMatRow,
+ * mat.f90:1155-1156, :1171-1185): the grid diagonal loses a2s4 (V - E)^2 and
+ * every stencil coefficient of grid point i becomes -clapl(v) + bder
+ * cgrad(v) dV/dx_k(i), bder = -a2s4 / (1 - a2s4 (V - E)), a2s4 = alpha^2 / 4,
+ * k the axis of neighbour v; the coefficient then carries the character
+ * and rvol factors as before and enters the interior, outer-sphere and atom
+ * couplings and the incoming wave.  These depend on V and E, so an
+ * assembler given these tables recomputes them per call from the gradient
+ * of the potential (fdmgpu_assembler_run_rel).  fdmgpu_cluster_rel_tables
+ * (fdmgpu_cluster.h) fills the struct; its link order is that of the
+ * assembly and atom tables.
+ */
+typedef struct {
+  int npoint, nrow_out, nlink, nar, nal;
+  double clapl[13];         /* clapl(0:12) */
+  double cgrad[13];         /* cgrad(1:12) at [1..12], laplac's gradient stencil (general.f90:9791-9822) */
+  const int64_t *rq;        /* [npoint*12] position written by neighbour v of point i (its column in i's row, the
+                               diagonal for the point's own image), -1: outside the sphere or inside an atom */
+  const double *rfac;       /* [npoint*12] character of the neighbour's operation times rvol(i) */
+  const double *rdiv;       /* [npoint*12] rvol of the neighbour */
+  const int *opoint;        /* [nrow_out] grid point of each row with outside neighbours */
+  const int *odir;          /* [nlink] direction (0..11) of each outside link */
+  const double *ofac;       /* [nlink] character times rvol(i) */
+  const int *apoint;        /* [nar] grid point of each (row, atom) pair (NULL without atoms) */
+  const int *adir;          /* [nal] direction of each atom link */
+} fdmgpu_rel_tables;
+/* Adds the relativistic tables to an assembler (checked and copied); after
+ * this the assembler is run only by fdmgpu_assembler_run_rel. */
+int fdmgpu_assembler_set_relativistic(fdmgpu_assembler *a, const fdmgpu_rel_tables *t);
+/* As fdmgpu_assembler_run_atoms (d_phi NULL without atoms), with d_G[k] the
+ * device gradient of point k's potential (npoint x 3, point-major, Ry/bohr). */
+int fdmgpu_assembler_run_rel(fdmgpu_assembler *a, int count, const double *Enervide, const double *Eimag,
+                             const double *Ecinetic_out, int E_comp, int basereel, int update_only,
+                             const double *const *d_V, const double *const *d_G, const double *const *d_phi,
+                             int phi_cplx, double *const *d_vals, double *const *d_rhs);
+
+/*
  * Post-solve reduction to the atom amplitude products (mat.f90:297-405,
  * regular solution, nspino = 1, one representation), on solutions already
  * on the device.  Replaces the CPU loop after mat_solve in mat():

@@ -169,6 +169,34 @@ int fdmgpu_cluster_assemble_atoms_ri(const fdmgpu_cluster *c, const double *V, d
                                      const double *phi_im, const double *phi2_re, const double *phi2_im,
                                      int64_t *row_ptr, int *col, double *re, double *im, double *rhs_re,
                                      double *rhs_im);
+/* Scalar-relativistic rows (the reference's Relativiste = T; calcMatRow,
+ * mat.f90:1155-1156, :1171-1185): fdmgpu_cluster_assemble and
+ * fdmgpu_cluster_assemble_atoms_ri with the gradient of the potential at the
+ * interior points, gradvr[npoint * 3] (point-major: dV/dx, dV/dy, dV/dz of
+ * point i at 3 i .. 3 i + 2, Ry/bohr; the reference's gradvr(nicm, 3)).  A
+ * grid row's diagonal loses a2s4 (V - E)^2 and the stencil coefficient of
+ * neighbour v, before the character and rvol factors, becomes
+ * -clapl(v) + bder cgrad(v) gradvr(i, k) with bder = -a2s4 / (1 - a2s4
+ * (V - E)), a2s4 = alpha^2 / 4 (alpha = 0.0072973525664), cgrad laplac's
+ * gradient stencil (general.f90:9791-9822) and k the axis of v; every use of
+ * the coefficient (interior, outer-sphere and atom neighbours, the incoming
+ * wave) takes the new value.  gradvr = NULL gives the plain rows. */
+int fdmgpu_cluster_assemble_rel(const fdmgpu_cluster *c, const double *V, const double *gradvr, double Enervide,
+                                double Eimag, double Ecinetic_out, int E_comp, int basereel, int64_t *row_ptr,
+                                int *col, double *re, double *im, double *rhs_re, double *rhs_im);
+int fdmgpu_cluster_assemble_atoms_rel(const fdmgpu_cluster *c, const double *V, const double *gradvr,
+                                      double Enervide, double Eimag, double Ecinetic_out, int E_comp, int basereel,
+                                      const double *phi_re, const double *phi_im, const double *phi2_re,
+                                      const double *phi2_im, int64_t *row_ptr, int *col, double *re, double *im,
+                                      double *rhs_re, double *rhs_im);
+/* The device assembler's tables of the relativistic rows
+ * (fdmgpu_rel_tables, fdmgpu.h): rq, rfac, rdiv [npoint*12]; opoint
+ * [nrow_out], odir, ofac [nlink] (sizes of fdmgpu_cluster_assembly_sizes);
+ * apoint [nar], adir [nal] (fdmgpu_cluster_atom_assembly_sizes; NULL without
+ * atoms); clapl, cgrad [13].  Links in the order of the assembly and atom
+ * tables.  Returns -2 for an irregular cluster. */
+int fdmgpu_cluster_rel_tables(const fdmgpu_cluster *c, int64_t *rq, double *rfac, double *rdiv, int *opoint,
+                              int *odir, double *ofac, int *apoint, int *adir, double *clapl, double *cgrad);
 
 /* E-independent tables of one structure's per-energy assembly: the parts of
  * calcMatRow (mat.f90:1142-1232 grid rows, :1312-1398 outer columns,

@@ -11,7 +11,8 @@
  *   cbessneu  sphere.f90:7226-7248, cbessneur sphere.f90:7356-7378
  *   newind    mat.f90:1761-1921 (atom blocks :1790-1810, :1836-1860)
  *   calcMatRow mat.f90:1034-1632 (grid rows :1142-1398, atom neighbours
- *             :1233-1309, outer rows :1401-1511, atom rows :1515-1632)
+ *             :1233-1309, outer rows :1401-1511, atom rows :1515-1632;
+ *             scalar-relativistic terms :1155-1156, :1171-1185)
  *   mat_solve mat_solve_gaussian.f90:67-358
  * Constants follow the reference's module declarations (main.f90:36-83).
  */
@@ -31,6 +32,7 @@ static const double EPS6 = 1.e-6;
 static const double EPS10 = 1.e-10;
 static const double EPS15 = 1.e-15;
 static const double EPSPOS = 1.e-4;
+static const double ALFA_SF = 0.0072973525664; /* fine-structure constant, main.f90:58 */
 
 struct orc_cluster {
   double rsort, adimp;
@@ -63,7 +65,14 @@ struct orc_cluster {
    * solutions (i_RI = 2 of phiato) */
   int irregular, n_ato_RI;
   double *phi2_re, *phi2_im;
+  /* scalar-relativistic rows (Relativiste, mat.f90:1155-1156, :1171-1185):
+   * gradient stencil cgrad(1:nvois) (laplac) and the potential's gradient
+   * gradvr(npoint, 3) set by orc_cluster_set_relativistic */
+  double cgrad[13];
+  int relativiste;
+  double *gradvr;
 };
+#define GRADVR(c, i, k) ((c)->gradvr[((k)-1) * (size_t)(c)->npoint + (i)-1])
 #define IBORD(c, ib, ia) ((c)->ibord[((ia)-1) * (c)->nbtm + (ib)-1])
 #define POIDSA(c, ib, ia) ((c)->poidsa[((ia)-1) * (c)->nbtm + (ib)-1])
 #define YLMATO(c, ib, lm, ia) ((c)->ylmato[(((size_t)(ia)-1) * (c)->nlmmax + (lm)-1) * (c)->nbtm + (ib)-1])
@@ -178,6 +187,22 @@ static int orc_laplac(orc_cluster *c) {
   double s = 0;
   for (int k = 1; k <= c->nvois; k++) s += c->clapl[k];
   c->clapl[0] = -s;
+  /* gradient coefficients, general.f90:9791-9822 (cubic grid) */
+  double cd;
+  coeford[0] = 1.0;
+  if (c->iord == 2) {
+    cd = 1 / (2 * c->adimp);
+  } else {
+    cd = (4.0 / 3.0) / (2 * c->adimp);
+    coeford[1] = -1.0 / 8;
+  }
+  iv = 0;
+  for (int io = 1; io <= c->iord / 2; io++)
+    for (int i = 1; i <= 3; i++)
+      for (int is = -1; is <= 1; is += 2) {
+        iv++;
+        c->cgrad[iv] = is * coeford[io - 1] * cd;
+      }
   /* mpres(-nx:nx,...) */
   int *mpres = calloc((size_t)w * w * w, sizeof(int));
   if (!mpres) return -1;
@@ -596,6 +621,7 @@ void orc_cluster_free(orc_cluster *c) {
   free(c->xyz); free(c->indice); free(c->numia); free(c->rvol); free(c->ivois);
   free(c->isrt); free(c->poidso); free(c->ylmso); free(c->lso); free(c->mso); free(c->iso);
   free(c->newp); free(c->lb1); free(c->lb2); free(c->newinv);
+  free(c->gradvr);
   free(c->posi); free(c->rmt); free(c->lmaxa); free(c->nlmsa); free(c->ianew); free(c->nbord); free(c->nbordf);
   free(c->ibord); free(c->poidsa); free(c->ylmato); free(c->phi_re); free(c->phi_im);
   free(c->phi2_re); free(c->phi2_im);
@@ -640,6 +666,19 @@ int orc_cluster_set_phiato_ri(orc_cluster *c, int i_ri, int cplx, const double *
       }
   return 0;
 }
+int orc_cluster_set_relativistic(orc_cluster *c, int on, const double *gradvr) {
+  free(c->gradvr);
+  c->gradvr = NULL;
+  c->relativiste = 0;
+  if (!on) return 0;
+  if (!gradvr) return -1;
+  c->gradvr = malloc(sizeof(double) * 3 * (size_t)c->npoint);
+  if (!c->gradvr) return -1;
+  memcpy(c->gradvr, gradvr, sizeof(double) * 3 * (size_t)c->npoint);
+  c->relativiste = 1;
+  return 0;
+}
+void orc_cluster_cgrad(const orc_cluster *c, double *cg) { memcpy(cg, c->cgrad + 1, sizeof(double) * c->nvois); }
 int orc_cluster_set_phiato(orc_cluster *c, int cplx, const double *re, const double *im) {
   return orc_cluster_set_phiato_ri(c, 1, cplx, re, im);
 }
@@ -766,12 +805,24 @@ static void orc_calcmatrow(const row_ctx *x, int ii, double *abvr, double *abvi,
     double vme = x->V[i - 1] - x->Enervide;
     abvr[ii] = vme - c->clapl[0];
     if (x->E_comp) abvi[ii] = -x->Eimag;
+    /* :1155-1156 (a2s4 :1123) */
+    const double a2s4 = 0.25 * (ALFA_SF * ALFA_SF);
+    double bder = 0;
+    if (c->relativiste) {
+      abvr[ii] = abvr[ii] - a2s4 * (vme * vme);
+      bder = -a2s4 / (1 - a2s4 * vme);
+    }
     for (int iv = 1; iv <= c->nvois; iv++) {
       int j = IVOIS(c, i, iv);
       int ia = c->numia[j - 1];
       /* :1164-1232 with Kar = 1, Kari = 0, Repres_comp = F */
       double cfvs_r = -c->clapl[iv];
       double cfvs_i = 0.0;
+      if (c->relativiste) { /* :1171-1185, cubic grid: k = x, y, z of the pair */
+        const int k = 1 + ((iv - 1) / 2) % 3;
+        double crelat = bder * c->cgrad[iv] * GRADVR(c, i, k);
+        cfvs_r = cfvs_r + crelat;
+      }
       cfvs_r = 1.0 * cfvs_r;
       cfvs_r = cfvs_r * c->rvol[i - 1];
       if (cal) cfvs_i = cfvs_i * c->rvol[i - 1];

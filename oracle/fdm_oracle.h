@@ -64,6 +64,13 @@ int orc_cluster_set_phiato(orc_cluster *c, int cplx, const double *re, const dou
 /* i_ri = 1: the regular atomic solutions (as orc_cluster_set_phiato); 2: the
  * irregular ones (phiato(.., i_RI = 2, ..)) of the irregular cluster solution */
 int orc_cluster_set_phiato_ri(orc_cluster *c, int i_ri, int cplx, const double *re, const double *im);
+/* Scalar-relativistic rows (Relativiste = T; mat.f90:1155-1156,
+ * :1171-1185): on != 0 with gradvr(npoint, 3) column-major (the gradient of
+ * V at the interior points, gradvr(nicm, 3) of the reference, Ry/bohr);
+ * on = 0 switches them off.  Returns 0, or -1 (no gradvr / memory). */
+int orc_cluster_set_relativistic(orc_cluster *c, int on, const double *gradvr);
+/* cgrad(1:nvois), laplac's gradient stencil (general.f90:9791-9822) */
+void orc_cluster_cgrad(const orc_cluster *c, double *cgrad);
 void orc_cluster_free(orc_cluster *c);
 
 /* sizes: [0]=npoint [1]=npso [2]=nligne [3]=nlmso [4]=nsortf [5]=nsort [6]=nvois */

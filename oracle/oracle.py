@@ -47,6 +47,8 @@ def lib():
                                                C.c_int]
         L.orc_cluster_set_phiato_ri.restype = C.c_int
         L.orc_cluster_set_phiato_ri.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, C.c_void_p]
+        L.orc_cluster_set_relativistic.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        L.orc_cluster_cgrad.argtypes = [C.c_void_p, _dp]
         L.orc_cluster_free.argtypes = [C.c_void_p]
         L.orc_cluster_sizes.argtypes = [C.c_void_p, _ip]
         L.orc_cluster_xyz.argtypes = [C.c_void_p, _dp]
@@ -166,6 +168,23 @@ class Cluster:
             im = np.ascontiguousarray(np.imag(flat), np.float64) if cplx else None
             if lib().orc_cluster_set_phiato_ri(self._h, i_ri, int(cplx), re, _ptr(im)):
                 raise MemoryError("orc_cluster_set_phiato failed")
+
+    def set_relativistic(self, gradvr):
+        """Scalar-relativistic rows (Relativiste, mat.f90:1155-1156,
+        :1171-1185) with gradvr (npoint, 3), the potential's gradient at the
+        interior points (Ry/bohr); None switches them off."""
+        if gradvr is None:
+            lib().orc_cluster_set_relativistic(self._h, 0, None)
+            return
+        g = np.asfortranarray(np.asarray(gradvr, np.float64).reshape(self.npoint, 3))
+        if lib().orc_cluster_set_relativistic(self._h, 1, g.ctypes.data_as(C.c_void_p)):
+            raise MemoryError("orc_cluster_set_relativistic failed")
+
+    @property
+    def cgrad(self):
+        a = np.zeros(self.nvois)
+        lib().orc_cluster_cgrad(self._h, a)
+        return a
 
     @property
     def xyz(self):

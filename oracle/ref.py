@@ -145,11 +145,14 @@ def phiso(cl, E_comp: bool, Ecinetic_out: float, Eimag: float, Enervide: float =
     return (bes, neu) if E_comp else (besr, neur)
 
 
-def mat_solve(cl, V, Enervide, Eimag, bes, neu, E_comp, cal_comp, basereel, rhs=None, fn=None):
+def mat_solve(cl, V, Enervide, Eimag, bes, neu, E_comp, cal_comp, basereel, rhs=None, fn=None, gradvr=None,
+              cgrad=None):
     """Reference mat_solve (93 arguments, mat_solve_gaussian.f90:5-12) with
     the reference calcMatRow assembling each row.  Returns the solution
     (nligne, nlmso), real or complex.  ``fn`` calls another procedure with
-    the same argument surface instead (a drop-in replacement)."""
+    the same argument surface instead (a drop-in replacement).  With gradvr
+    (npoint, 3) and cgrad (nvois,) the rows are scalar-relativistic
+    (Relativiste = T, nicm = npoint)."""
     A = _Args()
     n, nl, ns, lm = cl.nligne, cl.nlmso, cl.nsort, cl.lmaxso
     lb1, lb2, newinv, newp = cl.profile()
@@ -180,21 +183,25 @@ def mat_solve(cl, V, Enervide, Eimag, bes, neu, E_comp, cal_comp, basereel, rhs=
     tf = np.zeros(1)
     tt = np.zeros(1)
     p = lambda a: a.ctypes.data_as(C.c_void_p)
+    rel = gradvr is not None
+    gv = np.asfortranarray(np.asarray(gradvr, np.float64).reshape(cl.npoint, 3)) if rel else np.zeros(3)
+    cg = np.asarray(cgrad, np.float64) if rel else np.zeros(nvois)
     args = [
-        A.b(False), A.b(basereel), p(Bc), p(Br), A.b(cal_comp), A.arr(np.zeros(nvois), np.float64),
-        A.arr(cl.clapl, np.float64), A.b(E_comp), A.d(Eimag), A.d(Enervide), A.arr(np.zeros(3), np.float64),
+        A.b(False), A.b(basereel), p(Bc), p(Br), A.b(cal_comp), A.arr(cg, np.float64),
+        A.arr(cl.clapl, np.float64), A.b(E_comp), A.d(Eimag), A.d(Enervide), p(gv),
         A.arr([0], np.int32), A.arr([0], np.int32), A.arr([0], np.int32), A.i(0), A.i(1), A.i(1), A.i(0),
         A.b(False), A.arr([0], np.int32), A.arr(iso, np.int32), A.i(1), A.arr(isrt, np.int32),
         A.arr(isvois, np.int32), A.arr(cl.ivois, np.int32), A.arr(kar, np.float64), A.arr(kari, np.float64),
         A.arr([0], np.int32), A.arr(lb1, np.int32), A.arr(lb2, np.int32), A.i(lm), A.arr(lso, np.int32),
         A.arr([0], np.int32), A.i(1), A.i(0), A.arr(mso, np.int32), A.i(1), A.i(1), A.i(0), A.i(1),
         A.arr([0], np.int32), A.arr([0], np.int32), A.i(1), p(Nc), p(Nr), A.arr(newp, np.int32),
-        A.arr(newinv, np.int32), A.i(1), A.i(1), A.i(cl.npoint), A.i(n), A.i(n if cal_comp else 0),
+        A.arr(newinv, np.int32), A.i(1), A.i(cl.npoint if rel else 1), A.i(cl.npoint), A.i(n),
+        A.i(n if cal_comp else 0),
         A.i(cl.nligneso), A.i(1), A.i(1), A.i(1), A.i(nl), A.arr([0], np.int32), A.i(nl),
         A.i(nl if cal_comp else 0), A.i(1), A.i(1), A.i(cl.npoint), A.i(cl.npso), A.i(max(cl.nsortf, 1)),
         A.i(nl), A.i(ns), A.i(nsc), A.i(nsr), A.i(cl.nsortf), A.i(1), A.i(1), A.i(1), A.i(1), A.i(ns),
         A.arr(numia, np.int32), A.i(nvois), A.arr(np.zeros(1), np.float64), A.arr(np.zeros(1), np.float64),
-        A.arr(poidso, np.float64), A.b(False), A.b(False), A.arr(np.ones(cl.npoint), np.float64), p(smi), p(smr),
+        A.arr(poidso, np.float64), A.b(rel), A.b(False), A.arr(np.ones(cl.npoint), np.float64), p(smi), p(smr),
         A.b(False), A.i(0), p(tf), p(tt), A.arr(np.asarray(V, np.float64), np.float64), A.b(False),
         A.arr(np.zeros(1), np.float64), A.arr(cl.ylmso, np.float64),
     ]
@@ -365,7 +372,8 @@ def atom_system(rsort: float, adimp: float, lmaxso: int, posi, rmt, lmax, iord: 
     return dict(rsort=rsort, adimp=adimp, npoint=npoint, npr=npr, npso=npso, npsom=npsom, natome=natome, posi=posi,
                 rmt=rmt, lmax=lmax,
                 lmaxso=lmaxso, nlmso=nlmso, nligne=nligne, nligneso=nligne - nlmso, indice=indice, xyz=xyz, rvol=rvol,
-                numia=numia, clapl=clapl, ivois=ivois, isvois=isvois, nsort=nsort, nsortf=nsortf, nsm=nsm, nstm=nstm,
+                numia=numia, clapl=clapl, cgrad=cgrad, ivois=ivois, isvois=isvois, nsort=nsort, nsortf=nsortf, nsm=nsm,
+                nstm=nstm,
                 isrt=isrt, poidso=poidso, nbm=nbm, nbtm=nbtm, nbord=nbord, nbordf=nbordf, ibord=ibord, isbord=isbord,
                 poidsa=poidsa, nlmsa=nlmsa, nlmmax=nlmmax, ylmato=ylmato, ylmso=ylmso, ianew=ianew, lb1=lb1, lb2=lb2,
                 newinv=newinv, new=newp)
@@ -416,16 +424,20 @@ def irregular_layout(s):
     return dict(nligne=nligne, ianew=ianew, lb1=lb1, lb2=lb2, newinv=newinv, new=newp)
 
 
-def mat_solve_sys(s, V, Enervide, Eimag, bes, neu, E_comp, cal_comp, basereel, phiato, fn=None, phiato2=None):
+def mat_solve_sys(s, V, Enervide, Eimag, bes, neu, E_comp, cal_comp, basereel, phiato, fn=None, phiato2=None,
+                  gradvr=None):
     """Reference mat_solve (with its calcMatRow) on an atom_system dict.
     phiato: list over atoms of (nbord(ia), nlmsa(ia)) arrays, complex
     (nphiato7 = 2) or real.  Returns the solution (nligne, nlmso).  ``fn``
     calls another procedure with the same argument surface instead.  With
     phiato2 (the irregular atomic solutions) it solves the irregular cluster
     solution instead (irregular, n_ato_RI = n_RI = 2, Test_reg = 0, the
-    irregular_layout rows)."""
+    irregular_layout rows).  With gradvr (npoint, 3) the rows are
+    scalar-relativistic (Relativiste = T, nicm = npoint, laplac's cgrad)."""
     A = _Args()
     irregular = phiato2 is not None
+    rel = gradvr is not None
+    gv = np.asfortranarray(np.asarray(gradvr, np.float64).reshape(s["npoint"], 3)) if rel else np.zeros(3)
     if irregular:
         lay = irregular_layout(s)
         s = dict(s, nligne=lay["nligne"], nligneso=lay["nligne"], ianew=lay["ianew"], lb1=lay["lb1"],
@@ -463,8 +475,8 @@ def mat_solve_sys(s, V, Enervide, Eimag, bes, neu, E_comp, cal_comp, basereel, p
     p = lambda a: a.ctypes.data_as(C.c_void_p)
     npsom = s["npsom"]
     args = [
-        A.b(False), A.b(basereel), p(Bc), p(Br), A.b(cal_comp), A.arr(np.zeros(nvois), np.float64),
-        A.arr(s["clapl"], np.float64), A.b(E_comp), A.d(Eimag), A.d(Enervide), A.arr(np.zeros(3), np.float64),
+        A.b(False), A.b(basereel), p(Bc), p(Br), A.b(cal_comp), A.arr(s["cgrad"] if rel else np.zeros(nvois), np.float64),
+        A.arr(s["clapl"], np.float64), A.b(E_comp), A.d(Eimag), A.d(Enervide), p(gv),
         A.arr(s["ianew"], np.int32), p(iato), A.arr(s["ibord"], np.int32), A.i(0), A.i(1), A.i(1), A.i(0),
         A.b(irregular), A.arr(s["isbord"], np.int32), A.arr(iso, np.int32), A.i(1), A.arr(s["isrt"], np.int32),
         A.arr(s["isvois"], np.int32), A.arr(s["ivois"], np.int32), A.arr(kar, np.float64), A.arr(kari, np.float64),
@@ -472,12 +484,13 @@ def mat_solve_sys(s, V, Enervide, Eimag, bes, neu, E_comp, cal_comp, basereel, p
         p(mato), A.i(1), A.i(0), A.arr(mso, np.int32), A.i(2 if irregular else 1), A.i(2 if irregular else 1),
         A.i(natome), A.i(s["nbm"]),
         A.arr(s["nbord"], np.int32), A.arr(s["nbordf"], np.int32), A.i(nbtm), p(Nc), p(Nr),
-        A.arr(s["new"], np.int32), A.arr(s["newinv"], np.int32), A.i(1), A.i(1), A.i(s["npoint"]), A.i(n),
+        A.arr(s["new"], np.int32), A.arr(s["newinv"], np.int32), A.i(1), A.i(s["npoint"] if rel else 1),
+        A.i(s["npoint"]), A.i(n),
         A.i(n if cal_comp else 0), A.i(s["nligneso"]), A.i(nlmsam), A.i(nlmmax), A.i(nlmmax), A.i(nl),
         A.arr(s["nlmsa"], np.int32), A.i(nl), A.i(nl if cal_comp else 0), A.i(nbtm), A.i(nphi7),
         A.i(s["npoint"]), A.i(npsom), A.i(max(s["nsm"], 1)), A.i(nl), A.i(ns), A.i(nsc), A.i(nsr),
         A.i(s["nsortf"]), A.i(1), A.i(1), A.i(1), A.i(1), A.i(s["nstm"]), A.arr(s["numia"], np.int32), A.i(nvois),
-        p(phi), A.arr(s["poidsa"], np.float64), A.arr(s["poidso"], np.float64), A.b(False), A.b(False),
+        p(phi), A.arr(s["poidsa"], np.float64), A.arr(s["poidso"], np.float64), A.b(rel), A.b(False),
         A.arr(s["rvol"][: s["npoint"]], np.float64), p(smi), p(smr), A.b(False), A.i(0), p(tf), p(tt),
         A.arr(np.asarray(V, np.float64), np.float64), A.b(False), A.arr(s["ylmato"], np.float64),
         A.arr(s["ylmso"], np.float64),

@@ -20,6 +20,9 @@ clusters and stores inputs and outputs as .npz fixtures:
                           -> atom border lists, weights, Ylmato, atom blocks
                              in the profile, and the solution for the three
                              branch sets
+  mat_solve with scalar-relativistic rows (Relativiste, gradvr the
+  potential's gradient, laplac's cgrad), without and with the atoms
+                          -> the solution for the three branch sets
   mat (mat.f90:5) on that cluster
                           -> Taull after its symmetrisation (real and complex
                              basis), and with FDM_comp at complex energy the
@@ -180,6 +183,36 @@ def atom_tau_fixture():
     print("atom tau", lay["nligne"], np.abs(TR).max())
 
 
+def rel_fixtures():
+    """mat_solve with Relativiste = T: the P1 cluster of 1.2 A and the
+    three-atom cluster, a random potential and gradient."""
+    Ra, h, lm = 1.2, 0.25, 2
+    cl = O.Cluster(Ra / BOHR, h / BOHR, lm)
+    cgrad = R.grid(Ra / BOHR, h / BOHR)["cgrad"]
+    rng = np.random.default_rng(21)
+    V = rng.uniform(-0.5, 0.5, cl.npoint) - 1.0
+    G = rng.normal(size=(cl.npoint, 3)) * 40.0
+    out = dict(R=Ra, h=h, lmaxso=lm, V=V, gradvr=G, cgrad=cgrad)
+    for b, (E, Ei, E_comp, cal, basereel) in enumerate(ATOM_BRANCHES):
+        bes, neu = R.phiso(cl, E_comp, E, Ei, E)
+        out[f"x{b}"] = R.mat_solve(cl, V, E, Ei, bes, neu, E_comp, cal, basereel, gradvr=G, cgrad=cgrad)
+    np.savez_compressed(os.path.join(HERE, "rel_R1.2_h0.25.npz"), **out)
+    a = ATOMS
+    posi = np.array(a["posi"]) / BOHR
+    s = R.atom_system(a["R"] / BOHR, a["h"] / BOHR, a["lmaxso"], posi, np.array(a["rmt"]) / BOHR, a["lmax"])
+    rng = np.random.default_rng(22)
+    V = rng.uniform(-0.3, 0.3, s["npoint"]) - 0.8
+    G = rng.normal(size=(s["npoint"], 3)) * 40.0
+    rows = np.arange(0, s["nligne"], ROW_STRIDE)
+    out = dict(V=V, gradvr=G, rows=rows)
+    for b, (E, Ei, E_comp, cal, basereel) in enumerate(ATOM_BRANCHES):
+        bes, neu = R.phiso_sys(s, E_comp, E, Ei, E)
+        x = R.mat_solve_sys(s, V, E, Ei, bes, neu, E_comp, cal, basereel, atom_phiato(s, E, Ei, E_comp), gradvr=G)
+        out[f"x{b}"] = x[rows]
+    np.savez_compressed(os.path.join(HERE, "atoms_rel_R1.9_h0.25.npz"), **out)
+    print("relativistic", cl.nligne, s["nligne"])
+
+
 def main():
     if not R.available():
         raise SystemExit("oracle/_ref/libfdmnes_ref.so is missing: make -f oracle/Makefile.ref")
@@ -211,6 +244,7 @@ def main():
     grid_fixtures()
     atom_fixture()
     atom_tau_fixture()
+    rel_fixtures()
 
 
 if __name__ == "__main__":
