@@ -20,6 +20,9 @@ def _L():
         L.fdmgpu_cluster_create.argtypes = [C.c_double, C.c_double, C.c_int, C.POINTER(C.c_void_p)]
         L.fdmgpu_cluster_create_mmm.restype = C.c_int
         L.fdmgpu_cluster_create_mmm.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
+        L.fdmgpu_cluster_create_group_ylm.restype = C.c_int
+        L.fdmgpu_cluster_create_group_ylm.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int,
+                                                       C.POINTER(C.c_void_p)]
         L.fdmgpu_cluster_create_group.restype = C.c_int
         L.fdmgpu_cluster_create_group.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
                                                   C.POINTER(C.c_void_p)]
@@ -103,13 +106,17 @@ class Cluster:
 
     # point-group number -> the representation masks it takes
     GROUPS = {1: (), 3: (0, 4), 6: (0, 1, 2, 3), 8: tuple(range(8)), 2: (0, 1), 4: (0, 1), 5: (0, 1, 4, 5),
-              7: (0, 1, 2, 3)}
+              7: (0, 1, 2, 3), 9: (0, 1, 2, 3), 10: (0, 1, 2, 3), 11: tuple(range(8))}
+    # the four-fold groups 4, -4, 4/m: complex characters and harmonics
+    COMPLEX_GROUPS = (9, 10, 11)
 
     def __init__(self, rsort: float, adimp: float, lmaxso: int, irrep: int = -1, group: int = 8, atoms=None,
-                 irregular: bool = False):
+                 irregular: bool = False, ylm_comp: bool = False):
         """irregular (with atoms): the rows of the irregular cluster solution
         (fdmgpu_cluster_create_atoms_ri), assembled with
-        :meth:`assemble_atoms` given both atomic solutions."""
+        :meth:`assemble_atoms` given both atomic solutions.  ylm_comp:
+        complex harmonics in the outer sphere (the reference's Ylm_comp;
+        groups 1-5, always for 4, -4, 4/m: fdmgpu_cluster_create_group_ylm)."""
         L = _L()
         h = C.c_void_p()
         self.irregular = bool(irregular)
@@ -129,7 +136,10 @@ class Cluster:
                                                   np.ascontiguousarray(lmax, np.int32), int(self.irregular),
                                                   C.byref(h))
         else:
-            rc = L.fdmgpu_cluster_create_group(rsort, adimp, lmaxso, int(group), int(irrep), C.byref(h))
+            if group in self.COMPLEX_GROUPS and irrep >= 0:
+                ylm_comp = True
+            rc = L.fdmgpu_cluster_create_group_ylm(rsort, adimp, lmaxso, int(group), int(irrep), int(bool(ylm_comp)),
+                                                   C.byref(h))
         if rc:
             raise _lib.FdmGpuError(f"fdmgpu_cluster_create failed ({rc})")
         self._h = h
@@ -139,6 +149,7 @@ class Cluster:
         self.lmaxso = lmaxso
         self.irrep = int(irrep)
         self.group = int(group)
+        self.ylm_comp = bool(ylm_comp) and atoms is None
         self.rsort, self.adimp = rsort, adimp
         self.nnz = int(L.fdmgpu_cluster_nnz(h))
 
@@ -186,8 +197,9 @@ class Cluster:
                  gradvr=None):
         """Compressed rows and the incoming-wave RHS of one energy point.
         gradvr (npoint, 3): the potential's gradient (Ry/bohr) for
-        scalar-relativistic rows (the reference's Relativiste)."""
-        cal = bool(E_comp) or not basereel
+        scalar-relativistic rows (the reference's Relativiste).  Complex
+        harmonics or characters make the arithmetic complex."""
+        cal = bool(E_comp) or not basereel or self.ylm_comp
         n, nl = self.nligne, self.nlmso
         rp = np.zeros(n + 1, np.int64)
         col = np.zeros(self.nnz, np.int32)
@@ -306,6 +318,15 @@ class Cluster:
         t["row_ptr"] = rp
         t["clapl0"], t["pw"] = float(t["par"][0]), float(t["par"][1])
         return t
+
+    def outer_weights(self) -> np.ndarray:
+        """Quadrature weights of the inner outer-sphere points, 4 pi rvol /
+        sum(rvol) (bordure)."""
+        lh, w, ds = np.zeros(self.nlmso, np.int32), np.zeros(self.nsortf), np.zeros(3 * self.npoint)
+        rc = _L().fdmgpu_cluster_assembly_weights(self._h, lh, w, ds)
+        if rc:
+            raise _lib.FdmGpuError(f"fdmgpu_cluster_assembly_weights failed ({rc})")
+        return w
 
     def atom_assembly_tables(self) -> dict:
         """The device assembler's atom tables (fdmgpu_cluster_atom_assembly_tables)."""

@@ -142,6 +142,18 @@ struct fdmgpu_cluster {
   // outer-sphere rows (nso = nligne)
   bool irregular = false;
   int nri = 1;
+  // complex harmonics in the outer sphere (the reference's Ylm_comp): the
+  // outer unknowns and incoming waves are Yc(L, m) (mat.f90:1733), with
+  // their values at the outer points in ycmp (bnd.size() x nlmso)
+  bool ylm_comp = false;
+  std::vector<cd> ycmp;
+  // the groups 4, -4 and 4/m (numbers 9-11), whose four-fold operations are
+  // no axis sign flips: the operations as matrices (row-major, p -> M p) in
+  // the reference's order, per neighbour the one that maps it into the
+  // calculation area, and the representation's (complex) characters
+  std::vector<std::array<int, 9>> gmat;
+  std::vector<unsigned char> gop;
+  std::vector<cd> gchar;
 };
 
 namespace {
@@ -172,6 +184,9 @@ bool in_domain(int group, int x, int y, int z) {
     case 6: return x >= 0 && y >= 0;                                     // mm2
     case 7: return (x > 0 && y > 0) || (x == 0 && y >= 0 && z >= 0) || (y == 0 && x >= 0 && z >= 0);  // 222
     case 8: return x >= 0 && y >= 0 && z >= 0;                           // mmm
+    case 9: return (x > 0 && y >= 0) || (x == 0 && y == 0);              // 4
+    case 10: return (x > 0 && y >= 0) || (x == 0 && y == 0 && z >= 0);   // -4
+    case 11: return z >= 0 && ((x > 0 && y >= 0) || (x == 0 && y == 0));  // 4/m
     default: return true;
   }
 }
@@ -194,6 +209,68 @@ std::vector<int> group_ops(int group) {
 }
 
 inline bool is_mirror_group(int group) { return group == 3 || group == 6 || group == 8; }
+inline bool is_c4_group(int group) { return group >= 9 && group <= 11; }
+
+// The operations of 4 (9), -4 (10) and 4/m (11) in the reference's order
+// (opsym, general.f90:4587; group tables tab_data.f90:715), as opsym's
+// matrices act on a position: identity 1, C4z 18: (-y, x, z), -C4z 21:
+// (y, -x, z), C2z 24, inversion 25, S4z 28: (-y, x, -z), -S4z 31:
+// (y, -x, -z), mirror z 42
+std::vector<std::array<int, 9>> c4_ops(int group) {
+  const std::array<int, 9> e{1, 0, 0, 0, 1, 0, 0, 0, 1}, c4{0, -1, 0, 1, 0, 0, 0, 0, 1},
+      c4m{0, 1, 0, -1, 0, 0, 0, 0, 1}, c2{-1, 0, 0, 0, -1, 0, 0, 0, 1}, inv{-1, 0, 0, 0, -1, 0, 0, 0, -1},
+      s4{0, -1, 0, 1, 0, 0, 0, 0, -1}, s4m{0, 1, 0, -1, 0, 0, 0, 0, -1}, mz{1, 0, 0, 0, 1, 0, 0, 0, -1};
+  switch (group) {
+    case 9: return {e, c4, c4m, c2};
+    case 10: return {e, c2, s4, s4m};
+    case 11: return {e, c4, c4m, c2, inv, s4, s4m, mz};
+    default: return {};
+  }
+}
+// i^e exactly
+inline cd ipow(int e) {
+  e = ((e % 4) + 4) % 4;
+  return cd(e == 0 ? 1.0 : (e == 2 ? -1.0 : 0.0), e == 1 ? 1.0 : (e == 3 ? -1.0 : 0.0));
+}
+// quarter turns q of an operation's rotation about z: (cos, sin) of q pi/2 = (M00, M10)
+inline int quarter_turns(const std::array<int, 9> &M) {
+  return M[0] == 1 ? 0 : (M[3] == 1 ? 1 : (M[0] == -1 ? 2 : 3));
+}
+// Character of the operation in representation v = k + 4 [p = -1]: the
+// rotation by q pi/2 has exp(-i k q pi/2), a reflection of z adds p (p = 1
+// for -4, whose representations are k = 0..3)
+inline cd c4_char(const std::array<int, 9> &M, int v) {
+  const int k = v & 3;
+  cd chi = ipow(-k * quarter_turns(M));
+  if (M[8] < 0 && (v & 4)) chi = -chi;
+  return chi;
+}
+// The complex harmonic Yc(l, m) (exp(i m phi) dependence) belongs to v when
+// Y(g r) = chi(g) Y(r) for every operation g (the neighbour relation
+// psi(n) = chi(g^-1) psi(g n) of the grid rows): a rotation by q pi/2
+// multiplies it by i^(m q), a reflection of z by (-1)^(l+m)
+bool c4_harmonic_in(const std::vector<std::array<int, 9>> &ops, int l, int m, int v) {
+  for (const auto &M : ops) {
+    cd lam = ipow(m * quarter_turns(M));
+    if (M[8] < 0 && ((l + m) & 1)) lam = -lam;
+    if (c4_char(M, v) != lam) return false;
+  }
+  return true;
+}
+inline void apply(const std::array<int, 9> &M, int x, int y, int z, int &u, int &v, int &w) {
+  u = M[0] * x + M[1] * y + M[2] * z;
+  v = M[3] * x + M[4] * y + M[5] * z;
+  w = M[6] * x + M[7] * y + M[8] * z;
+}
+// Yc, mat.f90:1733-1755: the complex harmonic (l, m) from the real ones
+// y1 = Y(l, m), y2 = Y(l, -m)
+inline cd yc(int m, double y1, double y2) {
+  const double r = 1 / 1.41421356237309504880168872420970;
+  if (m == 0) return cd(y1, 0.0);
+  if (m < 0) return cd(y2 * r, -y1 * r);
+  const double sg = (m & 1) ? -1.0 : 1.0;
+  return cd(y1 * sg * r, y2 * sg * r);
+}
 
 // Does the real harmonic with flip parities o belong to representation v of
 // the group: its parity under every operation g equals the character of g
@@ -201,6 +278,12 @@ bool harmonic_in(const std::vector<int> &ops, int o, int v) {
   for (int g : ops)
     if (__builtin_popcount(g & (o ^ v)) & 1) return false;
   return true;
+}
+
+inline int lof(int lm) {  // l of the harmonic index l*l + l + m
+  int l = 0;
+  while ((l + 1) * (l + 1) <= lm) l++;
+  return l;
 }
 
 // The atom border lists, harmonics and weights (bordure, general.f90:
@@ -419,7 +502,20 @@ int build(fdmgpu_cluster *c) {
   // Point weights (reseau, general.f90:9582-9651): the number of grid points
   // a wedge point stands for, over the largest such number
   c->rvol.assign(n, 1.0);
-  if (sym) {
+  if (sym && is_c4_group(c->group)) {  // orbits under the matrices
+    double rvmax = 1.0;
+    for (int i = 0; i < n; i++) {
+      std::vector<std::array<int, 3>> img;
+      for (const auto &M : c->gmat) {
+        std::array<int, 3> u;
+        apply(M, p[i].ix, p[i].iy, p[i].iz, u[0], u[1], u[2]);
+        if (std::find(img.begin(), img.end(), u) == img.end()) img.push_back(u);
+      }
+      c->rvol[i] = (double)img.size();
+      rvmax = std::max(rvmax, c->rvol[i]);
+    }
+    for (int i = 0; i < n; i++) c->rvol[i] /= rvmax;
+  } else if (sym) {
     // the size of the point's orbit under the group (every image of an
     // area point lies in the grid's symmetric sphere)
     double rvmax = 1.0;
@@ -448,11 +544,34 @@ int build(fdmgpu_cluster *c) {
   for (int i = 0; i < n; i++) where[key(p[i].ix, p[i].iy, p[i].iz)] = i;
   c->nbr.assign((size_t)n * 12, -1);
   c->flip.assign((size_t)n * 12, 0);
+  if (is_c4_group(c->group)) c->gop.assign((size_t)n * 12, 0);
   for (int i = 0; i < n; i++)
     for (int v = 0; v < 12; v++) {
       int x = p[i].ix + kDir[v][0], y = p[i].iy + kDir[v][1], z = p[i].iz + kDir[v][2];
       if (!(std::abs(x) <= nx && std::abs(y) <= nx && std::abs(z) <= nx)) continue;
-      if (sym) {
+      if (sym && is_c4_group(c->group)) {
+        // the first operation g that maps it into the area; posequiv
+        // records g's inverse, whose character the coupling then carries
+        // (general.f90:9111, isym_inv :9124)
+        for (size_t g = 0; g < c->gmat.size(); g++) {
+          int u, v2, w;
+          apply(c->gmat[g], x, y, z, u, v2, w);
+          if (in_domain(c->group, u, v2, w)) {
+            size_t gi = 0;  // the inverse: the transposed matrix
+            for (; gi < c->gmat.size(); gi++) {
+              bool tr = true;
+              for (int r = 0; r < 3; r++)
+                for (int q = 0; q < 3; q++) tr &= c->gmat[gi][3 * r + q] == c->gmat[g][3 * q + r];
+              if (tr) break;
+            }
+            c->gop[(size_t)i * 12 + v] = (unsigned char)gi;
+            x = u;
+            y = v2;
+            z = w;
+            break;
+          }
+        }
+      } else if (sym) {
         int f = 0;
         if (is_mirror_group(c->group)) {  // the mirrors of the negative coordinates
           f = ((x < 0) | ((y < 0) << 1) | ((z < 0) << 2)) & mx;
@@ -525,15 +644,24 @@ int build(fdmgpu_cluster *c) {
   c->lmidx.clear();
   for (int l = 0; l <= c->lmaxso; l++)
     for (int m = -l; m <= l; m++)
-      if (!sym || harmonic_in(c->ops, harmonic_irrep(l, m), c->irrep)) c->lmidx.push_back(l * l + l + m);
+      if (!sym || (is_c4_group(c->group) ? c4_harmonic_in(c->gmat, l, m, c->irrep)
+                                         : harmonic_in(c->ops, harmonic_irrep(l, m), c->irrep)))
+        c->lmidx.push_back(l * l + l + m);
   c->nlmso = (int)c->lmidx.size();
   if (c->nlmso == 0) return -1;
   c->ylm.assign(c->bnd.size() * c->nlmso, 0.0);
+  if (c->ylm_comp) c->ycmp.assign(c->bnd.size() * c->nlmso, cd(0));
   std::vector<double> yall(nall);
   for (size_t b = 0; b < c->bnd.size(); b++) {
     const GridPoint &q = p[c->bnd[b]];
     real_ylm(c->lmaxso, q.ix * c->adimp, q.iy * c->adimp, q.iz * c->adimp, yall.data());
-    for (int k = 0; k < c->nlmso; k++) c->ylm[b * c->nlmso + k] = yall[c->lmidx[k]];
+    for (int k = 0; k < c->nlmso; k++) {
+      c->ylm[b * c->nlmso + k] = yall[c->lmidx[k]];
+      if (c->ylm_comp) {
+        const int l = lof(c->lmidx[k]), m = c->lmidx[k] - l * l - l;
+        c->ycmp[b * c->nlmso + k] = yc(m, yall[l * l + l + m], yall[l * l + l - m]);
+      }
+    }
   }
   if (c->natome > 0) return build_atoms(c);
   // Rows and profile (newind, mat.f90:1812-1884)
@@ -598,11 +726,6 @@ int build(fdmgpu_cluster *c) {
   return 0;
 }
 
-inline int lof(int lm) {  // l of the harmonic index l*l + l + m
-  int l = 0;
-  while ((l + 1) * (l + 1) <= lm) l++;
-  return l;
-}
 
 // stencil term of grid point i towards direction d: -clapl(v) times the
 // character of the mirrors that bring the neighbour into the wedge, times
@@ -635,6 +758,19 @@ inline double stencil_rel(const fdmgpu_cluster *c, int i, int d, double bder, co
   if (!G) return stencil(c, i, d);
   const double cf = -c->clapl[d + 1] + bder * c->cgrad[d + 1] * G[(size_t)i * 3 + (d >> 1) % 3];
   return (character(c->irrep, c->flip[(size_t)i * 12 + d]) * cf) * c->rvol[i];
+}
+// The same with the complex characters of 4, -4, 4/m: (Kar cf, Kari cf)
+// rvol(i) (calcMatRow's Repres_comp branch, mat.f90:1217-1223)
+inline cd stencil_c(const fdmgpu_cluster *c, int i, int d, double bder, const double *G) {
+  if (!is_c4_group(c->group)) return cd(stencil_rel(c, i, d, bder, G), 0.0);
+  double cf = -c->clapl[d + 1];
+  if (G) cf = cf + bder * c->cgrad[d + 1] * G[(size_t)i * 3 + (d >> 1) % 3];
+  const cd chi = c->gchar[c->gop[(size_t)i * 12 + d]];
+  return cd((chi.real() * cf) * c->rvol[i], (chi.imag() * cf) * c->rvol[i]);
+}
+// the outer unknown k's harmonic at outer point b: Yc with complex harmonics
+inline cd ysort(const fdmgpu_cluster *c, int b, int k) {
+  return c->ylm_comp ? c->ycmp[(size_t)b * c->nlmso + k] : cd(c->ylm[(size_t)b * c->nlmso + k], 0.0);
 }
 
 // Outer-sphere functions at every boundary point (phiso, non-Rydberg branch,
@@ -727,6 +863,38 @@ int atom_cluster_tables(const fdmgpu_cluster *c, double *base, int64_t *qdiag, i
 extern "C" {
 
 int fdmgpu_cluster_create_group(double rsort, double adimp, int lmaxso, int group, int irrep, fdmgpu_cluster **out) {
+  return fdmgpu_cluster_create_group_ylm(rsort, adimp, lmaxso, group, irrep, is_c4_group(group) ? 1 : 0, out);
+}
+
+int fdmgpu_cluster_create_group_ylm(double rsort, double adimp, int lmaxso, int group, int irrep, int ylm_comp,
+                                    fdmgpu_cluster **out) {
+  // complex harmonics: the groups whose operations keep each Yc(l, m) (the
+  // reference's subgroups accepting complex harmonics, tab_data.f90:547);
+  // 4, -4 and 4/m always take them (Atom_comp_cal, general.f90:9337)
+  if (ylm_comp && !(group <= 5 || is_c4_group(group))) return -1;
+  if (!ylm_comp && is_c4_group(group)) return -1;
+  if (is_c4_group(group)) {
+    if (!out || !(rsort > 0) || !(adimp > 0) || lmaxso < 0 || lmaxso > 40 || irrep < 0 ||
+        irrep > (group == 11 ? 7 : 3))
+      return -1;
+    fdmgpu_cluster *c = new (std::nothrow) fdmgpu_cluster();
+    if (!c) return -3;
+    c->rsort = rsort;
+    c->adimp = adimp;
+    c->lmaxso = lmaxso;
+    c->irrep = irrep;
+    c->group = group;
+    c->ops = {0};
+    c->ylm_comp = true;
+    c->gmat = c4_ops(group);
+    for (const auto &M : c->gmat) c->gchar.push_back(c4_char(M, irrep));
+    if (build(c)) {
+      delete c;
+      return -1;
+    }
+    *out = c;
+    return 0;
+  }
   // the reference's point-group numbers (general.f90:5207-5220) of the
   // mirror groups, in posequiv's orientation: m's mirror is normal to z and
   // mm2's two mirrors are normal to x and y (general.f90:8961-8978)
@@ -762,6 +930,7 @@ int fdmgpu_cluster_create_group(double rsort, double adimp, int lmaxso, int grou
   c->mirrors = mirrors;
   c->group = group;
   c->ops = ops;
+  c->ylm_comp = ylm_comp != 0;
   if (build(c)) {
     delete c;
     return -1;
@@ -901,7 +1070,10 @@ int fdmgpu_cluster_assemble_rel(const fdmgpu_cluster *c, const double *V, const 
                                 int *col, double *re, double *im, double *rhs_re, double *rhs_im) {
   if (c->natome > 0) return -2;  // fdmgpu_cluster_assemble_atoms
   const double *G = gradvr;
-  const bool cal = E_comp || !basereel;
+  // complex harmonics or characters make the arithmetic complex (mat.f90:97-104)
+  const bool cx = c->ylm_comp || is_c4_group(c->group);
+  const bool cal = E_comp || !basereel || cx;
+  if (cx && !im) return -1;
   const int np = c->npoint, nl = c->nlmso, nb = (int)c->bnd.size(), L = c->lmaxso;
   std::vector<cd> B((size_t)nb * (L + 1)), N((size_t)nb * (L + 1));
   outer_functions(c, Eimag, Ecinetic_out, E_comp, B.data(), N.data());
@@ -934,7 +1106,12 @@ int fdmgpu_cluster_assemble_rel(const fdmgpu_cluster *c, const double *V, const 
       const int j = col[q] - 1;
       cd v = (j == i) ? diag : cd(0);
       for (int d = 0; d < 12; d++)
-        if (c->nbr[(size_t)i * 12 + d] == j) v += stencil_rel(c, i, d, rp.bder, G) / c->rvol[j];
+        if (c->nbr[(size_t)i * 12 + d] == j) {
+          if (cx)
+            v += stencil_c(c, i, d, rp.bder, G) / c->rvol[j];
+          else
+            v += stencil_rel(c, i, d, rp.bder, G) / c->rvol[j];
+        }
       re[q] = v.real();
       if (cal) im[q] = v.imag();
     }
@@ -942,6 +1119,16 @@ int fdmgpu_cluster_assemble_rel(const fdmgpu_cluster *c, const double *V, const 
       const int j = c->nbr[(size_t)i * 12 + d];
       if (j < np) continue;
       const int b = c->slot[j];
+      if (cx) {  // complex harmonics (Ycomp) and characters, :1328-1377
+        const cd w = stencil_c(c, i, d, rp.bder, G);
+        for (int lm = 0; lm < nl; lm++) {
+          const int l = lof(c->lmidx[lm]);
+          const cd y = ysort(c, b, lm);
+          row[lm] += cfac(b, l) * y * w;
+          rrow[lm] -= y * B[(size_t)b * (L + 1) + l] * w;
+        }
+        continue;
+      }
       const double w = stencil_rel(c, i, d, rp.bder, G);
       for (int lm = 0; lm < nl; lm++) {
         const int l = lof(c->lmidx[lm]);
@@ -960,7 +1147,32 @@ int fdmgpu_cluster_assemble_rel(const fdmgpu_cluster *c, const double *V, const 
         if (cal && rhs_im) rhs_im[(size_t)i * nl + lm] = rrow[lm].imag();
       }
   }
-  for (int lm = 0; lm < nl; lm++) {  // outer rows, mat.f90:1401-1511, weights poidso (bordure)
+  for (int lm = 0; lm < nl && c->ylm_comp; lm++) {  // outer rows with complex harmonics, :1425-1497
+    const int ii = np + lm;
+    int64_t q = c->rp[ii];
+    for (int b = 0; b < c->nsortf; b++, q++) {
+      const cd f = c->wso[b] * std::conj(ysort(c, b, lm));
+      re[q] = -f.real() / c->rvol[c->bnd[b]];
+      im[q] = -f.imag() / c->rvol[c->bnd[b]];
+    }
+    for (int lp = 0; lp < nl; lp++, q++) {
+      const int l = lof(c->lmidx[lp]);
+      cd s = 0;
+      for (int b = 0; b < c->nsortf; b++) s += (c->wso[b] * (std::conj(ysort(c, b, lm)) * ysort(c, b, lp))) * cfac(b, l);
+      re[q] = s.real();
+      im[q] = s.imag();
+    }
+    if (rhs_re)
+      for (int lf = 0; lf < nl; lf++) {
+        const int l = lof(c->lmidx[lf]);
+        cd s = 0;
+        for (int b = 0; b < c->nsortf; b++)
+          s += ((c->wso[b] * std::conj(ysort(c, b, lm))) * ysort(c, b, lf)) * B[(size_t)b * (L + 1) + l];
+        rhs_re[(size_t)ii * nl + lf] = -s.real();
+        if (rhs_im) rhs_im[(size_t)ii * nl + lf] = -s.imag();
+      }
+  }
+  for (int lm = 0; lm < nl && !c->ylm_comp; lm++) {  // outer rows, mat.f90:1401-1511, weights poidso (bordure)
     const int ii = np + lm;
     int64_t q = c->rp[ii];
     for (int b = 0; b < c->nsortf; b++, q++) {  // :1476-1483
@@ -1165,6 +1377,8 @@ int fdmgpu_cluster_assembly_sizes(const fdmgpu_cluster *c, int64_t *sizes) {
 int fdmgpu_cluster_assembly_tables(const fdmgpu_cluster *c, double *base, int64_t *qdiag, int *orow,
                                    int64_t *oq, int *optr, int *ob, double *ow, double *ylm, double *brad,
                                    double *par) {
+  // the device tables hold real harmonics and real characters
+  if (c->ylm_comp || is_c4_group(c->group)) return -2;
   if (c->natome > 0) return atom_cluster_tables(c, base, qdiag, orow, oq, optr, ob, ow, ylm, brad, par);
   const int np = c->npoint, nl = c->nlmso;
   std::fill(base, base + c->cols.size(), 0.0);
@@ -1212,6 +1426,10 @@ int fdmgpu_cluster_assembly_tables(const fdmgpu_cluster *c, double *base, int64_
 int fdmgpu_cluster_assembly_weights(const fdmgpu_cluster *c, int *lharm, double *wso, double *dself) {
   for (int k = 0; k < c->nlmso; k++) lharm[k] = lof(c->lmidx[k]);
   std::memcpy(wso, c->wso.data(), sizeof(double) * c->nsortf);
+  if (is_c4_group(c->group)) {  // complex characters: no real self terms (and no device tables)
+    std::fill(dself, dself + 3 * (size_t)c->npoint, 0.0);
+    return 0;
+  }
   for (int i = 0; i < c->npoint; i++) {
     double *s = dself + 3 * (size_t)i;
     s[0] = s[1] = s[2] = 0.0;
@@ -1224,7 +1442,7 @@ int fdmgpu_cluster_assembly_weights(const fdmgpu_cluster *c, int *lharm, double 
 int fdmgpu_cluster_rel_tables(const fdmgpu_cluster *c, int64_t *rq, double *rfac, double *rdiv, int *opoint,
                               int *odir, double *ofac, int *apoint, int *adir, double *clapl, double *cgrad) {
   if (!c || !rq || !rfac || !rdiv || !clapl || !cgrad) return -1;
-  if (c->irregular) return -2;  // the device tables describe regular solutions
+  if (c->irregular || c->ylm_comp || is_c4_group(c->group)) return -2;  // regular, real harmonics and characters
   const int np = c->npoint;
   int r = 0, k = 0, n = 0, ka = 0;
   for (int i = 0; i < np; i++) {

@@ -70,6 +70,27 @@ int fdmgpu_cluster_create_mmm(double rsort, double adimp, int lmaxso, int irrep,
  * plane and axis); the potential must have the group's symmetry.  Returns -1
  * for any other group or irrep. */
 int fdmgpu_cluster_create_group(double rsort, double adimp, int lmaxso, int group, int irrep, fdmgpu_cluster **out);
+/* The same with the choice of complex harmonics in the outer sphere
+ * (the reference's Ylm_comp; mat.f90:1329, :1374, :1430-1436, :1476-1497):
+ * the outer unknowns and incoming waves are Yc(L, m) (mat.f90:1733, the
+ * exp(i m phi) combinations of the real harmonics) and the arithmetic is
+ * complex.  Allowed for the groups whose operations keep every Yc(L, m)
+ * (1 to 5; tab_data.f90:547) and required for the four-fold groups 4 (9),
+ * -4 (10) and 4/m (11) (Atom_comp_cal, general.f90:9337), whose
+ * representations are one-dimensional with complex characters (the
+ * reference's Kari, calcMatRow's Repres_comp branch, mat.f90:1217-1223):
+ * irrep = k + 4 [p = -1], the rotation by q pi/2 about z has the character
+ * exp(-i k q pi/2) and a reflection of z adds the factor p (p = 1 for -4);
+ * k = 0..3 for 4 and -4, 0..7 for 4/m.  Operations (opsym numbers): 4 =
+ * {1, 18, 21, 24}, -4 = {1, 24, 28, 31}, 4/m = {1, 18, 21, 24, 25, 28, 31,
+ * 42}; calculation areas and the mapping of neighbours as in posequiv
+ * (general.f90:8980-8985).  The reference solves one representation of each
+ * complex-conjugate pair and obtains the other's Taull by conjugation
+ * (mat.f90:405-425); these systems are all built here.  Such clusters have
+ * no device assembly tables (fdmgpu_cluster_assembly_tables returns -2):
+ * assemble them on the host (fdmgpu_cluster_assemble, complex buffers). */
+int fdmgpu_cluster_create_group_ylm(double rsort, double adimp, int lmaxso, int group, int irrep, int ylm_comp,
+                                    fdmgpu_cluster **out);
 /* A P1 cluster with natome muffin-tin atoms: posi[3*natome] (bohr, the
  * absorber at the origin, in non-decreasing distance from it, as the
  * reference sorts its cluster), sphere radii rmt[natome] (bohr), expansion

@@ -71,6 +71,9 @@ struct orc_cluster {
   double cgrad[13];
   int relativiste;
   double *gradvr;
+  /* complex harmonics in the outer sphere (Ylm_comp, mat.f90:1329, :1374,
+   * :1430-1436, :1476-1497), set by orc_cluster_set_ylm_comp */
+  int ylm_comp;
 };
 #define GRADVR(c, i, k) ((c)->gradvr[((k)-1) * (size_t)(c)->npoint + (i)-1])
 #define IBORD(c, ib, ia) ((c)->ibord[((ia)-1) * (c)->nbtm + (ib)-1])
@@ -678,6 +681,11 @@ int orc_cluster_set_relativistic(orc_cluster *c, int on, const double *gradvr) {
   c->relativiste = 1;
   return 0;
 }
+int orc_cluster_set_ylm_comp(orc_cluster *c, int on) {
+  if (c->natome > 0 && on) return -1; /* the atom branches are restated for real harmonics */
+  c->ylm_comp = on != 0;
+  return 0;
+}
 void orc_cluster_cgrad(const orc_cluster *c, double *cg) { memcpy(cg, c->cgrad + 1, sizeof(double) * c->nvois); }
 int orc_cluster_set_phiato(orc_cluster *c, int cplx, const double *re, const double *im) {
   return orc_cluster_set_phiato_ri(c, 1, cplx, re, im);
@@ -775,6 +783,22 @@ void orc_phiso(const orc_cluster *c, int E_comp, double Ecinetic_out, double Eim
 }
 
 /* ---------------------------------------------------------------- calcMatRow */
+/* Yc, mat.f90:1733-1755: the complex harmonic (L, m) from the real ones
+ * y1 = Y(L, m), y2 = Y(L, -m) */
+static const double SQRT_1O2 = 1 / 1.41421356237309504880168872420970;
+static double complex orc_yc(int m, double y1, double y2) {
+  if (m == 0) return CMPLX(y1, 0.0);
+  if (m < 0) return CMPLX(y2 * SQRT_1O2, -y1 * SQRT_1O2);
+  const double s = (m & 1) ? -1.0 : 1.0;
+  return CMPLX(y1 * s * SQRT_1O2, y2 * s * SQRT_1O2);
+}
+/* the outer-sphere harmonic of unknown (L, m) at boundary point ib: Yc with
+ * complex harmonics, else the real one */
+static double complex orc_ysort(const orc_cluster *c, int ib, int lm0, int m) {
+  if (c->ylm_comp && m != 0) return orc_yc(m, YLMSO(c, ib, lm0), YLMSO(c, ib, lm0 - 2 * m));
+  return CMPLX(YLMSO(c, ib, lm0), 0.0);
+}
+
 typedef struct {
   const orc_cluster *c;
   const double *V;
@@ -866,7 +890,7 @@ static void orc_calcmatrow(const row_ctx *x, int ii, double *abvr, double *abvi,
           int lm0 = L * L + L + 1 + m;
           int jjj = nligneso + lm;
           if (cal) {
-            double complex ycomp = CMPLX(YLMSO(c, ib, lm0), 0.0);
+            double complex ycomp = orc_ysort(c, ib, lm0, m); /* :1328-1332 */
             double complex cfac;
             if (x->basereel) {
               cfac = ctx_neu(x, ib, L);
@@ -886,7 +910,7 @@ static void orc_calcmatrow(const row_ctx *x, int ii, double *abvr, double *abvi,
           int L = c->lso[lm], m = c->mso[lm];
           int lm0 = L * L + L + 1 + m;
           if (cal) {
-            double complex ycomp = CMPLX(YLMSO(c, ib, lm0), 0.0);
+            double complex ycomp = orc_ysort(c, ib, lm0, m); /* :1373-1377 */
             double complex cfv;
             if (c->irregular) /* :1379-1381 */
               cfv = ycomp * (-I * ctx_bes(x, ib, L) + ctx_neu(x, ib, L)) * cfvs;
@@ -916,7 +940,16 @@ static void orc_calcmatrow(const row_ctx *x, int ii, double *abvr, double *abvi,
       double charmr = 0;
       for (int ib = 1; ib <= c->nsortf; ib++) {
         if (cal) {
-          double complex cfac = CMPLX(YLMSO(c, ib, lm0) * YLMSO(c, ib, lmp0), 0.0);
+          double complex cfac; /* :1429-1437 */
+          if (c->ylm_comp && m != 0 && mp != 0)
+            cfac = conj(orc_yc(m, YLMSO(c, ib, lm0), YLMSO(c, ib, lm0 - 2 * m))) *
+                   orc_yc(mp, YLMSO(c, ib, lmp0), YLMSO(c, ib, lmp0 - 2 * mp));
+          else if (c->ylm_comp && m != 0)
+            cfac = conj(orc_yc(m, YLMSO(c, ib, lm0), YLMSO(c, ib, lm0 - 2 * m))) * YLMSO(c, ib, lmp0);
+          else if (c->ylm_comp && mp != 0)
+            cfac = YLMSO(c, ib, lm0) * orc_yc(mp, YLMSO(c, ib, lmp0), YLMSO(c, ib, lmp0 - 2 * mp));
+          else
+            cfac = CMPLX(YLMSO(c, ib, lm0) * YLMSO(c, ib, lmp0), 0.0);
           cfac = c->poidso[ib - 1] * cfac;
           if (x->basereel) {
             charmc = charmc + cfac * ctx_neu(x, ib, L);
@@ -942,13 +975,18 @@ static void orc_calcmatrow(const row_ctx *x, int ii, double *abvr, double *abvi,
       int jj = c->newp[j] - 1 + isp;
       double fac = c->poidso[ib - 1] * YLMSO(c, ib, lm0);
       double complex cfac = CMPLX(fac, 0.0);
+      if (c->ylm_comp && m != 0) { /* :1474-1478 */
+        cfac = c->poidso[ib - 1] * conj(orc_yc(m, YLMSO(c, ib, lm0), YLMSO(c, ib, lm0 - 2 * m)));
+        abvr[jj] = abvr[jj] - creal(cfac) / c->rvol[j - 1];
+        abvi[jj] = abvi[jj] - cimag(cfac) / c->rvol[j - 1];
+      } else
       abvr[jj] = abvr[jj] - fac / c->rvol[j - 1];
       for (int lmf = 1; lmf <= nlmso; lmf++) {
         if (isp != c->iso[lmf]) continue;
         int L = c->lso[lmf], mf = c->mso[lmf];
         int lmf0 = L * L + L + 1 + mf;
         if (cal) {
-          double complex cfad = cfac * CMPLX(YLMSO(c, ib, lmf0), 0.0);
+          double complex cfad = cfac * orc_ysort(c, ib, lmf0, mf); /* :1492-1497 */
           cfad = cfad * (x->E_comp ? ctx_bes(x, ib, L) : CMPLX(creal(ctx_bes(x, ib, L)), 0.0));
           smr_ii[lmf - 1] = smr_ii[lmf - 1] - creal(cfad);
           smi_ii[lmf - 1] = smi_ii[lmf - 1] - cimag(cfad);

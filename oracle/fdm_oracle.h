@@ -69,6 +69,11 @@ int orc_cluster_set_phiato_ri(orc_cluster *c, int i_ri, int cplx, const double *
  * V at the interior points, gradvr(nicm, 3) of the reference, Ry/bohr);
  * on = 0 switches them off.  Returns 0, or -1 (no gradvr / memory). */
 int orc_cluster_set_relativistic(orc_cluster *c, int on, const double *gradvr);
+/* Complex harmonics in the outer sphere (Ylm_comp = T; mat.f90:1329,
+ * :1374, :1430-1436, :1476-1497): the outer unknowns and incoming waves
+ * are the complex Y(L, m) of Yc (mat.f90:1733), the arithmetic must then be
+ * complex (cal_comp).  Clusters without atoms only; returns -1 otherwise. */
+int orc_cluster_set_ylm_comp(orc_cluster *c, int on);
 /* cgrad(1:nvois), laplac's gradient stencil (general.f90:9791-9822) */
 void orc_cluster_cgrad(const orc_cluster *c, double *cgrad);
 void orc_cluster_free(orc_cluster *c);

@@ -49,6 +49,7 @@ def lib():
         L.orc_cluster_set_phiato_ri.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, C.c_void_p]
         L.orc_cluster_set_relativistic.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.orc_cluster_cgrad.argtypes = [C.c_void_p, _dp]
+        L.orc_cluster_set_ylm_comp.argtypes = [C.c_void_p, C.c_int]
         L.orc_cluster_free.argtypes = [C.c_void_p]
         L.orc_cluster_sizes.argtypes = [C.c_void_p, _ip]
         L.orc_cluster_xyz.argtypes = [C.c_void_p, _dp]
@@ -179,6 +180,12 @@ class Cluster:
         g = np.asfortranarray(np.asarray(gradvr, np.float64).reshape(self.npoint, 3))
         if lib().orc_cluster_set_relativistic(self._h, 1, g.ctypes.data_as(C.c_void_p)):
             raise MemoryError("orc_cluster_set_relativistic failed")
+
+    def set_ylm_comp(self, on: bool):
+        """Complex harmonics in the outer sphere (Ylm_comp); needs complex
+        arithmetic (cal_comp) in every solve and assembly."""
+        if lib().orc_cluster_set_ylm_comp(self._h, int(bool(on))):
+            raise ValueError("Ylm_comp is restated for clusters without atoms")
 
     @property
     def cgrad(self):

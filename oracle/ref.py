@@ -146,13 +146,14 @@ def phiso(cl, E_comp: bool, Ecinetic_out: float, Eimag: float, Enervide: float =
 
 
 def mat_solve(cl, V, Enervide, Eimag, bes, neu, E_comp, cal_comp, basereel, rhs=None, fn=None, gradvr=None,
-              cgrad=None):
+              cgrad=None, ylm_comp=False):
     """Reference mat_solve (93 arguments, mat_solve_gaussian.f90:5-12) with
     the reference calcMatRow assembling each row.  Returns the solution
     (nligne, nlmso), real or complex.  ``fn`` calls another procedure with
     the same argument surface instead (a drop-in replacement).  With gradvr
     (npoint, 3) and cgrad (nvois,) the rows are scalar-relativistic
-    (Relativiste = T, nicm = npoint)."""
+    (Relativiste = T, nicm = npoint).  ylm_comp: complex harmonics in the
+    outer sphere (Ylm_comp = T)."""
     A = _Args()
     n, nl, ns, lm = cl.nligne, cl.nlmso, cl.nsort, cl.lmaxso
     lb1, lb2, newinv, newp = cl.profile()
@@ -202,7 +203,7 @@ def mat_solve(cl, V, Enervide, Eimag, bes, neu, E_comp, cal_comp, basereel, rhs=
         A.i(nl), A.i(ns), A.i(nsc), A.i(nsr), A.i(cl.nsortf), A.i(1), A.i(1), A.i(1), A.i(1), A.i(ns),
         A.arr(numia, np.int32), A.i(nvois), A.arr(np.zeros(1), np.float64), A.arr(np.zeros(1), np.float64),
         A.arr(poidso, np.float64), A.b(rel), A.b(False), A.arr(np.ones(cl.npoint), np.float64), p(smi), p(smr),
-        A.b(False), A.i(0), p(tf), p(tt), A.arr(np.asarray(V, np.float64), np.float64), A.b(False),
+        A.b(False), A.i(0), p(tf), p(tt), A.arr(np.asarray(V, np.float64), np.float64), A.b(ylm_comp),
         A.arr(np.zeros(1), np.float64), A.arr(cl.ylmso, np.float64),
     ]
     assert len(args) == 93
@@ -222,6 +223,15 @@ def invcomp(A):
     n = a.shape[0]
     lib().invcomp_(A_.i(n), a.ctypes.data_as(C.c_void_p), A_.i(n), A_.i(n), A_.i(0), A_.b(True))
     return a
+
+
+def opsym(op: int):
+    """The reference's matrix of symmetry operation ``op`` (opsym,
+    general.f90:4587): a point p goes to M p."""
+    A = _Args()
+    m = np.zeros((3, 3), order="F")
+    lib().opsym_(A.i(op), m.ctypes.data_as(C.c_void_p))
+    return np.array(m)
 
 
 def grid(rsort: float, adimp: float, iord: int = 4, igrpt: int = 1, iopsymr=None):

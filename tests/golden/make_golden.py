@@ -12,9 +12,10 @@ clusters and stores inputs and outputs as .npz fixtures:
   reseau / laplac / bordure (general.f90:9439/9732/9895, lists read back by
   recup_bordure)          -> point order, neighbour table, stencil weights,
                              outer-sphere boundary points and weights, for
-                             point group 1 and for the subgroups of mmm:
-                             m, mm2, mmm, -1, 2, 2/m, 222 (calculation area,
-                             rvol, symmetry operation of each neighbour)
+                             point group 1, the subgroups of mmm (m, mm2,
+                             mmm, -1, 2, 2/m, 222) and 4, -4, 4/m
+                             (calculation area, rvol, symmetry operation of
+                             each neighbour)
   the same with muffin-tin atoms, then newind and mat_solve with a synthetic
   phiato (j_L(k r) Ylm at the atoms' border points)
                           -> atom border lists, weights, Ylmato, atom blocks
@@ -23,6 +24,8 @@ clusters and stores inputs and outputs as .npz fixtures:
   mat_solve with scalar-relativistic rows (Relativiste, gradvr the
   potential's gradient, laplac's cgrad), without and with the atoms
                           -> the solution for the three branch sets
+  mat_solve with complex harmonics in the outer sphere (Ylm_comp)
+                          -> the solution for three branch sets
   mat (mat.f90:5) on that cluster
                           -> Taull after its symmetrisation (real and complex
                              basis), and with FDM_comp at complex energy the
@@ -60,10 +63,12 @@ CASES = [
 
 GRID_CASES = [(1.2, 0.25), (1.6, 0.3), (2.0, 0.25)]
 # point group -> its operations in the reference's numbering (opsym,
-# general.f90:4587): identity 1, two-fold rotations about x, y, z 22-24,
-# inversion 25, mirrors normal to x, y, z 40-42
+# general.f90:4587): identity 1, four-fold rotations about z 18 and 21,
+# two-fold rotations about x, y, z 22-24, inversion 25, four-fold
+# rotation-reflections about z 28 and 31, mirrors normal to x, y, z 40-42
 GROUP_OPS = {3: [1, 42], 6: [1, 24, 40, 41], 8: [1, 22, 23, 24, 25, 40, 41, 42],
-             2: [1, 25], 4: [1, 24], 5: [1, 24, 25, 42], 7: [1, 22, 23, 24]}
+             2: [1, 25], 4: [1, 24], 5: [1, 24, 25, 42], 7: [1, 22, 23, 24],
+             9: [1, 18, 21, 24], 10: [1, 24, 28, 31], 11: [1, 18, 21, 24, 25, 28, 31, 42]}
 
 
 def grid_fixtures():
@@ -80,7 +85,9 @@ def grid_fixtures():
         Ra, h = 1.6, 0.25
         g = R.grid(Ra / BOHR, h / BOHR, igrpt=grp, iopsymr=io)
         npt = g["npoint"]
+        opmat = np.array([R.opsym(op) for op in ops])
         np.savez_compressed(os.path.join(HERE, f"grid_g{grp}_R{Ra}_h{h}.npz"), R=Ra, h=h, group=grp, ops=np.array(ops),
+                            opmat=opmat,
                             npoint=npt, npso=g["npso"], indice=g["indice"], radius=g["radius"], rvol=g["rvol"],
                             ivois=g["ivois"][:npt], isvois=g["isvois"][:npt], clapl=g["clapl"], nsortf=g["nsortf"],
                             nsort=g["nsort"], isrt=g["isrt"], poidso=g["poidso"])
@@ -213,6 +220,23 @@ def rel_fixtures():
     print("relativistic", cl.nligne, s["nligne"])
 
 
+def ylm_fixture():
+    """mat_solve with Ylm_comp = T (complex harmonics in the outer sphere)
+    on the P1 cluster of 1.2 A, lmaxso 3, three branch sets."""
+    Ra, h, lm = 1.2, 0.25, 3
+    cl = O.Cluster(Ra / BOHR, h / BOHR, lm)
+    V = np.random.default_rng(23).uniform(-0.5, 0.5, cl.npoint) - 1.0
+    out = dict(R=Ra, h=h, lmaxso=lm, V=V)
+    for b, (E, Ei, E_comp, basereel) in enumerate(YLM_BRANCHES):
+        bes, neu = R.phiso(cl, E_comp, E, Ei, E)
+        out[f"x{b}"] = R.mat_solve(cl, V, E, Ei, bes, neu, E_comp, True, basereel, ylm_comp=True)
+    np.savez_compressed(os.path.join(HERE, "ylmc_R1.2_h0.25.npz"), **out)
+    print("ylm_comp", cl.nligne)
+
+
+YLM_BRANCHES = [(0.7, 0.05, True, False), (0.7, 0.0, False, False), (0.7, 0.0, False, True)]
+
+
 def main():
     if not R.available():
         raise SystemExit("oracle/_ref/libfdmnes_ref.so is missing: make -f oracle/Makefile.ref")
@@ -245,6 +269,7 @@ def main():
     atom_fixture()
     atom_tau_fixture()
     rel_fixtures()
+    ylm_fixture()
 
 
 if __name__ == "__main__":

@@ -236,13 +236,13 @@ def test_assembly_tables_rebuild_representation_rows(E_comp, basereel, group):
 
 
 def test_group_and_irrep_checks():
-    """Only the subgroups of mmm are built, and a representation is named by
-    the smallest mask of its characters."""
+    """A representation of a subgroup of mmm is named by the smallest mask
+    of its characters; the groups beyond 4/m (number 11) are not built."""
     from fdmnes_amd import _lib
     from fdmnes_amd import cluster as CL
     R, h = 1.2 / BOHR, 0.25 / BOHR
-    # a representation's mask must be the smallest with its characters; no group 9
-    for group, irrep in [(6, 4), (3, 1), (3, 2), (5, 2), (2, 2), (4, 4), (7, 4), (8, 8), (9, 0)]:
+    # a representation's mask must be the smallest with its characters; no group 12
+    for group, irrep in [(6, 4), (3, 1), (3, 2), (5, 2), (2, 2), (4, 4), (7, 4), (8, 8), (12, 0)]:
         with pytest.raises(_lib.FdmGpuError):
             CL.Cluster(R, h, 2, irrep=irrep, group=group)
     whole = CL.Cluster(R, h, 2)

@@ -2,7 +2,8 @@
 stencil weights and outer-sphere boundary lists and weights from reseau,
 laplac and bordure (general.f90:9439, 9732, 9895, run from oracle/_ref by
 tests/golden/make_golden.py), for the oracle's restatement and the product's
-builder, point group 1 and the mirror groups m, mm2, mmm.
+builder, point group 1, the subgroups of mmm and the four-fold groups 4, -4,
+4/m.
 
 Beside the reference, checks that the mathematics fixes, each of which a
 plausible mistake fails:
@@ -14,11 +15,11 @@ plausible mistake fails:
 - the boundary lists are the interior points with an outside neighbour along
   the axes and the outside points with an interior neighbour, and their
   weights add up to 4 pi;
-- for the mirror groups, the product's grid-row couplings are the
-  reference's -clapl(v) chi(op) rvol(i) / rvol(j) (mat.f90:1159-1233) with
-  the neighbour and its symmetry operation from laplac and chi the
-  representation's character of that operation (between points whose wave
-  function the representation does not force to zero)."""
+- for the groups, the product's grid-row couplings are the reference's
+  -clapl(v) chi(op) rvol(i) / rvol(j) (mat.f90:1159-1233) with the neighbour
+  and its symmetry operation from laplac and chi the representation's
+  (complex, for 4, -4, 4/m) character of that operation (between points
+  whose wave function the representation does not force to zero)."""
 import os
 
 import numpy as np
@@ -30,9 +31,8 @@ BOHR = 0.52917721067
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 P1 = ["grid_p1_R1.2_h0.25", "grid_p1_R1.6_h0.3", "grid_p1_R2.0_h0.25"]
 GROUPS = ["grid_g3_R1.6_h0.25", "grid_g6_R1.6_h0.25", "grid_g8_R1.6_h0.25", "grid_g2_R1.6_h0.25",
-          "grid_g4_R1.6_h0.25", "grid_g5_R1.6_h0.25", "grid_g7_R1.6_h0.25"]
-# the reference's operations as the axes they negate (opsym, general.f90:4587)
-OP_AXES = {1: (), 40: (0,), 41: (1,), 42: (2,), 22: (1, 2), 23: (0, 2), 24: (0, 1), 25: (0, 1, 2)}
+          "grid_g4_R1.6_h0.25", "grid_g5_R1.6_h0.25", "grid_g7_R1.6_h0.25", "grid_g9_R1.6_h0.25",
+          "grid_g10_R1.6_h0.25", "grid_g11_R1.6_h0.25"]
 EPSPOS = 1e-4
 # stencil offsets in laplac's order (general.f90:9768-9781): per order io = 1, 2,
 # per axis x, y, z, the negative then the positive step
@@ -71,11 +71,16 @@ def test_product_grid_equals_reference(name):
     assert t["clapl0"] == g["clapl"][0]
 
 
-def _chi(op, irrep):
-    """Character of the reference's operation `op` in the representation
-    `irrep` (a character vector: the operation negating the axes in a set
-    has the character (-1)^(number of those axes whose bit is set))."""
-    return (-1.0) ** sum((irrep >> a) & 1 for a in OP_AXES[abs(op)])
+def _opmat(g, op):
+    """The reference's matrix of operation `op` (the fixture's copy of opsym)."""
+    return g["opmat"][list(g["ops"]).index(abs(int(op)))]
+
+
+def _chi(g, op, irrep):
+    """The builder's character of the reference's operation `op` in the
+    representation `irrep`."""
+    from .test_cluster_c4 import chi
+    return chi(int(g["group"]), _opmat(g, op), irrep)
 
 
 @pytest.mark.parametrize("name", GROUPS)
@@ -84,14 +89,14 @@ def test_product_mirror_group_builder_equals_reference(name):
     g = _load(name)
     grp, h = int(g["group"]), float(g["h"]) / BOHR
     npt = int(g["npoint"])
+    from .test_cluster_c4 import IRREPS
     from .test_cluster_mmm import _irreps
-    for irrep in _irreps(grp):
+    for irrep in (IRREPS[grp] if grp in (9, 10, 11) else _irreps(grp)):
         c = CL.Cluster(float(g["R"]) / BOHR, h, 3, irrep=irrep, group=grp)
         assert (c.npoint, c.npso, c.nsortf, c.nsort) == (npt, g["npso"], g["nsortf"], g["nsort"])
         assert np.array_equal(np.rint(c.points / h).astype(int), g["indice"][:npt])
         assert np.array_equal(c.rvol, g["rvol"])
-        t = c.assembly_tables()
-        assert np.array_equal(t["wso"], g["poidso"])
+        assert np.array_equal(c.outer_weights(), g["poidso"])
         # grid-row couplings between interior points, energy-independent
         rp, col, vals, _ = c.assemble(np.zeros(npt), 0.0, 0.0, 1.0, False, True)
         rvol, clapl = g["rvol"], g["clapl"]
@@ -103,9 +108,9 @@ def test_product_mirror_group_builder_equals_reference(name):
         # among such points are left out of the comparison
         ind = g["indice"]
         zero = np.zeros(len(ind), bool)
-        for op in g["ops"]:
-            if _chi(int(op), irrep) < 0:
-                zero |= np.all([ind[:, a] == 0 for a in OP_AXES[int(op)]], axis=0)
+        for op in g["ops"]:  # points the operation leaves in place, with a character other than 1
+            if abs(_chi(g, op, irrep) - 1) > 1e-12:
+                zero |= np.all(ind @ _opmat(g, op).T == ind, axis=1)
         worst = 0.0
         for i in range(npt):
             if zero[i]:
@@ -114,8 +119,8 @@ def test_product_mirror_group_builder_equals_reference(name):
             for v in range(12):
                 j, op = int(g["ivois"][i, v]), int(g["isvois"][i, v])
                 if j < npt and j != i and not zero[j]:
-                    want[j] = want.get(j, 0.0) - clapl[v + 1] * _chi(op, irrep) * rvol[i] / rvol[j]
-            got = {int(k) - 1: float(x) for k, x in zip(col[rp[i]:rp[i + 1]], vals[rp[i]:rp[i + 1]])
+                    want[j] = want.get(j, 0.0) - clapl[v + 1] * _chi(g, op, irrep) * rvol[i] / rvol[j]
+            got = {int(k) - 1: complex(x) for k, x in zip(col[rp[i]:rp[i + 1]], vals[rp[i]:rp[i + 1]])
                    if int(k) - 1 < npt and int(k) - 1 != i and not zero[int(k) - 1]}
             for j in set(want) | set(got):
                 worst = max(worst, abs(want.get(j, 0.0) - got.get(j, 0.0)) / abs(clapl[1]))
@@ -173,8 +178,14 @@ def test_neighbours_points_and_boundary(name):
         j = g["ivois"][:n_iv, v]
         ok = j >= 0
         if "group" in g:
-            # the neighbour is the image of the offset point: same |coordinates|
-            assert np.array_equal(np.abs(ind[j[ok]]), np.abs(ind[:n_iv][ok] + OFFSETS[v]))
+            # the neighbour is the image of the offset point under the
+            # inverse of the operation laplac records for it (posequiv
+            # records the operation that maps the area point back)
+            ops = g["isvois"][:, v][ok]
+            want = ind[:n_iv][ok] + OFFSETS[v]
+            for k, (jj, w) in enumerate(zip(j[ok], want)):
+                if k < len(ops):
+                    assert np.array_equal(ind[jj], np.rint(_opmat(g, ops[k]).T @ w).astype(int))
         else:
             assert np.array_equal(ind[j[ok]], ind[:n_iv][ok] + OFFSETS[v])
     # radial order and radii
@@ -191,10 +202,11 @@ def test_neighbours_points_and_boundary(name):
     assert len(set(got)) == len(got)
     if "group" in g:
         # one point per orbit of the group's operations
-        signs = [np.array([-1 if a in OP_AXES[int(op)] else 1 for a in range(3)]) for op in g["ops"]]
-        orbits = {frozenset(tuple(sg * np.array(p)) for sg in signs) for p in lattice}
+        mats = [np.rint(M).astype(int) for M in g["opmat"]]
+        orb = lambda p: frozenset(tuple(int(t) for t in M @ np.array(p)) for M in mats)  # noqa: E731
+        orbits = {orb(p) for p in lattice}
         assert len(got) == len(orbits)
-        assert {frozenset(tuple(sg * np.array(p)) for sg in signs) for p in got} == orbits
+        assert {orb(p) for p in got} == orbits
     else:
         assert set(got) == lattice
     # interior points: radius <= Rsort
