@@ -513,6 +513,107 @@ def mat_solve_sys(s, V, Enervide, Eimag, bes, neu, E_comp, cal_comp, basereel, p
     return x
 
 
+def spinorbit_layout(s):
+    """The reference newind (mat.f90:1761) with nspino = 2 for an
+    atom_system dict: two rows per interstitial point, atom blocks of
+    2 nlmsa unknowns (L, m, solution index, the index fastest, as lmrep,
+    general.f90:11227, lists them in P1) and 2 nlmso outer unknowns (L, m,
+    spin): nligne, nlmsa, nlmso, ianew, lb1, lb2, newinv, new."""
+    A = _Args()
+    p = lambda a: a.ctypes.data_as(C.c_void_p)
+    natome = s["natome"]
+    nlmsa = (2 * s["nlmsa"]).astype(np.int32)
+    nlmso = 2 * s["nlmso"]
+    nligne = 2 * s["npr"] + int(nlmsa.sum()) + nlmso
+    lb1 = np.zeros(nligne, np.int32)
+    lb2 = np.zeros(nligne, np.int32)
+    newinv = np.zeros(nligne, np.int32)
+    newp = np.zeros(s["npoint"] + 1, np.int32)
+    ianew = np.zeros(natome, np.int32)
+    distai = np.sqrt((s["posi"] ** 2).sum(axis=0))
+    lib().newind_(A.arr(distai, np.float64), p(ianew), A.arr(s["ibord"], np.int32), A.i(0), A.b(False),
+                  A.arr(s["isrt"], np.int32), A.arr(s["ivois"], np.int32), p(lb1), p(lb2), A.i(0), A.i(1),
+                  A.i(natome), A.arr(s["nbordf"], np.int32), A.i(s["nbtm"]), p(newp), p(newinv), A.i(nligne),
+                  A.arr(nlmsa, np.int32), A.i(nlmso), A.i(s["npoint"]), A.i(s["nsortf"]), A.i(2),
+                  A.i(s["npsom"]), A.i(s["nstm"]), A.arr(s["numia"], np.int32), A.i(s["ivois"].shape[1]),
+                  A.arr(s["xyz"], np.float64))
+    return dict(nligne=nligne, nlmsa=nlmsa, nlmso=nlmso, ianew=ianew, lb1=lb1, lb2=lb2, newinv=newinv, new=newp)
+
+
+def mat_solve_so(s, V, Enervide, Eimag, bes, neu, E_comp, basereel, phiato, gradvr, fn=None):
+    """Reference mat_solve (with its calcMatRow) on an atom_system dict with
+    spin-orbit coupling (Spinorbite = T, nspino = nspinp = 2; one potential,
+    nspin = nspinr = 1; complex arithmetic), rows of spinorbit_layout.
+    phiato: per atom a (nbord, (lmax+1)^2, 2, 2) array, phiato(ib, (L, m),
+    spin component, solution index); gradvr (npoint, 3).  Returns the
+    solution (nligne, 2 nlmso), columns (L, m, spin) with the spin fastest.
+    ``fn`` calls another procedure with the same argument surface."""
+    A = _Args()
+    lay = spinorbit_layout(s)
+    n, nl, ns, lm = lay["nligne"], lay["nlmso"], s["nsort"], s["lmaxso"]
+    natome, nbtm, nlmmax = s["natome"], s["nbtm"], s["nlmmax"]
+    nvois = s["ivois"].shape[1]
+    nlmsam = 2 * nlmmax
+    lato = np.zeros((nlmsam, natome, 1), np.int32, order="F")
+    mato = np.zeros((nlmsam, natome, 1), np.int32, order="F")
+    iato = np.ones((nlmsam, natome, 1), np.int32, order="F")
+    for ia in range(natome):
+        k = 0
+        for L in range(int(s["lmax"][ia]) + 1):
+            for m in range(-L, L + 1):
+                for isol in (1, 2):
+                    lato[k, ia, 0], mato[k, ia, 0], iato[k, ia, 0] = L, m, isol
+                    k += 1
+    lso = np.array([L for L in range(lm + 1) for m in range(-L, L + 1) for _ in (1, 2)], np.int32)
+    mso = np.array([m for L in range(lm + 1) for m in range(-L, L + 1) for _ in (1, 2)], np.int32)
+    iso = np.array([i for L in range(lm + 1) for m in range(-L, L + 1) for i in (1, 2)], np.int32)
+    phi = np.zeros((nbtm, nlmmax, 2, 2, 1, natome, 2), order="F")
+    for ia, x in enumerate(phiato):
+        nb, na = x.shape[:2]
+        phi[:nb, :na, :, :, 0, ia, 0] = np.real(x)
+        phi[:nb, :na, :, :, 0, ia, 1] = np.imag(x)
+    smr = np.zeros((nl, n), order="F")
+    smi = np.zeros((nl, n), order="F")
+    if E_comp:
+        Bc, Nc = np.asfortranarray(bes, np.complex128), np.asfortranarray(neu, np.complex128)
+        Br, Nr = np.zeros(1), np.zeros(1)
+        nsc, nsr = ns, 0
+    else:
+        Bc, Nc = np.zeros(1, np.complex128), np.zeros(1, np.complex128)
+        Br, Nr = np.asfortranarray(bes, np.float64), np.asfortranarray(neu, np.float64)
+        nsc, nsr = 0, ns
+    kar = np.ones((NOPSM, 2), order="F")
+    kari = np.zeros((NOPSM, 2), order="F")
+    gv = np.zeros((s["npoint"], 3, 2), order="F")
+    gv[:, :, 0] = gradvr
+    gv[:, :, 1] = gradvr
+    tf, tt = np.zeros(1), np.zeros(1)
+    p = lambda a: a.ctypes.data_as(C.c_void_p)
+    npsom = s["npsom"]
+    args = [
+        A.b(False), A.b(basereel), p(Bc), p(Br), A.b(True), A.arr(s["cgrad"], np.float64),
+        A.arr(s["clapl"], np.float64), A.b(E_comp), A.d(Eimag), A.d(Enervide), p(gv),
+        A.arr(lay["ianew"], np.int32), p(iato), A.arr(s["ibord"], np.int32), A.i(0), A.i(1), A.i(1), A.i(0),
+        A.b(False), A.arr(s["isbord"], np.int32), A.arr(iso, np.int32), A.i(1), A.arr(s["isrt"], np.int32),
+        A.arr(s["isvois"], np.int32), A.arr(s["ivois"], np.int32), A.arr(kar, np.float64), A.arr(kari, np.float64),
+        p(lato), A.arr(lay["lb1"], np.int32), A.arr(lay["lb2"], np.int32), A.i(lm), A.arr(lso, np.int32),
+        p(mato), A.i(1), A.i(0), A.arr(mso, np.int32), A.i(1), A.i(1), A.i(natome), A.i(s["nbm"]),
+        A.arr(s["nbord"], np.int32), A.arr(s["nbordf"], np.int32), A.i(nbtm), p(Nc), p(Nr),
+        A.arr(lay["new"], np.int32), A.arr(lay["newinv"], np.int32), A.i(1), A.i(s["npoint"]), A.i(s["npoint"]),
+        A.i(n), A.i(n), A.i(n - nl), A.i(nlmsam), A.i(nlmmax), A.i(nlmmax), A.i(s["nlmso"]),
+        A.arr(lay["nlmsa"], np.int32), A.i(nl), A.i(nl), A.i(nbtm), A.i(2),
+        A.i(s["npoint"]), A.i(npsom), A.i(max(s["nsm"], 1)), A.i(nl), A.i(ns), A.i(nsc), A.i(nsr),
+        A.i(s["nsortf"]), A.i(1), A.i(2), A.i(2), A.i(1), A.i(s["nstm"]), A.arr(s["numia"], np.int32), A.i(nvois),
+        p(phi), A.arr(s["poidsa"], np.float64), A.arr(s["poidso"], np.float64), A.b(False), A.b(False),
+        A.arr(s["rvol"][: s["npoint"]], np.float64), p(smi), p(smr), A.b(True), A.i(0), p(tf), p(tt),
+        A.arr(np.asarray(V, np.float64), np.float64), A.b(False), A.arr(s["ylmato"], np.float64),
+        A.arr(s["ylmso"], np.float64),
+    ]
+    assert len(args) == 93
+    (fn or lib().mat_solve_)(*args)
+    return smr.T + 1j * smi.T
+
+
 def _phi_array(s, phiato, phiato2=None):
     """phiato(nphiato1, nlmagm, nspinp, nspino, n_RI, natome, nphiato7) from
     per-atom (nbord, nlmsa) arrays of the regular (and irregular) atomic
