@@ -9,6 +9,7 @@
 #include "../../include/fdmgpu.h"
 #include "assemble.h"
 #include "batch.h"
+#include "sphere.h"
 #include "tau.h"
 
 using namespace fdmgpu;
@@ -632,6 +633,53 @@ int fdmgpu_assembler_run(fdmgpu_assembler *a, int count, const double *Enervide,
                          const double *const *d_V, double *const *d_vals, double *const *d_rhs) {
   return fdmgpu_assembler_run_atoms(a, count, Enervide, Eimag, Ecinetic_out, E_comp, basereel, update_only, d_V,
                                     nullptr, 0, d_vals, d_rhs);
+}
+
+struct fdmgpu_sphere {
+  SphereSolver s;
+};
+
+int fdmgpu_sphere_create(const fdmgpu_sphere_tables *t, int max_points, void *hip_stream, fdmgpu_sphere **out) {
+  if (!t || !out) {
+    set_error("fdmgpu_sphere_create: invalid argument");
+    return FDMGPU_EINVAL;
+  }
+  int rc = ensure_init();
+  if (rc) return rc;
+  std::unique_ptr<fdmgpu_sphere> h(new (std::nothrow) fdmgpu_sphere());
+  if (!h) return FDMGPU_ENOMEM;
+  rc = h->s.init(t, max_points, hip_stream ? (hipStream_t)hip_stream : g_lib.stream);
+  if (rc) return rc;
+  *out = h.release();
+  return 0;
+}
+
+void fdmgpu_sphere_destroy(fdmgpu_sphere *s) { delete s; }
+
+int fdmgpu_sphere_run(fdmgpu_sphere *s, int count, const double *Ecinetic, const double *Eimag,
+                      const double *Enervide, int complex_out, double *const *d_phi, double *const *d_tau) {
+  if (!s || !Ecinetic || !Eimag || !Enervide || !d_phi || count < 1 || count > SPH_MAXP) {
+    set_error("fdmgpu_sphere_run: invalid argument");
+    return FDMGPU_EINVAL;
+  }
+  SphPoints P{};
+  for (int k = 0; k < count; k++) {
+    if (!d_phi[k]) {
+      set_error("fdmgpu_sphere_run: null phiato buffer for point " + std::to_string(k));
+      return FDMGPU_EINVAL;
+    }
+    if (!complex_out && (std::fabs(Eimag[k]) > 1e-10 || Ecinetic[k] < 1e-10)) {
+      set_error("fdmgpu_sphere_run: complex radial solutions (Eimag != 0 or Ecinetic < 1e-10) need complex_out");
+      return FDMGPU_EINVAL;
+    }
+    P.ecin[k] = Ecinetic[k];
+    P.eimag[k] = Eimag[k];
+    P.ener[k] = Enervide[k];
+    P.phi[k] = d_phi[k];
+    P.tau[k] = d_tau ? d_tau[k] : nullptr;
+  }
+  FDM_HIP(hipSetDevice(g_lib.device));
+  return s->s.run(count, P, complex_out != 0);
 }
 
 int fdmgpu_tau_reduce(const fdmgpu_tau_desc *d, int count, const double *const *d_x, double *const *d_tau,

@@ -338,6 +338,50 @@ int fdmgpu_assembler_run_rel(fdmgpu_assembler *a, int count, const double *Enerv
                              int phi_cplx, double *const *d_vals, double *const *d_rhs);
 
 /*
+ * The atoms' radial solutions at their border points, phiato (Sphere,
+ * sphere.f90:8-210, per energy in the reference's energy loop,
+ * fdm.f90:3733): per atom the radial Schrodinger equation on its radial
+ * grid with a spherical potential (mod_V :216, coef_sch_rad :324,
+ * Sch_radial :413: series at the origin, outward three-point recurrence),
+ * matched to the normalised Bessel and Hankel functions at rmtg (Renormal
+ * :931, cal_ampl :1157: amplitude Wronskout / W(u, j), Tau = -W(u, j) /
+ * W(u, -i h)), then at each border point the solution of each L
+ * interpolated quadratically at the point's distance (cal_phiato :3186,
+ * f_interp2 tools.f90:504) times the real harmonic Ylmato.  One spin, no
+ * spin-orbit or Hubbard term, the regular solution, Renorm = T; with
+ * relativiste the scalar-relativistic coefficients of coef_sch_rad.
+ * The output layout is the device assembler's phiato (per atom an
+ * (nbord, nlmsa) row-major block, atoms concatenated), so a call can feed
+ * fdmgpu_assembler_run_atoms directly.
+ */
+typedef struct fdmgpu_sphere fdmgpu_sphere;
+typedef struct {
+  int natome;
+  const int *nr;            /* [natome] radial points of each atom (>= 4) */
+  const double *r;          /* [sum nr] radial grids (bohr, increasing from r > 0), concatenated */
+  const double *V;          /* [sum nr] spherical potentials (Ry) */
+  const int *Z;             /* [natome] atomic numbers */
+  const int *lmax;          /* [natome] (0..12); nlmsa = (lmax + 1)^2 */
+  const double *rmtg;       /* [natome] matching radii (bohr) */
+  const int *nbord;         /* [natome] border points */
+  const double *bxyz;       /* [3 sum nbord] their positions (bohr), atom after atom */
+  const double *pos;        /* [3 natome] atom positions (bohr) */
+  const double *ylmato;     /* [sum nbord * nlmsa] real harmonics about the atom at them, (nbord, nlmsa) per atom */
+  int relativiste;          /* the scalar-relativistic radial equation (Relativiste) */
+  double V_intmax;          /* V is capped at V_intmax when V_intmax < 1000 (mod_V) */
+} fdmgpu_sphere_tables;
+/* max_points (1..16) energy points per call; hip_stream may be NULL. */
+int fdmgpu_sphere_create(const fdmgpu_sphere_tables *t, int max_points, void *hip_stream, fdmgpu_sphere **out);
+void fdmgpu_sphere_destroy(fdmgpu_sphere *s);
+/* count points: host Ecinetic / Eimag / Enervide [count] (Ry); d_phi[k]:
+ * device phiato of point k (sum nbord * nlmsa values; interleaved (re, im)
+ * when complex_out, else the real parts, which are the solution when Eimag
+ * = 0 and Ecinetic >= 1e-10); d_tau: NULL or per point a device array of
+ * sum (lmax + 1) complex Tau (interleaved).  Asynchronous on the stream. */
+int fdmgpu_sphere_run(fdmgpu_sphere *s, int count, const double *Ecinetic, const double *Eimag,
+                      const double *Enervide, int complex_out, double *const *d_phi, double *const *d_tau);
+
+/*
  * Post-solve reduction to the atom amplitude products (mat.f90:297-405,
  * regular solution, nspino = 1, one representation), on solutions already
  * on the device.  Replaces the CPU loop after mat_solve in mat():

@@ -1729,3 +1729,185 @@ void orc_tau_ri(int nlmso, const double *xr_re, const double *xr_im, const int *
     off += (size_t)n_ri * na * na;
   }
 }
+
+/* ---------------------------------------------------------------- Sphere */
+/* Cal_Hankel (Second_kind = F) and Cal_Bessel, sphere.f90:7255-7350: the
+ * normalised h_L(k r), j_L(k r) and their derivatives in r, L = 0..lmax */
+static void orc_hankel_bessel(double complex konde, int lmax, double r, double complex *hs, double complex *dhs,
+                              double complex *bs, double complex *dbs) {
+  double complex z = konde * r;
+  double complex ht[64], bt[64];
+  ht[0] = -I;
+  ht[1] = -1.0 - I / z;
+  for (int L = 2; L <= lmax + 1; L++) ht[L] = ((double)(2 * L - 1) * ht[L - 1]) / z - ht[L - 2];
+  double complex fnorm = csqrt(konde / PI);
+  double complex cfac = fnorm * cexp(I * z) / z;
+  for (int L = 0; L <= lmax; L++) {
+    double complex d = ((double)L * ht[L]) / z - ht[L + 1];
+    hs[L] = cfac * ht[L];
+    dhs[L] = cfac * konde * d;
+  }
+  bt[0] = csin(z);
+  bt[1] = -ccos(z) + bt[0] / z;
+  for (int L = 2; L <= lmax + 1; L++) bt[L] = ((double)(2 * L - 1) * bt[L - 1]) / z - bt[L - 2];
+  cfac = fnorm / z;
+  for (int L = 0; L <= lmax; L++) {
+    double complex d = ((double)L * bt[L]) / z - bt[L + 1];
+    bs[L] = cfac * bt[L];
+    dbs[L] = cfac * konde * d;
+  }
+}
+
+/* f_interp2, tools.f90:504-522 */
+static double orc_interp2(double r, double rm, double r0, double rp, double fm, double f0, double fp) {
+  double dp0 = rp - r0, dpm = rp - rm, d0m = r0 - rm;
+  double a = (fm * dp0 - f0 * dpm + fp * d0m) / (d0m * dp0 * dpm);
+  double b = (f0 - fm) / d0m - a * (r0 + rm);
+  double c = f0 - a * (r0 * r0) - b * r0;
+  return a * (r * r) + b * r + c;
+}
+
+int orc_sphere(int nr, const double *r, const double *Vr, int numat, double rmtg, int lmax, int relativiste,
+               double Ecinetic, double Eimag, double Enervide, double V_intmax, int nbord, const double *bxyz,
+               const double *pos, const double *ylmato, int nlm, double *phi_re, double *phi_im, double *tau_re,
+               double *tau_im) {
+  if (nr < 4 || lmax < 0 || lmax > 60 || nlm < (lmax + 1) * (lmax + 1)) return -1;
+  const int n = nr;
+  double *V = malloc(sizeof(double) * n), *f2 = malloc(sizeof(double) * n), *g0 = malloc(sizeof(double) * n),
+         *gm = malloc(sizeof(double) * n), *gp = malloc(sizeof(double) * n), *gpi = malloc(sizeof(double) * n),
+         *cgm = malloc(sizeof(double) * n), *cgp = malloc(sizeof(double) * n), *cg0 = malloc(sizeof(double) * n),
+         *ur = malloc(sizeof(double) * n), *ui = malloc(sizeof(double) * n);
+  if (!V || !f2 || !g0 || !gm || !gp || !gpi || !cgm || !cgp || !cg0 || !ur || !ui) return -1;
+  /* mod_V, sphere.f90:216-268: spherical potential, capped at V_intmax */
+  for (int ir = 0; ir < n; ir++) V[ir] = Vr[ir];
+  if (V_intmax < 1000.0)
+    for (int ir = 0; ir < n; ir++) V[ir] = fmin(V[ir], V_intmax);
+  int nrmtg;
+  for (nrmtg = 1; nrmtg <= n - 1; nrmtg++)
+    if (r[nrmtg - 1] > rmtg + EPS10) break;
+  /* coef_sch_rad, sphere.f90:324-405 */
+  for (int i = 1; i <= n; i++) {
+    double rp = (i == n) ? r[n - 1] * r[n - 1] / r[n - 2] : r[i];
+    double rm = (i == 1) ? r[0] * r[0] / r[1] : r[i - 2];
+    double r0 = r[i - 1];
+    double dr = 0.5 * (rp - rm);
+    double claplm = 1 / ((r0 - rm) * dr), claplp = 1 / ((rp - r0) * dr);
+    double clapl0 = -claplm - claplp;
+    if (relativiste) {
+      cgm[i - 1] = (rp - r0) / ((rm - r0) * (rp - rm));
+      cgp[i - 1] = (rm - r0) / ((rp - r0) * (rm - rp));
+      cg0[i - 1] = -cgm[i - 1] - cgp[i - 1];
+    }
+    f2[i - 1] = 1 / (r[i - 1] * r[i - 1]);
+    double vme = V[i - 1] - Enervide;
+    g0[i - 1] = -clapl0 + vme;
+    gm[i - 1] = -claplm;
+    gp[i - 1] = -claplp;
+    if (relativiste) {
+      const double a2s4 = 0.25 * (ALFA_SF * ALFA_SF);
+      double bder = 1 / (1 - a2s4 * vme), dvr;
+      if (i == 1)
+        dvr = (double)(2 * numat) / (r[0] * r[0]);
+      else if (i == n)
+        dvr = cgm[i - 1] * V[i - 2] + cg0[i - 1] * V[i - 1] + cgp[i - 1] * V[i - 1];
+      else
+        dvr = cgm[i - 1] * V[i - 2] + cg0[i - 1] * V[i - 1] + cgp[i - 1] * V[i];
+      double fac = a2s4 * bder * dvr;
+      g0[i - 1] = g0[i - 1] - a2s4 * (vme * vme) - fac * (cg0[i - 1] - 1 / r[i - 1]);
+      gm[i - 1] = gm[i - 1] - fac * cgm[i - 1];
+      gp[i - 1] = gp[i - 1] - fac * cgp[i - 1];
+    }
+  }
+  for (int i = 0; i < n - 1; i++) gpi[i] = 1 / gp[i];
+  const int ecomp = fabs(Eimag) > EPS10 || Ecinetic < EPS10; /* Radial_comp = Ecomp (no Hubbard) */
+  const double complex konde = csqrt(CMPLX(Ecinetic, Eimag));
+  double complex hs[64], dhs[64], bs[64], dbs[64];
+  orc_hankel_bessel(konde, lmax, rmtg, hs, dhs, bs, dbs);
+  const int inr = nrmtg < n ? nrmtg : n;
+  const double d12 = r[inr - 2] - r[inr - 3], d01 = r[inr - 1] - r[inr - 2], d02 = r[inr - 1] - r[inr - 3];
+  const double d2 = 1 / (d12 * d02), d1 = 1 / (d12 * d01), d0 = 1 / (d01 * d02);
+  double *rrel = malloc(sizeof(double) * (nbord > 0 ? nbord : 1));
+  int *iint = malloc(sizeof(int) * (nbord > 0 ? nbord : 1));
+  for (int ib = 0; ib < nbord; ib++) { /* posrel (P1): distance to the atom; cal_phiato's three radii */
+    double u0 = bxyz[3 * ib] - pos[0], u1 = bxyz[3 * ib + 1] - pos[1], u2 = bxyz[3 * ib + 2] - pos[2];
+    rrel[ib] = sqrt(u0 * u0 + u1 * u1 + u2 * u2);
+    int i;
+    for (i = 2; i <= n; i++)
+      if (r[i - 1] > rrel[ib]) break;
+    if (i > n) i = n;
+    if (i < 3) i = 3;
+    iint[ib] = i;
+  }
+  memset(phi_re, 0, sizeof(double) * (size_t)nbord * nlm);
+  memset(phi_im, 0, sizeof(double) * (size_t)nbord * nlm);
+  for (int L = 0; L <= lmax; L++) {
+    /* Sch_radial, sphere.f90:413-730 (one spin, no spin-orbit, no Hubbard) */
+    const int L2 = L * (L + 1);
+    const double ci = -Eimag / (4 * L + 6);
+    const double br = -(numat / (L + 1.0));
+    const double cr = -((double)(2 * numat) * br + Ecinetic) / (4 * L + 6);
+    double p = relativiste ? sqrt((double)(L * L + L + 1) - (ALFA_SF * numat) * (ALFA_SF * numat)) : L + 1.0;
+    for (int k = 0; k < 2; k++) {
+      ur[k] = 1.0 * (1 + br * r[k] + cr * (r[k] * r[k])) * pow(r[k], p);
+      ui[k] = ecomp ? 1.0 * ci * pow(r[k], p + 2) : 0.0;
+    }
+    for (int i = 2; i <= n - 1; i++) { /* 1-based ir; im = ir - 1, ip = ir + 1 */
+      const int ir = i - 1, im = i - 2, ip = i;
+      double td = g0[ir] + L2 * f2[ir];
+      double a = td * ur[ir] + gm[ir] * ur[im];
+      double b = 0;
+      if (ecomp) {
+        b = td * ui[ir] + gm[ir] * ui[im];
+        a = a + Eimag * ui[ir];
+        b = b - Eimag * ur[ir];
+      }
+      ur[ip] = -a * gpi[ir];
+      ui[ip] = ecomp ? -b * gpi[ir] : 0.0;
+    }
+    for (int i = 0; i < n; i++) {
+      ur[i] = ur[i] / r[i];
+      if (ecomp) ui[i] = ui[i] / r[i];
+    }
+    /* Renormal, sphere.f90:931-1150, and cal_ampl :1157-1190 (nlm1 = nlm2 = 1, nspino = 1) */
+    double a = ur[inr - 3] * d2 - ur[inr - 2] * d1 + ur[inr - 1] * d0;
+    double b = (ur[inr - 2] - ur[inr - 3]) / d12 - a * (r[inr - 2] + r[inr - 3]);
+    double c = ur[inr - 1] - a * (r[inr - 1] * r[inr - 1]) - b * r[inr - 1];
+    double ai = 0, bi = 0, cc = 0;
+    if (ecomp) {
+      ai = ui[inr - 3] * d2 - ui[inr - 2] * d1 + ui[inr - 1] * d0;
+      bi = (ui[inr - 2] - ui[inr - 3]) / d12 - ai * (r[inr - 2] + r[inr - 3]);
+      cc = ui[inr - 1] - ai * (r[inr - 1] * r[inr - 1]) - bi * r[inr - 1];
+    }
+    double complex u1 = CMPLX(a * (rmtg * rmtg) + b * rmtg + c, ai * (rmtg * rmtg) + bi * rmtg + cc);
+    double complex u2 = CMPLX(2 * a * rmtg + b, 2 * ai * rmtg + bi);
+    double wout = 1 / (PI * (rmtg * rmtg));
+    double complex s1 = -I * hs[L], s2 = -I * dhs[L];
+    double complex ws = u1 * s2 - u2 * s1, we = u1 * dbs[L] - u2 * bs[L];
+    double complex ampl = wout / we;
+    double complex tau = -we / ws;
+    if (tau_re) tau_re[L] = creal(tau);
+    if (tau_im) tau_im[L] = cimag(tau);
+    for (int i = 0; i < n; i++) {
+      double complex z = ampl * (ecomp ? CMPLX(ur[i], ui[i]) : CMPLX(ur[i], 0.0));
+      ur[i] = creal(z);
+      ui[i] = ecomp ? cimag(z) : 0.0;
+    }
+    /* cal_phiato, sphere.f90:3186-3345 */
+    const int Lm0 = L * L + L + 1;
+    for (int ib = 0; ib < nbord; ib++) {
+      const int i = iint[ib];
+      const double rm = r[i - 3], r0 = r[i - 2], rp = r[i - 1];
+      double ph = orc_interp2(rrel[ib], rm, r0, rp, ur[i - 3], ur[i - 2], ur[i - 1]);
+      double phi = ecomp ? orc_interp2(rrel[ib], rm, r0, rp, ui[i - 3], ui[i - 2], ui[i - 1]) : 0.0;
+      for (int mpp = -L; mpp <= L; mpp++) {
+        const int Lm = Lm0 + mpp; /* 1-based */
+        const double y = ylmato[(size_t)ib * nlm + Lm - 1];
+        phi_re[(size_t)ib * nlm + Lm - 1] = phi_re[(size_t)ib * nlm + Lm - 1] + ph * y;
+        if (ecomp) phi_im[(size_t)ib * nlm + Lm - 1] = phi_im[(size_t)ib * nlm + Lm - 1] + phi * y;
+      }
+    }
+  }
+  free(V); free(f2); free(g0); free(gm); free(gp); free(gpi); free(cgm); free(cgp); free(cg0); free(ur); free(ui);
+  free(rrel); free(iint);
+  return ecomp;
+}

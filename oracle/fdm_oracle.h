@@ -165,6 +165,24 @@ int orc_tau(int nligne, int nlmso, int nligneso, int cal_comp, int basereel, int
             const double *x_re, const double *x_im, int natome, const int *ianew, const int *nlmsa,
             double *tau_re, double *tau_im, double *norm_re, double *norm_im);
 
+/* Sphere (sphere.f90:8-210) for one atom with a spherical potential, one
+ * spin, no spin-orbit or Hubbard term, the regular solution (n_RI = 1),
+ * renormalised (Renorm = T) and real harmonics: mod_V (:216), coef_sch_rad
+ * (:324, relativiste selects its Relativiste terms), Sch_radial (:413),
+ * Renormal (:931) with cal_ampl (:1157), and cal_phiato (:3186).
+ *   r[nr] radial grid, Vr[nr] potential (Ry), numat the atomic number,
+ *   rmtg the matching radius, lmax, Ecinetic / Eimag / Enervide (Ry),
+ *   V_intmax (cap on V when below 1000),
+ *   bxyz[3 nbord] the atom's border points, pos[3] the atom (bohr),
+ *   ylmato[nbord * nlm] (Lm fastest) the real harmonics there,
+ *   phi_re/phi_im[nbord * nlm]: phiato(ib, Lm) out,
+ *   tau_re/tau_im[lmax + 1]: the atomic amplitude Tau of each L (or NULL).
+ * Returns 1 when the radial solutions are complex (Radial_comp: complex
+ * energy or Ecinetic < 1e-10), 0 when real, -1 on a bad argument. */
+int orc_sphere(int nr, const double *r, const double *Vr, int numat, double rmtg, int lmax, int relativiste,
+               double Ecinetic, double Eimag, double Enervide, double V_intmax, int nbord, const double *bxyz,
+               const double *pos, const double *ylmato, int nlm, double *phi_re, double *phi_im, double *tau_re,
+               double *tau_im);
 /* Spherical Bessel/Neumann recurrences (sphere.f90:7226-7248, 7356-7378). */
 void orc_cbessneu(double fn_re, double fn_im, double z_re, double z_im, int lmax,
                   double *bes, double *neu);

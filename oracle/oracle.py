@@ -50,6 +50,9 @@ def lib():
         L.orc_cluster_set_relativistic.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.orc_cluster_cgrad.argtypes = [C.c_void_p, _dp]
         L.orc_cluster_set_ylm_comp.argtypes = [C.c_void_p, C.c_int]
+        L.orc_sphere.argtypes = [C.c_int, _dp, _dp, C.c_int, C.c_double, C.c_int, C.c_int, C.c_double, C.c_double,
+                                 C.c_double, C.c_double, C.c_int, _dp, _dp, _dp, C.c_int, _dp, _dp, C.c_void_p,
+                                 C.c_void_p]
         L.orc_cluster_free.argtypes = [C.c_void_p]
         L.orc_cluster_sizes.argtypes = [C.c_void_p, _ip]
         L.orc_cluster_xyz.argtypes = [C.c_void_p, _dp]
@@ -389,6 +392,24 @@ def tau(x, nligneso: int, basereel: bool, fdm_comp: bool, atoms, tau0=None, want
         off += k * k
     norm = (nr + 1j * ni).reshape(nl, nl).T.copy() if want_norm else None
     return out, norm, info
+
+
+def sphere(r, Vr, numat, rmtg, lmax, relativiste, Ecinetic, Eimag, Enervide, bxyz, pos, ylmato, V_intmax=1.0e6):
+    """orc_sphere: phiato (nbord, nlm) complex at the atom's border points
+    bxyz (nbord, 3) with their real harmonics ylmato (nbord, nlm), and the
+    atomic amplitudes Tau of L = 0..lmax; the flag of complex radial
+    solutions."""
+    ylm = np.ascontiguousarray(ylmato, np.float64)
+    nb, nlm = ylm.shape
+    pr, pi = np.zeros(nb * nlm), np.zeros(nb * nlm)
+    tr, ti = np.zeros(lmax + 1), np.zeros(lmax + 1)
+    rc = lib().orc_sphere(len(r), np.ascontiguousarray(r, np.float64), np.ascontiguousarray(Vr, np.float64),
+                          int(numat), float(rmtg), int(lmax), int(bool(relativiste)), float(Ecinetic), float(Eimag),
+                          float(Enervide), float(V_intmax), nb, np.ascontiguousarray(bxyz, np.float64).reshape(-1),
+                          np.ascontiguousarray(pos, np.float64), ylm.reshape(-1), nlm, pr, pi, _ptr(tr), _ptr(ti))
+    if rc < 0:
+        raise ValueError("orc_sphere: bad argument")
+    return (pr + 1j * pi).reshape(nb, nlm), tr + 1j * ti, bool(rc)
 
 
 def tau_sym(taus, ylm_comp: bool = False):

@@ -26,6 +26,9 @@ clusters and stores inputs and outputs as .npz fixtures:
                           -> the solution for the three branch sets
   mat_solve with complex harmonics in the outer sphere (Ylm_comp)
                           -> the solution for three branch sets
+  Sphere (sphere.f90:8) for each atom of that cluster
+                          -> phiato at the border points and Tau, complex
+                             and real energy, with and without Relativiste
   mat (mat.f90:5) on that cluster
                           -> Taull after its symmetrisation (real and complex
                              basis), and with FDM_comp at complex energy the
@@ -237,6 +240,41 @@ def ylm_fixture():
 YLM_BRANCHES = [(0.7, 0.05, True, False), (0.7, 0.0, False, False), (0.7, 0.0, False, True)]
 
 
+def sphere_inputs(s, ia):
+    """A radial grid (geometric from 1e-4 bohr, ratio 1.03, to 1.3 x the
+    farthest border point) and a screened Coulomb potential for atom ia."""
+    nb = int(s["nbord"][ia])
+    bx = s["xyz"][:3, s["ibord"][:nb, ia] - 1].T
+    dmax = np.linalg.norm(bx - s["posi"][:, ia], axis=1).max()
+    r = [1e-4]
+    while r[-1] < 1.3 * dmax:
+        r.append(r[-1] * 1.03)
+    r = np.array(r)
+    Z = (26, 14, 8)[ia]
+    return r, -2 * Z / r * np.exp(-r / 0.4) - 0.5, Z
+
+
+SPHERE_CASES = [(0.7, 0.05, False), (0.9, 0.0, False), (0.7, 0.05, True), (0.9, 0.0, True)]
+
+
+def sphere_fixture():
+    """The reference's Sphere (sphere.f90:8) for each atom of the three-atom
+    cluster: phiato at its border points and the atomic Tau, complex and
+    real energy, with and without the scalar-relativistic terms."""
+    a = ATOMS
+    posi = np.array(a["posi"]) / BOHR
+    s = R.atom_system(a["R"] / BOHR, a["h"] / BOHR, a["lmaxso"], posi, np.array(a["rmt"]) / BOHR, a["lmax"])
+    out = {}
+    for ia in range(s["natome"]):
+        r, V, Z = sphere_inputs(s, ia)
+        out[f"r{ia}"], out[f"V{ia}"], out[f"Z{ia}"] = r, V, Z
+        for c, (Ec, Ei, rel) in enumerate(SPHERE_CASES):
+            phi, tau = R.sphere(s, ia + 1, r, V, Z, a["rmt"][ia] / BOHR, int(s["lmax"][ia]), rel, Ec, Ei, Ec + 0.3)
+            out[f"phi{ia}_{c}"], out[f"tau{ia}_{c}"] = phi, tau
+    np.savez_compressed(os.path.join(HERE, "sphere_R1.9_h0.25.npz"), **out)
+    print("sphere", [len(out[f"r{ia}"]) for ia in range(s["natome"])])
+
+
 def main():
     if not R.available():
         raise SystemExit("oracle/_ref/libfdmnes_ref.so is missing: make -f oracle/Makefile.ref")
@@ -270,6 +308,7 @@ def main():
     atom_tau_fixture()
     rel_fixtures()
     ylm_fixture()
+    sphere_fixture()
 
 
 if __name__ == "__main__":

@@ -1,0 +1,107 @@
+"""The atoms' radial solutions and phiato on the device (fdmgpu_sphere_*):
+for the three atoms of the atom fixture at complex and real energies, with
+and without the scalar-relativistic terms, the device's phiato at every
+border point and the atomic Tau match the oracle's Sphere restatement
+(pinned to the reference's own Sphere, tests/test_oracle_sphere.py) within
+1e-12; real output equals the complex output's real part and is refused for
+complex radial solutions; and the device phiato, fed to the device assembler
+of the cluster with atoms and solved by the grouped solver, gives the
+solution of the oracle's whole chain (its own Sphere, rows and elimination)
+within 1e-10."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+from .test_oracle_sphere import G, SPHERE, _cluster, border
+
+pytestmark = pytest.mark.gpu
+ENERGIES = [(0.7, 0.05, 1.0), (0.9, 0.0, 1.2), (0.45, 0.02, 0.6), (1.3, 0.0, 1.5)]  # Ecinetic, Eimag, Enervide
+
+
+def _atoms(g, o, f):
+    out = []
+    for ia in range(o.natome):
+        bx, pos, ylm = border(o, g, ia)
+        out.append(dict(r=f[f"r{ia}"], V=f[f"V{ia}"], Z=int(f[f"Z{ia}"]), rmtg=g["rmt"][ia] / BOHR,
+                        lmax=int(g["lmax"][ia]), bxyz=bx, pos=pos, ylmato=ylm))
+    return out
+
+
+BOHR = 0.52917721067
+
+
+@pytest.mark.parametrize("rel", [False, True])
+def test_device_sphere_matches_oracle(rel):
+    import torch
+    from fdmnes_amd.sphere import DeviceSphere
+    g, o = _cluster()
+    f = np.load(SPHERE)
+    atoms = _atoms(g, o, f)
+    print(f"device Sphere: {len(atoms)} atoms, relativistic {rel}, {len(ENERGIES)} energies", flush=True)
+    ds = DeviceSphere(atoms, relativiste=rel, max_points=3)  # four points in two calls
+    phi, tau = ds.run(ENERGIES, complex_out=True, with_tau=True)
+    torch.cuda.synchronize()
+    for k, (Ec, Ei, Ev) in enumerate(ENERGIES):
+        dp, dt = phi[k].cpu().numpy(), tau[k].cpu().numpy()
+        op, ot = [], []
+        for a in atoms:
+            p, t, _ = O.sphere(a["r"], a["V"], a["Z"], a["rmtg"], a["lmax"], rel, Ec, Ei, Ev, a["bxyz"], a["pos"],
+                               a["ylmato"])
+            op.append(p.reshape(-1))
+            ot.append(t)
+        op, ot = np.concatenate(op), np.concatenate(ot)
+        rp = np.abs(dp - op).max() / np.abs(op).max()
+        rt = np.abs(dt - ot).max() / np.abs(ot).max()
+        print(f"energy {k}: phiato rel {rp:.2e}, Tau rel {rt:.2e}", flush=True)
+        assert rp <= 1e-12 and rt <= 1e-12
+    # real output for the real energies
+    real_e = [e for e in ENERGIES if e[1] == 0]
+    pr = ds.run(real_e, complex_out=False)
+    pc = ds.run(real_e, complex_out=True)
+    for a, b in zip(pr, pc):
+        assert torch.equal(a, b.real.contiguous())
+        assert torch.all(b.imag == 0)
+    from fdmnes_amd import _lib
+    with pytest.raises(_lib.FdmGpuError):
+        ds.run(ENERGIES[:1], complex_out=False)
+
+
+def test_device_sphere_feeds_the_assembler_and_solver():
+    import torch
+    from fdmnes_amd import batch as B
+    from fdmnes_amd import cluster as CL
+    from fdmnes_amd.assemble import DeviceAssembler
+    from fdmnes_amd.sphere import DeviceSphere
+    g, o = _cluster()
+    f = np.load(SPHERE)
+    atoms = _atoms(g, o, f)
+    at = (g["posi"] / BOHR, g["rmt"] / BOHR, g["lmax"])
+    c = CL.Cluster(float(g["R"]) / BOHR, float(g["h"]) / BOHR, int(g["lmaxso"]), atoms=at)
+    ens = [(1.0, 0.05, 0.7), (0.8, 0.03, 0.5)]  # (Enervide, Eimag, Ecinetic_out) for the assembler
+    V = g["V"][: c.npoint]
+    ds = DeviceSphere(atoms)
+    phi = ds.run([(Ec, Ei, Ev) for Ev, Ei, Ec in ens], complex_out=True)
+    asm = DeviceAssembler(c, max_points=2)
+    vals = [torch.zeros(c.nnz, dtype=torch.complex128, device="cuda") for _ in ens]
+    rhs = [torch.zeros((c.nligne, c.nlmso), dtype=torch.complex128, device="cuda") for _ in ens]
+    asm.run([torch.from_numpy(V).cuda() for _ in ens], ens, True, False, vals, rhs, phi=phi)
+    lb1, lb2 = c.profile()
+    rp, col, _, _ = c.assemble_atoms(V, *ens[0], True, False, [np.zeros((len(a["bxyz"]), (a["lmax"] + 1) ** 2))
+                                                               for a in atoms])
+    sv = B.DeviceSolver(B.FP64, True, c.nlmso)
+    slots = [sv.add_system(lb1, lb2, rp, col) for _ in ens]
+    xs = [torch.zeros_like(r) for r in rhs]
+    sv.solve(slots, vals, rhs, xs)
+    for (Ev, Ei, Ec), x in zip(ens, xs):
+        ophi = []
+        for a in atoms:
+            p, _, _ = O.sphere(a["r"], a["V"], a["Z"], a["rmtg"], a["lmax"], False, Ec, Ei, Ev, a["bxyz"], a["pos"],
+                               a["ylmato"])
+            ophi.append(p)
+        o.set_phiato(ophi)
+        st, xo, *_ = o.solve(g["V"], Ev, Ei, Ec, True, True, False)
+        assert st == 0
+        rel = np.abs(x.cpu().numpy() - xo).max() / np.abs(xo).max()
+        print(f"energy {Ev}: device Sphere + assembler + solver vs the oracle chain {rel:.2e}", flush=True)
+        assert rel <= 1e-10
