@@ -29,20 +29,35 @@ class EnergyLoop:
     sigma(E) = -Im sum_L b_L^T x_L on them."""
 
     def __init__(self, cluster, precision: int = FP32, group: int = 12, E_comp: bool = True,
-                 basereel: bool = False, atoms=None, fdm_comp: bool = False):
+                 basereel: bool = False, atoms=None, fdm_comp: bool = False, sphere=None):
+        """sphere: for a cluster with muffin-tin atoms, the
+        ``fdmnes_amd.sphere.DeviceSphere`` of its atoms, which computes their
+        phiato for every energy on the device (Sphere, sphere.f90:8); tau is
+        then reduced over the atoms' row blocks unless atoms says otherwise."""
         if not 1 <= group <= 32:
             raise ValueError("group: 1..32 energy points per grouped solve")
         self.cl = cluster
         self.G = int(group)
         self.E_comp, self.basereel, self.fdm_comp = bool(E_comp), bool(basereel), bool(fdm_comp)
         self.cal = self.E_comp or not self.basereel
+        self.natome = int(getattr(cluster, "natome", 0))
+        if bool(self.natome) != (sphere is not None):
+            raise ValueError("a cluster with atoms needs its DeviceSphere (and one without atoms none)")
+        self.sphere = sphere
+        if atoms is None and self.natome:
+            atoms = [(cluster.atom(a)["ianew"], cluster.atom(a)["nlmsa"]) for a in range(self.natome)]
         self.atoms = list(atoms or [])
         self.nso = cluster.nligne - cluster.nlmso
         self.asm = DeviceAssembler(cluster, max_points=min(self.G, 16))
         self.precision = precision
         self.sv = DeviceSolver(precision, self.cal, cluster.nlmso)
         lb1, lb2 = cluster.profile()
-        rp, col, _, _ = cluster.assemble(np.zeros(cluster.npoint), 0.0, 0.0, 1.0, self.E_comp, self.basereel)
+        if self.natome:
+            zero = [np.zeros((cluster.atom(a)["nbord"], cluster.atom(a)["nlmsa"])) for a in range(self.natome)]
+            rp, col, _, _ = cluster.assemble_atoms(np.zeros(cluster.npoint), 0.0, 0.0, 1.0, self.E_comp,
+                                                   self.basereel, zero)
+        else:
+            rp, col, _, _ = cluster.assemble(np.zeros(cluster.npoint), 0.0, 0.0, 1.0, self.E_comp, self.basereel)
         self.slots = [self.sv.add_system(lb1, lb2, rp, col) for _ in range(self.G)]
         dt = torch.complex128 if self.cal else torch.float64
         n, nl = cluster.nligne, cluster.nlmso
@@ -78,8 +93,11 @@ class EnergyLoop:
             # buffers 0..k-1 of a group after the first already hold this
             # structure's rows: only the energy-dependent entries change
             upd = self._filled >= k
+            phi = None
+            if self.sphere is not None:  # the atoms' phiato of these energies (Ecinetic, Eimag, Enervide)
+                phi = self.sphere.run([(e[2], e[1], e[0]) for e in energies[e0:e1]], complex_out=self.cal)
             self.asm.run(vg, energies[e0:e1], self.E_comp, self.basereel, self.vals[:k], self.rhs[:k],
-                         update_only=upd)
+                         update_only=upd, phi=phi)
             self._filled = max(self._filled, k)
             _, sw, _ = self.sv.solve(self.slots[:k], self.vals[:k], self.rhs[:k], self.x[:k])
             sweeps += list(sw)

@@ -105,3 +105,34 @@ def test_device_sphere_feeds_the_assembler_and_solver():
         rel = np.abs(x.cpu().numpy() - xo).max() / np.abs(xo).max()
         print(f"energy {Ev}: device Sphere + assembler + solver vs the oracle chain {rel:.2e}", flush=True)
         assert rel <= 1e-10
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_energy_loop_with_atoms_and_device_sphere(precision):
+    """EnergyLoop over a cluster with atoms, phiato from the device's radial
+    solver per energy: solutions and tau against the oracle's chain (its
+    Sphere, rows, elimination and tau reduction)."""
+    from fdmnes_amd import cluster as CL
+    from fdmnes_amd.energy_loop import EnergyLoop
+    from fdmnes_amd.sphere import DeviceSphere
+    g, o = _cluster()
+    f = np.load(SPHERE)
+    atoms = _atoms(g, o, f)
+    at = (g["posi"] / BOHR, g["rmt"] / BOHR, g["lmax"])
+    c = CL.Cluster(float(g["R"]) / BOHR, float(g["h"]) / BOHR, int(g["lmaxso"]), atoms=at)
+    ens = [(1.0, 0.05, 0.7), (0.8, 0.03, 0.5), (1.3, 0.04, 0.9)]  # (Enervide, Eimag, Ecinetic_out)
+    V = g["V"][: c.npoint]
+    loop = EnergyLoop(c, precision=precision, group=2, sphere=DeviceSphere(atoms))
+    out = loop.run(V, ens, keep_solutions=True)
+    blocks = [(c.atom(a)["ianew"], c.atom(a)["nlmsa"]) for a in range(c.natome)]
+    for k, (Ev, Ei, Ec) in enumerate(ens):
+        o.set_phiato([O.sphere(a["r"], a["V"], a["Z"], a["rmtg"], a["lmax"], False, Ec, Ei, Ev, a["bxyz"],
+                               a["pos"], a["ylmato"])[0] for a in atoms])
+        st, xo, *_ = o.solve(g["V"], Ev, Ei, Ec, True, True, False)
+        assert st == 0
+        rx = np.abs(out["x"][k] - xo).max() / np.abs(xo).max()
+        to = O.tau(xo, o.nligne - o.nlmso, False, False, blocks)[0]
+        tg = out["tau"][k]
+        rt = max(np.abs(a - b).max() / np.abs(b).max() for a, b in zip(tg, to))
+        print(f"precision {precision} energy {k}: solution rel {rx:.2e}, tau rel {rt:.2e}", flush=True)
+        assert rx <= 1e-10 and rt <= 1e-9
