@@ -1767,10 +1767,12 @@ static double orc_interp2(double r, double rm, double r0, double rp, double fm, 
   return a * (r * r) + b * r + c;
 }
 
-int orc_sphere(int nr, const double *r, const double *Vr, int numat, double rmtg, int lmax, int relativiste,
-               double Ecinetic, double Eimag, double Enervide, double V_intmax, int nbord, const double *bxyz,
-               const double *pos, const double *ylmato, int nlm, double *phi_re, double *phi_im, double *tau_re,
-               double *tau_im) {
+/* Sphere's work with, when u_re is given, the renormalised radial solutions
+ * of every L (u_re/u_im[L * nr + ir]) kept as well. */
+static int sphere_impl(int nr, const double *r, const double *Vr, int numat, double rmtg, int lmax, int relativiste,
+                       double Ecinetic, double Eimag, double Enervide, double V_intmax, int nbord, const double *bxyz,
+                       const double *pos, const double *ylmato, int nlm, double *phi_re, double *phi_im,
+                       double *tau_re, double *tau_im, double *u_re, double *u_im) {
   if (nr < 4 || lmax < 0 || lmax > 60 || nlm < (lmax + 1) * (lmax + 1)) return -1;
   const int n = nr;
   double *V = malloc(sizeof(double) * n), *f2 = malloc(sizeof(double) * n), *g0 = malloc(sizeof(double) * n),
@@ -1838,8 +1840,10 @@ int orc_sphere(int nr, const double *r, const double *Vr, int numat, double rmtg
     if (i < 3) i = 3;
     iint[ib] = i;
   }
-  memset(phi_re, 0, sizeof(double) * (size_t)nbord * nlm);
-  memset(phi_im, 0, sizeof(double) * (size_t)nbord * nlm);
+  if (nbord > 0) {
+    memset(phi_re, 0, sizeof(double) * (size_t)nbord * nlm);
+    memset(phi_im, 0, sizeof(double) * (size_t)nbord * nlm);
+  }
   for (int L = 0; L <= lmax; L++) {
     /* Sch_radial, sphere.f90:413-730 (one spin, no spin-orbit, no Hubbard) */
     const int L2 = L * (L + 1);
@@ -1892,6 +1896,11 @@ int orc_sphere(int nr, const double *r, const double *Vr, int numat, double rmtg
       ur[i] = creal(z);
       ui[i] = ecomp ? cimag(z) : 0.0;
     }
+    if (u_re)
+      for (int i = 0; i < n; i++) {
+        u_re[(size_t)L * n + i] = ur[i];
+        u_im[(size_t)L * n + i] = ui[i];
+      }
     /* cal_phiato, sphere.f90:3186-3345 */
     const int Lm0 = L * L + L + 1;
     for (int ib = 0; ib < nbord; ib++) {
@@ -1911,3 +1920,534 @@ int orc_sphere(int nr, const double *r, const double *Vr, int numat, double rmtg
   free(rrel); free(iint);
   return ecomp;
 }
+
+int orc_sphere(int nr, const double *r, const double *Vr, int numat, double rmtg, int lmax, int relativiste,
+               double Ecinetic, double Eimag, double Enervide, double V_intmax, int nbord, const double *bxyz,
+               const double *pos, const double *ylmato, int nlm, double *phi_re, double *phi_im, double *tau_re,
+               double *tau_im) {
+  return sphere_impl(nr, r, Vr, numat, rmtg, lmax, relativiste, Ecinetic, Eimag, Enervide, V_intmax, nbord, bxyz, pos,
+                     ylmato, nlm, phi_re, phi_im, tau_re, tau_im, NULL, NULL);
+}
+
+/* ------------------------------------------------------------------------
+ * The cross-section tensors of the absorbing atom (tensor.f90, sphere.f90
+ * Radial_integral): radial integrals of the golden rule, Gaunt
+ * coefficients, tens_ab's contraction of rof x rof x Taull and tenseur_car's
+ * Cartesian tensors.
+ * ------------------------------------------------------------------------ */
+
+/* f_integr3, tools.f90:308-419, with ir0 = 1: the integral of fct from 0 to
+ * Radius, piecewise cubic through four grid points (the first interval from
+ * 0 with the value at 0 extrapolated by a parabola; the last interval, cut at
+ * Radius, reuses the cubic of the interval before it). */
+double orc_f_integr3(const double *r, const double *fct, int nrm, double Radius) {
+#define RR(i) r[(i)-1]
+#define FF(i) fct[(i)-1]
+  double res = 0;
+  if (Radius < EPS10) return 0;
+  if (nrm < 4) return NAN;
+  const double tiers = 1.0 / 3;
+  int the_end = 0;
+  double r1 = 0, r2 = 0, r3 = 0, r4 = 0, f1 = 0, f2 = 0, f3 = 0, f4 = 0, rm = 0, rp = 0;
+  for (int i = 1; i <= nrm - 1; i++) {
+    if (i == 1) {
+      const double rmm = RR(1), r0 = RR(2), rpp = RR(3), fm = FF(1), f0 = FF(2), fp = FF(3);
+      if (fabs(fp) < 1e-20 && fabs(f0) < 1e-20 && fabs(fm) < 1e-20) continue;
+      const double dp0 = rpp - r0, dpm = rpp - rmm, d0m = r0 - rmm;
+      const double a = (fm * dp0 - f0 * dpm + fp * d0m) / (d0m * dp0 * dpm);
+      const double b = (f0 - fm) / d0m - a * (r0 + rmm);
+      const double c = f0 - a * (r0 * r0) - b * r0;
+      f1 = c;
+      r1 = 0;
+      r2 = RR(i); f2 = FF(i);
+      r3 = RR(i + 1); f3 = FF(i + 1);
+      r4 = RR(i + 2); f4 = FF(i + 2);
+      rm = 0;
+      rp = r3;
+    } else if (i == nrm - 1 || RR(i + 1) > Radius) {
+      rm = RR(i);
+      rp = RR(nrm);
+      if (i < 3) {
+        r1 = RR(i - 1); f1 = FF(i - 1);
+        r2 = RR(i); f2 = FF(i);
+        r3 = RR(i + 1); f3 = FF(i + 1);
+        r4 = RR(i + 2); f4 = FF(i + 2);
+      }
+    } else {
+      r1 = RR(i - 1); f1 = FF(i - 1);
+      r2 = RR(i); f2 = FF(i);
+      r3 = RR(i + 1); f3 = FF(i + 1);
+      r4 = RR(i + 2); f4 = FF(i + 2);
+      rm = r2;
+      rp = r3;
+    }
+    if (rp > Radius) {
+      rp = Radius;
+      the_end = 1;
+    }
+    if (fabs(f2) < 1e-20 && fabs(f3) < 1e-20) continue;
+    const double rap14 = (f1 - f4) / (r1 - r4), rap24 = (f2 - f4) / (r2 - r4), rap34 = (f3 - f4) / (r3 - r4);
+    const double fac1 = (rap14 - rap34) / (r1 - r3), fac2 = (rap24 - rap34) / (r2 - r3);
+    const double a = (fac1 - fac2) / (r1 - r2);
+    const double b = fac1 - a * (r1 + r3 + r4);
+    const double c = rap14 - a * (r1 * r1 + r1 * r4 + r4 * r4) - b * (r1 + r4);
+    const double d = f1 - (a * (r1 * r1) + b * r1 + c) * r1;
+    res = res + (0.25 * (rm + rp) * (rm * rm + rp * rp) * a + tiers * (rm * rm + rm * rp + rp * rp) * b +
+                 0.5 * (rm + rp) * c + d) * (rp - rm);
+    if (the_end) break;
+  }
+  return res;
+#undef RR
+#undef FF
+}
+
+/* coef_init, general.f90:774-839: the core states |j, mj> of the edge(s),
+ * their spin-up / spin-down weights coef_g[init][isp] and orbital m_g. */
+int orc_coef_init(int jseuil, int lseuil, int nbseuil, int ninit, double *coef_g, int *m_g, int *is_g) {
+  for (int k = 0; k < 2 * ninit; k++) {
+    coef_g[k] = 0;
+    m_g[k] = 1000;
+  }
+  int is_init = (jseuil == 1 || jseuil == 3 || jseuil == 5 || jseuil == 7) ? 1 : -1;
+  int init = 0;
+  for (int iseuil = 1; iseuil <= nbseuil; iseuil++) {
+    if (iseuil == 2) is_init = -is_init;
+    const int li = lseuil;
+    const double J = li + 0.5 * is_init;
+    const int ni = (int)lround(2 * J + 1);
+    for (int i = 1; i <= ni; i++) {
+      if (init >= ninit) return -1;
+      init++;
+      const double Jz = -J + i - 1;
+      is_g[init - 1] = is_init;
+      for (int isp = 1; isp <= 2; isp++) {
+        const double spin = isp == 1 ? 0.5 : -0.5;
+        const int is = isp == 1 ? is_init : 1;
+        const int mi = (int)lround(Jz - spin);
+        if (abs(mi) > li) continue;
+        const double ci2 = (li + 0.5 + 2 * spin * is_init * Jz) / (2 * li + 1);
+        if (ci2 < EPS6) continue;
+        coef_g[2 * (init - 1) + isp - 1] = is * sqrt(ci2);
+        m_g[2 * (init - 1) + isp - 1] = mi;
+      }
+    }
+  }
+  return init;
+}
+
+static double fact_d(int n) {
+  double f = 1;
+  for (int k = 2; k <= n; k++) f *= k;
+  return f;
+}
+
+/* <l1 m1 l2 m2 | l3 m1+m2>, the Clebsch-Gordan coefficient that cgc
+ * (tensor.f90:1678-1790, Rose eq. 3.18) computes, here by Racah's closed
+ * formula (Edmonds, Angular Momentum in Quantum Mechanics, eq. 3.6.11). */
+double orc_cgc(int l1, int l2, int l3, int m1, int m2) {
+  const int m3 = m1 + m2;
+  if (l3 < abs(l1 - l2) || l3 > l1 + l2 || abs(m1) > l1 || abs(m2) > l2 || abs(m3) > l3) return 0;
+  const double pre = sqrt((2 * l3 + 1) * fact_d(l1 + l2 - l3) * fact_d(l1 - l2 + l3) * fact_d(-l1 + l2 + l3) /
+                          fact_d(l1 + l2 + l3 + 1) * fact_d(l1 + m1) * fact_d(l1 - m1) * fact_d(l2 + m2) *
+                          fact_d(l2 - m2) * fact_d(l3 + m3) * fact_d(l3 - m3));
+  double s = 0;
+  for (int k = 0; k <= l1 + l2 - l3; k++) {
+    const int a = l1 + l2 - l3 - k, b = l1 - m1 - k, c = l2 + m2 - k, d = l3 - l2 + m1 + k, e = l3 - l1 - m2 + k;
+    if (a < 0 || b < 0 || c < 0 || d < 0 || e < 0) continue;
+    s += ((k & 1) ? -1.0 : 1.0) / (fact_d(k) * fact_d(a) * fact_d(b) * fact_d(c) * fact_d(d) * fact_d(e));
+  }
+  return pre * s;
+}
+
+static int ipow_m1(int k) { return (k & 1) ? -1 : 1; }
+
+/* Gaunt_r, tensor.f90:1796-1866: the integral of three real harmonics
+ * Y_l1m1 Y_l2m2 Y_l3m3, from complex-harmonic Clebsch-Gordan products. */
+double orc_gaunt_r(int l1, int m1, int l2, int m2, int l3, int m3) {
+  if ((l3 + l1 + l2) % 2 != 0 || abs(l1 - l3) > l2 || (l1 + l3) < l2) return 0;
+  const int mi = abs(m1), mgam = abs(m2), m = abs(m3);
+  if (mi + mgam != m && mi + m != mgam && m + mgam != mi) return 0;
+  const double sqr2i = 1 / sqrt(2.0);
+  const int ltsti = l1, itr = l2, ltst = l3;
+  const double pre =
+      sqrt((2 * ltst + 1) * (2 * itr + 1) / ((2 * ltsti + 1) * (4 * PI))) * orc_cgc(ltst, itr, ltsti, 0, 0);
+  const int itypi = m1 == 0 ? 0 : mi / m1, itygam = m2 == 0 ? 0 : mgam / m2, ityp = m3 == 0 ? 0 : m / m3;
+  const int nexp = abs(itypi) + abs(itygam) + abs(ityp);
+  int ns = itypi + ityp + itygam;
+  if (ns == 0) ns = 1;
+  const int nsf = ns > 0 ? 1 : -1;
+  double sp = 1;
+  for (int k = 0; k < nexp; k++) sp *= sqr2i;
+  const double gf = nsf * ipow_m1(mi) * sp * pre;
+  double ab;
+  if (m + mgam + mi == 0) {
+    ab = orc_cgc(ltst, itr, ltsti, 0, 0);
+  } else {
+    const int f1 = itypi + itygam * ityp, f2 = itygam + itypi * ityp, f3 = ityp + itypi * itygam;
+    ab = 0;
+    if (m + mgam == mi && f1 != 0) ab = ipow_m1(mi) * orc_cgc(ltst, itr, ltsti, m, mgam) * f1;
+    if (m + mi == mgam && f2 != 0) ab = ab + ipow_m1(mgam) * orc_cgc(ltst, itr, ltsti, -m, mgam) * f2;
+    if (mgam + mi == m && f3 != 0) ab = ab + ipow_m1(m) * orc_cgc(ltst, itr, ltsti, m, -mgam) * f3;
+  }
+  return gf * ab;
+}
+
+/* Gaunt_xrc, tensor.f90:1870-1912, real harmonics (Ylm_comp false): Y_lm
+ * and Y_lomo real, Y_limi complex. */
+void orc_gaunt_xrc(int l, int m, int lo, int mo, int li, int mi, double *gr, double *gi) {
+  const double s2 = sqrt(2.0);
+  if (mi == 0) {
+    *gr = orc_gaunt_r(l, m, lo, mo, li, mi);
+    *gi = 0;
+  } else if (mi > 0) {
+    *gr = ipow_m1(mi) * orc_gaunt_r(l, m, lo, mo, li, mi) / s2;
+    *gi = ipow_m1(mi) * orc_gaunt_r(l, m, lo, mo, li, -mi) / s2;
+  } else {
+    *gr = orc_gaunt_r(l, m, lo, mo, li, -mi) / s2;
+    *gi = -orc_gaunt_r(l, m, lo, mo, li, mi) / s2;
+  }
+}
+
+/* Radial_integral, sphere.f90:1490-1680, and radial_matrix, :2209-2310 (one
+ * spin, spherical potential, no spin-orbit, Hubbard or full potential, no
+ * TDDFT or NRIXS): Sphere's renormalised regular solution u_L (Sch_radial),
+ * then rof(L, ip, is) = fac(ip) * integral_0^Rmtsd psii_is(r) r u_L(r) r^ip dr
+ * (f_integr3) with fac 0.5 alfa_sf, 1, Vecond / 2, -Vecond^2 / 6 for ip = 0..3,
+ * Vecond = alfa_sf / 2 * max(0.001, Energ + Eseuil(is)); the imaginary part
+ * from Im u_L.  psii[is * nr + ir]; rof out [((is * nip) + ip - ip0) *
+ * (lmax + 1) + L].  Returns Radial_comp (1 complex solutions, 0 real) or -1. */
+int orc_radial_integral(int nr, const double *r, const double *Vr, int numat, double rmtg, double rmtsd, int lmax,
+                        int relativiste, double Ecinetic, double Eimag, double Enervide, double Energ,
+                        double V_intmax, int nbseuil, const double *Eseuil, const double *psii, int ip0, int ip_max,
+                        double *rof_re, double *rof_im) {
+  if (ip0 < 0 || ip_max > 3 || ip_max < ip0 || nbseuil < 1 || nbseuil > 2) return -1;
+  double *ur = malloc(sizeof(double) * (size_t)nr * (lmax + 1)), *ui = malloc(sizeof(double) * (size_t)nr * (lmax + 1));
+  double *fct = malloc(sizeof(double) * nr), *rb = malloc(sizeof(double) * nr);
+  const int ecomp = sphere_impl(nr, r, Vr, numat, rmtg, lmax, relativiste, Ecinetic, Eimag, Enervide, V_intmax, 0,
+                                NULL, NULL, NULL, (lmax + 1) * (lmax + 1), NULL, NULL, NULL, NULL, ur, ui);
+  if (ecomp < 0) return -1;
+  const int nip = ip_max - ip0 + 1;
+  double vecond[2];
+  for (int i = 0; i < nbseuil; i++) vecond[i] = 0.5 * ALFA_SF * fmax(0.001, Energ + Eseuil[i]);
+  for (int L = 0; L <= lmax; L++)
+    for (int ip = ip0; ip <= ip_max; ip++) {
+      for (int ir = 0; ir < nr; ir++) rb[ir] = ip == 0 ? 1.0 : ip == 1 ? r[ir] : ip == 2 ? r[ir] * r[ir]
+                                                                                    : r[ir] * r[ir] * r[ir];
+      for (int is = 0; is < nbseuil; is++) {
+        const double fac = ip == 0 ? 0.5 * ALFA_SF : ip == 1 ? 1.0 : ip == 2 ? 0.5 * vecond[is]
+                                                                          : -(1.0 / 6) * (vecond[is] * vecond[is]);
+        for (int ir = 0; ir < nr; ir++) fct[ir] = psii[(size_t)is * nr + ir] * r[ir] * ur[(size_t)L * nr + ir] * rb[ir];
+        const double radlr = fac * orc_f_integr3(r, fct, nr, rmtsd);
+        double radli = 0;
+        if (ecomp) {
+          for (int ir = 0; ir < nr; ir++)
+            fct[ir] = psii[(size_t)is * nr + ir] * r[ir] * ui[(size_t)L * nr + ir] * rb[ir];
+          radli = fac * orc_f_integr3(r, fct, nr, rmtsd);
+        }
+        const size_t o = ((size_t)is * nip + ip - ip0) * (lmax + 1) + L;
+        rof_re[o] = radlr;
+        rof_im[o] = radli;
+      }
+    }
+  free(ur); free(ui); free(fct); free(rb);
+  return ecomp;
+}
+
+/* tens_ab, tensor.f90:947-1345, for the branch of an FDM absorption
+ * calculation: no TDDFT, NRIXS, Green's-function or FDM_comp output, no
+ * singular solution, no spin-orbit, one final-state spin, spherical
+ * potential, real harmonics, electric multipoles (irang, jrang >= 1). */
+static void tens_ab_r(const orc_tens_in *in, int ninitr, int irang, int jrang, int l_s, int m_s, int l_i, int m_i,
+                      const double complex *rof, const double complex *taull, double complex *Ten) {
+  const int li = in->lseuil, nlm = in->nlm_probe, nip = in->ip_max - in->ip0 + 1;
+  for (int k = 0; k < ninitr; k++) Ten[k] = 0;
+  for (int ig = 1; ig <= in->ninit; ig++) {
+    const int iseuil = ig <= in->ninit1 ? 1 : 2;
+    const int initr = in->core_resolved ? ig : iseuil;
+    for (int isg1 = 1; isg1 <= 2; isg1++) {
+      const int mi1 = in->m_g[2 * (ig - 1) + isg1 - 1];
+      const double ci1 = in->coef_g[2 * (ig - 1) + isg1 - 1];
+      if (fabs(ci1) < EPS6) continue;
+      for (int isg2 = 1; isg2 <= 2; isg2++) {
+        if (li == 0 && isg1 != isg2) continue;
+        const int mi2 = in->m_g[2 * (ig - 1) + isg2 - 1];
+        const double ci2 = in->coef_g[2 * (ig - 1) + isg2 - 1];
+        if (fabs(ci2) < EPS6) continue;
+        const double cc = ci1 * ci2;
+        for (int lf1 = 0; lf1 <= in->lmax; lf1++) {
+          if (in->lplus1 && lf1 < li + 1) continue;
+          if (in->lmoins1 && lf1 > li) continue;
+          if (lf1 > li + irang || lf1 < li - irang || lf1 % 2 != (li + irang) % 2) continue;
+          for (int mf1 = -lf1; mf1 <= lf1; mf1++) {
+            const int lm1 = lf1 * lf1 + lf1 + 1 + mf1;
+            if (lm1 > nlm) continue;
+            for (int ispf1 = 1; ispf1 <= 2; ispf1++) {
+              if (ispf1 != isg1) continue;
+              double gr, gi;
+              orc_gaunt_xrc(lf1, mf1, l_s, m_s, li, mi1, &gr, &gi);
+              const double complex gs = CMPLX(gr, gi);
+              if (cabs(gs) < EPS10) continue;
+              for (int lf2 = 0; lf2 <= in->lmax; lf2++) {
+                if (in->lplus1 && lf2 < li + 1) continue;
+                if (in->lmoins1 && lf2 > li) continue;
+                if (lf2 > li + jrang || lf2 < li - jrang || lf2 % 2 != (li + jrang) % 2) continue;
+                for (int mf2 = -lf2; mf2 <= lf2; mf2++) {
+                  const int lm2 = lf2 * lf2 + lf2 + 1 + mf2;
+                  if (lm2 > nlm) continue;
+                  for (int ispf2 = 1; ispf2 <= 2; ispf2++) {
+                    if (ispf2 != isg2) continue;
+                    if (ispf2 != ispf1) continue;
+                    orc_gaunt_xrc(lf2, mf2, l_i, m_i, li, mi2, &gr, &gi);
+                    const double complex gii = CMPLX(gr, gi);
+                    if (cabs(gii) < EPS10) continue;
+                    const double complex cg = cc * conj(gs) * gii;
+                    const double complex rof1 = rof[((size_t)(iseuil - 1) * nip + irang - in->ip0) * nlm + lm1 - 1];
+                    const double complex rof2 = rof[((size_t)(iseuil - 1) * nip + jrang - in->ip0) * nlm + lm2 - 1];
+                    const double complex cfe = conj(rof1) * rof2 * taull[(size_t)(lm1 - 1) * nlm + lm2 - 1];
+                    const double complex cfs = conj(rof2) * rof1 * taull[(size_t)(lm2 - 1) * nlm + lm1 - 1];
+                    const double complex tau_rad = 0.5 * (cfe - conj(cfs));
+                    Ten[initr - 1] = Ten[initr - 1] + cg * tau_rad;
+                  }
+                }
+              }
+            }
+          }
+        }
+      }
+    }
+  }
+  /* times i, so that the real part is the absorption (tensor.f90:1326-1327) */
+  for (int k = 0; k < ninitr; k++) Ten[k] = I * Ten[k];
+}
+
+/* The transformation coefficients Cartesian -> spherical of the dipole and
+ * quadrupole operators, tenseur_car tensor.f90:240-275 (lm 0-based). */
+static void clm_table(double clm[3][3][3][16]) {
+  memset(clm, 0, sizeof(double) * 3 * 3 * 3 * 16);
+  const double c1 = sqrt(4 * PI / 3);
+  for (int j = 0; j < 3; j++) {
+    clm[1][0][j][3] = c1;
+    clm[1][1][j][1] = c1;
+    clm[1][2][j][2] = c1;
+  }
+  const double c0 = sqrt(4 * PI) / 3, c = sqrt(4 * PI / 15), c3 = c / sqrt(3.0);
+  clm[2][0][0][0] = c0; clm[2][0][0][6] = -c3; clm[2][0][0][8] = c;
+  clm[2][0][1][4] = c;
+  clm[2][0][2][7] = c;
+  clm[2][1][1][0] = c0; clm[2][1][1][6] = -c3; clm[2][1][1][8] = -c;
+  clm[2][1][2][5] = c;
+  clm[2][2][2][0] = c0; clm[2][2][2][6] = 2 * c3;
+  for (int i = 0; i < 3; i++)
+    for (int j = i + 1; j < 3; j++)
+      for (int lm = 0; lm < 16; lm++) clm[2][j][i][lm] = clm[2][i][j][lm];
+}
+
+#define I3(a, b) ((a) * 3 + (b))
+#define I4(a, b, c) (((a) * 3 + (b)) * 3 + (c))
+#define I5(a, b, c, d) ((((a) * 3 + (b)) * 3 + (c)) * 3 + (d))
+
+/* tenseur_car, tensor.f90:15-946, electric dipole and quadrupole tensors
+ * (E1E1, E1E2, E2E2; n_rel = 1, not Green_int): for every independent
+ * Cartesian element (ldip, lqua, the msym tables) the sum of clm x clm x
+ * tens_ab over the spherical components, spread to the elements equal by
+ * symmetry, then turned by rot_int x rot_atom_abs^T.  Outputs complex
+ * interleaved, C order: secdd[initr][k_s][k_i], secdq[initr][k_s][k_i][j_i],
+ * secqq[initr][k_s][j_s][k_i][j_i]. */
+int orc_tenseur(const orc_tens_in *in, const double *rof_re, const double *rof_im, const double *taull_re,
+                const double *taull_im, double *secdd_out, double *secdq_out, double *secqq_out) {
+  const int nlm = in->nlm_probe, nip = in->ip_max - in->ip0 + 1;
+  const int ninitr = in->core_resolved ? in->ninit : in->nbseuil;
+  if (in->lmax > 3 || nlm > 16 || ninitr > 16 || in->ip0 > 1 || in->ip_max < ((in->e1e2 || in->e2e2) ? 2 : 1))
+    return -1;
+  double complex *rof = malloc(sizeof(double complex) * (size_t)in->nbseuil * nip * nlm);
+  double complex *taull = malloc(sizeof(double complex) * (size_t)nlm * nlm);
+  for (size_t k = 0; k < (size_t)in->nbseuil * nip * nlm; k++) rof[k] = CMPLX(rof_re[k], rof_im[k]);
+  for (size_t k = 0; k < (size_t)nlm * nlm; k++) taull[k] = CMPLX(taull_re[k], taull_im[k]);
+  double clm[3][3][3][16];
+  clm_table(clm);
+  double complex *dd = calloc((size_t)ninitr * 9, sizeof(double complex));
+  double complex *dq = calloc((size_t)ninitr * 27, sizeof(double complex));
+  double complex *qq = calloc((size_t)ninitr * 81, sizeof(double complex));
+  double complex Tens_lm[9][9][16], Ten[16], Tens[16];
+  const int nrang = (in->e1e2 || in->e2e2) ? 2 : 1;
+  for (int irang = 1; irang <= nrang; irang++)
+    for (int jrang = irang; jrang <= nrang; jrang++) {
+      memset(Tens_lm, 0, sizeof(Tens_lm));
+      if (!in->e1e1 && irang == 1 && jrang == 1) continue;
+      if (!in->e1e2 && irang == 1 && jrang == 2) continue;
+      if (!in->e2e2 && irang == 2 && jrang == 2) continue;
+      for (int ks = 0; ks < 3; ks++) {
+        if (irang == 1 && in->ldip[ks] != 1) continue;
+        const int njs = irang > 1 ? 3 : 1;
+        for (int js = 0; js < njs; js++) {
+          if (irang == 2 && in->lqua[I3(ks, js)] != 1) continue;
+          for (int ki = 0; ki < 3; ki++) {
+            if (jrang == 1 && in->ldip[ki] != 1) continue;
+            const int nji = jrang > 1 ? 3 : 1;
+            for (int ji = 0; ji < nji; ji++) {
+              if (jrang == 2 && in->lqua[I3(ki, ji)] != 1) continue;
+              double sa = 0;
+              if (irang == 1 && jrang == 1) {
+                if (ki < ks) continue;
+                if (in->msymdd[I3(ks, ki)] == 0 && in->msymddi[I3(ks, ki)] == 0) continue;
+                for (int k = 0; k < ninitr; k++) sa += cabs(dd[k * 9 + I3(ks, ki)]);
+              } else if (irang == 1 && jrang == 2) {
+                if (in->msymdq[I4(ks, ki, ji)] == 0 && in->msymdqi[I4(ks, ki, ji)] == 0) continue;
+                for (int k = 0; k < ninitr; k++) sa += cabs(dq[k * 27 + I4(ks, ki, ji)]);
+              } else {
+                if (in->msymqq[I5(ks, js, ki, ji)] == 0 && in->msymqqi[I5(ks, js, ki, ji)] == 0) continue;
+                for (int k = 0; k < ninitr; k++) sa += cabs(qq[k * 81 + I5(ks, js, ki, ji)]);
+              }
+              if (sa > 1e-15) continue;
+              for (int k = 0; k < ninitr; k++) Tens[k] = 0;
+              int lms = 0;
+              for (int l_s = 0; l_s <= irang; l_s++)
+                for (int m_s = -l_s; m_s <= l_s; m_s++, lms++) {
+                  const double clm_s = clm[irang][ks][js][lms];
+                  if (fabs(clm_s) < EPS10) continue;
+                  int lmi = 0;
+                  for (int l_i = 0; l_i <= jrang; l_i++)
+                    for (int m_i = -l_i; m_i <= l_i; m_i++, lmi++) {
+                      const double clm_i = clm[jrang][ki][ji][lmi];
+                      if (fabs(clm_i) < EPS10) continue;
+                      double st = 0;
+                      for (int k = 0; k < ninitr; k++) st += cabs(Tens_lm[lms][lmi][k]);
+                      if (st < 1e-15) {
+                        tens_ab_r(in, ninitr, irang, jrang, l_s, m_s, l_i, m_i, rof, taull, Ten);
+                        if (in->numat == 1)
+                          for (int k = 0; k < ninitr; k++) Ten[k] = Ten[k] / 2;
+                        for (int k = 0; k < ninitr; k++) Tens_lm[lms][lmi][k] = Ten[k];
+                      }
+                      for (int k = 0; k < ninitr; k++) Tens[k] = Tens[k] + clm_s * clm_i * Tens_lm[lms][lmi][k];
+                    }
+                }
+              /* spread to the elements equal by symmetry (tensor.f90:529-700) */
+              if (irang == 1 && jrang == 1) {
+                const int a0 = in->msymdd[I3(ks, ki)], b0 = in->msymddi[I3(ks, ki)];
+                for (int kks = 0; kks < 3; kks++)
+                  for (int kki = kks; kki < 3; kki++) {
+                    const int a = in->msymdd[I3(kks, kki)], b = in->msymddi[I3(kks, kki)];
+                    if (abs(a) != abs(a0) || abs(b) != abs(b0)) continue;
+                    const int is = a != 0 ? a / a0 : 0, isi = b != 0 ? b / b0 : 0;
+                    for (int k = 0; k < ninitr; k++) {
+                      const double complex te = CMPLX(is * creal(Tens[k]), isi * cimag(Tens[k]));
+                      dd[k * 9 + I3(kks, kki)] = te;
+                      dd[k * 9 + I3(kki, kks)] = conj(te);
+                    }
+                  }
+              } else if (irang == 1 && jrang == 2) {
+                const int a0 = in->msymdq[I4(ks, ki, ji)], b0 = in->msymdqi[I4(ks, ki, ji)];
+                for (int kks = 0; kks < 3; kks++)
+                  for (int kki = 0; kki < 3; kki++)
+                    for (int jji = kki; jji < 3; jji++) {
+                      const int a = in->msymdq[I4(kks, kki, jji)], b = in->msymdqi[I4(kks, kki, jji)];
+                      if (abs(a) != abs(a0) || abs(b) != abs(b0)) continue;
+                      const int is = a != 0 ? a / a0 : 0, isi = b != 0 ? b / b0 : 0;
+                      for (int k = 0; k < ninitr; k++) {
+                        const double complex te = CMPLX(is * creal(Tens[k]), isi * cimag(Tens[k]));
+                        dq[k * 27 + I4(kks, kki, jji)] = te;
+                        dq[k * 27 + I4(kks, jji, kki)] = te;
+                      }
+                    }
+              } else {
+                const int a0 = in->msymqq[I5(ks, js, ki, ji)], b0 = in->msymqqi[I5(ks, js, ki, ji)];
+                for (int kks = 0; kks < 3; kks++)
+                  for (int jjs = 0; jjs < 3; jjs++)
+                    for (int kki = 0; kki < 3; kki++)
+                      for (int jji = 0; jji < 3; jji++) {
+                        double s2 = 0;
+                        for (int k = 0; k < ninitr; k++) s2 += cabs(qq[k * 81 + I5(kks, jjs, kki, jji)]);
+                        if (s2 > 1e-15) continue;
+                        const int a = in->msymqq[I5(kks, jjs, kki, jji)], b = in->msymqqi[I5(kks, jjs, kki, jji)];
+                        if (abs(a) != abs(a0) || abs(b) != abs(b0)) continue;
+                        const int is = a != 0 ? a / a0 : 0, isi = b != 0 ? b / b0 : 0;
+                        for (int k = 0; k < ninitr; k++) {
+                          double complex te = CMPLX(is * creal(Tens[k]), isi * cimag(Tens[k]));
+                          double complex *q = qq + k * 81;
+                          q[I5(kks, jjs, kki, jji)] = te;
+                          q[I5(jjs, kks, kki, jji)] = te;
+                          q[I5(kks, jjs, jji, kki)] = te;
+                          q[I5(jjs, kks, jji, kki)] = te;
+                          te = conj(te);
+                          q[I5(kki, jji, kks, jjs)] = te;
+                          q[I5(jji, kki, kks, jjs)] = te;
+                          q[I5(kki, jji, jjs, kks)] = te;
+                          q[I5(jji, kki, jjs, kks)] = te;
+                        }
+                      }
+              }
+            }
+          }
+        }
+      }
+    }
+  /* rotation to the R1 basis (tensor.f90:781-806; rot_tensor_2/3/4, :2112-2210) */
+  double rot[3][3];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      double t = 0;
+      for (int k = 0; k < 3; k++) t += in->rot_int[I3(i, k)] * in->rot_atom_abs[I3(j, k)];
+      rot[i][j] = t;
+    }
+  for (int k = 0; k < ninitr; k++) {
+    if (in->e1e1) {
+      const double complex *m2 = dd + k * 9;
+      double complex out[9] = {0};
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+          for (int kk = 0; kk < 3; kk++) {
+            double complex cmat = 0;
+            for (int l = 0; l < 3; l++) cmat += rot[j][l] * m2[I3(kk, l)];
+            out[I3(i, j)] = out[I3(i, j)] + rot[i][kk] * cmat;
+          }
+      memcpy(dd + k * 9, out, sizeof(out));
+    }
+    if (in->e1e2) {
+      const double complex *m3 = dq + k * 27;
+      double complex out[27] = {0};
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+          for (int kk = 0; kk < 3; kk++)
+            for (int l = 0; l < 3; l++) {
+              double complex cmas = 0;
+              for (int m = 0; m < 3; m++) {
+                double complex cmat = 0;
+                for (int n = 0; n < 3; n++) cmat += rot[kk][n] * m3[I4(l, m, n)];
+                cmas = cmas + rot[j][m] * cmat;
+              }
+              out[I4(i, j, kk)] = out[I4(i, j, kk)] + rot[i][l] * cmas;
+            }
+      memcpy(dq + k * 27, out, sizeof(out));
+    }
+    if (in->e2e2) {
+      const double complex *m4 = qq + k * 81;
+      double complex out[81] = {0};
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+          for (int kk = 0; kk < 3; kk++)
+            for (int l = 0; l < 3; l++)
+              for (int m = 0; m < 3; m++) {
+                double complex cmar = 0;
+                for (int n = 0; n < 3; n++) {
+                  double complex cmas = 0;
+                  for (int n1 = 0; n1 < 3; n1++) {
+                    double complex cmat = 0;
+                    for (int p = 0; p < 3; p++) cmat += rot[l][p] * m4[I5(m, n, n1, p)];
+                    cmas = cmas + rot[kk][n1] * cmat;
+                  }
+                  cmar = cmar + rot[j][n] * cmas;
+                }
+                out[I5(i, j, kk, l)] = out[I5(i, j, kk, l)] + rot[i][m] * cmar;
+              }
+      memcpy(qq + k * 81, out, sizeof(out));
+    }
+  }
+  for (int k = 0; k < ninitr * 9; k++) { secdd_out[2 * k] = creal(dd[k]); secdd_out[2 * k + 1] = cimag(dd[k]); }
+  for (int k = 0; k < ninitr * 27; k++) { secdq_out[2 * k] = creal(dq[k]); secdq_out[2 * k + 1] = cimag(dq[k]); }
+  for (int k = 0; k < ninitr * 81; k++) { secqq_out[2 * k] = creal(qq[k]); secqq_out[2 * k + 1] = cimag(qq[k]); }
+  free(rof); free(taull); free(dd); free(dq); free(qq);
+  return ninitr;
+}
+#undef I3
+#undef I4
+#undef I5

@@ -190,6 +190,33 @@ void orc_cbessneur(double fnorm, double z, int lmax, double *bes, double *neu);
 /* Real spherical harmonics of a direction (general.f90:11606-11720). */
 void orc_ylm_real(int lmax, const double *v, double r, double *ylmr);
 
+/* The absorbing atom's cross-section tensors (tensor.f90, the radial
+ * integrals of sphere.f90 Radial_integral). */
+double orc_f_integr3(const double *r, const double *fct, int nrm, double Radius);
+int orc_coef_init(int jseuil, int lseuil, int nbseuil, int ninit, double *coef_g, int *m_g, int *is_g);
+double orc_cgc(int l1, int l2, int l3, int m1, int m2);
+double orc_gaunt_r(int l1, int m1, int l2, int m2, int l3, int m3);
+void orc_gaunt_xrc(int l, int m, int lo, int mo, int li, int mi, double *gr, double *gi);
+int orc_radial_integral(int nr, const double *r, const double *Vr, int numat, double rmtg, double rmtsd, int lmax,
+                        int relativiste, double Ecinetic, double Eimag, double Enervide, double Energ,
+                        double V_intmax, int nbseuil, const double *Eseuil, const double *psii, int ip0, int ip_max,
+                        double *rof_re, double *rof_im);
+/* tenseur_car's inputs: the edge (lseuil, core states coef_g / m_g as
+ * [ninit][2], the first edge's ninit1), the probe's lmax and nlm_probe, the
+ * multipoles, the tensor shapes (ldip[3], lqua[3][3]) and symmetry tables
+ * (msym*[...] C order of the Fortran indices), rot_int and rot_atom_abs
+ * ([i][j]), the radial-integral range ip0..ip_max.  rof [is][ip - ip0][lm]
+ * (nlm_probe entries), Taull [lm1][lm2]. */
+typedef struct {
+  int lseuil, ninit, ninit1, nbseuil, core_resolved, lmax, nlm_probe, lplus1, lmoins1, numat;
+  int e1e1, e1e2, e2e2, ip0, ip_max;
+  const double *coef_g;
+  const int *m_g, *ldip, *lqua, *msymdd, *msymddi, *msymdq, *msymdqi, *msymqq, *msymqqi;
+  const double *rot_int, *rot_atom_abs;
+} orc_tens_in;
+int orc_tenseur(const orc_tens_in *in, const double *rof_re, const double *rof_im, const double *taull_re,
+                const double *taull_im, double *secdd, double *secdq, double *secqq);
+
 #ifdef __cplusplus
 }
 #endif

@@ -54,6 +54,18 @@ def lib():
                                  C.c_double, C.c_double, C.c_int, _dp, _dp, _dp, C.c_int, _dp, _dp, C.c_void_p,
                                  C.c_void_p]
         L.orc_cluster_free.argtypes = [C.c_void_p]
+        L.orc_f_integr3.argtypes = [_dp, _dp, C.c_int, C.c_double]
+        L.orc_f_integr3.restype = C.c_double
+        L.orc_coef_init.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _dp, _ip, _ip]
+        L.orc_cgc.argtypes = [C.c_int] * 5
+        L.orc_cgc.restype = C.c_double
+        L.orc_gaunt_r.argtypes = [C.c_int] * 6
+        L.orc_gaunt_r.restype = C.c_double
+        L.orc_gaunt_xrc.argtypes = [C.c_int] * 6 + [C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.orc_radial_integral.argtypes = [C.c_int, _dp, _dp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
+                                          C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, _dp,
+                                          _dp, C.c_int, C.c_int, _dp, _dp]
+        L.orc_tenseur.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         L.orc_cluster_sizes.argtypes = [C.c_void_p, _ip]
         L.orc_cluster_xyz.argtypes = [C.c_void_p, _dp]
         L.orc_cluster_indice.argtypes = [C.c_void_p, _ip]
@@ -456,3 +468,112 @@ def tau_ri(x_reg, x_irr, ianew_reg, ianew_irr, nlmsa, n_ato_ri: int = 2, n_ri: i
         out.append((o_r[off:off + m] + 1j * o_i[off:off + m]).reshape(n_ri, k, k))
         off += m
     return out
+
+
+# ---- the absorbing atom's cross-section tensors (tensor.f90) ----
+
+def f_integr3(r, fct, radius: float) -> float:
+    """orc_f_integr3: the reference's integral of fct from 0 to radius."""
+    r = np.ascontiguousarray(r, np.float64)
+    return lib().orc_f_integr3(r, np.ascontiguousarray(fct, np.float64), len(r), float(radius))
+
+
+def ninit_of(jseuil: int, lseuil: int, nbseuil: int):
+    """(ninit, ninit1): the number of core states |j, mj> of the edge(s) and
+    of the first edge (coef_init's loop, general.f90:798-806)."""
+    is_init = 1 if jseuil in (1, 3, 5, 7) else -1
+    counts = []
+    for k in range(nbseuil):
+        if k == 1:
+            is_init = -is_init
+        counts.append(int(round(2 * (lseuil + 0.5 * is_init) + 1)))
+    return sum(counts), counts[0]
+
+
+def coef_init(jseuil: int, lseuil: int, nbseuil: int):
+    """orc_coef_init: coef_g (ninit, 2), m_g (ninit, 2), is_g (ninit,)."""
+    ninit, _ = ninit_of(jseuil, lseuil, nbseuil)
+    cg, mg, ig = np.zeros(2 * ninit), np.zeros(2 * ninit, np.int32), np.zeros(ninit, np.int32)
+    n = lib().orc_coef_init(jseuil, lseuil, nbseuil, ninit, cg, mg, ig)
+    if n != ninit:
+        raise ValueError("orc_coef_init: inconsistent ninit")
+    return cg.reshape(ninit, 2), mg.reshape(ninit, 2), ig
+
+
+def cgc(l1, l2, l3, m1, m2) -> float:
+    return lib().orc_cgc(l1, l2, l3, m1, m2)
+
+
+def gaunt_r(l1, m1, l2, m2, l3, m3) -> float:
+    return lib().orc_gaunt_r(l1, m1, l2, m2, l3, m3)
+
+
+def gaunt_xrc(l, m, lo, mo, li, mi) -> complex:
+    gr, gi = C.c_double(), C.c_double()
+    lib().orc_gaunt_xrc(l, m, lo, mo, li, mi, C.byref(gr), C.byref(gi))
+    return complex(gr.value, gi.value)
+
+
+def radial_integral(r, Vr, numat, rmtg, rmtsd, lmax, relativiste, Ecinetic, Eimag, Enervide, Energ, Eseuil, psii,
+                    ip0=1, ip_max=2, V_intmax=1.0e6):
+    """orc_radial_integral: rof (nbseuil, ip_max - ip0 + 1, lmax + 1) complex
+    of the golden rule for core functions psii (nbseuil, nr) at edge
+    energies Eseuil; and the flag of complex radial solutions."""
+    r = np.ascontiguousarray(r, np.float64)
+    psii = np.ascontiguousarray(np.atleast_2d(psii), np.float64)
+    nb = psii.shape[0]
+    nip = ip_max - ip0 + 1
+    re, im = np.zeros(nb * nip * (lmax + 1)), np.zeros(nb * nip * (lmax + 1))
+    rc = lib().orc_radial_integral(len(r), r, np.ascontiguousarray(Vr, np.float64), int(numat), float(rmtg),
+                                   float(rmtsd), int(lmax), int(bool(relativiste)), float(Ecinetic), float(Eimag),
+                                   float(Enervide), float(Energ), float(V_intmax), nb,
+                                   np.ascontiguousarray(Eseuil, np.float64).reshape(nb), psii.reshape(-1), ip0, ip_max,
+                                   re, im)
+    if rc < 0:
+        raise ValueError("orc_radial_integral: bad argument")
+    return (re + 1j * im).reshape(nb, nip, lmax + 1), bool(rc)
+
+
+class _TensIn(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("lseuil", "ninit", "ninit1", "nbseuil", "core_resolved", "lmax", "nlm_probe",
+                                        "lplus1", "lmoins1", "numat", "e1e1", "e1e2", "e2e2", "ip0", "ip_max")] + \
+               [("coef_g", C.c_void_p)] + [(n, C.c_void_p) for n in ("m_g", "ldip", "lqua", "msymdd", "msymddi",
+                                                                    "msymdq", "msymdqi", "msymqq", "msymqqi")] + \
+               [("rot_int", C.c_void_p), ("rot_atom_abs", C.c_void_p)]
+
+
+def tenseur(rof, taull, lseuil, coef_g, m_g, ninit1, nbseuil, lmax, tables, multipoles=(True, True, True),
+            core_resolved=False, lplus1=False, lmoins1=False, numat=26, rot_int=None, rot_atom_abs=None, ip0=1,
+            ip_max=2):
+    """orc_tenseur: (secdd (ninitr, 3, 3), secdq (ninitr, 3, 3, 3), secqq
+    (ninitr, 3, 3, 3, 3)) complex.  rof (nbseuil, ip_max - ip0 + 1,
+    nlm_probe) complex per (L, m) (Radial_integral's rof, the same for every m
+    of L); taull (nlm_probe, nlm_probe) complex; tables: dict of ldip (3,),
+    lqua (3, 3), msymdd / msymddi (3, 3), msymdq / msymdqi (3, 3, 3), msymqq /
+    msymqqi (3, 3, 3, 3) in the reference's index order."""
+    rof = np.ascontiguousarray(rof, np.complex128)
+    taull = np.ascontiguousarray(taull, np.complex128)
+    nlm = taull.shape[0]
+    ninit = len(coef_g)
+    ninitr = ninit if core_resolved else nbseuil
+    keep = {}
+
+    def arr(name, a, dt):
+        keep[name] = np.ascontiguousarray(a, dt)
+        return keep[name].ctypes.data_as(C.c_void_p)
+    eye = np.eye(3)
+    d = _TensIn(int(lseuil), ninit, int(ninit1), int(nbseuil), int(bool(core_resolved)), int(lmax), nlm,
+                int(bool(lplus1)), int(bool(lmoins1)), int(numat), *[int(bool(m)) for m in multipoles], ip0, ip_max,
+                arr("cg", coef_g, np.float64), arr("mg", m_g, np.int32), arr("ldip", tables["ldip"], np.int32),
+                arr("lqua", tables["lqua"], np.int32), *[arr(n, tables[n], np.int32) for n in
+                                                         ("msymdd", "msymddi", "msymdq", "msymdqi", "msymqq",
+                                                          "msymqqi")],
+                arr("ri", eye if rot_int is None else rot_int, np.float64),
+                arr("ra", eye if rot_atom_abs is None else rot_atom_abs, np.float64))
+    dd, dq, qq = np.zeros(ninitr * 18), np.zeros(ninitr * 54), np.zeros(ninitr * 162)
+    rc = lib().orc_tenseur(C.byref(d), np.ascontiguousarray(rof.real), np.ascontiguousarray(rof.imag),
+                           np.ascontiguousarray(taull.real), np.ascontiguousarray(taull.imag), dd, dq, qq)
+    if rc < 0:
+        raise ValueError("orc_tenseur: unsupported case")
+    c = lambda a, shp: (a[0::2] + 1j * a[1::2]).reshape(shp)
+    return c(dd, (ninitr, 3, 3)), c(dq, (ninitr, 3, 3, 3)), c(qq, (ninitr, 3, 3, 3, 3))

@@ -754,3 +754,128 @@ def mat_tau(s, adimp, V, E, Eimag, E_comp, fdm_comp, basereel, phiato, phiato2=N
     lib().mat_(*args)
     return (np.transpose(taull[:, 0, :, 0, :], (2, 0, 1)).copy(),
             np.transpose(tau_ri[:, 0, :, 0, :, :], (3, 2, 0, 1)).copy())
+
+
+# ---- the absorbing atom's cross-section tensors (tensor.f90) ----
+
+class _Cplx(C.Structure):
+    _fields_ = [("re", C.c_double), ("im", C.c_double)]
+
+
+def f_integr3(r, fct, radius: float) -> float:
+    """The reference's f_integr3 (tools.f90:308) with ir0 = 1."""
+    A = _Args()
+    L = lib()
+    L.f_integr3_.restype = C.c_double
+    r = np.asarray(r, np.float64)
+    return L.f_integr3_(A.arr(r, np.float64), A.arr(fct, np.float64), A.i(1), A.i(len(r)), A.d(radius))
+
+
+def gaunt_xrc(l, m, lo, mo, li, mi, ylm_comp=False) -> complex:
+    """The reference's Gaunt_xrc (tensor.f90:1870)."""
+    A = _Args()
+    L = lib()
+    L.gaunt_xrc_.restype = _Cplx
+    v = L.gaunt_xrc_(A.i(l), A.i(m), A.i(lo), A.i(mo), A.i(li), A.i(mi), A.b(ylm_comp))
+    return complex(v.re, v.im)
+
+
+def coef_init(ninit: int, jseuil: int, lseuil: int, nbseuil: int):
+    """The reference's coef_init (general.f90:774): coef_g (ninit, 2), m_g
+    (ninit, 2), is_g (ninit,)."""
+    A = _Args()
+    cg = np.zeros((ninit, 2), order="F")
+    mg = np.zeros((ninit, 2), np.int32, order="F")
+    ig = np.zeros(ninit, np.int32)
+    lib().coef_init_(cg.ctypes.data_as(C.c_void_p), ig.ctypes.data_as(C.c_void_p), A.i(ninit), A.i(jseuil),
+                     A.i(lseuil), mg.ctypes.data_as(C.c_void_p), A.i(nbseuil))
+    return np.ascontiguousarray(cg), np.ascontiguousarray(mg), ig
+
+
+def tensor_tables(ops=(1,), magnet=False):
+    """The reference's tensdd / tensdq / tensqq (general.f90:5899, 6020,
+    6127) for the point group of symmetry operations ops (1-based
+    numbers): msymdd / msymddi (3, 3), msymdq / msymdqi (3, 3, 3), msymqq /
+    msymqqi (3, 3, 3, 3) in C order of the Fortran indices, and ldip / lqua
+    set for every element."""
+    A = _Args()
+    io = np.zeros(NOPSM, np.int32)
+    io[np.asarray(ops) - 1] = 1
+    out = {}
+    for name, shp in (("dd", (3, 3)), ("dq", (3, 3, 3)), ("qq", (3, 3, 3, 3))):
+        a = np.zeros(shp, np.int32, order="F")
+        b = np.zeros(shp, np.int32, order="F")
+        getattr(lib(), f"tens{name}_")(A.arr(io, np.int32), A.b(magnet), a.ctypes.data_as(C.c_void_p),
+                                       b.ctypes.data_as(C.c_void_p))
+        out[f"msym{name}"] = np.ascontiguousarray(a)
+        out[f"msym{name}i"] = np.ascontiguousarray(b)
+    out["ldip"] = np.ones(3, np.int32)
+    out["lqua"] = np.ones((3, 3), np.int32)
+    return out
+
+
+def tenseur_car(r, Vr, numat, rmtg, rmtsd, lmax, Ecinetic, Eimag, Enervide, Energ, Eseuil, psii, taull, jseuil,
+                lseuil, tables, multipoles=(True, True, True), core_resolved=False, relativiste=False, rot_int=None,
+                rot_atom_abs=None, V_intmax=1.0e6, ip0=1, ip_max=2):
+    """The reference's Tenseur_car (tensor.f90:15) as fdm.f90:3975 calls it
+    for the FDM absorption (no TDDFT, optic, NRIXS, Green or FDM_comp
+    output, no singular solution, no spin-orbit, one spin, real harmonics):
+    Radial_integral of the absorber's radial grid r, potential Vr, core
+    functions psii (nbseuil, nr) at edge energies Eseuil, then tens_ab with
+    taull (nlm_probe, nlm_probe).  multipoles: (E1E1, E1E2, E2E2).  Returns
+    (secdd (ninitr, 3, 3), secdq (ninitr, 3, 3, 3), secqq (ninitr, 3, 3, 3, 3))
+    in C order of the Fortran indices."""
+    from .oracle import ninit_of
+    A = _Args()
+    p = lambda a: a.ctypes.data_as(C.c_void_p)
+    nr = len(r)
+    psii = np.atleast_2d(np.asarray(psii, np.float64))
+    nbseuil = psii.shape[0]
+    ninit, ninit1 = ninit_of(jseuil, lseuil, nbseuil)
+    cg, mg, ig = coef_init(ninit, jseuil, lseuil, nbseuil)
+    ninitr = ninit if core_resolved else nbseuil
+    nlm = (lmax + 1) ** 2
+    mult = np.zeros(10, np.int32)
+    mult[0], mult[1], mult[5] = [1 if m else 0 for m in multipoles]
+    secdd = np.zeros((3, 3, 1, ninitr, 1), np.complex128, order="F")
+    secdq = np.zeros((3, 3, 3, ninitr, 1), np.complex128, order="F")
+    secqq = np.zeros((3, 3, 3, 3, ninitr, 1), np.complex128, order="F")
+    secdo = np.zeros((3, 3, 3, 3, ninitr, 1), np.complex128, order="F")
+    sec2 = [np.zeros((3, 3, ninitr, 1), np.complex128, order="F") for _ in range(4)]
+    secoo = [np.zeros((3, 1, 3, 1, ninitr, 1), np.complex128, order="F") for _ in range(2)]
+    secm = [np.zeros_like(secdd), np.zeros_like(secdo), np.zeros_like(secdq), np.zeros_like(secqq)]
+    ta = np.zeros((nlm, nlm, 1, 1, 2, 2, 1), np.complex128, order="F")
+    ta[:, :, 0, 0, 0, 0, 0] = taull
+    ta[:, :, 0, 0, 1, 1, 0] = taull
+    eye = np.eye(3)
+    Fm = lambda a: A.arr(np.asarray(a), np.int32)  # C-order (i, j, ...) tables -> Fortran arrays of the same indices
+    args = [
+        A.arr(cg, np.float64), A.b(core_resolved), A.arr(np.array([[Ecinetic]]), np.float64), A.d(Eimag),
+        A.d(Energ), A.arr([Enervide], np.float64), A.arr(np.asarray(Eseuil, np.float64).reshape(nbseuil), np.float64),
+        A.b(False), A.b(False), A.b(False), A.b(False), A.b(False), A.b(False), A.b(False), A.b(False),
+        A.i(0), A.i(1), A.i(ip_max), A.i(ip0), A.arr(ig, np.int32), A.i(lmax), A.i(0), Fm(tables["ldip"]),
+        A.b(False), Fm(np.zeros((3, 3, 3), np.int32)), A.b(False), Fm(tables["lqua"]), A.i(lseuil), A.arr(mg, np.int32),
+        A.i(0), A.i(1), A.i(0), A.i(0),
+        Fm(tables["msymdd"]), Fm(tables["msymddi"]), Fm(tables["msymdq"]), Fm(tables["msymdqi"]),
+        Fm(np.zeros((3, 3, 3, 3), np.int32)), Fm(np.zeros((3, 3, 3, 3), np.int32)),
+        Fm(np.zeros((3, 1, 3, 1), np.int32)), Fm(np.zeros((3, 1, 3, 1), np.int32)),
+        Fm(tables["msymqq"]), Fm(tables["msymqqi"]), A.arr(mult, np.int32),
+        A.i(1), A.i(1), A.i(1), A.i(1), A.i(1), A.i(nbseuil), A.i(1), A.i(1), A.i(1), A.i(ninit1), A.i(ninit),
+        A.i(ninitr), A.i(nbseuil), A.i(1), A.i(nlm), A.i(1), A.i(nlm), A.i(nr), A.i(nr), A.i(1), A.i(1), A.i(1),
+        A.i(numat), A.arr(psii.T, np.float64), A.arr(np.asarray(r, np.float64), np.float64), A.b(relativiste),
+        A.b(True), A.d(rmtg), A.d(rmtsd), A.arr(np.zeros((1, nlm, 1, 1, nbseuil), np.complex128), np.complex128),
+        A.arr(eye if rot_atom_abs is None else rot_atom_abs, np.float64),
+        A.arr(eye if rot_int is None else rot_int, np.float64),
+        p(secdd), p(secm[0]), p(secdo), p(secm[1]), p(secdq), p(secm[2]), p(sec2[0]), p(sec2[1]), p(sec2[2]),
+        p(sec2[3]), p(secoo[0]), p(secoo[1]), p(secqq), p(secm[3]),
+        A.b(False), A.b(False), A.b(False), A.arr(np.zeros((nlm, 1, nlm, 1, 1), np.complex128), np.complex128), p(ta),
+        A.b(False), A.arr(np.zeros((1, 1, 1, 1), np.complex128), np.complex128), A.d(V_intmax),
+        A.arr(np.zeros((1, 1)), np.float64), A.arr(np.asarray(Vr, np.float64).reshape(nr, 1, 1, 1), np.float64),
+        A.b(False),
+    ]
+    assert len(args) == 101, len(args)
+    lib().tenseur_car_(*args)
+    dd = np.stack([secdd[:, :, 0, k, 0] for k in range(ninitr)])
+    dq = np.stack([secdq[:, :, :, k, 0] for k in range(ninitr)])
+    qq = np.stack([secqq[:, :, :, :, k, 0] for k in range(ninitr)])
+    return dd, dq, qq

@@ -275,6 +275,48 @@ def sphere_fixture():
     print("sphere", [len(out[f"r{ia}"]) for ia in range(s["natome"])])
 
 
+# (lmax, Jseuil, Lseuil, nbseuil, point-group operations, Core_resolved, Ecinetic, Eimag, relativistic,
+#  rotated frames, atomic number)
+TENSOR_CASES = [(2, 1, 0, 1, [1], False, 0.7, 0.05, False, False, 26),
+                (2, 1, 0, 1, [1], False, 0.9, 0.0, False, True, 26),
+                (3, 3, 1, 1, [1], True, 0.7, 0.05, False, True, 26),
+                (3, 2, 1, 2, [1, 22, 23, 24, 25, 40, 41, 42], False, 0.9, 0.02, False, False, 26),
+                (2, 1, 0, 1, [1], False, 0.6, 0.03, True, True, 26),
+                (2, 1, 0, 1, [1], False, 0.7, 0.05, False, False, 1)]
+
+
+def tensor_fixture():
+    """The reference's Tenseur_car (tensor.f90:15, with its Radial_integral)
+    for the absorbing atom of the three-atom cluster (atom 0's radial grid
+    and potential): K, L3 (core resolved) and L23 edges, E1E1, E1E2 and E2E2,
+    no symmetry and mmm, complex and real energy, rotated frames, a random
+    Taull and synthetic core functions."""
+    a = ATOMS
+    posi = np.array(a["posi"]) / BOHR
+    s = R.atom_system(a["R"] / BOHR, a["h"] / BOHR, a["lmaxso"], posi, np.array(a["rmt"]) / BOHR, a["lmax"])
+    r, V, _ = sphere_inputs(s, 0)
+    rmt = a["rmt"][0] / BOHR
+    out = dict(r=r, V=V, rmt=rmt, rmtsd=1.1 * rmt, ncase=len(TENSOR_CASES))
+    rng = np.random.default_rng(11)
+    for k, (lmax, js, ls, nb, ops, cr, Ec, Ei, rel, rot, Z) in enumerate(TENSOR_CASES):
+        nlm = (lmax + 1) ** 2
+        tl = rng.normal(size=(nlm, nlm)) + 1j * rng.normal(size=(nlm, nlm))
+        psii = np.stack([r * np.exp(-26 * r / (1 + j)) * (1 + 0.3 * j * r) for j in range(nb)])
+        Es = np.array([520.0, 510.0][:nb])
+        ri = np.linalg.qr(rng.normal(size=(3, 3)))[0] if rot else np.eye(3)
+        ra = np.linalg.qr(rng.normal(size=(3, 3)))[0] if rot else np.eye(3)
+        tabs = R.tensor_tables(ops)
+        dd, dq, qq = R.tenseur_car(r, V, Z, rmt, 1.1 * rmt, lmax, Ec, Ei, Ec + 0.3, Ec, Es, psii, tl, js, ls, tabs,
+                                   core_resolved=cr, relativiste=rel, rot_int=ri, rot_atom_abs=ra)
+        c = f"c{k}_"
+        out.update({c + "case": np.array([lmax, js, ls, nb, int(cr), int(rel), Z]), c + "E": np.array([Ec, Ei]),
+                    c + "Eseuil": Es, c + "psii": psii, c + "taull": tl, c + "rot_int": ri, c + "rot_atom_abs": ra,
+                    c + "dd": dd, c + "dq": dq, c + "qq": qq})
+        out.update({c + n: v for n, v in tabs.items()})
+    np.savez_compressed(os.path.join(HERE, "tensor_R1.9.npz"), **out)
+    print("tensor cases", len(TENSOR_CASES))
+
+
 def main():
     if not R.available():
         raise SystemExit("oracle/_ref/libfdmnes_ref.so is missing: make -f oracle/Makefile.ref")
@@ -309,6 +351,7 @@ def main():
     rel_fixtures()
     ylm_fixture()
     sphere_fixture()
+    tensor_fixture()
 
 
 if __name__ == "__main__":

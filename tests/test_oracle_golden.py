@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 # the mat_solve cases (the grid builders' fixtures, grid_*, are tests/test_oracle_grid.py's,
 # the atom cluster's, atoms_*, tests/test_oracle_atoms.py's)
 GOLD = sorted(p for p in glob.glob(os.path.join(HERE, "golden", "*_R*.npz"))
-              if not os.path.basename(p).startswith(("grid_", "atoms_", "rel_", "ylmc_", "sphere_")))
+              if not os.path.basename(p).startswith(("grid_", "atoms_", "rel_", "ylmc_", "sphere_", "tensor_")))
 BOHR = 0.52917721067
 
 

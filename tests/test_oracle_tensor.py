@@ -1,0 +1,159 @@
+"""The absorbing atom's cross-section tensors (tensor.f90): the oracle's
+restatement of Tenseur_car with tens_ab (E1E1, E1E2, E2E2) and of the radial
+integrals of Radial_integral / radial_matrix (sphere.f90:1490, :2209) against
+the reference's own Tenseur_car (tests/golden/tensor_R1.9.npz, written by
+make_golden.py from oracle/_ref: K, L3 core-resolved and L23 edges, no
+symmetry and mmm, complex and real energy, rotated frames, the scalar-
+relativistic radial equation, hydrogen) within 1e-10 of each tensor's
+largest element; the building blocks (f_integr3, coef_init, Gaunt_xrc)
+against the reference's own routines (marked ref) and against what the
+mathematics fixes: f_integr3 exact on quadratics, Clebsch-Gordan
+orthonormality, Gaunt coefficients as quadratures of three real harmonics,
+unit core-state weights, the K-edge dipole tensor in closed form and the
+rotation as R T R^T."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+from .test_oracle_atoms import G
+
+TENSOR = G.replace("atoms_R1.9_h0.25.npz", "tensor_R1.9.npz")
+
+
+def _case(f, k):
+    c = f"c{k}_"
+    lmax, js, ls, nb, cr, rel, Z = (int(v) for v in f[c + "case"])
+    tabs = {n: f[c + n] for n in ("ldip", "lqua", "msymdd", "msymddi", "msymdq", "msymdqi", "msymqq", "msymqqi")}
+    return c, lmax, js, ls, nb, bool(cr), bool(rel), Z, tabs
+
+
+def oracle_tensors(f, k, lmax_rof=None):
+    c, lmax, js, ls, nb, cr, rel, Z, tabs = _case(f, k)
+    Ec, Ei = f[c + "E"]
+    rof, _ = O.radial_integral(f["r"], f["V"], Z, float(f["rmt"]), float(f["rmtsd"]), lmax, rel, Ec, Ei, Ec + 0.3,
+                               Ec, f[c + "Eseuil"], f[c + "psii"])
+    rofl = np.repeat(rof, [2 * L + 1 for L in range(lmax + 1)], axis=2)
+    n, n1 = O.ninit_of(js, ls, nb)
+    cg, mg, _ = O.coef_init(js, ls, nb)
+    return O.tenseur(rofl, f[c + "taull"], ls, cg, mg, n1, nb, lmax, tabs, core_resolved=cr, numat=Z,
+                     rot_int=f[c + "rot_int"], rot_atom_abs=f[c + "rot_atom_abs"]), rof
+
+
+@pytest.mark.parametrize("k", range(6))
+def test_oracle_tensors_match_reference(k):
+    f = np.load(TENSOR)
+    c = f"c{k}_"
+    (dd, dq, qq), _ = oracle_tensors(f, k)
+    for name, a in (("dd", dd), ("dq", dq), ("qq", qq)):
+        b = f[c + name]
+        scale = np.abs(b).max()
+        assert np.abs(a - b).max() <= 1e-10 * max(scale, 1e-300), (name, scale)
+        if scale == 0:
+            assert np.all(a == 0)
+    assert np.abs(f[c + "dd"]).max() > 0 and np.abs(f[c + "qq"]).max() > 0
+
+
+def test_f_integr3_is_exact_on_quadratics():
+    r = np.exp(np.linspace(np.log(1e-3), np.log(2.5), 180))
+    for R0 in (0.731, 1.5, float(r[100]), 3.0):
+        want = min(R0, r[-1])
+        v = O.f_integr3(r, 1 + 2 * r - 3 * r * r, R0)
+        assert abs(v - (want + want ** 2 - want ** 3)) <= 1e-12, R0
+
+
+def test_clebsch_gordan_orthonormal():
+    assert abs(O.cgc(1, 1, 0, 1, -1) - 1 / np.sqrt(3)) <= 1e-15
+    for l1 in range(4):
+        for l2 in range(3):
+            for L in range(abs(l1 - l2), l1 + l2 + 1):
+                for Lp in range(abs(l1 - l2), l1 + l2 + 1):
+                    for M in range(-min(L, Lp), min(L, Lp) + 1):
+                        s = sum(O.cgc(l1, l2, L, m1, M - m1) * O.cgc(l1, l2, Lp, m1, M - m1)
+                                for m1 in range(-l1, l1 + 1) if abs(M - m1) <= l2)
+                        assert abs(s - (L == Lp)) <= 1e-13
+
+
+def _sphere_quadrature(n=24):
+    x, w = np.polynomial.legendre.leggauss(n)
+    ph = 2 * np.pi * np.arange(2 * n) / (2 * n)
+    ct, P = np.meshgrid(x, ph, indexing="ij")
+    st = np.sqrt(1 - ct * ct)
+    v = np.stack([st * np.cos(P), st * np.sin(P), ct], axis=-1).reshape(-1, 3)
+    wt = (w[:, None] * np.full(2 * n, 2 * np.pi / (2 * n))[None, :]).reshape(-1)
+    return v, wt
+
+
+def test_gaunt_coefficients_are_integrals_of_real_harmonics():
+    v, wt = _sphere_quadrature()
+    Y = np.array([O.ylm_real(3, p, 1.0) for p in v])  # (npts, 16), index l*l + l + m
+    for l1 in range(4):
+        for l2 in range(3):
+            for l3 in range(3):
+                for m1 in range(-l1, l1 + 1):
+                    for m2 in range(-l2, l2 + 1):
+                        for m3 in range(-l3, l3 + 1):
+                            q = np.sum(wt * Y[:, l1 * l1 + l1 + m1] * Y[:, l2 * l2 + l2 + m2] * Y[:, l3 * l3 + l3 + m3])
+                            assert abs(O.gaunt_r(l1, m1, l2, m2, l3, m3) - q) <= 1e-13, (l1, m1, l2, m2, l3, m3)
+
+
+@pytest.mark.parametrize("js,ls,nb", [(1, 0, 1), (1, 1, 1), (2, 1, 1), (3, 1, 1), (2, 1, 2), (3, 2, 1)])
+def test_core_states_have_unit_weight(js, ls, nb):
+    cg, mg, ig = O.coef_init(js, ls, nb)
+    assert np.allclose((cg ** 2).sum(axis=1), 1.0, atol=1e-14)
+    for i in range(len(cg)):
+        for isp in range(2):
+            assert (mg[i, isp] == 1000) == (cg[i, isp] == 0)
+
+
+def test_k_edge_dipole_closed_form_and_rotation():
+    f = np.load(TENSOR)
+    c, lmax, js, ls, nb, cr, rel, Z, tabs = _case(f, 0)
+    Ec, Ei = f[c + "E"]
+    rof, _ = O.radial_integral(f["r"], f["V"], Z, float(f["rmt"]), float(f["rmtsd"]), lmax, False, Ec, Ei, Ec + 0.3,
+                               Ec, f[c + "Eseuil"], f[c + "psii"])
+    rofl = np.repeat(rof, [2 * L + 1 for L in range(lmax + 1)], axis=2)
+    nlm = (lmax + 1) ** 2
+    t = 0.4 - 1.3j
+    cg, mg, _ = O.coef_init(1, 0, 1)
+    dd, dq, qq = O.tenseur(rofl, t * np.eye(nlm), 0, cg, mg, 2, 1, lmax, tabs, numat=Z)
+    # two core spins, Gaunt 1/sqrt(4 pi), clm^2 = 4 pi / 3, Tau_rad = i |rof|^2 Im t, times i
+    want = -(2.0 / 3.0) * abs(rof[0, 0, 1]) ** 2 * t.imag
+    assert np.abs(dd[0] - want * np.eye(3)).max() <= 1e-14 * abs(want)
+    dd1, _, _ = O.tenseur(rofl, t * np.eye(nlm), 0, cg, mg, 2, 1, lmax, tabs, numat=1)
+    assert np.abs(dd1[0] - dd[0] / 2).max() <= 1e-15 * abs(want)
+    # rotated frames: R T R^T with R = rot_int rot_atom_abs^T
+    tl = f[c + "taull"]
+    base = O.tenseur(rofl, tl, 0, cg, mg, 2, 1, lmax, tabs, numat=Z)
+    rng = np.random.default_rng(5)
+    ri, ra = np.linalg.qr(rng.normal(size=(3, 3)))[0], np.linalg.qr(rng.normal(size=(3, 3)))[0]
+    rot = O.tenseur(rofl, tl, 0, cg, mg, 2, 1, lmax, tabs, numat=Z, rot_int=ri, rot_atom_abs=ra)
+    Rm = ri @ ra.T
+    sc = np.abs(base[0]).max()
+    assert np.abs(rot[0][0] - Rm @ base[0][0] @ Rm.T).max() <= 1e-13 * sc
+    assert np.abs(rot[1][0] - np.einsum("ia,jb,kc,abc->ijk", Rm, Rm, Rm, base[1][0])).max() <= 1e-13 * sc
+    assert np.abs(rot[2][0] - np.einsum("ia,jb,kc,ld,abcd->ijkl", Rm, Rm, Rm, Rm, base[2][0])).max() <= 1e-13 * sc
+
+
+@pytest.mark.ref
+def test_building_blocks_match_reference():
+    from oracle import ref as R
+    if not R.available():
+        pytest.skip("needs oracle/_ref (built where the reference sources are)")
+    worst = 0.0
+    for l in range(4):
+        for m in range(-l, l + 1):
+            for lo in range(3):
+                for mo in range(-lo, lo + 1):
+                    for li in range(3):
+                        for mi in range(-li, li + 1):
+                            worst = max(worst, abs(O.gaunt_xrc(l, m, lo, mo, li, mi) - R.gaunt_xrc(l, m, lo, mo, li, mi)))
+    assert worst <= 1e-15
+    for js, ls, nb in [(1, 0, 1), (1, 1, 1), (2, 1, 1), (3, 1, 1), (2, 1, 2), (3, 2, 1), (4, 2, 2)]:
+        n, _ = O.ninit_of(js, ls, nb)
+        a, b = O.coef_init(js, ls, nb), R.coef_init(n, js, ls, nb)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+    r = np.exp(np.linspace(np.log(1e-4), np.log(3.0), 300))
+    for fct, R0 in ((np.sin(3 * r) * r * r, 1.7), (np.exp(-r) * r ** 5, 0.9), (np.where(r < 2e-4, 0.0, r), 2.2),
+                    (np.cos(r), 5.0), (r * r, float(r[57]))):
+        assert O.f_integr3(r, fct, R0) == pytest.approx(R.f_integr3(r, fct, R0), rel=1e-14, abs=1e-300)
