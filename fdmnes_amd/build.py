@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfdmgpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["solver.hip", "refine.hip", "batch.hip", "assemble.hip", "tau.hip", "sphere.hip", "api.hip"]
+SOURCES = ["solver.hip", "refine.hip", "batch.hip", "assemble.hip", "tau.hip", "sphere.hip", "tensor.hip", "api.hip"]
 HOST_LIB = os.path.join(HERE, "libfdmgpu_host.so")
 HOST_SOURCES = ["cluster.cpp"]  # host-only: no HIP runtime dependency
 

@@ -10,6 +10,7 @@
 #include "assemble.h"
 #include "batch.h"
 #include "sphere.h"
+#include "tensor.h"
 #include "tau.h"
 
 using namespace fdmgpu;
@@ -638,6 +639,46 @@ int fdmgpu_assembler_run(fdmgpu_assembler *a, int count, const double *Enervide,
 struct fdmgpu_sphere {
   SphereSolver s;
 };
+struct fdmgpu_tensor {
+  TensorPlan p;
+};
+
+int fdmgpu_tensor_create(const fdmgpu_tensor_tables *t, int max_points, void *hip_stream, fdmgpu_tensor **out) {
+  if (!t || !out) {
+    set_error("fdmgpu_tensor_create: invalid argument");
+    return FDMGPU_EINVAL;
+  }
+  int rc = ensure_init();
+  if (rc) return rc;
+  std::unique_ptr<fdmgpu_tensor> h(new (std::nothrow) fdmgpu_tensor());
+  if (!h) return FDMGPU_ENOMEM;
+  rc = h->p.init(t, max_points, hip_stream ? (hipStream_t)hip_stream : g_lib.stream);
+  if (rc) return rc;
+  *out = h.release();
+  return 0;
+}
+
+void fdmgpu_tensor_destroy(fdmgpu_tensor *h) { delete h; }
+
+int fdmgpu_tensor_ninitr(const fdmgpu_tensor *h) { return h ? h->p.ninitr : -1; }
+
+int fdmgpu_tensor_run(fdmgpu_tensor *h, int count, const double *const *d_rof, const double *const *d_tau,
+                      double *const *d_secdd, double *const *d_secdq, double *const *d_secqq) {
+  if (!h || !d_rof || !d_tau || count < 1 || count > TENS_MAXP) {
+    set_error("fdmgpu_tensor_run: invalid argument");
+    return FDMGPU_EINVAL;
+  }
+  TensPoints P{};
+  for (int k = 0; k < count; k++) {
+    P.rof[k] = d_rof[k];
+    P.taull[k] = d_tau[k];
+    P.dd[k] = d_secdd ? d_secdd[k] : nullptr;
+    P.dq[k] = d_secdq ? d_secdq[k] : nullptr;
+    P.qq[k] = d_secqq ? d_secqq[k] : nullptr;
+  }
+  FDM_HIP(hipSetDevice(g_lib.device));
+  return h->p.run(count, P);
+}
 
 int fdmgpu_sphere_create(const fdmgpu_sphere_tables *t, int max_points, void *hip_stream, fdmgpu_sphere **out) {
   if (!t || !out) {
@@ -656,15 +697,31 @@ int fdmgpu_sphere_create(const fdmgpu_sphere_tables *t, int max_points, void *hi
 
 void fdmgpu_sphere_destroy(fdmgpu_sphere *s) { delete s; }
 
+int fdmgpu_sphere_set_integrals(fdmgpu_sphere *s, int atom, int nbseuil, const double *psii, const double *Eseuil,
+                                double Rmtsd, int ip0, int ip_max) {
+  if (!s) {
+    set_error("fdmgpu_sphere_set_integrals: invalid argument");
+    return FDMGPU_EINVAL;
+  }
+  FDM_HIP(hipSetDevice(g_lib.device));
+  return s->s.set_integrals(atom, nbseuil, psii, Eseuil, Rmtsd, ip0, ip_max);
+}
+
 int fdmgpu_sphere_run(fdmgpu_sphere *s, int count, const double *Ecinetic, const double *Eimag,
                       const double *Enervide, int complex_out, double *const *d_phi, double *const *d_tau) {
-  if (!s || !Ecinetic || !Eimag || !Enervide || !d_phi || count < 1 || count > SPH_MAXP) {
+  return fdmgpu_sphere_run_integrals(s, count, Ecinetic, Eimag, Enervide, nullptr, complex_out, d_phi, d_tau, nullptr);
+}
+
+int fdmgpu_sphere_run_integrals(fdmgpu_sphere *s, int count, const double *Ecinetic, const double *Eimag,
+                                const double *Enervide, const double *Energ, int complex_out, double *const *d_phi,
+                                double *const *d_tau, double *const *d_rof) {
+  if (!s || !Ecinetic || !Eimag || !Enervide || !d_phi || count < 1 || count > SPH_MAXP || (d_rof && !Energ)) {
     set_error("fdmgpu_sphere_run: invalid argument");
     return FDMGPU_EINVAL;
   }
   SphPoints P{};
   for (int k = 0; k < count; k++) {
-    if (!d_phi[k]) {
+    if (!d_phi[k] && s->s.nphi > 0) {  // atoms without border points have no phiato
       set_error("fdmgpu_sphere_run: null phiato buffer for point " + std::to_string(k));
       return FDMGPU_EINVAL;
     }
@@ -675,8 +732,10 @@ int fdmgpu_sphere_run(fdmgpu_sphere *s, int count, const double *Ecinetic, const
     P.ecin[k] = Ecinetic[k];
     P.eimag[k] = Eimag[k];
     P.ener[k] = Enervide[k];
+    P.energ[k] = Energ ? Energ[k] : 0.0;
     P.phi[k] = d_phi[k];
     P.tau[k] = d_tau ? d_tau[k] : nullptr;
+    P.rof[k] = d_rof ? d_rof[k] : nullptr;
   }
   FDM_HIP(hipSetDevice(g_lib.device));
   return s->s.run(count, P, complex_out != 0);

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "../../include/fdmgpu.h"
 
 namespace fdmgpu {
@@ -24,8 +26,17 @@ struct SphTabs {
 };
 struct SphPoints {
   double ecin[SPH_MAXP], eimag[SPH_MAXP], ener[SPH_MAXP];
+  double energ[SPH_MAXP];  // photoelectron energy Energ (the quadrupole factor of the radial integrals)
   double *phi[SPH_MAXP];  // device phiato per point, (nbord, nlmsa) per atom, concatenated
   double *tau[SPH_MAXP];  // device Tau per point, (lmax + 1) complex per atom, or null
+  double *rof[SPH_MAXP];  // device radial integrals of the absorbing atom per point, or null
+};
+// the absorbing atom's radial integrals of the golden rule (kernel argument)
+struct RofTabs {
+  int atom = -1, lmax = 0, nbs = 0, ip0 = 1, ip_max = 1, nlast = 0;
+  double eseuil[2] = {0, 0};
+  const double *psii = nullptr;  // [nbs][nr of the atom] core functions on the atom's grid
+  double rmtsd = 0;
 };
 
 struct SphereSolver {
@@ -50,9 +61,16 @@ struct SphereSolver {
     }
   } d;
   double *u = nullptr;  // renormalised radial solutions: [point][atom][L] x (re[nrmax], im[nrmax])
+  std::vector<int> h_nr, h_lmax;  // host copies of the atoms' sizes
+  std::vector<std::vector<double>> h_r;
+  RofTabs rt;                      // set by set_integrals
+  double *d_psii = nullptr;
 
   ~SphereSolver();
   int init(const fdmgpu_sphere_tables *t, int max_points, hipStream_t s);
+  int set_integrals(int atom, int nbseuil, const double *psii, const double *Eseuil, double rmtsd, int ip0,
+                    int ip_max);
+  int64_t nrof() const { return rt.atom < 0 ? 0 : (int64_t)rt.nbs * (rt.ip_max - rt.ip0 + 1) * (rt.lmax + 1); }
   int run(int count, const SphPoints &P, bool cplx);
 };
 

@@ -219,6 +219,105 @@ __global__ void k_sph_phiato(SphTabs T, SphPoints P, int lmax_all, int nlm_max, 
   }
 }
 
+// The absorbing atom's radial integrals of the golden rule (Radial_integral
+// sphere.f90:1490 -> radial_matrix :2209): one wave per (point, edge, ip, L),
+// rof = fac(ip) * integral_0^rmtsd psii r u_L r^ip dr by f_integr3's
+// piecewise cubics (tools.f90:308-419), every interval's share computed
+// independently from the four points its cubic goes through, the shares
+// summed across the wave.
+__device__ inline double integr3_share(const double *r, int i, int nlast, int nrm, double radius, double f1,
+                                       double f2, double f3, double f4, double f5) {
+  // i: 1-based interval; f1..f5 the integrand at points i-2 .. i+2 (those outside the grid unused)
+  double r1, r2, r3, r4, g1, g2, g3, g4, rm, rp;
+  if (i == 1) {  // from 0, with the value at 0 from the parabola through points 1..3
+    if (fabs(f5) < 1e-20 && fabs(f4) < 1e-20 && fabs(f3) < 1e-20) return 0.0;
+    const double rmm = r[0], r0 = r[1], rpp = r[2];
+    const double dp0 = rpp - r0, dpm = rpp - rmm, d0m = r0 - rmm;
+    const double a = (f3 * dp0 - f4 * dpm + f5 * d0m) / (d0m * dp0 * dpm);
+    const double b = (f4 - f3) / d0m - a * (r0 + rmm);
+    g1 = f4 - a * (r0 * r0) - b * r0;
+    r1 = 0;
+    r2 = r[0]; g2 = f3;
+    r3 = r[1]; g3 = f4;
+    r4 = r[2]; g4 = f5;
+    rm = 0;
+    rp = r3;
+  } else if (i == nlast) {  // cut at radius (or the grid's end), with the cubic of interval i - 1
+    rm = r[i - 1];
+    rp = r[nrm - 1];
+    if (i < 3) {
+      r1 = r[i - 2]; g1 = f2;
+      r2 = r[i - 1]; g2 = f3;
+      r3 = r[i]; g3 = f4;
+      r4 = r[i + 1]; g4 = f5;
+    } else {
+      r1 = r[i - 3]; g1 = f1;
+      r2 = r[i - 2]; g2 = f2;
+      r3 = r[i - 1]; g3 = f3;
+      r4 = r[i]; g4 = f4;
+    }
+  } else {
+    r1 = r[i - 2]; g1 = f2;
+    r2 = r[i - 1]; g2 = f3;
+    r3 = r[i]; g3 = f4;
+    r4 = r[i + 1]; g4 = f5;
+    rm = r2;
+    rp = r3;
+  }
+  if (rp > radius) rp = radius;
+  if (fabs(g2) < 1e-20 && fabs(g3) < 1e-20) return 0.0;
+  const double rap14 = (g1 - g4) / (r1 - r4), rap24 = (g2 - g4) / (r2 - r4), rap34 = (g3 - g4) / (r3 - r4);
+  const double fac1 = (rap14 - rap34) / (r1 - r3), fac2 = (rap24 - rap34) / (r2 - r3);
+  const double a = (fac1 - fac2) / (r1 - r2);
+  const double b = fac1 - a * (r1 + r3 + r4);
+  const double c = rap14 - a * (r1 * r1 + r1 * r4 + r4 * r4) - b * (r1 + r4);
+  const double d = g1 - (a * (r1 * r1) + b * r1 + c) * r1;
+  return (0.25 * (rm + rp) * (rm * rm + rp * rp) * a + (1.0 / 3) * (rm * rm + rm * rp + rp * rp) * b +
+          0.5 * (rm + rp) * c + d) *
+         (rp - rm);
+}
+
+__global__ void __launch_bounds__(64) k_sph_rof(SphTabs T, SphPoints P, RofTabs R, int lmax_all,
+                                                const double *__restrict__ u, int64_t ustride) {
+  const int g = blockIdx.y, nip = R.ip_max - R.ip0 + 1;
+  int t = blockIdx.x;
+  const int L = t % (R.lmax + 1);
+  t /= R.lmax + 1;
+  const int ipo = t % nip, is = t / nip, ip = R.ip0 + ipo;
+  const int a = R.atom, n = T.nr[a];
+  const double *r = T.r + T.roff[a];
+  const double *ur = u + (((int64_t)g * T.natome + a) * (lmax_all + 1) + L) * ustride;
+  const double *ui = ur + T.nrmax;
+  const double *ps = R.psii + (int64_t)is * n;
+  const bool ecomp = fabs(P.eimag[g]) > 1e-10 || P.ecin[g] < 1e-10;
+  // the integrand at 1-based point j (0 outside the grid; such values are never used)
+  auto F = [&](const double *uu, int j) {
+    if (j < 1 || j > n) return 0.0;
+    const double x = r[j - 1];
+    const double rb = ip == 0 ? 1.0 : ip == 1 ? x : ip == 2 ? x * x : x * x * x;
+    return ps[j - 1] * x * uu[j - 1] * rb;
+  };
+  double sr = 0, si = 0;
+  for (int i = 1 + (int)threadIdx.x; i <= R.nlast; i += 64) {
+    sr += integr3_share(r, i, R.nlast, n, R.rmtsd, F(ur, i - 2), F(ur, i - 1), F(ur, i), F(ur, i + 1),
+                        F(ur, i + 2));
+    if (ecomp)
+      si += integr3_share(r, i, R.nlast, n, R.rmtsd, F(ui, i - 2), F(ui, i - 1), F(ui, i), F(ui, i + 1),
+                          F(ui, i + 2));
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    sr += __shfl_down(sr, off);
+    si += __shfl_down(si, off);
+  }
+  if (threadIdx.x == 0) {
+    const double vecond = 0.5 * kAlfa * fmax(0.001, P.energ[g] + R.eseuil[is]);
+    const double fac = ip == 0 ? 0.5 * kAlfa : ip == 1 ? 1.0 : ip == 2 ? 0.5 * vecond : -(1.0 / 6) * (vecond * vecond);
+    double *o = P.rof[g] + 2 * (((int64_t)is * nip + ipo) * (R.lmax + 1) + L);
+    o[0] = fac * sr;
+    o[1] = fac * si;
+  }
+}
+
 template <class T>
 int upload(T **d, const T *h, int64_t n) {
   FDM_HIP(hipMalloc(d, sizeof(T) * (n > 0 ? n : 1)));
@@ -232,7 +331,7 @@ SphereSolver::~SphereSolver() {
   for (void *p : {(void *)d.r, (void *)d.V, (void *)d.clm, (void *)d.clp, (void *)d.cl0, (void *)d.cgm, (void *)d.cgp,
                   (void *)d.cg0, (void *)d.f2, (void *)d.roff, (void *)d.nr, (void *)d.Z, (void *)d.lmax,
                   (void *)d.rmtg, (void *)d.inr, (void *)d.toff, (void *)d.batom, (void *)d.bint, (void *)d.rrel,
-                  (void *)d.yoff, (void *)d.ylmato, (void *)u})
+                  (void *)d.yoff, (void *)d.ylmato, (void *)u, (void *)d_psii})
     if (p) (void)hipFree(p);
 }
 
@@ -261,6 +360,9 @@ int SphereSolver::init(const fdmgpu_sphere_tables *t, int maxp, hipStream_t s) {
         set_error("fdmgpu_sphere_create: the radial grid of atom " + std::to_string(a) + " must increase from r > 0");
         return FDMGPU_EINVAL;
       }
+    h_nr.push_back(n);
+    h_lmax.push_back(t->lmax[a]);
+    h_r.emplace_back(r, r + n);
     roff[a] = nrt;
     toff[a] = nt;
     nt += t->lmax[a] + 1;
@@ -336,10 +438,62 @@ int SphereSolver::init(const fdmgpu_sphere_tables *t, int maxp, hipStream_t s) {
   return 0;
 }
 
+int SphereSolver::set_integrals(int atom, int nbs, const double *psii, const double *Eseuil, double rmtsd, int ip0,
+                                int ip_max) {
+  if (atom < 0 || atom >= natome || nbs < 1 || nbs > 2 || !psii || !Eseuil || ip0 < 0 || ip_max > 3 ||
+      ip_max < ip0) {
+    set_error("fdmgpu_sphere_set_integrals: atom in 0..natome-1, 1 or 2 edges, 0 <= ip0 <= ip_max <= 3");
+    return FDMGPU_EINVAL;
+  }
+  const std::vector<double> &r = h_r[atom];
+  const int n = h_nr[atom];
+  if (!(rmtsd > r[2]) || !(rmtsd < 1e30)) {
+    set_error("fdmgpu_sphere_set_integrals: rmtsd must lie beyond the atom's third radial point");
+    return FDMGPU_EINVAL;
+  }
+  // f_integr3's last interval: the first i >= 2 with r(i+1) > rmtsd, or nr - 1
+  int nlast = n - 1;
+  for (int i = 2; i <= n - 1; i++)
+    if (i == n - 1 || r[i] > rmtsd) {
+      nlast = i;
+      break;
+    }
+  if (d_psii) {
+    FDM_HIP(hipFree(d_psii));
+    d_psii = nullptr;
+  }
+  if (upload(&d_psii, psii, (int64_t)nbs * n)) return FDMGPU_EHIP;
+  rt = RofTabs{};
+  rt.atom = atom;
+  rt.lmax = h_lmax[atom];
+  rt.nbs = nbs;
+  rt.ip0 = ip0;
+  rt.ip_max = ip_max;
+  rt.nlast = nlast;
+  rt.eseuil[0] = Eseuil[0];
+  rt.eseuil[1] = nbs > 1 ? Eseuil[1] : 0.0;
+  rt.psii = d_psii;
+  rt.rmtsd = rmtsd;
+  return 0;
+}
+
 int SphereSolver::run(int count, const SphPoints &P, bool cplx) {
   if (count < 1 || count > max_points) {
     set_error("fdmgpu_sphere_run: count must be in 1.." + std::to_string(max_points));
     return FDMGPU_EINVAL;
+  }
+  bool want_rof = false;
+  for (int k = 0; k < count; k++) want_rof = want_rof || P.rof[k] != nullptr;
+  if (want_rof) {
+    if (rt.atom < 0) {
+      set_error("fdmgpu_sphere_run_integrals: no absorbing atom (fdmgpu_sphere_set_integrals)");
+      return FDMGPU_EINVAL;
+    }
+    for (int k = 0; k < count; k++)
+      if (!P.rof[k]) {
+        set_error("fdmgpu_sphere_run_integrals: null rof buffer for point " + std::to_string(k));
+        return FDMGPU_EINVAL;
+      }
   }
   const int NT = 64;
   const int nth = natome * (lmax_all + 1);
@@ -349,6 +503,8 @@ int SphereSolver::run(int count, const SphPoints &P, bool cplx) {
   if (nph > 0)
     hipLaunchKernelGGL(k_sph_phiato, dim3((unsigned)((nph + 255) / 256), count), dim3(256), 0, stream, d, P, lmax_all,
                        nlm_max, u, ustride, (int)cplx);
+  if (want_rof)
+    hipLaunchKernelGGL(k_sph_rof, dim3((unsigned)nrof(), count), dim3(64), 0, stream, d, P, rt, lmax_all, u, ustride);
   FDM_HIP(hipGetLastError());
   return 0;
 }

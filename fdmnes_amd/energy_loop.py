@@ -13,7 +13,7 @@ import torch
 
 from .assemble import DeviceAssembler
 from .batch import FP32, FP64, DeviceSolver
-from .tau import split, tau_reduce
+from .tau import split, symmetrize, tau_reduce
 
 __all__ = ["EnergyLoop", "RepresentationLoop", "FP32", "FP64"]
 
@@ -29,11 +29,16 @@ class EnergyLoop:
     sigma(E) = -Im sum_L b_L^T x_L on them."""
 
     def __init__(self, cluster, precision: int = FP32, group: int = 12, E_comp: bool = True,
-                 basereel: bool = False, atoms=None, fdm_comp: bool = False, sphere=None):
+                 basereel: bool = False, atoms=None, fdm_comp: bool = False, sphere=None, tensor=None):
         """sphere: for a cluster with muffin-tin atoms, the
         ``fdmnes_amd.sphere.DeviceSphere`` of its atoms, which computes their
         phiato for every energy on the device (Sphere, sphere.f90:8); tau is
-        then reduced over the atoms' row blocks unless atoms says otherwise."""
+        then reduced over the atoms' row blocks unless atoms says otherwise.
+        tensor: a ``fdmnes_amd.tensor.DeviceTensor`` of the absorbing atom
+        (the sphere's radial integrals set by ``set_integrals``): each
+        point's tau is then symmetrised as mat leaves it (mat.f90:596-641)
+        and turned into the Cartesian cross-section tensors (Tenseur_car,
+        tensor.f90:15)."""
         if not 1 <= group <= 32:
             raise ValueError("group: 1..32 energy points per grouped solve")
         self.cl = cluster
@@ -44,6 +49,9 @@ class EnergyLoop:
         if bool(self.natome) != (sphere is not None):
             raise ValueError("a cluster with atoms needs its DeviceSphere (and one without atoms none)")
         self.sphere = sphere
+        if tensor is not None and (sphere is None or getattr(sphere, "nrof", 0) == 0):
+            raise ValueError("the tensors need the DeviceSphere's radial integrals (set_integrals)")
+        self.tensor = tensor
         if atoms is None and self.natome:
             atoms = [(cluster.atom(a)["ianew"], cluster.atom(a)["nlmsa"]) for a in range(self.natome)]
         self.atoms = list(atoms or [])
@@ -66,7 +74,7 @@ class EnergyLoop:
         self.x = [torch.zeros((n, nl), dtype=dt, device="cuda") for _ in range(self.G)]
         self._filled = 0  # buffers holding an earlier group's assembly (update_only is valid for them)
 
-    def run(self, V, energies, keep_solutions: bool = False, gradient: bool = False) -> dict:
+    def run(self, V, energies, keep_solutions: bool = False, gradient: bool = False, energ=None) -> dict:
         """V: one potential (npoint, Ry) for every energy, or one per energy;
         energies: [(Enervide, Eimag, Ecinetic_out), ...] in Ry.  Returns
         {"sigma": (nE,), "tau": per energy a list of per-atom (nrows, nrows)
@@ -74,7 +82,10 @@ class EnergyLoop:
         solutions (host arrays) when keep_solutions, and "dsigma_dV": (nE,
         npoint) when gradient: d sigma(E) / d V_i by the adjoint (the
         transposed substitutions with each group's factors, FP64 complex
-        loops only)."""
+        loops only).  With a tensor: "secdd", "secdq", "secqq" per energy
+        (host arrays (ninitr, 3, ...), None for a multipole not computed);
+        energ: per energy the photoelectron energy Energ of the quadrupole
+        factor (default Enervide)."""
         if gradient and (self.precision != FP64 or not self.cal):
             raise ValueError("the gradient needs an FP64 loop in complex arithmetic")
         nE = len(energies)
@@ -86,6 +97,10 @@ class EnergyLoop:
         sigma = np.zeros(nE)
         tau, sweeps, xs = [], [], []
         grad = np.zeros((nE, self.cl.npoint)) if gradient else None
+        secs = {"secdd": [], "secdq": [], "secqq": []}
+        energ = [e[0] for e in energies] if energ is None else list(energ)
+        if len(energ) != nE:
+            raise ValueError("one Energ per energy point")
         for e0 in range(0, nE, self.G):
             e1 = min(nE, e0 + self.G)
             k = e1 - e0
@@ -93,9 +108,13 @@ class EnergyLoop:
             # buffers 0..k-1 of a group after the first already hold this
             # structure's rows: only the energy-dependent entries change
             upd = self._filled >= k
-            phi = None
+            phi = rof = None
             if self.sphere is not None:  # the atoms' phiato of these energies (Ecinetic, Eimag, Enervide)
-                phi = self.sphere.run([(e[2], e[1], e[0]) for e in energies[e0:e1]], complex_out=self.cal)
+                ens = [(e[2], e[1], e[0]) for e in energies[e0:e1]]
+                if self.tensor is not None:
+                    phi, rof = self.sphere.run(ens, complex_out=self.cal, energ=energ[e0:e1])
+                else:
+                    phi = self.sphere.run(ens, complex_out=self.cal)
             self.asm.run(vg, energies[e0:e1], self.E_comp, self.basereel, self.vals[:k], self.rhs[:k],
                          update_only=upd, phi=phi)
             self._filled = max(self._filled, k)
@@ -109,10 +128,16 @@ class EnergyLoop:
                     grad[e0 + j] = g[j][:self.cl.npoint].cpu().numpy()
             if self.atoms:
                 t, _, _ = tau_reduce(self.x[:k], self.nso, self.basereel, self.fdm_comp, self.atoms)
+                if self.tensor is not None:
+                    symmetrize(t, [na for _, na in self.atoms])
+                    for name, o in zip(("secdd", "secdq", "secqq"), self.tensor.run(rof, t)):
+                        secs[name] += [None] * k if o is None else [v.cpu().numpy() for v in o]
                 tau += [[b.cpu().numpy() for b in split(ti, self.atoms)] for ti in t]
             if keep_solutions:
                 xs += [xi.cpu().numpy() for xi in self.x[:k]]
         out = {"sigma": sigma, "tau": tau, "sweeps": sweeps}
+        if self.tensor is not None:
+            out.update(secs)
         if keep_solutions:
             out["x"] = xs
         if gradient:

@@ -32,6 +32,10 @@ def _sigs(L):
     L.fdmgpu_sphere_destroy.argtypes = [_V]
     L.fdmgpu_sphere_destroy.restype = None
     L.fdmgpu_sphere_run.argtypes = [_V, C.c_int, P(C.c_double), P(C.c_double), P(C.c_double), C.c_int, P(_V), P(_V)]
+    L.fdmgpu_sphere_set_integrals.argtypes = [_V, C.c_int, C.c_int, P(C.c_double), P(C.c_double), C.c_double,
+                                              C.c_int, C.c_int]
+    L.fdmgpu_sphere_run_integrals.argtypes = [_V, C.c_int, P(C.c_double), P(C.c_double), P(C.c_double),
+                                              P(C.c_double), C.c_int, P(_V), P(_V), P(_V)]
     L._sph_sigs = True
 
 
@@ -52,6 +56,7 @@ class DeviceSphere:
         self.L = L
         self.natome = len(atoms)
         self.nlmsa = [(int(a["lmax"]) + 1) ** 2 for a in atoms]
+        self.lmax = [int(a["lmax"]) for a in atoms]
         self.nbord = [len(a["bxyz"]) for a in atoms]
         self.nphi = sum(nb * nl for nb, nl in zip(self.nbord, self.nlmsa))
         self.ntau = sum(int(a["lmax"]) + 1 for a in atoms)
@@ -83,17 +88,47 @@ class DeviceSphere:
             self.L.fdmgpu_sphere_destroy(self._h)
             self._h = None
 
-    def run(self, energies, complex_out: bool = True, with_tau: bool = False):
+    def set_integrals(self, atom: int, psii, Eseuil, Rmtsd: float, ip0: int = 1, ip_max: int = 2):
+        """Names the absorbing atom (0-based) for the radial integrals of
+        the golden rule (Radial_integral, sphere.f90:1490): its core
+        functions psii (nbseuil, nr of the atom), the edge energies Eseuil
+        (Ry), the integration radius Rmtsd (bohr) and the powers ip0..ip_max
+        of r (1 dipole, 2 quadrupole)."""
+        psii = np.ascontiguousarray(np.atleast_2d(psii), np.float64)
+        es = np.ascontiguousarray(np.atleast_1d(Eseuil), np.float64)
+        D = C.POINTER(C.c_double)
+        if len(es) != psii.shape[0]:
+            raise ValueError("one edge energy per core function")
+        _lib.check(self.L.fdmgpu_sphere_set_integrals(self._h, int(atom), psii.shape[0], psii.ctypes.data_as(D),
+                                                      es.ctypes.data_as(D), float(Rmtsd), int(ip0), int(ip_max)))
+        self.rof_layout = dict(atom=int(atom), nbseuil=psii.shape[0], ip0=int(ip0), ip_max=int(ip_max),
+                               lmax=int(self.lmax[atom]))
+        self.nrof = psii.shape[0] * (ip_max - ip0 + 1) * (self.lmax[atom] + 1)
+
+    def run(self, energies, complex_out: bool = True, with_tau: bool = False, energ=None):
         """energies: list of (Ecinetic, Eimag, Enervide) in Ry.  Returns per
         point a device tensor of the nphi phiato values (complex128, or
         float64 with complex_out False, which needs real radial solutions)
-        and, with with_tau, per point the sum (lmax+1) complex Tau."""
+        and, with with_tau, per point the sum (lmax+1) complex Tau.  With
+        energ (per point the photoelectron energy Energ, Ry) the absorbing
+        atom's radial integrals come last in the returned tuple: per point
+        a complex128 device tensor (nbseuil, ip_max - ip0 + 1, lmax + 1)."""
         n = len(energies)
         dt = torch.complex128 if complex_out else torch.float64
         phi = [torch.empty(self.nphi, dtype=dt, device="cuda") for _ in range(n)]
         tau = [torch.empty(self.ntau, dtype=torch.complex128, device="cuda") for _ in range(n)] if with_tau else None
         E = np.array(energies, np.float64).reshape(n, 3)
         D = C.POINTER(C.c_double)
+        rof = None
+        if energ is not None:
+            if getattr(self, "nrof", 0) == 0:
+                raise ValueError("set_integrals first")
+            en = np.ascontiguousarray(np.atleast_1d(energ), np.float64)
+            if len(en) != n:
+                raise ValueError("one Energ per point")
+            lay = self.rof_layout
+            rof = [torch.empty((lay["nbseuil"], lay["ip_max"] - lay["ip0"] + 1, lay["lmax"] + 1),
+                               dtype=torch.complex128, device="cuda") for _ in range(n)]
 
         def ptrs(ts):
             arr = (_V * len(ts))()
@@ -105,13 +140,24 @@ class DeviceSphere:
             for k0 in range(0, n, self.max_points):
                 k1 = min(n, k0 + self.max_points)
                 cols = [np.ascontiguousarray(E[k0:k1, j]) for j in range(3)]
-                rc = self.L.fdmgpu_sphere_run(self._h, k1 - k0, *[c.ctypes.data_as(D) for c in cols],
-                                              int(bool(complex_out)), ptrs(phi[k0:k1]),
-                                              ptrs(tau[k0:k1]) if tau is not None else None)
+                if rof is None:
+                    rc = self.L.fdmgpu_sphere_run(self._h, k1 - k0, *[c.ctypes.data_as(D) for c in cols],
+                                                  int(bool(complex_out)), ptrs(phi[k0:k1]),
+                                                  ptrs(tau[k0:k1]) if tau is not None else None)
+                else:
+                    ek = np.ascontiguousarray(en[k0:k1])
+                    rc = self.L.fdmgpu_sphere_run_integrals(self._h, k1 - k0, *[c.ctypes.data_as(D) for c in cols],
+                                                            ek.ctypes.data_as(D), int(bool(complex_out)),
+                                                            ptrs(phi[k0:k1]),
+                                                            ptrs(tau[k0:k1]) if tau is not None else None,
+                                                            ptrs(rof[k0:k1]))
                 if rc:
                     break
         _lib.check(rc)
-        return (phi, tau) if with_tau else phi
+        out = (phi, tau) if with_tau else (phi,)
+        if rof is not None:
+            out = out + (rof,)
+        return out if len(out) > 1 else out[0]
 
 
 def atoms_of(cluster, r_grids, potentials, Z, rmtg, posi):

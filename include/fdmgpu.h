@@ -380,6 +380,75 @@ void fdmgpu_sphere_destroy(fdmgpu_sphere *s);
  * sum (lmax + 1) complex Tau (interleaved).  Asynchronous on the stream. */
 int fdmgpu_sphere_run(fdmgpu_sphere *s, int count, const double *Ecinetic, const double *Eimag,
                       const double *Enervide, int complex_out, double *const *d_phi, double *const *d_tau);
+/*
+ * The absorbing atom's radial integrals of the golden rule (Radial_integral,
+ * sphere.f90:1490, its call of radial_matrix :2209, as Tenseur_car makes it,
+ * tensor.f90:185): with the same renormalised regular solution u_L as
+ * phiato, rof(is, ip, L) = fac(ip) * integral_0^Rmtsd psii_is(r) r u_L(r) r^ip
+ * dr by f_integr3 (tools.f90:308), fac = alfa_sf / 2, 1, Vecond / 2,
+ * -Vecond^2 / 6 for ip = 0..3 with Vecond = alfa_sf / 2 * max(0.001, Energ +
+ * Eseuil(is)); the imaginary part from Im u_L.  set_integrals names the
+ * absorbing atom (0-based), its core functions psii [nbseuil][nr of the
+ * atom] (the reference's psii(1:nr, is)), the edge energies Eseuil (Ry), the
+ * integration radius Rmtsd (beyond the third radial point) and ip0..ip_max
+ * (0..3).  run_integrals is fdmgpu_sphere_run with, per point, Energ and a
+ * device array d_rof[k] of nbseuil * (ip_max - ip0 + 1) * (lmax + 1) complex
+ * values (interleaved), index (is * nip + ip - ip0) * (lmax + 1) + L.
+ */
+int fdmgpu_sphere_set_integrals(fdmgpu_sphere *s, int atom, int nbseuil, const double *psii, const double *Eseuil,
+                                double Rmtsd, int ip0, int ip_max);
+int fdmgpu_sphere_run_integrals(fdmgpu_sphere *s, int count, const double *Ecinetic, const double *Eimag,
+                                const double *Enervide, const double *Energ, int complex_out, double *const *d_phi,
+                                double *const *d_tau, double *const *d_rof);
+
+/*
+ * The absorbing atom's Cartesian cross-section tensors per energy point
+ * (Tenseur_car, tensor.f90:15-946, with tens_ab :947-1345; called by the
+ * energy loop at fdm.f90:3975): the electric dipole-dipole secdd(3, 3),
+ * dipole-quadrupole secdq(3, 3, 3) and quadrupole-quadrupole secqq(3, 3, 3,
+ * 3) tensors of every resolved core state or edge (ninitr), from the
+ * atom's Taull and radial integrals rof:
+ *   Ten(lm_s, lm_i) = i * sum Cg * (conj(rof1) rof2 Taull(lm1, lm2)
+ *                     - conj(conj(rof2) rof1 Taull(lm2, lm1))) / 2
+ * over core states and final-state harmonics (Cg = Clebsch-Gordan weights
+ * times Gaunt coefficients), secXX = sum clm clm Ten over the spherical
+ * components of the Cartesian operators, spread to the elements equal by
+ * symmetry (ldip, lqua and the msym tables, as the reference's Tensor_shape
+ * and tensdd / tensdq / tensqq give them), then turned by rot_int x
+ * rot_atom_abs^T.  The branch of an FDM absorption calculation: no TDDFT,
+ * optic, NRIXS, Green's-function or FDM_comp output, no singular solution,
+ * no spin-orbit, one final-state spin, spherical potential, real harmonics,
+ * electric multipoles E1E1, E1E2, E2E2.
+ */
+typedef struct fdmgpu_tensor fdmgpu_tensor;
+typedef struct {
+  int jseuil, lseuil, nbseuil; /* the edge (coef_init's Jseuil, Lseuil) and the number of edges (1, 2) */
+  int core_resolved;           /* one tensor per core state (else per edge) */
+  int lmax;                    /* the probe's lmax (0..3), nlm_probe = (lmax + 1)^2 */
+  int lplus1, lmoins1;         /* final states l > lseuil only / l <= lseuil only */
+  int numat;                   /* atomic number (1 halves the tensors: one core state) */
+  int e1e1, e1e2, e2e2;        /* the multipoles computed */
+  const int *ldip;             /* [3] */
+  const int *lqua;             /* [3][3] */
+  const int *msymdd, *msymddi; /* [3][3] */
+  const int *msymdq, *msymdqi; /* [3][3][3] */
+  const int *msymqq, *msymqqi; /* [3][3][3][3]: tables in C order of the reference's Fortran indices */
+  const double *rot_int, *rot_atom_abs; /* [3][3], (i, j) at [3 i + j] */
+  int rof_lmax, rof_nbseuil, rof_ip0, rof_ip_max; /* the layout of fdmgpu_sphere_run_integrals's rof */
+  int64_t taull_offset;        /* complex offset of the absorber's block in a point's tau */
+  int taull_ld;                /* its row length (the absorber's nlmsa) */
+} fdmgpu_tensor_tables;
+/* max_points 1..32; hip_stream may be NULL.  ninitr: tensors per point. */
+int fdmgpu_tensor_create(const fdmgpu_tensor_tables *t, int max_points, void *hip_stream, fdmgpu_tensor **out);
+void fdmgpu_tensor_destroy(fdmgpu_tensor *h);
+int fdmgpu_tensor_ninitr(const fdmgpu_tensor *h);
+/* count points: d_rof[k] (fdmgpu_sphere_run_integrals's output), d_tau[k]
+ * (fdmgpu_tau_reduce's, after fdmgpu_tau_symmetrize as mat does); outputs
+ * per point, complex interleaved, C order: d_secdd[k] [ninitr][3][3],
+ * d_secdq[k] [ninitr][3][3][3], d_secqq[k] [ninitr][3][3][3][3] (NULL for a
+ * multipole not computed).  Asynchronous on the stream. */
+int fdmgpu_tensor_run(fdmgpu_tensor *h, int count, const double *const *d_rof, const double *const *d_tau,
+                      double *const *d_secdd, double *const *d_secdq, double *const *d_secqq);
 
 /*
  * Post-solve reduction to the atom amplitude products (mat.f90:297-405,

@@ -157,3 +157,14 @@ def test_building_blocks_match_reference():
     for fct, R0 in ((np.sin(3 * r) * r * r, 1.7), (np.exp(-r) * r ** 5, 0.9), (np.where(r < 2e-4, 0.0, r), 2.2),
                     (np.cos(r), 5.0), (r * r, float(r[57]))):
         assert O.f_integr3(r, fct, R0) == pytest.approx(R.f_integr3(r, fct, R0), rel=1e-14, abs=1e-300)
+
+
+@pytest.mark.ref
+def test_no_symmetry_tables_are_the_references():
+    from oracle import ref as R
+    if not R.available():
+        pytest.skip("needs oracle/_ref (built where the reference sources are)")
+    from fdmnes_amd.tensor import no_symmetry_tables
+    a, b = R.tensor_tables([1]), no_symmetry_tables()
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
