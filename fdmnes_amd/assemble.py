@@ -111,6 +111,14 @@ class DeviceAssembler:
                                                  C.byref(h)))
         self._h = h
         self.max_points = int(max_points)
+        self.cx = bool(getattr(cluster, "ylm_comp", False))
+        if self.cx:  # complex harmonics / characters: the imaginary parts of the tables
+            P = C.POINTER(C.c_double)
+            L.fdmgpu_assembler_set_complex.argtypes = [_V, P, P, P, P, P]
+            self._keep_cx = {k: np.ascontiguousarray(t[k]) for k in ("base_im", "ow_im", "ylm_im", "dself",
+                                                                      "dself_im")}
+            _lib.check(L.fdmgpu_assembler_set_complex(h, *[self._keep_cx[k].ctypes.data_as(P) for k in
+                                                           ("base_im", "ow_im", "ylm_im", "dself", "dself_im")]))
         self.relativistic = bool(relativistic)
         if self.relativistic:
             from .cluster import REL_TABLES
@@ -131,7 +139,7 @@ class DeviceAssembler:
         """V: list of float64 device tensors (npoint); energies: list of
         (Enervide, Eimag, Ecinetic_out) in Ry; vals (and rhs) lists of
         contiguous device tensors, complex128 when the arithmetic is complex
-        (E_comp or not basereel), float64 otherwise.  update_only: the
+        (E_comp or not basereel, or complex harmonics), float64 otherwise.  update_only: the
         buffers hold an earlier call's output for this cluster (same
         arithmetic branch); only the energy- and V-dependent entries are
         rewritten.  phi (clusters with atoms): per point a device tensor
@@ -144,7 +152,7 @@ class DeviceAssembler:
             return
         if not (len(V) == len(vals) == n) or (rhs is not None and len(rhs) != n):
             raise ValueError("one V, vals (and rhs) per energy point")
-        cal = bool(E_comp) or not basereel
+        cal = bool(E_comp) or not basereel or self.cx
         for t in list(vals) + list(rhs or []):
             if t.is_complex() != cal:
                 raise ValueError("value and RHS buffers must be complex exactly when the arithmetic is")

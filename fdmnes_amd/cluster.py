@@ -51,6 +51,10 @@ def _L():
                                                         C.c_void_p, C.c_void_p, C.c_void_p]
         L.fdmgpu_cluster_assembly_sizes.restype = C.c_int
         L.fdmgpu_cluster_assembly_sizes.argtypes = [C.c_void_p, _lib.lp]
+        L.fdmgpu_cluster_assembly_tables_complex.restype = C.c_int
+        L.fdmgpu_cluster_assembly_tables_complex.argtypes = [C.c_void_p, _lib.dp, _lib.dp, _lib.lp, _lib.ip, _lib.lp,
+                                                             _lib.ip, _lib.ip, _lib.dp, _lib.dp, _lib.dp, _lib.dp,
+                                                             _lib.dp, _lib.dp, _lib.dp, _lib.dp]
         L.fdmgpu_cluster_assembly_tables.restype = C.c_int
         L.fdmgpu_cluster_assembly_tables.argtypes = [C.c_void_p, _lib.dp, _lib.lp, _lib.ip, _lib.lp, _lib.ip,
                                                      _lib.ip, _lib.dp, _lib.dp, _lib.dp, _lib.dp]
@@ -300,8 +304,16 @@ class Cluster:
                  oq=np.zeros(max(nrow, 1), np.int64), optr=np.zeros(nrow + 1, np.int32),
                  ob=np.zeros(max(nlink, 1), np.int32), ow=np.zeros(max(nlink, 1)),
                  ylm=np.zeros(nb * self.nlmso), brad=np.zeros(nb), par=np.zeros(2))
-        rc = L.fdmgpu_cluster_assembly_tables(self._h, t["base"], t["qdiag"], t["orow"], t["oq"], t["optr"], t["ob"],
-                                              t["ow"], t["ylm"], t["brad"], t["par"])
+        if self.ylm_comp:  # complex harmonics (and, for 4, -4, 4/m, complex characters)
+            t.update(base_im=np.zeros(nnz), ow_im=np.zeros(max(nlink, 1)), ylm_im=np.zeros(nb * self.nlmso),
+                     dself_c=np.zeros(3 * self.npoint), dself_im=np.zeros(3 * self.npoint))
+            rc = L.fdmgpu_cluster_assembly_tables_complex(self._h, t["base"], t["base_im"], t["qdiag"], t["orow"],
+                                                          t["oq"], t["optr"], t["ob"], t["ow"], t["ow_im"], t["ylm"],
+                                                          t["ylm_im"], t["brad"], t["par"], t["dself_c"],
+                                                          t["dself_im"])
+        else:
+            rc = L.fdmgpu_cluster_assembly_tables(self._h, t["base"], t["qdiag"], t["orow"], t["oq"], t["optr"],
+                                                  t["ob"], t["ow"], t["ylm"], t["brad"], t["par"])
         if rc:
             raise _lib.FdmGpuError(f"fdmgpu_cluster_assembly_tables failed ({rc})")
         t["lharm"] = np.zeros(self.nlmso, np.int32)
@@ -310,6 +322,8 @@ class Cluster:
         rc = L.fdmgpu_cluster_assembly_weights(self._h, t["lharm"], t["wso"], t["dself"])
         if rc:
             raise _lib.FdmGpuError(f"fdmgpu_cluster_assembly_weights failed ({rc})")
+        if self.ylm_comp:
+            t["dself"] = t.pop("dself_c")
         if self.natome:
             phi = [np.zeros((self.atom(a)["nbord"], self.atom(a)["nlmsa"])) for a in range(self.natome)]
             rp, _, _, _ = self.assemble_atoms(np.zeros(self.npoint), 0.0, 0.0, 1.0, False, True, phi)

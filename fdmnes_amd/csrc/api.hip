@@ -591,6 +591,16 @@ int fdmgpu_assembler_set_relativistic(fdmgpu_assembler *a, const fdmgpu_rel_tabl
   return a->a.init_rel(t);
 }
 
+int fdmgpu_assembler_set_complex(fdmgpu_assembler *a, const double *base_im, const double *ow_im,
+                                 const double *ylm_im, const double *dself, const double *dself_im) {
+  if (!a) {
+    set_error("fdmgpu_assembler_set_complex: invalid argument");
+    return FDMGPU_EINVAL;
+  }
+  FDM_HIP(hipSetDevice(g_lib.device));
+  return a->a.init_complex(base_im, ow_im, ylm_im, dself, dself_im);
+}
+
 int fdmgpu_assembler_run_atoms(fdmgpu_assembler *a, int count, const double *Enervide, const double *Eimag,
                                const double *Ecinetic_out, int E_comp, int basereel, int update_only,
                                const double *const *d_V, const double *const *d_phi, int phi_cplx,

@@ -59,11 +59,19 @@ struct Assembler {
   double *rfac = nullptr, *rdiv = nullptr, *ofac = nullptr;
   int *opoint = nullptr, *odir = nullptr, *apoint = nullptr, *adir = nullptr;
 
+  // complex harmonics or characters (init_complex): the imaginary parts of
+  // base, of the outside links' weights, of the outer harmonics and of the
+  // points' own-image terms; the arithmetic is then complex
+  bool cx = false;
+  double *base_im = nullptr, *ow_im = nullptr, *ylm_im = nullptr, *dself_im = nullptr;
+
   ~Assembler();
   // at: null for a cluster without atoms
   int init(const fdmgpu_assembly_tables *t, const fdmgpu_atom_tables *at, int max_points, hipStream_t s);
   int init_atoms(const fdmgpu_atom_tables *at, int64_t natrow, const int64_t *row_ptr_h);
   int init_rel(const fdmgpu_rel_tables *t);
+  int init_complex(const double *base_im_h, const double *ow_im_h, const double *ylm_im_h, const double *dself_h,
+                   const double *dself_im_h);
   // update_only: vals/rhs already hold an earlier call's output for this
   // structure; only the energy-dependent entries are rewritten.  phi_cplx:
   // the points' phiato are complex (atoms only).  An assembler with

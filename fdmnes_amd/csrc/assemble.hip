@@ -151,7 +151,8 @@ __device__ inline cz cfac(const cz *f, int l, bool basereel) {
 
 // 2. values from the base table; RHS cleared
 template <bool CAL>
-__global__ void k_asm_base(int64_t nnz, int64_t nrhs_el, const double *__restrict__ base, AsmPoints P) {
+__global__ void k_asm_base(int64_t nnz, int64_t nrhs_el, const double *__restrict__ base,
+                           const double *__restrict__ base_im, AsmPoints P) {
   const int g = blockIdx.y;
   double *v = P.vals[g], *r = P.rhs[g];
   const int64_t n = nnz > nrhs_el ? nnz : nrhs_el;
@@ -159,7 +160,7 @@ __global__ void k_asm_base(int64_t nnz, int64_t nrhs_el, const double *__restric
     if (q < nnz) {
       if constexpr (CAL) {
         v[2 * q] = base[q];
-        v[2 * q + 1] = 0;
+        v[2 * q + 1] = base_im ? base_im[q] : 0.0;
       } else {
         v[q] = base[q];
       }
@@ -178,7 +179,8 @@ __global__ void k_asm_base(int64_t nnz, int64_t nrhs_el, const double *__restric
 // 3. grid diagonals
 template <bool CAL>
 __global__ void k_asm_diag(int npoint, const int64_t *__restrict__ qdiag, double clapl0,
-                           const double *__restrict__ dself, AsmPoints P, int e_comp, int rel) {
+                           const double *__restrict__ dself, const double *__restrict__ dself_im, AsmPoints P,
+                           int e_comp, int rel) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x, g = blockIdx.y;
   if (i >= npoint) return;
   const AsmEnergy E = P.e[g];
@@ -194,8 +196,14 @@ __global__ void k_asm_diag(int npoint, const int64_t *__restrict__ qdiag, double
     d = d + dself[3 * i + 2];
   }
   if constexpr (CAL) {
+    double di = e_comp ? -E.eimag : 0.0;
+    if (dself_im) {  // complex characters: the images' terms are complex
+      di = di + dself_im[3 * i];
+      di = di + dself_im[3 * i + 1];
+      di = di + dself_im[3 * i + 2];
+    }
     P.vals[g][2 * q] = d;
-    P.vals[g][2 * q + 1] = e_comp ? -E.eimag : 0.0;
+    P.vals[g][2 * q + 1] = di;
   } else {
     P.vals[g][q] = d;
   }
@@ -212,7 +220,8 @@ __global__ void k_asm_outcols(int nrow_out, int nl, int nb, int L, const int *__
                               const double *__restrict__ ylm, const int *__restrict__ lharm,
                               const cz *__restrict__ fun, AsmPoints P, int basereel,
                               const int *__restrict__ opoint, const int *__restrict__ odir,
-                              const double *__restrict__ ofac, RelCoef rc) {
+                              const double *__restrict__ ofac, RelCoef rc, const double *__restrict__ ow_im,
+                              const double *__restrict__ ylm_im) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x, g = blockIdx.y;
   if (t >= nrow_out * nl) return;
   const int r = t / nl, lm = t % nl, l = lharm[lm];
@@ -234,6 +243,12 @@ __global__ void k_asm_outcols(int nrow_out, int nl, int nb, int L, const int *__
       w = ow[k];
     const double y = ylm[(int64_t)b * nl + lm];
     const cz *f = fg + (int64_t)b * (L + 1) * 2;
+    if (CAL && ylm_im) {  // complex harmonics (Yc) and weights chi -clapl(v) rvol(i) (mat.f90:1328-1377)
+      const cz yc = {y, ylm_im[(int64_t)b * nl + lm]}, wc = {w, ow_im ? ow_im[k] : 0.0};
+      row = add(row, mul(mul(cfac(f, l, basereel), yc), wc));
+      rrow = sub(rrow, mul(mul(f[2 * l], yc), wc));
+      continue;
+    }
     row = add(row, scale(scale(cfac(f, l, basereel), y), w));
     rrow = sub(rrow, scale(scale(f[2 * l], y), w));
   }
@@ -264,7 +279,8 @@ __global__ __launch_bounds__(256) void k_asm_outrows_part(int nl, int nb, int L,
                                                            const double *__restrict__ ylm,
                                                            const int *__restrict__ lharm,
                                                            const cz *__restrict__ fun, int basereel,
-                                                           cz *__restrict__ part) {
+                                                           cz *__restrict__ part,
+                                                           const double *__restrict__ ylm_im) {
   const int run = blockIdx.x, nrun = gridDim.x, g = blockIdx.z;
   const int pr = blockIdx.y * blockDim.x + threadIdx.x;
   if (pr >= nl * nl) return;
@@ -272,12 +288,23 @@ __global__ __launch_bounds__(256) void k_asm_outrows_part(int nl, int nb, int L,
   const cz *fg = fun + (int64_t)g * nb * (L + 1) * 2;
   cz s = {0, 0}, sr = {0, 0};
   const int b1 = min(nsortf, (run + 1) * ASM_CB);
+  if (ylm_im) {  // complex harmonics: w conj(Yc_lm) Yc_lp (mat.f90:1425-1497)
+    for (int b = run * ASM_CB; b < b1; b++) {
+      const cz y1 = {ylm[(int64_t)b * nl + lm], -ylm_im[(int64_t)b * nl + lm]};
+      const cz y2 = {ylm[(int64_t)b * nl + lp], ylm_im[(int64_t)b * nl + lp]};
+      const cz cb = scale(mul(y1, y2), wso[b]), cr = mul(scale(y1, wso[b]), y2);
+      const cz *f = fg + (int64_t)b * (L + 1) * 2;
+      s = add(s, mul(cb, cfac(f, l, basereel)));
+      sr = add(sr, mul(cr, f[2 * l]));
+    }
+  } else {
 #pragma unroll 8
   for (int b = run * ASM_CB; b < b1; b++) {
     const double c = wso[b] * ylm[(int64_t)b * nl + lm] * ylm[(int64_t)b * nl + lp];
     const cz *f = fg + (int64_t)b * (L + 1) * 2;
     s = add(s, scale(cfac(f, l, basereel), c));
     sr = add(sr, scale(f[2 * l], c));
+  }
   }
   cz *o = part + (((int64_t)g * nrun + run) * nl * nl + pr) * 2;
   o[0] = s;
@@ -446,7 +473,8 @@ Assembler::~Assembler() {
                   (void *)part, (void *)aq, (void *)alphi, (void *)bq, (void *)phoff, (void *)poff, (void *)yoff,
                   (void *)aa, (void *)aptr, (void *)ianew, (void *)nlmsa, (void *)nbordf, (void *)arow_atom,
                   (void *)arow_lm, (void *)alw, (void *)poidsa, (void *)ylmato, (void *)rq, (void *)rfac,
-                  (void *)rdiv, (void *)ofac, (void *)opoint, (void *)odir, (void *)apoint, (void *)adir})
+                  (void *)rdiv, (void *)ofac, (void *)opoint, (void *)odir, (void *)apoint, (void *)adir,
+                  (void *)base_im, (void *)ow_im, (void *)ylm_im, (void *)dself_im})
     if (p) (void)hipFree(p);
 }
 
@@ -675,6 +703,27 @@ int Assembler::init_rel(const fdmgpu_rel_tables *t) {
   return 0;
 }
 
+int Assembler::init_complex(const double *bi, const double *wi, const double *yi, const double *ds,
+                            const double *dsi) {
+  if (cx || rel || natome > 0) {
+    set_error("fdmgpu_assembler_set_complex: already complex, relativistic or with atoms");
+    return FDMGPU_EINVAL;
+  }
+  if (!bi || !yi || !ds || !dsi || (nlink > 0 && !wi)) {
+    set_error("fdmgpu_assembler_set_complex: null table");
+    return FDMGPU_EINVAL;
+  }
+  if (dself) {
+    FDM_HIP(hipFree(dself));
+    dself = nullptr;
+  }
+  if (upload(&base_im, bi, nnz) || upload(&ow_im, wi, nlink) || upload(&ylm_im, yi, (int64_t)nb * nlmso) ||
+      upload(&dself, ds, 3 * (int64_t)npoint) || upload(&dself_im, dsi, 3 * (int64_t)npoint))
+    return FDMGPU_EHIP;
+  cx = true;
+  return 0;
+}
+
 int Assembler::run(int count, const AsmPoints &P, bool e_comp, bool basereel, bool update_only, bool phi_cplx) {
   if (count < 1 || count > max_points) {
     set_error("fdmgpu_assembler_run: count must be in 1.." + std::to_string(max_points));
@@ -686,7 +735,7 @@ int Assembler::run(int count, const AsmPoints &P, bool e_comp, bool basereel, bo
                     : "fdmgpu_assembler_run_rel: the assembler holds no relativistic tables");
       return FDMGPU_EINVAL;
     }
-  const bool cal = e_comp || !basereel;
+  const bool cal = e_comp || !basereel || cx;  // complex harmonics or characters make it complex (mat.f90:97-104)
   if (phi_cplx && !cal) {
     set_error("fdmgpu_assembler_run_atoms: complex phiato needs complex arithmetic (E_comp or !basereel)");
     return FDMGPU_EINVAL;
@@ -704,31 +753,35 @@ int Assembler::run(int count, const AsmPoints &P, bool e_comp, bool basereel, bo
   const dim3 gp(nrun, npr, count), gs(npr, count);
   cz *Q = (cz *)part;
   if (cal) {
-    if (!update_only) hipLaunchKernelGGL(k_asm_base<true>, dim3(gb, count), dim3(NT), 0, stream, nnz, nrhs_el, base, P);
-    hipLaunchKernelGGL(k_asm_diag<true>, dim3(gd, count), dim3(NT), 0, stream, npoint, qdiag, clapl0, dself, P,
-                       (int)e_comp, (int)rel);
+    if (!update_only)
+      hipLaunchKernelGGL(k_asm_base<true>, dim3(gb, count), dim3(NT), 0, stream, nnz, nrhs_el, base, base_im, P);
+    hipLaunchKernelGGL(k_asm_diag<true>, dim3(gd, count), dim3(NT), 0, stream, npoint, qdiag, clapl0, dself, dself_im,
+                       P, (int)e_comp, (int)rel);
     if (rel)
       hipLaunchKernelGGL(k_asm_rel<true>, dim3(gd, count), dim3(NT), 0, stream, npoint, qdiag, rq, rfac, rdiv, relc, P);
     auto oc = rel ? k_asm_outcols<true, true> : k_asm_outcols<true, false>;
     if (nrow_out > 0)
       hipLaunchKernelGGL(oc, dim3(go, count), dim3(NT), 0, stream, nrow_out, nlmso, nb, lmaxso, orow, oq, optr, ob, ow,
-                         ylm, lharm, F, P, (int)basereel, opoint, odir, ofac, relc);
+                         ylm, lharm, F, P, (int)basereel, opoint, odir, ofac, relc, ow_im, ylm_im);
     hipLaunchKernelGGL(k_asm_outrows_part, gp, dim3(256), 0, stream, nlmso, nb, lmaxso, nsortf, wso, ylm, lharm,
-                       F, (int)basereel, Q);
+                       F, (int)basereel, Q, ylm_im);
     hipLaunchKernelGGL(k_asm_outrows_sum<true>, gs, dim3(256), 0, stream, nso, nlmso, nsortf, nrun, row_ptr, Q, P);
   } else {
-    if (!update_only) hipLaunchKernelGGL(k_asm_base<false>, dim3(gb, count), dim3(NT), 0, stream, nnz, nrhs_el, base, P);
-    hipLaunchKernelGGL(k_asm_diag<false>, dim3(gd, count), dim3(NT), 0, stream, npoint, qdiag, clapl0, dself, P, 0,
-                       (int)rel);
+    if (!update_only)
+      hipLaunchKernelGGL(k_asm_base<false>, dim3(gb, count), dim3(NT), 0, stream, nnz, nrhs_el, base,
+                         (const double *)nullptr, P);
+    hipLaunchKernelGGL(k_asm_diag<false>, dim3(gd, count), dim3(NT), 0, stream, npoint, qdiag, clapl0, dself,
+                       (const double *)nullptr, P, 0, (int)rel);
     if (rel)
       hipLaunchKernelGGL(k_asm_rel<false>, dim3(gd, count), dim3(NT), 0, stream, npoint, qdiag, rq, rfac, rdiv, relc,
                          P);
     auto oc = rel ? k_asm_outcols<false, true> : k_asm_outcols<false, false>;
     if (nrow_out > 0)
       hipLaunchKernelGGL(oc, dim3(go, count), dim3(NT), 0, stream, nrow_out, nlmso, nb, lmaxso, orow, oq, optr, ob, ow,
-                         ylm, lharm, F, P, 1, opoint, odir, ofac, relc);
+                         ylm, lharm, F, P, 1, opoint, odir, ofac, relc, (const double *)nullptr,
+                         (const double *)nullptr);
     hipLaunchKernelGGL(k_asm_outrows_part, gp, dim3(256), 0, stream, nlmso, nb, lmaxso, nsortf, wso, ylm, lharm,
-                       F, 1, Q);
+                       F, 1, Q, (const double *)nullptr);
     hipLaunchKernelGGL(k_asm_outrows_sum<false>, gs, dim3(256), 0, stream, nso, nlmso, nsortf, nrun, row_ptr, Q, P);
   }
   if (natome > 0) {

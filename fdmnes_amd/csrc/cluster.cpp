@@ -1336,14 +1336,20 @@ int fdmgpu_cluster_assembly_sizes(const fdmgpu_cluster *c, int64_t *sizes) {
   return 0;
 }
 
-int fdmgpu_cluster_assembly_tables(const fdmgpu_cluster *c, double *base, int64_t *qdiag, int *orow,
-                                   int64_t *oq, int *optr, int *ob, double *ow, double *ylm, double *brad,
-                                   double *par) {
-  // the device tables hold real harmonics and real characters
-  if (c->ylm_comp || is_c4_group(c->group)) return -2;
-  if (c->natome > 0) return atom_cluster_tables(c, base, qdiag, orow, oq, optr, ob, ow, ylm, brad, par);
+// The device assembler's tables of a cluster without atoms; with complex
+// harmonics or characters (cx) the imaginary parts go to base_im, ow_im,
+// ylm_im and the point's own images' terms to dself / dself_im.
+static int cluster_tables(const fdmgpu_cluster *c, double *base, double *base_im, int64_t *qdiag, int *orow,
+                          int64_t *oq, int *optr, int *ob, double *ow, double *ow_im, double *ylm, double *ylm_im,
+                          double *brad, double *par, double *dself, double *dself_im) {
+  const bool cx = c->ylm_comp || is_c4_group(c->group);
   const int np = c->npoint, nl = c->nlmso;
   std::fill(base, base + c->cols.size(), 0.0);
+  if (cx) {
+    std::fill(base_im, base_im + c->cols.size(), 0.0);
+    std::fill(dself, dself + 3 * (size_t)np, 0.0);
+    std::fill(dself_im, dself_im + 3 * (size_t)np, 0.0);
+  }
   int r = 0, k = 0;
   optr[0] = 0;
   for (int i = 0; i < np; i++) {
@@ -1351,19 +1357,40 @@ int fdmgpu_cluster_assembly_tables(const fdmgpu_cluster *c, double *base, int64_
     int64_t q = q0;
     for (; q < q1 && c->cols[q] <= np; q++) {
       const int j = c->cols[q] - 1;
-      if (j == i) {  // E-dependent; a point's own mirror images are in dself
+      if (j == i) {  // E-dependent; a point's own images are in dself
         qdiag[i] = q;
+        if (cx)
+          for (int d = 0; d < 12; d++)
+            if (c->nbr[(size_t)i * 12 + d] == i) {
+              const cd v = stencil_c(c, i, d, 0.0, nullptr) / c->rvol[i];
+              dself[3 * (size_t)i + (d % 6) / 2] += v.real();
+              dself_im[3 * (size_t)i + (d % 6) / 2] += v.imag();
+            }
         continue;
       }
       for (int d = 0; d < 12; d++)
-        if (c->nbr[(size_t)i * 12 + d] == j) base[q] += stencil(c, i, d) / c->rvol[j];
+        if (c->nbr[(size_t)i * 12 + d] == j) {
+          if (cx) {
+            const cd v = stencil_c(c, i, d, 0.0, nullptr) / c->rvol[j];
+            base[q] += v.real();
+            base_im[q] += v.imag();
+          } else {
+            base[q] += stencil(c, i, d) / c->rvol[j];
+          }
+        }
     }
     bool out = false;
     for (int d = 0; d < 12; d++) {
       const int j = c->nbr[(size_t)i * 12 + d];
       if (j < np) continue;
       ob[k] = c->slot[j];
-      ow[k] = stencil(c, i, d);
+      if (cx) {
+        const cd w = stencil_c(c, i, d, 0.0, nullptr);
+        ow[k] = w.real();
+        ow_im[k] = w.imag();
+      } else {
+        ow[k] = stencil(c, i, d);
+      }
       k++;
       out = true;
     }
@@ -1376,13 +1403,48 @@ int fdmgpu_cluster_assembly_tables(const fdmgpu_cluster *c, double *base, int64_
   const double pw = 4 * kPi / c->nsortf;
   for (int lm = 0; lm < nl; lm++) {
     const int64_t q = c->rp[np + lm];
-    for (int b = 0; b < c->nsortf; b++) base[q + b] = -(c->wso[b] * c->ylm[(size_t)b * nl + lm]) / c->rvol[c->bnd[b]];
+    for (int b = 0; b < c->nsortf; b++) {
+      if (cx) {  // -poidso conj(Yc) / rvol (mat.f90:1476-1483 with Ylm_comp)
+        const cd f = c->wso[b] * std::conj(ysort(c, b, lm));
+        base[q + b] = -f.real() / c->rvol[c->bnd[b]];
+        base_im[q + b] = -f.imag() / c->rvol[c->bnd[b]];
+      } else {
+        base[q + b] = -(c->wso[b] * c->ylm[(size_t)b * nl + lm]) / c->rvol[c->bnd[b]];
+      }
+    }
   }
-  std::memcpy(ylm, c->ylm.data(), sizeof(double) * c->ylm.size());
+  for (size_t b = 0; b < c->bnd.size(); b++)
+    for (int lm = 0; lm < nl; lm++) {
+      const cd y = ysort(c, (int)b, lm);
+      ylm[b * nl + lm] = y.real();
+      if (cx) ylm_im[b * nl + lm] = y.imag();
+    }
   for (size_t b = 0; b < c->bnd.size(); b++) brad[b] = c->rad[c->bnd[b]];
   par[0] = c->clapl[0];
   par[1] = pw;
   return 0;
+}
+
+int fdmgpu_cluster_assembly_tables(const fdmgpu_cluster *c, double *base, int64_t *qdiag, int *orow,
+                                   int64_t *oq, int *optr, int *ob, double *ow, double *ylm, double *brad,
+                                   double *par) {
+  // real harmonics and characters only (fdmgpu_cluster_assembly_tables_complex)
+  if (c->ylm_comp || is_c4_group(c->group)) return -2;
+  if (c->natome > 0) return atom_cluster_tables(c, base, qdiag, orow, oq, optr, ob, ow, ylm, brad, par);
+  return cluster_tables(c, base, nullptr, qdiag, orow, oq, optr, ob, ow, nullptr, ylm, nullptr, brad, par, nullptr,
+                        nullptr);
+}
+
+int fdmgpu_cluster_assembly_tables_complex(const fdmgpu_cluster *c, double *base, double *base_im, int64_t *qdiag,
+                                           int *orow, int64_t *oq, int *optr, int *ob, double *ow, double *ow_im,
+                                           double *ylm, double *ylm_im, double *brad, double *par, double *dself,
+                                           double *dself_im) {
+  if (!c || !base || !base_im || !qdiag || !orow || !oq || !optr || !ob || !ow || !ow_im || !ylm || !ylm_im ||
+      !brad || !par || !dself || !dself_im)
+    return -1;
+  if (!(c->ylm_comp || is_c4_group(c->group)) || c->natome > 0 || c->irregular) return -2;
+  return cluster_tables(c, base, base_im, qdiag, orow, oq, optr, ob, ow, ow_im, ylm, ylm_im, brad, par, dself,
+                        dself_im);
 }
 
 int fdmgpu_cluster_assembly_weights(const fdmgpu_cluster *c, int *lharm, double *wso, double *dself) {

@@ -44,7 +44,7 @@ class EnergyLoop:
         self.cl = cluster
         self.G = int(group)
         self.E_comp, self.basereel, self.fdm_comp = bool(E_comp), bool(basereel), bool(fdm_comp)
-        self.cal = self.E_comp or not self.basereel
+        self.cal = self.E_comp or not self.basereel or bool(getattr(cluster, "ylm_comp", False))
         self.natome = int(getattr(cluster, "natome", 0))
         if bool(self.natome) != (sphere is not None):
             raise ValueError("a cluster with atoms needs its DeviceSphere (and one without atoms none)")
@@ -161,7 +161,7 @@ class RepresentationLoop:
         if not self.cls or len(self.cls) > self.MAX_SYSTEMS:
             raise ValueError(f"1..{self.MAX_SYSTEMS} representations")
         self.E_comp, self.basereel = bool(E_comp), bool(basereel)
-        self.cal = self.E_comp or not self.basereel
+        self.cal = self.E_comp or not self.basereel or any(getattr(c, "ylm_comp", False) for c in self.cls)
         self.per = self.MAX_SYSTEMS // len(self.cls)  # energies per grouped solve
         self.nrhs = max(c.nlmso for c in self.cls)
         self.asm = [DeviceAssembler(c, max_points=min(self.per, 16)) for c in self.cls]

@@ -327,6 +327,15 @@ typedef struct {
   const int *apoint;        /* [nar] grid point of each (row, atom) pair (NULL without atoms) */
   const int *adir;          /* [nal] direction of each atom link */
 } fdmgpu_rel_tables;
+/* Complex harmonics or characters (Ylm_comp, the four-fold groups 4, -4,
+ * 4/m; calcMatRow mat.f90:1164-1231, :1328-1497): the assembler was created
+ * from the real parts of fdmgpu_cluster_assembly_tables_complex; this adds
+ * the imaginary parts base_im [nnz], ow_im [nlink], ylm_im [nb * nlmso] and
+ * the points' own-image terms dself / dself_im [3 * npoint] (replacing the
+ * real ones).  The assembler then runs in complex arithmetic only (not with
+ * atoms or relativistic tables). */
+int fdmgpu_assembler_set_complex(fdmgpu_assembler *a, const double *base_im, const double *ow_im,
+                                 const double *ylm_im, const double *dself, const double *dself_im);
 /* Adds the relativistic tables to an assembler (checked and copied); after
  * this the assembler is run only by fdmgpu_assembler_run_rel. */
 int fdmgpu_assembler_set_relativistic(fdmgpu_assembler *a, const fdmgpu_rel_tables *t);

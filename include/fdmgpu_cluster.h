@@ -248,6 +248,20 @@ int fdmgpu_cluster_assembly_sizes(const fdmgpu_cluster *c, int64_t *sizes);
 int fdmgpu_cluster_assembly_tables(const fdmgpu_cluster *c, double *base, int64_t *qdiag, int *orow,
                                    int64_t *oq, int *optr, int *ob, double *ow, double *ylm, double *brad,
                                    double *par);
+/* The same tables for a cluster with complex harmonics or characters
+ * (Ylm_comp, the four-fold groups): the real parts where
+ * fdmgpu_cluster_assembly_tables writes its values, the imaginary parts of
+ * base (interior couplings chi -clapl(v) rvol(i) / rvol(j), the outer rows'
+ * boundary columns -poidso conj(Yc) / rvol(j)), of the outside links'
+ * weights ow and of the harmonics at the outer points (Yc) in base_im,
+ * ow_im and ylm_im, and the terms of a point's own images (per axis, real
+ * and imaginary, dself / dself_im [3*npoint]), which replace
+ * fdmgpu_cluster_assembly_weights's dself.  -2 for a cluster with real
+ * harmonics and characters, with atoms or irregular. */
+int fdmgpu_cluster_assembly_tables_complex(const fdmgpu_cluster *c, double *base, double *base_im, int64_t *qdiag,
+                                           int *orow, int64_t *oq, int *optr, int *ob, double *ow, double *ow_im,
+                                           double *ylm, double *ylm_im, double *brad, double *par, double *dself,
+                                           double *dself_im);
 /* The rest of a device assembler's tables:
  * lharm[nlmso]    l of each outer unknown
  * wso[nsortf]     outer-row quadrature weight of each inner boundary point

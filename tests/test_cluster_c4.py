@@ -179,14 +179,84 @@ def test_complex_harmonic_representations_reproduce_whole_cluster(group):
     assert sorted(covered) == list(range(o.nlmso))
 
 
-def test_complex_groups_need_complex_harmonics_and_refuse_device_tables():
+def test_complex_groups_need_complex_harmonics():
     from fdmnes_amd import _lib
     from fdmnes_amd import cluster as CL
     c = CL.Cluster(1.2 / BOHR, 0.3 / BOHR, 2, irrep=1, group=9)
     assert c.ylm_comp
-    with pytest.raises(_lib.FdmGpuError):  # the device assembler's tables are real
-        c.assembly_tables()
     with pytest.raises(_lib.FdmGpuError):  # mm2's mirrors mix Yc(l, m) and Yc(l, -m)
         CL.Cluster(1.2 / BOHR, 0.3 / BOHR, 2, irrep=0, group=6, ylm_comp=True)
     with pytest.raises(_lib.FdmGpuError):  # k = 0..3 for 4
         CL.Cluster(1.2 / BOHR, 0.3 / BOHR, 2, irrep=4, group=9)
+
+
+def _emulate_complex(c, V, E, Ei, Ec, E_comp, basereel):
+    """The device assembler's kernels, in numpy, on the complex tables
+    (fdmgpu_cluster_assembly_tables_complex), with the outer functions from
+    scipy: rows and RHS of one energy."""
+    from scipy.special import spherical_jn, spherical_yn
+    t = c.assembly_tables()
+    nl, nso = c.nlmso, c.nligne - c.nlmso
+    lof = t["lharm"]
+    Y = (t["ylm"] + 1j * t["ylm_im"]).reshape(t["nb"], nl)
+    k = np.sqrt(Ec + 1j * Ei) if E_comp else np.sqrt(Ec)
+    fn = np.sqrt(k / np.pi)
+    z = k * t["brad"]
+    B = np.stack([fn * spherical_jn(l, z) for l in range(c.lmaxso + 1)], axis=1)
+    N = np.stack([fn * spherical_yn(l, z) for l in range(c.lmaxso + 1)], axis=1)
+    cf = N if basereel else B + 1j * N
+    vals = t["base"] + 1j * t["base_im"]
+    rhs = np.zeros((c.nligne, nl), np.complex128)
+    qd = t["qdiag"]
+    ds = (t["dself"] + 1j * t["dself_im"]).reshape(-1, 3).sum(axis=1)
+    vals[qd] = (V - E - t["clapl0"]) + (-1j * Ei if E_comp else 0) + ds
+    w_all = t["ow"] + 1j * t["ow_im"]
+    for r in range(t["nrow_out"]):
+        k0, k1 = t["optr"][r], t["optr"][r + 1]
+        b, w = t["ob"][k0:k1], w_all[k0:k1]
+        for lm in range(nl):
+            vals[t["oq"][r] + lm] = np.sum(cf[b, lof[lm]] * Y[b, lm] * w)
+            rhs[t["orow"][r], lm] = -np.sum(B[b, lof[lm]] * Y[b, lm] * w)
+    ns = t["nsortf"]
+    for lm in range(nl):
+        q = t["row_ptr"][nso + lm] + ns
+        for lp in range(nl):
+            f = t["wso"] * (np.conj(Y[:ns, lm]) * Y[:ns, lp])
+            vals[q + lp] = np.sum(f * cf[:ns, lof[lp]])
+            rhs[nso + lm, lp] = -np.sum(f * B[:ns, lof[lp]])
+    return vals, rhs
+
+
+@pytest.mark.parametrize("group,irrep", [(9, 1), (9, 2), (10, 3), (11, 5), (1, -1), (5, 1)])
+@pytest.mark.parametrize("E_comp,basereel", [(True, False), (False, True)])
+def test_complex_assembly_tables_rebuild_the_host_rows(group, irrep, E_comp, basereel):
+    """The device assembler's complex tables (imaginary parts of the
+    couplings, outer links and harmonics; complex own-image terms) rebuild
+    the host's rows: energy-independent entries exactly, the rest to the
+    outer functions' accuracy (scipy here, the recurrence on the host)."""
+    from fdmnes_amd import cluster as CL
+    kw = dict(irrep=irrep, group=group) if irrep >= 0 else dict(ylm_comp=True)
+    if group == 5:  # 2/m: complex harmonics on request (its operations keep Yc(l, m) apart from Yc(l, -m))
+        kw["ylm_comp"] = True
+    c = CL.Cluster(1.3 / BOHR, 0.3 / BOHR, 2, **kw)
+    assert c.ylm_comp
+    rng = np.random.default_rng(group * 10 + irrep)
+    V = rng.uniform(-1.5, -0.5, c.npoint)
+    E, Ei, Ec = 0.9, (0.04 if E_comp else 0.0), 0.8
+    rp, col, hv, hr = c.assemble(V, E, Ei, Ec, E_comp, basereel)
+    ev, er = _emulate_complex(c, V, E, Ei, Ec, E_comp, basereel)
+    t = c.assembly_tables()
+    fixed = np.ones(c.nnz, bool)
+    fixed[t["qdiag"]] = False
+    for r in range(t["nrow_out"]):
+        fixed[t["oq"][r]:t["oq"][r] + c.nlmso] = False
+    nso = c.nligne - c.nlmso
+    for lm in range(c.nlmso):
+        q = t["row_ptr"][nso + lm] + t["nsortf"]
+        fixed[q:q + c.nlmso] = False
+    assert np.array_equal(ev[fixed], hv[fixed])
+    assert np.abs(ev[t["qdiag"]] - hv[t["qdiag"]]).max() <= 1e-14 * np.abs(hv).max()
+    sc = np.abs(hv).max()
+    assert np.abs(ev - hv).max() <= 1e-10 * sc
+    assert np.abs(er - hr).max() <= 1e-10 * np.abs(hr).max()
+    assert np.abs(hv.imag).max() > 0  # the systems are complex
