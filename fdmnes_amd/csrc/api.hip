@@ -722,9 +722,29 @@ int fdmgpu_sphere_run(fdmgpu_sphere *s, int count, const double *Ecinetic, const
   return fdmgpu_sphere_run_integrals(s, count, Ecinetic, Eimag, Enervide, nullptr, complex_out, d_phi, d_tau, nullptr);
 }
 
+static int sphere_run(fdmgpu_sphere *s, int count, const double *Ecinetic, const double *Eimag,
+                      const double *Enervide, const double *Energ, int complex_out, double *const *d_phi,
+                      double *const *d_tau, double *const *d_rof, double *const *d_phi2);
+
 int fdmgpu_sphere_run_integrals(fdmgpu_sphere *s, int count, const double *Ecinetic, const double *Eimag,
                                 const double *Enervide, const double *Energ, int complex_out, double *const *d_phi,
                                 double *const *d_tau, double *const *d_rof) {
+  return sphere_run(s, count, Ecinetic, Eimag, Enervide, Energ, complex_out, d_phi, d_tau, d_rof, nullptr);
+}
+
+int fdmgpu_sphere_run_ri(fdmgpu_sphere *s, int count, const double *Ecinetic, const double *Eimag,
+                         const double *Enervide, int complex_out, double *const *d_phi, double *const *d_phi2,
+                         double *const *d_tau) {
+  if (!d_phi2) {
+    set_error("fdmgpu_sphere_run_ri: invalid argument");
+    return FDMGPU_EINVAL;
+  }
+  return sphere_run(s, count, Ecinetic, Eimag, Enervide, nullptr, complex_out, d_phi, d_tau, nullptr, d_phi2);
+}
+
+static int sphere_run(fdmgpu_sphere *s, int count, const double *Ecinetic, const double *Eimag,
+                      const double *Enervide, const double *Energ, int complex_out, double *const *d_phi,
+                      double *const *d_tau, double *const *d_rof, double *const *d_phi2) {
   if (!s || !Ecinetic || !Eimag || !Enervide || !d_phi || count < 1 || count > SPH_MAXP || (d_rof && !Energ)) {
     set_error("fdmgpu_sphere_run: invalid argument");
     return FDMGPU_EINVAL;
@@ -746,6 +766,7 @@ int fdmgpu_sphere_run_integrals(fdmgpu_sphere *s, int count, const double *Ecine
     P.phi[k] = d_phi[k];
     P.tau[k] = d_tau ? d_tau[k] : nullptr;
     P.rof[k] = d_rof ? d_rof[k] : nullptr;
+    P.phi2[k] = d_phi2 ? d_phi2[k] : nullptr;
   }
   FDM_HIP(hipSetDevice(g_lib.device));
   return s->s.run(count, P, complex_out != 0);

@@ -30,6 +30,7 @@ struct SphPoints {
   double *phi[SPH_MAXP];  // device phiato per point, (nbord, nlmsa) per atom, concatenated
   double *tau[SPH_MAXP];  // device Tau per point, (lmax + 1) complex per atom, or null
   double *rof[SPH_MAXP];  // device radial integrals of the absorbing atom per point, or null
+  double *phi2[SPH_MAXP];  // device phiato of the irregular solution per point, or null
 };
 // the absorbing atom's radial integrals of the golden rule (kernel argument)
 struct RofTabs {
@@ -61,7 +62,8 @@ struct SphereSolver {
     }
   } d;
   double *u = nullptr;  // renormalised radial solutions: [point][atom][L] x (re[nrmax], im[nrmax])
-  std::vector<int> h_nr, h_lmax;  // host copies of the atoms' sizes
+  double *u2 = nullptr;  // the irregular solutions over r, laid out as u (allocated on first use)
+  std::vector<int> h_nr, h_lmax, h_inr;  // host copies of the atoms' sizes and matching points
   std::vector<std::vector<double>> h_r;
   RofTabs rt;                      // set by set_integrals
   double *d_psii = nullptr;

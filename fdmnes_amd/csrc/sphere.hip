@@ -58,7 +58,8 @@ constexpr double kAlfa = 0.0072973525664;
 
 // One thread per (point, atom, L): the integration, the matching and the
 // renormalised solution written to u (re, im per radial point)
-__global__ void k_sph_radial(SphTabs T, SphPoints P, int lmax_all, double *__restrict__ u, int64_t ustride) {
+__global__ void k_sph_radial(SphTabs T, SphPoints P, int lmax_all, double *__restrict__ u, int64_t ustride,
+                             double *__restrict__ uirr) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x, g = blockIdx.y;
   if (t >= T.natome * (lmax_all + 1)) return;
   const int a = t / (lmax_all + 1), L = t % (lmax_all + 1);
@@ -180,13 +181,97 @@ __global__ void k_sph_radial(SphTabs T, SphPoints P, int lmax_all, double *__res
     ur[i] = w.re;
     ui[i] = ecomp ? w.im : 0.0;
   }
+  if (!P.phi2[g]) return;
+  // the irregular solution (Sch_radial_solsing, sphere.f90:2591-2806, as
+  // Sphere calls it at :152-162): -i r h_L(k r) from the matching point
+  // outwards, h = fnorm (j + i n) by cbessneu's upward recurrence
+  // (:7226-7248; its real form :7356-7378 for real energies), then inwards
+  // by the same three-point recurrence with gm and gp exchanged; over r
+  double *sr = uirr + (((int64_t)g * T.natome + a) * (lmax_all + 1) + L) * ustride;
+  double *si = sr + T.nrmax;
+  const cz fnc = csqrt_({k.re / kPi, k.im / kPi});
+  const double kr = k.re, fnr = sqrt(kr / kPi);
+  for (int ir = inr; ir <= n; ir++) {
+    const double x = r[ir - 1];
+    cz bl, nl;
+    if (ecomp) {
+      const cz zz = cscale(k, x);
+      cz n0 = cscale(cdiv(cmul(fnc, ccos_(zz)), zz), -1.0), b0 = cdiv(cmul(fnc, csin_(zz)), zz);
+      bl = b0;
+      nl = n0;
+      if (L >= 1) {
+        cz n1 = csub(cdiv(n0, zz), b0), b1 = cadd(cdiv(b0, zz), n0);
+        for (int l = 2; l <= L; l++) {
+          const cz n2 = csub(cdiv(cscale(n1, (double)(2 * l - 1)), zz), n0);
+          const cz b2 = csub(cdiv(cscale(b1, (double)(2 * l - 1)), zz), b0);
+          n0 = n1;
+          n1 = n2;
+          b0 = b1;
+          b1 = b2;
+        }
+        bl = b1;
+        nl = n1;
+      }
+    } else {
+      const double zz = kr * x;
+      double n0 = -fnr * cos(zz) / zz, b0 = fnr * sin(zz) / zz;
+      bl = {b0, 0.0};
+      nl = {n0, 0.0};
+      if (L >= 1) {
+        double n1 = n0 / zz - b0, b1 = b0 / zz + n0;
+        for (int l = 2; l <= L; l++) {
+          const double n2 = (2 * l - 1) * n1 / zz - n0, b2 = (2 * l - 1) * b1 / zz - b0;
+          n0 = n1;
+          n1 = n2;
+          b0 = b1;
+          b1 = b2;
+        }
+        bl = {b1, 0.0};
+        nl = {n1, 0.0};
+      }
+    }
+    const cz h = {bl.re - nl.im, bl.im + nl.re};  // bess + i neum
+    sr[ir - 1] = x * h.im;                        // -i r h
+    si[ir - 1] = -(x * h.re);
+  }
+  for (int ir = inr; ir >= 2; ir--) {
+    const int i0 = ir - 1, ip = ir - 2, im = ir;  // 0-based: the point, the next one inwards, the one outwards
+    const double vme = V[i0] - Ev;
+    double g0 = -cl0[i0] + vme, gm = -clm[i0], gp = -clp[i0];
+    if (T.rel) {
+      const double bder = 1 / (1 - kA2s4 * vme);
+      double dvr;
+      if (i0 == 0)
+        dvr = (double)(2 * Z) / (r[0] * r[0]);
+      else if (i0 == n - 1)
+        dvr = cgm[i0] * V[i0 - 1] + cg0[i0] * V[i0] + cgp[i0] * V[i0];
+      else
+        dvr = cgm[i0] * V[i0 - 1] + cg0[i0] * V[i0] + cgp[i0] * V[i0 + 1];
+      const double fac = kA2s4 * bder * dvr;
+      g0 = g0 - kA2s4 * (vme * vme) - fac * (cg0[i0] - 1 / r[i0]);
+      gm = gm - fac * cgm[i0];
+      gp = gp - fac * cgp[i0];
+    }
+    const double td = g0 + L2 * f2[i0], gmi = 1 / gm;
+    double x = td * sr[i0] + gp * sr[im], y = td * si[i0] + gp * si[im];
+    if (ecomp) {  // - i Eimag us(ir)
+      x = x + Ei * si[i0];
+      y = y - Ei * sr[i0];
+    }
+    sr[ip] = -x * gmi;
+    si[ip] = -y * gmi;
+  }
+  for (int i = 0; i < n; i++) {
+    sr[i] = sr[i] / r[i];
+    si[i] = si[i] / r[i];
+  }
 }
 
 // One thread per (point, border point of all atoms, Lm): phiato by
 // quadratic interpolation of the renormalised solution of l(Lm) at the
 // point's distance, times its real harmonic (cal_phiato, :3224-3345)
 __global__ void k_sph_phiato(SphTabs T, SphPoints P, int lmax_all, int nlm_max, const double *__restrict__ u,
-                             int64_t ustride, int cplx) {
+                             int64_t ustride, int cplx, const double *__restrict__ u2) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x, g = blockIdx.y;
   if (t >= T.nball * nlm_max) return;
   const int q = t / nlm_max, Lm = t % nlm_max;
@@ -216,6 +301,18 @@ __global__ void k_sph_phiato(SphTabs T, SphPoints P, int lmax_all, int nlm_max, 
     P.phi[g][2 * o + 1] = phi * y;
   } else {
     P.phi[g][o] = ph * y;
+  }
+  if (P.phi2[g]) {  // the irregular solution (i_RI = 2); its imaginary part only for complex radial solutions
+    const bool ecomp = fabs(P.eimag[g]) > 1e-10 || P.ecin[g] < 1e-10;
+    const double *sr = u2 + (((int64_t)g * T.natome + a) * (lmax_all + 1) + L) * ustride, *si = sr + T.nrmax;
+    const double p2 = interp(sr[i - 3], sr[i - 2], sr[i - 1]);
+    const double p2i = ecomp ? interp(si[i - 3], si[i - 2], si[i - 1]) : 0.0;
+    if (cplx) {
+      P.phi2[g][2 * o] = p2 * y;
+      P.phi2[g][2 * o + 1] = p2i * y;
+    } else {
+      P.phi2[g][o] = p2 * y;
+    }
   }
 }
 
@@ -331,7 +428,7 @@ SphereSolver::~SphereSolver() {
   for (void *p : {(void *)d.r, (void *)d.V, (void *)d.clm, (void *)d.clp, (void *)d.cl0, (void *)d.cgm, (void *)d.cgp,
                   (void *)d.cg0, (void *)d.f2, (void *)d.roff, (void *)d.nr, (void *)d.Z, (void *)d.lmax,
                   (void *)d.rmtg, (void *)d.inr, (void *)d.toff, (void *)d.batom, (void *)d.bint, (void *)d.rrel,
-                  (void *)d.yoff, (void *)d.ylmato, (void *)u, (void *)d_psii})
+                  (void *)d.yoff, (void *)d.ylmato, (void *)u, (void *)d_psii, (void *)u2})
     if (p) (void)hipFree(p);
 }
 
@@ -389,6 +486,7 @@ int SphereSolver::init(const fdmgpu_sphere_tables *t, int maxp, hipStream_t s) {
     for (nrmtg = 1; nrmtg <= n - 1; nrmtg++)
       if (r[nrmtg - 1] > t->rmtg[a] + 1e-10) break;
     inr[a] = std::min(nrmtg, n);
+    h_inr.push_back(inr[a]);
     if (inr[a] < 3) {
       set_error("fdmgpu_sphere_create: rmtg of atom " + std::to_string(a) + " below the third radial point");
       return FDMGPU_EINVAL;
@@ -495,14 +593,31 @@ int SphereSolver::run(int count, const SphPoints &P, bool cplx) {
         return FDMGPU_EINVAL;
       }
   }
+  bool want_irr = false;
+  for (int k = 0; k < count; k++) want_irr = want_irr || P.phi2[k] != nullptr;
+  if (want_irr) {
+    for (int k = 0; k < count; k++)
+      if (!P.phi2[k] && nphi > 0) {
+        set_error("fdmgpu_sphere_run_ri: null irregular phiato buffer for point " + std::to_string(k));
+        return FDMGPU_EINVAL;
+      }
+    for (int a = 0; a < natome; a++)
+      if (h_inr[a] > h_nr[a] - 1) {
+        set_error("fdmgpu_sphere_run_ri: atom " + std::to_string(a) +
+                  " needs a radial point beyond its matching point for the irregular solution");
+        return FDMGPU_EINVAL;
+      }
+    if (!u2) FDM_HIP(hipMalloc(&u2, sizeof(double) * (size_t)max_points * natome * (lmax_all + 1) * ustride));
+  }
   const int NT = 64;
   const int nth = natome * (lmax_all + 1);
-  hipLaunchKernelGGL(k_sph_radial, dim3((nth + NT - 1) / NT, count), dim3(NT), 0, stream, d, P, lmax_all, u, ustride);
+  hipLaunchKernelGGL(k_sph_radial, dim3((nth + NT - 1) / NT, count), dim3(NT), 0, stream, d, P, lmax_all, u, ustride,
+                     u2);
   const int nlm_max = (lmax_all + 1) * (lmax_all + 1);
   const int64_t nph = (int64_t)nball * nlm_max;
   if (nph > 0)
     hipLaunchKernelGGL(k_sph_phiato, dim3((unsigned)((nph + 255) / 256), count), dim3(256), 0, stream, d, P, lmax_all,
-                       nlm_max, u, ustride, (int)cplx);
+                       nlm_max, u, ustride, (int)cplx, (const double *)u2);
   if (want_rof)
     hipLaunchKernelGGL(k_sph_rof, dim3((unsigned)nrof(), count), dim3(64), 0, stream, d, P, rt, lmax_all, u, ustride);
   FDM_HIP(hipGetLastError());

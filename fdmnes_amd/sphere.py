@@ -32,6 +32,8 @@ def _sigs(L):
     L.fdmgpu_sphere_destroy.argtypes = [_V]
     L.fdmgpu_sphere_destroy.restype = None
     L.fdmgpu_sphere_run.argtypes = [_V, C.c_int, P(C.c_double), P(C.c_double), P(C.c_double), C.c_int, P(_V), P(_V)]
+    L.fdmgpu_sphere_run_ri.argtypes = [_V, C.c_int, P(C.c_double), P(C.c_double), P(C.c_double), C.c_int, P(_V),
+                                       P(_V), P(_V)]
     L.fdmgpu_sphere_set_integrals.argtypes = [_V, C.c_int, C.c_int, P(C.c_double), P(C.c_double), C.c_double,
                                               C.c_int, C.c_int]
     L.fdmgpu_sphere_run_integrals.argtypes = [_V, C.c_int, P(C.c_double), P(C.c_double), P(C.c_double),
@@ -104,6 +106,34 @@ class DeviceSphere:
         self.rof_layout = dict(atom=int(atom), nbseuil=psii.shape[0], ip0=int(ip0), ip_max=int(ip_max),
                                lmax=int(self.lmax[atom]))
         self.nrof = psii.shape[0] * (ip_max - ip0 + 1) * (self.lmax[atom] + 1)
+
+    def run_ri(self, energies, complex_out: bool = True):
+        """The regular and the irregular solutions' phiato (Sch_radial_solsing,
+        sphere.f90:2591): per point two device tensors (phi, phi2) in the
+        layout of ``run``, for the irregular cluster system."""
+        n = len(energies)
+        dt = torch.complex128 if complex_out else torch.float64
+        phi = [torch.empty(self.nphi, dtype=dt, device="cuda") for _ in range(n)]
+        phi2 = [torch.empty(self.nphi, dtype=dt, device="cuda") for _ in range(n)]
+        E = np.array(energies, np.float64).reshape(n, 3)
+        D = C.POINTER(C.c_double)
+
+        def ptrs(ts):
+            arr = (_V * len(ts))()
+            for j, t in enumerate(ts):
+                arr[j] = t.data_ptr()
+            return arr
+        rc = 0
+        with Ordered(self.stream):
+            for k0 in range(0, n, self.max_points):
+                k1 = min(n, k0 + self.max_points)
+                cols = [np.ascontiguousarray(E[k0:k1, j]) for j in range(3)]
+                rc = self.L.fdmgpu_sphere_run_ri(self._h, k1 - k0, *[c.ctypes.data_as(D) for c in cols],
+                                                 int(bool(complex_out)), ptrs(phi[k0:k1]), ptrs(phi2[k0:k1]), None)
+                if rc:
+                    break
+        _lib.check(rc)
+        return list(zip(phi, phi2))
 
     def run(self, energies, complex_out: bool = True, with_tau: bool = False, energ=None):
         """energies: list of (Ecinetic, Eimag, Enervide) in Ry.  Returns per

@@ -404,6 +404,16 @@ int fdmgpu_sphere_run(fdmgpu_sphere *s, int count, const double *Ecinetic, const
  * device array d_rof[k] of nbseuil * (ip_max - ip0 + 1) * (lmax + 1) complex
  * values (interleaved), index (is * nip + ip - ip0) * (lmax + 1) + L.
  */
+/* fdmgpu_sphere_run with the irregular solution as well (Sch_radial_solsing,
+ * sphere.f90:2591, as Sphere calls it with n_RI = 2, :152-162): -i r h_L(k r)
+ * from the matching point outwards, integrated inwards by the same
+ * three-point recurrence; d_phi2[k] its values at the border points in
+ * d_phi's layout (the imaginary part zero for real radial solutions), the
+ * phiato of the irregular cluster system (fdmgpu_cluster_assemble_atoms_ri's
+ * phiato2).  Every atom needs a radial point beyond its matching point. */
+int fdmgpu_sphere_run_ri(fdmgpu_sphere *s, int count, const double *Ecinetic, const double *Eimag,
+                         const double *Enervide, int complex_out, double *const *d_phi, double *const *d_phi2,
+                         double *const *d_tau);
 int fdmgpu_sphere_set_integrals(fdmgpu_sphere *s, int atom, int nbseuil, const double *psii, const double *Eseuil,
                                 double Rmtsd, int ip0, int ip_max);
 int fdmgpu_sphere_run_integrals(fdmgpu_sphere *s, int count, const double *Ecinetic, const double *Eimag,

@@ -1772,7 +1772,8 @@ static double orc_interp2(double r, double rm, double r0, double rp, double fm, 
 static int sphere_impl(int nr, const double *r, const double *Vr, int numat, double rmtg, int lmax, int relativiste,
                        double Ecinetic, double Eimag, double Enervide, double V_intmax, int nbord, const double *bxyz,
                        const double *pos, const double *ylmato, int nlm, double *phi_re, double *phi_im,
-                       double *tau_re, double *tau_im, double *u_re, double *u_im) {
+                       double *tau_re, double *tau_im, double *u_re, double *u_im, double *phi2_re,
+                       double *phi2_im) {
   if (nr < 4 || lmax < 0 || lmax > 60 || nlm < (lmax + 1) * (lmax + 1)) return -1;
   const int n = nr;
   double *V = malloc(sizeof(double) * n), *f2 = malloc(sizeof(double) * n), *g0 = malloc(sizeof(double) * n),
@@ -1843,7 +1844,14 @@ static int sphere_impl(int nr, const double *r, const double *Vr, int numat, dou
   if (nbord > 0) {
     memset(phi_re, 0, sizeof(double) * (size_t)nbord * nlm);
     memset(phi_im, 0, sizeof(double) * (size_t)nbord * nlm);
+    if (phi2_re) {
+      memset(phi2_re, 0, sizeof(double) * (size_t)nbord * nlm);
+      memset(phi2_im, 0, sizeof(double) * (size_t)nbord * nlm);
+    }
   }
+  if (phi2_re && nrmtg > n - 1) return -1; /* the irregular solution starts one point beyond the matching point */
+  double complex *us = phi2_re ? malloc(sizeof(double complex) * n) : NULL;
+  double *hb = malloc(sizeof(double) * 2 * (lmax + 1)), *hn = malloc(sizeof(double) * 2 * (lmax + 1));
   for (int L = 0; L <= lmax; L++) {
     /* Sch_radial, sphere.f90:413-730 (one spin, no spin-orbit, no Hubbard) */
     const int L2 = L * (L + 1);
@@ -1901,6 +1909,31 @@ static int sphere_impl(int nr, const double *r, const double *Vr, int numat, dou
         u_re[(size_t)L * n + i] = ur[i];
         u_im[(size_t)L * n + i] = ui[i];
       }
+    /* the irregular solution (Sch_radial_solsing, sphere.f90:2591-2806, Classic_irreg, as Sphere calls it
+     * at :155-162): -i r h_L(k r) from the matching point outwards (cbessneu / cbessneur), integrated
+     * inwards by the same three-point recurrence with the roles of gm and gp exchanged, then over r */
+    if (us) {
+      for (int ir = nrmtg; ir <= n; ir++) {
+        double complex h;
+        if (ecomp) {
+          const double complex fnc = csqrt(konde / PI), z = konde * r[ir - 1];
+          orc_cbessneu(creal(fnc), cimag(fnc), creal(z), cimag(z), lmax, hb, hn);
+          h = CMPLX(hb[2 * L], hb[2 * L + 1]) + I * CMPLX(hn[2 * L], hn[2 * L + 1]);
+        } else {
+          const double kr = creal(konde), fn = sqrt(kr / PI);
+          orc_cbessneur(fn, kr * r[ir - 1], lmax, hb, hn);
+          h = hb[L] + I * hn[L];
+        }
+        us[ir - 1] = -I * r[ir - 1] * h;
+      }
+      for (int ir = nrmtg; ir >= 2; ir--) {
+        const int ip = ir - 1, im = ir + 1;
+        const double td = g0[ir - 1] + L2 * f2[ir - 1];
+        double complex v = td * us[ir - 1] + gp[ir - 1] * us[im - 1];
+        if (ecomp) v = v - I * Eimag * us[ir - 1];
+        us[ip - 1] = -v * (1 / gm[ir - 1]);
+      }
+    }
     /* cal_phiato, sphere.f90:3186-3345 */
     const int Lm0 = L * L + L + 1;
     for (int ib = 0; ib < nbord; ib++) {
@@ -1914,8 +1947,23 @@ static int sphere_impl(int nr, const double *r, const double *Vr, int numat, dou
         phi_re[(size_t)ib * nlm + Lm - 1] = phi_re[(size_t)ib * nlm + Lm - 1] + ph * y;
         if (ecomp) phi_im[(size_t)ib * nlm + Lm - 1] = phi_im[(size_t)ib * nlm + Lm - 1] + phi * y;
       }
+      if (us) { /* i_RI = 2: the irregular solution over r (its imaginary part only with Radial_comp) */
+        const double sm = creal(us[i - 3]) / r[i - 3], s0 = creal(us[i - 2]) / r[i - 2], sp = creal(us[i - 1]) / r[i - 1];
+        const double p2 = orc_interp2(rrel[ib], rm, r0, rp, sm, s0, sp);
+        double p2i = 0;
+        if (ecomp)
+          p2i = orc_interp2(rrel[ib], rm, r0, rp, cimag(us[i - 3]) / r[i - 3], cimag(us[i - 2]) / r[i - 2],
+                            cimag(us[i - 1]) / r[i - 1]);
+        for (int mpp = -L; mpp <= L; mpp++) {
+          const size_t o = (size_t)ib * nlm + Lm0 + mpp - 1;
+          const double y = ylmato[o];
+          phi2_re[o] = phi2_re[o] + p2 * y;
+          if (ecomp) phi2_im[o] = phi2_im[o] + p2i * y;
+        }
+      }
     }
   }
+  free(us); free(hb); free(hn);
   free(V); free(f2); free(g0); free(gm); free(gp); free(gpi); free(cgm); free(cgp); free(cg0); free(ur); free(ui);
   free(rrel); free(iint);
   return ecomp;
@@ -1926,7 +1974,16 @@ int orc_sphere(int nr, const double *r, const double *Vr, int numat, double rmtg
                const double *pos, const double *ylmato, int nlm, double *phi_re, double *phi_im, double *tau_re,
                double *tau_im) {
   return sphere_impl(nr, r, Vr, numat, rmtg, lmax, relativiste, Ecinetic, Eimag, Enervide, V_intmax, nbord, bxyz, pos,
-                     ylmato, nlm, phi_re, phi_im, tau_re, tau_im, NULL, NULL);
+                     ylmato, nlm, phi_re, phi_im, tau_re, tau_im, NULL, NULL, NULL, NULL);
+}
+
+int orc_sphere_ri(int nr, const double *r, const double *Vr, int numat, double rmtg, int lmax, int relativiste,
+                  double Ecinetic, double Eimag, double Enervide, double V_intmax, int nbord, const double *bxyz,
+                  const double *pos, const double *ylmato, int nlm, double *phi_re, double *phi_im, double *phi2_re,
+                  double *phi2_im, double *tau_re, double *tau_im) {
+  if (!phi2_re || !phi2_im) return -1;
+  return sphere_impl(nr, r, Vr, numat, rmtg, lmax, relativiste, Ecinetic, Eimag, Enervide, V_intmax, nbord, bxyz, pos,
+                     ylmato, nlm, phi_re, phi_im, tau_re, tau_im, NULL, NULL, phi2_re, phi2_im);
 }
 
 /* ------------------------------------------------------------------------
@@ -2124,7 +2181,7 @@ int orc_radial_integral(int nr, const double *r, const double *Vr, int numat, do
   double *ur = malloc(sizeof(double) * (size_t)nr * (lmax + 1)), *ui = malloc(sizeof(double) * (size_t)nr * (lmax + 1));
   double *fct = malloc(sizeof(double) * nr), *rb = malloc(sizeof(double) * nr);
   const int ecomp = sphere_impl(nr, r, Vr, numat, rmtg, lmax, relativiste, Ecinetic, Eimag, Enervide, V_intmax, 0,
-                                NULL, NULL, NULL, (lmax + 1) * (lmax + 1), NULL, NULL, NULL, NULL, ur, ui);
+                                NULL, NULL, NULL, (lmax + 1) * (lmax + 1), NULL, NULL, NULL, NULL, ur, ui, NULL, NULL);
   if (ecomp < 0) return -1;
   const int nip = ip_max - ip0 + 1;
   double vecond[2];

@@ -183,6 +183,14 @@ int orc_sphere(int nr, const double *r, const double *Vr, int numat, double rmtg
                double Ecinetic, double Eimag, double Enervide, double V_intmax, int nbord, const double *bxyz,
                const double *pos, const double *ylmato, int nlm, double *phi_re, double *phi_im, double *tau_re,
                double *tau_im);
+/* orc_sphere with the irregular solution as well (Sch_radial_solsing,
+ * sphere.f90:2591, Sphere :152-162 with n_RI = 2): phi2_re/phi2_im its
+ * values at the border points (the imaginary part only for complex radial
+ * solutions).  Needs the matching point inside the grid's last interval. */
+int orc_sphere_ri(int nr, const double *r, const double *Vr, int numat, double rmtg, int lmax, int relativiste,
+                  double Ecinetic, double Eimag, double Enervide, double V_intmax, int nbord, const double *bxyz,
+                  const double *pos, const double *ylmato, int nlm, double *phi_re, double *phi_im, double *phi2_re,
+                  double *phi2_im, double *tau_re, double *tau_im);
 /* Spherical Bessel/Neumann recurrences (sphere.f90:7226-7248, 7356-7378). */
 void orc_cbessneu(double fn_re, double fn_im, double z_re, double z_im, int lmax,
                   double *bes, double *neu);

@@ -54,6 +54,9 @@ def lib():
                                  C.c_double, C.c_double, C.c_int, _dp, _dp, _dp, C.c_int, _dp, _dp, C.c_void_p,
                                  C.c_void_p]
         L.orc_cluster_free.argtypes = [C.c_void_p]
+        L.orc_sphere_ri.argtypes = [C.c_int, _dp, _dp, C.c_int, C.c_double, C.c_int, C.c_int, C.c_double,
+                                    C.c_double, C.c_double, C.c_double, C.c_int, _dp, _dp, _dp, C.c_int, _dp, _dp,
+                                    _dp, _dp, C.c_void_p, C.c_void_p]
         L.orc_f_integr3.argtypes = [_dp, _dp, C.c_int, C.c_double]
         L.orc_f_integr3.restype = C.c_double
         L.orc_coef_init.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _dp, _ip, _ip]
@@ -422,6 +425,23 @@ def sphere(r, Vr, numat, rmtg, lmax, relativiste, Ecinetic, Eimag, Enervide, bxy
     if rc < 0:
         raise ValueError("orc_sphere: bad argument")
     return (pr + 1j * pi).reshape(nb, nlm), tr + 1j * ti, bool(rc)
+
+
+def sphere_ri(r, Vr, numat, rmtg, lmax, relativiste, Ecinetic, Eimag, Enervide, bxyz, pos, ylmato, V_intmax=1.0e6):
+    """orc_sphere_ri: as sphere, with the irregular solution's phiato:
+    (phi, phi2, tau, complex flag)."""
+    ylm = np.ascontiguousarray(ylmato, np.float64)
+    nb, nlm = ylm.shape
+    pr, pi, qr, qi = (np.zeros(nb * nlm) for _ in range(4))
+    tr, ti = np.zeros(lmax + 1), np.zeros(lmax + 1)
+    rc = lib().orc_sphere_ri(len(r), np.ascontiguousarray(r, np.float64), np.ascontiguousarray(Vr, np.float64),
+                             int(numat), float(rmtg), int(lmax), int(bool(relativiste)), float(Ecinetic), float(Eimag),
+                             float(Enervide), float(V_intmax), nb, np.ascontiguousarray(bxyz, np.float64).reshape(-1),
+                             np.ascontiguousarray(pos, np.float64), ylm.reshape(-1), nlm, pr, pi, qr, qi, _ptr(tr),
+                             _ptr(ti))
+    if rc < 0:
+        raise ValueError("orc_sphere_ri: bad argument")
+    return (pr + 1j * pi).reshape(nb, nlm), (qr + 1j * qi).reshape(nb, nlm), tr + 1j * ti, bool(rc)
 
 
 def tau_sym(taus, ylm_comp: bool = False):

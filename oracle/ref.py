@@ -614,21 +614,22 @@ def mat_solve_so(s, V, Enervide, Eimag, bes, neu, E_comp, basereel, phiato, grad
     return smr.T + 1j * smi.T
 
 
-def sphere(s, iapr, r, Vr, numat, rmtg, lmax, relativiste, Ecinetic, Eimag, Enervide, V_intmax=1.0e6):
+def sphere(s, iapr, r, Vr, numat, rmtg, lmax, relativiste, Ecinetic, Eimag, Enervide, V_intmax=1.0e6, n_ri=1):
     """The reference's Sphere (sphere.f90:8) for the atoms of one prototype
     of an atom_system dict (every atom its own prototype: atom iapr, 1-based),
     one spin, spherical potential Vr(r) (Ry) on the radial grid r (bohr),
     atomic number numat, matching radius rmtg, no spin-orbit or Hubbard
     term, the regular solution, Renorm = T, real harmonics.  Returns
     (phiato of the atom, (nbord, (lmax+1)^2) complex, Tau_ato of the atom,
-    ((lmax+1)^2,) diagonal)."""
+    ((lmax+1)^2,) diagonal); with n_ri = 2 the irregular solution's phiato
+    (Sch_radial_solsing, sphere.f90:152-162) comes second."""
     A = _Args()
     p = lambda a: a.ctypes.data_as(C.c_void_p)
     natome, nbtm = s["natome"], s["nbtm"]
     nlmagm = (lmax + 1) ** 2
     nlmmax = s["nlmmax"]
     nr = len(r)
-    phiato = np.zeros((nbtm, nlmagm, 1, 1, 1, natome, 2), order="F")
+    phiato = np.zeros((nbtm, nlmagm, 1, 1, n_ri, natome, 2), order="F")
     tau = np.zeros((nlmagm, 1, nlmagm, 1, natome), np.complex128, order="F")
     iopsymr = np.zeros(NOPSM, np.int32)
     iopsymr[0] = 1
@@ -637,7 +638,7 @@ def sphere(s, iapr, r, Vr, numat, rmtg, lmax, relativiste, Ecinetic, Eimag, Ener
         A.b(False), A.b(False), A.b(False), A.b(False), A.b(False), A.i(1), A.i(iapr),
         A.arr(np.arange(1, natome + 1), np.int32), A.arr(s["ibord"], np.int32), A.i(0),
         A.arr(np.zeros((natome + 1, 1), np.int32), np.int32), A.arr(np.ones(natome, np.int32), np.int32),
-        A.arr(iopsymr, np.int32), A.i(lmax), A.i(0), A.i(0), A.i(1), A.i(natome), A.i(natome), A.i(1), A.i(natome),
+        A.arr(iopsymr, np.int32), A.i(lmax), A.i(0), A.i(0), A.i(1), A.i(natome), A.i(natome), A.i(n_ri), A.i(natome),
         A.arr(s["nbord"], np.int32), A.i(nbtm), A.i(nbtm), A.i(1), A.i(1), A.i(1), A.i(nlmagm), A.i(nlmmax), A.i(nbtm),
         A.i(2), A.i(s["npsom"]), A.i(nr), A.i(1), A.i(1), A.i(1), A.i(numat), p(phiato), A.arr(s["posi"], np.float64),
         A.arr(np.asarray(r, np.float64), np.float64), A.b(relativiste), A.b(True), A.d(rmtg), A.d(rmtg), A.b(False),
@@ -650,7 +651,10 @@ def sphere(s, iapr, r, Vr, numat, rmtg, lmax, relativiste, Ecinetic, Eimag, Ener
     ia = iapr - 1
     nb = int(s["nbord"][ia])
     phi = phiato[:nb, :, 0, 0, 0, ia, 0] + 1j * phiato[:nb, :, 0, 0, 0, ia, 1]
-    return phi, np.array([tau[k, 0, k, 0, ia] for k in range(nlmagm)])
+    taus = np.array([tau[k, 0, k, 0, ia] for k in range(nlmagm)])
+    if n_ri == 2:
+        return phi, phiato[:nb, :, 0, 0, 1, ia, 0] + 1j * phiato[:nb, :, 0, 0, 1, ia, 1], taus
+    return phi, taus
 
 
 def _phi_array(s, phiato, phiato2=None):

@@ -260,7 +260,8 @@ SPHERE_CASES = [(0.7, 0.05, False), (0.9, 0.0, False), (0.7, 0.05, True), (0.9, 
 def sphere_fixture():
     """The reference's Sphere (sphere.f90:8) for each atom of the three-atom
     cluster: phiato at its border points and the atomic Tau, complex and
-    real energy, with and without the scalar-relativistic terms."""
+    real energy, with and without the scalar-relativistic terms; and with
+    n_RI = 2 the irregular solution's phiato."""
     a = ATOMS
     posi = np.array(a["posi"]) / BOHR
     s = R.atom_system(a["R"] / BOHR, a["h"] / BOHR, a["lmaxso"], posi, np.array(a["rmt"]) / BOHR, a["lmax"])
@@ -271,6 +272,9 @@ def sphere_fixture():
         for c, (Ec, Ei, rel) in enumerate(SPHERE_CASES):
             phi, tau = R.sphere(s, ia + 1, r, V, Z, a["rmt"][ia] / BOHR, int(s["lmax"][ia]), rel, Ec, Ei, Ec + 0.3)
             out[f"phi{ia}_{c}"], out[f"tau{ia}_{c}"] = phi, tau
+            # the irregular solution's phiato (n_RI = 2)
+            _, out[f"irr{ia}_{c}"], _ = R.sphere(s, ia + 1, r, V, Z, a["rmt"][ia] / BOHR, int(s["lmax"][ia]), rel,
+                                                 Ec, Ei, Ec + 0.3, n_ri=2)
     np.savez_compressed(os.path.join(HERE, "sphere_R1.9_h0.25.npz"), **out)
     print("sphere", [len(out[f"r{ia}"]) for ia in range(s["natome"])])
 

@@ -136,3 +136,60 @@ def test_energy_loop_with_atoms_and_device_sphere(precision):
         rt = max(np.abs(a - b).max() / np.abs(b).max() for a, b in zip(tg, to))
         print(f"precision {precision} energy {k}: solution rel {rx:.2e}, tau rel {rt:.2e}", flush=True)
         assert rx <= 1e-10 and rt <= 1e-9
+
+
+@pytest.mark.parametrize("rel", [False, True])
+def test_device_irregular_solution_matches_oracle(rel):
+    """fdmgpu_sphere_run_ri: the regular and the irregular solutions'
+    phiato against the oracle's Sphere with n_RI = 2 (pinned to the
+    reference's, tests/test_oracle_sphere.py) within 1e-12."""
+    import torch
+    from fdmnes_amd.sphere import DeviceSphere
+    g, o = _cluster()
+    f = np.load(SPHERE)
+    atoms = _atoms(g, o, f)
+    print(f"device irregular Sphere: relativistic {rel}, {len(ENERGIES)} energies", flush=True)
+    ds = DeviceSphere(atoms, relativiste=rel, max_points=3)
+    out = ds.run_ri(ENERGIES, complex_out=True)
+    torch.cuda.synchronize()
+    for k, (Ec, Ei, Ev) in enumerate(ENERGIES):
+        o1, o2 = [], []
+        for a in atoms:
+            p, p2, _, _ = O.sphere_ri(a["r"], a["V"], a["Z"], a["rmtg"], a["lmax"], rel, Ec, Ei, Ev, a["bxyz"],
+                                      a["pos"], a["ylmato"])
+            o1.append(p.reshape(-1))
+            o2.append(p2.reshape(-1))
+        o1, o2 = np.concatenate(o1), np.concatenate(o2)
+        d1, d2 = out[k][0].cpu().numpy(), out[k][1].cpu().numpy()
+        r1 = np.abs(d1 - o1).max() / np.abs(o1).max()
+        r2 = np.abs(d2 - o2).max() / np.abs(o2).max()
+        print(f"energy {k}: regular rel {r1:.2e}, irregular rel {r2:.2e}", flush=True)
+        assert r1 <= 1e-12 and r2 <= 1e-12
+
+
+def test_device_irregular_phiato_assembles_the_irregular_system():
+    """The device's regular and irregular phiato, given to the host
+    assembler of the irregular cluster system (calcMatRow's irregular
+    branches), write the rows the oracle's Sphere values write (1e-12)."""
+    import torch
+    from fdmnes_amd import cluster as CL
+    from fdmnes_amd.sphere import DeviceSphere
+    g, o = _cluster()
+    f = np.load(SPHERE)
+    atoms = _atoms(g, o, f)
+    at = (g["posi"] / BOHR, g["rmt"] / BOHR, g["lmax"])
+    ci = CL.Cluster(float(g["R"]) / BOHR, float(g["h"]) / BOHR, int(g["lmaxso"]), atoms=at, irregular=True)
+    Ev, Ei, Ec = 1.0, 0.05, 0.7
+    out = DeviceSphere(atoms).run_ri([(Ec, Ei, Ev)])
+    torch.cuda.synchronize()
+    d1, d2 = out[0][0].cpu().numpy(), out[0][1].cpu().numpy()
+    blocks = lambda flat: [flat[s:s + len(a["bxyz"]) * (a["lmax"] + 1) ** 2].reshape(len(a["bxyz"]), -1)
+                           for s, a in zip(np.cumsum([0] + [len(a["bxyz"]) * (a["lmax"] + 1) ** 2
+                                                            for a in atoms])[:-1], atoms)]
+    op = [O.sphere_ri(a["r"], a["V"], a["Z"], a["rmtg"], a["lmax"], False, Ec, Ei, Ev, a["bxyz"], a["pos"],
+                      a["ylmato"]) for a in atoms]
+    V = g["V"][: ci.npoint]
+    _, _, vd, rd = ci.assemble_atoms(V, Ev, Ei, Ec, True, False, blocks(d1), blocks(d2))
+    _, _, vo, ro = ci.assemble_atoms(V, Ev, Ei, Ec, True, False, [p[0] for p in op], [p[1] for p in op])
+    assert np.abs(vd - vo).max() <= 1e-12 * np.abs(vo).max()
+    assert np.abs(rd - ro).max() <= 1e-12 * max(np.abs(ro).max(), 1e-300)

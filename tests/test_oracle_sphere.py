@@ -80,3 +80,51 @@ def test_constant_potential_closed_forms():
         ws = j * (-1j * dhk) - dj * (-1j * hk)
         t = -we / ws
         assert abs(tau[L] - t) <= 2e-3 * abs(t), (L, tau[L], t)
+
+
+@pytest.mark.parametrize("case", range(4))
+@pytest.mark.parametrize("ia", range(3))
+def test_oracle_irregular_solution_matches_reference(ia, case):
+    """The irregular solution's phiato (Sch_radial_solsing, sphere.f90:2591,
+    Sphere with n_RI = 2) against the reference's own Sphere within 1e-12;
+    the regular phiato of the same call unchanged."""
+    g, o = _cluster()
+    f = np.load(SPHERE)
+    Ec, Ei, rel = CASES[case]
+    bx, pos, ylm = border(o, g, ia)
+    lmax = int(g["lmax"][ia])
+    phi, phi2, tau, cx = O.sphere_ri(f[f"r{ia}"], f[f"V{ia}"], int(f[f"Z{ia}"]), g["rmt"][ia] / BOHR, lmax, rel, Ec,
+                                     Ei, Ec + 0.3, bx, pos, ylm)
+    ref = f[f"irr{ia}_{case}"]
+    assert np.abs(phi2 - ref).max() <= 1e-12 * np.abs(ref).max()
+    assert np.abs(phi - f[f"phi{ia}_{case}"]).max() <= 1e-12 * np.abs(f[f"phi{ia}_{case}"]).max()
+    if Ei == 0:  # real radial solutions: the irregular phiato keeps only its real part
+        assert np.all(phi2.imag == 0)
+
+
+def test_irregular_solution_in_a_constant_potential_is_the_outgoing_wave():
+    """With the same kinetic energy inside the sphere as outside (V0 =
+    Enervide - Ecinetic) the inward integration continues -i r h_L(k r):
+    phiato2 / Ylm follows -i fnorm h_L(k r) with fnorm = sqrt(k / pi), to the
+    radial grid's discretisation."""
+    from scipy.special import spherical_jn, spherical_yn
+    g, o = _cluster()
+    ia = 0
+    bx, pos, ylm = border(o, g, ia)
+    rel = np.linalg.norm(bx - pos, axis=1)
+    r = [1e-4]
+    while r[-1] < 1.2 * rel.max():
+        r.append(r[-1] * 1.002)
+    r = np.array(r)
+    Ec, Ei, Ev = 0.7, 0.05, 0.4
+    V0 = Ev - Ec
+    rmt, lmax = g["rmt"][ia] / BOHR, 2
+    _, phi2, _, cx = O.sphere_ri(r, np.full(len(r), V0), 0, rmt, lmax, False, Ec, Ei, Ev, bx, pos, ylm)
+    assert cx
+    k = np.sqrt(Ec + 1j * Ei)
+    fn = np.sqrt(k / np.pi)
+    for L in range(lmax + 1):
+        sel = (np.abs(ylm[:, L * L + L]) > 0.2) & (rel > 0.5 * rmt)
+        want = -1j * fn * (spherical_jn(L, k * rel[sel]) + 1j * spherical_yn(L, k * rel[sel]))
+        got = phi2[sel, L * L + L] / ylm[sel, L * L + L]
+        assert np.abs(got - want).max() <= 2e-4 * np.abs(want).max(), L
