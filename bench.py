@@ -521,6 +521,75 @@ def mmm_rates(w, G):
     return res
 
 
+def atoms_rates(w, G):
+    """The energy loop with muffin-tin atoms on the benchmark's grid: the
+    Fe sites within 3 A of the absorber as atoms (rmt 1.0 A, lmax 2,
+    screened-Coulomb spherical potentials), per group of G energies the
+    device radial solver (regular solution, phiato, the absorber's radial
+    integrals), device assembly, the mixed-precision grouped solve, tau
+    and the absorber's E1E1 / E1E2 / E2E2 tensors (K edge).  One warm-up
+    group, then two timed groups (wall clock around synchronised runs);
+    the radial solver and the tensor kernel also timed alone (HIP events,
+    20 calls of G points)."""
+    import torch
+    from fdmnes_amd import cluster as CL
+    from fdmnes_amd import configs as cfg
+    from fdmnes_amd.energy_loop import FP32, EnergyLoop
+    from fdmnes_amd.sphere import DeviceSphere, atoms_of
+    from fdmnes_amd.tensor import DeviceTensor, no_symmetry_tables
+    sites = cfg.lattice_sites(w.lattice, w.a, 3.0) / cfg.BOHR
+    rmt = np.full(len(sites), 1.0 / cfg.BOHR)
+    lmax = np.full(len(sites), 2, np.int32)
+    t0 = time.perf_counter()
+    cl = CL.Cluster(w.R / cfg.BOHR, w.h / cfg.BOHR, w.lmaxso, atoms=(sites, rmt, lmax))
+    t_build = time.perf_counter() - t0
+    lam = w.screening / cfg.BOHR
+    grids, pots = [], []
+    for a in range(len(sites)):
+        rr = [1e-4]
+        while rr[-1] < 1.3 * rmt[a] + 2 * w.h / cfg.BOHR:
+            rr.append(rr[-1] * 1.03)
+        rr = np.array(rr)
+        grids.append(rr)
+        pots.append(-2.0 * w.Z * np.exp(-rr / lam) / rr - 0.5)
+    sph = DeviceSphere(atoms_of(cl, grids, pots, [w.Z] * len(sites), rmt, sites))
+    psii = (grids[0] * np.exp(-w.Z * grids[0]))[None]
+    sph.set_integrals(0, psii, [7112.0 / cfg.RYDB], 1.1 * rmt[0])
+    nl0 = int((lmax[0] + 1) ** 2)
+    ten = DeviceTensor(1, 0, 1, 2, no_symmetry_tables(), sph.rof_layout, 0, nl0, numat=w.Z)
+    V = cfg.potential(w, cl.points)
+    E = [float(w.energies_Ry[(k * 7) % len(w.energies_Ry)]) for k in range(3 * G)]
+    ens = [(e, w.eimag_Ry, cfg.ecinetic_out(e)) for e in E]
+    loop = EnergyLoop(cl, precision=FP32, group=G, sphere=sph, tensor=ten)
+    loop.run(V, ens[:G], energ=E[:G])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = loop.run(V, ens[G:], energ=E[G:])
+    torch.cuda.synchronize()
+    t_loop = time.perf_counter() - t0
+    sens = [(e[2], e[1], e[0]) for e in ens[:G]]
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record(sph.stream)
+    for _ in range(20):
+        phi, rof = sph.run(sens, energ=E[:G])
+    ev1.record(sph.stream)
+    torch.cuda.synchronize()
+    t_sph = ev0.elapsed_time(ev1) / 1e3 / 20
+    tau = [torch.zeros(sum(int(cl.atom(a)["nlmsa"]) ** 2 for a in range(cl.natome)), dtype=torch.complex128,
+                       device="cuda") + 0.1j for _ in range(G)]
+    ev0.record(ten.stream)
+    for _ in range(20):
+        ten.run(rof, tau)
+    ev1.record(ten.stream)
+    torch.cuda.synchronize()
+    t_ten = ev0.elapsed_time(ev1) / 1e3 / 20
+    log(f"atoms: {cl.natome} atoms, {cl.nligne} rows, {2 * G} points in {t_loop:.2f} s")
+    return {"natome": int(cl.natome), "nligne": int(cl.nligne), "nlmso": int(cl.nlmso), "build_s": t_build,
+            "points": 2 * G, "points_per_s": 2 * G / t_loop, "sphere_us_per_point": 1e6 * t_sph / G,
+            "tensor_us_per_point": 1e6 * t_ten / G, "sweeps": [int(v) for v in out["sweeps"]],
+            "secdd_trace_first": float(np.trace(out["secdd"][0][0]).real)}
+
+
 def traffic_from_profile():
     """HBM bytes per k_trail launch measured by rocprofv3 PMC runs (FETCH_SIZE,
     WRITE_SIZE) of this benchmark, scaled by the same counters on a plain
@@ -576,6 +645,8 @@ def main():
     ap.add_argument("--no-graphs", action="store_true", help="launch eagerly instead of replaying HIP graphs")
     ap.add_argument("--overlap", type=int, default=None, choices=(0, 1),
                     help="per-step overlap of the trailing update (fdmgpu_solver_overlap); default: the library's")
+    ap.add_argument("--atoms", action="store_true",
+                    help="also time the energy loop with muffin-tin atoms, radial solver and tensors")
     ap.add_argument("--mmm", action="store_true",
                     help="also time the mirror-symmetric workload: whole cluster against its eight representations")
     ap.add_argument("--dropin", action="store_true",
@@ -715,6 +786,8 @@ def main():
         out.update(dropin_rates(cl, lb1, lb2, V, s, w))
     if args.mmm:
         out["mmm"] = mmm_rates(w, G)
+    if args.atoms:
+        out["atoms"] = atoms_rates(w, min(G, 6))
     cpu = None
     if not args.no_cpu:
         procs = args.cpu_procs or min(16, os.cpu_count() or 1)
