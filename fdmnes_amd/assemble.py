@@ -33,7 +33,7 @@ class Tables(C.Structure):
 class AtomTables(C.Structure):
     _fields_ = [("natome", C.c_int), ("nar", C.c_int), ("nal", C.c_int), ("nphi", C.c_int64),
                 ("aq", _V), ("aa", _V), ("aptr", _V), ("alphi", _V), ("alw", _V), ("ianew", _V), ("nlmsa", _V),
-                ("nbordf", _V), ("phoff", _V), ("bq", _V), ("poidsa", _V), ("ylmato", _V)]
+                ("nbordf", _V), ("phoff", _V), ("bq", _V), ("poidsa", _V), ("ylmato", _V), ("nri", C.c_int)]
 
 
 class RelTables(C.Structure):
@@ -46,6 +46,9 @@ class RelTables(C.Structure):
 def _sigs(L):
     if getattr(L, "_asm_sigs", False):
         return
+    Pd, Pv = C.POINTER(C.c_double), C.POINTER(_V)
+    L.fdmgpu_assembler_run_ri.argtypes = [_V, C.c_int, Pd, Pd, Pd, C.c_int, C.c_int, C.c_int, Pv, Pv, Pv, C.c_int, Pv,
+                                          Pv]
     P = C.POINTER
     L.fdmgpu_assembler_create.argtypes = [P(Tables), C.c_int, _V, P(_V)]
     L.fdmgpu_assembler_destroy.argtypes = [_V]
@@ -97,13 +100,15 @@ class DeviceAssembler:
         self.stream = library_stream(stream)
         h = _V()
         self.natome = int(getattr(cluster, "natome", 0))
+        self.irregular = False
         if self.natome:
             from .cluster import ATOM_TABLES
             at = cluster.atom_assembly_tables()
             self.nphi = at["nphi"]
             self._keep_at = {k: np.ascontiguousarray(at[k]) for k in ATOM_TABLES}
+            self.irregular = bool(getattr(cluster, "irregular", False))
             atb = AtomTables(at["natome"], at["nar"], at["nal"], at["nphi"],
-                             *[self._keep_at[k].ctypes.data_as(_V) for k in ATOM_TABLES])
+                             *[self._keep_at[k].ctypes.data_as(_V) for k in ATOM_TABLES], 2 if self.irregular else 1)
             _lib.check(L.fdmgpu_assembler_create_atoms(C.byref(tb), C.byref(atb), int(max_points),
                                                        _V(self.stream.cuda_stream), C.byref(h)))
         else:
@@ -135,7 +140,7 @@ class DeviceAssembler:
             self._h = None
 
     def run(self, V, energies, E_comp: bool, basereel: bool, vals, rhs=None, update_only: bool = False, phi=None,
-            G=None):
+            G=None, phi2=None):
         """V: list of float64 device tensors (npoint); energies: list of
         (Enervide, Eimag, Ecinetic_out) in Ry; vals (and rhs) lists of
         contiguous device tensors, complex128 when the arithmetic is complex
@@ -167,6 +172,11 @@ class DeviceAssembler:
                 raise ValueError("V: float64, npoint entries")
         if (phi is not None) != bool(self.natome):
             raise ValueError("phi is given exactly for a cluster with atoms")
+        if (phi2 is not None) != self.irregular:
+            raise ValueError("phi2 (the irregular phiato) is given exactly for the irregular cluster solution")
+        if phi2 is not None and (len(phi2) != n or any(t.numel() != self.nphi or t.dtype != phi[0].dtype
+                                                       for t in phi2)):
+            raise ValueError("phi2: one per energy point, like phi")
         if (G is not None) != self.relativistic:
             raise ValueError("G is given exactly for a relativistic assembler")
         if G is not None:
@@ -192,12 +202,19 @@ class DeviceAssembler:
         with Ordered(self.stream):
             for k0 in range(0, n, self.max_points):
                 k1 = min(n, k0 + self.max_points)
-                rc = self.L.fdmgpu_assembler_run_rel(
-                    self._h, k1 - k0, *[np.ascontiguousarray(a[k0:k1]).ctypes.data_as(D) for a in ev],
-                    int(bool(E_comp)), int(bool(basereel)), int(bool(update_only)), _dptrs(V[k0:k1]),
-                    _dptrs(G[k0:k1]) if G is not None else None,
-                    _dptrs(phi[k0:k1]) if phi is not None else None, int(phi_cplx), _dptrs(vals[k0:k1]),
-                    _dptrs(rhs[k0:k1]) if rhs is not None else None)
+                if self.irregular:
+                    rc = self.L.fdmgpu_assembler_run_ri(
+                        self._h, k1 - k0, *[np.ascontiguousarray(a[k0:k1]).ctypes.data_as(D) for a in ev],
+                        int(bool(E_comp)), int(bool(basereel)), int(bool(update_only)), _dptrs(V[k0:k1]),
+                        _dptrs(phi[k0:k1]), _dptrs(phi2[k0:k1]), int(phi_cplx), _dptrs(vals[k0:k1]),
+                        _dptrs(rhs[k0:k1]) if rhs is not None else None)
+                else:
+                    rc = self.L.fdmgpu_assembler_run_rel(
+                        self._h, k1 - k0, *[np.ascontiguousarray(a[k0:k1]).ctypes.data_as(D) for a in ev],
+                        int(bool(E_comp)), int(bool(basereel)), int(bool(update_only)), _dptrs(V[k0:k1]),
+                        _dptrs(G[k0:k1]) if G is not None else None,
+                        _dptrs(phi[k0:k1]) if phi is not None else None, int(phi_cplx), _dptrs(vals[k0:k1]),
+                        _dptrs(rhs[k0:k1]) if rhs is not None else None)
                 if rc:
                     break
         _lib.check(rc)

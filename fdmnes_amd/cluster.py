@@ -326,7 +326,11 @@ class Cluster:
             t["dself"] = t.pop("dself_c")
         if self.natome:
             phi = [np.zeros((self.atom(a)["nbord"], self.atom(a)["nlmsa"])) for a in range(self.natome)]
-            rp, _, _, _ = self.assemble_atoms(np.zeros(self.npoint), 0.0, 0.0, 1.0, False, True, phi)
+            if self.irregular:  # its rows exist with complex energy, in the complex basis, with both solutions
+                rp, _, _, _ = self.assemble_atoms(np.zeros(self.npoint), 0.0, 0.01, 1.0, True, False, phi,
+                                                  [np.zeros_like(p) for p in phi])
+            else:
+                rp, _, _, _ = self.assemble_atoms(np.zeros(self.npoint), 0.0, 0.0, 1.0, False, True, phi)
         else:
             rp, _, _, _ = self.assemble(np.zeros(self.npoint), 0.0, 0.0, 1.0, False, True)
         t["row_ptr"] = rp

@@ -609,10 +609,44 @@ int fdmgpu_assembler_run_atoms(fdmgpu_assembler *a, int count, const double *Ene
                                   nullptr, d_phi, phi_cplx, d_vals, d_rhs);
 }
 
+static int assembler_run(fdmgpu_assembler *a, int count, const double *Enervide, const double *Eimag,
+                         const double *Ecinetic_out, int E_comp, int basereel, int update_only,
+                         const double *const *d_V, const double *const *d_G, const double *const *d_phi,
+                         const double *const *d_phi2, int phi_cplx, double *const *d_vals, double *const *d_rhs);
+
 int fdmgpu_assembler_run_rel(fdmgpu_assembler *a, int count, const double *Enervide, const double *Eimag,
                              const double *Ecinetic_out, int E_comp, int basereel, int update_only,
                              const double *const *d_V, const double *const *d_G, const double *const *d_phi,
                              int phi_cplx, double *const *d_vals, double *const *d_rhs) {
+  if (a && a->a.nri == 2) {
+    set_error("fdmgpu_assembler_run: the irregular solution's assembler runs by fdmgpu_assembler_run_ri");
+    return FDMGPU_EINVAL;
+  }
+  return assembler_run(a, count, Enervide, Eimag, Ecinetic_out, E_comp, basereel, update_only, d_V, d_G, d_phi,
+                       nullptr, phi_cplx, d_vals, d_rhs);
+}
+
+int fdmgpu_assembler_run_ri(fdmgpu_assembler *a, int count, const double *Enervide, const double *Eimag,
+                            const double *Ecinetic_out, int E_comp, int basereel, int update_only,
+                            const double *const *d_V, const double *const *d_phi, const double *const *d_phi2,
+                            int phi_cplx, double *const *d_vals, double *const *d_rhs) {
+  if (!a || a->a.nri != 2 || !d_phi2 || !d_phi) {
+    set_error("fdmgpu_assembler_run_ri: an assembler of the irregular solution and both phiato");
+    return FDMGPU_EINVAL;
+  }
+  for (int k = 0; k < count && k < ASM_MAXP; k++)
+    if (!d_phi2[k]) {
+      set_error("fdmgpu_assembler_run_ri: null irregular phiato for point " + std::to_string(k));
+      return FDMGPU_EINVAL;
+    }
+  return assembler_run(a, count, Enervide, Eimag, Ecinetic_out, E_comp, basereel, update_only, d_V, nullptr, d_phi,
+                       d_phi2, phi_cplx, d_vals, d_rhs);
+}
+
+static int assembler_run(fdmgpu_assembler *a, int count, const double *Enervide, const double *Eimag,
+                         const double *Ecinetic_out, int E_comp, int basereel, int update_only,
+                         const double *const *d_V, const double *const *d_G, const double *const *d_phi,
+                         const double *const *d_phi2, int phi_cplx, double *const *d_vals, double *const *d_rhs) {
   if (!a || !Enervide || !Eimag || !Ecinetic_out || !d_V || !d_vals || count < 1 || count > ASM_MAXP) {
     set_error("fdmgpu_assembler_run: invalid argument");
     return FDMGPU_EINVAL;
@@ -629,6 +663,7 @@ int fdmgpu_assembler_run_rel(fdmgpu_assembler *a, int count, const double *Enerv
       return FDMGPU_EINVAL;
     }
     P.G[k] = d_G ? d_G[k] : nullptr;
+    P.phi2[k] = d_phi2 ? d_phi2[k] : nullptr;
     P.e[k] = {Enervide[k], Eimag[k], Ecinetic_out[k]};
     P.V[k] = d_V[k];
     P.vals[k] = d_vals[k];

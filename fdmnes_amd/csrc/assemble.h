@@ -20,6 +20,7 @@ struct AsmPoints {
   double *vals[ASM_MAXP];     // device, nnz (interleaved when complex)
   double *rhs[ASM_MAXP];      // device, (nligne, nlmso) row-major, or null
   const double *phi[ASM_MAXP];  // device, phiato of a cluster with atoms (interleaved when complex)
+  const double *phi2[ASM_MAXP];  // device, the irregular atomic solutions' phiato (irregular system)
   const double *G[ASM_MAXP];    // device, gradient of V (npoint x 3, point-major) for relativistic rows
 };
 // stencil and gradient coefficients of the relativistic terms, 1..12 in
@@ -44,6 +45,7 @@ struct Assembler {
   double *part = nullptr;  // outer-row partial sums: [point][run][nlmso^2] x {block, RHS} complex
   // muffin-tin atoms (natome = 0: none)
   int natome = 0, nar = 0, nal = 0, nrow_atom = 0, nlmax = 0;
+  int nri = 1;  // 2: the irregular cluster solution (two blocks per atom, no outer unknowns)
   int64_t nphi = 0;
   int64_t *aq = nullptr, *alphi = nullptr, *bq = nullptr, *phoff = nullptr, *poff = nullptr, *yoff = nullptr;
   int *aa = nullptr, *aptr = nullptr, *ianew = nullptr, *nlmsa = nullptr, *nbordf = nullptr, *arow_atom = nullptr,

@@ -221,7 +221,7 @@ __global__ void k_asm_outcols(int nrow_out, int nl, int nb, int L, const int *__
                               const cz *__restrict__ fun, AsmPoints P, int basereel,
                               const int *__restrict__ opoint, const int *__restrict__ odir,
                               const double *__restrict__ ofac, RelCoef rc, const double *__restrict__ ow_im,
-                              const double *__restrict__ ylm_im) {
+                              const double *__restrict__ ylm_im, int irr) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x, g = blockIdx.y;
   if (t >= nrow_out * nl) return;
   const int r = t / nl, lm = t % nl, l = lharm[lm];
@@ -243,6 +243,11 @@ __global__ void k_asm_outcols(int nrow_out, int nl, int nb, int L, const int *__
       w = ow[k];
     const double y = ylm[(int64_t)b * nl + lm];
     const cz *f = fg + (int64_t)b * (L + 1) * 2;
+    if (CAL && irr) {  // the irregular solution: no outer unknowns, incoming wave -i j + n (mat.f90:1318-1398)
+      const cz c = {f[2 * l].im + f[2 * l + 1].re, -f[2 * l].re + f[2 * l + 1].im};
+      rrow = sub(rrow, scale(scale(c, y), w));
+      continue;
+    }
     if (CAL && ylm_im) {  // complex harmonics (Yc) and weights chi -clapl(v) rvol(i) (mat.f90:1328-1377)
       const cz yc = {y, ylm_im[(int64_t)b * nl + lm]}, wc = {w, ow_im ? ow_im[k] : 0.0};
       row = add(row, mul(mul(cfac(f, l, basereel), yc), wc));
@@ -255,8 +260,10 @@ __global__ void k_asm_outcols(int nrow_out, int nl, int nb, int L, const int *__
   const int64_t q = oq[r] + lm, qr = (int64_t)orow[r] * nl + lm;
   double *v = P.vals[g], *rh = P.rhs[g];
   if constexpr (CAL) {
-    v[2 * q] = row.re;
-    v[2 * q + 1] = row.im;
+    if (!irr) {
+      v[2 * q] = row.re;
+      v[2 * q + 1] = row.im;
+    }
     if (rh) {
       rh[2 * qr] = rrow.re;
       rh[2 * qr + 1] = rrow.im;
@@ -344,15 +351,17 @@ __global__ void k_asm_outrows_sum(int nso, int nl, int nsortf, int nrun, const i
 // (relativistic rows: each link's weight recomputed from its direction and
 // the row's point; atoms are built in P1, where chi rvol(i) = 1)
 template <bool CAL, bool PHIC, bool REL>
-__global__ void k_asm_atomcols(int nar, int nlmax, const int64_t *__restrict__ aq, const int *__restrict__ aa,
+__global__ void k_asm_atomcols(int nar, int nlmax, int nri, const int64_t *__restrict__ aq, const int *__restrict__ aa,
                                const int *__restrict__ aptr, const int64_t *__restrict__ alphi,
                                const double *__restrict__ alw, const int *__restrict__ nlmsa, AsmPoints P,
                                const int *__restrict__ apoint, const int *__restrict__ adir, RelCoef rc) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x, g = blockIdx.y;
   if (t >= nar * nlmax) return;
-  const int r = t / nlmax, lm = t % nlmax;
-  if (lm >= nlmsa[aa[r]]) return;
-  const double *phi = P.phi[g];
+  const int r = t / nlmax, lmb = t % nlmax, na = nlmsa[aa[r]];
+  if (lmb >= nri * na) return;
+  // the irregular solution's second column block takes the irregular phiato (mat.f90:1266-1307)
+  const int ri = lmb >= na, lm = lmb - ri * na;
+  const double *phi = ri ? P.phi2[g] : P.phi[g];
   double bder = 0;
   const double *Gi = nullptr;
   if constexpr (REL) {
@@ -375,7 +384,7 @@ __global__ void k_asm_atomcols(int nar, int nlmax, const int64_t *__restrict__ a
       sr = sr + w * phi[o];
     }
   }
-  const int64_t q = aq[r] + lm;
+  const int64_t q = aq[r] + lmb;
   double *v = P.vals[g];
   if constexpr (CAL) {
     v[2 * q] = sr;
@@ -389,7 +398,7 @@ __global__ void k_asm_atomcols(int nar, int nlmax, const int64_t *__restrict__ a
 // outside border points, in border order, of poidsa Ylmato(Lm) phiato(Lm');
 // one thread per (atom row, Lm')
 template <bool CAL, bool PHIC>
-__global__ void k_asm_atomrows(int nrow_atom, int nlmax, const int *__restrict__ arow_atom,
+__global__ void k_asm_atomrows(int nrow_atom, int nlmax, int nri, const int *__restrict__ arow_atom,
                                const int *__restrict__ arow_lm, const int64_t *__restrict__ bq,
                                const int *__restrict__ nlmsa, const int *__restrict__ nbordf,
                                const int64_t *__restrict__ phoff, const int64_t *__restrict__ poff,
@@ -397,11 +406,14 @@ __global__ void k_asm_atomrows(int nrow_atom, int nlmax, const int *__restrict__
                                const double *__restrict__ ylmato, AsmPoints P) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x, g = blockIdx.y;
   if (t >= nrow_atom * nlmax) return;
-  const int k = t / nlmax, lp = t % nlmax, a = arow_atom[k], lm = arow_lm[k], na = nlmsa[a];
-  if (lp >= na) return;
-  const double *phi = P.phi[g], *pw = poidsa + poff[a], *Y = ylmato + yoff[a];
+  const int k = t / nlmax, lpb = t % nlmax, a = arow_atom[k], lm = arow_lm[k], na = nlmsa[a];
+  if (lpb >= nri * na) return;
+  // irregular solution: column block rj from its phiato over its half of the border (mat.f90:1580-1630)
+  const int rj = lpb >= na, lp = lpb - rj * na, nh = nbordf[a] / 2;
+  const int b0 = (nri == 2 && rj) ? nh : 0, b1 = (nri == 2 && !rj) ? nh : nbordf[a];
+  const double *phi = rj ? P.phi2[g] : P.phi[g], *pw = poidsa + poff[a], *Y = ylmato + yoff[a];
   double sr = 0, si = 0;
-  for (int b = 0; b < nbordf[a]; b++) {
+  for (int b = b0; b < b1; b++) {
     const double y = Y[(int64_t)b * na + lm];
     const int64_t o = phoff[a] + (int64_t)b * na + lp;
     if constexpr (!CAL) {
@@ -413,7 +425,7 @@ __global__ void k_asm_atomrows(int nrow_atom, int nlmax, const int *__restrict__
       sr = sr + pw[b] * (phi[o] * y);
     }
   }
-  const int64_t q = bq[k] + lp;
+  const int64_t q = bq[k] + lpb;
   double *v = P.vals[g];
   if constexpr (CAL) {
     v[2 * q] = sr;
@@ -481,6 +493,8 @@ Assembler::~Assembler() {
 int Assembler::init(const fdmgpu_assembly_tables *t, const fdmgpu_atom_tables *at, int maxp, hipStream_t s) {
   const int nall = (t->lmaxso + 1) * (t->lmaxso + 1);  // a representation of mmm has a subset (lharm)
   int64_t natrow = 0;  // rows of the atoms' expansions
+  const int nrr = (at && at->nri == 2) ? 2 : 1;
+  const int nout_rows = nrr == 2 ? 0 : t->nlmso;  // outer rows (none in the irregular solution)
   if (at) {
     if (at->natome < 1 || at->nar < 0 || at->nal < at->nar || at->nphi < 1 || !at->ianew || !at->nlmsa ||
         !at->nbordf || !at->phoff || !at->bq || !at->poidsa || !at->ylmato ||
@@ -490,15 +504,15 @@ int Assembler::init(const fdmgpu_assembly_tables *t, const fdmgpu_atom_tables *a
     }
     for (int a = 0; a < at->natome; a++) {
       if (at->nlmsa[a] < 1 || at->nlmsa[a] > 169 || at->nbordf[a] < 1 || at->ianew[a] < 0 ||
-          at->ianew[a] + at->nlmsa[a] > t->nligne - t->nlmso || at->phoff[a] < 0 ||
+          at->ianew[a] + nrr * at->nlmsa[a] > t->nligne - nout_rows || at->phoff[a] < 0 ||
           at->phoff[a] + (int64_t)at->nbordf[a] * at->nlmsa[a] > at->nphi) {
         set_error("fdmgpu_assembler_create_atoms: atom " + std::to_string(a) + " outside the rows or phiato");
         return FDMGPU_EINVAL;
       }
-      natrow += at->nlmsa[a];
+      natrow += nrr * at->nlmsa[a];
     }
   }
-  const int64_t npr = t->nligne - t->nlmso - natrow;  // interstitial grid rows
+  const int64_t npr = t->nligne - nout_rows - natrow;  // interstitial grid rows
   if (t->npoint <= 0 || (at ? (npr < 1 || npr > t->npoint) : t->nligne != t->npoint + t->nlmso) || t->lmaxso < 0 ||
       t->lmaxso > 40 ||
       (t->lharm ? (t->nlmso < 1 || t->nlmso > nall) : t->nlmso != nall) || t->nsortf <= 0 || t->nb < t->nsortf ||
@@ -517,7 +531,8 @@ int Assembler::init(const fdmgpu_assembly_tables *t, const fdmgpu_atom_tables *a
     set_error("fdmgpu_assembler_create: row_ptr does not span nnz");
     return FDMGPU_EINVAL;
   }
-  const int nso = t->nligne - t->nlmso;  // first outer row
+  const int nri_ = (at && at->nri == 2) ? 2 : 1;       // the irregular cluster solution: no outer rows
+  const int nso = nri_ == 2 ? t->nligne : t->nligne - t->nlmso;  // first outer row
   for (int i = 0; i < t->npoint; i++)
     if (at ? (t->qdiag[i] < -1 || t->qdiag[i] >= t->nnz)
            : (t->qdiag[i] < t->row_ptr[i] || t->qdiag[i] >= t->row_ptr[i + 1])) {
@@ -526,7 +541,9 @@ int Assembler::init(const fdmgpu_assembly_tables *t, const fdmgpu_atom_tables *a
     }
   for (int r = 0; r < t->nrow_out; r++) {
     const int i = t->orow[r];
-    if (i < 0 || i >= nso || t->oq[r] < t->row_ptr[i] || t->oq[r] + t->nlmso > t->row_ptr[i + 1] ||
+    const bool cols_ok = nri_ == 2 ? t->oq[r] == -1
+                                   : (t->oq[r] >= t->row_ptr[i] && t->oq[r] + t->nlmso <= t->row_ptr[i + 1]);
+    if (i < 0 || i >= nso || !cols_ok ||
         t->optr[r + 1] < t->optr[r] || t->optr[r + 1] > t->nlink) {
       set_error("fdmgpu_assembler_create: outer-column table inconsistent at entry " + std::to_string(r));
       return FDMGPU_EINVAL;
@@ -547,7 +564,7 @@ int Assembler::init(const fdmgpu_assembly_tables *t, const fdmgpu_atom_tables *a
       set_error("fdmgpu_assembler_create: outer point index out of range");
       return FDMGPU_EINVAL;
     }
-  for (int lm = 0; lm < t->nlmso; lm++)
+  for (int lm = 0; lm < t->nlmso && nri_ == 1; lm++)
     if (t->row_ptr[nso + lm + 1] - t->row_ptr[nso + lm] != t->nsortf + t->nlmso) {
       set_error("fdmgpu_assembler_create: outer row " + std::to_string(lm) + " is not nsortf + nlmso long");
       return FDMGPU_EINVAL;
@@ -594,9 +611,10 @@ int Assembler::init_atoms(const fdmgpu_atom_tables *at, int64_t natrow, const in
     set_error("fdmgpu_assembler_create_atoms: aptr does not span nal");
     return FDMGPU_EINVAL;
   }
+  const int nr2 = at->nri == 2 ? 2 : 1;
   for (int r = 0; r < at->nar; r++) {
     const int a = at->aa[r];
-    if (a < 0 || a >= at->natome || at->aq[r] < 0 || at->aq[r] + at->nlmsa[a] > nnz ||
+    if (a < 0 || a >= at->natome || at->aq[r] < 0 || at->aq[r] + nr2 * at->nlmsa[a] > nnz ||
         at->aptr[r + 1] < at->aptr[r]) {
       set_error("fdmgpu_assembler_create_atoms: row-atom pair " + std::to_string(r) + " inconsistent");
       return FDMGPU_EINVAL;
@@ -615,18 +633,27 @@ int Assembler::init_atoms(const fdmgpu_atom_tables *at, int64_t natrow, const in
     yo[a] = yb;
     pb += at->nbordf[a];
     yb += (int64_t)at->nbordf[a] * at->nlmsa[a];
-    nlmax = std::max(nlmax, at->nlmsa[a]);
-    for (int lm = 0; lm < at->nlmsa[a]; lm++) {
+    nlmax = std::max(nlmax, nr2 * at->nlmsa[a]);
+    for (int lm = 0; lm < nr2 * at->nlmsa[a]; lm++) {  // both row blocks of the irregular solution
       const int row = at->ianew[a] + lm;
+      if ((int64_t)ra.size() >= natrow) {
+        set_error("fdmgpu_assembler_create_atoms: more atom rows than the sizes report");
+        return FDMGPU_EINVAL;
+      }
       const int64_t q = at->bq[ra.size()];
-      if (q < row_ptr_h[row] || q + at->nlmsa[a] > row_ptr_h[row + 1]) {
+      if (q < row_ptr_h[row] || q + nr2 * at->nlmsa[a] > row_ptr_h[row + 1]) {
         set_error("fdmgpu_assembler_create_atoms: block of atom row " + std::to_string(row + 1) + " outside the row");
         return FDMGPU_EINVAL;
       }
       ra.push_back(a);
-      rl.push_back(lm);
+      rl.push_back(lm % at->nlmsa[a]);  // the harmonic of the row (its Ylmato column)
     }
   }
+  if ((int64_t)ra.size() != natrow) {
+    set_error("fdmgpu_assembler_create_atoms: atom rows differ from the sizes");
+    return FDMGPU_EINVAL;
+  }
+  nri = nr2;
   natome = at->natome;
   nar = at->nar;
   nal = at->nal;
@@ -736,6 +763,17 @@ int Assembler::run(int count, const AsmPoints &P, bool e_comp, bool basereel, bo
       return FDMGPU_EINVAL;
     }
   const bool cal = e_comp || !basereel || cx;  // complex harmonics or characters make it complex (mat.f90:97-104)
+  if (nri == 2) {
+    if (!e_comp || basereel) {
+      set_error("fdmgpu_assembler_run_ri: the irregular solution needs complex energy and the complex basis");
+      return FDMGPU_EINVAL;
+    }
+    for (int k = 0; k < count; k++)
+      if (!P.phi2[k]) {
+        set_error("fdmgpu_assembler_run_ri: the irregular solution needs the irregular phiato");
+        return FDMGPU_EINVAL;
+      }
+  }
   if (phi_cplx && !cal) {
     set_error("fdmgpu_assembler_run_atoms: complex phiato needs complex arithmetic (E_comp or !basereel)");
     return FDMGPU_EINVAL;
@@ -762,10 +800,12 @@ int Assembler::run(int count, const AsmPoints &P, bool e_comp, bool basereel, bo
     auto oc = rel ? k_asm_outcols<true, true> : k_asm_outcols<true, false>;
     if (nrow_out > 0)
       hipLaunchKernelGGL(oc, dim3(go, count), dim3(NT), 0, stream, nrow_out, nlmso, nb, lmaxso, orow, oq, optr, ob, ow,
-                         ylm, lharm, F, P, (int)basereel, opoint, odir, ofac, relc, ow_im, ylm_im);
-    hipLaunchKernelGGL(k_asm_outrows_part, gp, dim3(256), 0, stream, nlmso, nb, lmaxso, nsortf, wso, ylm, lharm,
-                       F, (int)basereel, Q, ylm_im);
-    hipLaunchKernelGGL(k_asm_outrows_sum<true>, gs, dim3(256), 0, stream, nso, nlmso, nsortf, nrun, row_ptr, Q, P);
+                         ylm, lharm, F, P, (int)basereel, opoint, odir, ofac, relc, ow_im, ylm_im, (int)(nri == 2));
+    if (nri == 1) {  // the irregular solution has no outer rows
+      hipLaunchKernelGGL(k_asm_outrows_part, gp, dim3(256), 0, stream, nlmso, nb, lmaxso, nsortf, wso, ylm, lharm,
+                         F, (int)basereel, Q, ylm_im);
+      hipLaunchKernelGGL(k_asm_outrows_sum<true>, gs, dim3(256), 0, stream, nso, nlmso, nsortf, nrun, row_ptr, Q, P);
+    }
   } else {
     if (!update_only)
       hipLaunchKernelGGL(k_asm_base<false>, dim3(gb, count), dim3(NT), 0, stream, nnz, nrhs_el, base,
@@ -779,7 +819,7 @@ int Assembler::run(int count, const AsmPoints &P, bool e_comp, bool basereel, bo
     if (nrow_out > 0)
       hipLaunchKernelGGL(oc, dim3(go, count), dim3(NT), 0, stream, nrow_out, nlmso, nb, lmaxso, orow, oq, optr, ob, ow,
                          ylm, lharm, F, P, 1, opoint, odir, ofac, relc, (const double *)nullptr,
-                         (const double *)nullptr);
+                         (const double *)nullptr, 0);
     hipLaunchKernelGGL(k_asm_outrows_part, gp, dim3(256), 0, stream, nlmso, nb, lmaxso, nsortf, wso, ylm, lharm,
                        F, 1, Q, (const double *)nullptr);
     hipLaunchKernelGGL(k_asm_outrows_sum<false>, gs, dim3(256), 0, stream, nso, nlmso, nsortf, nrun, row_ptr, Q, P);
@@ -793,9 +833,9 @@ int Assembler::run(int count, const AsmPoints &P, bool e_comp, bool basereel, bo
     auto rows = cal ? (phi_cplx ? k_asm_atomrows<true, true> : k_asm_atomrows<true, false>)
                     : k_asm_atomrows<false, false>;
     if (nar > 0)
-      hipLaunchKernelGGL(cols, gc, dim3(NT), 0, stream, nar, nlmax, aq, aa, aptr, alphi, alw, nlmsa, P, apoint, adir,
-                         relc);
-    hipLaunchKernelGGL(rows, gr, dim3(NT), 0, stream, nrow_atom, nlmax, arow_atom, arow_lm, bq, nlmsa, nbordf, phoff,
+      hipLaunchKernelGGL(cols, gc, dim3(NT), 0, stream, nar, nlmax, nri, aq, aa, aptr, alphi, alw, nlmsa, P, apoint,
+                         adir, relc);
+    hipLaunchKernelGGL(rows, gr, dim3(NT), 0, stream, nrow_atom, nlmax, nri, arow_atom, arow_lm, bq, nlmsa, nbordf, phoff,
                        poff, yoff, poidsa, ylmato, P);
   }
   FDM_HIP(hipGetLastError());

@@ -798,20 +798,23 @@ int atom_cluster_tables(const fdmgpu_cluster *c, double *base, int64_t *qdiag, i
     }
     if (out) {
       orow[n] = r;
-      oq[n] = position(c, r, c->nso + 1);
+      oq[n] = c->irregular ? -1 : position(c, r, c->nso + 1);  // the irregular solution has no outer columns
       optr[++n] = k;
     }
   }
-  for (int lm = 0; lm < nl; lm++) {
+  for (int lm = 0; lm < nl && !c->irregular; lm++) {
     const int r = c->nso + lm;
     for (int b = 0; b < c->nsortf; b++)
       base[position(c, r, c->row[c->bnd[b]] + 1)] = -(c->wso[b] * c->ylm[(size_t)b * nl + lm]);
   }
   for (int a = 0; a < c->natome; a++)
-    for (int lm = 0; lm < c->nlmsa[a]; lm++) {
-      const int r = c->ianew[a] + lm;
-      for (int b = 0; b < c->nbordf[a]; b++)
-        base[position(c, r, c->row[c->abord[a][b]] + 1)] = -(c->apoids[a][b] * c->aylm[a][(size_t)b * c->nlmsa[a] + lm]);
+    for (int lm = 0; lm < c->nri * c->nlmsa[a]; lm++) {  // irregular: each row block on its half of the border
+      const int r = c->ianew[a] + lm, ri = lm >= c->nlmsa[a], l0 = lm - ri * c->nlmsa[a];
+      const int nh = c->nbordf[a] / 2;
+      for (int b = 0; b < c->nbordf[a]; b++) {
+        if (c->irregular && (ri ? b < nh : b >= nh)) continue;
+        base[position(c, r, c->row[c->abord[a][b]] + 1)] = -(c->apoids[a][b] * c->aylm[a][(size_t)b * c->nlmsa[a] + l0]);
+      }
     }
   std::memcpy(ylm, c->ylm.data(), sizeof(double) * c->ylm.size());
   for (size_t b = 0; b < c->bnd.size(); b++) brad[b] = c->rad[c->bnd[b]];
@@ -1523,7 +1526,6 @@ int fdmgpu_cluster_rel_tables(const fdmgpu_cluster *c, int64_t *rq, double *rfac
 }
 
 int fdmgpu_cluster_atom_assembly_sizes(const fdmgpu_cluster *c, int64_t *sizes) {
-  if (c->irregular) return -2;  // the device tables describe regular solutions
   int64_t nar = 0, nal = 0, nphi = 0, nrow = 0, nbf = 0, nyl = 0;
   for (int i = 0; i < c->npoint; i++) {
     if (c->numia[i]) continue;
@@ -1538,7 +1540,7 @@ int fdmgpu_cluster_atom_assembly_sizes(const fdmgpu_cluster *c, int64_t *sizes) 
   }
   for (int a = 0; a < c->natome; a++) {
     nphi += (int64_t)c->abord[a].size() * c->nlmsa[a];
-    nrow += c->nlmsa[a];
+    nrow += c->nri * c->nlmsa[a];
     nbf += c->nbordf[a];
     nyl += (int64_t)c->nbordf[a] * c->nlmsa[a];
   }
@@ -1556,7 +1558,6 @@ int fdmgpu_cluster_atom_assembly_tables(const fdmgpu_cluster *c, int64_t *aq, in
                                         double *alw, int *ianew, int *nlmsa, int *nbordf, int64_t *phoff,
                                         int64_t *bq, double *poidsa, double *ylmato) {
   if (c->natome < 1) return -1;
-  if (c->irregular) return -2;
   std::vector<int64_t> off(c->natome + 1, 0);
   for (int a = 0; a < c->natome; a++) off[a + 1] = off[a] + (int64_t)c->abord[a].size() * c->nlmsa[a];
   int n = 0, k = 0;
@@ -1591,7 +1592,7 @@ int fdmgpu_cluster_atom_assembly_tables(const fdmgpu_cluster *c, int64_t *aq, in
     nlmsa[a] = c->nlmsa[a];
     nbordf[a] = c->nbordf[a];
     phoff[a] = off[a];
-    for (int lm = 0; lm < c->nlmsa[a]; lm++) bq[r++] = position(c, c->ianew[a] + lm, c->ianew[a] + 1);
+    for (int lm = 0; lm < c->nri * c->nlmsa[a]; lm++) bq[r++] = position(c, c->ianew[a] + lm, c->ianew[a] + 1);
     for (int b = 0; b < c->nbordf[a]; b++) {
       poidsa[pb++] = c->apoids[a][b];
       for (int lm = 0; lm < c->nlmsa[a]; lm++) ylmato[yb++] = c->aylm[a][(size_t)b * c->nlmsa[a] + lm];

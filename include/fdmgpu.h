@@ -285,6 +285,7 @@ typedef struct {
   const int64_t *bq;        /* [sum nlmsa] per atom row, position of the atom block's first column */
   const double *poidsa;     /* [sum nbordf] border weights, atom after atom */
   const double *ylmato;     /* [sum nbordf*nlmsa] Ylm about the atom at them, (nbordf, nlmsa) per atom */
+  int nri;                  /* 2: the irregular cluster solution (two blocks per atom), else 1 */
 } fdmgpu_atom_tables;
 /* As fdmgpu_assembler_create, with the atom tables (checked and copied).
  * nligne must be the interstitial rows + sum nlmsa + nlmso. */
@@ -339,6 +340,15 @@ int fdmgpu_assembler_set_complex(fdmgpu_assembler *a, const double *base_im, con
 /* Adds the relativistic tables to an assembler (checked and copied); after
  * this the assembler is run only by fdmgpu_assembler_run_rel. */
 int fdmgpu_assembler_set_relativistic(fdmgpu_assembler *a, const fdmgpu_rel_tables *t);
+/* The irregular cluster solution (an assembler created with nri = 2):
+ * d_phi2[k] the irregular atomic solutions' phiato (fdmgpu_sphere_run_ri);
+ * complex energy and the complex basis only (E_comp, !basereel), as the
+ * reference solves it (FDM_comp); the RHS is the incoming wave -i j + n of
+ * mat.f90:1318-1398. */
+int fdmgpu_assembler_run_ri(fdmgpu_assembler *a, int count, const double *Enervide, const double *Eimag,
+                            const double *Ecinetic_out, int E_comp, int basereel, int update_only,
+                            const double *const *d_V, const double *const *d_phi, const double *const *d_phi2,
+                            int phi_cplx, double *const *d_vals, double *const *d_rhs);
 /* As fdmgpu_assembler_run_atoms (d_phi NULL without atoms), with d_G[k] the
  * device gradient of point k's potential (npoint x 3, point-major, Ry/bohr). */
 int fdmgpu_assembler_run_rel(fdmgpu_assembler *a, int count, const double *Enervide, const double *Eimag,

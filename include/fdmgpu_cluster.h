@@ -276,8 +276,13 @@ int fdmgpu_cluster_assembly_weights(const fdmgpu_cluster *c, int *lharm, double 
  * -poidsa Ylmato in the columns of their outside border points
  * (mat.f90:1560-1581).  The entries that depend on phiato come from these
  * tables (fdmgpu_atom_tables in fdmgpu.h):
- * sizes[0..6] = natome, nar, nal, nphi, sum nlmsa, sum nbordf,
- * sum nbordf * nlmsa */
+ * sizes[0..6] = natome, nar, nal, nphi, sum nlmsa (2 sum nlmsa for the
+ * irregular solution's two row blocks), sum nbordf, sum nbordf * nlmsa.
+ * For the irregular cluster solution (mat.f90:140-147) every atom has two
+ * row and column blocks (regular then irregular atomic solutions, each row
+ * block on its half of the outside border), bq lists both row blocks, the
+ * grid rows touching the outside have no outer columns (oq = -1) and there
+ * are no outer rows. */
 int fdmgpu_cluster_atom_assembly_sizes(const fdmgpu_cluster *c, int64_t *sizes);
 int fdmgpu_cluster_atom_assembly_tables(const fdmgpu_cluster *c, int64_t *aq, int *aa, int *aptr, int64_t *alphi,
                                         double *alw, int *ianew, int *nlmsa, int *nbordf, int64_t *phoff,

@@ -68,8 +68,9 @@ def test_irregular_assembly_refuses_mismatched_inputs():
         c.assemble_atoms(V, E, Ei, E, True, False, phi)
     with pytest.raises(CL._lib.FdmGpuError):  # and complex energy
         c.assemble_atoms(V, E, 0.0, E, False, False, phi, phi2)
-    with pytest.raises(CL._lib.FdmGpuError):  # the device assembler's tables describe regular solutions
-        c.atom_assembly_tables()
+    at = c.atom_assembly_tables()  # two row blocks per atom, no outer columns
+    assert len(at["bq"]) == 2 * sum(c.atom(a)["nlmsa"] for a in range(c.natome))
+    assert np.all(c.assembly_tables()["oq"][:max(c.assembly_tables()["nrow_out"], 1)] == -1)
     reg = CL.Cluster(float(g["R"]) / BOHR, float(g["h"]) / BOHR, int(g["lmaxso"]),
                      atoms=(g["posi"] / BOHR, g["rmt"] / BOHR, g["lmax"]))
     with pytest.raises(CL._lib.FdmGpuError):  # a regular cluster takes no irregular solutions

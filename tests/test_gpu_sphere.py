@@ -193,3 +193,41 @@ def test_device_irregular_phiato_assembles_the_irregular_system():
     _, _, vo, ro = ci.assemble_atoms(V, Ev, Ei, Ec, True, False, [p[0] for p in op], [p[1] for p in op])
     assert np.abs(vd - vo).max() <= 1e-12 * np.abs(vo).max()
     assert np.abs(rd - ro).max() <= 1e-12 * max(np.abs(ro).max(), 1e-300)
+
+
+def test_device_assembly_of_the_irregular_system():
+    """fdmgpu_assembler_run_ri: the device assembler of the irregular cluster
+    solution (two blocks per atom, each atom row block on its half of the
+    border, no outer unknowns, incoming wave -i j + n), fed the device's
+    regular and irregular phiato, writes the host assembler's rows: fixed
+    entries bit for bit, the rest within 1e-13; fresh and updated in place."""
+    import torch
+    from fdmnes_amd import cluster as CL
+    from fdmnes_amd.assemble import DeviceAssembler
+    from fdmnes_amd.sphere import DeviceSphere
+    g, o = _cluster()
+    f = np.load(SPHERE)
+    atoms = _atoms(g, o, f)
+    at = (g["posi"] / BOHR, g["rmt"] / BOHR, g["lmax"])
+    ci = CL.Cluster(float(g["R"]) / BOHR, float(g["h"]) / BOHR, int(g["lmaxso"]), atoms=at, irregular=True)
+    ens = [(1.0, 0.05, 0.7), (0.8, 0.03, 0.5)]  # (Enervide, Eimag, Ecinetic_out)
+    V = g["V"][: ci.npoint]
+    pp = DeviceSphere(atoms).run_ri([(Ec, Ei, Ev) for Ev, Ei, Ec in ens])
+    phi, phi2 = [p[0] for p in pp], [p[1] for p in pp]
+    asm = DeviceAssembler(ci, max_points=2)
+    vals = [torch.full((ci.nnz,), 5.0, dtype=torch.complex128, device="cuda") for _ in ens]
+    rhs = [torch.full((ci.nligne, ci.nlmso), 5.0, dtype=torch.complex128, device="cuda") for _ in ens]
+    Vd = [torch.from_numpy(np.ascontiguousarray(V)).cuda() for _ in ens]
+    asm.run(Vd, ens, True, False, vals, rhs, phi=phi, phi2=phi2)
+    asm.run(Vd[::-1], ens[::-1], True, False, vals, rhs, update_only=True, phi=phi[::-1], phi2=phi2[::-1])
+    torch.cuda.synchronize()
+    offs = np.cumsum([0] + [len(a["bxyz"]) * (a["lmax"] + 1) ** 2 for a in atoms])
+    blocks = lambda t: [t.cpu().numpy()[offs[k]:offs[k + 1]].reshape(len(a["bxyz"]), -1) for k, a in enumerate(atoms)]
+    for k, e in enumerate(ens[::-1]):
+        kk = len(ens) - 1 - k
+        _, _, hv, hr = ci.assemble_atoms(V, *e, True, False, blocks(phi[kk]), blocks(phi2[kk]))
+        dv, dr = vals[k].cpu().numpy(), rhs[k].cpu().numpy()
+        err_v = np.abs(dv - hv).max() / np.abs(hv).max()
+        err_r = np.abs(dr - hr).max() / np.abs(hr).max()
+        print(f"irregular system energy {k}: rows rel {err_v:.2e}, rhs rel {err_r:.2e}", flush=True)
+        assert err_v <= 1e-13 and err_r <= 1e-13
