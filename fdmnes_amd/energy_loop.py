@@ -15,7 +15,7 @@ from .assemble import DeviceAssembler
 from .batch import FP32, FP64, DeviceSolver
 from .tau import split, symmetrize, tau_reduce
 
-__all__ = ["EnergyLoop", "RepresentationLoop", "FP32", "FP64"]
+__all__ = ["EnergyLoop", "RepresentationLoop", "FdmCompLoop", "FP32", "FP64"]
 
 
 class EnergyLoop:
@@ -218,3 +218,68 @@ class RepresentationLoop:
         if keep_solutions:
             out["x"] = xs
         return out
+
+
+class FdmCompLoop:
+    """The energy loop of FDM_comp (mat.f90:135-295, Loop_RI): at complex
+    energy each point solves the regular cluster system and the irregular
+    one (``Cluster(..., atoms=..., irregular=True)``), and the atoms'
+    amplitude products come from both solutions as Tau_RI.  Per group of
+    energies: the device radial solver's regular and irregular phiato
+    (``DeviceSphere.run_ri``), device assembly of both systems, one grouped
+    solve of the 2 x group systems, Tau_RI on the device
+    (``fdmnes_amd.tau.tau_ri``)."""
+
+    def __init__(self, cluster, cluster_irr, sphere, precision: int = FP64, group: int = 8):
+        if not 1 <= group <= 16:
+            raise ValueError("group: 1..16 energy points (two systems each) per grouped solve")
+        if not cluster.natome or not getattr(cluster_irr, "irregular", False):
+            raise ValueError("a regular cluster with atoms and its irregular system")
+        self.cl, self.ci, self.sphere, self.G = cluster, cluster_irr, sphere, int(group)
+        self.asm = DeviceAssembler(cluster, max_points=self.G)
+        self.asm_i = DeviceAssembler(cluster_irr, max_points=self.G)
+        nl = cluster.nlmso
+        self.sv = DeviceSolver(precision, True, nl)
+        zero = [np.zeros((cluster.atom(a)["nbord"], cluster.atom(a)["nlmsa"])) for a in range(cluster.natome)]
+        rp, col, _, _ = cluster.assemble_atoms(np.zeros(cluster.npoint), 0.0, 0.01, 1.0, True, False, zero)
+        rpi, coli, _, _ = cluster_irr.assemble_atoms(np.zeros(cluster_irr.npoint), 0.0, 0.01, 1.0, True, False, zero,
+                                                     zero)
+        pr, pi = cluster.profile(), cluster_irr.profile()
+        self.slots = [self.sv.add_system(*pr, rp, col) for _ in range(self.G)]
+        self.slots_i = [self.sv.add_system(*pi, rpi, coli) for _ in range(self.G)]
+        c128 = torch.complex128
+        self.vals = [torch.zeros(cluster.nnz, dtype=c128, device="cuda") for _ in range(self.G)]
+        self.vals_i = [torch.zeros(cluster_irr.nnz, dtype=c128, device="cuda") for _ in range(self.G)]
+        self.rhs = [torch.zeros((cluster.nligne, nl), dtype=c128, device="cuda") for _ in range(self.G)]
+        self.rhs_i = [torch.zeros((cluster_irr.nligne, nl), dtype=c128, device="cuda") for _ in range(self.G)]
+        self.x = [torch.zeros_like(r) for r in self.rhs]
+        self.x_i = [torch.zeros_like(r) for r in self.rhs_i]
+        self.atoms = [(cluster.atom(a)["ianew"], cluster.atom(a)["nlmsa"]) for a in range(cluster.natome)]
+        self.atoms_i = [(cluster_irr.atom(a)["ianew"], cluster_irr.atom(a)["nlmsa"]) for a in range(cluster.natome)]
+
+    def run(self, V, energies) -> dict:
+        """V: the potential (npoint, Ry); energies: [(Enervide, Eimag,
+        Ecinetic_out), ...] with Eimag != 0.  Returns {"tau_ri": per energy
+        a list of per-atom (2, nlmsa, nlmsa) complex arrays (i_RI = 1, 2)}."""
+        from .tau import tau_ri
+        if any(abs(e[1]) <= 1e-10 for e in energies):
+            raise ValueError("FDM_comp solves at complex energy (Eimag != 0)")
+        Vd = torch.as_tensor(np.ascontiguousarray(V, np.float64)).cuda()
+        out = []
+        for e0 in range(0, len(energies), self.G):
+            ens = energies[e0:e0 + self.G]
+            k = len(ens)
+            pp = self.sphere.run_ri([(e[2], e[1], e[0]) for e in ens], complex_out=True)
+            phi, phi2 = [p[0] for p in pp], [p[1] for p in pp]
+            self.asm.run([Vd] * k, ens, True, False, self.vals[:k], self.rhs[:k], phi=phi)
+            self.asm_i.run([Vd] * k, ens, True, False, self.vals_i[:k], self.rhs_i[:k], phi=phi, phi2=phi2)
+            self.sv.solve(self.slots[:k] + self.slots_i[:k], self.vals[:k] + self.vals_i[:k],
+                          self.rhs[:k] + self.rhs_i[:k], self.x[:k] + self.x_i[:k])
+            t = tau_ri(self.x[:k], self.x_i[:k], self.atoms, self.atoms_i)
+            for ti in t:
+                flat, blocks, off = ti.cpu().numpy(), [], 0
+                for _, na in self.atoms:
+                    blocks.append(flat[off:off + 2 * na * na].reshape(2, na, na))
+                    off += 2 * na * na
+                out.append(blocks)
+        return {"tau_ri": out}

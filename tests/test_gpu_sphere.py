@@ -231,3 +231,47 @@ def test_device_assembly_of_the_irregular_system():
         err_r = np.abs(dr - hr).max() / np.abs(hr).max()
         print(f"irregular system energy {k}: rows rel {err_v:.2e}, rhs rel {err_r:.2e}", flush=True)
         assert err_v <= 1e-13 and err_r <= 1e-13
+
+
+def test_fdm_comp_loop_gives_the_host_chains_tau_ri():
+    """FdmCompLoop (device radial solver with the irregular solution, device
+    assembly of the regular and irregular systems, one grouped solve, device
+    Tau_RI) against the same chain with the oracle's Sphere, the host
+    assemblers and the FP64 grouped solve, three complex energies (1e-9)."""
+    import torch
+    from fdmnes_amd import batch as B
+    from fdmnes_amd import cluster as CL
+    from fdmnes_amd import tau as T
+    from fdmnes_amd.energy_loop import FdmCompLoop
+    from fdmnes_amd.sphere import DeviceSphere
+    g, o = _cluster()
+    f = np.load(SPHERE)
+    atoms = _atoms(g, o, f)
+    at = (g["posi"] / BOHR, g["rmt"] / BOHR, g["lmax"])
+    R, h, lm = float(g["R"]) / BOHR, float(g["h"]) / BOHR, int(g["lmaxso"])
+    c = CL.Cluster(R, h, lm, atoms=at)
+    ci = CL.Cluster(R, h, lm, atoms=at, irregular=True)
+    ens = [(1.0, 0.05, 0.7), (0.8, 0.03, 0.5), (1.3, 0.04, 0.9)]
+    V = g["V"][: c.npoint]
+    out = FdmCompLoop(c, ci, DeviceSphere(atoms), group=2).run(V, ens)
+    nl = c.nlmso
+    for k, (Ev, Ei, Ec) in enumerate(ens):
+        op = [O.sphere_ri(a["r"], a["V"], a["Z"], a["rmtg"], a["lmax"], False, Ec, Ei, Ev, a["bxyz"], a["pos"],
+                          a["ylmato"]) for a in atoms]
+        phi, phi2 = [p[0] for p in op], [p[1] for p in op]
+        reg = c.assemble_atoms(V, Ev, Ei, Ec, True, False, phi)
+        irr = ci.assemble_atoms(V, Ev, Ei, Ec, True, False, phi, phi2)
+        sv = B.DeviceSolver(B.FP64, True, nl)
+        slots = [sv.add_system(*c.profile(), reg[0], reg[1]), sv.add_system(*ci.profile(), irr[0], irr[1])]
+        vals = [torch.from_numpy(np.ascontiguousarray(s[2], np.complex128)).cuda() for s in (reg, irr)]
+        src = [torch.from_numpy(np.ascontiguousarray(s[3], np.complex128)).cuda() for s in (reg, irr)]
+        xs = [torch.zeros_like(s) for s in src]
+        sv.solve(slots, vals, src, xs)
+        ar = [(c.atom(a)["ianew"], c.atom(a)["nlmsa"]) for a in range(c.natome)]
+        ai = [(ci.atom(a)["ianew"], ci.atom(a)["nlmsa"]) for a in range(ci.natome)]
+        want = T.tau_ri([xs[0]], [xs[1]], ar, ai)[0].cpu().numpy()
+        got = np.concatenate([b.reshape(-1) for b in out["tau_ri"][k]])
+        rel = np.abs(got - want).max() / np.abs(want).max()
+        print(f"energy {k}: FdmCompLoop Tau_RI vs the host chain {rel:.2e}", flush=True)
+        assert rel <= 1e-9
+        del sv
